@@ -1,0 +1,110 @@
+/*
+ * rebound_amd.h -- MI355X additions beside the drop-in REBOUND ABI.
+ *
+ * Nothing here changes a struct of rebound.h.  Per-simulation knobs are set
+ * through functions that take the simulation pointer; batched and serving
+ * entry points take plain pointers and sizes (no framework types).
+ */
+#ifndef REBOUND_AMD_H
+#define REBOUND_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include "rebound.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ precision
+ * Arithmetic of the direct-summation force (REB_GRAVITY_BASIC and
+ * REB_GRAVITY_COMPENSATED; gravity.c:167-531).
+ *
+ *  FP64   (default) FP64 throughout.  r^-3 from the hardware reciprocal square
+ *         root refined to full FP64 accuracy, fused multiply-adds, source
+ *         range split across wavefronts.  Per particle
+ *         |a - a_ref| <= 1e-14 * sum_j |a_ij|.
+ *  EXACT  IEEE-754 sqrt and division, no contraction, each target summing its
+ *         sources in ascending index order exactly as the reference's
+ *         OpenMP loop: accelerations are bit-identical to the reference.
+ *  SPLIT  Displacement and r^2 in FP64, r^-3 * m_j in FP32 on the vector
+ *         units, accumulation in FP64 with Kahan compensation applied per
+ *         tile of sources.
+ */
+enum REB_AMD_PRECISION {
+    REB_AMD_PRECISION_FP64 = 0,
+    REB_AMD_PRECISION_EXACT = 1,
+    REB_AMD_PRECISION_SPLIT = 2,
+};
+REB_API int reb_amd_set_precision(struct reb_simulation* r, int precision);
+REB_API int reb_amd_get_precision(const struct reb_simulation* r);
+
+/* ------------------------------------------------------------ device
+ * 0 when no HIP device is usable (every stepping call then records an error
+ * on the simulation instead of computing). */
+REB_API int reb_amd_device_count(void);
+REB_API int reb_amd_device_name(char* buf, size_t len);
+REB_API const char* reb_amd_version(void);
+REB_API const char* reb_amd_last_error(void);
+
+/* Device time (ms) spent in the last reb_simulation_steps/integrate call by
+ * category, as REBOUND's PROFILING categories (output.h:28-49):
+ * out[0]=gravity, out[1]=integrator, out[2]=boundary, out[3]=collision.
+ * Timing is recorded only when enabled with reb_amd_set_profiling(r,1). */
+REB_API void reb_amd_set_profiling(struct reb_simulation* r, int on);
+REB_API void reb_amd_profile(const struct reb_simulation* r, double* out4);
+
+/* With profiling on, each launch of the direct-gravity pair kernel is
+ * bracketed by HIP events on the simulation's stream; this waits for the
+ * stream and returns the summed kernel time (ms) and the number of launches
+ * since profiling was switched on or last read. */
+REB_API int reb_amd_kernel_time(struct reb_simulation* r, double* total_ms, int* launches);
+
+/* ------------------------------------------------------------ residency
+ * By default every reb_simulation_steps / integrate / update_acceleration /
+ * energy call uploads r->particles first and downloads it on return (host
+ * code may write r->particles between calls, as with REBOUND).  A resident
+ * simulation keeps its state on the GPU between calls: the host copy is
+ * refreshed only by reb_amd_sync_host, and host edits must be announced with
+ * reb_amd_mark_host_modified (reb_simulation_add does so itself). */
+REB_API void reb_amd_set_resident(struct reb_simulation* r, int on);
+REB_API int reb_amd_sync_host(struct reb_simulation* r);
+REB_API void reb_amd_mark_host_modified(struct reb_simulation* r);
+REB_API int reb_amd_synchronize_device(struct reb_simulation* r);
+
+/* ------------------------------------------------------------ raw kernels
+ * Entry points on device-resident arrays (hipMalloc'd pointers), for callers
+ * that keep their own state on the GPU.  `stream` is a hipStream_t (NULL:
+ * default stream).  Return 0 on success, nonzero on a launch error; they do
+ * not synchronize.
+ *
+ * reb_amd_gravity_direct replaces reb_gravity_basic_calculate_acceleration
+ * (gravity.c:167-282) and, with compensated=1,
+ * reb_gravity_compensated_calculate_acceleration (gravity.c:284-531), without
+ * ghost boxes.  ignore_terms takes REB_GRAVITY_IGNORE_TERMS_* values.
+ * `workspace` must hold reb_amd_gravity_workspace_bytes(N, precision) bytes. */
+REB_API size_t reb_amd_gravity_workspace_bytes(size_t N, int precision);
+REB_API int reb_amd_gravity_direct(const double* x, const double* y, const double* z, const double* m,
+                                   double* ax, double* ay, double* az,
+                                   size_t N, size_t N_active, int testparticle_type, int ignore_terms,
+                                   double G, double softening, int precision, int compensated,
+                                   void* workspace, void* stream);
+
+/* Kepler drift of n independent bodies (integrator_whfast.c:241-427): state
+ * is n rows of (x,y,z,vx,vy,vz) in host memory, updated in place; mu and dt
+ * per body.  Runs on the GPU and waits for it. */
+REB_API int reb_amd_kepler_solver_batch(double* state6, const double* mu, const double* dt, size_t n);
+
+/* Builds the octree of the simulation's current particles on the GPU
+ * (tree.c:254-271, moments tree.c:147-229) and copies up to `cap` cells into
+ * `out` in pre-order, root box by root box, 9 doubles per cell:
+ * centre x y z, width, mass, centre of mass x y z, and pt (the body index of a
+ * leaf, minus the body count of an internal cell), exactly the fields of
+ * struct reb_treecell (tree.h:34-56).  Returns the number of cells or -1. */
+REB_API long reb_amd_tree_dump(struct reb_simulation* r, double* out, long cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* REBOUND_AMD_H */

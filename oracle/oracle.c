@@ -1,0 +1,1063 @@
+/*
+ * CPU oracle -- test infrastructure only (see oracle.h).  Restates the
+ * reference's arithmetic expression by expression; compiled with
+ * -ffp-contract=off like the reference's ISO C build.
+ */
+#define _GNU_SOURCE
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define COL(s, i, c) ((s)[(size_t)(i) * ORC_NCOL + (c)])
+enum { X = 0, Y, Z, VX, VY, VZ, AX, AY, AZ, M, RR };
+
+/* ------------------------------------------------------------------ RNG
+ * glibc rand_r (restated in rebound.c:238-259) and tools.c:64-105 */
+int orc_rand_r(unsigned int* seed) {
+    unsigned int next = *seed;
+    int result;
+    next *= 1103515245;
+    next += 12345;
+    result = (unsigned int)(next / 65536) % 2048;
+    next *= 1103515245;
+    next += 12345;
+    result <<= 10;
+    result ^= (unsigned int)(next / 65536) % 1024;
+    next *= 1103515245;
+    next += 12345;
+    result <<= 10;
+    result ^= (unsigned int)(next / 65536) % 1024;
+    *seed = next;
+    return result;
+}
+
+double orc_random_uniform(unsigned int* seed, double min, double max) {
+    return ((double)orc_rand_r(seed)) / ((double)(2147483647)) * (max - min) + min;
+}
+
+double orc_random_powerlaw(unsigned int* seed, double min, double max, double slope) {
+    double y = orc_random_uniform(seed, 0., 1.);
+    if (slope == -1) return exp(y * log(max / min) + log(min));
+    return pow((pow(max, slope + 1.) - pow(min, slope + 1.)) * y + pow(min, slope + 1.), 1. / (slope + 1.));
+}
+
+double orc_random_normal(unsigned int* seed, double variance) {
+    double v1 = 0., v2 = 0., rsq = 1.;
+    while (rsq >= 1. || rsq < 1.0e-12) {
+        v1 = 2. * ((double)orc_rand_r(seed)) / ((double)(2147483647)) - 1.0;
+        v2 = 2. * ((double)orc_rand_r(seed)) / ((double)(2147483647)) - 1.0;
+        rsq = v1 * v1 + v2 * v2;
+    }
+    return v1 * sqrt(-2. * log(rsq) / rsq * variance);
+}
+
+double orc_random_rayleigh(unsigned int* seed, double sigma) {
+    double y = orc_random_uniform(seed, 0., 1.);
+    return sigma * sqrt(-2 * log(y));
+}
+
+/* ------------------------------------------------------------------ inputs */
+void orc_plummer(unsigned int* seed, size_t N, double M_, double R, double* out) {
+    double E = 3. / 64. * M_PI * M_ * M_ / R;
+    for (size_t i = 0; i < N; i++) {
+        double* p = out + i * ORC_NCOL;
+        memset(p, 0, sizeof(double) * ORC_NCOL);
+        double _r = pow(pow(orc_random_uniform(seed, 0, 1), -2. / 3.) - 1., -1. / 2.);
+        double x2 = orc_random_uniform(seed, 0, 1);
+        double x3 = orc_random_uniform(seed, 0, 2. * M_PI);
+        p[Z] = (1. - 2. * x2) * _r;
+        p[X] = sqrt(_r * _r - p[Z] * p[Z]) * cos(x3);
+        p[Y] = sqrt(_r * _r - p[Z] * p[Z]) * sin(x3);
+        double x5, g, q;
+        do {
+            x5 = orc_random_uniform(seed, 0., 1.);
+            q = orc_random_uniform(seed, 0., 1.);
+            g = q * q * pow(1. - q * q, 7. / 2.);
+        } while (0.1 * x5 > g);
+        double ve = pow(2., 1. / 2.) * pow(1. + _r * _r, -1. / 4.);
+        double v = q * ve;
+        double x6 = orc_random_uniform(seed, 0., 1.);
+        double x7 = orc_random_uniform(seed, 0., 2. * M_PI);
+        p[VZ] = (1. - 2. * x6) * v;
+        p[VX] = sqrt(v * v - p[VZ] * p[VZ]) * cos(x7);
+        p[VY] = sqrt(v * v - p[VZ] * p[VZ]) * sin(x7);
+        p[X] *= 3. * M_PI / 64. * M_ * M_ / E;
+        p[Y] *= 3. * M_PI / 64. * M_ * M_ / E;
+        p[Z] *= 3. * M_PI / 64. * M_ * M_ / E;
+        p[VX] *= sqrt(E * 64. / 3. / M_PI / M_);
+        p[VY] *= sqrt(E * 64. / 3. / M_PI / M_);
+        p[VZ] *= sqrt(E * 64. / 3. / M_PI / M_);
+        p[M] = M_ / (double)N;
+    }
+}
+
+void orc_move_to_com(double* s, size_t N) {
+    /* reb_simulation_com: pairwise mass-weighted merge in index order (tools.c:370-408) */
+    double c[10] = {0};  /* x y z vx vy vz ax ay az m */
+    for (size_t i = 0; i < N; i++) {
+        const double* p = s + i * ORC_NCOL;
+        for (int k = 0; k < 9; k++) c[k] = c[k] * c[9] + p[k] * p[M];
+        c[9] += p[M];
+        if (c[9] > 0.)
+            for (int k = 0; k < 9; k++) c[k] /= c[9];
+    }
+    for (size_t i = 0; i < N; i++)
+        for (int k = 0; k < 6; k++) COL(s, i, k) -= c[k];
+}
+
+int orc_particle_from_orbit(double G, const double* pr, double m, double a, double e, double inc, double Omega,
+                            double omega, double f, double* o) {
+    if (e == 1.) return 1;
+    if (e < 0.) return 2;
+    if (e > 1.) {
+        if (a > 0.) return 3;
+    } else {
+        if (a < 0.) return 4;
+    }
+    if (e * cos(f) < -1.) return 5;
+    if (pr[6] < 1.E-308) return 6;
+    double r = a * (1 - e * e) / (1 + e * cos(f));
+    double v0 = sqrt(G * (m + pr[6]) / a / (1. - e * e));
+    double cO = cos(Omega), sO = sin(Omega), co = cos(omega), so = sin(omega);
+    double cf = cos(f), sf = sin(f), ci = cos(inc), si = sin(inc);
+    o[0] = pr[0] + r * (cO * (co * cf - so * sf) - sO * (so * cf + co * sf) * ci);
+    o[1] = pr[1] + r * (sO * (co * cf - so * sf) + cO * (so * cf + co * sf) * ci);
+    o[2] = pr[2] + r * (so * cf + co * sf) * si;
+    o[3] = pr[3] + v0 * ((e + cf) * (-ci * co * sO - cO * so) - sf * (co * cO - ci * so * sO));
+    o[4] = pr[4] + v0 * ((e + cf) * (ci * co * cO - sO * so) - sf * (co * sO + ci * so * cO));
+    o[5] = pr[5] + v0 * ((e + cf) * co * si - sf * si * so);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ gravity */
+static int skip_pair(size_t i, size_t j, int ignore) {
+    if (ignore == 1 && ((j == 1 && i == 0) || (i == 1 && j == 0))) return 1;
+    if (ignore == 2 && (j == 0 || i == 0)) return 1;
+    return i == j;
+}
+
+void orc_gravity_basic(const double* s, size_t N, size_t N_active, int tp_type, int ignore, double G, double soft,
+                       const double* gbs, int ngb, double* acc) {
+    const double softening2 = soft * soft;
+    if (N_active > N) N_active = N;
+    for (size_t i = 0; i < N; i++) acc[3 * i] = acc[3 * i + 1] = acc[3 * i + 2] = 0;
+    for (int g = 0; g < ngb; g++) {
+        const double gx = gbs[3 * g], gy = gbs[3 * g + 1], gz = gbs[3 * g + 2];
+        for (size_t i = 0; i < N; i++) {
+            for (size_t j = 0; j < N_active; j++) {
+                if (skip_pair(i, j, ignore)) continue;
+                const double dx = (gx + COL(s, i, X)) - COL(s, j, X);
+                const double dy = (gy + COL(s, i, Y)) - COL(s, j, Y);
+                const double dz = (gz + COL(s, i, Z)) - COL(s, j, Z);
+                const double _r = sqrt(dx * dx + dy * dy + dz * dz + softening2);
+                const double prefact = -G / (_r * _r * _r) * COL(s, j, M);
+                acc[3 * i] += prefact * dx;
+                acc[3 * i + 1] += prefact * dy;
+                acc[3 * i + 2] += prefact * dz;
+            }
+        }
+        if (tp_type) {
+            for (size_t i = 0; i < N_active; i++) {
+                for (size_t j = N_active; j < N; j++) {
+                    if (ignore == 1 && (j == 1 && i == 0)) continue;
+                    if (ignore == 2 && (j == 0 || i == 0)) continue;
+                    const double dx = (gx + COL(s, i, X)) - COL(s, j, X);
+                    const double dy = (gy + COL(s, i, Y)) - COL(s, j, Y);
+                    const double dz = (gz + COL(s, i, Z)) - COL(s, j, Z);
+                    const double _r = sqrt(dx * dx + dy * dy + dz * dz + softening2);
+                    const double prefact = -G / (_r * _r * _r) * COL(s, j, M);
+                    acc[3 * i] += prefact * dx;
+                    acc[3 * i + 1] += prefact * dy;
+                    acc[3 * i + 2] += prefact * dz;
+                }
+            }
+        }
+    }
+}
+
+static void kahan(double term, double* a, double* c) {
+    double yv = term - *c;
+    double tv = *a + yv;
+    *c = (tv - *a) - yv;
+    *a = tv;
+}
+
+void orc_gravity_compensated(const double* s, size_t N, size_t N_active, int tp_type, int ignore, double G,
+                             double soft, double* acc) {
+    const double softening2 = soft * soft;
+    if (N_active > N) N_active = N;
+    double* cs = calloc(3 * N + 3, sizeof(double));
+    for (size_t i = 0; i < 3 * N; i++) acc[i] = 0.;
+    /* targets i < N_active: sources j < N_active (gravity.c:308-343);
+     * targets i >= N_active: sources j < N_active (:346-378) */
+    for (size_t i = 0; i < N; i++) {
+        for (size_t j = 0; j < N_active; j++) {
+            if (skip_pair(i, j, ignore)) continue;
+            const double dx = COL(s, i, X) - COL(s, j, X);
+            const double dy = COL(s, i, Y) - COL(s, j, Y);
+            const double dz = COL(s, i, Z) - COL(s, j, Z);
+            const double r2 = dx * dx + dy * dy + dz * dz + softening2;
+            const double r = sqrt(r2);
+            const double prefact = G / (r2 * r);
+            const double prefactj = -prefact * COL(s, j, M);
+            kahan(prefactj * dx, &acc[3 * i], &cs[3 * i]);
+            kahan(prefactj * dy, &acc[3 * i + 1], &cs[3 * i + 1]);
+            kahan(prefactj * dz, &acc[3 * i + 2], &cs[3 * i + 2]);
+        }
+    }
+    if (tp_type) { /* :379-410 active targets feel test particles, in test-particle order */
+        for (size_t j = 0; j < N_active; j++) {
+            for (size_t i = N_active; i < N; i++) {
+                if (skip_pair(i, j, ignore)) continue;
+                const double dx = COL(s, i, X) - COL(s, j, X);
+                const double dy = COL(s, i, Y) - COL(s, j, Y);
+                const double dz = COL(s, i, Z) - COL(s, j, Z);
+                const double r2 = dx * dx + dy * dy + dz * dz + softening2;
+                const double r = sqrt(r2);
+                const double prefact = G / (r2 * r);
+                const double prefacti = prefact * COL(s, i, M);
+                kahan(prefacti * dx, &acc[3 * j], &cs[3 * j]);
+                kahan(prefacti * dy, &acc[3 * j + 1], &cs[3 * j + 1]);
+                kahan(prefacti * dz, &acc[3 * j + 2], &cs[3 * j + 2]);
+            }
+        }
+    }
+    free(cs);
+}
+
+double orc_energy(const double* s, size_t N, size_t N_active, int tp_type, double G) {
+    if (N_active > N) N_active = N;
+    double e_kin = 0., e_pot = 0.;
+    size_t N_interact = (tp_type == 0) ? N_active : N;
+    for (size_t i = 0; i < N_interact; i++) {
+        const double* p = s + i * ORC_NCOL;
+        e_kin += 0.5 * p[M] * (p[VX] * p[VX] + p[VY] * p[VY] + p[VZ] * p[VZ]);
+    }
+    for (size_t i = 0; i < N_active; i++) {
+        for (size_t j = i + 1; j < N_interact; j++) {
+            double dx = COL(s, i, X) - COL(s, j, X);
+            double dy = COL(s, i, Y) - COL(s, j, Y);
+            double dz = COL(s, i, Z) - COL(s, j, Z);
+            e_pot -= G * COL(s, j, M) * COL(s, i, M) / sqrt(dx * dx + dy * dy + dz * dz);
+        }
+    }
+    return e_kin + e_pot;
+}
+
+/* ------------------------------------------------------------------ leapfrog */
+static void drift(double* s, size_t N, double dt, double* t) {
+    for (size_t i = 0; i < N; i++) {
+        COL(s, i, X) += dt * COL(s, i, VX);
+        COL(s, i, Y) += dt * COL(s, i, VY);
+        COL(s, i, Z) += dt * COL(s, i, VZ);
+    }
+    *t += dt;
+}
+static void kick(double* s, size_t N, double dt) {
+    for (size_t i = 0; i < N; i++) {
+        COL(s, i, VX) += dt * COL(s, i, AX);
+        COL(s, i, VY) += dt * COL(s, i, AY);
+        COL(s, i, VZ) += dt * COL(s, i, AZ);
+    }
+}
+
+void orc_leapfrog(double* s, size_t N, double G, double soft, double dt, long steps, double* t, long every,
+                  double* e_out) {
+    double* acc = malloc(sizeof(double) * 3 * N + 8);
+    const double gb[3] = {0, 0, 0};
+    long k = 0;
+    if (e_out && every > 0) e_out[k++] = orc_energy(s, N, N, 0, G);
+    for (long st = 0; st < steps; st++) {
+        drift(s, N, dt * 0.5, t);
+        orc_gravity_basic(s, N, N, 0, 0, G, soft, gb, 1, acc);
+        for (size_t i = 0; i < N; i++) {
+            COL(s, i, AX) = acc[3 * i];
+            COL(s, i, AY) = acc[3 * i + 1];
+            COL(s, i, AZ) = acc[3 * i + 2];
+        }
+        kick(s, N, dt);
+        drift(s, N, dt * 0.5, t);
+        if (e_out && every > 0 && (st + 1) % every == 0) e_out[k++] = orc_energy(s, N, N, 0, G);
+    }
+    free(acc);
+}
+
+/* ------------------------------------------------------------------ Kepler (integrator_whfast.c:161-427) */
+static const double invfactorial[14] = {1., 1., 1. / 2., 1. / 6., 1. / 24., 1. / 120., 1. / 720., 1. / 5040.,
+                                        1. / 40320., 1. / 362880., 1. / 3628800., 1. / 39916800.,
+                                        1. / 479001600., 1. / 6227020800.};
+
+static double fastabs(double x) { return (x > 0.) ? x : -x; }
+
+static void stumpff_cs3(double* cs, double z) {
+    unsigned int n = 0;
+    while (fabs(z) > 0.1) {
+        z = z / 4.;
+        n++;
+    }
+    double c_odd = invfactorial[13];
+    double c_even = invfactorial[12];
+    for (int np = 11; np >= 3; np -= 2) {
+        c_odd = invfactorial[np] - z * c_odd;
+        c_even = invfactorial[np - 1] - z * c_even;
+    }
+    cs[3] = c_odd;
+    cs[2] = c_even;
+    cs[1] = invfactorial[1] - z * c_odd;
+    cs[0] = invfactorial[0] - z * c_even;
+    for (; n > 0; n--) {
+        cs[3] = (cs[2] + cs[0] * cs[3]) * 0.25;
+        cs[2] = cs[1] * cs[1] * 0.5;
+        cs[1] = cs[0] * cs[1];
+        cs[0] = 2. * cs[0] * cs[0] - 1.;
+    }
+}
+
+static void stiefel_Gs3(double* Gs, double beta, double X_) {
+    double X2 = X_ * X_;
+    stumpff_cs3(Gs, beta * X2);
+    Gs[1] *= X_;
+    Gs[2] *= X2;
+    Gs[3] *= X2 * X_;
+}
+
+int orc_kepler(double* p, double mu, double dt) {
+    const double p1[6] = {p[0], p[1], p[2], p[3], p[4], p[5]};
+    int warn = 0;
+    const double r0 = sqrt(p1[0] * p1[0] + p1[1] * p1[1] + p1[2] * p1[2]);
+    const double r0i = 1. / r0;
+    const double v2 = p1[3] * p1[3] + p1[4] * p1[4] + p1[5] * p1[5];
+    const double beta = 2. * mu * r0i - v2;
+    const double eta0 = p1[0] * p1[3] + p1[1] * p1[4] + p1[2] * p1[5];
+    const double zeta0 = mu - beta * r0;
+    double Xv, Gs[6];
+    double invperiod = 0;
+    double X_per_period = nan("");
+    if (beta > 0.) {
+        const double sqrt_beta = sqrt(beta);
+        invperiod = sqrt_beta * beta / (2. * M_PI * mu);
+        X_per_period = 2. * M_PI / sqrt_beta;
+        if (fabs(dt) * invperiod > 1.) warn = 1;
+        const double dtr0i = dt * r0i;
+        Xv = dtr0i * (1. - dtr0i * eta0 * 0.5 * r0i);
+    } else {
+        Xv = 0.;
+    }
+    unsigned int converged = 0;
+    double oldX = Xv;
+    stiefel_Gs3(Gs, beta, Xv);
+    const double eg = eta0 * Gs[1] + zeta0 * Gs[2];
+    double ri = 1. / (r0 + eg);
+    Xv = ri * (Xv * eg - eta0 * Gs[2] - zeta0 * Gs[3] + dt);
+    if (fastabs(Xv - oldX) > 0.01 * X_per_period) {
+        Xv = beta * dt / mu;
+        double prevX[65];
+        for (int n_lag = 1; n_lag < 64; n_lag++) {
+            stiefel_Gs3(Gs, beta, Xv);
+            const double f = r0 * Xv + eta0 * Gs[2] + zeta0 * Gs[3] - dt;
+            const double fp = r0 + eta0 * Gs[1] + zeta0 * Gs[2];
+            const double fpp = eta0 * Gs[0] + zeta0 * Gs[1];
+            const double denom = fp + sqrt(fabs(16. * fp * fp - 20. * f * fpp));
+            Xv = (Xv * denom - 5. * f) / denom;
+            for (int i = 1; i < n_lag; i++) {
+                if (Xv == prevX[i]) {
+                    n_lag = 64;
+                    converged = 1;
+                    break;
+                }
+            }
+            prevX[n_lag] = Xv;
+        }
+        const double e1 = eta0 * Gs[1] + zeta0 * Gs[2];
+        ri = 1. / (r0 + e1);
+    } else {
+        double oldX2 = nan("");
+        for (int n_hg = 1; n_hg < 32; n_hg++) {
+            oldX2 = oldX;
+            oldX = Xv;
+            stiefel_Gs3(Gs, beta, Xv);
+            const double e1 = eta0 * Gs[1] + zeta0 * Gs[2];
+            ri = 1. / (r0 + e1);
+            Xv = ri * (Xv * e1 - eta0 * Gs[2] - zeta0 * Gs[3] + dt);
+            if (Xv == oldX || Xv == oldX2) {
+                converged = 1;
+                break;
+            }
+        }
+    }
+    if (converged == 0) {
+        double X_min, X_max;
+        if (beta > 0.) {
+            X_min = X_per_period * floor(dt * invperiod);
+            X_max = X_min + X_per_period;
+        } else {
+            double h2 = r0 * r0 * v2 - eta0 * eta0;
+            double q = h2 / mu / (1. + sqrt(1. - h2 * beta / (mu * mu)));
+            double vq = copysign(sqrt(h2) / q, dt);
+            X_min = dt / (fastabs(vq * dt) + r0);
+            X_max = dt / q;
+            if (dt < 0.) {
+                double temp = X_min;
+                X_min = X_max;
+                X_max = temp;
+            }
+        }
+        Xv = (X_max + X_min) / 2.;
+        do {
+            stiefel_Gs3(Gs, beta, Xv);
+            double sv = r0 * Xv + eta0 * Gs[2] + zeta0 * Gs[3] - dt;
+            if (sv >= 0.) {
+                X_max = Xv;
+            } else {
+                X_min = Xv;
+            }
+            Xv = (X_max + X_min) / 2.;
+        } while (fastabs((X_max - X_min)) > fastabs((X_max + X_min) * 1e-15));
+        const double e1 = eta0 * Gs[1] + zeta0 * Gs[2];
+        ri = 1. / (r0 + e1);
+    }
+    if (isnan(ri)) {
+        ri = 0.;
+        Gs[1] = 0.;
+        Gs[2] = 0.;
+        Gs[3] = 0.;
+    }
+    double f = -mu * Gs[2] * r0i;
+    double g = dt - mu * Gs[3];
+    double fd = -mu * Gs[1] * r0i * ri;
+    double gd = -mu * Gs[2] * ri;
+    p[0] += f * p1[0] + g * p1[3];
+    p[1] += f * p1[1] + g * p1[4];
+    p[2] += f * p1[2] + g * p1[5];
+    p[3] += fd * p1[0] + gd * p1[3];
+    p[4] += fd * p1[1] + gd * p1[4];
+    p[5] += fd * p1[2] + gd * p1[5];
+    return warn;
+}
+
+/* ------------------------------------------------------------------ WHFast DH */
+#define PH(ph, i, c) ((ph)[(size_t)(i) * 7 + (c)])
+
+static void dh_from_inertial(const double* s, double* ph, size_t N, size_t NA) {
+    /* transformations.c:388-424 */
+    double x0 = 0., y0 = 0., z0 = 0., vx0 = 0., vy0 = 0., vz0 = 0., m0 = 0.;
+    for (size_t i = 0; i < NA; i++) {
+        double m = COL(s, i, M);
+        x0 += COL(s, i, X) * m;
+        y0 += COL(s, i, Y) * m;
+        z0 += COL(s, i, Z) * m;
+        vx0 += COL(s, i, VX) * m;
+        vy0 += COL(s, i, VY) * m;
+        vz0 += COL(s, i, VZ) * m;
+        m0 += m;
+    }
+    PH(ph, 0, 0) = x0 / m0;
+    PH(ph, 0, 1) = y0 / m0;
+    PH(ph, 0, 2) = z0 / m0;
+    PH(ph, 0, 3) = vx0 / m0;
+    PH(ph, 0, 4) = vy0 / m0;
+    PH(ph, 0, 5) = vz0 / m0;
+    PH(ph, 0, 6) = m0;
+    for (size_t i = 1; i < N; i++) {
+        PH(ph, i, 0) = COL(s, i, X) - COL(s, 0, X);
+        PH(ph, i, 1) = COL(s, i, Y) - COL(s, 0, Y);
+        PH(ph, i, 2) = COL(s, i, Z) - COL(s, 0, Z);
+        PH(ph, i, 3) = COL(s, i, VX) - PH(ph, 0, 3);
+        PH(ph, i, 4) = COL(s, i, VY) - PH(ph, 0, 4);
+        PH(ph, i, 5) = COL(s, i, VZ) - PH(ph, 0, 5);
+        PH(ph, i, 6) = COL(s, i, M);
+    }
+}
+
+static void dh_to_inertial(double* s, const double* ph, size_t N, size_t NA) {
+    /* transformations.c:426-477 */
+    const double mtot = PH(ph, 0, 6);
+    double x0 = 0., y0 = 0., z0 = 0.;
+    for (size_t i = 1; i < NA; i++) {
+        double m = PH(ph, i, 6);
+        x0 += PH(ph, i, 0) * m / mtot;
+        y0 += PH(ph, i, 1) * m / mtot;
+        z0 += PH(ph, i, 2) * m / mtot;
+        COL(s, i, M) = m;
+    }
+    COL(s, 0, X) = PH(ph, 0, 0) - x0;
+    COL(s, 0, Y) = PH(ph, 0, 1) - y0;
+    COL(s, 0, Z) = PH(ph, 0, 2) - z0;
+    for (size_t i = 1; i < N; i++) {
+        COL(s, i, X) = PH(ph, i, 0) + COL(s, 0, X);
+        COL(s, i, Y) = PH(ph, i, 1) + COL(s, 0, Y);
+        COL(s, i, Z) = PH(ph, i, 2) + COL(s, 0, Z);
+    }
+    const double m0 = COL(s, 0, M);
+    for (size_t i = 1; i < N; i++) {
+        COL(s, i, VX) = PH(ph, i, 3) + PH(ph, 0, 3);
+        COL(s, i, VY) = PH(ph, i, 4) + PH(ph, 0, 4);
+        COL(s, i, VZ) = PH(ph, i, 5) + PH(ph, 0, 5);
+    }
+    double vx0 = 0., vy0 = 0., vz0 = 0.;
+    for (size_t i = 1; i < NA; i++) {
+        double m = COL(s, i, M);
+        vx0 += PH(ph, i, 3) * m / m0;
+        vy0 += PH(ph, i, 4) * m / m0;
+        vz0 += PH(ph, i, 5) * m / m0;
+    }
+    COL(s, 0, VX) = PH(ph, 0, 3) - vx0;
+    COL(s, 0, VY) = PH(ph, 0, 4) - vy0;
+    COL(s, 0, VZ) = PH(ph, 0, 5) - vz0;
+}
+
+static void dh_kepler_com(const double* s, double* ph, size_t N, double G, double h) {
+    const double m0 = COL(s, 0, M);
+    for (size_t i = 1; i < N; i++) {
+        double st[6];
+        for (int k = 0; k < 6; k++) st[k] = PH(ph, i, k);
+        orc_kepler(st, m0 * G, h);
+        for (int k = 0; k < 6; k++) PH(ph, i, k) = st[k];
+    }
+    PH(ph, 0, 0) += h * PH(ph, 0, 3);
+    PH(ph, 0, 1) += h * PH(ph, 0, 4);
+    PH(ph, 0, 2) += h * PH(ph, 0, 5);
+}
+
+static void dh_jump(const double* s, double* ph, size_t N, size_t NA, double h) {
+    /* integrator_whfast.c:537-554 */
+    const double m0 = COL(s, 0, M);
+    double px = 0, py = 0, pz = 0;
+    for (size_t i = 1; i < NA; i++) {
+        const double m = COL(s, i, M);
+        px += m * PH(ph, i, 3);
+        py += m * PH(ph, i, 4);
+        pz += m * PH(ph, i, 5);
+    }
+    for (size_t i = 1; i < N; i++) {
+        PH(ph, i, 0) += h * (px / m0);
+        PH(ph, i, 1) += h * (py / m0);
+        PH(ph, i, 2) += h * (pz / m0);
+    }
+}
+
+void orc_whfast_dh(double* s, double* ph, size_t N, double G, double soft, double dt, long steps, int safe_mode,
+                   int* is_sync, int did_modify) {
+    const size_t NA = N;
+    double* acc = malloc(sizeof(double) * 3 * N + 8);
+    const double gb[3] = {0, 0, 0};
+    for (long st = 0; st < steps; st++) {
+        if (safe_mode || did_modify) {
+            if (*is_sync == 0) {
+                dh_kepler_com(s, ph, N, G, dt / 2.);
+                dh_to_inertial(s, ph, N, NA);
+                *is_sync = 1;
+            }
+            dh_from_inertial(s, ph, N, NA);
+        }
+        if (*is_sync) {
+            dh_kepler_com(s, ph, N, G, dt / 2.);
+        } else {
+            dh_kepler_com(s, ph, N, G, dt);
+        }
+        dh_jump(s, ph, N, NA, dt / 2.);
+        dh_to_inertial(s, ph, N, NA);
+        orc_gravity_basic(s, N, N, 0, 2, G, soft, gb, 1, acc);
+        for (size_t i = 0; i < N; i++) {
+            COL(s, i, AX) = acc[3 * i];
+            COL(s, i, AY) = acc[3 * i + 1];
+            COL(s, i, AZ) = acc[3 * i + 2];
+        }
+        for (size_t i = 1; i < N; i++) {
+            PH(ph, i, 3) += dt * COL(s, i, AX);
+            PH(ph, i, 4) += dt * COL(s, i, AY);
+            PH(ph, i, 5) += dt * COL(s, i, AZ);
+        }
+        dh_jump(s, ph, N, NA, dt / 2.);
+        *is_sync = 0;
+        if (safe_mode) {
+            dh_kepler_com(s, ph, N, G, dt / 2.);
+            dh_to_inertial(s, ph, N, NA);
+            *is_sync = 1;
+        }
+        did_modify = 0;
+    }
+    /* reb_simulation_steps ends with a synchronize (simulation.c:510) */
+    if (*is_sync == 0) {
+        dh_kepler_com(s, ph, N, G, dt / 2.);
+        dh_to_inertial(s, ph, N, NA);
+        *is_sync = 1;
+    }
+    free(acc);
+}
+
+/* ------------------------------------------------------------------ SEI + shear boundary */
+void orc_sei_operator(double* s, size_t N, double OMEGA, double OMEGAZ, double dt, int with_kick) {
+    const double sindt = sin(OMEGA * (-dt / 2.));
+    const double tandt = tan(OMEGA * (-dt / 4.));
+    const double sindtz = sin(OMEGAZ * (-dt / 2.));
+    const double tandtz = tan(OMEGAZ * (-dt / 4.));
+    for (size_t i = 0; i < N; i++) {
+        double* p = s + i * ORC_NCOL;
+        if (with_kick) {
+            p[VX] += p[AX] * dt;
+            p[VY] += p[AY] * dt;
+            p[VZ] += p[AZ] * dt;
+        }
+        const double zx = p[Z] * OMEGAZ;
+        const double zy = p[VZ];
+        const double zt1 = zx - tandtz * zy;
+        const double zyt = sindtz * zt1 + zy;
+        const double zxt = zt1 - tandtz * zyt;
+        p[Z] = zxt / OMEGAZ;
+        p[VZ] = zyt;
+        const double aO = 2. * p[VY] + 4. * p[X] * OMEGA;
+        const double bO = p[Y] * OMEGA - 2. * p[VX];
+        const double ys = (p[Y] * OMEGA - bO) / 2.;
+        const double xs = (p[X] * OMEGA - aO);
+        const double xst1 = xs - tandt * ys;
+        const double yst = sindt * xst1 + ys;
+        const double xst = xst1 - tandt * yst;
+        p[X] = (xst + aO) / OMEGA;
+        p[Y] = (yst * 2. + bO) / OMEGA - 3. / 4. * aO * dt;
+        p[VX] = yst;
+        p[VY] = -xst * 2. - 3. / 2. * aO;
+    }
+}
+
+void orc_shear_wrap(double* s, size_t N, double OMEGA, double bx, double by, double bz, double t) {
+    const double offsetp1 = -fmod(-1.5 * OMEGA * bx * t + by / 2., by) - by / 2.;
+    const double offsetm1 = -fmod(1.5 * OMEGA * bx * t - by / 2., by) + by / 2.;
+    for (size_t i = 0; i < N; i++) {
+        double* p = s + i * ORC_NCOL;
+        while (p[X] > bx / 2.) {
+            p[X] -= bx;
+            p[Y] += offsetp1;
+            p[VY] += 3. / 2. * OMEGA * bx;
+        }
+        while (p[X] < -bx / 2.) {
+            p[X] += bx;
+            p[Y] += offsetm1;
+            p[VY] -= 3. / 2. * OMEGA * bx;
+        }
+        while (p[Y] > by / 2.) p[Y] -= by;
+        while (p[Y] < -by / 2.) p[Y] += by;
+        while (p[Z] > bz / 2.) p[Z] -= bz;
+        while (p[Z] < -bz / 2.) p[Z] += bz;
+    }
+}
+
+void orc_ghostbox_shear(double OMEGA, double bx, double by, double bz, double t, int i, int j, int k, double* gb) {
+    gb[3] = 0.;
+    gb[4] = -1.5 * (double)i * OMEGA * bx;
+    gb[5] = 0.;
+    double shift;
+    if (i == 0) {
+        shift = -fmod(gb[4] * t, by);
+    } else if (i > 0) {
+        shift = -fmod(gb[4] * t - by / 2., by) - by / 2.;
+    } else {
+        shift = -fmod(gb[4] * t + by / 2., by) + by / 2.;
+    }
+    gb[0] = bx * (double)i;
+    gb[1] = by * (double)j - shift;
+    gb[2] = bz * (double)k;
+}
+
+/* ------------------------------------------------------------------ tree (tree.c) */
+typedef struct cell {
+    double x, y, z, w, m, mx, my, mz;
+    struct cell* oct[8];
+    int pt;
+} cell;
+
+typedef struct {
+    const double* s;
+    size_t N;
+    double root_size;
+    int nrx, nry, nrz;
+    int err;
+} treectx;
+
+static int octant(const double* p, const cell* c) {
+    int o = 0;
+    if (p[X] < c->x) o += 1;
+    if (p[Y] < c->y) o += 2;
+    if (p[Z] < c->z) o += 4;
+    return o;
+}
+
+static cell* add_to_cell(treectx* T, cell* node, int pt, cell* parent, int o) {
+    const double* s = T->s;
+    if (node == NULL) {
+        node = calloc(1, sizeof(cell));
+        const double* p = s + (size_t)pt * ORC_NCOL;
+        if (parent == NULL) {
+            node->w = T->root_size;
+            double bx = T->root_size * (double)T->nrx, by = T->root_size * (double)T->nry,
+                   bz = T->root_size * (double)T->nrz;
+            int i = ((int)floor((p[X] + bx / 2.) / T->root_size)) % T->nrx;
+            int j = ((int)floor((p[Y] + by / 2.) / T->root_size)) % T->nry;
+            int k = ((int)floor((p[Z] + bz / 2.) / T->root_size)) % T->nrz;
+            node->x = -bx / 2. + T->root_size * (0.5 + (double)i);
+            node->y = -by / 2. + T->root_size * (0.5 + (double)j);
+            node->z = -bz / 2. + T->root_size * (0.5 + (double)k);
+        } else {
+            node->w = parent->w / 2.;
+            node->x = parent->x + node->w / 2. * ((o >> 0) % 2 == 0 ? 1. : -1);
+            node->y = parent->y + node->w / 2. * ((o >> 1) % 2 == 0 ? 1. : -1);
+            node->z = parent->z + node->w / 2. * ((o >> 2) % 2 == 0 ? 1. : -1);
+        }
+        if (node->w <= 0.0) {
+            T->err = 1;
+            free(node);
+            return NULL;
+        }
+        node->pt = pt;
+        return node;
+    }
+    if (node->pt >= 0) {
+        const double* a = s + (size_t)node->pt * ORC_NCOL;
+        const double* b = s + (size_t)pt * ORC_NCOL;
+        int o1 = octant(a, node), o2 = octant(b, node);
+        if (o1 == o2 && a[X] == b[X] && a[Y] == b[Y] && a[Z] == b[Z]) {
+            T->err = 2;
+            return node;
+        }
+        node->oct[o1] = add_to_cell(T, node->oct[o1], node->pt, node, o1);
+        node->oct[o2] = add_to_cell(T, node->oct[o2], pt, node, o2);
+        node->pt = -2;
+    } else {
+        node->pt--;
+        int o = octant(s + (size_t)pt * ORC_NCOL, node);
+        node->oct[o] = add_to_cell(T, node->oct[o], pt, node, o);
+    }
+    return node;
+}
+
+static void moments(const double* s, cell* c) {
+    if (c->pt < 0) {
+        c->m = 0;
+        c->mx = 0;
+        c->my = 0;
+        c->mz = 0;
+        for (int o = 0; o < 8; o++) {
+            cell* d = c->oct[o];
+            if (d) {
+                moments(s, d);
+                double d_m = d->m;
+                c->mx += d->mx * d_m;
+                c->my += d->my * d_m;
+                c->mz += d->mz * d_m;
+                c->m += d_m;
+            }
+        }
+        double m_tot = c->m;
+        if (m_tot > 0) {
+            c->mx /= m_tot;
+            c->my /= m_tot;
+            c->mz /= m_tot;
+        }
+    } else {
+        const double* p = s + (size_t)c->pt * ORC_NCOL;
+        c->m = p[M];
+        c->mx = p[X];
+        c->my = p[Y];
+        c->mz = p[Z];
+    }
+}
+
+static void delete_cell(cell* c) {
+    if (!c) return;
+    for (int o = 0; o < 8; o++) delete_cell(c->oct[o]);
+    free(c);
+}
+
+static int rootbox_index(const treectx* T, const double* p) {
+    int i = ((int)floor((p[X] + T->root_size * (double)T->nrx / 2.) / T->root_size) + T->nrx) % T->nrx;
+    int j = ((int)floor((p[Y] + T->root_size * (double)T->nry / 2.) / T->root_size) + T->nry) % T->nry;
+    int k = ((int)floor((p[Z] + T->root_size * (double)T->nrz / 2.) / T->root_size) + T->nrz) % T->nrz;
+    return (k * T->nry + j) * T->nrx + i;
+}
+
+static cell** build(treectx* T) {
+    int nroot = T->nrx * T->nry * T->nrz;
+    cell** roots = calloc(nroot, sizeof(cell*));
+    for (size_t i = 0; i < T->N; i++) {
+        const double* p = T->s + i * ORC_NCOL;
+        if (fabs(p[X]) > T->root_size * (double)T->nrx / 2. || fabs(p[Y]) > T->root_size * (double)T->nry / 2. ||
+            fabs(p[Z]) > T->root_size * (double)T->nrz / 2.) {
+            T->err = 3;
+            return roots;
+        }
+        if (!isfinite(p[X]) || !isfinite(p[Y]) || !isfinite(p[Z])) {
+            T->err = 4;
+            return roots;
+        }
+        int rb = rootbox_index(T, p);
+        roots[rb] = add_to_cell(T, roots[rb], (int)i, NULL, 0);
+    }
+    for (int r = 0; r < nroot; r++)
+        if (roots[r]) moments(T->s, roots[r]);
+    return roots;
+}
+
+static void dump(const cell* c, double* out, long cap, long* n) {
+    if (!c) return;
+    if (out && *n < cap) {
+        double row[9] = {c->x, c->y, c->z, c->w, c->m, c->mx, c->my, c->mz, (double)c->pt};
+        memcpy(out + 9 * (*n), row, sizeof(row));
+    }
+    (*n)++;
+    for (int o = 0; o < 8; o++) dump(c->oct[o], out, cap, n);
+}
+
+long orc_tree_dump(const double* s, size_t N, double root_size, int nrx, int nry, int nrz, double* cells, long cap,
+                   long* rootstart) {
+    treectx T = {s, N, root_size, nrx, nry, nrz, 0};
+    cell** roots = build(&T);
+    long n = 0;
+    int nroot = nrx * nry * nrz;
+    for (int r = 0; r < nroot; r++) {
+        if (rootstart) rootstart[r] = n;
+        dump(roots[r], cells, cap, &n);
+    }
+    if (rootstart) rootstart[nroot] = n;
+    for (int r = 0; r < nroot; r++) delete_cell(roots[r]);
+    free(roots);
+    return T.err ? -1 : n;
+}
+
+typedef struct {
+    const double* s;
+    double G, soft2, theta2;
+    long inter;
+} walkctx;
+
+static void walk(walkctx* W, int pt, const cell* node, const double* gb, double* a) {
+    const double dx = gb[0] - node->mx;
+    const double dy = gb[1] - node->my;
+    const double dz = gb[2] - node->mz;
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    if (node->pt < 0) {
+        if (node->w * node->w > W->theta2 * r2) {
+            for (int o = 0; o < 8; o++)
+                if (node->oct[o]) walk(W, pt, node->oct[o], gb, a);
+        } else {
+            double _r = sqrt(r2 + W->soft2);
+            double prefact = -W->G / (_r * _r * _r) * node->m;
+            a[0] += prefact * dx;
+            a[1] += prefact * dy;
+            a[2] += prefact * dz;
+            W->inter++;
+        }
+    } else {
+        if (node->pt == pt) return;
+        double _r = sqrt(r2 + W->soft2);
+        double prefact = -W->G / (_r * _r * _r) * node->m;
+        a[0] += prefact * dx;
+        a[1] += prefact * dy;
+        a[2] += prefact * dz;
+        W->inter++;
+    }
+}
+
+int orc_tree_gravity(const double* s, size_t N, double root_size, int nrx, int nry, int nrz, double G, double soft,
+                     double opening_angle2, const double* gbs, int ngb, double* acc, long* interactions) {
+    treectx T = {s, N, root_size, nrx, nry, nrz, 0};
+    cell** roots = build(&T);
+    int nroot = nrx * nry * nrz;
+    walkctx W = {s, G, soft * soft, opening_angle2, 0};
+    for (size_t i = 0; i < 3 * N; i++) acc[i] = 0.;
+    if (!T.err) {
+        for (int g = 0; g < ngb; g++) {
+            for (size_t i = 0; i < N; i++) {
+                double gb[3] = {gbs[3 * g] + COL(s, i, X), gbs[3 * g + 1] + COL(s, i, Y), gbs[3 * g + 2] + COL(s, i, Z)};
+                for (int r = 0; r < nroot; r++)
+                    if (roots[r]) walk(&W, (int)i, roots[r], gb, acc + 3 * i);
+            }
+        }
+    }
+    for (int r = 0; r < nroot; r++) delete_cell(roots[r]);
+    free(roots);
+    if (interactions) *interactions = W.inter;
+    return T.err;
+}
+
+/* ------------------------------------------------------------------ collisions (collision.c) */
+typedef struct {
+    const double* s;
+    double second_r;
+    int64_t* pairs;
+    double* gbo;
+    long cap, n;
+} collctx;
+
+static void nn_in_cell(collctx* C, const double* gb, const double* gbunmod, size_t p1, double p1_r, const cell* c) {
+    const double* s = C->s;
+    if (c->pt >= 0) {
+        if ((size_t)c->pt == p1) return;
+        const double* p2 = s + (size_t)c->pt * ORC_NCOL;
+        double dx = gb[0] - p2[X];
+        double dy = gb[1] - p2[Y];
+        double dz = gb[2] - p2[Z];
+        double r2 = dx * dx + dy * dy + dz * dz;
+        double rp = p1_r + p2[RR];
+        if (r2 > rp * rp) return;
+        double dvx = gb[3] - p2[VX];
+        double dvy = gb[4] - p2[VY];
+        double dvz = gb[5] - p2[VZ];
+        if (dvx * dx + dvy * dy + dvz * dz > 0) return;
+        if (C->n < C->cap) {
+            C->pairs[2 * C->n] = (int64_t)p1;
+            C->pairs[2 * C->n + 1] = c->pt;
+            memcpy(C->gbo + 6 * C->n, gbunmod, 6 * sizeof(double));
+        }
+        C->n++;
+    } else {
+        double dx = gb[0] - c->x;
+        double dy = gb[1] - c->y;
+        double dz = gb[2] - c->z;
+        double r2 = dx * dx + dy * dy + dz * dz;
+        double rp = p1_r + C->second_r + 0.86602540378443 * c->w;
+        if (r2 < rp * rp) {
+            for (int o = 0; o < 8; o++)
+                if (c->oct[o]) nn_in_cell(C, gb, gbunmod, p1, p1_r, c->oct[o]);
+        }
+    }
+}
+
+static double second_largest_radius(const double* s, size_t N) {
+    /* simulation.c:766-779 (serial) */
+    double largest1 = -1.0, largest2 = -1.0;
+    long p1 = -1, p2 = -1;
+    for (size_t i = 0; i < N; i++) {
+        if (COL(s, i, RR) > largest1) {
+            largest2 = largest1;
+            p2 = p1;
+            largest1 = COL(s, i, RR);
+            p1 = (long)i;
+        } else if (COL(s, i, RR) > largest2) {
+            largest2 = COL(s, i, RR);
+            p2 = (long)i;
+        }
+    }
+    return p2 >= 0 ? COL(s, p2, RR) : 0.;
+}
+
+long orc_collision_tree(const double* s, size_t N, double root_size, int nrx, int nry, int nrz, const double* gbs,
+                        int ngb, int64_t* pairs, double* gbo, long cap) {
+    treectx T = {s, N, root_size, nrx, nry, nrz, 0};
+    cell** roots = build(&T);
+    int nroot = nrx * nry * nrz;
+    collctx C = {s, second_largest_radius(s, N), pairs, gbo, cap, 0};
+    if (!T.err) {
+        for (size_t i = 0; i < N; i++) {
+            const double* p1 = s + i * ORC_NCOL;
+            for (int g = 0; g < ngb; g++) {
+                const double* gu = gbs + 6 * g;
+                double gb[6] = {gu[0] + p1[X], gu[1] + p1[Y], gu[2] + p1[Z],
+                                gu[3] + p1[VX], gu[4] + p1[VY], gu[5] + p1[VZ]};
+                for (int r = 0; r < nroot; r++)
+                    if (roots[r]) nn_in_cell(&C, gb, gu, i, p1[RR], roots[r]);
+            }
+        }
+    }
+    for (int r = 0; r < nroot; r++) delete_cell(roots[r]);
+    free(roots);
+    return T.err ? -1 : C.n;
+}
+
+long orc_collision_direct(const double* s, size_t N, const double* gbs, int ngb, int64_t* pairs, double* gbo,
+                          long cap) {
+    long n = 0;
+    for (int g = 0; g < ngb; g++) {
+        const double* gu = gbs + 6 * g;
+        for (size_t i = 0; i < N; i++) {
+            const double* p1 = s + i * ORC_NCOL;
+            double gb[6] = {gu[0] + p1[X], gu[1] + p1[Y], gu[2] + p1[Z], gu[3] + p1[VX], gu[4] + p1[VY], gu[5] + p1[VZ]};
+            for (size_t j = 0; j < N; j++) {
+                if (i == j) continue;
+                const double* p2 = s + j * ORC_NCOL;
+                double dx = gb[0] - p2[X];
+                double dy = gb[1] - p2[Y];
+                double dz = gb[2] - p2[Z];
+                double sr = p1[RR] + p2[RR];
+                double r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 > sr * sr) continue;
+                double dvx = gb[3] - p2[VX];
+                double dvy = gb[4] - p2[VY];
+                double dvz = gb[5] - p2[VZ];
+                if (dvx * dx + dvy * dy + dvz * dz > 0) continue;
+                if (n < cap) {
+                    pairs[2 * n] = (int64_t)i;
+                    pairs[2 * n + 1] = (int64_t)j;
+                    memcpy(gbo + 6 * n, gu, 6 * sizeof(double));
+                }
+                n++;
+            }
+        }
+    }
+    return n;
+}
+
+void orc_collision_shuffle(int64_t* pairs, double* gbo, long n, unsigned int* seed) {
+    for (long i = 0; i < n; i++) {
+        long nw = (long)((unsigned int)orc_rand_r(seed) % (unsigned long)n);
+        int64_t a0 = pairs[2 * i], a1 = pairs[2 * i + 1];
+        double g[6];
+        memcpy(g, gbo + 6 * i, sizeof(g));
+        pairs[2 * i] = pairs[2 * nw];
+        pairs[2 * i + 1] = pairs[2 * nw + 1];
+        memcpy(gbo + 6 * i, gbo + 6 * nw, sizeof(g));
+        pairs[2 * nw] = a0;
+        pairs[2 * nw + 1] = a1;
+        memcpy(gbo + 6 * nw, g, sizeof(g));
+    }
+}
+
+double orc_hardsphere(double* s, int64_t i1, int64_t i2, const double* gb, double eps_c, double min_v, long* log_n) {
+    double* P1 = s + (size_t)i1 * ORC_NCOL;
+    double* P2 = s + (size_t)i2 * ORC_NCOL;
+    double p1[ORC_NCOL], p2[ORC_NCOL];
+    memcpy(p1, P1, sizeof(p1));
+    memcpy(p2, P2, sizeof(p2));
+    double x21 = p1[X] + gb[0] - p2[X];
+    double y21 = p1[Y] + gb[1] - p2[Y];
+    double z21 = p1[Z] + gb[2] - p2[Z];
+    double rp = p1[RR] + p2[RR];
+    double oldvyouter = (x21 > 0) ? p1[VY] : p2[VY];
+    if (rp * rp < x21 * x21 + y21 * y21 + z21 * z21) return 0.;
+    double vx21 = p1[VX] + gb[3] - p2[VX];
+    double vy21 = p1[VY] + gb[4] - p2[VY];
+    double vz21 = p1[VZ] + gb[5] - p2[VZ];
+    if (vx21 * x21 + vy21 * y21 + vz21 * z21 > 0) return 0.;
+    double theta = atan2(z21, y21);
+    double stheta = sin(theta), ctheta = cos(theta);
+    double vy21n = ctheta * vy21 + stheta * vz21;
+    double y21n = ctheta * y21 + stheta * z21;
+    double phi = atan2(y21n, x21);
+    double cphi = cos(phi), sphi = sin(phi);
+    double vx21nn = cphi * vx21 + sphi * vy21n;
+    double eps = eps_c < 0 ? 1 : eps_c;
+    double dvx2 = -(1.0 + eps) * vx21nn;
+    double minr = (p1[RR] > p2[RR]) ? p2[RR] : p1[RR];
+    double maxr = (p1[RR] < p2[RR]) ? p2[RR] : p1[RR];
+    double mindv = minr * min_v;
+    double _r = sqrt(x21 * x21 + y21 * y21 + z21 * z21);
+    mindv *= 1. - (_r - maxr) / minr;
+    if (mindv > maxr * min_v) mindv = maxr * min_v;
+    if (dvx2 < mindv) dvx2 = mindv;
+    double dvx2n = cphi * dvx2;
+    double dvy2n = sphi * dvx2;
+    double dvy2nn = ctheta * dvy2n;
+    double dvz2nn = stheta * dvy2n;
+    const double p2pf = p1[M] / (p1[M] + p2[M]);
+    P2[VX] -= p2pf * dvx2n;
+    P2[VY] -= p2pf * dvy2nn;
+    P2[VZ] -= p2pf * dvz2nn;
+    const double p1pf = p2[M] / (p1[M] + p2[M]);
+    P1[VX] += p1pf * dvx2n;
+    P1[VY] += p1pf * dvy2nn;
+    P1[VZ] += p1pf * dvz2nn;
+    (*log_n)++;
+    if (x21 > 0) return -fabs(x21) * (oldvyouter - P1[VY]) * p1[M];
+    return -fabs(x21) * (oldvyouter - P2[VY]) * p2[M];
+}

@@ -1,0 +1,130 @@
+/*
+ * CPU-baseline timing driver for the reference's OpenMP build
+ * (oracle/_ref/omp/librebound.so, compiled from the reference's sources).
+ * Prints one JSON line.  Only the step loop is timed (CLOCK_MONOTONIC).
+ *
+ *   ref_bench plummer N STEPS SOFTENING DT basic|compensated SEED
+ *       Plummer sphere (tools.c:463-502), leapfrog (integrator_leapfrog.c:97-209),
+ *       direct gravity in the reference's OpenMP per-i form (gravity.c:215-232).
+ *   ref_bench ensemble NSYS STEPS SEED SAFE_MODE KMIN KMAX
+ *       WHFast democratic-heliocentric planetary systems, one simulation per
+ *       system, `omp parallel for` over systems (each integrated serially).
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#include <time.h>
+#include <omp.h>
+#include "rebound.h"
+
+static double now(void){
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9*ts.tv_nsec;
+}
+
+static int bench_plummer(int argc, char** argv){
+    if (argc < 8){ fprintf(stderr, "plummer N STEPS SOFT DT MODE SEED\n"); return 1; }
+    size_t N = strtoull(argv[2], 0, 10);
+    int steps = atoi(argv[3]);
+    double soft = atof(argv[4]);
+    double dt = atof(argv[5]);
+    int comp = strcmp(argv[6], "compensated") == 0;
+    unsigned int seed = (unsigned int)strtoul(argv[7], 0, 10);
+    struct reb_simulation* r = reb_simulation_create();
+    r->rand_seed = seed;
+    r->softening = soft;
+    r->G = 1.;
+    reb_simulation_add_plummer(r, N, 1., 1.);
+    reb_simulation_move_to_com(r);
+    reb_simulation_set_integrator(r, "leapfrog");
+    r->gravity = comp ? REB_GRAVITY_COMPENSATED : REB_GRAVITY_BASIC;
+    r->dt = dt;
+    reb_simulation_steps(r, 1);                 /* warm-up step (not timed) */
+    double t0 = now();
+    reb_simulation_steps(r, steps);
+    double t1 = now();
+    double pairs = (double)N*(double)(N-1)*steps;
+    printf("{\"workload\": \"plummer\", \"N\": %zu, \"steps\": %d, \"mode\": \"%s\", "
+           "\"seconds\": %.6f, \"pairs_per_s\": %.6e, \"threads\": %d}\n",
+           N, steps, comp ? "compensated" : "basic", t1-t0, pairs/(t1-t0), omp_get_max_threads());
+    reb_simulation_free(r);
+    return 0;
+}
+
+/* Planetary-system generator shared (by construction) with the product's
+ * ensemble workload: star m=1, K planets, a_1=1, log-uniform period ratios in
+ * [1.3,2.5], log-uniform masses in [1e-6,1e-3], Rayleigh e (0.02) and i (0.01),
+ * uniform angles, primary = star, then move_to_com; dt = P_1/20. */
+static struct reb_simulation* planetary_system(struct reb_simulation* g, int kmin, int kmax, int safe_mode){
+    struct reb_simulation* r = reb_simulation_create();
+    int K = kmin + (int)floor(reb_random_uniform(g, 0., 1.)*(double)(kmax-kmin+1));
+    if (K > kmax) K = kmax;
+    struct reb_particle star = {0}; star.m = 1.;
+    reb_simulation_add(r, star);
+    double a = 1.;
+    for (int k=0; k<K; k++){
+        if (k > 0){
+            double ratio = exp(reb_random_uniform(g, log(1.3), log(2.5)));
+            a *= pow(ratio, 2./3.);
+        }
+        double m  = exp(reb_random_uniform(g, log(1e-6), log(1e-3)));
+        double e  = reb_random_rayleigh(g, 0.02);
+        double in = reb_random_rayleigh(g, 0.01);
+        double Om = reb_random_uniform(g, 0., 2.*M_PI);
+        double om = reb_random_uniform(g, 0., 2.*M_PI);
+        double f  = reb_random_uniform(g, 0., 2.*M_PI);
+        reb_simulation_add(r, reb_particle_from_orbit(r->G, r->particles[0], m, a, e, in, Om, om, f));
+    }
+    reb_simulation_move_to_com(r);
+    struct reb_integrator_whfast_state* wh = reb_simulation_set_integrator(r, "whfast");
+    wh->coordinates = REB_INTEGRATOR_WHFAST_COORDINATES_DEMOCRATICHELIOCENTRIC;
+    wh->safe_mode = safe_mode;
+    r->dt = 2.*M_PI*sqrt(1./(1.+r->particles[1].m))/20.;
+    return r;
+}
+
+static int bench_ensemble(int argc, char** argv){
+    if (argc < 8){ fprintf(stderr, "ensemble NSYS STEPS SEED SAFE KMIN KMAX\n"); return 1; }
+    int nsys = atoi(argv[2]);
+    long steps = atol(argv[3]);
+    unsigned int seed = (unsigned int)strtoul(argv[4], 0, 10);
+    int safe = atoi(argv[5]);
+    int kmin = atoi(argv[6]), kmax = atoi(argv[7]);
+    struct reb_simulation* g = reb_simulation_create();
+    g->rand_seed = seed;
+    struct reb_simulation** sims = malloc(sizeof(*sims)*nsys);
+    for (int s=0; s<nsys; s++) sims[s] = planetary_system(g, kmin, kmax, safe);
+    double* e0 = malloc(sizeof(double)*nsys);
+    for (int s=0; s<nsys; s++) e0[s] = reb_simulation_energy(sims[s]);
+    double t0 = now();
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int s=0; s<nsys; s++){
+        reb_simulation_steps(sims[s], (size_t)steps);
+    }
+    double t1 = now();
+    double maxerr = 0., sumerr = 0.;
+    for (int s=0; s<nsys; s++){
+        double de = fabs((reb_simulation_energy(sims[s]) - e0[s])/e0[s]);
+        if (de > maxerr) maxerr = de;
+        sumerr += de;
+        reb_simulation_free(sims[s]);
+    }
+    printf("{\"workload\": \"ensemble\", \"systems\": %d, \"steps\": %ld, \"safe_mode\": %d, "
+           "\"seconds\": %.6f, \"system_steps_per_s\": %.6e, \"threads\": %d, "
+           "\"max_rel_energy_error\": %.3e, \"mean_rel_energy_error\": %.3e}\n",
+           nsys, steps, safe, t1-t0, (double)nsys*(double)steps/(t1-t0), omp_get_max_threads(),
+           maxerr, sumerr/nsys);
+    free(sims); free(e0);
+    reb_simulation_free(g);
+    return 0;
+}
+
+int main(int argc, char** argv){
+    if (argc < 2){ fprintf(stderr, "usage: ref_bench plummer|ensemble ...\n"); return 1; }
+    if (strcmp(argv[1], "plummer") == 0) return bench_plummer(argc, argv);
+    if (strcmp(argv[1], "ensemble") == 0) return bench_ensemble(argc, argv);
+    fprintf(stderr, "unknown workload %s\n", argv[1]);
+    return 1;
+}

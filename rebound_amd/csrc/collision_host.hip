@@ -1,0 +1,290 @@
+// Collision resolution (collision.c:336-404 and the built-in resolvers
+// :573-737).
+//
+// The search runs on the GPU (tree.hip) and hands over the collisions in the
+// reference's detection order.  Resolution is sequential by definition (a
+// body may collide twice in one step, each resolution sees the velocities the
+// previous one produced) and calls user callbacks (collision_resolve,
+// coefficient_of_restitution), so it runs on the host: the shuffle draws from
+// r->rand_seed with rand_r as the reference does, and the built-in
+// hard-sphere resolver uses the host libm (atan2/sin/cos) like the reference.
+// Only the bodies named in the collision list travel between device and host
+// when the resolver is one of the built-ins that touch p1 and p2 alone.
+#include <cmath>
+#include <cstring>
+#include <unordered_map>
+
+#include "internal.h"
+
+namespace rbamd {
+
+// Gathers / scatters the full state of a list of bodies.
+__global__ void k_gather_bodies(const double* __restrict__ soa, size_t cap, const int* __restrict__ ids, int n,
+                                double* __restrict__ out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int i = ids[k];
+    for (int f = 0; f < NFIELD; f++) out[(size_t)k * NFIELD + f] = soa[(size_t)f * cap + i];
+}
+
+__global__ void k_scatter_bodies(double* __restrict__ soa, size_t cap, const int* __restrict__ ids, int n,
+                                 const double* __restrict__ in) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int i = ids[k];
+    for (int f = 0; f < NFIELD; f++) soa[(size_t)f * cap + i] = in[(size_t)k * NFIELD + f];
+}
+
+static void copy_body_in(struct reb_particle& p, const double* s) {
+    p.x = s[FX]; p.y = s[FY]; p.z = s[FZ];
+    p.vx = s[FVX]; p.vy = s[FVY]; p.vz = s[FVZ];
+    p.ax = s[FAX]; p.ay = s[FAY]; p.az = s[FAZ];
+    p.m = s[FM]; p.r = s[FR];
+}
+static void copy_body_out(const struct reb_particle& p, double* s) {
+    s[FX] = p.x; s[FY] = p.y; s[FZ] = p.z;
+    s[FVX] = p.vx; s[FVY] = p.vy; s[FVZ] = p.vz;
+    s[FAX] = p.ax; s[FAY] = p.ay; s[FAZ] = p.az;
+    s[FM] = p.m; s[FR] = p.r;
+}
+
+// Brings the listed bodies' device state into r->particles (host), or back.
+static bool move_bodies(struct reb_simulation* r, const std::vector<int>& ids, bool to_host) {
+    DeviceState* d = dev_of(r);
+    const int n = (int)ids.size();
+    if (n == 0) return true;
+    const size_t idbytes = sizeof(int) * n, databytes = sizeof(double) * NFIELD * n;
+    if (!d->aos.ensure(idbytes + databytes + 256)) return false;
+    int* dids = d->aos.as<int>();
+    double* ddata = reinterpret_cast<double*>(d->aos.as<char>() + ((idbytes + 255) / 256) * 256);
+    std::vector<double> h((size_t)NFIELD * n);
+    if (!RB_HIP(r, hipMemcpyAsync(dids, ids.data(), idbytes, hipMemcpyHostToDevice, d->stream))) return false;
+    if (to_host) {
+        hipLaunchKernelGGL(k_gather_bodies, dim3((n + 127) / 128), dim3(128), 0, d->stream, d->soa.as<double>(),
+                           d->cap, dids, n, ddata);
+        if (!RB_HIP(r, hipMemcpyAsync(h.data(), ddata, databytes, hipMemcpyDeviceToHost, d->stream))) return false;
+        if (!stream_sync(r)) return false;
+        for (int k = 0; k < n; k++) copy_body_in(r->particles[ids[k]], h.data() + (size_t)k * NFIELD);
+    } else {
+        for (int k = 0; k < n; k++) copy_body_out(r->particles[ids[k]], h.data() + (size_t)k * NFIELD);
+        if (!RB_HIP(r, hipMemcpyAsync(ddata, h.data(), databytes, hipMemcpyHostToDevice, d->stream))) return false;
+        hipLaunchKernelGGL(k_scatter_bodies, dim3((n + 127) / 128), dim3(128), 0, d->stream, d->soa.as<double>(),
+                           d->cap, dids, n, ddata);
+        if (!stream_sync(r)) return false;
+    }
+    return RB_HIP(r, hipGetLastError());
+}
+
+static int rand_r_step(unsigned int* seed) {
+    unsigned int next = *seed;
+    int result;
+    next *= 1103515245u;
+    next += 12345u;
+    result = (unsigned int)(next / 65536) % 2048;
+    next *= 1103515245u;
+    next += 12345u;
+    result <<= 10;
+    result ^= (unsigned int)(next / 65536) % 1024;
+    next *= 1103515245u;
+    next += 12345u;
+    result <<= 10;
+    result ^= (unsigned int)(next / 65536) % 1024;
+    *seed = next;
+    return result;
+}
+
+void collision_resolve_host(struct reb_simulation* r) {
+    // shuffle (collision.c:336-342)
+    for (size_t i = 0; i < r->N_collisions; i++) {
+        size_t nw = (size_t)rand_r_step(&r->rand_seed) % r->N_collisions;
+        struct reb_collision c1 = r->collisions[i];
+        r->collisions[i] = r->collisions[nw];
+        r->collisions[nw] = c1;
+    }
+    if (r->N_collisions == 0) return;
+    enum REB_COLLISION_RESOLVE_OUTCOME (*resolve)(struct reb_simulation* const, struct reb_collision) =
+        r->collision_resolve;
+    if (resolve == nullptr) resolve = reb_collision_resolve_halt;
+    // The built-in hard-sphere and halt resolvers touch only p1 and p2: move
+    // just those bodies.  Anything else sees the whole host state.
+    const bool local = (resolve == reb_collision_resolve_hardsphere || resolve == reb_collision_resolve_halt);
+    std::vector<int> ids;
+    if (local) {
+        std::unordered_map<size_t, int> seen;
+        seen.reserve(2 * r->N_collisions);
+        for (size_t k = 0; k < r->N_collisions; k++) {
+            for (size_t p : {r->collisions[k].p1, r->collisions[k].p2})
+                if (seen.emplace(p, 1).second) ids.push_back((int)p);
+        }
+        if (!move_bodies(r, ids, true)) return;
+    } else {
+        ensure_host(r);
+    }
+    // resolve in order (collision.c:345-404)
+    bool removed_any = false;
+    for (size_t i = 0; i < r->N_collisions; i++) {
+        struct reb_collision c = r->collisions[i];
+        if (c.p1 == SIZE_MAX || c.p2 == SIZE_MAX) continue;
+        enum REB_COLLISION_RESOLVE_OUTCOME outcome = resolve(r, c);
+        if (outcome & REB_COLLISION_RESOLVE_OUTCOME_REMOVE_P1) {
+            const int removedp1 = !reb_simulation_remove_particle(r, c.p1);
+            removed_any = true;
+            if (removedp1) {
+                if (c.p2 > c.p1 && c.p2 != SIZE_MAX) c.p2--;
+                for (size_t j = i + 1; j < r->N_collisions; j++) {
+                    struct reb_collision* cp = &(r->collisions[j]);
+                    if (cp->p1 == c.p1 || cp->p2 == c.p1) {
+                        cp->p1 = SIZE_MAX;
+                        cp->p2 = SIZE_MAX;
+                    }
+                    if (cp->p1 > c.p1 && cp->p1 != SIZE_MAX) cp->p1--;
+                    if (cp->p2 > c.p1 && cp->p2 != SIZE_MAX) cp->p2--;
+                }
+            }
+        }
+        if (outcome & REB_COLLISION_RESOLVE_OUTCOME_REMOVE_P2) {
+            const int removedp2 = !reb_simulation_remove_particle(r, c.p2);
+            removed_any = true;
+            if (removedp2) {
+                for (size_t j = i + 1; j < r->N_collisions; j++) {
+                    struct reb_collision* cp = &(r->collisions[j]);
+                    if (cp->p1 == c.p2 || cp->p2 == c.p2) {
+                        cp->p1 = SIZE_MAX;
+                        cp->p2 = SIZE_MAX;
+                    }
+                    if (cp->p1 > c.p2 && cp->p1 != SIZE_MAX) cp->p1--;
+                    if (cp->p2 > c.p2 && cp->p2 != SIZE_MAX) cp->p2--;
+                }
+            }
+        }
+    }
+    if (local && !removed_any) {
+        move_bodies(r, ids, false);
+    } else {
+        mark_host_newer(r);
+    }
+}
+
+}  // namespace rbamd
+
+using namespace rbamd;
+
+extern "C" {
+
+// collision.c:573-665
+REB_API enum REB_COLLISION_RESOLVE_OUTCOME reb_collision_resolve_hardsphere(struct reb_simulation* const r,
+                                                                            struct reb_collision c) {
+    struct reb_particle* const particles = r->particles;
+    struct reb_particle p1 = particles[c.p1];
+    struct reb_particle p2 = particles[c.p2];
+    struct reb_vec6d gb = c.gb;
+    double x21 = p1.x + gb.x - p2.x;
+    double y21 = p1.y + gb.y - p2.y;
+    double z21 = p1.z + gb.z - p2.z;
+    double rp = p1.r + p2.r;
+    double oldvyouter;
+    if (x21 > 0) {
+        oldvyouter = p1.vy;
+    } else {
+        oldvyouter = p2.vy;
+    }
+    if (rp * rp < x21 * x21 + y21 * y21 + z21 * z21) return REB_COLLISION_RESOLVE_OUTCOME_REMOVE_NONE;
+    double vx21 = p1.vx + gb.vx - p2.vx;
+    double vy21 = p1.vy + gb.vy - p2.vy;
+    double vz21 = p1.vz + gb.vz - p2.vz;
+    if (vx21 * x21 + vy21 * y21 + vz21 * z21 > 0) return REB_COLLISION_RESOLVE_OUTCOME_REMOVE_NONE;
+    // rotate the pair into the xy plane, then onto the x axis
+    double theta = atan2(z21, y21);
+    double stheta = sin(theta);
+    double ctheta = cos(theta);
+    double vy21n = ctheta * vy21 + stheta * vz21;
+    double y21n = ctheta * y21 + stheta * z21;
+    double phi = atan2(y21n, x21);
+    double cphi = cos(phi);
+    double sphi = sin(phi);
+    double vx21nn = cphi * vx21 + sphi * vy21n;
+    double eps = 1;
+    if (r->coefficient_of_restitution) eps = r->coefficient_of_restitution(r, vx21nn);
+    double dvx2 = -(1.0 + eps) * vx21nn;
+    double minr = (p1.r > p2.r) ? p2.r : p1.r;
+    double maxr = (p1.r < p2.r) ? p2.r : p1.r;
+    double mindv = minr * r->minimum_collision_velocity;
+    double _r = sqrt(x21 * x21 + y21 * y21 + z21 * z21);
+    mindv *= 1. - (_r - maxr) / minr;
+    if (mindv > maxr * r->minimum_collision_velocity) mindv = maxr * r->minimum_collision_velocity;
+    if (dvx2 < mindv) dvx2 = mindv;
+    double dvx2n = cphi * dvx2;
+    double dvy2n = sphi * dvx2;
+    double dvy2nn = ctheta * dvy2n;
+    double dvz2nn = stheta * dvy2n;
+    const double p2pf = p1.m / (p1.m + p2.m);
+    particles[c.p2].vx -= p2pf * dvx2n;
+    particles[c.p2].vy -= p2pf * dvy2nn;
+    particles[c.p2].vz -= p2pf * dvz2nn;
+    const double p1pf = p2.m / (p1.m + p2.m);
+    particles[c.p1].vx += p1pf * dvx2n;
+    particles[c.p1].vy += p1pf * dvy2nn;
+    particles[c.p1].vz += p1pf * dvz2nn;
+    if (x21 > 0) {
+        r->collisions_plog += -fabs(x21) * (oldvyouter - particles[c.p1].vy) * p1.m;
+        r->collisions_log_n++;
+    } else {
+        r->collisions_plog += -fabs(x21) * (oldvyouter - particles[c.p2].vy) * p2.m;
+        r->collisions_log_n++;
+    }
+    return REB_COLLISION_RESOLVE_OUTCOME_REMOVE_NONE;
+}
+
+// collision.c:667-672
+REB_API enum REB_COLLISION_RESOLVE_OUTCOME reb_collision_resolve_halt(struct reb_simulation* const r,
+                                                                      struct reb_collision c) {
+    (void)c;
+    r->status = REB_STATUS_COLLISION;
+    return REB_COLLISION_RESOLVE_OUTCOME_REMOVE_NONE;
+}
+
+// collision.c:674-737: merge into the lower index, conserving mass, momentum
+// and volume; the energy change is tracked when track_energy_offset is set.
+REB_API enum REB_COLLISION_RESOLVE_OUTCOME reb_collision_resolve_merge(struct reb_simulation* const r,
+                                                                       struct reb_collision c) {
+    int swap = 0;
+    size_t i = c.p1;
+    size_t j = c.p2;
+    if (j < i) {
+        swap = 1;
+        i = c.p2;
+        j = c.p1;
+    }
+    struct reb_particle* pi = &(r->particles[i]);
+    struct reb_particle* pj = &(r->particles[j]);
+    double invmass = 1.0 / (pi->m + pj->m);
+    double Ei = 0, Ef = 0;
+    if (r->track_energy_offset) {
+        double vix = pi->vx, viy = pi->vy, viz = pi->vz;
+        Ei += 0.5 * pi->m * (vix * vix + viy * viy + viz * viz);
+        double vjx = pj->vx, vjy = pj->vy, vjz = pj->vz;
+        Ei += 0.5 * pj->m * (vjx * vjx + vjy * vjy + vjz * vjz);
+        const size_t N_active = ((r->N_active == SIZE_MAX) ? r->N : r->N_active);
+        if (i < N_active || j < N_active) {
+            double x = pi->x - pj->x, y = pi->y - pj->y, z = pi->z - pj->z;
+            double _r = sqrt(x * x + y * y + z * z);
+            Ei += -r->G * pi->m * pj->m / _r;
+        }
+    }
+    pi->vx = (pi->vx * pi->m + pj->vx * pj->m) * invmass;
+    pi->vy = (pi->vy * pi->m + pj->vy * pj->m) * invmass;
+    pi->vz = (pi->vz * pi->m + pj->vz * pj->m) * invmass;
+    pi->x = (pi->x * pi->m + pj->x * pj->m) * invmass;
+    pi->y = (pi->y * pi->m + pj->y * pj->m) * invmass;
+    pi->z = (pi->z * pi->m + pj->z * pj->m) * invmass;
+    pi->m = pi->m + pj->m;
+    pi->r = cbrt(pi->r * pi->r * pi->r + pj->r * pj->r * pj->r);
+    if (r->track_energy_offset) {
+        double vx = pi->vx, vy = pi->vy, vz = pi->vz;
+        Ef += 0.5 * pi->m * (vx * vx + vy * vy + vz * vz);
+        r->energy_offset += Ei - Ef;
+    }
+    return swap ? REB_COLLISION_RESOLVE_OUTCOME_REMOVE_P1 : REB_COLLISION_RESOLVE_OUTCOME_REMOVE_P2;
+}
+
+}  // extern "C"

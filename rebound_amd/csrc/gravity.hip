@@ -1,0 +1,602 @@
+// Direct O(N^2) gravity on the GPU (reference: gravity.c:167-531) and the
+// energy diagnostic (tools.c:108-162).
+//
+// Three precisions (rebound_amd.h):
+//   EXACT  one lane per target i, sources j streamed through LDS tiles in
+//          ascending order, IEEE sqrt/div and no contraction (the library is
+//          built with -ffp-contract=off): every term and every partial sum is
+//          the reference's OpenMP loop (gravity.c:215-232, 308-343), so the
+//          result is bit-identical.
+//   FP64   r^-3 from v_rsq_f64 refined to full double accuracy, fused
+//          multiply-adds, and the source range split over blockIdx.y so that
+//          N=65536 fills every SIMD; the partial sums are added in a fixed
+//          order by a second kernel (deterministic).
+//   SPLIT  displacement and r^2 in FP64, r^-3 * m_j in FP32 (v_rsq_f32),
+//          accumulation in FP64; each LDS tile's FP64 partial is folded into
+//          the running sum with Kahan compensation.
+//
+// Data layout: structure of arrays (x, y, z, m as separate double arrays).
+// Sources are staged per 256-wide tile into LDS as double4 (x, y, z, m) and
+// read back as broadcasts (every lane of a wavefront reads the same entry).
+#include <cmath>
+
+#include "internal.h"
+
+namespace rbamd {
+
+static constexpr int kBlock = 256;     // targets per workgroup = sources per LDS tile
+
+struct GB3 { double x, y, z; };
+// Ghost-box offsets travel by value in the kernel arguments (kernarg segment,
+// read with scalar loads), so no copy or synchronisation is needed per call.
+static constexpr int kMaxGB = 125;       // N_ghost up to 2 in all three axes
+struct GBList {
+    int n;
+    GB3 g[kMaxGB];
+};
+
+__device__ __forceinline__ bool skip_pair(int i, int j, int ignore) {
+    if (ignore == 1 && ((j == 1 && i == 0) || (i == 1 && j == 0))) return true;
+    if (ignore == 2 && (j == 0 || i == 0)) return true;
+    return i == j;
+}
+
+// ---------------------------------------------------------------- EXACT, BASIC
+// gravity.c:217-231 (targets 0..N-1, sources 0..N_active-1 for every ghost
+// box), then :259-276 (active targets also feel test particles when
+// testparticle_type is set).
+__global__ __launch_bounds__(kBlock) void k_basic_exact(
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+    const double* __restrict__ m, double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
+    int N, int N_active, int tp_type, int ignore, double G, double soft2, GBList gl) {
+    __shared__ double4 tile[kBlock];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const bool live = i < N;
+    const double xi = live ? x[i] : 0., yi = live ? y[i] : 0., zi = live ? z[i] : 0.;
+    double a_x = 0., a_y = 0., a_z = 0.;
+    for (int g = 0; g < gl.n; g++) {
+        const GB3 gb = gl.g[g];
+        for (int pass = 0; pass < 2; pass++) {
+            int j0 = pass == 0 ? 0 : N_active;
+            int j1 = pass == 0 ? N_active : N;
+            if (pass == 1 && !tp_type) break;
+            for (int t = j0; t < j1; t += kBlock) {
+                __syncthreads();
+                const int jj = t + (int)threadIdx.x;
+                if (jj < j1) tile[threadIdx.x] = make_double4(x[jj], y[jj], z[jj], m[jj]);
+                __syncthreads();
+                const int n = min(kBlock, j1 - t);
+                if (!live || (pass == 1 && i >= N_active)) continue;
+                for (int k = 0; k < n; k++) {
+                    const int j = t + k;
+                    if (pass == 0) {
+                        if (skip_pair(i, j, ignore)) continue;
+                    } else {
+                        if (ignore == 1 && j == 1 && i == 0) continue;
+                        if (ignore == 2 && (j == 0 || i == 0)) continue;
+                    }
+                    const double4 s = tile[k];
+                    const double dx = (gb.x + xi) - s.x;
+                    const double dy = (gb.y + yi) - s.y;
+                    const double dz = (gb.z + zi) - s.z;
+                    const double _r = sqrt(dx * dx + dy * dy + dz * dz + soft2);
+                    const double prefact = -G / (_r * _r * _r) * s.w;
+                    a_x += prefact * dx;
+                    a_y += prefact * dy;
+                    a_z += prefact * dz;
+                }
+            }
+        }
+    }
+    if (live) {
+        ax[i] = a_x;
+        ay[i] = a_y;
+        az[i] = a_z;
+    }
+}
+
+// ---------------------------------------------------------------- EXACT, COMPENSATED
+// gravity.c:308-343 (active targets), :346-378 (test-particle targets), :379-410
+// (active targets feel test particles).  No ghost boxes in the reference.
+__device__ __forceinline__ void kahan_add(double term, double& a, double& c) {
+    const double yv = term - c;
+    const double tv = a + yv;
+    c = (tv - a) - yv;
+    a = tv;
+}
+
+__global__ __launch_bounds__(kBlock) void k_comp_exact(
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+    const double* __restrict__ m, double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
+    int N, int N_active, int tp_type, int ignore, double G, double soft2) {
+    __shared__ double4 tile[kBlock];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const bool live = i < N;
+    const double xi = live ? x[i] : 0., yi = live ? y[i] : 0., zi = live ? z[i] : 0.;
+    double a_x = 0., a_y = 0., a_z = 0., c_x = 0., c_y = 0., c_z = 0.;
+    for (int pass = 0; pass < 2; pass++) {
+        int j0 = pass == 0 ? 0 : N_active;
+        int j1 = pass == 0 ? N_active : N;
+        if (pass == 1 && !tp_type) break;
+        for (int t = j0; t < j1; t += kBlock) {
+            __syncthreads();
+            const int jj = t + (int)threadIdx.x;
+            if (jj < j1) tile[threadIdx.x] = make_double4(x[jj], y[jj], z[jj], m[jj]);
+            __syncthreads();
+            const int n = min(kBlock, j1 - t);
+            if (!live || (pass == 1 && i >= N_active)) continue;
+            for (int k = 0; k < n; k++) {
+                const int j = t + k;
+                if (skip_pair(i, j, ignore)) continue;
+                const double4 s = tile[k];
+                const double dx = xi - s.x;
+                const double dy = yi - s.y;
+                const double dz = zi - s.z;
+                const double r2 = dx * dx + dy * dy + dz * dz + soft2;
+                const double rr = sqrt(r2);
+                const double prefact = G / (r2 * rr);
+                const double prefactj = -prefact * s.w;
+                kahan_add(prefactj * dx, a_x, c_x);
+                kahan_add(prefactj * dy, a_y, c_y);
+                kahan_add(prefactj * dz, a_z, c_z);
+            }
+        }
+    }
+    if (live) {
+        ax[i] = a_x;
+        ay[i] = a_y;
+        az[i] = a_z;
+    }
+}
+
+// ---------------------------------------------------------------- FP64 / SPLIT / COMP-FP64
+// Pair kernel of the fast modes.  Accumulates S_i = sum_j m_j r_ij^-3 d_ij over
+// one chunk of sources (blockIdx.y) and writes it (and for the compensated
+// variant its Kahan carry) to the partial buffer; k_reduce applies -G.
+enum FastMode { FM_FP64 = 0, FM_SPLIT = 1, FM_COMP = 2 };
+
+// r^-3 in FP64.  v_rsq_f64 is refined through e = 1 - r2*y^2:
+//   r^-3 = y^3 (1 - e)^(-3/2) = y^3 (1 + 3/2 e + 15/8 e^2 + 35/16 e^3 + ...)
+__device__ __forceinline__ double rinv3_fp64(double r2) {
+    const double yv = __builtin_amdgcn_rsq(r2);
+    const double h = yv * yv;
+    const double e = fma(-r2, h, 1.0);
+    const double y3 = h * yv;
+    const double p = fma(e, fma(e, 2.1875, 1.875), 1.5);
+    return fma(y3, e * p, y3);
+}
+
+template <int MODE, bool CHECK>
+__device__ __forceinline__ void pair_tile(const double4* __restrict__ tile, const float* __restrict__ mf, int n,
+                                          int jbase, int i, int ignore, double gx, double gy, double gz,
+                                          double soft2, double& sx, double& sy, double& sz, double& cx, double& cy,
+                                          double& cz) {
+#pragma unroll 4
+    for (int k = 0; k < n; k++) {
+        const double4 s = tile[k];
+        const double dx = gx - s.x;
+        const double dy = gy - s.y;
+        const double dz = gz - s.z;
+        const double r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, soft2)));
+        double w;
+        if (MODE == FM_SPLIT) {
+            const float rf = (float)r2;
+            const float yf = __builtin_amdgcn_rsqf(rf);
+            w = (double)(yf * yf * yf * mf[k]);
+        } else {
+            w = s.w * rinv3_fp64(r2);
+        }
+        if (CHECK) {
+            const int j = jbase + k;
+            if (skip_pair(i, j, ignore)) w = 0.;
+        }
+        if (MODE == FM_COMP) {
+            kahan_add(w * dx, sx, cx);
+            kahan_add(w * dy, sy, cy);
+            kahan_add(w * dz, sz, cz);
+        } else {
+            sx = fma(w, dx, sx);
+            sy = fma(w, dy, sy);
+            sz = fma(w, dz, sz);
+        }
+    }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_direct_fast(
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+    const double* __restrict__ m, int n_recv, int j_begin, int j_end, int chunk, int ignore, double soft2,
+    GBList gl, double* __restrict__ partial, int pstride) {
+    __shared__ double4 tile[kBlock];
+    __shared__ float mtile[kBlock];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const bool live = i < n_recv;
+    const double xi = live ? x[i] : 0., yi = live ? y[i] : 0., zi = live ? z[i] : 0.;
+    const int j0 = j_begin + (int)blockIdx.y * chunk;
+    const int j1 = min(j_end, j0 + chunk);
+    const int blk_lo = blockIdx.x * kBlock;
+    const int blk_hi = blk_lo + kBlock;
+    double sx = 0., sy = 0., sz = 0.;      // running sums
+    double cx = 0., cy = 0., cz = 0.;      // Kahan carries (COMP per pair, SPLIT per tile)
+    for (int g = 0; g < gl.n; g++) {
+        const GB3 gb = gl.g[g];
+        const double gx = gb.x + xi, gy = gb.y + yi, gz = gb.z + zi;
+        for (int t = j0; t < j1; t += kBlock) {
+            __syncthreads();
+            const int jj = t + (int)threadIdx.x;
+            if (jj < j1) {
+                const double mj = m[jj];
+                tile[threadIdx.x] = make_double4(x[jj], y[jj], z[jj], mj);
+                mtile[threadIdx.x] = (float)mj;
+            }
+            __syncthreads();
+            const int n = min(kBlock, j1 - t);
+            // Only tiles holding a target of this block, or sources 0/1 when
+            // terms are ignored, need the per-pair skip test.
+            const bool check = (t < blk_hi && t + n > blk_lo) || (ignore != 0 && t < 2);
+            if (MODE == FM_SPLIT) {
+                double tx = 0., ty = 0., tz = 0., d0 = 0., d1 = 0., d2 = 0.;
+                if (check)
+                    pair_tile<MODE, true>(tile, mtile, n, t, i, ignore, gx, gy, gz, soft2, tx, ty, tz, d0, d1, d2);
+                else
+                    pair_tile<MODE, false>(tile, mtile, n, t, i, ignore, gx, gy, gz, soft2, tx, ty, tz, d0, d1, d2);
+                kahan_add(tx, sx, cx);
+                kahan_add(ty, sy, cy);
+                kahan_add(tz, sz, cz);
+            } else {
+                if (check)
+                    pair_tile<MODE, true>(tile, mtile, n, t, i, ignore, gx, gy, gz, soft2, sx, sy, sz, cx, cy, cz);
+                else
+                    pair_tile<MODE, false>(tile, mtile, n, t, i, ignore, gx, gy, gz, soft2, sx, sy, sz, cx, cy, cz);
+            }
+        }
+    }
+    if (live) {
+        double* p = partial + (size_t)blockIdx.y * 6 * pstride;
+        p[0 * pstride + i] = sx;
+        p[1 * pstride + i] = sy;
+        p[2 * pstride + i] = sz;
+        p[3 * pstride + i] = cx;
+        p[4 * pstride + i] = cy;
+        p[5 * pstride + i] = cz;
+    }
+}
+
+// Sum the chunk partials in chunk order (Kahan), apply -G and write (or add to)
+// the acceleration.  ignore==2 zeroes particle 0 (it feels nothing).
+__global__ void k_reduce(const double* __restrict__ partial, int pstride, int nchunk, int n_recv, double negG,
+                         int accumulate, int ignore, double* __restrict__ ax, double* __restrict__ ay,
+                         double* __restrict__ az) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_recv) return;
+    double out[3];
+    for (int c = 0; c < 3; c++) {
+        double s = 0., k = 0.;
+        for (int q = 0; q < nchunk; q++) {
+            const double* p = partial + (size_t)q * 6 * pstride;
+            kahan_add(p[c * pstride + i], s, k);
+            kahan_add(-p[(c + 3) * pstride + i], s, k);
+        }
+        out[c] = negG * s;
+    }
+    if (ignore == 2 && i == 0) out[0] = out[1] = out[2] = 0.;
+    if (accumulate) {
+        ax[i] += out[0];
+        ay[i] += out[1];
+        az[i] += out[2];
+    } else {
+        ax[i] = out[0];
+        ay[i] = out[1];
+        az[i] = out[2];
+    }
+}
+
+__global__ void k_zero3(double* __restrict__ a, double* __restrict__ b, double* __restrict__ c, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = b[i] = c[i] = 0.;
+}
+
+// Number of source chunks for the fast modes: enough workgroups to put about
+// eight wavefronts on each SIMD of the 256-CU part, chunks no shorter than
+// four LDS tiles.
+static int pick_chunks(int n_recv, int n_src, int ngb) {
+    const int blocks_x = (n_recv + kBlock - 1) / kBlock;
+    const int target_blocks = 2048;
+    int s = (target_blocks + blocks_x - 1) / blocks_x;
+    int max_s = (n_src * ngb) / (4 * kBlock);
+    if (max_s < 1) max_s = 1;
+    if (s > max_s) s = max_s;
+    if (s > 64) s = 64;
+    if (s < 1) s = 1;
+    return s;
+}
+
+static size_t workspace_bytes(int n, int nchunk) { return sizeof(double) * 6 * (size_t)n * (size_t)nchunk; }
+
+// Launch one fast-mode pass: targets [0,n_recv) feel sources [j_begin,j_end).
+static int launch_fast(int mode, const double* x, const double* y, const double* z, const double* m, int n_recv,
+                       int j_begin, int j_end, int ignore, double G, double soft2, const GBList& gl,
+                       double* partial, int nchunk, int accumulate, double* ax, double* ay, double* az,
+                       hipStream_t st, hipEvent_t* ev) {
+    const int nsrc = j_end - j_begin;
+    int chunk = (nsrc + nchunk - 1) / nchunk;
+    chunk = ((chunk + kBlock - 1) / kBlock) * kBlock;
+    const int used = nsrc > 0 ? (nsrc + chunk - 1) / chunk : 1;
+    dim3 grid((n_recv + kBlock - 1) / kBlock, used);
+    if (ev) (void)hipEventRecord(ev[0], st);
+    switch (mode) {
+        case FM_FP64:
+            hipLaunchKernelGGL(k_direct_fast<FM_FP64>, grid, dim3(kBlock), 0, st, x, y, z, m, n_recv, j_begin, j_end,
+                               chunk, ignore, soft2, gl, partial, n_recv);
+            break;
+        case FM_SPLIT:
+            hipLaunchKernelGGL(k_direct_fast<FM_SPLIT>, grid, dim3(kBlock), 0, st, x, y, z, m, n_recv, j_begin,
+                               j_end, chunk, ignore, soft2, gl, partial, n_recv);
+            break;
+        default:
+            hipLaunchKernelGGL(k_direct_fast<FM_COMP>, grid, dim3(kBlock), 0, st, x, y, z, m, n_recv, j_begin, j_end,
+                               chunk, ignore, soft2, gl, partial, n_recv);
+            break;
+    }
+    if (ev) (void)hipEventRecord(ev[1], st);
+    hipLaunchKernelGGL(k_reduce, dim3((n_recv + 255) / 256), dim3(256), 0, st, partial, n_recv, used, n_recv, -G,
+                       accumulate, ignore, ax, ay, az);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+// Shared dispatch for the simulation path and the raw entry point.
+static int direct_dispatch(const double* x, const double* y, const double* z, const double* m, double* ax,
+                           double* ay, double* az, int N, int N_active, int tp_type, int ignore, double G,
+                           double soft2, int precision, int compensated, const GBList& gl,
+                           double* workspace, hipStream_t st, hipEvent_t* ev = nullptr) {
+    if (N == 0) return 0;
+    if (precision == REB_AMD_PRECISION_EXACT) {
+        dim3 grid((N + kBlock - 1) / kBlock);
+        if (ev) (void)hipEventRecord(ev[0], st);
+        if (compensated)
+            hipLaunchKernelGGL(k_comp_exact, grid, dim3(kBlock), 0, st, x, y, z, m, ax, ay, az, N, N_active, tp_type,
+                               ignore, G, soft2);
+        else
+            hipLaunchKernelGGL(k_basic_exact, grid, dim3(kBlock), 0, st, x, y, z, m, ax, ay, az, N, N_active,
+                               tp_type, ignore, G, soft2, gl);
+        if (ev) (void)hipEventRecord(ev[1], st);
+        return hipGetLastError() == hipSuccess ? 0 : 1;
+    }
+    const int mode = compensated ? FM_COMP : (precision == REB_AMD_PRECISION_SPLIT ? FM_SPLIT : FM_FP64);
+    const int nchunk = pick_chunks(N, N_active, gl.n);
+    int rc = 0;
+    if (N_active > 0) {
+        rc |= launch_fast(mode, x, y, z, m, N, 0, N_active, ignore, G, soft2, gl, workspace, nchunk, 0, ax, ay, az,
+                          st, ev);
+    } else {
+        hipLaunchKernelGGL(k_zero3, dim3((N + 255) / 256), dim3(256), 0, st, ax, ay, az, N);
+    }
+    if (tp_type && N_active < N) {
+        // Active targets also feel the test particles (gravity.c:259-276).
+        const int nchunk2 = pick_chunks(N_active, N - N_active, gl.n);
+        rc |= launch_fast(mode, x, y, z, m, N_active, N_active, N, ignore, G, soft2, gl, workspace,
+                          nchunk2 < nchunk ? nchunk2 : nchunk, 1, ax, ay, az, st, nullptr);
+    }
+    return rc;
+}
+
+// ---------------------------------------------------------------- simulation path
+bool gravity_zero(struct reb_simulation* r) {
+    DeviceState* d = dev_of(r);
+    const int N = (int)r->N;
+    if (N == 0) return true;
+    hipLaunchKernelGGL(k_zero3, dim3((N + 255) / 256), dim3(256), 0, d->stream, d->field(FAX), d->field(FAY),
+                       d->field(FAZ), N);
+    return RB_HIP(r, hipGetLastError());
+}
+
+bool gravity_direct(struct reb_simulation* r, bool compensated) {
+    DeviceState* d = dev_of(r);
+    const int N = (int)r->N;
+    if (N == 0) return true;
+    const int N_active = (r->N_active == SIZE_MAX) ? N : (int)r->N_active;
+    const double soft2 = r->softening * r->softening;
+    // Ghost boxes (BASIC only; gravity.c:184-186).  COMPENSATED has none.
+    std::vector<struct reb_vec6d> gbv;
+    if (compensated)
+        gbv.push_back(reb_vec6d{0, 0, 0, 0, 0, 0});
+    else
+        gbv = ghostboxes(r, r->N_ghost_x, r->N_ghost_y, r->N_ghost_z);
+    const int ngb = (int)gbv.size();
+    if (ngb > kMaxGB) {
+        reb_simulation_error(r, "Too many ghost boxes for direct gravity (at most 125).");
+        return false;
+    }
+    GBList gl;
+    gl.n = ngb;
+    for (int g = 0; g < ngb; g++) gl.g[g] = GB3{gbv[g].x, gbv[g].y, gbv[g].z};
+    const int nchunk = pick_chunks(N, N_active, ngb);
+    if (!d->partial.ensure(workspace_bytes(N, nchunk) + 256)) {
+        reb_simulation_error(r, "Out of device memory for the gravity workspace.");
+        return false;
+    }
+    hipEvent_t* ev = nullptr;
+    if (d->profiling) {
+        if (d->kev_used + 2 > d->kev.size()) {
+            for (int k = 0; k < 64; k++) {
+                hipEvent_t e;
+                if (!RB_HIP(r, hipEventCreate(&e))) return false;
+                d->kev.push_back(e);
+            }
+        }
+        ev = &d->kev[d->kev_used];
+        d->kev_used += 2;
+    }
+    int rc = direct_dispatch(d->field(FX), d->field(FY), d->field(FZ), d->field(FM), d->field(FAX), d->field(FAY),
+                             d->field(FAZ), N, N_active, r->testparticle_type, (int)r->gravity_ignore_terms, r->G,
+                             soft2, d->precision, compensated ? 1 : 0, gl, d->partial.as<double>(), d->stream, ev);
+    if (rc) return RB_HIP(r, hipGetLastError());
+    return true;
+}
+
+// ---------------------------------------------------------------- energy
+// tools.c:108-162.  EXACT: the kinetic and potential sums are accumulated in
+// the reference's sequential order by one wavefront (lanes evaluate 64 terms,
+// lane 0 folds them in order), so the value is bit-identical.  Otherwise each
+// target's row sum over j>i is compensated and rows are folded with Kahan.
+__global__ __launch_bounds__(64) void k_energy_exact(const double* __restrict__ x, const double* __restrict__ y,
+                                                     const double* __restrict__ z, const double* __restrict__ vx,
+                                                     const double* __restrict__ vy, const double* __restrict__ vz,
+                                                     const double* __restrict__ m, int N_active, int N_interact,
+                                                     double G, double* __restrict__ out) {
+    __shared__ double terms[64];
+    const int lane = threadIdx.x;
+    double e_kin = 0., e_pot = 0.;
+    for (int b = 0; b < N_interact; b += 64) {
+        const int i = b + lane;
+        if (i < N_interact) {
+            const double vv = vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i];
+            terms[lane] = 0.5 * m[i] * vv;
+        }
+        __syncthreads();
+        if (lane == 0) {
+            const int n = min(64, N_interact - b);
+            for (int k = 0; k < n; k++) e_kin += terms[k];
+        }
+        __syncthreads();
+    }
+    for (int i = 0; i < N_active; i++) {
+        const double xi = x[i], yi = y[i], zi = z[i], mi = m[i];
+        for (int b = i + 1; b < N_interact; b += 64) {
+            const int j = b + lane;
+            if (j < N_interact) {
+                const double dx = xi - x[j];
+                const double dy = yi - y[j];
+                const double dz = zi - z[j];
+                terms[lane] = G * m[j] * mi / sqrt(dx * dx + dy * dy + dz * dz);
+            }
+            __syncthreads();
+            if (lane == 0) {
+                const int n = min(64, N_interact - b);
+                for (int k = 0; k < n; k++) e_pot -= terms[k];
+            }
+            __syncthreads();
+        }
+    }
+    if (lane == 0) {
+        out[0] = e_kin;
+        out[1] = e_pot;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_energy_rows(const double* __restrict__ x, const double* __restrict__ y,
+                                                     const double* __restrict__ z, const double* __restrict__ vx,
+                                                     const double* __restrict__ vy, const double* __restrict__ vz,
+                                                     const double* __restrict__ m, int N_active, int N_interact,
+                                                     double G, double* __restrict__ rows) {
+    __shared__ double4 tile[256];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const bool live = i < N_interact;
+    const double xi = live ? x[i] : 0., yi = live ? y[i] : 0., zi = live ? z[i] : 0., mi = live ? m[i] : 0.;
+    double s = 0., c = 0.;
+    const int jstart = blockIdx.x * 256;
+    for (int t = jstart; t < N_interact; t += 256) {
+        __syncthreads();
+        const int jj = t + threadIdx.x;
+        if (jj < N_interact) tile[threadIdx.x] = make_double4(x[jj], y[jj], z[jj], m[jj]);
+        __syncthreads();
+        const int n = min(256, N_interact - t);
+        if (live && i < N_active) {
+            for (int k = 0; k < n; k++) {
+                const int j = t + k;
+                if (j <= i) continue;
+                const double4 q = tile[k];
+                const double dx = xi - q.x, dy = yi - q.y, dz = zi - q.z;
+                kahan_add(-(G * q.w * mi / sqrt(dx * dx + dy * dy + dz * dz)), s, c);
+            }
+        }
+    }
+    if (live) {
+        rows[i] = s - c;
+        rows[N_interact + i] = 0.5 * mi * (vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i]);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_energy_fold(const double* __restrict__ rows, int n, double* __restrict__ out) {
+    // One workgroup: each lane folds a strided subset with Kahan, then lane 0
+    // folds the 256 partials in lane order.
+    __shared__ double ps[256], pk[256];
+    for (int which = 0; which < 2; which++) {
+        double s = 0., c = 0.;
+        for (int i = threadIdx.x; i < n; i += 256) kahan_add(rows[which * n + i], s, c);
+        ps[threadIdx.x] = s;
+        pk[threadIdx.x] = c;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double S = 0., C = 0.;
+            for (int k = 0; k < 256; k++) {
+                kahan_add(ps[k], S, C);
+                kahan_add(-pk[k], S, C);
+            }
+            out[which == 0 ? 1 : 0] = S;
+        }
+        __syncthreads();
+    }
+}
+
+double energy_device(struct reb_simulation* r) {
+    DeviceState* d = dev_of(r);
+    const size_t N = r->N;
+    const size_t N_active = (r->N_active == SIZE_MAX) ? N : r->N_active;
+    const size_t N_interact = (r->testparticle_type == 0) ? N_active : N;
+    if (N == 0) return r->energy_offset;
+    if (!d->scratch.ensure(sizeof(double) * (2 * N + 16))) {
+        reb_simulation_error(r, "Out of device memory for the energy reduction.");
+        return nan("");
+    }
+    double* out = d->scratch.as<double>();
+    if (d->precision == REB_AMD_PRECISION_EXACT && N <= 8192) {
+        hipLaunchKernelGGL(k_energy_exact, dim3(1), dim3(64), 0, d->stream, d->field(FX), d->field(FY), d->field(FZ),
+                           d->field(FVX), d->field(FVY), d->field(FVZ), d->field(FM), (int)N_active, (int)N_interact,
+                           r->G, out);
+    } else {
+        double* rows = out + 16;
+        hipLaunchKernelGGL(k_energy_rows, dim3((N_interact + 255) / 256), dim3(256), 0, d->stream, d->field(FX),
+                           d->field(FY), d->field(FZ), d->field(FVX), d->field(FVY), d->field(FVZ), d->field(FM),
+                           (int)N_active, (int)N_interact, r->G, rows);
+        hipLaunchKernelGGL(k_energy_fold, dim3(1), dim3(256), 0, d->stream, rows, (int)N_interact, out);
+    }
+    double h[2] = {0, 0};
+    if (!RB_HIP(r, hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, d->stream))) return nan("");
+    if (!stream_sync(r)) return nan("");
+    return h[0] + h[1] + r->energy_offset;
+}
+
+}  // namespace rbamd
+
+using namespace rbamd;
+
+extern "C" {
+
+REB_API size_t reb_amd_gravity_workspace_bytes(size_t N, int precision) {
+    if (precision == REB_AMD_PRECISION_EXACT) return 256;
+    return workspace_bytes((int)N, pick_chunks((int)N, (int)N, 1)) + 256;
+}
+
+REB_API int reb_amd_gravity_direct(const double* x, const double* y, const double* z, const double* m, double* ax,
+                                   double* ay, double* az, size_t N, size_t N_active, int testparticle_type,
+                                   int ignore_terms, double G, double softening, int precision, int compensated,
+                                   void* workspace, void* stream) {
+    if (N > (size_t)INT32_MAX / 8) {
+        set_last_error("reb_amd_gravity_direct: N too large");
+        return 2;
+    }
+    if (N_active == SIZE_MAX || N_active > N) N_active = N;
+    if (precision != REB_AMD_PRECISION_EXACT && workspace == nullptr) {
+        set_last_error("reb_amd_gravity_direct: workspace required for this precision");
+        return 3;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    GBList gl;
+    gl.n = 1;
+    gl.g[0] = GB3{0., 0., 0.};
+    return direct_dispatch(x, y, z, m, ax, ay, az, (int)N, (int)N_active, testparticle_type, ignore_terms, G,
+                           softening * softening, precision, compensated, gl, static_cast<double*>(workspace), st);
+}
+
+}  // extern "C"

@@ -1,0 +1,155 @@
+// Internal declarations shared by the host runtime and the HIP kernels of the
+// MI355X force-and-step engine.  Nothing here is part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "rebound.h"
+#include "rebound_amd.h"
+
+// The enums declared inside struct reb_simulation / reb_integrator_whfast_state
+// are file-scope in C but member-scope in C++; bring their names into scope.
+#define RB_SIM_ENUM(name) constexpr auto name = reb_simulation::name;
+RB_SIM_ENUM(REB_GRAVITY_IGNORE_TERMS_NONE)
+RB_SIM_ENUM(REB_GRAVITY_IGNORE_TERMS_BETWEEN_0_AND_1)
+RB_SIM_ENUM(REB_GRAVITY_IGNORE_TERMS_INVOLVING_0)
+RB_SIM_ENUM(REB_COLLISION_NONE)
+RB_SIM_ENUM(REB_COLLISION_DIRECT)
+RB_SIM_ENUM(REB_COLLISION_TREE)
+RB_SIM_ENUM(REB_COLLISION_LINE)
+RB_SIM_ENUM(REB_COLLISION_LINETREE)
+RB_SIM_ENUM(REB_BOUNDARY_NONE)
+RB_SIM_ENUM(REB_BOUNDARY_OPEN)
+RB_SIM_ENUM(REB_BOUNDARY_PERIODIC)
+RB_SIM_ENUM(REB_BOUNDARY_SHEAR)
+RB_SIM_ENUM(REB_GRAVITY_NONE)
+RB_SIM_ENUM(REB_GRAVITY_BASIC)
+RB_SIM_ENUM(REB_GRAVITY_COMPENSATED)
+RB_SIM_ENUM(REB_GRAVITY_TREE)
+RB_SIM_ENUM(REB_GRAVITY_JACOBI)
+RB_SIM_ENUM(REB_GRAVITY_CUSTOM)
+#undef RB_SIM_ENUM
+#define RB_WH_ENUM(name) constexpr auto name = reb_integrator_whfast_state::name;
+RB_WH_ENUM(REB_INTEGRATOR_WHFAST_KERNEL_DEFAULT)
+RB_WH_ENUM(REB_INTEGRATOR_WHFAST_COORDINATES_JACOBI)
+RB_WH_ENUM(REB_INTEGRATOR_WHFAST_COORDINATES_DEMOCRATICHELIOCENTRIC)
+#undef RB_WH_ENUM
+
+namespace rbamd {
+
+// ------------------------------------------------------------------ errors
+// Record a HIP failure on the simulation (turns into REB_STATUS_GENERIC_ERROR
+// at the next exit check, as reb_simulation_error does in the reference).
+bool hip_check(struct reb_simulation* r, hipError_t e, const char* what);
+#define RB_HIP(r, call) ::rbamd::hip_check((r), (call), #call)
+
+void set_last_error(const std::string& msg);
+
+// ------------------------------------------------------------------ device buffers
+// A growable device allocation.  Freed explicitly (never in a destructor that
+// could run after the HIP runtime has shut down).
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    bool ensure(size_t need);            // grow (contents not preserved)
+    void release();
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Field order of the structure-of-arrays particle block on the device.
+enum Field { FX = 0, FY, FZ, FVX, FVY, FVZ, FAX, FAY, FAZ, FM, FR, NFIELD };
+
+// Where the authoritative copy of the particle state lives.
+enum Loc { LOC_HOST = 0, LOC_DEVICE = 1, LOC_BOTH = 2 };
+
+struct TreeBuffers;          // tree.hip
+struct WHFastDevice;         // whfast.hip
+
+// Device-side state of one simulation.  Allocated next to the public struct
+// by reb_simulation_create; never visible through the ABI.
+struct DeviceState {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    size_t cap = 0;                 // particle capacity of `soa`
+    DevBuf soa;                     // NFIELD * cap doubles
+    DevBuf partial;                 // j-split partial sums for direct gravity
+    DevBuf scratch;                 // small reductions (energy, flags)
+    DevBuf aos;                     // AoS staging for uploads/downloads
+    double* host_stage = nullptr;   // pinned staging, NFIELD * host_cap doubles
+    size_t host_cap = 0;
+    int loc = LOC_HOST;
+    int precision = REB_AMD_PRECISION_FP64;
+    int gravity_tile = 0;           // autotuned j-split (0 = pick)
+    TreeBuffers* tree = nullptr;
+    WHFastDevice* whfast = nullptr;
+    // per-category device time of the last steps()/integrate() call (ms)
+    hipEvent_t ev[8] = {nullptr};
+    double prof_ms[4] = {0, 0, 0, 0};
+    // kernel timing: when `profiling` is set, every launch of the dominant
+    // pair kernel of direct gravity is bracketed by an event pair from `kev`
+    bool profiling = false;
+    std::vector<hipEvent_t> kev;
+    size_t kev_used = 0;
+    // device residency: when set, steps()/integrate() neither upload nor
+    // download r->particles; reb_amd_sync_host / reb_amd_mark_host_modified
+    // move the state explicitly
+    bool resident = false;
+    double* field(int f) const { return soa.as<double>() + (size_t)f * cap; }
+};
+
+// The allocation returned by reb_simulation_create: public struct first.
+struct SimBlock {
+    struct reb_simulation pub;
+    DeviceState dev;
+};
+inline DeviceState* dev_of(struct reb_simulation* r) { return &reinterpret_cast<SimBlock*>(r)->dev; }
+inline const DeviceState* dev_of(const struct reb_simulation* r) { return &reinterpret_cast<const SimBlock*>(r)->dev; }
+
+// ------------------------------------------------------------------ residency
+bool device_ready(struct reb_simulation* r);            // pick device, create stream
+bool ensure_device(struct reb_simulation* r);            // host -> device if host is newer
+bool ensure_host(struct reb_simulation* r);              // device -> host if device is newer
+void mark_host_newer(struct reb_simulation* r);
+void mark_device_newer(struct reb_simulation* r);
+bool stream_sync(struct reb_simulation* r);
+void device_release(struct reb_simulation* r);
+
+// ------------------------------------------------------------------ ghost boxes
+// All ghost-box offsets of the (2Nx+1)(2Ny+1)(2Nz+1) grid in the reference's
+// loop order (x outer, z inner).  gravity.c:84-86, boundary.c:145-201.
+std::vector<struct reb_vec6d> ghostboxes(struct reb_simulation* r, int nx, int ny, int nz);
+
+// ------------------------------------------------------------------ kernels (launchers)
+// Direct O(N^2) gravity (gravity.c:167-531) on the simulation's device state.
+bool gravity_direct(struct reb_simulation* r, bool compensated);
+bool gravity_zero(struct reb_simulation* r);
+bool gravity_tree(struct reb_simulation* r);                 // tree.hip
+bool collision_search_device(struct reb_simulation* r);      // tree.hip / collision
+double energy_device(struct reb_simulation* r);              // tools.c:108-162
+
+// Streams (integrator_leapfrog.c:72-93, integrator_sei.c:126-174, boundary.c:35-141)
+bool launch_drift(struct reb_simulation* r, double dt);
+bool launch_kick(struct reb_simulation* r, double dt);
+bool launch_sei_h012(struct reb_simulation* r, double dt, double sindt, double tandt,
+                     double sindtz, double tandtz, int with_kick);
+bool boundary_device(struct reb_simulation* r);              // returns false on error
+
+// WHFast (whfast.hip)
+void whfast_step_device(struct reb_simulation* r, struct reb_integrator_whfast_state* s);
+void whfast_synchronize_device(struct reb_simulation* r, struct reb_integrator_whfast_state* s);
+void whfast_release(struct reb_simulation* r);
+void tree_release(struct reb_simulation* r);
+
+// Host-side driver pieces shared between files
+void collision_resolve_host(struct reb_simulation* r);       // collision.c:336-404
+void run_host_callback(struct reb_simulation* r, void (*cb)(struct reb_simulation* const));
+
+// Monotonic clock (seconds)
+double now_seconds();
+
+}  // namespace rbamd

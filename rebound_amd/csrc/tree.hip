@@ -1,0 +1,981 @@
+// Barnes-Hut octree on the GPU: construction from sorted keys, monopole
+// moments, tree gravity, and the collision search that uses the same tree.
+//
+// Reference: tree.c:64-328 (insertion octree, one body per leaf, moments,
+// recursive walk), gravity.c:47-106 (tree gravity driver), collision.c:197-269
+// and :422-503 (tree collision search), simulation.c:718-799 (two largest radii).
+//
+// Construction.  The reference inserts bodies one by one; a cell is split as
+// soon as it holds two bodies, so the finished tree is the canonical point-
+// region octree: a cell is internal iff it holds >= 2 bodies, children are
+// visited in octant order 0..7.  Here each body's octant digits are derived
+// with the reference's own arithmetic (octant bit = x < cx, child centre =
+// parent centre +/- (w/2)/2, tree.c:98-103 and :136-142), packed into a key
+// (root box index first), and sorted.  In sorted order the pre-order of the
+// octree is "by first body, then by level": a body s starts the internal cells
+// at levels lcp(s-1,s)+1 .. lcp(s,s+1) and its own leaf at level
+// max(lcp(s-1,s), lcp(s,s+1)) + 1, where lcp is the number of common leading
+// digits of neighbouring keys.  A scan of those counts places every cell; each
+// internal cell finds the end of its body range by a binary search on keys,
+// which gives its pre-order successor (`next`), the skip pointer of the
+// stackless walks.  Moments are accumulated bottom-up, level by level, in
+// child (octant) order with the reference's expressions, so cell masses and
+// centres of mass are bit-identical to tree.c:147-229.
+//
+// Walks.  One wavefront walks one pre-order sequence for 64 bodies (in key
+// order, so neighbours share most of the path): at cell c a lane that accepts
+// the cell (or has accepted an ancestor) skips c's subtree; the wave descends
+// into c if any lane opens it.  Each lane therefore sees its own accepted
+// cells in the reference's depth-first order and sums them in that order.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "internal.h"
+
+namespace rbamd {
+
+static constexpr int kMaxGBT = 125;
+struct GBT {
+    int n;
+    double g[kMaxGBT][6];
+};
+
+struct TreeBuffers {
+    DevBuf key0, key1, idx, key0b, key1b, idxb;   // keys and body permutation
+    DevBuf lcp, cnt, base;                        // per body
+    DevBuf cub;                                   // hipCUB temporary storage
+    DevBuf node_d;                                // 8 doubles per cell: cx cy cz w m mx my mz
+    DevBuf node_i;                                // 4 ints per cell: next pt level start
+    DevBuf info;                                  // total, maxlevel, flags
+    DevBuf coll_cnt, coll_off, coll_rec;          // collision search
+    size_t body_cap = 0, node_cap = 0, rec_cap = 0;
+    int nnodes = 0;
+    int maxlevel = 0;
+    int L1 = 0, L2 = 0, rbits = 0;
+    bool two_words = false;
+    int nroot = 0;
+};
+
+// ---------------------------------------------------------------- key construction
+struct TreeGeom {
+    double root_size;
+    int nrx, nry, nrz;
+    int L1, L2;     // digits in word 0 (below the root index) and word 1
+};
+
+__device__ __forceinline__ int root_index(const TreeGeom& g, double x, double y, double z) {
+    // particle.c:119-126
+    const int i = ((int)floor((x + g.root_size * (double)g.nrx / 2.) / g.root_size) + g.nrx) % g.nrx;
+    const int j = ((int)floor((y + g.root_size * (double)g.nry / 2.) / g.root_size) + g.nry) % g.nry;
+    const int k = ((int)floor((z + g.root_size * (double)g.nrz / 2.) / g.root_size) + g.nrz) % g.nrz;
+    return (k * g.nry + j) * g.nrx + i;
+}
+
+// Centre of root box `ri` exactly as tree.c:91-97 computes it from a member body.
+__device__ __forceinline__ void root_center(const TreeGeom& g, int ri, double& cx, double& cy, double& cz) {
+    const int i = ri % g.nrx;
+    const int j = (ri / g.nrx) % g.nry;
+    const int k = ri / (g.nrx * g.nry);
+    const double bx = g.root_size * (double)g.nrx, by = g.root_size * (double)g.nry, bz = g.root_size * (double)g.nrz;
+    cx = -bx / 2. + g.root_size * (0.5 + (double)i);
+    cy = -by / 2. + g.root_size * (0.5 + (double)j);
+    cz = -bz / 2. + g.root_size * (0.5 + (double)k);
+}
+
+// One level down (tree.c:98-103): child w = w/2, centre +/- child w / 2.
+__device__ __forceinline__ void descend(int o, double& cx, double& cy, double& cz, double& w) {
+    w = w / 2.;
+    cx = cx + w / 2. * (((o >> 0) % 2 == 0) ? 1. : -1);
+    cy = cy + w / 2. * (((o >> 1) % 2 == 0) ? 1. : -1);
+    cz = cz + w / 2. * (((o >> 2) % 2 == 0) ? 1. : -1);
+}
+
+// flags: bit0 outside box, bit1 non-finite, bit2 unresolved at max depth, bit3 identical positions
+__global__ void k_tree_keys(const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+                            int n, TreeGeom g, int with_word1, unsigned long long* __restrict__ key0,
+                            unsigned long long* __restrict__ key1, int* __restrict__ idx, int* __restrict__ info) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double px = x[i], py = y[i], pz = z[i];
+    idx[i] = i;
+    if (fabs(px) > g.root_size * (double)g.nrx / 2. || fabs(py) > g.root_size * (double)g.nry / 2. ||
+        fabs(pz) > g.root_size * (double)g.nrz / 2.) {
+        atomicOr(&info[2], 1);
+        key0[i] = 0;
+        if (with_word1) key1[i] = 0;
+        return;
+    }
+    if (!isfinite(px) || !isfinite(py) || !isfinite(pz)) {
+        atomicOr(&info[2], 2);
+        key0[i] = 0;
+        if (with_word1) key1[i] = 0;
+        return;
+    }
+    const int ri = root_index(g, px, py, pz);
+    double cx, cy, cz, w = g.root_size;
+    root_center(g, ri, cx, cy, cz);
+    unsigned long long k0 = (unsigned long long)ri;
+    for (int l = 0; l < g.L1; l++) {
+        int o = 0;
+        if (px < cx) o += 1;
+        if (py < cy) o += 2;
+        if (pz < cz) o += 4;
+        k0 = (k0 << 3) | (unsigned long long)o;
+        descend(o, cx, cy, cz, w);
+    }
+    key0[i] = k0;
+    if (with_word1) {
+        unsigned long long k1 = 0;
+        for (int l = 0; l < g.L2; l++) {
+            int o = 0;
+            if (px < cx) o += 1;
+            if (py < cy) o += 2;
+            if (pz < cz) o += 4;
+            k1 = (k1 << 3) | (unsigned long long)o;
+            descend(o, cx, cy, cz, w);
+        }
+        key1[i] = k1;
+    }
+}
+
+// Number of common leading octant digits of two keys (-1: different root box).
+__device__ __forceinline__ int key_lcp(unsigned long long a0, unsigned long long a1, unsigned long long b0,
+                                       unsigned long long b1, int L1, int L2, int two) {
+    const unsigned long long x0 = a0 ^ b0;
+    if ((x0 >> (3 * L1)) != 0ull) return -1;
+    if (x0 != 0ull) return (__clzll((long long)x0) - (64 - 3 * L1)) / 3;
+    if (!two) return L1;
+    const unsigned long long x1 = a1 ^ b1;
+    if (x1 != 0ull) return L1 + (__clzll((long long)x1) - (64 - 3 * L2)) / 3;
+    return L1 + L2;
+}
+
+__global__ void k_gather_u64(const unsigned long long* __restrict__ src, const int* __restrict__ perm,
+                             unsigned long long* __restrict__ dst, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[perm[i]];
+}
+
+// lcp[s] = common digits of sorted keys s and s+1 (lcp[n-1] = -1); counts of
+// the cells each body starts; the deepest level.
+__global__ void k_tree_lcp(const unsigned long long* __restrict__ k0, const unsigned long long* __restrict__ k1,
+                           int n, int L1, int L2, int two, const int* __restrict__ idx, const double* __restrict__ x,
+                           const double* __restrict__ y, const double* __restrict__ z, int* __restrict__ lcp,
+                           int* __restrict__ info) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    int l = -1;
+    if (s + 1 < n) {
+        l = key_lcp(k0[s], two ? k1[s] : 0ull, k0[s + 1], two ? k1[s + 1] : 0ull, L1, L2, two);
+        if (l == L1 + (two ? L2 : 0)) {
+            // both bodies share the deepest representable cell
+            const int a = idx[s], b = idx[s + 1];
+            if (x[a] == x[b] && y[a] == y[b] && z[a] == z[b])
+                atomicOr(&info[2], 8);
+            else
+                atomicOr(&info[2], 4);
+        }
+    }
+    lcp[s] = l;
+}
+
+__global__ void k_tree_count(const int* __restrict__ lcp, int n, int* __restrict__ cnt, int* __restrict__ info) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int prev = s > 0 ? lcp[s - 1] : -1;
+    const int cur = lcp[s];
+    const int nint = cur > prev ? cur - prev : 0;
+    cnt[s] = nint + 1;
+    atomicMax(&info[1], (prev > cur ? prev : cur) + 1);
+}
+
+struct NodeArrays {
+    double *cx, *cy, *cz, *w, *m, *mx, *my, *mz;
+    int *next, *pt, *level, *start;
+};
+
+__device__ __forceinline__ NodeArrays node_arrays(double* d, int* ii, size_t cap) {
+    NodeArrays A;
+    A.cx = d;
+    A.cy = d + cap;
+    A.cz = d + 2 * cap;
+    A.w = d + 3 * cap;
+    A.m = d + 4 * cap;
+    A.mx = d + 5 * cap;
+    A.my = d + 6 * cap;
+    A.mz = d + 7 * cap;
+    A.next = ii;
+    A.pt = ii + cap;
+    A.level = ii + 2 * cap;
+    A.start = ii + 3 * cap;
+    return A;
+}
+
+// Geometry of the cell at `level` on the path of key (k0,k1): level 0 = root box.
+__device__ __forceinline__ void cell_geometry(const TreeGeom& g, unsigned long long k0, unsigned long long k1,
+                                              int level, double& cx, double& cy, double& cz, double& w) {
+    const int ri = (int)(k0 >> (3 * g.L1));
+    root_center(g, ri, cx, cy, cz);
+    w = g.root_size;
+    for (int l = 1; l <= level; l++) {
+        int o;
+        if (l <= g.L1)
+            o = (int)((k0 >> (3 * (g.L1 - l))) & 7ull);
+        else
+            o = (int)((k1 >> (3 * (g.L2 - (l - g.L1)))) & 7ull);
+        descend(o, cx, cy, cz, w);
+    }
+}
+
+__global__ void k_tree_nodes(const unsigned long long* __restrict__ k0, const unsigned long long* __restrict__ k1,
+                             const int* __restrict__ idx, const int* __restrict__ lcp, const int* __restrict__ base,
+                             int n, int total, TreeGeom g, int two, double* __restrict__ nd, int* __restrict__ ni,
+                             size_t cap, const double* __restrict__ pm, const double* __restrict__ px,
+                             const double* __restrict__ py, const double* __restrict__ pz) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    NodeArrays A = node_arrays(nd, ni, cap);
+    const int prev = s > 0 ? lcp[s - 1] : -1;
+    const int cur = lcp[s];
+    const unsigned long long a0 = k0[s], a1 = two ? k1[s] : 0ull;
+    const int b = base[s];
+    int id = b;
+    for (int d = prev + 1; d <= cur; d++, id++) {
+        // end of the body range of this cell: largest e with lcp(key_s, key_e) >= d
+        int lo = s + 1, step = 1, hi;
+        while (true) {
+            const int j = s + step;
+            if (j >= n) {
+                hi = n;
+                break;
+            }
+            const int l = key_lcp(a0, a1, k0[j], two ? k1[j] : 0ull, g.L1, g.L2, two);
+            if (l < d) {
+                hi = j;
+                break;
+            }
+            lo = j + 1;
+            step *= 2;
+        }
+        // first index in [lo, hi) whose lcp < d (hi if none)
+        while (lo < hi) {
+            const int mid = lo + (hi - lo) / 2;
+            const int l = key_lcp(a0, a1, k0[mid], two ? k1[mid] : 0ull, g.L1, g.L2, two);
+            if (l >= d)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        const int e = lo - 1;
+        A.next[id] = (e + 1 < n) ? base[e + 1] : total;
+        A.pt[id] = -(e - s + 1);
+        A.level[id] = d;
+        A.start[id] = s;
+        double cx, cy, cz, w;
+        cell_geometry(g, a0, a1, d, cx, cy, cz, w);
+        A.cx[id] = cx;
+        A.cy[id] = cy;
+        A.cz[id] = cz;
+        A.w[id] = w;
+    }
+    // the leaf of body s
+    const int leaf_level = (prev > cur ? prev : cur) + 1;
+    const int p = idx[s];
+    A.next[id] = id + 1;
+    A.pt[id] = p;
+    A.level[id] = leaf_level;
+    A.start[id] = s;
+    double cx, cy, cz, w;
+    cell_geometry(g, a0, a1, leaf_level, cx, cy, cz, w);
+    A.cx[id] = cx;
+    A.cy[id] = cy;
+    A.cz[id] = cz;
+    A.w[id] = w;
+    // leaf moments (tree.c:220-226)
+    A.m[id] = pm[p];
+    A.mx[id] = px[p];
+    A.my[id] = py[p];
+    A.mz[id] = pz[p];
+}
+
+// Internal cells of one level: children in octant order (tree.c:160-177).
+__global__ void k_tree_moments(double* __restrict__ nd, int* __restrict__ ni, size_t cap, int total, int level) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= total) return;
+    NodeArrays A = node_arrays(nd, ni, cap);
+    if (A.pt[c] >= 0 || A.level[c] != level) return;
+    double m = 0, mx = 0, my = 0, mz = 0;
+    const int end = A.next[c];
+    for (int d = c + 1; d < end; d = A.next[d]) {
+        const double d_m = A.m[d];
+        mx += A.mx[d] * d_m;
+        my += A.my[d] * d_m;
+        mz += A.mz[d] * d_m;
+        m += d_m;
+    }
+    const double m_tot = m;
+    if (m_tot > 0) {
+        mx /= m_tot;
+        my /= m_tot;
+        mz /= m_tot;
+    }
+    A.m[c] = m;
+    A.mx[c] = mx;
+    A.my[c] = my;
+    A.mz[c] = mz;
+}
+
+// ---------------------------------------------------------------- host: build
+static bool tree_alloc(struct reb_simulation* r, TreeBuffers* T, size_t n) {
+    if (n <= T->body_cap && T->key0.p) return true;
+    size_t cap = T->body_cap ? T->body_cap : 1024;
+    while (cap < n) cap *= 2;
+    bool ok = T->key0.ensure(8 * cap) && T->key1.ensure(8 * cap) && T->idx.ensure(4 * cap) &&
+              T->key0b.ensure(8 * cap) && T->key1b.ensure(8 * cap) && T->idxb.ensure(4 * cap) &&
+              T->lcp.ensure(4 * cap) && T->cnt.ensure(4 * cap) && T->base.ensure(4 * cap) && T->info.ensure(64);
+    if (!ok) {
+        reb_simulation_error(r, "Out of device memory for the tree.");
+        return false;
+    }
+    T->body_cap = cap;
+    return true;
+}
+
+static TreeGeom tree_geom(struct reb_simulation* r, TreeBuffers* T) {
+    TreeGeom g;
+    g.root_size = r->root_size;
+    g.nrx = (int)r->N_root_x;
+    g.nry = (int)r->N_root_y;
+    g.nrz = (int)r->N_root_z;
+    int nroot = g.nrx * g.nry * g.nrz;
+    int rbits = 0;
+    while ((1 << rbits) < nroot) rbits++;
+    g.L1 = (63 - rbits) / 3;
+    g.L2 = 21;
+    T->L1 = g.L1;
+    T->L2 = g.L2;
+    T->rbits = rbits;
+    T->nroot = nroot;
+    return g;
+}
+
+// Sort (key0[,key1]) with the body permutation; the result is in key0/key1/idx.
+static bool tree_sort(struct reb_simulation* r, TreeBuffers* T, int n, bool two) {
+    DeviceState* d = dev_of(r);
+    auto* K0 = T->key0.as<unsigned long long>();
+    auto* K1 = T->key1.as<unsigned long long>();
+    auto* K0b = T->key0b.as<unsigned long long>();
+    auto* K1b = T->key1b.as<unsigned long long>();
+    int* I = T->idx.as<int>();
+    int* Ib = T->idxb.as<int>();
+    const int keybits = 3 * T->L1 + T->rbits;
+    size_t tmp = 0;
+    hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, K0, K0b, I, Ib, n, 0, keybits, d->stream);
+    size_t tmp2 = 0;
+    hipcub::DeviceRadixSort::SortPairs(nullptr, tmp2, K1, K1b, I, Ib, n, 0, 3 * T->L2, d->stream);
+    tmp = std::max(tmp, tmp2);
+    if (!T->cub.ensure(tmp + 256)) return false;
+    if (!two) {
+        if (hipcub::DeviceRadixSort::SortPairs(T->cub.p, tmp, K0, K0b, I, Ib, n, 0, keybits, d->stream) !=
+            hipSuccess)
+            return RB_HIP(r, hipGetLastError());
+        std::swap(T->key0, T->key0b);
+        std::swap(T->idx, T->idxb);
+        return true;
+    }
+    // least significant word first, then a stable sort on the most significant
+    if (hipcub::DeviceRadixSort::SortPairs(T->cub.p, tmp, K1, K1b, I, Ib, n, 0, 3 * T->L2, d->stream) != hipSuccess)
+        return RB_HIP(r, hipGetLastError());
+    hipLaunchKernelGGL(k_gather_u64, dim3((n + 255) / 256), dim3(256), 0, d->stream, K0, Ib, K0b, n);
+    // K0b: key0 in key1 order; Ib: permutation; sort (K0b, Ib) -> (K0, I)
+    if (hipcub::DeviceRadixSort::SortPairs(T->cub.p, tmp, K0b, K0, Ib, I, n, 0, keybits, d->stream) != hipSuccess)
+        return RB_HIP(r, hipGetLastError());
+    hipLaunchKernelGGL(k_gather_u64, dim3((n + 255) / 256), dim3(256), 0, d->stream, K1, I, K1b, n);
+    std::swap(T->key1, T->key1b);   // key1 now in final order
+    return RB_HIP(r, hipGetLastError());
+}
+
+// Builds the tree of the current device positions.  tree.c:254-271 + :147-229.
+bool tree_build(struct reb_simulation* r) {
+    DeviceState* d = dev_of(r);
+    if (r->root_size <= 0.0) {
+        reb_simulation_error(r, "Set root_size to a finite value to use a tree based gravity or collision solver.");
+        return false;
+    }
+    if (!d->tree) d->tree = new TreeBuffers();
+    TreeBuffers* T = d->tree;
+    const int n = (int)r->N;
+    if (!tree_alloc(r, T, n ? n : 1)) return false;
+    TreeGeom g = tree_geom(r, T);
+    int* info = T->info.as<int>();
+    bool two = false;
+    int hinfo[4] = {0, 0, 0, 0};
+    for (int attempt = 0; attempt < 2; attempt++) {
+        two = attempt == 1;
+        if (!RB_HIP(r, hipMemsetAsync(info, 0, 16, d->stream))) return false;
+        if (n == 0) break;
+        hipLaunchKernelGGL(k_tree_keys, dim3((n + 255) / 256), dim3(256), 0, d->stream, d->field(FX), d->field(FY),
+                           d->field(FZ), n, g, two ? 1 : 0, T->key0.as<unsigned long long>(),
+                           T->key1.as<unsigned long long>(), T->idx.as<int>(), info);
+        if (!tree_sort(r, T, n, two)) return false;
+        hipLaunchKernelGGL(k_tree_lcp, dim3((n + 255) / 256), dim3(256), 0, d->stream,
+                           T->key0.as<unsigned long long>(), T->key1.as<unsigned long long>(), n, g.L1, g.L2,
+                           two ? 1 : 0, T->idx.as<int>(), d->field(FX), d->field(FY), d->field(FZ),
+                           T->lcp.as<int>(), info);
+        if (!RB_HIP(r, hipMemcpyAsync(hinfo, info, 16, hipMemcpyDeviceToHost, d->stream))) return false;
+        if (!stream_sync(r)) return false;
+        if (hinfo[2] & 1) {
+            reb_simulation_error(r, "Particle is outside of simulation box. Cannot add to tree.");
+            return false;
+        }
+        if (hinfo[2] & 2) {
+            reb_simulation_error(r, "Particle has non-finite coordinates. Cannot add to tree.");
+            return false;
+        }
+        if (hinfo[2] & 8) {
+            reb_simulation_error(r, "Cannot add two particles with the same coordinates to the tree.");
+            return false;
+        }
+        if (!(hinfo[2] & 4)) break;
+        if (two) {
+            reb_simulation_error(r, "Tree deeper than 63 levels (two bodies closer than root_size/2^62).");
+            return false;
+        }
+    }
+    T->two_words = two;
+    T->nnodes = 0;
+    if (n == 0) return true;
+    hipLaunchKernelGGL(k_tree_count, dim3((n + 255) / 256), dim3(256), 0, d->stream, T->lcp.as<int>(), n,
+                       T->cnt.as<int>(), info);
+    size_t tmp = 0;
+    hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, T->cnt.as<int>(), T->base.as<int>(), n, d->stream);
+    if (!T->cub.ensure(tmp + 256)) return false;
+    hipcub::DeviceScan::ExclusiveSum(T->cub.p, tmp, T->cnt.as<int>(), T->base.as<int>(), n, d->stream);
+    int last[2] = {0, 0};
+    if (!RB_HIP(r, hipMemcpyAsync(&last[0], T->base.as<int>() + (n - 1), 4, hipMemcpyDeviceToHost, d->stream)))
+        return false;
+    if (!RB_HIP(r, hipMemcpyAsync(&last[1], T->cnt.as<int>() + (n - 1), 4, hipMemcpyDeviceToHost, d->stream)))
+        return false;
+    if (!RB_HIP(r, hipMemcpyAsync(hinfo, info, 16, hipMemcpyDeviceToHost, d->stream))) return false;
+    if (!stream_sync(r)) return false;
+    const int total = last[0] + last[1];
+    T->maxlevel = hinfo[1];
+    if ((size_t)total > T->node_cap) {
+        size_t cap = T->node_cap ? T->node_cap : 4096;
+        while (cap < (size_t)total) cap *= 2;
+        if (!T->node_d.ensure(sizeof(double) * 8 * cap) || !T->node_i.ensure(sizeof(int) * 4 * cap)) {
+            reb_simulation_error(r, "Out of device memory for the tree cells.");
+            return false;
+        }
+        T->node_cap = cap;
+    }
+    T->nnodes = total;
+    hipLaunchKernelGGL(k_tree_nodes, dim3((n + 255) / 256), dim3(256), 0, d->stream,
+                       T->key0.as<unsigned long long>(), T->key1.as<unsigned long long>(), T->idx.as<int>(),
+                       T->lcp.as<int>(), T->base.as<int>(), n, total, g, two ? 1 : 0, T->node_d.as<double>(),
+                       T->node_i.as<int>(), T->node_cap, d->field(FM), d->field(FX), d->field(FY), d->field(FZ));
+    for (int lev = T->maxlevel; lev >= 0; lev--)
+        hipLaunchKernelGGL(k_tree_moments, dim3((total + 255) / 256), dim3(256), 0, d->stream,
+                           T->node_d.as<double>(), T->node_i.as<int>(), T->node_cap, total, lev);
+    return RB_HIP(r, hipGetLastError());
+}
+
+// ---------------------------------------------------------------- tree gravity
+// tree.c:275-328 for every ghost box (gravity.c:84-102), in the reference's
+// expressions; 64 bodies (in key order) per wavefront.
+__global__ __launch_bounds__(64) void k_tree_gravity(const double* __restrict__ nd, const int* __restrict__ ni,
+                                                     size_t cap, int total, const int* __restrict__ idx, int n,
+                                                     const double* __restrict__ x, const double* __restrict__ y,
+                                                     const double* __restrict__ z, GBT gl, double G, double soft2,
+                                                     double theta2, double* __restrict__ ax,
+                                                     double* __restrict__ ay, double* __restrict__ az,
+                                                     unsigned long long* __restrict__ counter) {
+    const double* __restrict__ Cm = nd + 4 * cap;
+    const double* __restrict__ Cmx = nd + 5 * cap;
+    const double* __restrict__ Cmy = nd + 6 * cap;
+    const double* __restrict__ Cmz = nd + 7 * cap;
+    const double* __restrict__ Cw = nd + 3 * cap;
+    const int* __restrict__ Cnext = ni;
+    const int* __restrict__ Cpt = ni + cap;
+    const int s = blockIdx.x * 64 + threadIdx.x;
+    const bool live = s < n;
+    const int me = live ? idx[s] : -1;
+    const double xi = live ? x[me] : 0., yi = live ? y[me] : 0., zi = live ? z[me] : 0.;
+    double a_x = 0., a_y = 0., a_z = 0.;
+    unsigned long long inter = 0;
+    for (int g = 0; g < gl.n; g++) {
+        const double gx = gl.g[g][0] + xi, gy = gl.g[g][1] + yi, gz = gl.g[g][2] + zi;
+        int skip = live ? 0 : 0x7fffffff;
+        int c = 0;
+        while (c < total) {
+            c = __builtin_amdgcn_readfirstlane(c);
+            const int pt = Cpt[c];
+            const int nx = Cnext[c];
+            bool open = false;
+            if (c >= skip) {
+                const double dx = gx - Cmx[c];
+                const double dy = gy - Cmy[c];
+                const double dz = gz - Cmz[c];
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                bool use = true;
+                if (pt < 0) {
+                    const double w = Cw[c];
+                    if (w * w > theta2 * r2) {
+                        open = true;
+                        use = false;
+                    } else {
+                        skip = nx;
+                    }
+                } else if (pt == me) {
+                    use = false;
+                }
+                if (use) {
+                    const double _r = sqrt(r2 + soft2);
+                    const double prefact = -G / (_r * _r * _r) * Cm[c];
+                    a_x += prefact * dx;
+                    a_y += prefact * dy;
+                    a_z += prefact * dz;
+                    inter++;
+                }
+            }
+            c = (__ballot(open) != 0ull) ? c + 1 : nx;
+        }
+    }
+    if (live) {
+        ax[me] = a_x;
+        ay[me] = a_y;
+        az[me] = a_z;
+    }
+    if (counter) {
+        // per-wave reduction of the interaction count
+        for (int off = 32; off > 0; off >>= 1) inter += __shfl_down(inter, off);
+        if (threadIdx.x == 0) atomicAdd(counter, inter);
+    }
+}
+
+static unsigned long long g_last_interactions = 0;
+
+bool gravity_tree(struct reb_simulation* r) {
+    DeviceState* d = dev_of(r);
+    // gravity.c:54-56: wrap into the box before building
+    if (!boundary_device(r)) return false;
+    if (!tree_build(r)) return false;
+    TreeBuffers* T = d->tree;
+    const int n = (int)r->N;
+    if (n == 0) return true;
+    auto gbv = ghostboxes(r, r->N_ghost_x, r->N_ghost_y, r->N_ghost_z);
+    if ((int)gbv.size() > kMaxGBT) {
+        reb_simulation_error(r, "Too many ghost boxes for tree gravity (at most 125).");
+        return false;
+    }
+    GBT gl;
+    gl.n = (int)gbv.size();
+    for (int k = 0; k < gl.n; k++) {
+        gl.g[k][0] = gbv[k].x;
+        gl.g[k][1] = gbv[k].y;
+        gl.g[k][2] = gbv[k].z;
+        gl.g[k][3] = gbv[k].vx;
+        gl.g[k][4] = gbv[k].vy;
+        gl.g[k][5] = gbv[k].vz;
+    }
+    unsigned long long* counter = reinterpret_cast<unsigned long long*>(T->info.as<int>() + 8);
+    if (!RB_HIP(r, hipMemsetAsync(counter, 0, 8, d->stream))) return false;
+    hipLaunchKernelGGL(k_tree_gravity, dim3((n + 63) / 64), dim3(64), 0, d->stream, T->node_d.as<double>(),
+                       T->node_i.as<int>(), T->node_cap, T->nnodes, T->idx.as<int>(), n, d->field(FX),
+                       d->field(FY), d->field(FZ), gl, r->G, r->softening * r->softening, r->opening_angle2,
+                       d->field(FAX), d->field(FAY), d->field(FAZ), counter);
+    mark_device_newer(r);
+    return RB_HIP(r, hipGetLastError());
+}
+
+// ---------------------------------------------------------------- collision search
+// Largest and second-largest radius (simulation.c:718-799); only the second
+// value is used by the search.  Value-based, so independent of the order.
+__global__ __launch_bounds__(256) void k_top2(const double* __restrict__ rad, int n, double* __restrict__ out) {
+    __shared__ double s1[256], s2[256];
+    double a1 = -1.0, a2 = -1.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const double v = rad[i];
+        if (v > a1) {
+            a2 = a1;
+            a1 = v;
+        } else if (v > a2) {
+            a2 = v;
+        }
+    }
+    s1[threadIdx.x] = a1;
+    s2[threadIdx.x] = a2;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b1 = -1.0, b2 = -1.0;
+        for (int k = 0; k < 256; k++) {
+            const double vs[2] = {s1[k], s2[k]};
+            for (double v : vs) {
+                if (v > b1) {
+                    b2 = b1;
+                    b1 = v;
+                } else if (v > b2) {
+                    b2 = v;
+                }
+            }
+        }
+        out[2 * blockIdx.x] = b1;
+        out[2 * blockIdx.x + 1] = b2;
+    }
+}
+
+// Record of one detected collision: p1, p2, ghost box index, root box index.
+struct CollRec {
+    int p1, p2, gb, ri;
+};
+
+// Tree search of collision.c:197-269 + :422-503.  WRITE=false counts the
+// collisions of each body, WRITE=true stores them at the body's offset in
+// detection order (ghost box, root box, depth-first).
+template <bool WRITE>
+__global__ __launch_bounds__(64) void k_tree_collide(const double* __restrict__ nd, const int* __restrict__ ni,
+                                                     size_t cap, int total, const int* __restrict__ idx, int n,
+                                                     const double* __restrict__ x, const double* __restrict__ y,
+                                                     const double* __restrict__ z, const double* __restrict__ vx,
+                                                     const double* __restrict__ vy, const double* __restrict__ vz,
+                                                     const double* __restrict__ rad, GBT gl, double second_r,
+                                                     int* __restrict__ counts, const int* __restrict__ offsets,
+                                                     CollRec* __restrict__ rec) {
+    const double* __restrict__ Ccx = nd;
+    const double* __restrict__ Ccy = nd + cap;
+    const double* __restrict__ Ccz = nd + 2 * cap;
+    const double* __restrict__ Cw = nd + 3 * cap;
+    const int* __restrict__ Cnext = ni;
+    const int* __restrict__ Cpt = ni + cap;
+    const int s = blockIdx.x * 64 + threadIdx.x;
+    const bool live = s < n;
+    const int me = live ? idx[s] : -1;
+    const double px = live ? x[me] : 0., py = live ? y[me] : 0., pz = live ? z[me] : 0.;
+    const double pvx = live ? vx[me] : 0., pvy = live ? vy[me] : 0., pvz = live ? vz[me] : 0.;
+    const double p1_r = live ? rad[me] : 0.;
+    int found = 0;
+    int out = (WRITE && live) ? offsets[me] : 0;
+    for (int g = 0; g < gl.n; g++) {
+        const double gx = gl.g[g][0] + px, gy = gl.g[g][1] + py, gz = gl.g[g][2] + pz;
+        const double gvx = gl.g[g][3] + pvx, gvy = gl.g[g][4] + pvy, gvz = gl.g[g][5] + pvz;
+        int skip = live ? 0 : 0x7fffffff;
+        int c = 0;
+        int ri = -1;
+        while (c < total) {
+            c = __builtin_amdgcn_readfirstlane(c);
+            const int pt = Cpt[c];
+            const int nx = Cnext[c];
+            bool open = false;
+            if (c >= skip) {
+                if (pt >= 0) {
+                    if (pt != me) {
+                        const double dx = gx - x[pt];
+                        const double dy = gy - y[pt];
+                        const double dz = gz - z[pt];
+                        const double r2 = dx * dx + dy * dy + dz * dz;
+                        const double rp = p1_r + rad[pt];
+                        if (!(r2 > rp * rp)) {
+                            const double dvx = gvx - vx[pt];
+                            const double dvy = gvy - vy[pt];
+                            const double dvz = gvz - vz[pt];
+                            if (!(dvx * dx + dvy * dy + dvz * dz > 0)) {
+                                if (WRITE) rec[out++] = CollRec{me, pt, g, ri};
+                                found++;
+                            }
+                        }
+                    }
+                } else {
+                    const double dx = gx - Ccx[c];
+                    const double dy = gy - Ccy[c];
+                    const double dz = gz - Ccz[c];
+                    const double r2 = dx * dx + dy * dy + dz * dz;
+                    const double rp = p1_r + second_r + 0.86602540378443 * Cw[c];
+                    if (r2 < rp * rp)
+                        open = true;
+                    else
+                        skip = nx;
+                }
+            }
+            c = (__ballot(open) != 0ull) ? c + 1 : nx;
+        }
+    }
+    if (!WRITE && live) counts[me] = found;
+}
+
+// Root box of each record: the reference stores the root box being searched.
+// Recomputed on the host from p2's position is not exact for wrapped bodies;
+// the search kernel cannot cheaply know which root subtree it is in, so the
+// root index comes from the leaf's key (same value: the leaf lies in that root).
+__global__ void k_coll_root(CollRec* __restrict__ rec, int nrec, const unsigned long long* __restrict__ key0,
+                            const int* __restrict__ pos_of, int L1) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nrec) return;
+    rec[k].ri = (int)(key0[pos_of[rec[k].p2]] >> (3 * L1));
+}
+
+__global__ void k_inverse_perm(const int* __restrict__ idx, int* __restrict__ pos_of, int n) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) pos_of[idx[s]] = s;
+}
+
+// ---------------------------------------------------------------- direct search
+// collision.c:64-124: ghost box outer, projectile i, target j ascending.
+template <bool WRITE>
+__global__ __launch_bounds__(256) void k_direct_collide(const double* __restrict__ x, const double* __restrict__ y,
+                                                        const double* __restrict__ z, const double* __restrict__ vx,
+                                                        const double* __restrict__ vy, const double* __restrict__ vz,
+                                                        const double* __restrict__ rad, int n, int n_targets,
+                                                        GBT gl, int* __restrict__ counts,
+                                                        const int* __restrict__ offsets, CollRec* __restrict__ rec) {
+    __shared__ double4 tp[256];
+    __shared__ double4 tv[256];
+    const int g = blockIdx.y;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const bool live = i < n;
+    const double gx = live ? gl.g[g][0] + x[i] : 0., gy = live ? gl.g[g][1] + y[i] : 0.,
+                 gz = live ? gl.g[g][2] + z[i] : 0.;
+    const double gvx = live ? gl.g[g][3] + vx[i] : 0., gvy = live ? gl.g[g][4] + vy[i] : 0.,
+                 gvz = live ? gl.g[g][5] + vz[i] : 0.;
+    const double p1_r = live ? rad[i] : 0.;
+    int found = 0;
+    int out = (WRITE && live) ? offsets[(size_t)g * n + i] : 0;
+    for (int t = 0; t < n_targets; t += 256) {
+        __syncthreads();
+        const int jj = t + threadIdx.x;
+        if (jj < n_targets) {
+            tp[threadIdx.x] = make_double4(x[jj], y[jj], z[jj], rad[jj]);
+            tv[threadIdx.x] = make_double4(vx[jj], vy[jj], vz[jj], 0.);
+        }
+        __syncthreads();
+        if (!live) continue;
+        const int m = min(256, n_targets - t);
+        for (int k = 0; k < m; k++) {
+            const int j = t + k;
+            if (i == j) continue;
+            const double4 q = tp[k];
+            const double dx = gx - q.x, dy = gy - q.y, dz = gz - q.z;
+            const double sr = p1_r + q.w;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 > sr * sr) continue;
+            const double4 qv = tv[k];
+            const double dvx = gvx - qv.x, dvy = gvy - qv.y, dvz = gvz - qv.z;
+            if (dvx * dx + dvy * dy + dvz * dz > 0) continue;
+            if (WRITE) rec[out++] = CollRec{i, j, g, 0};
+            found++;
+        }
+    }
+    if (!WRITE && live) counts[(size_t)g * n + i] = found;
+}
+
+static bool ensure_coll(struct reb_simulation* r, TreeBuffers* T, size_t nslots, size_t nrec) {
+    if (!T->coll_cnt.ensure(4 * (nslots + 1)) || !T->coll_off.ensure(4 * (nslots + 1))) return false;
+    if (nrec > T->rec_cap || !T->coll_rec.p) {
+        size_t cap = T->rec_cap ? T->rec_cap : 4096;
+        while (cap < nrec) cap *= 2;
+        if (!T->coll_rec.ensure(sizeof(CollRec) * cap)) {
+            reb_simulation_error(r, "Out of device memory for the collision list.");
+            return false;
+        }
+        T->rec_cap = cap;
+    }
+    return true;
+}
+
+// Exclusive scan of counts -> offsets; returns the total.
+static long scan_counts(struct reb_simulation* r, TreeBuffers* T, int nslots) {
+    DeviceState* d = dev_of(r);
+    size_t tmp = 0;
+    hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, T->coll_cnt.as<int>(), T->coll_off.as<int>(), nslots + 1,
+                                     d->stream);
+    if (!T->cub.ensure(tmp + 256)) return -1;
+    // counts[nslots] = 0 so that offsets[nslots] is the total
+    if (!RB_HIP(r, hipMemsetAsync(T->coll_cnt.as<int>() + nslots, 0, 4, d->stream))) return -1;
+    hipcub::DeviceScan::ExclusiveSum(T->cub.p, tmp, T->coll_cnt.as<int>(), T->coll_off.as<int>(), nslots + 1,
+                                     d->stream);
+    int total = 0;
+    if (!RB_HIP(r, hipMemcpyAsync(&total, T->coll_off.as<int>() + nslots, 4, hipMemcpyDeviceToHost, d->stream)))
+        return -1;
+    if (!stream_sync(r)) return -1;
+    return total;
+}
+
+// Copies the device records into r->collisions in detection order.
+static bool fetch_collisions(struct reb_simulation* r, TreeBuffers* T, long total, const GBT& gl) {
+    DeviceState* d = dev_of(r);
+    r->N_collisions = 0;
+    if (total <= 0) return true;
+    std::vector<CollRec> h((size_t)total);
+    if (!RB_HIP(r, hipMemcpyAsync(h.data(), T->coll_rec.p, sizeof(CollRec) * total, hipMemcpyDeviceToHost,
+                                  d->stream)))
+        return false;
+    if (!stream_sync(r)) return false;
+    if (r->N_allocated_collisions < (size_t)total) {
+        size_t cap = r->N_allocated_collisions ? r->N_allocated_collisions : 32;
+        while (cap < (size_t)total) cap *= 2;
+        r->collisions = (struct reb_collision*)realloc(r->collisions, sizeof(struct reb_collision) * cap);
+        r->N_allocated_collisions = cap;
+    }
+    for (long k = 0; k < total; k++) {
+        struct reb_collision c;
+        c.p1 = (size_t)h[k].p1;
+        c.p2 = (size_t)h[k].p2;
+        const double* g = gl.g[h[k].gb];
+        c.gb = reb_vec6d{g[0], g[1], g[2], g[3], g[4], g[5]};
+        c.ri = (size_t)h[k].ri;
+        r->collisions[k] = c;
+    }
+    r->N_collisions = (size_t)total;
+    return true;
+}
+
+bool collision_search_device(struct reb_simulation* r) {
+    DeviceState* d = dev_of(r);
+    r->N_collisions = 0;
+    if (!ensure_device(r)) return false;
+    const int n = (int)r->N;
+    if (n == 0) return true;
+    if (r->map) {
+        reb_simulation_error(r, "r->map is not supported by the collision search of this engine.");
+        return false;
+    }
+    // inner ring of ghost boxes (collision.c:66-69, :204-206)
+    const int gx = r->N_ghost_x > 1 ? 1 : r->N_ghost_x;
+    const int gy = r->N_ghost_y > 1 ? 1 : r->N_ghost_y;
+    const int gz = r->N_ghost_z > 1 ? 1 : r->N_ghost_z;
+    auto gbv = ghostboxes(r, gx, gy, gz);
+    GBT gl;
+    gl.n = (int)gbv.size();
+    for (int k = 0; k < gl.n; k++) {
+        gl.g[k][0] = gbv[k].x;
+        gl.g[k][1] = gbv[k].y;
+        gl.g[k][2] = gbv[k].z;
+        gl.g[k][3] = gbv[k].vx;
+        gl.g[k][4] = gbv[k].vy;
+        gl.g[k][5] = gbv[k].vz;
+    }
+    if (!d->tree) d->tree = new TreeBuffers();
+    TreeBuffers* T = d->tree;
+    switch (r->collision) {
+        case REB_COLLISION_DIRECT: {
+            const int n_targets = (r->N_targets != SIZE_MAX) ? (int)r->N_targets : n;
+            const int nslots = gl.n * n;
+            if (!ensure_coll(r, T, nslots, 1)) return false;
+            dim3 grid((n + 255) / 256, gl.n);
+            hipLaunchKernelGGL(k_direct_collide<false>, grid, dim3(256), 0, d->stream, d->field(FX), d->field(FY),
+                               d->field(FZ), d->field(FVX), d->field(FVY), d->field(FVZ), d->field(FR), n, n_targets,
+                               gl, T->coll_cnt.as<int>(), nullptr, nullptr);
+            const long total = scan_counts(r, T, nslots);
+            if (total < 0) return false;
+            if (!ensure_coll(r, T, nslots, (size_t)total + 1)) return false;
+            if (total > 0)
+                hipLaunchKernelGGL(k_direct_collide<true>, grid, dim3(256), 0, d->stream, d->field(FX), d->field(FY),
+                                   d->field(FZ), d->field(FVX), d->field(FVY), d->field(FVZ), d->field(FR), n,
+                                   n_targets, gl, nullptr, T->coll_off.as<int>(), T->coll_rec.as<CollRec>());
+            return fetch_collisions(r, T, total, gl);
+        }
+        case REB_COLLISION_TREE: {
+            if (!tree_build(r)) return false;
+            // second largest radius
+            double* top = reinterpret_cast<double*>(T->info.as<int>() + 16);
+            const int nb = std::min(64, (n + 255) / 256);
+            hipLaunchKernelGGL(k_top2, dim3(nb), dim3(256), 0, d->stream, d->field(FR), n, top);
+            std::vector<double> h(2 * nb);
+            if (!RB_HIP(r, hipMemcpyAsync(h.data(), top, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, d->stream)))
+                return false;
+            if (!stream_sync(r)) return false;
+            double b1 = -1.0, b2 = -1.0;
+            for (double v : h) {
+                if (v > b1) {
+                    b2 = b1;
+                    b1 = v;
+                } else if (v > b2) {
+                    b2 = v;
+                }
+            }
+            const double second_r = (n >= 2) ? b2 : 0.;
+            if (!ensure_coll(r, T, n, 1)) return false;
+            const dim3 grid((n + 63) / 64);
+            hipLaunchKernelGGL(k_tree_collide<false>, grid, dim3(64), 0, d->stream, T->node_d.as<double>(),
+                               T->node_i.as<int>(), T->node_cap, T->nnodes, T->idx.as<int>(), n, d->field(FX),
+                               d->field(FY), d->field(FZ), d->field(FVX), d->field(FVY), d->field(FVZ), d->field(FR),
+                               gl, second_r, T->coll_cnt.as<int>(), nullptr, nullptr);
+            const long total = scan_counts(r, T, n);
+            if (total < 0) return false;
+            if (!ensure_coll(r, T, n, (size_t)total + 1)) return false;
+            if (total > 0) {
+                hipLaunchKernelGGL(k_tree_collide<true>, grid, dim3(64), 0, d->stream, T->node_d.as<double>(),
+                                   T->node_i.as<int>(), T->node_cap, T->nnodes, T->idx.as<int>(), n, d->field(FX),
+                                   d->field(FY), d->field(FZ), d->field(FVX), d->field(FVY), d->field(FVZ),
+                                   d->field(FR), gl, second_r, nullptr, T->coll_off.as<int>(),
+                                   T->coll_rec.as<CollRec>());
+                // root index of each record (the lcp buffer is free now: reuse as pos_of)
+                int* pos_of = T->lcp.as<int>();
+                hipLaunchKernelGGL(k_inverse_perm, dim3((n + 255) / 256), dim3(256), 0, d->stream, T->idx.as<int>(),
+                                   pos_of, n);
+                hipLaunchKernelGGL(k_coll_root, dim3((total + 255) / 256), dim3(256), 0, d->stream,
+                                   T->coll_rec.as<CollRec>(), (int)total, T->key0.as<unsigned long long>(), pos_of,
+                                   T->L1);
+            }
+            return fetch_collisions(r, T, total, gl);
+        }
+        case REB_COLLISION_LINE:
+        case REB_COLLISION_LINETREE:
+            reb_simulation_error(r, "LINE and LINETREE collision searches are not part of this engine.");
+            return false;
+        default:
+            return true;
+    }
+}
+
+void tree_release(struct reb_simulation* r) {
+    DeviceState* d = dev_of(r);
+    if (!d->tree) return;
+    TreeBuffers* T = d->tree;
+    for (DevBuf* b : {&T->key0, &T->key1, &T->idx, &T->key0b, &T->key1b, &T->idxb, &T->lcp, &T->cnt, &T->base,
+                      &T->cub, &T->node_d, &T->node_i, &T->info, &T->coll_cnt, &T->coll_off, &T->coll_rec})
+        b->release();
+    delete T;
+    d->tree = nullptr;
+}
+
+// Tree dump for tests (pre-order rows x y z w m mx my mz pt, root order).
+int tree_dump_device(struct reb_simulation* r, double* out, long cap) {
+    DeviceState* d = dev_of(r);
+    TreeBuffers* T = d->tree;
+    if (!T) return -1;
+    const int total = T->nnodes;
+    if (!out) return total;
+    std::vector<double> hd(8 * (size_t)total);
+    std::vector<int> hi(2 * (size_t)total);
+    for (int f = 0; f < 8; f++)
+        (void)hipMemcpyAsync(hd.data() + (size_t)f * total, T->node_d.as<double>() + (size_t)f * T->node_cap,
+                             sizeof(double) * total, hipMemcpyDeviceToHost, d->stream);
+    (void)hipMemcpyAsync(hi.data(), T->node_i.as<int>() + T->node_cap, sizeof(int) * total, hipMemcpyDeviceToHost,
+                         d->stream);
+    (void)hipStreamSynchronize(d->stream);
+    const long nrow = std::min<long>(cap, total);
+    for (long k = 0; k < nrow; k++) {
+        for (int f = 0; f < 8; f++) out[9 * k + f] = hd[(size_t)f * total + k];
+        out[9 * k + 8] = (double)hi[k];
+    }
+    return total;
+}
+
+}  // namespace rbamd
+
+extern "C" {
+
+// Test hook: builds the tree of the simulation's current particles on the GPU
+// and copies the cells out in pre-order.  Returns the number of cells, or -1.
+REB_API long reb_amd_tree_dump(struct reb_simulation* r, double* out, long cap) {
+    using namespace rbamd;
+    mark_host_newer(r);
+    if (!ensure_device(r)) return -1;
+    if (!tree_build(r)) return -1;
+    return tree_dump_device(r, out, cap);
+}
+
+}  // extern "C"

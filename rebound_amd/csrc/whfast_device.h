@@ -1,0 +1,53 @@
+// Shared descriptors of the WHFast group kernel (whfast.hip), used by the
+// per-simulation plugin and by the batched ensemble (ensemble.hip).
+#pragma once
+
+#include <cstddef>
+
+namespace rbamd {
+
+// One planetary system inside a body array (structure of arrays, `stride`
+// doubles per field): bodies body0 .. body0+N-1, body0 is the central body.
+struct WHSys {
+    int body0;
+    int N;           // bodies including the central one
+    int Nact;        // N_active used by the WHFast sums (N unless test particles of type 0)
+    int Nact_grav;   // N_active used by the gravity loop
+    int tp_type;     // testparticle_type
+    int pad;
+    double dt;
+};
+
+// A wavefront's unit of work: `nsys` systems of one group size G (lanes per
+// system), taken from order[sys0 .. sys0+nsys).
+struct WHTask {
+    int G;
+    int sys0;
+    int nsys;
+};
+
+struct WHParams {
+    const WHSys* sys;
+    const WHTask* tasks;
+    const int* order;
+    unsigned int* flags;      // per system: bit0 synchronized, bit1 period warning, bit2 resync warning
+    unsigned int* counter;    // work counter of the persistent grid
+    int ntasks;
+    int grid;                 // persistent workgroups (one wavefront each)
+    double* body;             // fields x y z vx vy vz ax ay az m (r unused), stride doubles apart
+    double* ph;               // heliocentric fields x y z vx vy vz m
+    size_t stride;
+    double G;
+    double soft2;
+    int compensated;
+    long nsteps;
+    int safe_mode;
+    int did_modify;
+    int sync_at_end;
+};
+
+int whfast_launch(const WHParams& P, hipStream_t st);
+int whfast_energy_launch(const WHParams& P, double* out, int nsys, hipStream_t st);
+int whfast_group_size(int K);
+
+}  // namespace rbamd

@@ -1,0 +1,464 @@
+/*
+ * Golden-vector generator.  Links the reference library compiled from
+ * /root/reference/src by oracle/Makefile (target `ref`) and writes small .npy
+ * fixtures into tests/golden/.  Runs only where the reference sources exist;
+ * the fixtures it writes are committed and are what the GPU box sees.
+ *
+ * Build + run: python tests/golden/make_golden.py
+ *
+ * Every fixture here is produced by the reference's own functions:
+ *   rand_r / reb_random_*                tools.c:64-105, rebound.c:238-259
+ *   reb_simulation_add_plummer           tools.c:463-502
+ *   reb_gravity_{basic,compensated}      gravity.c:167-531 (through update_acceleration)
+ *   leapfrog + energy                    integrator_leapfrog.c:97-209, tools.c:108-162
+ *   whfast kepler solver                 integrator_whfast.c:241-427
+ *   whfast DH integration                integrator_whfast.c:1113-1259
+ *   tree build / moments / walk          tree.c:64-328, gravity.c:47-106
+ *   collision search + hardsphere        collision.c:49-405, 573-665
+ *   SEI + shear boundary                 integrator_sei.c:86-174, boundary.c:35-201
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#include <stdint.h>
+#include "rebound.h"
+#include "tree.h"
+
+static const char* outdir = ".";
+
+/* ---------------------------------------------------------------- npy out */
+static void write_npy(const char* name, const char* descr, size_t elsize,
+                      const void* data, int ndim, const size_t* shape){
+    char path[1024];
+    snprintf(path, sizeof(path), "%s/%s", outdir, name);
+    FILE* f = fopen(path, "wb");
+    if (!f){ perror(path); exit(1); }
+    char shp[256] = {0};
+    size_t n = 1;
+    if (ndim == 1){
+        snprintf(shp, sizeof(shp), "(%zu,)", shape[0]);
+    }else{
+        int off = snprintf(shp, sizeof(shp), "(");
+        for (int d=0; d<ndim; d++){
+            off += snprintf(shp+off, sizeof(shp)-off, "%zu%s", shape[d], d==ndim-1?")":", ");
+        }
+    }
+    for (int d=0; d<ndim; d++) n *= shape[d];
+    char header[512];
+    int hl = snprintf(header, sizeof(header),
+        "{'descr': '%s', 'fortran_order': False, 'shape': %s, }", descr, shp);
+    int total = 10 + hl + 1;
+    int pad = (64 - total % 64) % 64;
+    fwrite("\x93NUMPY\x01\x00", 1, 8, f);
+    uint16_t hlen = (uint16_t)(hl + pad + 1);
+    fwrite(&hlen, 2, 1, f);
+    fwrite(header, 1, hl, f);
+    for (int i=0; i<pad; i++) fputc(' ', f);
+    fputc('\n', f);
+    fwrite(data, elsize, n, f);
+    fclose(f);
+}
+static void wd(const char* name, const double* d, size_t n0, size_t n1){
+    size_t shape[2] = {n0, n1};
+    write_npy(name, "<f8", 8, d, n1 ? 2 : 1, shape);
+}
+static void wi(const char* name, const int64_t* d, size_t n0, size_t n1){
+    size_t shape[2] = {n0, n1};
+    write_npy(name, "<i8", 8, d, n1 ? 2 : 1, shape);
+}
+
+/* state layout used by every fixture: x y z vx vy vz ax ay az m r */
+#define NCOL 11
+static double* state_of(struct reb_simulation* r){
+    double* s = malloc(sizeof(double)*NCOL*r->N);
+    for (size_t i=0; i<r->N; i++){
+        struct reb_particle p = r->particles[i];
+        double row[NCOL] = {p.x,p.y,p.z,p.vx,p.vy,p.vz,p.ax,p.ay,p.az,p.m,p.r};
+        memcpy(s+NCOL*i, row, sizeof(row));
+    }
+    return s;
+}
+static void dump_state(const char* name, struct reb_simulation* r){
+    double* s = state_of(r);
+    wd(name, s, r->N, NCOL);
+    free(s);
+}
+static double* acc_of(struct reb_simulation* r){
+    double* a = malloc(sizeof(double)*3*r->N);
+    for (size_t i=0; i<r->N; i++){
+        a[3*i+0] = r->particles[i].ax;
+        a[3*i+1] = r->particles[i].ay;
+        a[3*i+2] = r->particles[i].az;
+    }
+    return a;
+}
+
+/* ------------------------------------------------------------ fixtures */
+static void gen_rng(void){
+    unsigned int seeds[3] = {1u, 42u, 12345u};
+    int64_t out[3*64];
+    for (int s=0; s<3; s++){
+        unsigned int seed = seeds[s];
+        for (int i=0; i<64; i++) out[s*64+i] = rand_r(&seed);
+    }
+    wi("rand_r_streams.npy", out, 3, 64);
+
+    struct reb_simulation* r = reb_simulation_create();
+    r->rand_seed = 7;
+    double st[4*256];
+    for (int i=0; i<256; i++) st[0*256+i] = reb_random_uniform(r, -1., 3.);
+    for (int i=0; i<256; i++) st[1*256+i] = reb_random_normal(r, 2.);
+    for (int i=0; i<256; i++) st[2*256+i] = reb_random_powerlaw(r, 1., 4., -3.);
+    for (int i=0; i<256; i++) st[3*256+i] = reb_random_rayleigh(r, 0.02);
+    wd("random_streams_seed7.npy", st, 4, 256);
+    reb_simulation_free(r);
+}
+
+static struct reb_simulation* plummer(size_t N, unsigned int seed, double soft){
+    struct reb_simulation* r = reb_simulation_create();
+    r->rand_seed = seed;
+    r->softening = soft;
+    r->G = 1.;
+    reb_simulation_add_plummer(r, N, 1., 1.);
+    reb_simulation_move_to_com(r);
+    return r;
+}
+
+static void gen_plummer(void){
+    /* P1 state, BASIC + COMPENSATED accelerations, leapfrog energy history */
+    struct reb_simulation* r = plummer(1024, 1u, 0.01);
+    dump_state("plummer1024_seed1_state.npy", r);
+    r->gravity = REB_GRAVITY_BASIC;
+    reb_simulation_update_acceleration(r);
+    double* a = acc_of(r); wd("plummer1024_seed1_acc_basic.npy", a, r->N, 3); free(a);
+    r->gravity = REB_GRAVITY_COMPENSATED;
+    reb_simulation_update_acceleration(r);
+    a = acc_of(r); wd("plummer1024_seed1_acc_comp.npy", a, r->N, 3); free(a);
+
+    /* test-particle variant: the last quarter are test particles of type 1 */
+    r->N_active = 768; r->testparticle_type = 1;
+    r->gravity = REB_GRAVITY_BASIC;
+    reb_simulation_update_acceleration(r);
+    a = acc_of(r); wd("plummer1024_seed1_acc_basic_tp1.npy", a, r->N, 3); free(a);
+    r->gravity = REB_GRAVITY_COMPENSATED;
+    reb_simulation_update_acceleration(r);
+    a = acc_of(r); wd("plummer1024_seed1_acc_comp_tp1.npy", a, r->N, 3); free(a);
+    r->N_active = SIZE_MAX; r->testparticle_type = 0;
+
+    /* 1000 leapfrog steps, energy every 10 steps */
+    r->gravity = REB_GRAVITY_BASIC;
+    reb_simulation_set_integrator(r, "leapfrog");
+    r->dt = 1e-3;
+    double e[101];
+    e[0] = reb_simulation_energy(r);
+    for (int k=1; k<=100; k++){
+        reb_simulation_steps(r, 10);
+        e[k] = reb_simulation_energy(r);
+    }
+    wd("plummer1024_seed1_lf_energy.npy", e, 101, 0);
+    dump_state("plummer1024_seed1_lf_final.npy", r);
+    reb_simulation_free(r);
+
+    /* split-precision error model input: N=4096 */
+    r = plummer(4096, 2u, 0.005);
+    dump_state("plummer4096_seed2_state.npy", r);
+    reb_simulation_update_acceleration(r);
+    a = acc_of(r); wd("plummer4096_seed2_acc_basic.npy", a, r->N, 3); free(a);
+    reb_simulation_free(r);
+}
+
+/* ghost boxes make the serial and OpenMP BASIC orders differ; the OpenMP
+ * per-i order is the one emitted by the OMP binary of this generator */
+static void gen_ghostbox_basic(const char* tag){
+    struct reb_simulation* r = reb_simulation_create();
+    r->rand_seed = 3;
+    r->boundary = REB_BOUNDARY_PERIODIC;
+    r->root_size = 10.; r->N_root_x = 1; r->N_root_y = 1; r->N_root_z = 1;
+    r->N_ghost_x = 1; r->N_ghost_y = 1; r->N_ghost_z = 0;
+    r->softening = 0.05;
+    for (int i=0; i<256; i++){
+        struct reb_particle p = {0};
+        p.x = reb_random_uniform(r, -5., 5.);
+        p.y = reb_random_uniform(r, -5., 5.);
+        p.z = reb_random_uniform(r, -5., 5.);
+        p.m = reb_random_uniform(r, 0.5, 1.5)/256.;
+        reb_simulation_add(r, p);
+    }
+    char name[128];
+    snprintf(name, sizeof(name), "periodic256_state.npy");
+    dump_state(name, r);
+    reb_simulation_update_acceleration(r);
+    double* a = acc_of(r);
+    snprintf(name, sizeof(name), "periodic256_acc_basic_%s.npy", tag);
+    wd(name, a, r->N, 3); free(a);
+    reb_simulation_free(r);
+}
+
+static void gen_plummer_omp(void){
+    struct reb_simulation* r = plummer(1024, 1u, 0.01);
+    r->gravity = REB_GRAVITY_BASIC;
+    reb_simulation_update_acceleration(r);
+    double* a = acc_of(r); wd("plummer1024_seed1_acc_basic_omp.npy", a, r->N, 3); free(a);
+    r->gravity = REB_GRAVITY_COMPENSATED;
+    reb_simulation_update_acceleration(r);
+    a = acc_of(r); wd("plummer1024_seed1_acc_comp_omp.npy", a, r->N, 3); free(a);
+    reb_simulation_free(r);
+}
+
+/* Kepler-solver vectors: (x y z vx vy vz mu dt) -> (x y z vx vy vz) */
+static void gen_kepler(void){
+    struct reb_simulation* r = reb_simulation_create();
+    r->rand_seed = 11;
+    const int NK = 640;
+    double* in = malloc(sizeof(double)*NK*8);
+    double* out = malloc(sizeof(double)*NK*6);
+    struct reb_particle star = {0}; star.m = 1.;
+    for (int k=0; k<NK; k++){
+        struct reb_particle p;
+        double mu = reb_random_uniform(r, 0.5, 2.);
+        double dt;
+        int cls = k % 8;
+        if (cls < 4){       /* elliptic, small to moderate steps */
+            double e = cls==3 ? reb_random_uniform(r, 0.9, 0.99) : reb_random_uniform(r, 0., 0.6);
+            double a = reb_random_uniform(r, 0.5, 3.);
+            p = reb_particle_from_orbit(mu, star, 0., a, e, reb_random_uniform(r, 0., M_PI),
+                    reb_random_uniform(r, 0., 2.*M_PI), reb_random_uniform(r, 0., 2.*M_PI),
+                    reb_random_uniform(r, 0., 2.*M_PI));
+            double P = 2.*M_PI*sqrt(a*a*a/mu);
+            dt = P*reb_random_uniform(r, -0.2, 0.2);
+        }else if (cls < 6){ /* elliptic, steps of several orbits (quartic / bisection branches) */
+            double e = reb_random_uniform(r, 0., 0.8);
+            double a = reb_random_uniform(r, 0.5, 3.);
+            p = reb_particle_from_orbit(mu, star, 0., a, e, reb_random_uniform(r, 0., M_PI),
+                    reb_random_uniform(r, 0., 2.*M_PI), reb_random_uniform(r, 0., 2.*M_PI),
+                    reb_random_uniform(r, 0., 2.*M_PI));
+            double P = 2.*M_PI*sqrt(a*a*a/mu);
+            dt = P*reb_random_uniform(r, 1.5, 7.);
+        }else if (cls == 6){ /* hyperbolic */
+            double e = reb_random_uniform(r, 1.1, 5.);
+            double a = -reb_random_uniform(r, 0.5, 3.);
+            double fmax = acos(-1./e)*0.9;
+            p = reb_particle_from_orbit(mu, star, 0., a, e, reb_random_uniform(r, 0., M_PI),
+                    reb_random_uniform(r, 0., 2.*M_PI), reb_random_uniform(r, 0., 2.*M_PI),
+                    reb_random_uniform(r, -fmax, fmax));
+            dt = reb_random_uniform(r, -2., 2.);
+        }else{               /* (almost) straight line: huge velocity */
+            p = (struct reb_particle){0};
+            p.x = reb_random_uniform(r, -1., 1.); p.y = reb_random_uniform(r, -1., 1.); p.z = reb_random_uniform(r, -1., 1.);
+            p.vx = 1e8*reb_random_uniform(r, -1., 1.); p.vy = 1e8*reb_random_uniform(r, -1., 1.); p.vz = 1e8*reb_random_uniform(r, -1., 1.);
+            dt = reb_random_uniform(r, 1e-3, 1e-2);
+        }
+        double row[8] = {p.x,p.y,p.z,p.vx,p.vy,p.vz,mu,dt};
+        memcpy(in+8*k, row, sizeof(row));
+        reb_integrator_whfast_kepler_solver(&p, mu, dt, NULL);
+        double o[6] = {p.x,p.y,p.z,p.vx,p.vy,p.vz};
+        memcpy(out+6*k, o, sizeof(o));
+    }
+    wd("kepler_in.npy", in, NK, 8);
+    wd("kepler_out.npy", out, NK, 6);
+    free(in); free(out);
+    reb_simulation_free(r);
+}
+
+/* Planetary systems as in the batched-ensemble workload; 32 systems, K=2..8 */
+static void gen_whfast(void){
+    const int NS = 32;
+    struct reb_simulation* g = reb_simulation_create();
+    g->rand_seed = 2024;
+    size_t maxb = 9;
+    double* init = calloc(NS*maxb*NCOL, sizeof(double));
+    int64_t* meta = calloc(NS*2, sizeof(int64_t));
+    double* dts = calloc(NS, sizeof(double));
+    for (int mode=0; mode<2; mode++){
+        double* e = calloc(NS*11, sizeof(double));
+        double* fin = calloc(NS*maxb*NCOL, sizeof(double));
+        unsigned int gseed = 2024;
+        g->rand_seed = gseed;
+        for (int s=0; s<NS; s++){
+            struct reb_simulation* r = reb_simulation_create();
+            int K = 2 + s % 7;
+            struct reb_particle star = {0}; star.m = 1.;
+            reb_simulation_add(r, star);
+            double a = 1.;
+            for (int k=0; k<K; k++){
+                if (k>0){
+                    double ratio = exp(reb_random_uniform(g, log(1.3), log(2.5)));
+                    a *= pow(ratio, 2./3.);
+                }
+                double m = exp(reb_random_uniform(g, log(1e-6), log(1e-3)));
+                double e_ = reb_random_rayleigh(g, 0.02);
+                double inc = reb_random_rayleigh(g, 0.01);
+                double Om = reb_random_uniform(g, 0., 2.*M_PI);
+                double om = reb_random_uniform(g, 0., 2.*M_PI);
+                double f  = reb_random_uniform(g, 0., 2.*M_PI);
+                struct reb_particle p = reb_particle_from_orbit(r->G, r->particles[0], m, a, e_, inc, Om, om, f);
+                reb_simulation_add(r, p);
+            }
+            reb_simulation_move_to_com(r);
+            struct reb_integrator_whfast_state* wh = reb_simulation_set_integrator(r, "whfast");
+            wh->coordinates = REB_INTEGRATOR_WHFAST_COORDINATES_DEMOCRATICHELIOCENTRIC;
+            wh->safe_mode = mode;
+            r->dt = 2.*M_PI*sqrt(1./(1.+r->particles[1].m))/20.;  /* inner period / 20 */
+            if (mode==0){
+                meta[2*s] = K; meta[2*s+1] = K+1;
+                dts[s] = r->dt;
+                double* st = state_of(r);
+                memcpy(init+s*maxb*NCOL, st, sizeof(double)*NCOL*r->N);
+                free(st);
+            }
+            e[s*11] = reb_simulation_energy(r);
+            for (int c=1; c<=10; c++){
+                reb_simulation_steps(r, 100);
+                e[s*11+c] = reb_simulation_energy(r);
+            }
+            double* st = state_of(r);
+            memcpy(fin+s*maxb*NCOL, st, sizeof(double)*NCOL*r->N);
+            free(st);
+            reb_simulation_free(r);
+        }
+        char name[128];
+        snprintf(name, sizeof(name), "whfast_dh32_safe%d_energy.npy", mode);
+        wd(name, e, NS, 11);
+        snprintf(name, sizeof(name), "whfast_dh32_safe%d_final.npy", mode);
+        size_t shape[3] = {NS, maxb, NCOL};
+        write_npy(name, "<f8", 8, fin, 3, shape);
+        free(e); free(fin);
+    }
+    size_t shape[3] = {NS, maxb, NCOL};
+    write_npy("whfast_dh32_init.npy", "<f8", 8, init, 3, shape);
+    wi("whfast_dh32_meta.npy", meta, NS, 2);
+    wd("whfast_dh32_dt.npy", dts, NS, 0);
+    free(init); free(meta); free(dts);
+    reb_simulation_free(g);
+}
+
+/* --------------------------------------------------- shearing sheet */
+static double cor_half(const struct reb_simulation* const r, double v){ (void)r; (void)v; return 0.5; }
+
+static int64_t* rec_pairs = NULL;
+static double*  rec_gb = NULL;
+static size_t   rec_n = 0, rec_cap = 0;
+static enum REB_COLLISION_RESOLVE_OUTCOME rec_resolve(struct reb_simulation* const r, struct reb_collision c){
+    if (rec_n == rec_cap){
+        rec_cap = rec_cap ? 2*rec_cap : 1024;
+        rec_pairs = realloc(rec_pairs, sizeof(int64_t)*2*rec_cap);
+        rec_gb = realloc(rec_gb, sizeof(double)*6*rec_cap);
+    }
+    rec_pairs[2*rec_n] = (int64_t)c.p1; rec_pairs[2*rec_n+1] = (int64_t)c.p2;
+    double g[6] = {c.gb.x,c.gb.y,c.gb.z,c.gb.vx,c.gb.vy,c.gb.vz};
+    memcpy(rec_gb+6*rec_n, g, sizeof(g));
+    rec_n++;
+    return reb_collision_resolve_hardsphere(r, c);
+}
+
+/* Small sheet following the shearing-sheet workload recipe (Omega=1, rho=400,
+ * G such that G*rho/Omega^2 = 1.546, radii R^-3 in [1,4]). */
+static struct reb_simulation* sheet(size_t N, double root_size, int nrx, int nry, unsigned int seed){
+    struct reb_simulation* r = reb_simulation_create();
+    r->rand_seed = seed;
+    r->OMEGA = 1.;
+    r->G = 1.546/400.;
+    r->softening = 0.1;
+    r->dt = 1e-3*2.*M_PI;
+    r->root_size = root_size; r->N_root_x = nrx; r->N_root_y = nry; r->N_root_z = 1;
+    r->N_ghost_x = 2; r->N_ghost_y = 2; r->N_ghost_z = 0;
+    r->boundary = REB_BOUNDARY_SHEAR;
+    r->gravity = REB_GRAVITY_TREE;
+    r->opening_angle2 = 0.25;
+    reb_simulation_set_integrator(r, "sei");
+    double bx = root_size*nrx, by = root_size*nry;
+    for (size_t i=0; i<N; i++){
+        struct reb_particle p = {0};
+        p.x = reb_random_uniform(r, -bx/2., bx/2.);
+        p.y = reb_random_uniform(r, -by/2., by/2.);
+        p.z = reb_random_normal(r, 1.);
+        double R = reb_random_powerlaw(r, 1., 4., -3.);
+        p.vx = reb_random_normal(r, 3.2*3.2);
+        p.vy = -1.5*p.x*r->OMEGA + reb_random_normal(r, 3.2*3.2);
+        p.vz = reb_random_normal(r, 3.2*3.2)*0.5;
+        p.r = R;
+        p.m = 400.*4./3.*M_PI*R*R*R;
+        reb_simulation_add(r, p);
+    }
+    r->coefficient_of_restitution = cor_half;
+    return r;
+}
+
+/* pre-order dump of the reference tree: center, w, m, com, pt */
+static void dump_cell(struct reb_treecell* c, double** buf, size_t* n, size_t* cap){
+    if (!c) return;
+    if (*n == *cap){ *cap = *cap ? 2**cap : 4096; *buf = realloc(*buf, sizeof(double)*9**cap); }
+    double row[9] = {c->x, c->y, c->z, c->w, c->m, c->mx, c->my, c->mz, (double)c->pt};
+    memcpy(*buf + 9**n, row, sizeof(row));
+    (*n)++;
+    for (int o=0; o<8; o++) dump_cell(c->oct[o], buf, n, cap);
+}
+
+static void gen_sheet(void){
+    struct reb_simulation* r = sheet(2000, 50., 2, 2, 99u);
+    dump_state("sheet2000_init.npy", r);
+    int64_t seedinit = r->rand_seed;
+    wi("sheet2000_seed_init.npy", &seedinit, 1, 0);
+    /* tree + tree gravity on the initial state (boundary wrap is a no-op here) */
+    reb_tree_construct(r);
+    reb_tree_calculate_gravity_data(r);
+    double* buf = NULL; size_t n = 0, cap = 0;
+    size_t Nroot = r->N_root_x*r->N_root_y*r->N_root_z;
+    int64_t* rootstart = calloc(Nroot+1, sizeof(int64_t));
+    for (size_t i=0; i<Nroot; i++){ rootstart[i] = (int64_t)n; dump_cell(r->tree_root[i], &buf, &n, &cap); }
+    rootstart[Nroot] = (int64_t)n;
+    reb_tree_delete(r);
+    wd("sheet2000_tree.npy", buf, n, 9);
+    wi("sheet2000_tree_rootstart.npy", rootstart, Nroot+1, 0);
+    free(buf); free(rootstart);
+    reb_simulation_update_acceleration(r);
+    double* a = acc_of(r); wd("sheet2000_acc_tree.npy", a, r->N, 3); free(a);
+    /* opening_angle2 = 0 opens every cell: tree gravity equals direct summation (order differs) */
+    /* five SEI steps with tree collisions + hardsphere, collisions recorded in resolve order */
+    r->collision = REB_COLLISION_TREE;
+    r->collision_resolve = rec_resolve;
+    int64_t counts[5];
+    for (int s=0; s<5; s++){
+        size_t before = rec_n;
+        reb_simulation_steps(r, 1);
+        counts[s] = (int64_t)(rec_n - before);
+        char name[128];
+        snprintf(name, sizeof(name), "sheet2000_step%d_state.npy", s+1);
+        dump_state(name, r);
+    }
+    wi("sheet2000_collision_pairs.npy", rec_pairs, rec_n, 2);
+    wd("sheet2000_collision_gb.npy", rec_gb, rec_n, 6);
+    wi("sheet2000_collision_counts.npy", counts, 5, 0);
+    int64_t seedafter = r->rand_seed;
+    wi("sheet2000_seed_after.npy", &seedafter, 1, 0);
+    reb_simulation_free(r);
+
+    /* DIRECT search on the same initial state, one step */
+    rec_n = 0;
+    r = sheet(2000, 50., 2, 2, 99u);
+    r->collision = REB_COLLISION_DIRECT;
+    r->collision_resolve = rec_resolve;
+    reb_simulation_steps(r, 1);
+    wi("sheet2000_direct_pairs.npy", rec_pairs, rec_n, 2);
+    wd("sheet2000_direct_gb.npy", rec_gb, rec_n, 6);
+    dump_state("sheet2000_direct_step1_state.npy", r);
+    reb_simulation_free(r);
+}
+
+int main(int argc, char** argv){
+    if (argc < 3){ fprintf(stderr, "usage: gen_golden OUTDIR serial|omp\n"); return 1; }
+    outdir = argv[1];
+    if (strcmp(argv[2], "omp") == 0){
+        gen_plummer_omp();
+        gen_ghostbox_basic("omp");
+        return 0;
+    }
+    gen_rng();
+    gen_plummer();
+    gen_ghostbox_basic("serial");
+    gen_kepler();
+    gen_whfast();
+    gen_sheet();
+    return 0;
+}

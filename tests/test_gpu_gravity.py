@@ -1,0 +1,168 @@
+"""GPU parity of direct gravity (gravity.c:167-531) and the energy (tools.c:108-162).
+
+EXACT is bit-identical to the reference; FP64 within 1e-14 of the summed
+magnitudes sum_j |a_ij| per body; SPLIT within 1e-6 of it (FP32 r^-3)."""
+import numpy as np
+import pytest
+
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+TOL_FP64 = 1e-14
+TOL_SPLIT = 1e-6
+
+
+def make_sim(amd, state, soft, precision, G=1.0):
+    sim = amd.Sim()
+    sim.s.G = G
+    sim.s.softening = soft
+    sim.set_state(state[:, :11] if state.shape[1] >= 11 else state)
+    sim.precision = precision
+    return sim
+
+
+def abs_sum(state, soft, rows=None):
+    """sum_j |a_ij| for the target rows (all bodies as sources): the
+    conditioning of each body's sum."""
+    x = state[:, :3]
+    m = state[:, 9]
+    rows = np.arange(x.shape[0]) if rows is None else np.asarray(rows)
+    out = np.zeros(len(rows))
+    for b in range(0, len(rows), 256):
+        t = rows[b:b + 256]
+        d = x[t, None, :] - x[None, :, :]
+        dist2 = (d * d).sum(-1)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            w = m[None, :] * np.sqrt(dist2) / (dist2 + soft * soft) ** 1.5
+        w[~np.isfinite(w)] = 0
+        out[b:b + 256] = w.sum(1)
+    return out
+
+
+@pytest.mark.parametrize("which,grav", [("basic", "basic"), ("comp", "compensated")])
+def test_exact_bitwise_plummer1024(gpu, which, grav):
+    st = golden("plummer1024_seed1_state.npy")
+    sim = make_sim(gpu, st, 0.01, "exact")
+    sim.s.gravity = gpu.GRAVITY[grav]
+    acc = sim.update_acceleration()
+    assert np.array_equal(acc, golden("plummer1024_seed1_acc_%s.npy" % which))
+
+
+@pytest.mark.parametrize("which,grav", [("basic", "basic"), ("comp", "compensated")])
+def test_exact_bitwise_testparticles(gpu, which, grav):
+    st = golden("plummer1024_seed1_state.npy")
+    sim = make_sim(gpu, st, 0.01, "exact")
+    sim.s.gravity = gpu.GRAVITY[grav]
+    sim.s.N_active = 768
+    sim.s.testparticle_type = 1
+    acc = sim.update_acceleration()
+    assert np.array_equal(acc, golden("plummer1024_seed1_acc_%s_tp1.npy" % which))
+
+
+def test_exact_bitwise_ghostboxes(gpu):
+    st = golden("periodic256_state.npy")
+    sim = make_sim(gpu, st, 0.05, "exact")
+    sim.s.boundary = gpu.BOUNDARY["periodic"]
+    sim.s.root_size = 10.0
+    sim.s.N_ghost_x = 1
+    sim.s.N_ghost_y = 1
+    acc = sim.update_acceleration()
+    assert np.array_equal(acc, golden("periodic256_acc_basic_omp.npy"))
+
+
+@pytest.mark.parametrize("grav,prec,tol", [("basic", "fp64", TOL_FP64), ("compensated", "fp64", TOL_FP64),
+                                           ("basic", "split", TOL_SPLIT)])
+def test_fast_modes_plummer4096(gpu, grav, prec, tol):
+    st = golden("plummer4096_seed2_state.npy")
+    ref = golden("plummer4096_seed2_acc_basic.npy")
+    sim = make_sim(gpu, st, 0.005, prec)
+    sim.s.gravity = gpu.GRAVITY[grav]
+    acc = sim.update_acceleration()
+    err = np.sqrt(((acc - ref) ** 2).sum(1)) / abs_sum(st, 0.005)
+    print("%s/%s max normalised error %.3e, median relative %.3e" % (
+        grav, prec, err.max(), np.median(np.sqrt(((acc - ref) ** 2).sum(1)) / np.sqrt((ref ** 2).sum(1)))))
+    assert err.max() <= tol
+
+
+@pytest.mark.parametrize("prec", ["fp64", "split"])
+def test_fast_modes_testparticles_and_ignore(gpu, orc, prec):
+    st = golden("plummer1024_seed1_state.npy")
+    tol = TOL_FP64 if prec == "fp64" else TOL_SPLIT
+    for N_active, tp, ign in ((768, 1, 0), (768, 0, 0), (1024, 0, 1), (1024, 0, 2), (500, 1, 2)):
+        sim = make_sim(gpu, st, 0.01, prec)
+        sim.s.N_active = N_active
+        sim.s.testparticle_type = tp
+        sim.s.gravity_ignore_terms = ign
+        acc = sim.update_acceleration()
+        ref = orc.gravity_basic(st, soft=0.01, N_active=N_active, tp_type=tp, ignore=ign)
+        scale = abs_sum(st, 0.01) + 1e-300
+        err = np.sqrt(((acc - ref) ** 2).sum(1)) / scale
+        assert err.max() <= tol, (N_active, tp, ign, err.max())
+        if ign == 2:
+            assert np.all(acc[0] == 0)
+
+
+def test_zero_softening_self_term(gpu, orc):
+    # softening 0: the self pair must be skipped, not evaluated (0 * inf)
+    rng = np.random.default_rng(0)
+    st = np.zeros((700, 11))
+    st[:, :3] = rng.normal(size=(700, 3))
+    st[:, 9] = 1.0 / 700
+    for prec in ("exact", "fp64", "split"):
+        sim = make_sim(gpu, st, 0.0, prec)
+        acc = sim.update_acceleration()
+        assert np.all(np.isfinite(acc)), prec
+    ref = orc.gravity_basic(st, soft=0.0)
+    sim = make_sim(gpu, st, 0.0, "exact")
+    assert np.array_equal(sim.update_acceleration(), ref)
+
+
+def test_energy_exact_bitwise(gpu):
+    st = golden("plummer1024_seed1_state.npy")
+    sim = make_sim(gpu, st, 0.01, "exact")
+    assert sim.energy() == golden("plummer1024_seed1_lf_energy.npy")[0]
+
+
+def test_energy_parallel(gpu, orc):
+    st = golden("plummer4096_seed2_state.npy")
+    sim = make_sim(gpu, st, 0.005, "fp64")
+    e = sim.energy()
+    ref = orc.energy(st)
+    assert abs(e - ref) <= 1e-13 * abs(ref)
+
+
+def test_raw_entry_point(gpu, orc):
+    """reb_amd_gravity_direct on device arrays owned by the caller (torch tensors)."""
+    torch = pytest.importorskip("torch")
+    st = golden("plummer1024_seed1_state.npy")
+    dev = torch.device("cuda:0")
+    cols = [torch.tensor(np.ascontiguousarray(st[:, k]), device=dev) for k in (0, 1, 2, 9)]
+    acc = [torch.zeros(st.shape[0], dtype=torch.float64, device=dev) for _ in range(3)]
+    L = gpu.lib()
+    for prec, comp, ref in ((1, 0, "basic"), (1, 1, "comp"), (0, 0, "basic")):
+        ws_bytes = L.reb_amd_gravity_workspace_bytes(st.shape[0], prec)
+        ws = torch.empty(ws_bytes // 8 + 1, dtype=torch.float64, device=dev)
+        rc = L.reb_amd_gravity_direct(*(c.data_ptr() for c in cols), *(a.data_ptr() for a in acc),
+                                      st.shape[0], st.shape[0], 0, 0, 1.0, 0.01, prec, comp, ws.data_ptr(),
+                                      torch.cuda.current_stream().cuda_stream)
+        assert rc == 0
+        torch.cuda.synchronize()
+        got = np.stack([a.cpu().numpy() for a in acc], axis=1)
+        want = golden("plummer1024_seed1_acc_%s.npy" % ref)
+        if prec == 1:
+            assert np.array_equal(got, want)
+        else:
+            assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
+
+
+@pytest.mark.slow
+def test_fp64_large_n_sampled_rows(gpu, orc):
+    """N=16384 Plummer: the fast mode against the oracle's full sum."""
+    s, _ = orc.plummer(16384, 3)
+    sim = make_sim(gpu, s, 0.005, "fp64")
+    acc = sim.update_acceleration()
+    ref = orc.gravity_basic(s, soft=0.005)
+    rows = np.random.default_rng(1).choice(16384, 512, replace=False)
+    err = np.sqrt(((acc[rows] - ref[rows]) ** 2).sum(1)) / abs_sum(s, 0.005, rows)
+    assert err.max() <= TOL_FP64

@@ -1,0 +1,162 @@
+"""GPU parity of the integrators: leapfrog (integrator_leapfrog.c), WHFast
+democratic heliocentric (integrator_whfast.c) and its Kepler solver, SEI
+(integrator_sei.c) with the shear boundary (boundary.c)."""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+
+def sim_from(amd, state, **fields):
+    sim = amd.Sim()
+    for k, v in fields.items():
+        setattr(sim.s, k, v)
+    sim.set_state(state)
+    return sim
+
+
+# ------------------------------------------------------------------ leapfrog
+def test_leapfrog_energy_history_exact(gpu):
+    st = golden("plummer1024_seed1_state.npy")
+    sim = sim_from(gpu, st, softening=0.01, dt=1e-3)
+    sim.set_integrator("leapfrog")
+    sim.precision = "exact"
+    E = [sim.energy()]
+    for _ in range(100):
+        sim.steps(10)
+        E.append(sim.energy())
+    assert np.array_equal(np.array(E), golden("plummer1024_seed1_lf_energy.npy"))
+    fin = golden("plummer1024_seed1_lf_final.npy")
+    assert np.array_equal(sim.state()[:, :9], fin[:, :9])
+    assert sim.s.t == pytest.approx(1.0, abs=1e-12)
+
+
+def test_leapfrog_energy_history_fp64(gpu):
+    st = golden("plummer1024_seed1_state.npy")
+    ref = golden("plummer1024_seed1_lf_energy.npy")
+    sim = sim_from(gpu, st, softening=0.01, dt=1e-3)
+    sim.set_integrator("leapfrog")
+    E = [sim.energy()]
+    for _ in range(100):
+        sim.steps(10)
+        E.append(sim.energy())
+    E = np.array(E)
+    # same energy-error history to 1e-10 of |E| over the whole run
+    assert np.abs(E - ref).max() <= 1e-10 * abs(ref[0])
+
+
+@pytest.mark.parametrize("order", [4, 6, 8])
+def test_leapfrog_higher_orders(gpu, orc, order):
+    # time reversibility: forward then backward returns to the start
+    st = golden("plummer1024_seed1_state.npy")[:128].copy()
+    sim = sim_from(gpu, st, softening=0.05, dt=1e-3)
+    lf = sim.set_integrator("leapfrog")
+    lf.order = order
+    sim.precision = "exact"
+    sim.steps(20)
+    sim.s.dt = -1e-3
+    sim.steps(20)
+    assert np.abs(sim.state()[:, :6] - st[:, :6]).max() < 1e-12
+
+
+# ------------------------------------------------------------------ Kepler
+def test_kepler_vectors_bitwise(gpu):
+    inp = golden("kepler_in.npy")
+    out = golden("kepler_out.npy")
+    s6 = np.ascontiguousarray(inp[:, :6]).copy()
+    mu = np.ascontiguousarray(inp[:, 6])
+    dt = np.ascontiguousarray(inp[:, 7])
+    P = ctypes.POINTER(ctypes.c_double)
+    rc = gpu.lib().reb_amd_kepler_solver_batch(s6.ctypes.data_as(P), mu.ctypes.data_as(P), dt.ctypes.data_as(P),
+                                               len(mu))
+    assert rc == 0
+    bad = np.where(~np.all(s6 == out, axis=1))[0]
+    assert len(bad) == 0, bad[:10]
+
+
+def test_kepler_public_helper(gpu):
+    p = gpu.Particle(x=1.0, vy=1.0)
+    gpu.lib().reb_integrator_whfast_kepler_solver(ctypes.byref(p), 1.0, 0.5 * math.pi, None)
+    assert abs(p.x) < 1e-14 and abs(p.y - 1.0) < 1e-14 and abs(p.vx + 1.0) < 1e-14
+
+
+# ------------------------------------------------------------------ WHFast DH
+def _systems():
+    init = golden("whfast_dh32_init.npy")
+    meta = golden("whfast_dh32_meta.npy")
+    dts = golden("whfast_dh32_dt.npy")
+    return [(init[s, : meta[s, 1]], dts[s]) for s in range(init.shape[0])]
+
+
+@pytest.mark.parametrize("safe", [0, 1])
+def test_whfast_dh_bitwise(gpu, safe):
+    fin = golden("whfast_dh32_safe%d_final.npy" % safe)
+    en = golden("whfast_dh32_safe%d_energy.npy" % safe)
+    for k, (s0, dt) in enumerate(_systems()):
+        sim = sim_from(gpu, s0, dt=dt)
+        wh = sim.set_integrator("whfast")
+        wh.coordinates = gpu.WHFAST_COORDINATES["democraticheliocentric"]
+        wh.safe_mode = safe
+        sim.precision = "exact"
+        E = [sim.energy()]
+        for _ in range(10):
+            sim.steps(100)
+            E.append(sim.energy())
+        n = s0.shape[0]
+        assert np.array_equal(sim.state()[:, :6], fin[k, :n, :6]), k
+        assert np.array_equal(np.array(E), en[k]), k
+
+
+def test_whfast_rejects_unsupported(gpu):
+    s0, dt = _systems()[0]
+    sim = sim_from(gpu, s0, dt=dt)
+    sim.set_integrator("whfast")      # Jacobi by default
+    with pytest.raises(gpu.RebError, match="democratic heliocentric"):
+        sim.steps(1)
+
+
+# ------------------------------------------------------------------ SEI
+def test_sei_epicycle_closed_form(gpu):
+    # no forces: H0 is exact, bodies return after one epicycle period 2 pi / Omega
+    rng = np.random.default_rng(5)
+    st = np.zeros((300, 11))
+    st[:, 0] = rng.uniform(-5, 5, 300)
+    st[:, 1] = rng.uniform(-5, 5, 300)
+    st[:, 2] = rng.uniform(-1, 1, 300)
+    st[:, 3:6] = rng.normal(scale=0.1, size=(300, 3))
+    st[:, 4] += -1.5 * st[:, 0]
+    st[:, 9] = 1.0
+    sim = sim_from(gpu, st, OMEGA=1.0, dt=2 * math.pi / 200)
+    sim.set_integrator("sei")
+    sim.s.gravity = gpu.GRAVITY["none"]
+    sim.steps(200)
+    got = sim.state()
+    # guiding-centre drift: y shifts by -3/2 Omega x_gc per unit time (exact solution)
+    xgc = 4 * st[:, 0] + 2 * st[:, 4]
+    expect_y = st[:, 1] - 1.5 * xgc * 2 * math.pi
+    assert np.abs(got[:, 0] - st[:, 0]).max() < 1e-10
+    assert np.abs(got[:, 1] - expect_y).max() < 1e-9
+    assert np.abs(got[:, 2] - st[:, 2]).max() < 1e-10
+
+
+def test_sei_operator_bitwise_vs_oracle(gpu, orc):
+    rng = np.random.default_rng(6)
+    st = np.zeros((500, 11))
+    st[:, :3] = rng.uniform(-3, 3, (500, 3))
+    st[:, 3:6] = rng.normal(size=(500, 3))
+    st[:, 9] = 1e-3
+    sim = sim_from(gpu, st, OMEGA=1.0, dt=1e-3 * 2 * math.pi, softening=0.1)
+    sim.set_integrator("sei")
+    sim.precision = "exact"
+    sim.steps(1)
+    ref = st.copy()
+    dt = 1e-3 * 2 * math.pi
+    orc.lib().orc_sei_operator(ref, 500, 1.0, 1.0, dt, 0)
+    ref[:, 6:9] = orc.gravity_basic(ref, soft=0.1)
+    orc.lib().orc_sei_operator(ref, 500, 1.0, 1.0, dt, 1)
+    assert np.array_equal(sim.state()[:, :6], ref[:, :6])

@@ -1,0 +1,139 @@
+"""GPU parity of the octree (tree.c), tree gravity (gravity.c:47-106) and the
+collision searches with the shearing-sheet ghost boxes (collision.c), on the
+small sheet written by the compiled reference (2000 bodies, 2x2 root boxes,
+SEI, hard spheres with restitution 0.5)."""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+
+def sheet(amd, collision="tree"):
+    st = golden("sheet2000_init.npy")
+    sim = amd.Sim()
+    s = sim.s
+    s.rand_seed = 99
+    s.OMEGA = 1.0
+    s.G = 1.546 / 400.0
+    s.softening = 0.1
+    s.dt = 1e-3 * 2.0 * math.pi
+    s.root_size = 50.0
+    s.N_root_x = 2
+    s.N_root_y = 2
+    s.N_root_z = 1
+    s.N_ghost_x = 2
+    s.N_ghost_y = 2
+    s.N_ghost_z = 0
+    s.boundary = amd.BOUNDARY["shear"]
+    s.gravity = amd.GRAVITY["tree"]
+    s.opening_angle2 = 0.25
+    sim.set_integrator("sei")
+    sim.set_state(st)
+    # the reference drew the bodies from r->rand_seed: continue its stream from there
+    s.rand_seed = int(golden("sheet2000_seed_init.npy")[0])
+    return sim, st
+
+
+def test_tree_structure_bitwise(gpu):
+    sim, st = sheet(gpu)
+    ref = golden("sheet2000_tree.npy")
+    out = np.zeros((ref.shape[0] + 16, 9))
+    n = gpu.lib().reb_amd_tree_dump(sim.ptr, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), out.shape[0])
+    sim.check()
+    assert n == ref.shape[0]
+    assert np.array_equal(out[:n], ref)
+
+
+def test_tree_gravity_bitwise(gpu):
+    sim, st = sheet(gpu)
+    acc = sim.update_acceleration()
+    assert np.array_equal(acc, golden("sheet2000_acc_tree.npy"))
+
+
+def test_tree_gravity_opening_zero_is_direct(gpu, orc):
+    rng = np.random.default_rng(2)
+    st = np.zeros((3000, 11))
+    st[:, :3] = rng.uniform(-20, 20, (3000, 3))
+    st[:, 9] = rng.uniform(0.1, 1.0, 3000)
+    sim = gpu.Sim()
+    sim.s.root_size = 40.0
+    sim.s.gravity = gpu.GRAVITY["tree"]
+    sim.s.opening_angle2 = 0.0
+    sim.s.softening = 0.1
+    sim.set_state(st)
+    acc = sim.update_acceleration()
+    ref = orc.gravity_basic(st, soft=0.1)
+    assert np.abs(acc - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+def test_tree_errors(gpu):
+    sim = gpu.Sim()
+    sim.s.root_size = 10.0
+    sim.s.gravity = gpu.GRAVITY["tree"]
+    sim.add(x=1.0, m=1.0)
+    sim.add(x=1.0, m=1.0)
+    with pytest.raises(gpu.RebError, match="same coordinates"):
+        sim.update_acceleration()
+    sim2 = gpu.Sim()
+    sim2.s.gravity = gpu.GRAVITY["tree"]
+    sim2.add(x=1.0, m=1.0)
+    with pytest.raises(gpu.RebError, match="root_size"):
+        sim2.update_acceleration()
+
+
+def _run_steps(gpu, collision, record):
+    sim, st = sheet(gpu)
+    sim.s.collision = gpu.COLLISION[collision]
+    rec = []
+    cor = gpu._abi.COR(lambda r, v: 0.5)
+    if record:
+        L = gpu.lib()
+
+        def resolve(r, c):
+            rec.append((c.p1, c.p2, c.gb.x, c.gb.y, c.gb.z, c.gb.vx, c.gb.vy, c.gb.vz))
+            return L.reb_collision_resolve_hardsphere(r, c)
+
+        cb = gpu._abi.RESOLVE(resolve)
+        sim._keep.append(cb)
+        sim.s.collision_resolve = cb
+    else:
+        # the built-in resolver: only the colliding bodies travel to the host
+        hs = ctypes.cast(gpu.lib().reb_collision_resolve_hardsphere, ctypes.c_void_p).value
+        sim.s.collision_resolve = ctypes.cast(hs, gpu._abi.RESOLVE)
+    sim._keep.append(cor)
+    sim.s.coefficient_of_restitution = cor
+    return sim, rec
+
+
+@pytest.mark.parametrize("record", [True, False])
+def test_sheet_five_steps_bitwise(gpu, record):
+    sim, rec = _run_steps(gpu, "tree", record)
+    counts = golden("sheet2000_collision_counts.npy")
+    for s in range(5):
+        sim.steps(1)
+        ref = golden("sheet2000_step%d_state.npy" % (s + 1))
+        assert np.array_equal(sim.state()[:, :6], ref[:, :6]), s
+    assert sim.s.collisions_log_n > 0
+    if record:
+        pairs = golden("sheet2000_collision_pairs.npy")
+        gb = golden("sheet2000_collision_gb.npy")
+        got = np.array(rec)
+        assert got.shape[0] == counts.sum()
+        assert np.array_equal(got[:, :2].astype(np.int64), pairs)
+        assert np.array_equal(got[:, 2:], gb)
+    assert sim.s.rand_seed == int(golden("sheet2000_seed_after.npy")[0])
+
+
+def test_sheet_direct_search_bitwise(gpu):
+    sim, rec = _run_steps(gpu, "direct", True)
+    sim.steps(1)
+    pairs = golden("sheet2000_direct_pairs.npy")
+    got = np.array(rec)
+    assert np.array_equal(got[:, :2].astype(np.int64), pairs)
+    assert np.array_equal(got[:, 2:], golden("sheet2000_direct_gb.npy"))
+    assert np.array_equal(sim.state()[:, :6], golden("sheet2000_direct_step1_state.npy")[:, :6])

@@ -1,0 +1,250 @@
+"""Pins the CPU oracle (oracle/oracle.c) before it is trusted as the checker.
+
+Every function is compared with golden vectors written by the compiled
+reference (tests/golden/gen_golden.c) and, where the mathematics fixes the
+answer, with closed forms chosen so that a dropped term, a wrong sign or a
+transposed index fails.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import golden
+
+
+# ------------------------------------------------------------------ RNG and generators
+def test_rand_r_streams(orc):
+    ref = golden("rand_r_streams.npy")
+    for row, seed in zip(ref, (1, 42, 12345)):
+        sd = orc.Seed(seed)
+        got = [orc.lib().orc_rand_r(sd.ref) for _ in range(64)]
+        assert np.array_equal(np.array(got), row)
+
+
+def test_random_transforms(orc):
+    ref = golden("random_streams_seed7.npy")
+    sd = orc.Seed(7)
+    L = orc.lib()
+    u = [L.orc_random_uniform(sd.ref, -1.0, 3.0) for _ in range(256)]
+    n = [L.orc_random_normal(sd.ref, 2.0) for _ in range(256)]
+    p = [L.orc_random_powerlaw(sd.ref, 1.0, 4.0, -3.0) for _ in range(256)]
+    r = [L.orc_random_rayleigh(sd.ref, 0.02) for _ in range(256)]
+    assert np.array_equal(np.array([u, n, p, r]), ref)
+
+
+def test_plummer_generator_bitwise(orc):
+    s, _ = orc.plummer(1024, 1)
+    ref = golden("plummer1024_seed1_state.npy")
+    assert np.array_equal(s[:, [0, 1, 2, 3, 4, 5, 9]], ref[:, [0, 1, 2, 3, 4, 5, 9]])
+
+
+def test_plummer_statistics(orc):
+    # Plummer sphere with M=R=1: half-mass radius 1.305, virial ratio 2T/|W| = 1 (closed forms)
+    s, _ = orc.plummer(4096, 5)
+    r = np.sqrt((s[:, :3] ** 2).sum(1))
+    assert abs(np.median(r) - 1.3048) < 0.06
+    T = 0.5 * (s[:, 9] * (s[:, 3:6] ** 2).sum(1)).sum()
+    W = orc.energy(s) - T
+    assert abs(2 * T / abs(W) - 1.0) < 0.08
+
+
+# ------------------------------------------------------------------ direct gravity
+@pytest.mark.parametrize("which", ["basic", "comp"])
+def test_direct_gravity_bitwise(orc, which):
+    s = golden("plummer1024_seed1_state.npy")
+    f = orc.gravity_basic if which == "basic" else orc.gravity_compensated
+    acc = f(s, G=1.0, soft=0.01)
+    assert np.array_equal(acc, golden("plummer1024_seed1_acc_%s.npy" % which))
+
+
+@pytest.mark.parametrize("which", ["basic", "comp"])
+def test_direct_gravity_testparticles_bitwise(orc, which):
+    s = golden("plummer1024_seed1_state.npy")
+    f = orc.gravity_basic if which == "basic" else orc.gravity_compensated
+    acc = f(s, G=1.0, soft=0.01, N_active=768, tp_type=1)
+    assert np.array_equal(acc, golden("plummer1024_seed1_acc_%s_tp1.npy" % which))
+
+
+def test_direct_gravity_ghostboxes_openmp_order(orc):
+    s = golden("periodic256_state.npy")
+    gbs = np.array([[10.0 * i, 10.0 * j, 0.0] for i in (-1, 0, 1) for j in (-1, 0, 1)])
+    acc = orc.gravity_basic(s, G=1.0, soft=0.05, gbs=gbs)
+    assert np.array_equal(acc, golden("periodic256_acc_basic_omp.npy"))
+    # the serial reference sums ghost-box images in another order: same value to rounding
+    ser = golden("periodic256_acc_basic_serial.npy")
+    assert np.abs(acc - ser).max() <= 1e-13 * np.abs(ser).max()
+
+
+def test_direct_gravity_closed_forms(orc):
+    # two bodies: |a_1| = G m_2 / (d^2 + eps^2)^(3/2) * d, pointing at body 2
+    s = np.zeros((2, 11))
+    s[0, 0], s[1, 0] = -0.5, 1.5
+    s[0, 9], s[1, 9] = 3.0, 5.0
+    acc = orc.gravity_basic(s, G=2.0, soft=0.5)
+    d = 2.0
+    mag = 2.0 * d / (d * d + 0.25) ** 1.5
+    assert math.isclose(acc[0, 0], 5.0 * mag, rel_tol=1e-15)
+    assert math.isclose(acc[1, 0], -3.0 * mag, rel_tol=1e-15)
+    # Newton's third law on a random cloud: sum m a = 0 to rounding
+    rng = np.random.default_rng(3)
+    s = np.zeros((64, 11))
+    s[:, :3] = rng.normal(size=(64, 3))
+    s[:, 9] = rng.uniform(0.5, 1.5, 64)
+    acc = orc.gravity_basic(s, soft=0.1)
+    mom = (s[:, 9:10] * acc).sum(0)
+    assert np.abs(mom).max() < 1e-13 * np.abs(s[:, 9:10] * acc).sum()
+    # compensated == basic up to rounding
+    acc2 = orc.gravity_compensated(s, soft=0.1)
+    assert np.abs(acc2 - acc).max() < 1e-13 * np.abs(acc).max()
+    # brute force in extended precision (long double through numpy)
+    x = s[:, :3].astype(np.longdouble)
+    m = s[:, 9].astype(np.longdouble)
+    d = x[:, None, :] - x[None, :, :]
+    r2 = (d * d).sum(-1) + np.longdouble(0.01)
+    w = m[None, :] / r2 ** np.longdouble(1.5)
+    np.fill_diagonal(w, 0)
+    exact = -(w[:, :, None] * d).sum(1)
+    assert np.abs(acc - exact.astype(np.float64)).max() < 1e-14 * float(np.abs(exact).max())
+
+
+def test_ignore_terms(orc):
+    rng = np.random.default_rng(4)
+    s = np.zeros((6, 11))
+    s[:, :3] = rng.normal(size=(6, 3))
+    s[:, 9] = 1.0
+    full = orc.gravity_basic(s, soft=0.0)
+    inv0 = orc.gravity_basic(s, soft=0.0, ignore=2)
+    assert np.all(inv0[0] == 0)
+    # body k>0 without body 0's pull
+    only0 = orc.gravity_basic(s[[0, 1]], soft=0.0)
+    assert np.allclose(inv0[1], full[1] - only0[1], rtol=1e-12, atol=1e-14)
+
+
+def test_energy_history_bitwise(orc):
+    s = golden("plummer1024_seed1_state.npy")
+    e0 = orc.energy(s)
+    ref = golden("plummer1024_seed1_lf_energy.npy")
+    assert e0 == ref[0]
+
+
+@pytest.mark.slow
+def test_leapfrog_history_bitwise(orc):
+    s = golden("plummer1024_seed1_state.npy")
+    fin, t, e = orc.leapfrog(s, 1.0, 0.01, 1e-3, 1000, every=10)
+    assert np.array_equal(e, golden("plummer1024_seed1_lf_energy.npy"))
+    ref = golden("plummer1024_seed1_lf_final.npy")
+    assert np.array_equal(fin[:, :9], ref[:, :9])
+
+
+# ------------------------------------------------------------------ Kepler / WHFast
+def test_kepler_vectors_bitwise(orc):
+    inp = golden("kepler_in.npy")
+    out = golden("kepler_out.npy")
+    for k in range(inp.shape[0]):
+        st, _ = orc.kepler(inp[k, :6], inp[k, 6], inp[k, 7])
+        assert np.array_equal(st, out[k]), k
+
+
+def test_kepler_closed_form(orc):
+    # circular orbit: after a quarter period the position is rotated by 90 degrees
+    mu, a = 1.3, 2.0
+    v = math.sqrt(mu / a)
+    P = 2 * math.pi * math.sqrt(a ** 3 / mu)
+    st, _ = orc.kepler([a, 0, 0, 0, v, 0], mu, P / 4)
+    assert np.allclose(st, [0, a, 0, -v, 0, 0], atol=1e-12)
+    # eccentric orbit: energy and angular momentum conserved, back-and-forth returns
+    s0 = np.array([0.7, 0.2, 0.1, -0.3, 1.1, 0.05])
+    st, _ = orc.kepler(s0, 1.0, 3.7)
+    E = lambda q: 0.5 * (q[3:] ** 2).sum() - 1.0 / np.linalg.norm(q[:3])
+    L = lambda q: np.cross(q[:3], q[3:])
+    assert abs(E(st) - E(s0)) < 1e-13 and np.abs(L(st) - L(s0)).max() < 1e-13
+    back, _ = orc.kepler(st, 1.0, -3.7)
+    assert np.abs(back - s0).max() < 1e-12
+    # very hyperbolic limit: straight line x + v dt (test_whfast.py:32-42)
+    st, _ = orc.kepler([1.0, 0, 0, 0, 1e10, 0], 1.0, 1e-3)
+    assert abs(st[1] - 1e7) / 1e7 < 1e-12
+
+
+def _systems():
+    init = golden("whfast_dh32_init.npy")
+    meta = golden("whfast_dh32_meta.npy")
+    dts = golden("whfast_dh32_dt.npy")
+    return [(init[s, : meta[s, 1]], dts[s]) for s in range(init.shape[0])]
+
+
+@pytest.mark.parametrize("safe", [0, 1])
+def test_whfast_dh_bitwise(orc, safe):
+    fin = golden("whfast_dh32_safe%d_final.npy" % safe)
+    en = golden("whfast_dh32_safe%d_energy.npy" % safe)
+    for k, (s0, dt) in enumerate(_systems()[:12]):
+        s = s0.copy()
+        ph = None
+        sync = 1
+        E = [orc.energy(s)]
+        for c in range(10):
+            s, ph, sync = orc.whfast_dh(s, 1.0, dt, 100, safe, ph=ph, is_sync=sync, did_modify=1 if c == 0 else 0)
+            E.append(orc.energy(s))
+        n = s.shape[0]
+        assert np.array_equal(s[:, :6], fin[k, :n, :6]), k
+        assert np.array_equal(np.array(E), en[k]), k
+
+
+# ------------------------------------------------------------------ tree and collisions
+SHEET = dict(root_size=50.0, nr=(2, 2, 1))
+
+
+def test_tree_structure_bitwise(orc):
+    s = golden("sheet2000_init.npy")
+    cells, rs = orc.tree_dump(s, SHEET["root_size"], SHEET["nr"])
+    ref = golden("sheet2000_tree.npy")
+    assert cells.shape == ref.shape
+    assert np.array_equal(cells, ref)
+    assert np.array_equal(rs, golden("sheet2000_tree_rootstart.npy"))
+    # every particle in exactly one leaf; cell mass = sum of member masses
+    leaves = cells[cells[:, 8] >= 0, 8].astype(int)
+    assert np.array_equal(np.sort(leaves), np.arange(s.shape[0]))
+    roots_mass = cells[rs[:-1], 4].sum()
+    assert math.isclose(roots_mass, s[:, 9].sum(), rel_tol=1e-12)
+
+
+def _sheet_gbs(t, ng=2, six=False):
+    from oracle import lib
+    out = []
+    for i in range(-ng, ng + 1):
+        for j in range(-ng, ng + 1):
+            g = np.zeros(6)
+            lib().orc_ghostbox_shear(1.0, 100.0, 100.0, 50.0, t, i, j, 0, g)
+            out.append(g if six else g[:3])
+    return np.array(out)
+
+
+def test_tree_gravity_bitwise(orc):
+    s = golden("sheet2000_init.npy")
+    acc, inter = orc.tree_gravity(s, SHEET["root_size"], SHEET["nr"], 1.546 / 400.0, 0.1, 0.25, _sheet_gbs(0.0))
+    assert np.array_equal(acc, golden("sheet2000_acc_tree.npy"))
+    assert inter > 0
+
+
+def test_tree_gravity_reduces_to_direct(orc):
+    # opening_angle2 = 0 opens every cell: the walk visits every other particle
+    rng = np.random.default_rng(1)
+    s = np.zeros((300, 11))
+    s[:, :3] = rng.uniform(-10, 10, (300, 3))
+    s[:, 9] = rng.uniform(0.1, 1.0, 300)
+    acc_t, inter = orc.tree_gravity(s, 20.0, (1, 1, 1), 1.0, 0.1, 0.0, np.zeros((1, 3)))
+    acc_d = orc.gravity_basic(s, soft=0.1)
+    assert inter == 300 * 299
+    assert np.abs(acc_t - acc_d).max() < 1e-12 * np.abs(acc_d).max()
+
+
+def test_collision_direct_first_step(orc):
+    # the reference's first step starts from the initial state after one SEI step;
+    # compare the DIRECT search of the recorded pairs' set on the initial state instead:
+    s = golden("sheet2000_init.npy")
+    gbs = _sheet_gbs(0.0, ng=1, six=True)
+    pairs, _ = orc.collisions_direct(s, gbs)
+    pairs_t, _ = orc.collisions_tree(s, SHEET["root_size"], SHEET["nr"], gbs)
+    # tree search finds the same overlapping, approaching pairs as the direct search
+    assert set(map(tuple, pairs)) == set(map(tuple, pairs_t))
+    assert len(pairs) > 100
