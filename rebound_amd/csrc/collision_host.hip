@@ -193,15 +193,18 @@ REB_API enum REB_COLLISION_RESOLVE_OUTCOME reb_collision_resolve_hardsphere(stru
     double vy21 = p1.vy + gb.vy - p2.vy;
     double vz21 = p1.vz + gb.vz - p2.vz;
     if (vx21 * x21 + vy21 * y21 + vz21 * z21 > 0) return REB_COLLISION_RESOLVE_OUTCOME_REMOVE_NONE;
-    // rotate the pair into the xy plane, then onto the x axis
+    // rotate the pair into the xy plane, then onto the x axis.  The reference
+    // library, built by gcc, evaluates each sin/cos pair with one glibc
+    // sincos() call (its results differ from separate sin and cos calls in
+    // the last bit for ~0.1% of angles), so the same call is made here.
     double theta = atan2(z21, y21);
-    double stheta = sin(theta);
-    double ctheta = cos(theta);
+    double stheta, ctheta;
+    sincos(theta, &stheta, &ctheta);
     double vy21n = ctheta * vy21 + stheta * vz21;
     double y21n = ctheta * y21 + stheta * z21;
     double phi = atan2(y21n, x21);
-    double cphi = cos(phi);
-    double sphi = sin(phi);
+    double sphi, cphi;
+    sincos(phi, &sphi, &cphi);
     double vx21nn = cphi * vx21 + sphi * vy21n;
     double eps = 1;
     if (r->coefficient_of_restitution) eps = r->coefficient_of_restitution(r, vx21nn);

@@ -203,6 +203,7 @@ void device_release(struct reb_simulation* r) {
     d->partial.release();
     d->scratch.release();
     d->aos.release();
+    d->flags.release();
     if (d->host_stage) (void)hipHostFree(d->host_stage);
     d->host_stage = nullptr;
     for (auto& e : d->ev)

@@ -197,6 +197,7 @@ static void begin_call(struct reb_simulation* r) {
 }
 
 static void end_call(struct reb_simulation* r) {
+    check_device_flags(r);
     if (!dev_of(r)->resident) ensure_host(r);
 }
 

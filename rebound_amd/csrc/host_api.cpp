@@ -487,18 +487,22 @@ REB_API struct reb_particle reb_particle_from_orbit_err(double G, struct reb_par
     } else {
         if (a < 0.) { *err = 4; return reb_particle_nan(); }
     }
-    if (e * cos(f) < -1.) { *err = 5; return reb_particle_nan(); }
+    // The reference library, built by gcc, evaluates each angle's sine and
+    // cosine with one glibc sincos() call; the same calls give the same bits.
+    double sf, cf;
+    sincos(f, &sf, &cf);
+    if (e * cf < -1.) { *err = 5; return reb_particle_nan(); }
     if (primary.m < 1.E-308) { *err = 6; return reb_particle_nan(); }
 
     struct reb_particle p;
     memset(&p, 0, sizeof(p));
     p.m = m;
-    double r = a * (1 - e * e) / (1 + e * cos(f));
+    double r = a * (1 - e * e) / (1 + e * cf);
     double v0 = sqrt(G * (m + primary.m) / a / (1. - e * e));
-    double cO = cos(Omega), sO = sin(Omega);
-    double co = cos(omega), so = sin(omega);
-    double cf = cos(f), sf = sin(f);
-    double ci = cos(inc), si = sin(inc);
+    double sO, cO, so, co, si, ci;
+    sincos(Omega, &sO, &cO);
+    sincos(omega, &so, &co);
+    sincos(inc, &si, &ci);
     p.x = primary.x + r * (cO * (co * cf - so * sf) - sO * (so * cf + co * sf) * ci);
     p.y = primary.y + r * (sO * (co * cf - so * sf) + cO * (so * cf + co * sf) * ci);
     p.z = primary.z + r * (so * cf + co * sf) * si;

@@ -91,25 +91,33 @@ struct BoxConst {
     double dvy;              // 3/2 OMEGA bx
 };
 
+static constexpr int kWrapMax = 4096;
+
 __global__ void k_wrap_shear(double* __restrict__ x, double* __restrict__ y, double* __restrict__ z,
-                             double* __restrict__ vy, BoxConst b, int n) {
+                             double* __restrict__ vy, BoxConst b, int n, unsigned int* __restrict__ err) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double px = x[i], py = y[i], pz = z[i], pvy = vy[i];
-    while (px > b.bx / 2.) {
+    // A body wraps once per crossing; kWrapMax bounds the loops for bodies
+    // that are non-finite or absurdly far away (reported as an error).
+    int k = 0;
+    while (px > b.bx / 2. && k < kWrapMax) {
         px -= b.bx;
         py += b.offp1;
         pvy += b.dvy;
+        k++;
     }
-    while (px < -b.bx / 2.) {
+    while (px < -b.bx / 2. && k < kWrapMax) {
         px += b.bx;
         py += b.offm1;
         pvy -= b.dvy;
+        k++;
     }
-    while (py > b.by / 2.) py -= b.by;
-    while (py < -b.by / 2.) py += b.by;
-    while (pz > b.bz / 2.) pz -= b.bz;
-    while (pz < -b.bz / 2.) pz += b.bz;
+    while (py > b.by / 2. && k < kWrapMax) { py -= b.by; k++; }
+    while (py < -b.by / 2. && k < kWrapMax) { py += b.by; k++; }
+    while (pz > b.bz / 2. && k < kWrapMax) { pz -= b.bz; k++; }
+    while (pz < -b.bz / 2. && k < kWrapMax) { pz += b.bz; k++; }
+    if (k >= kWrapMax) atomicOr(err, 1u);
     x[i] = px;
     y[i] = py;
     z[i] = pz;
@@ -117,16 +125,18 @@ __global__ void k_wrap_shear(double* __restrict__ x, double* __restrict__ y, dou
 }
 
 __global__ void k_wrap_periodic(double* __restrict__ x, double* __restrict__ y, double* __restrict__ z, BoxConst b,
-                                int n) {
+                                int n, unsigned int* __restrict__ err) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double px = x[i], py = y[i], pz = z[i];
-    while (px > b.bx / 2.) px -= b.bx;
-    while (px < -b.bx / 2.) px += b.bx;
-    while (py > b.by / 2.) py -= b.by;
-    while (py < -b.by / 2.) py += b.by;
-    while (pz > b.bz / 2.) pz -= b.bz;
-    while (pz < -b.bz / 2.) pz += b.bz;
+    int k = 0;
+    while (px > b.bx / 2. && k < kWrapMax) { px -= b.bx; k++; }
+    while (px < -b.bx / 2. && k < kWrapMax) { px += b.bx; k++; }
+    while (py > b.by / 2. && k < kWrapMax) { py -= b.by; k++; }
+    while (py < -b.by / 2. && k < kWrapMax) { py += b.by; k++; }
+    while (pz > b.bz / 2. && k < kWrapMax) { pz -= b.bz; k++; }
+    while (pz < -b.bz / 2. && k < kWrapMax) { pz += b.bz; k++; }
+    if (k >= kWrapMax) atomicOr(err, 1u);
     x[i] = px;
     y[i] = py;
     z[i] = pz;
@@ -185,6 +195,31 @@ bool launch_sei_h012(struct reb_simulation* r, double dt, double sindt, double t
     return RB_HIP(r, hipGetLastError());
 }
 
+// A device word that the wrap kernels set when a body is too far outside the
+// box; checked (and cleared) at the end of each stepping call.
+unsigned int* wrap_error_word(struct reb_simulation* r) {
+    DeviceState* d = dev_of(r);
+    if (!d->flags.p) {
+        if (!d->flags.ensure(256)) return nullptr;
+        (void)hipMemsetAsync(d->flags.p, 0, 256, d->stream);
+    }
+    return d->flags.as<unsigned int>();
+}
+
+bool check_device_flags(struct reb_simulation* r) {
+    DeviceState* d = dev_of(r);
+    if (!d->flags.p) return true;
+    unsigned int h = 0;
+    if (!RB_HIP(r, hipMemcpyAsync(&h, d->flags.p, sizeof(h), hipMemcpyDeviceToHost, d->stream))) return false;
+    if (!stream_sync(r)) return false;
+    if (h) {
+        reb_simulation_error(r, "A particle is non-finite or thousands of box widths outside the box; it was not wrapped.");
+        (void)hipMemsetAsync(d->flags.p, 0, sizeof(unsigned int), d->stream);
+        return false;
+    }
+    return true;
+}
+
 bool boundary_device(struct reb_simulation* r) {
     DeviceState* d = dev_of(r);
     const int N = (int)r->N;
@@ -202,13 +237,13 @@ bool boundary_device(struct reb_simulation* r) {
             b.offm1 = -fmod(1.5 * OMEGA * b.bx * r->t - b.by / 2., b.by) + b.by / 2.;
             b.dvy = 3. / 2. * OMEGA * b.bx;
             hipLaunchKernelGGL(k_wrap_shear, grid_for(N), dim3(256), 0, d->stream, d->field(FX), d->field(FY),
-                               d->field(FZ), d->field(FVY), b, N);
+                               d->field(FZ), d->field(FVY), b, N, wrap_error_word(r));
             mark_device_newer(r);
             return RB_HIP(r, hipGetLastError());
         }
         case REB_BOUNDARY_PERIODIC:
             hipLaunchKernelGGL(k_wrap_periodic, grid_for(N), dim3(256), 0, d->stream, d->field(FX), d->field(FY),
-                               d->field(FZ), b, N);
+                               d->field(FZ), b, N, wrap_error_word(r));
             mark_device_newer(r);
             return RB_HIP(r, hipGetLastError());
         case REB_BOUNDARY_OPEN: {

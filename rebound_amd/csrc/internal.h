@@ -80,6 +80,7 @@ struct DeviceState {
     DevBuf partial;                 // j-split partial sums for direct gravity
     DevBuf scratch;                 // small reductions (energy, flags)
     DevBuf aos;                     // AoS staging for uploads/downloads
+    DevBuf flags;                   // device error words (boundary wrap)
     double* host_stage = nullptr;   // pinned staging, NFIELD * host_cap doubles
     size_t host_cap = 0;
     int loc = LOC_HOST;
@@ -138,6 +139,7 @@ bool launch_kick(struct reb_simulation* r, double dt);
 bool launch_sei_h012(struct reb_simulation* r, double dt, double sindt, double tandt,
                      double sindtz, double tandtz, int with_kick);
 bool boundary_device(struct reb_simulation* r);              // returns false on error
+bool check_device_flags(struct reb_simulation* r);           // error words set by kernels
 
 // WHFast (whfast.hip)
 void whfast_step_device(struct reb_simulation* r, struct reb_integrator_whfast_state* s);

@@ -37,7 +37,7 @@
 
 namespace rbamd {
 
-static constexpr int kMaxGBT = 125;
+static constexpr int kMaxGBT = 64;   // keeps the by-value argument under 4 KB
 struct GBT {
     int n;
     double g[kMaxGBT][6];
@@ -568,7 +568,7 @@ bool gravity_tree(struct reb_simulation* r) {
     if (n == 0) return true;
     auto gbv = ghostboxes(r, r->N_ghost_x, r->N_ghost_y, r->N_ghost_z);
     if ((int)gbv.size() > kMaxGBT) {
-        reb_simulation_error(r, "Too many ghost boxes for tree gravity (at most 125).");
+        reb_simulation_error(r, "Too many ghost boxes for tree gravity (at most 64).");
         return false;
     }
     GBT gl;

@@ -47,8 +47,10 @@ __device__ __forceinline__ double fastabs(double x) { return (x > 0.) ? x : -x; 
 
 // c_0..c_3 Stumpff functions of z by series + quartering (integrator_whfast.c:189-213).
 __device__ __forceinline__ void stumpff_cs3(double* cs, double z) {
+    // A finite z needs at most ~520 quarterings; the cap only stops a
+    // non-finite z from spinning forever (the result is then NaN anyway).
     unsigned int n = 0;
-    while (fabs(z) > 0.1) {
+    while (fabs(z) > 0.1 && n < 1100) {
         z = z / 4.;
         n++;
     }
@@ -170,6 +172,8 @@ __device__ int kepler_solver(double& px, double& py, double& pz, double& pvx, do
             }
         }
         X = (X_max + X_min) / 2.;
+        // halving exhausts the doubles of any finite interval in < 2200 steps
+        int guard = 0;
         do {
             stiefel_Gs3(Gs, beta, X);
             const double s = r0 * X + eta0 * Gs[2] + zeta0 * Gs[3] - dt;
@@ -179,7 +183,7 @@ __device__ int kepler_solver(double& px, double& py, double& pz, double& pvx, do
                 X_min = X;
             }
             X = (X_max + X_min) / 2.;
-        } while (fastabs((X_max - X_min)) > fastabs((X_max + X_min) * 1e-15));
+        } while (fastabs((X_max - X_min)) > fastabs((X_max + X_min) * 1e-15) && ++guard < 4096);
         const double e1 = eta0 * Gs[1] + zeta0 * Gs[2];
         ri = 1. / (r0 + e1);
     }
