@@ -8,9 +8,10 @@
 //          the reference's OpenMP loop (gravity.c:215-232, 308-343), so the
 //          result is bit-identical.
 //   FP64   r^-3 from v_rsq_f64 refined to full double accuracy, fused
-//          multiply-adds, and the source range split over blockIdx.y so that
-//          N=65536 fills every SIMD; the partial sums are added in a fixed
-//          order by a second kernel (deterministic).
+//          multiply-adds, each LDS tile's sum folded into the running sum
+//          with Kahan compensation, and the source range split over
+//          blockIdx.y so that N=65536 fills every SIMD; the partial sums are
+//          added in a fixed order by a second kernel (deterministic).
 //   SPLIT  displacement and r^2 in FP64, r^-3 * m_j in FP32 (v_rsq_f32),
 //          accumulation in FP64; each LDS tile's FP64 partial is folded into
 //          the running sum with Kahan compensation.
@@ -159,20 +160,29 @@ enum FastMode { FM_FP64 = 0, FM_SPLIT = 1, FM_COMP = 2 };
 // refined through e = 1 - r2*y^2 (|e| < 2^-23):
 //   r^-3 = y^3 (1 - e)^(-3/2) = y^3 (1 + 3/2 e + 15/8 e^2 + O(e^3)),
 // leaving a relative error of order 2^-69, far below FP64 rounding.
-__device__ __forceinline__ double rinv3_fp64(double r2) {
+// `c15` is 1.5 held in a register: a GFX9 VOP3 instruction takes at most one
+// scalar or literal operand, so fma(e, 1.875, 1.5) with both constants would
+// be rebuilt from two v_mov per pair.
+__device__ __forceinline__ double rinv3_fp64(double r2, double c15, double c1875) {
     const double yv = __builtin_amdgcn_rsq(r2);
     const double h = yv * yv;
     const double e = fma(-r2, h, 1.0);
     const double y3 = h * yv;
-    const double p = fma(e, 1.875, 1.5);
+    const double p = fma(e, c1875, c15);
     return fma(y3, e * p, y3);
+}
+
+// A loop-invariant constant kept in a VGPR (opaque to constant folding).
+__device__ __forceinline__ double vreg_const(double v) {
+    asm volatile("" : "+v"(v));
+    return v;
 }
 
 template <int MODE, bool CHECK>
 __device__ __forceinline__ void pair_tile(const double4* __restrict__ tile, const float* __restrict__ mf, int n,
                                           int jbase, int i, int ignore, double gx, double gy, double gz,
                                           double soft2, double& sx, double& sy, double& sz, double& cx, double& cy,
-                                          double& cz) {
+                                          double& cz, double c15, double c1875) {
 #pragma unroll 4
     for (int k = 0; k < n; k++) {
         const double4 s = tile[k];
@@ -186,7 +196,7 @@ __device__ __forceinline__ void pair_tile(const double4* __restrict__ tile, cons
             const float yf = __builtin_amdgcn_rsqf(rf);
             w = (double)(yf * yf * yf * mf[k]);
         } else {
-            w = s.w * rinv3_fp64(r2);
+            w = s.w * rinv3_fp64(r2, c15, c1875);
         }
         if (CHECK) {
             const int j = jbase + k;
@@ -204,8 +214,10 @@ __device__ __forceinline__ void pair_tile(const double4* __restrict__ tile, cons
     }
 }
 
+// Eight wavefronts per SIMD: at N=65536 the grid (256 target blocks x 8
+// source chunks) is then exactly one wave of resident workgroups on 256 CUs.
 template <int MODE>
-__global__ __launch_bounds__(kBlock) void k_direct_fast(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_direct_fast(
     const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
     const double* __restrict__ m, int n_recv, int j_begin, int j_end, int chunk, int ignore, double soft2,
     GBList gl, double* __restrict__ partial, int pstride) {
@@ -219,7 +231,9 @@ __global__ __launch_bounds__(kBlock) void k_direct_fast(
     const int blk_lo = blockIdx.x * kBlock;
     const int blk_hi = blk_lo + kBlock;
     double sx = 0., sy = 0., sz = 0.;      // running sums
-    double cx = 0., cy = 0., cz = 0.;      // Kahan carries (COMP per pair, SPLIT per tile)
+    double cx = 0., cy = 0., cz = 0.;      // Kahan carries (COMP per pair, FP64/SPLIT per tile)
+    const double c15 = vreg_const(1.5);
+    const double c1875 = vreg_const(1.875);
     for (int g = 0; g < gl.n; g++) {
         const GB3 gb = gl.g[g];
         const double gx = gb.x + xi, gy = gb.y + yi, gz = gb.z + zi;
@@ -236,20 +250,25 @@ __global__ __launch_bounds__(kBlock) void k_direct_fast(
             // Only tiles holding a target of this block, or sources 0/1 when
             // terms are ignored, need the per-pair skip test.
             const bool check = (t < blk_hi && t + n > blk_lo) || (ignore != 0 && t < 2);
-            if (MODE == FM_SPLIT) {
+            if (MODE != FM_COMP) {
+                // plain FP64 sum over the tile, folded into the running sum with Kahan
                 double tx = 0., ty = 0., tz = 0., d0 = 0., d1 = 0., d2 = 0.;
                 if (check)
-                    pair_tile<MODE, true>(tile, mtile, n, t, i, ignore, gx, gy, gz, soft2, tx, ty, tz, d0, d1, d2);
+                    pair_tile<MODE, true>(tile, mtile, n, t, i, ignore, gx, gy, gz, soft2, tx, ty, tz, d0, d1, d2,
+                                          c15, c1875);
                 else
-                    pair_tile<MODE, false>(tile, mtile, n, t, i, ignore, gx, gy, gz, soft2, tx, ty, tz, d0, d1, d2);
+                    pair_tile<MODE, false>(tile, mtile, n, t, i, ignore, gx, gy, gz, soft2, tx, ty, tz, d0, d1, d2,
+                                           c15, c1875);
                 kahan_add(tx, sx, cx);
                 kahan_add(ty, sy, cy);
                 kahan_add(tz, sz, cz);
             } else {
                 if (check)
-                    pair_tile<MODE, true>(tile, mtile, n, t, i, ignore, gx, gy, gz, soft2, sx, sy, sz, cx, cy, cz);
+                    pair_tile<MODE, true>(tile, mtile, n, t, i, ignore, gx, gy, gz, soft2, sx, sy, sz, cx, cy, cz,
+                                          c15, c1875);
                 else
-                    pair_tile<MODE, false>(tile, mtile, n, t, i, ignore, gx, gy, gz, soft2, sx, sy, sz, cx, cy, cz);
+                    pair_tile<MODE, false>(tile, mtile, n, t, i, ignore, gx, gy, gz, soft2, sx, sy, sz, cx, cy, cz,
+                                           c15, c1875);
             }
         }
     }
