@@ -85,8 +85,11 @@ def run_mode(amd, args, name, precision, gravity, steps, warmup):
 
 
 def accuracy(amd, args):
-    """Per-particle error of each mode's initial accelerations against the
-    bit-exact FP64 path (identical to the reference's BASIC sum)."""
+    """Per-particle error of each mode's initial accelerations, on sampled
+    rows, against (1) the bit-exact FP64 path (identical to the reference's
+    BASIC sum) and (2) the exactly summed reference forces: the reference's
+    per-pair terms (gravity.c:222-230) summed in extended precision.  The
+    normalised error divides by sum_j |a_ij| (conditioning of the sum)."""
     def acc_of(prec, gravity="basic"):
         sim = plummer_sim(amd, args.n, args.seed, args.softening, args.dt, prec, gravity)
         a = sim.update_acceleration()
@@ -97,25 +100,44 @@ def accuracy(amd, args):
     ref, st = acc_of("exact")
     x = st[:, :3]
     m = st[:, 9]
-    rows = np.random.default_rng(0).choice(args.n, min(args.n, 2048), replace=False)
-    d = x[rows, None, :] - x[None, :, :]
-    dist2 = (d * d).sum(-1)
-    with np.errstate(divide="ignore", invalid="ignore"):
-        w = m[None, :] * np.sqrt(dist2) / (dist2 + args.softening ** 2) ** 1.5
-    w[~np.isfinite(w)] = 0
-    cond = w.sum(1)
-    out = {}
+    rows = np.sort(np.random.default_rng(0).choice(args.n, min(args.n, args.acc_rows), replace=False))
+    truth = np.zeros((len(rows), 3))
+    cond = np.zeros(len(rows))
+    soft2 = args.softening * args.softening
+    for b in range(0, len(rows), 64):
+        r = rows[b:b + 64]
+        d = x[r, None, :] - x[None, :, :]
+        rr = np.sqrt(d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1] + d[..., 2] * d[..., 2] + soft2)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            pre = -1.0 / (rr * rr * rr) * m[None, :]
+        pre[np.arange(len(r)), r] = 0.0
+        t = pre[..., None] * d
+        truth[b:b + 64] = t.astype(np.longdouble).sum(1).astype(np.float64)
+        cond[b:b + 64] = np.sqrt((t * t).sum(2)).sum(1)
+    tn = np.sqrt((truth ** 2).sum(1))
+
+    def stats(a):
+        da = np.sqrt(((a[rows] - truth) ** 2).sum(1))
+        return {"rel_max": float((da / tn).max()), "rel_p99": float(np.quantile(da / tn, 0.99)),
+                "rel_median": float(np.median(da / tn)), "normalised_max": float((da / cond).max())}
+
+    out = {"exact": {"vs_exact_sum": stats(ref)}}
+    log("accuracy %-12s vs exactly summed: rel max %.2e p99 %.2e | normalised max %.2e" % (
+        "exact(ref)", out["exact"]["vs_exact_sum"]["rel_max"], out["exact"]["vs_exact_sum"]["rel_p99"],
+        out["exact"]["vs_exact_sum"]["normalised_max"]))
     for name, prec, grav in (("fp64", "fp64", "basic"), ("compensated", "fp64", "compensated"),
                              ("split", "split", "basic")):
         a, _ = acc_of(prec, grav)
         da = np.sqrt(((a[rows] - ref[rows]) ** 2).sum(1))
         rel = da / np.sqrt((ref[rows] ** 2).sum(1))
-        nrm = da / cond
-        out[name] = {"rel_max": float(rel.max()), "rel_p99": float(np.quantile(rel, 0.99)),
-                     "rel_median": float(np.median(rel)), "normalised_max": float(nrm.max()),
-                     "sampled_rows": int(len(rows))}
-        log("accuracy %-12s rel max %.2e p99 %.2e median %.2e | |da|/sum|a_ij| max %.2e" % (
-            name, rel.max(), np.quantile(rel, 0.99), np.median(rel), nrm.max()))
+        out[name] = {"vs_reference": {"rel_max": float(rel.max()), "rel_p99": float(np.quantile(rel, 0.99)),
+                                      "rel_median": float(np.median(rel)), "normalised_max": float((da / cond).max())},
+                     "vs_exact_sum": stats(a), "sampled_rows": int(len(rows))}
+        v = out[name]
+        log("accuracy %-12s vs reference: rel max %.2e p99 %.2e | vs exactly summed: rel max %.2e p99 %.2e "
+            "normalised max %.2e" % (name, v["vs_reference"]["rel_max"], v["vs_reference"]["rel_p99"],
+                                     v["vs_exact_sum"]["rel_max"], v["vs_exact_sum"]["rel_p99"],
+                                     v["vs_exact_sum"]["normalised_max"]))
     return out
 
 
@@ -138,6 +160,96 @@ def cpu_baseline(args):
                       "steps after 1 warm-up step, %.1f s" % (args.n, steps, res["seconds"])}
 
 
+def ensemble_leg(amd, args):
+    """Batched WHFast ensemble: 16384 systems (star + K planets, K uniform in
+    {2..8}), democratic heliocentric, safe_mode 0, dt = inner period / 20,
+    `ens_steps` steps in one request (state resident in HBM)."""
+    import ctypes
+    nb, st, dt = amd.planetary_systems(args.ens_systems, args.ens_seed, 2, 8)
+    ens = amd.Ensemble(nb, st, dt, safe_mode=0)
+    e0 = ens.energy()
+    tune_ms, shape = ens.tune(probe_steps=args.ens_probe)
+    log("ensemble: %d systems, %d bodies, %d tasks; tuned launch %s (%.2f ms / %d steps)" % (
+        len(nb), int(nb.sum()), shape["tasks"], shape, tune_ms, args.ens_probe))
+    L = amd.lib()
+    hip = ctypes.CDLL("libamdhip64.so")
+    t0 = time.perf_counter()
+    ens.steps(args.ens_steps)
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    e1 = ens.energy()
+    rel = np.abs((e1 - e0) / e0)
+    flags = ens.flags()
+    sys_steps = float(len(nb)) * args.ens_steps
+    res = {
+        "metric": "ensemble systems*steps/s (WHFast DH, 16384 systems, K in 2..8)",
+        "value": sys_steps / wall, "unit": "system-steps/s", "seconds": wall, "steps": args.ens_steps,
+        "systems": int(len(nb)), "bodies": int(nb.sum()), "launch": shape,
+        "us_per_step": 1e6 * wall / args.ens_steps,
+        "rel_energy_error": {"median": float(np.median(rel)), "p99": float(np.quantile(rel, 0.99)),
+                             "max": float(rel.max()), "mean": float(rel.mean())},
+        "kepler_period_warnings": int(((flags >> 1) & 1).sum()),
+    }
+    log("ensemble: %.3e system-steps/s (%.2f us/step, %.2f s), rel energy error median %.2e mean %.2e" % (
+        res["value"], res["us_per_step"], wall, np.median(rel), rel.mean()))
+    del hip, L
+    if not args.no_cpu:
+        cpu = ensemble_cpu(args)
+        if cpu is not None:
+            # the same systems on the GPU for the CPU's step count: energy errors must match
+            ens2 = amd.Ensemble(nb, st, dt, safe_mode=0)
+            ens2.steps(args.ens_cpu_steps)
+            r2 = np.abs((ens2.energy() - e0) / e0)
+            ens2.free()
+            cpu["gpu_at_same_steps"] = {"max_rel_energy_error": float(r2.max()),
+                                        "mean_rel_energy_error": float(r2.mean())}
+            cpu["energy_error_matches"] = bool(
+                "%.3e" % r2.max() == "%.3e" % cpu["max_rel_energy_error"] and
+                "%.3e" % r2.mean() == "%.3e" % cpu["mean_rel_energy_error"])
+            res["cpu_baseline"] = cpu
+            res["speedup_vs_cpu"] = res["value"] / cpu["value"]
+            log("ensemble cpu: %.3e system-steps/s on %d threads -> GPU %.0fx; energy errors match: %s" % (
+                cpu["value"], cpu["cores"], res["speedup_vs_cpu"], cpu["energy_error_matches"]))
+    ens.free()
+    return res
+
+
+def ensemble_cpu(args):
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_bench")
+    if not os.path.exists(exe):
+        return None
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    cmd = [exe, "ensemble", str(args.ens_systems), str(args.ens_cpu_steps), str(args.ens_seed), "0", "2", "8"]
+    log("ensemble cpu baseline:", " ".join(cmd), "threads", threads)
+    out = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=900)
+    if out.returncode != 0:
+        log(out.stderr)
+        return None
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    return {"value": r["system_steps_per_s"], "unit": "system-steps/s", "cores": r["threads"], "kind": "reference",
+            "sample": "reference OpenMP build, omp parallel for over the same %d systems (one reb_simulation "
+                      "each, reb_simulation_steps), %d steps, %.1f s" % (args.ens_systems, args.ens_cpu_steps,
+                                                                          r["seconds"]),
+            "max_rel_energy_error": r["max_rel_energy_error"], "mean_rel_energy_error": r["mean_rel_energy_error"]}
+
+
+def copy_calibration(mib, reps=10):
+    """Device-to-device copies of `mib` MiB: the reference reading of the
+    FETCH_SIZE / WRITE_SIZE counters for a known byte count."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(n)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    print(json.dumps({"copy_calibration_bytes": n, "reps": reps}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -150,7 +262,19 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline mode only")
+    ap.add_argument("--acc-rows", type=int, default=512, help="sampled rows of the accuracy check")
+    ap.add_argument("--ens-systems", type=int, default=16384)
+    ap.add_argument("--ens-steps", type=int, default=100000)
+    ap.add_argument("--ens-seed", type=int, default=1)
+    ap.add_argument("--ens-probe", type=int, default=500, help="steps per autotuner probe")
+    ap.add_argument("--ens-cpu-steps", type=int, default=500)
+    ap.add_argument("--only", choices=["direct", "ensemble"], default=None)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
@@ -160,6 +284,9 @@ def main():
         raise SystemExit("no HIP device")
     log("device:", amd.device_name())
 
+    if args.only == "ensemble":
+        print(json.dumps({"ensemble": ensemble_leg(amd, args)}))
+        return
     head, _ = run_mode(amd, args, "fp64", "fp64", "basic", args.steps, args.warmup)
     modes = {"fp64": head}
     if not args.quick:
@@ -169,10 +296,10 @@ def main():
             modes[name], _ = run_mode(amd, args, name, prec, grav, k2, 2)
         acc = accuracy(amd, args)
         for k, v in acc.items():
-            modes[k]["accuracy_vs_exact"] = v
+            modes[k]["accuracy"] = v
     cpu = None if args.no_cpu else cpu_baseline(args)
+    ens = None if (args.quick or args.only == "direct") else ensemble_leg(amd, args)
 
-    kernel_s = head["kernel_ms"] * 1e-3
     achieved = head["algorithmic_tflops"]
     line = {
         "metric": "Gpair-interactions/s at N=65536 (direct gravity, FP64)",
@@ -196,6 +323,7 @@ def main():
                      "flop_per_pair": FLOP_PER_PAIR["fp64"], "pairs_per_launch": float(args.n) * (args.n - 1)},
         "cpu_baseline": cpu,
         "modes": modes,
+        "ensemble": ens,
     }
     print(json.dumps(line))
 

@@ -295,6 +295,12 @@ REB_API void reb_simulation_free(struct reb_simulation* const r);
 REB_API struct reb_simulation* reb_simulation_copy(struct reb_simulation* r);
 REB_API size_t reb_simulation_struct_size(void);                     /* simulation.c:65-67 */
 
+/* Build, version and hash strings (rebound.c:41-43); the reference's Python
+ * package reads reb_version_str and reb_build_str with ctypes in_dll. */
+REB_API extern const char* reb_build_str;
+REB_API extern const char* reb_version_str;
+REB_API extern const char* reb_githash_str;
+
 /* ---------------------------------------------------------------- integrators
  * simulation.c:201-238, rebound.c:60-113 */
 REB_API void* reb_simulation_set_integrator(struct reb_simulation* r, const char* name);

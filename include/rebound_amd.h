@@ -103,6 +103,49 @@ REB_API int reb_amd_kepler_solver_batch(double* state6, const double* mu, const 
  * struct reb_treecell (tree.h:34-56).  Returns the number of cells or -1. */
 REB_API long reb_amd_tree_dump(struct reb_simulation* r, double* out, long cap);
 
+/* ---------------------------------------------------------------------------
+ * Batched ensembles: many independent planetary systems integrated together
+ * with WHFast in democratic heliocentric coordinates (no correctors, default
+ * kernel, gravity_ignore_terms INVOLVING_0), state resident in device memory.
+ * Each system evolves bit-identically to a reb_simulation holding the same
+ * particles with integrator "whfast", coordinates DEMOCRATICHELIOCENTRIC, the
+ * given safe_mode, dt and G, driven by reb_simulation_steps
+ * (integrator_whfast.c:1113-1259, simulation.c:504-513).  The reference has no
+ * batched form: its equivalent is a loop over reb_simulation_steps calls.
+ * ------------------------------------------------------------------------- */
+struct reb_amd_ensemble;
+
+/* n_systems systems; system s has n_bodies[s] bodies (1..65, central body
+ * first), stored consecutively in `bodies`; dt[s] its timestep.  gravity is
+ * REB_GRAVITY_BASIC or REB_GRAVITY_COMPENSATED.  NULL on error
+ * (reb_amd_last_error). */
+REB_API struct reb_amd_ensemble* reb_amd_ensemble_create(size_t n_systems, const size_t* n_bodies,
+                                                         const struct reb_particle* bodies, const double* dt,
+                                                         double G, int safe_mode, int gravity);
+REB_API void reb_amd_ensemble_free(struct reb_amd_ensemble* e);
+/* Queues n steps of every system, ending synchronized; does not wait. */
+REB_API int reb_amd_ensemble_steps(struct reb_amd_ensemble* e, uint64_t n_steps);
+REB_API int reb_amd_ensemble_wait(struct reb_amd_ensemble* e);
+REB_API uint64_t reb_amd_ensemble_steps_done(const struct reb_amd_ensemble* e);
+REB_API size_t reb_amd_ensemble_n_systems(const struct reb_amd_ensemble* e);
+REB_API size_t reb_amd_ensemble_n_bodies(const struct reb_amd_ensemble* e);
+/* Inertial x y z vx vy vz ax ay az m of all bodies, creation order. */
+REB_API int reb_amd_ensemble_get_particles(struct reb_amd_ensemble* e, struct reb_particle* out);
+/* reb_simulation_energy of every system (tools.c:108-162). */
+REB_API int reb_amd_ensemble_energy(struct reb_amd_ensemble* e, double* out);
+/* Per system: bit 0 synchronized, bit 1 Kepler step longer than an orbital
+ * period, bit 2 particles modified while not synchronized. */
+REB_API int reb_amd_ensemble_flags(struct reb_amd_ensemble* e, unsigned int* out);
+/* Launch shape: persistent workgroups (0 = default) and steps per launch
+ * (0 = a whole request in one launch). */
+REB_API int reb_amd_ensemble_set_launch(struct reb_amd_ensemble* e, int grid, long chunk);
+REB_API int reb_amd_ensemble_get_launch(const struct reb_amd_ensemble* e, int* grid, long* chunk, int* ntasks,
+                                        int* max_resident);
+/* Times candidate launch shapes for probe_steps steps each (state restored
+ * afterwards) and keeps the fastest; best_ms receives its time. */
+REB_API int reb_amd_ensemble_tune(struct reb_amd_ensemble* e, long probe_steps, double* best_ms);
+REB_API const char* reb_amd_ensemble_error(const struct reb_amd_ensemble* e);
+
 #ifdef __cplusplus
 }
 #endif

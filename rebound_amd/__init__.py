@@ -187,3 +187,6 @@ class Sim:
 
     def move_to_com(self):
         self.lib.reb_simulation_move_to_com(self.ptr)
+
+
+from .ensemble import Ensemble, planetary_systems  # noqa: E402

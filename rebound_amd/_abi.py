@@ -171,6 +171,22 @@ def declare(lib):
         "reb_amd_sync_host": (c_int, [S]),
         "reb_amd_mark_host_modified": (None, [S]),
         "reb_amd_synchronize_device": (c_int, [S]),
+        "reb_amd_ensemble_create": (c_void_p, [c_size_t, POINTER(c_size_t), POINTER(Particle), POINTER(c_double),
+                                               c_double, c_int, c_int]),
+        "reb_amd_ensemble_free": (None, [c_void_p]),
+        "reb_amd_ensemble_steps": (c_int, [c_void_p, ctypes.c_uint64]),
+        "reb_amd_ensemble_wait": (c_int, [c_void_p]),
+        "reb_amd_ensemble_steps_done": (ctypes.c_uint64, [c_void_p]),
+        "reb_amd_ensemble_n_systems": (c_size_t, [c_void_p]),
+        "reb_amd_ensemble_n_bodies": (c_size_t, [c_void_p]),
+        "reb_amd_ensemble_get_particles": (c_int, [c_void_p, POINTER(Particle)]),
+        "reb_amd_ensemble_energy": (c_int, [c_void_p, POINTER(c_double)]),
+        "reb_amd_ensemble_flags": (c_int, [c_void_p, POINTER(ctypes.c_uint)]),
+        "reb_amd_ensemble_set_launch": (c_int, [c_void_p, c_int, ctypes.c_long]),
+        "reb_amd_ensemble_get_launch": (c_int, [c_void_p, POINTER(c_int), POINTER(ctypes.c_long), POINTER(c_int),
+                                                POINTER(c_int)]),
+        "reb_amd_ensemble_tune": (c_int, [c_void_p, ctypes.c_long, POINTER(c_double)]),
+        "reb_amd_ensemble_error": (c_char_p, [c_void_p]),
     }
     for name, (res, args) in sig.items():
         if hasattr(lib, name):

@@ -1,0 +1,438 @@
+// Batched ensembles of independent planetary systems, integrated with WHFast
+// in democratic heliocentric coordinates (integrator_whfast.c:1113-1259,
+// transformations.c:388-477) entirely on the GPU.
+//
+// Layout.  All bodies of all systems live in one structure-of-arrays block
+// (x y z vx vy vz ax ay az m, `stride` doubles per field) plus the heliocentric
+// copy (x y z vx vy vz m).  Systems are bucketed by planet count K; a wave task
+// holds floor(64/K) systems of one bucket, one lane per planet.  The persistent
+// kernel (whfast.hip) pulls tasks from a work counter, keeps a task's whole
+// state in registers for all steps of the launch and writes it back once.
+//
+// Step loop.  Every launch is a replay of a captured hipGraph of
+// [reset work counter, persistent kernel for `nsteps` steps]; the counter reset
+// is a kernel (not a memset) so a graph can be replayed any number of times.
+// A request of n steps is split into launches of `chunk` steps (0: one launch),
+// the last one ending with the synchronization that reb_simulation_steps does
+// (simulation.c:504-513).  Grid and chunk come from reb_amd_ensemble_tune or
+// the defaults (one resident wavefront per task, one launch per request).
+//
+// Each system's arithmetic is exactly the per-simulation WHFast plugin's, so
+// every system evolves bit-identically to the reference integrating it alone.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "internal.h"
+#include "whfast_device.h"
+
+namespace rbamd {
+
+struct GraphEntry {
+    long nsteps = -1;
+    int did_modify = 0, sync_at_end = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    unsigned long long last_use = 0;
+};
+
+struct Ensemble {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t nsys = 0, nbody = 0, stride = 0;
+    std::vector<size_t> n_bodies, first;
+    double G = 1.;
+    int safe_mode = 0;
+    int compensated = 0;
+    DevBuf body, ph, sys, tasks, order, flags, counter, energy;
+    int ntasks = 0;
+    int grid = 0;
+    long chunk = 0;
+    bool first_step_done = false;
+    uint64_t steps_done = 0;
+    int max_resident = 0;
+    GraphEntry graphs[4];
+    unsigned long long use_clock = 0;
+    std::string err;
+};
+
+static bool ens_ok(Ensemble* e, hipError_t rc, const char* what) {
+    if (rc == hipSuccess) return true;
+    e->err = std::string("HIP error ") + hipGetErrorString(rc) + " in " + what;
+    set_last_error(e->err);
+    return false;
+}
+#define ENS_HIP(e, call) ens_ok((e), (call), #call)
+
+static WHParams ens_params(const Ensemble* e) {
+    WHParams P;
+    P.sys = e->sys.as<WHSys>();
+    P.tasks = e->tasks.as<WHTask>();
+    P.order = e->order.as<int>();
+    P.flags = e->flags.as<unsigned int>();
+    P.counter = e->counter.as<unsigned int>();
+    P.ntasks = e->ntasks;
+    P.grid = e->grid;
+    P.body = e->body.as<double>();
+    P.ph = e->ph.as<double>();
+    P.stride = e->stride;
+    P.G = e->G;
+    P.soft2 = 0.;
+    P.compensated = e->compensated;
+    P.nsteps = 0;
+    P.safe_mode = e->safe_mode;
+    P.did_modify = 0;
+    P.sync_at_end = 0;
+    return P;
+}
+
+static void drop_graph(GraphEntry& g) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+    g = GraphEntry();
+}
+
+static void drop_graphs(Ensemble* e) {
+    for (auto& g : e->graphs) drop_graph(g);
+}
+
+// The graph of one launch shape, captured on first use (a small LRU cache).
+static GraphEntry* graph_for(Ensemble* e, long nsteps, int did_modify, int sync_at_end) {
+    GraphEntry* slot = nullptr;
+    for (auto& g : e->graphs) {
+        if (g.exec && g.nsteps == nsteps && g.did_modify == did_modify && g.sync_at_end == sync_at_end) {
+            g.last_use = ++e->use_clock;
+            return &g;
+        }
+        if (!slot || !g.exec || (slot->exec && g.last_use < slot->last_use)) slot = &g;
+    }
+    drop_graph(*slot);
+    WHParams P = ens_params(e);
+    P.nsteps = nsteps;
+    P.did_modify = did_modify;
+    P.sync_at_end = sync_at_end;
+    if (!ENS_HIP(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal))) return nullptr;
+    const int rc = whfast_launch(P, e->stream);
+    hipGraph_t graph = nullptr;
+    const hipError_t end = hipStreamEndCapture(e->stream, &graph);
+    if (rc || end != hipSuccess || !graph) {
+        if (graph) (void)hipGraphDestroy(graph);
+        e->err = "capture of the ensemble step launch failed";
+        set_last_error(e->err);
+        return nullptr;
+    }
+    slot->graph = graph;
+    if (!ENS_HIP(e, hipGraphInstantiate(&slot->exec, graph, nullptr, nullptr, 0))) {
+        drop_graph(*slot);
+        return nullptr;
+    }
+    slot->nsteps = nsteps;
+    slot->did_modify = did_modify;
+    slot->sync_at_end = sync_at_end;
+    slot->last_use = ++e->use_clock;
+    return slot;
+}
+
+static bool launch(Ensemble* e, long nsteps, int did_modify, int sync_at_end) {
+    GraphEntry* g = graph_for(e, nsteps, did_modify, sync_at_end);
+    return g && ENS_HIP(e, hipGraphLaunch(g->exec, e->stream));
+}
+
+// n steps, ending synchronized like reb_simulation_steps.
+static bool ens_steps(Ensemble* e, uint64_t n) {
+    if (!ENS_HIP(e, hipSetDevice(e->device))) return false;
+    if (n == 0) return true;
+    const uint64_t chunk = e->chunk > 0 ? (uint64_t)e->chunk : n;
+    uint64_t left = n;
+    while (left > 0) {
+        const uint64_t k = std::min(chunk, left);
+        // The first launch derives the heliocentric state from the inertial one
+        // (particles were added: integrator_whfast.c:1120-1135).
+        const int did_modify = e->first_step_done ? 0 : 1;
+        if (!launch(e, (long)k, did_modify, k == left ? 1 : 0)) return false;
+        e->first_step_done = true;
+        left -= k;
+    }
+    e->steps_done += n;
+    return true;
+}
+
+}  // namespace rbamd
+
+using namespace rbamd;
+
+struct reb_amd_ensemble {
+    Ensemble e;
+};
+
+extern "C" {
+
+REB_API void reb_amd_ensemble_free(struct reb_amd_ensemble* h);
+
+REB_API struct reb_amd_ensemble* reb_amd_ensemble_create(size_t n_systems, const size_t* n_bodies,
+                                                         const struct reb_particle* bodies, const double* dt,
+                                                         double G, int safe_mode, int gravity) {
+    if (reb_amd_device_count() <= 0) {
+        set_last_error("reb_amd_ensemble_create: no HIP device is available");
+        return nullptr;
+    }
+    if (n_systems == 0 || !n_bodies || !bodies || !dt) {
+        set_last_error("reb_amd_ensemble_create: empty ensemble or null argument");
+        return nullptr;
+    }
+    if (gravity != REB_GRAVITY_BASIC && gravity != REB_GRAVITY_COMPENSATED) {
+        set_last_error("reb_amd_ensemble_create: gravity must be REB_GRAVITY_BASIC or REB_GRAVITY_COMPENSATED");
+        return nullptr;
+    }
+    if (n_systems > (size_t)INT32_MAX / 65) {
+        set_last_error("reb_amd_ensemble_create: too many systems");
+        return nullptr;
+    }
+    size_t tot = 0;
+    for (size_t s = 0; s < n_systems; s++) {
+        if (n_bodies[s] < 1 || n_bodies[s] > 65) {
+            set_last_error("reb_amd_ensemble_create: each system needs 1 to 65 bodies, the central body first");
+            return nullptr;
+        }
+        tot += n_bodies[s];
+    }
+    auto* h = new reb_amd_ensemble();
+    Ensemble* e = &h->e;
+    (void)hipGetDevice(&e->device);
+    e->nsys = n_systems;
+    e->G = G;
+    e->safe_mode = safe_mode ? 1 : 0;
+    e->compensated = gravity == REB_GRAVITY_COMPENSATED ? 1 : 0;
+    e->n_bodies.assign(n_bodies, n_bodies + n_systems);
+    e->first.resize(n_systems);
+    for (size_t s = 0, b = 0; s < n_systems; s++) {
+        e->first[s] = b;
+        b += n_bodies[s];
+    }
+    e->nbody = tot;
+    e->stride = (tot + 63) / 64 * 64;
+    if (!ENS_HIP(e, hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking))) {
+        delete h;
+        return nullptr;
+    }
+    std::vector<double> hb(10 * e->stride, 0.), hp(7 * e->stride, 0.);
+    for (size_t b = 0; b < tot; b++) {
+        const struct reb_particle& p = bodies[b];
+        const double f[10] = {p.x, p.y, p.z, p.vx, p.vy, p.vz, p.ax, p.ay, p.az, p.m};
+        for (int k = 0; k < 10; k++) hb[k * e->stride + b] = f[k];
+    }
+    std::vector<WHSys> sys(n_systems);
+    std::vector<int> order(n_systems);
+    for (size_t s = 0; s < n_systems; s++) {
+        sys[s].body0 = (int)e->first[s];
+        sys[s].N = (int)n_bodies[s];
+        sys[s].Nact = (int)n_bodies[s];
+        sys[s].Nact_grav = (int)n_bodies[s];
+        sys[s].tp_type = 0;
+        sys[s].steps = -1;
+        sys[s].dt = dt[s];
+        order[s] = (int)s;
+    }
+    // Bucket by planet count, largest first (the longest tasks start first).
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return sys[a].N > sys[b].N; });
+    std::vector<WHTask> tasks;
+    for (size_t k = 0; k < n_systems;) {
+        const int Gs = whfast_group_size(sys[order[k]].N - 1);
+        const int per = 64 / Gs;
+        size_t m = 0;
+        while (k + m < n_systems && (int)m < per && sys[order[k + m]].N == sys[order[k]].N) m++;
+        tasks.push_back(WHTask{Gs, (int)k, (int)m});
+        k += m;
+    }
+    e->ntasks = (int)tasks.size();
+    std::vector<unsigned int> flags(n_systems, 1u);   // freshly added: synchronized
+    bool ok = e->body.ensure(sizeof(double) * hb.size()) && e->ph.ensure(sizeof(double) * hp.size()) &&
+              e->sys.ensure(sizeof(WHSys) * n_systems) && e->tasks.ensure(sizeof(WHTask) * tasks.size()) &&
+              e->order.ensure(sizeof(int) * n_systems) && e->flags.ensure(sizeof(unsigned int) * n_systems) &&
+              e->counter.ensure(256) && e->energy.ensure(sizeof(double) * n_systems);
+    if (!ok) {
+        set_last_error("reb_amd_ensemble_create: out of device memory");
+        reb_amd_ensemble_free(h);
+        return nullptr;
+    }
+    ok = ENS_HIP(e, hipMemcpy(e->body.p, hb.data(), sizeof(double) * hb.size(), hipMemcpyHostToDevice)) &&
+         ENS_HIP(e, hipMemcpy(e->ph.p, hp.data(), sizeof(double) * hp.size(), hipMemcpyHostToDevice)) &&
+         ENS_HIP(e, hipMemcpy(e->sys.p, sys.data(), sizeof(WHSys) * n_systems, hipMemcpyHostToDevice)) &&
+         ENS_HIP(e, hipMemcpy(e->tasks.p, tasks.data(), sizeof(WHTask) * tasks.size(), hipMemcpyHostToDevice)) &&
+         ENS_HIP(e, hipMemcpy(e->order.p, order.data(), sizeof(int) * n_systems, hipMemcpyHostToDevice)) &&
+         ENS_HIP(e, hipMemcpy(e->flags.p, flags.data(), sizeof(unsigned int) * n_systems, hipMemcpyHostToDevice)) &&
+         ENS_HIP(e, hipMemset(e->counter.p, 0, 256));
+    if (!ok) {
+        reb_amd_ensemble_free(h);
+        return nullptr;
+    }
+    e->max_resident = whfast_max_resident(e->device);
+    e->grid = std::min(e->ntasks, e->max_resident);
+    e->chunk = 0;
+    return h;
+}
+
+REB_API void reb_amd_ensemble_free(struct reb_amd_ensemble* h) {
+    if (!h) return;
+    Ensemble* e = &h->e;
+    (void)hipSetDevice(e->device);
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    drop_graphs(e);
+    for (DevBuf* b : {&e->body, &e->ph, &e->sys, &e->tasks, &e->order, &e->flags, &e->counter, &e->energy})
+        b->release();
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete h;
+}
+
+// Queues n steps of every system (ending synchronized); returns without
+// waiting for the device.  0 on success.
+REB_API int reb_amd_ensemble_steps(struct reb_amd_ensemble* h, uint64_t n_steps) {
+    return ens_steps(&h->e, n_steps) ? 0 : 1;
+}
+
+REB_API int reb_amd_ensemble_wait(struct reb_amd_ensemble* h) {
+    return ENS_HIP(&h->e, hipStreamSynchronize(h->e.stream)) ? 0 : 1;
+}
+
+REB_API uint64_t reb_amd_ensemble_steps_done(const struct reb_amd_ensemble* h) { return h->e.steps_done; }
+
+REB_API size_t reb_amd_ensemble_n_systems(const struct reb_amd_ensemble* h) { return h->e.nsys; }
+
+REB_API size_t reb_amd_ensemble_n_bodies(const struct reb_amd_ensemble* h) { return h->e.nbody; }
+
+// Inertial state of all bodies in creation order (x y z vx vy vz ax ay az m
+// filled; the remaining fields of `out` are left as they are).
+REB_API int reb_amd_ensemble_get_particles(struct reb_amd_ensemble* h, struct reb_particle* out) {
+    Ensemble* e = &h->e;
+    std::vector<double> hb(10 * e->stride);
+    if (!ENS_HIP(e, hipSetDevice(e->device)) ||
+        !ENS_HIP(e, hipMemcpyAsync(hb.data(), e->body.p, sizeof(double) * hb.size(), hipMemcpyDeviceToHost,
+                                   e->stream)) ||
+        !ENS_HIP(e, hipStreamSynchronize(e->stream)))
+        return 1;
+    for (size_t b = 0; b < e->nbody; b++) {
+        struct reb_particle& p = out[b];
+        double* f[10] = {&p.x, &p.y, &p.z, &p.vx, &p.vy, &p.vz, &p.ax, &p.ay, &p.az, &p.m};
+        for (int k = 0; k < 10; k++) *f[k] = hb[k * e->stride + b];
+    }
+    return 0;
+}
+
+// Total energy of every system (tools.c:108-162 order), into out[n_systems].
+REB_API int reb_amd_ensemble_energy(struct reb_amd_ensemble* h, double* out) {
+    Ensemble* e = &h->e;
+    if (!ENS_HIP(e, hipSetDevice(e->device))) return 1;
+    const WHParams P = ens_params(e);
+    if (whfast_energy_launch(P, e->energy.as<double>(), (int)e->nsys, e->stream)) {
+        ENS_HIP(e, hipGetLastError());
+        return 1;
+    }
+    return ENS_HIP(e, hipMemcpyAsync(out, e->energy.p, sizeof(double) * e->nsys, hipMemcpyDeviceToHost, e->stream)) &&
+                   ENS_HIP(e, hipStreamSynchronize(e->stream))
+               ? 0
+               : 1;
+}
+
+// Per-system status bits: 1 synchronized, 2 a Kepler step longer than an
+// orbital period was taken, 4 particles modified while unsynchronized.
+REB_API int reb_amd_ensemble_flags(struct reb_amd_ensemble* h, unsigned int* out) {
+    Ensemble* e = &h->e;
+    return ENS_HIP(e, hipSetDevice(e->device)) &&
+                   ENS_HIP(e, hipMemcpyAsync(out, e->flags.p, sizeof(unsigned int) * e->nsys, hipMemcpyDeviceToHost,
+                                             e->stream)) &&
+                   ENS_HIP(e, hipStreamSynchronize(e->stream))
+               ? 0
+               : 1;
+}
+
+// Launch shape: persistent workgroups (0: one per task, capped at what is
+// resident at once) and steps per launch (0: a whole request in one launch).
+REB_API int reb_amd_ensemble_set_launch(struct reb_amd_ensemble* h, int grid, long chunk) {
+    Ensemble* e = &h->e;
+    const int g = grid > 0 ? std::min(grid, e->ntasks) : std::min(e->ntasks, e->max_resident);
+    const long c = chunk > 0 ? chunk : 0;
+    if (g != e->grid) drop_graphs(e);
+    e->grid = g;
+    e->chunk = c;
+    return 0;
+}
+
+REB_API int reb_amd_ensemble_get_launch(const struct reb_amd_ensemble* h, int* grid, long* chunk, int* ntasks,
+                                        int* max_resident) {
+    if (grid) *grid = h->e.grid;
+    if (chunk) *chunk = h->e.chunk;
+    if (ntasks) *ntasks = h->e.ntasks;
+    if (max_resident) *max_resident = h->e.max_resident;
+    return 0;
+}
+
+// Autotuner: times `probe_steps` steps of every candidate (grid, chunk) on the
+// ensemble's own state, restoring the state after each probe, and keeps the
+// fastest shape.  The ensemble is left exactly as it was before the call.
+REB_API int reb_amd_ensemble_tune(struct reb_amd_ensemble* h, long probe_steps, double* best_ms) {
+    Ensemble* e = &h->e;
+    if (probe_steps < 4) probe_steps = 4;
+    if (!ENS_HIP(e, hipSetDevice(e->device))) return 1;
+    DevBuf sb, sp, sf;
+    const size_t bb = sizeof(double) * 10 * e->stride, pb = sizeof(double) * 7 * e->stride,
+                 fb = sizeof(unsigned int) * e->nsys;
+    if (!sb.ensure(bb) || !sp.ensure(pb) || !sf.ensure(fb)) {
+        set_last_error("reb_amd_ensemble_tune: out of device memory");
+        return 1;
+    }
+    auto copy_state = [&](void* db, void* dp, void* df, const void* xb, const void* xp, const void* xf) {
+        return ENS_HIP(e, hipMemcpyAsync(db, xb, bb, hipMemcpyDeviceToDevice, e->stream)) &&
+               ENS_HIP(e, hipMemcpyAsync(dp, xp, pb, hipMemcpyDeviceToDevice, e->stream)) &&
+               ENS_HIP(e, hipMemcpyAsync(df, xf, fb, hipMemcpyDeviceToDevice, e->stream));
+    };
+    bool ok = copy_state(sb.p, sp.p, sf.p, e->body.p, e->ph.p, e->flags.p);
+    const bool first_done = e->first_step_done;
+    const uint64_t done = e->steps_done;
+    const int grid0 = e->grid;
+    const long chunk0 = e->chunk;
+    std::vector<int> grids = {std::min(e->ntasks, e->max_resident)};
+    for (int g : {e->ntasks / 2, e->ntasks / 4, 1024, 512})
+        if (g > 0 && g < grids[0] && std::find(grids.begin(), grids.end(), g) == grids.end()) grids.push_back(g);
+    const std::vector<long> chunks = {0, std::max(1L, probe_steps / 4), std::max(1L, probe_steps / 16)};
+    hipEvent_t a = nullptr, b = nullptr;
+    ok = ok && ENS_HIP(e, hipEventCreate(&a)) && ENS_HIP(e, hipEventCreate(&b));
+    double best = 1e300;
+    int bg = grid0;
+    long bc = chunk0;
+    for (int g : grids) {
+        for (long c : chunks) {
+            if (!ok) break;
+            reb_amd_ensemble_set_launch(h, g, c);
+            ok = ens_steps(e, (uint64_t)probe_steps);     // captures the graphs, warms up
+            ok = ok && copy_state(e->body.p, e->ph.p, e->flags.p, sb.p, sp.p, sf.p);
+            e->first_step_done = first_done;
+            ok = ok && ENS_HIP(e, hipEventRecord(a, e->stream)) && ens_steps(e, (uint64_t)probe_steps) &&
+                 ENS_HIP(e, hipEventRecord(b, e->stream)) && ENS_HIP(e, hipEventSynchronize(b));
+            float ms = 0.f;
+            ok = ok && ENS_HIP(e, hipEventElapsedTime(&ms, a, b));
+            if (ok && ms < best) {
+                best = ms;
+                bg = g;
+                bc = c;
+            }
+            ok = ok && copy_state(e->body.p, e->ph.p, e->flags.p, sb.p, sp.p, sf.p);
+            e->first_step_done = first_done;
+            e->steps_done = done;
+        }
+    }
+    ok = ENS_HIP(e, hipStreamSynchronize(e->stream)) && ok;
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    sb.release();
+    sp.release();
+    sf.release();
+    reb_amd_ensemble_set_launch(h, bg, bc);
+    if (best_ms) *best_ms = best;
+    return ok ? 0 : 1;
+}
+
+REB_API const char* reb_amd_ensemble_error(const struct reb_amd_ensemble* h) { return h->e.err.c_str(); }
+
+}  // extern "C"

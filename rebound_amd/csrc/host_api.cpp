@@ -99,6 +99,10 @@ static void rb_unsupported_step(struct reb_simulation* r, void* state) {
     reb_simulation_error(r, msg);
 }
 
+const char* reb_build_str = __DATE__ " " __TIME__;
+const char* reb_version_str = "5.0.0";
+const char* reb_githash_str = "rebound-amd";
+
 const struct reb_integrator reb_integrator_none = {
     (char*)"Does nothing other than advance time.", rb_none_step, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
 static const struct reb_integrator rb_integrator_leapfrog = {

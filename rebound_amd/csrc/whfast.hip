@@ -254,8 +254,8 @@ __global__ __launch_bounds__(64) void k_whfast_dh(WHParams P) {
         const int N = sys_ok ? S.N : 0;      // bodies incl. the star
         const int me = gl + 1;               // my body index
         const bool live = me < N;
-        const int NA = S.Nact;               // N_active of the whfast sums
-        const int NAG = S.Nact_grav;         // N_active of the gravity loop
+        const int NA = sys_ok ? S.Nact : 0;        // N_active of the whfast sums
+        const int NAG = sys_ok ? S.Nact_grav : 0;  // N_active of the gravity loop
         const size_t st = P.stride;
         const size_t b0 = (size_t)S.body0;
         // ---- load
@@ -282,25 +282,28 @@ __global__ __launch_bounds__(64) void k_whfast_dh(WHParams P) {
             hvx = P.ph[3 * st + b]; hvy = P.ph[4 * st + b]; hvz = P.ph[5 * st + b];
         }
         double hm = im;   // p_h[i].m
-        int sync = sys_ok ? (P.flags[sidx] & 1) : 1;
-        int warn = 0;
+        const unsigned int flags0 = sys_ok ? P.flags[sidx] : 1u;
+        int sync = flags0 & 1u;
+        int warn = (flags0 >> 1) & 3u;        // warnings stay set until the host clears them
+        // Steps of this system in this launch (a system may stop before the
+        // launch's last step; its lanes keep taking part in the barriers).
+        const long my_steps = !sys_ok ? 0 : (S.steps >= 0 && S.steps < P.nsteps ? (long)S.steps : P.nsteps);
         const double dt = S.dt;
         const double Gc = P.G;
 
-        // publish my planet's inertial state into LDS
-        auto publish_inertial = [&]() {
+        // Each lane writes its planet's values into LDS; the group then forms
+        // the reference's sequential sums over planets 1..NA-1 in index order.
+        // A term such as p_h[i].x*m/mtot is evaluated once by its owner lane,
+        // with the reference's expression, so every sum has the reference's bits.
+        auto publish = [&](const Slot& v) {
             __syncthreads();
-            slot[lane] = Slot{ix, iy, iz, ivx, ivy, ivz, im};
-            __syncthreads();
-        };
-        auto publish_helio = [&]() {
-            __syncthreads();
-            slot[lane] = Slot{hx, hy, hz, hvx, hvy, hvz, hm};
+            slot[lane] = v;
             __syncthreads();
         };
+        auto publish_inertial = [&]() { publish(Slot{ix, iy, iz, ivx, ivy, ivz, im}); };
         // transformations.c:388-424 (inertial -> democratic heliocentric)
         auto from_inertial = [&]() {
-            publish_inertial();
+            publish(Slot{ix * im, iy * im, iz * im, ivx * im, ivy * im, ivz * im, im});
             double x0 = 0., y0 = 0., z0 = 0., vx0 = 0., vy0 = 0., vz0 = 0., m0 = 0.;
             if (0 < NA) {
                 const double m = sm;
@@ -310,10 +313,9 @@ __global__ __launch_bounds__(64) void k_whfast_dh(WHParams P) {
             }
             GROUP_LOOP(k, 1, NA) {
                 const Slot q = slot[gbase + k - 1];
-                const double m = q.m;
-                x0 += q.x * m; y0 += q.y * m; z0 += q.z * m;
-                vx0 += q.vx * m; vy0 += q.vy * m; vz0 += q.vz * m;
-                m0 += m;
+                x0 += q.x; y0 += q.y; z0 += q.z;
+                vx0 += q.vx; vy0 += q.vy; vz0 += q.vz;
+                m0 += q.m;
             }
             cx = x0 / m0; cy = y0 / m0; cz = z0 / m0;
             cvx = vx0 / m0; cvy = vy0 / m0; cvz = vz0 / m0;
@@ -325,52 +327,61 @@ __global__ __launch_bounds__(64) void k_whfast_dh(WHParams P) {
         // transformations.c:426-477 (democratic heliocentric -> inertial,
         // positions then velocities).  Barriers are executed by every lane;
         // lanes whose system does not take part (en == false) keep their state.
-        auto to_inertial = [&](bool en) {
-            publish_helio();
+        // The velocity pass is skipped (with_vel == false) where nothing reads
+        // the inertial velocities before they are recomputed.
+        auto to_inertial = [&](bool en, bool with_vel) {
             const double mtot = cm;
-            double x0 = 0., y0 = 0., z0 = 0.;
-            GROUP_LOOP(k, 1, NA) {
-                const Slot q = slot[gbase + k - 1];
-                const double m = q.m;
-                x0 += q.x * m / mtot;
-                y0 += q.y * m / mtot;
-                z0 += q.z * m / mtot;
+            const double m0 = sm;   // particles[0].m; particles[i].m == p_h[i].m after the position pass
+            Slot t{hx * hm / mtot, hy * hm / mtot, hz * hm / mtot, 0., 0., 0., 0.};
+            if (with_vel) {
+                t.vx = hvx * hm / m0;
+                t.vy = hvy * hm / m0;
+                t.vz = hvz * hm / m0;
             }
-            const double m0 = sm;
+            publish(t);
+            double x0 = 0., y0 = 0., z0 = 0.;
             double vx0 = 0., vy0 = 0., vz0 = 0.;
             GROUP_LOOP(k, 1, NA) {
                 const Slot q = slot[gbase + k - 1];
-                const double m = q.m;   // particles[k].m == p_h[k].m after the position pass
-                vx0 += q.vx * m / m0;
-                vy0 += q.vy * m / m0;
-                vz0 += q.vz * m / m0;
+                x0 += q.x;
+                y0 += q.y;
+                z0 += q.z;
+                if (with_vel) {
+                    vx0 += q.vx;
+                    vy0 += q.vy;
+                    vz0 += q.vz;
+                }
             }
             if (!en) return;
             if (me < NA) im = hm;
             sx = cx - x0; sy = cy - y0; sz = cz - z0;
             ix = hx + sx; iy = hy + sy; iz = hz + sz;
-            ivx = hvx + cvx; ivy = hvy + cvy; ivz = hvz + cvz;
-            svx = cvx - vx0; svy = cvy - vy0; svz = cvz - vz0;
+            if (with_vel) {
+                ivx = hvx + cvx; ivy = hvy + cvy; ivz = hvz + cvz;
+                svx = cvx - vx0; svy = cvy - vy0; svz = cvz - vz0;
+            }
         };
         // integrator_whfast.c:612-617 and :638-651
-        auto kepler_com = [&](double h) {
+        auto kepler_com = [&](double h, bool en) {
+            if (!en) return;
             if (live) warn |= kepler_solver(hx, hy, hz, hvx, hvy, hvz, sm * Gc, h);
             cx += h * cvx;
             cy += h * cvy;
             cz += h * cvz;
         };
-        // integrator_whfast.c:537-554 (masses are the inertial ones)
-        auto jump = [&](double h) {
-            publish_helio();
+        // integrator_whfast.c:537-554 (masses are the inertial ones, equal to
+        // p_h[i].m here: the engine changes no masses)
+        auto jump = [&](double h, bool en) {
+            publish(Slot{hm * hvx, hm * hvy, hm * hvz, 0., 0., 0., 0.});
             double qx = 0, qy = 0, qz = 0;
             GROUP_LOOP(k, 1, NA) {
                 const Slot q = slot[gbase + k - 1];
-                qx += q.m * q.vx;
-                qy += q.m * q.vy;
-                qz += q.m * q.vz;
+                qx += q.x;
+                qy += q.y;
+                qz += q.z;
             }
             const double m0 = sm;
-            if (live) {
+            if (live && en) {
                 hx += h * (qx / m0);
                 hy += h * (qy / m0);
                 hz += h * (qz / m0);
@@ -378,10 +389,10 @@ __global__ __launch_bounds__(64) void k_whfast_dh(WHParams P) {
         };
         // BASIC (gravity.c:215-232) or COMPENSATED (:308-410) with
         // gravity_ignore_terms = INVOLVING_0: planets only, ascending j.
-        auto gravity = [&]() {
+        auto gravity = [&](bool en) {
             publish_inertial();
+            if (!live || !en) return;
             ax = 0.; ay = 0.; az = 0.;
-            if (!live) return;
             double kx = 0., ky = 0., kz = 0.;
             for (int pass = 0; pass < 2; pass++) {
                 const int j0 = pass == 0 ? 1 : (NAG > 1 ? NAG : 1);
@@ -412,40 +423,53 @@ __global__ __launch_bounds__(64) void k_whfast_dh(WHParams P) {
                 }
             }
         };
-        // integrator_whfast.c:1051-1111 (default kernel, no correctors)
-        auto synchronize = [&]() {
-            const bool en = !sync;
-            if (en) kepler_com(dt / 2.);
-            to_inertial(en);
-            sync = 1;
+        // integrator_whfast.c:1051-1111 (default kernel, no correctors).  Every
+        // lane executes the barriers; `want` selects the systems that take part.
+        auto synchronize = [&](bool want) {
+            const bool en = want && !sync;
+            kepler_com(dt / 2., en);
+            to_inertial(en, true);
+            if (want) sync = 1;
+        };
+        auto from_inertial_if = [&](bool en) {
+            // the inertial -> heliocentric transform writes only when en
+            const double sx0 = cx, sy0 = cy, sz0 = cz, svx0 = cvx, svy0 = cvy, svz0 = cvz, sm0 = cm;
+            const double hx0 = hx, hy0 = hy, hz0 = hz, hvx0 = hvx, hvy0 = hvy, hvz0 = hvz, hm0 = hm;
+            from_inertial();
+            if (!en) {
+                cx = sx0; cy = sy0; cz = sz0; cvx = svx0; cvy = svy0; cvz = svz0; cm = sm0;
+                hx = hx0; hy = hy0; hz = hz0; hvx = hvx0; hvy = hvy0; hvz = hvz0; hm = hm0;
+            }
         };
 
-        int did_modify = P.did_modify;
+        // integrator_whfast.c:1113-1259 (DH, default kernel, no correctors)
+        bool modified = P.did_modify || ((flags0 >> 4) & 1u);
         for (long s = 0; s < P.nsteps; s++) {
-            if (P.safe_mode || did_modify) {
-                if (sync == 0) warn |= 2;
-                synchronize();
-                from_inertial();
+            const bool act = s < my_steps;
+            const bool recalc = act && (P.safe_mode || modified);
+            if (__any(recalc)) {
+                if (recalc && sync == 0) warn |= 2;
+                synchronize(recalc);
+                from_inertial_if(recalc);
             }
-            if (sync) {
-                kepler_com(dt / 2.);
-            } else {
-                kepler_com(dt);
-            }
-            jump(dt / 2.);
-            to_inertial(true);
-            gravity();
-            if (live) {
+            kepler_com(sync ? dt / 2. : dt, act);
+            jump(dt / 2., act);
+            // Inertial velocities of the middle of a step are visible only
+            // after the system's last step of the launch (or are replaced by a
+            // synchronization before anything reads them).
+            to_inertial(act, act && s == my_steps - 1);
+            gravity(act);
+            if (live && act) {
                 hvx += dt * ax;
                 hvy += dt * ay;
                 hvz += dt * az;
             }
-            jump(dt / 2.);
-            sync = 0;
-            if (P.safe_mode) synchronize();
-            did_modify = 0;
+            jump(dt / 2., act);
+            if (act) sync = 0;
+            if (P.safe_mode) synchronize(act);
+            if (act) modified = false;
         }
-        if (P.sync_at_end) synchronize();
+        synchronize(sys_ok && (P.sync_at_end || ((flags0 >> 3) & 1u)));
 
         // ---- store
         if (live) {
@@ -465,7 +489,7 @@ __global__ __launch_bounds__(64) void k_whfast_dh(WHParams P) {
             P.ph[0 * st + b0] = cx; P.ph[1 * st + b0] = cy; P.ph[2 * st + b0] = cz;
             P.ph[3 * st + b0] = cvx; P.ph[4 * st + b0] = cvy; P.ph[5 * st + b0] = cvz;
             P.ph[6 * st + b0] = cm;
-            P.flags[sidx] = (unsigned)(sync & 1) | (unsigned)(warn << 1);
+            P.flags[sidx] = (unsigned)(sync & 1) | (unsigned)((warn & 3) << 1);
         }
         __syncthreads();
     }
@@ -514,11 +538,18 @@ int whfast_energy_launch(const WHParams& P, double* out, int nsys, hipStream_t s
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
-// Group size for a system with K planets: smallest power of two >= K (>= 2).
-int whfast_group_size(int K) {
-    int g = 2;
-    while (g < K) g *= 2;
-    return g;
+// Lanes per system with K planets: one per planet, so a wavefront packs
+// floor(64 / K) systems of one planet count.
+int whfast_group_size(int K) { return K < 1 ? 1 : (K > 64 ? 64 : K); }
+
+// Persistent workgroups (one wavefront each) that fit on the device at once.
+int whfast_max_resident(int device) {
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_whfast_dh, 64, 0) != hipSuccess || per_cu < 1)
+        per_cu = 8;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1)
+        cus = 256;
+    return per_cu * cus;
 }
 
 // ---------------------------------------------------------------- per-simulation plugin
@@ -603,6 +634,7 @@ static bool whfast_prepare(struct reb_simulation* r, WHParams& P) {
     sys.Nact = (int)NA;
     sys.Nact_grav = (int)((r->N_active == SIZE_MAX) ? r->N : r->N_active);
     sys.tp_type = r->testparticle_type;
+    sys.steps = -1;
     sys.dt = r->dt;
     WHTask task{whfast_group_size((int)r->N - 1), 0, 1};
     struct Meta {

@@ -14,7 +14,7 @@ struct WHSys {
     int Nact;        // N_active used by the WHFast sums (N unless test particles of type 0)
     int Nact_grav;   // N_active used by the gravity loop
     int tp_type;     // testparticle_type
-    int pad;
+    int steps;       // steps in the current launch (-1: the launch's count)
     double dt;
 };
 
@@ -30,7 +30,9 @@ struct WHParams {
     const WHSys* sys;
     const WHTask* tasks;
     const int* order;
-    unsigned int* flags;      // per system: bit0 synchronized, bit1 period warning, bit2 resync warning
+    unsigned int* flags;      // per system: bit0 synchronized, bit1 period warning, bit2 resync warning;
+                              // set by the host: bit3 synchronize at the end of this launch,
+                              // bit4 particles modified (recompute the heliocentric state)
     unsigned int* counter;    // work counter of the persistent grid
     int ntasks;
     int grid;                 // persistent workgroups (one wavefront each)
@@ -49,5 +51,6 @@ struct WHParams {
 int whfast_launch(const WHParams& P, hipStream_t st);
 int whfast_energy_launch(const WHParams& P, double* out, int nsys, hipStream_t st);
 int whfast_group_size(int K);
+int whfast_max_resident(int device);
 
 }  // namespace rbamd

@@ -446,6 +446,127 @@ static void gen_sheet(void){
     reb_simulation_free(r);
 }
 
+/* Ensemble workload generator (the draw order of the product's
+ * planetary_systems and of oracle/ref_bench.c): K uniform in {kmin..kmax},
+ * then per planet period ratio (k>0), mass, e, inc, Omega, omega, f. */
+static struct reb_simulation* ens_system(struct reb_simulation* g, int kmin, int kmax){
+    struct reb_simulation* r = reb_simulation_create();
+    int K = kmin + (int)floor(reb_random_uniform(g, 0., 1.)*(double)(kmax-kmin+1));
+    if (K > kmax) K = kmax;
+    struct reb_particle star = {0}; star.m = 1.;
+    reb_simulation_add(r, star);
+    double a = 1.;
+    for (int k=0; k<K; k++){
+        if (k > 0){
+            double ratio = exp(reb_random_uniform(g, log(1.3), log(2.5)));
+            a *= pow(ratio, 2./3.);
+        }
+        double m  = exp(reb_random_uniform(g, log(1e-6), log(1e-3)));
+        double e  = reb_random_rayleigh(g, 0.02);
+        double in = reb_random_rayleigh(g, 0.01);
+        double Om = reb_random_uniform(g, 0., 2.*M_PI);
+        double om = reb_random_uniform(g, 0., 2.*M_PI);
+        double f  = reb_random_uniform(g, 0., 2.*M_PI);
+        reb_simulation_add(r, reb_particle_from_orbit(r->G, r->particles[0], m, a, e, in, Om, om, f));
+    }
+    reb_simulation_move_to_com(r);
+    return r;
+}
+
+/* An ensemble of NS systems: initial bodies, dt, energies after 0, n1, n1+n2,
+ * n1+n2+n3 steps (reb_simulation_steps calls) and the final bodies. */
+static void gen_ensemble(const char* tag, unsigned int seed, int NS, int kmin, int kmax, int safe, const int* nsteps){
+    struct reb_simulation* g = reb_simulation_create();
+    g->rand_seed = seed;
+    size_t cap = (size_t)NS*(kmax+1);
+    double* init = calloc(cap*NCOL, sizeof(double));
+    double* fin = calloc(cap*NCOL, sizeof(double));
+    int64_t* nb = calloc(NS, sizeof(int64_t));
+    double* dts = calloc(NS, sizeof(double));
+    double* en = calloc(NS*4, sizeof(double));
+    size_t row = 0;
+    for (int s=0; s<NS; s++){
+        struct reb_simulation* r = ens_system(g, kmin, kmax);
+        struct reb_integrator_whfast_state* wh = reb_simulation_set_integrator(r, "whfast");
+        wh->coordinates = REB_INTEGRATOR_WHFAST_COORDINATES_DEMOCRATICHELIOCENTRIC;
+        wh->safe_mode = safe;
+        r->dt = r->N > 1 ? 2.*M_PI*sqrt(1./(1.+r->particles[1].m))/20. : 1e-3;
+        nb[s] = (int64_t)r->N;
+        dts[s] = r->dt;
+        double* st = state_of(r);
+        memcpy(init+row*NCOL, st, sizeof(double)*NCOL*r->N);
+        free(st);
+        en[s*4] = reb_simulation_energy(r);
+        for (int c=0; c<3; c++){
+            reb_simulation_steps(r, (size_t)nsteps[c]);
+            en[s*4+1+c] = reb_simulation_energy(r);
+        }
+        st = state_of(r);
+        memcpy(fin+row*NCOL, st, sizeof(double)*NCOL*r->N);
+        free(st);
+        row += r->N;
+        reb_simulation_free(r);
+    }
+    char name[128];
+    snprintf(name, sizeof(name), "%s_init.npy", tag); wd(name, init, row, NCOL);
+    snprintf(name, sizeof(name), "%s_final.npy", tag); wd(name, fin, row, NCOL);
+    snprintf(name, sizeof(name), "%s_nbodies.npy", tag); wi(name, nb, NS, 0);
+    snprintf(name, sizeof(name), "%s_dt.npy", tag); wd(name, dts, NS, 0);
+    snprintf(name, sizeof(name), "%s_energy.npy", tag); wd(name, en, NS, 4);
+    free(init); free(fin); free(nb); free(dts); free(en);
+    reb_simulation_free(g);
+}
+
+/* reb_collision_resolve_merge on random three-body states: inputs, the
+ * collision (p1, p2), track_energy_offset and N_active; outputs the bodies,
+ * the outcome, energy_offset and G. */
+static void gen_merge(void){
+    const int NC = 2000;
+    struct reb_simulation* g = reb_simulation_create();
+    g->rand_seed = 77;
+    double* in = calloc((size_t)NC*3*NCOL, sizeof(double));
+    double* out = calloc((size_t)NC*3*NCOL, sizeof(double));
+    int64_t* meta = calloc((size_t)NC*4, sizeof(int64_t));
+    double* res = calloc((size_t)NC*3, sizeof(double));
+    for (int c=0; c<NC; c++){
+        struct reb_simulation* r = reb_simulation_create();
+        r->G = reb_random_uniform(g, 0.5, 2.);
+        for (int k=0; k<3; k++){
+            struct reb_particle p = {0};
+            p.x = reb_random_normal(g, 1.); p.y = reb_random_normal(g, 1.); p.z = reb_random_normal(g, 1.);
+            p.vx = reb_random_normal(g, 1.); p.vy = reb_random_normal(g, 1.); p.vz = reb_random_normal(g, 1.);
+            p.m = reb_random_uniform(g, 0.1, 3.);
+            p.r = reb_random_uniform(g, 0.01, 0.5);
+            reb_simulation_add(r, p);
+        }
+        int p1 = (int)(reb_random_uniform(g, 0., 3.)); if (p1 > 2) p1 = 2;
+        int p2 = (p1 + 1 + (int)(reb_random_uniform(g, 0., 2.))%2) % 3;
+        r->track_energy_offset = c % 2;
+        if (c % 3 == 0) r->N_active = 1;
+        r->energy_offset = 0.25;
+        double* st = state_of(r);
+        memcpy(in + (size_t)c*3*NCOL, st, sizeof(double)*3*NCOL);
+        free(st);
+        struct reb_collision col = {0};
+        col.p1 = p1; col.p2 = p2;
+        int outcome = (int)reb_collision_resolve_merge(r, col);
+        st = state_of(r);
+        memcpy(out + (size_t)c*3*NCOL, st, sizeof(double)*3*NCOL);
+        free(st);
+        meta[4*c] = p1; meta[4*c+1] = p2; meta[4*c+2] = r->track_energy_offset;
+        meta[4*c+3] = (r->N_active == SIZE_MAX) ? -1 : (int64_t)r->N_active;
+        res[3*c] = outcome; res[3*c+1] = r->energy_offset; res[3*c+2] = r->G;
+        reb_simulation_free(r);
+    }
+    size_t shape[3] = {NC, 3, NCOL};
+    write_npy("merge_in.npy", "<f8", 8, in, 3, shape);
+    write_npy("merge_out.npy", "<f8", 8, out, 3, shape);
+    wi("merge_meta.npy", meta, NC, 4);
+    wd("merge_res.npy", res, NC, 3);
+    free(in); free(out); free(meta); free(res);
+    reb_simulation_free(g);
+}
+
 int main(int argc, char** argv){
     if (argc < 3){ fprintf(stderr, "usage: gen_golden OUTDIR serial|omp\n"); return 1; }
     outdir = argv[1];
@@ -459,6 +580,13 @@ int main(int argc, char** argv){
     gen_ghostbox_basic("serial");
     gen_kepler();
     gen_whfast();
+    {
+        const int n_a[3] = {37, 1, 64};
+        gen_ensemble("ensemble200_seed99", 99u, 200, 1, 12, 0, n_a);
+        const int n_b[3] = {20, 5, 25};
+        gen_ensemble("ensemble8_big_seed5", 5u, 8, 30, 64, 1, n_b);
+    }
     gen_sheet();
+    gen_merge();
     return 0;
 }

@@ -1,0 +1,122 @@
+"""Batched WHFast ensembles against fixtures written by the reference library.
+
+Every system of an ensemble must evolve bit-identically to a reference
+reb_simulation integrating it alone with WHFast (democratic heliocentric,
+integrator_whfast.c:1113-1259) through reb_simulation_steps; the fixtures hold
+the reference's energies after each steps() call and its final bodies."""
+import numpy as np
+import pytest
+
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+
+def _fixture(tag):
+    return (golden(tag + "_nbodies.npy"), golden(tag + "_init.npy"), golden(tag + "_dt.npy"),
+            golden(tag + "_energy.npy"), golden(tag + "_final.npy"))
+
+
+CASES = {
+    # tag, safe_mode, the three steps() calls of the fixture
+    "ensemble200_seed99": (0, (37, 1, 64)),
+    "ensemble8_big_seed5": (1, (20, 5, 25)),
+}
+
+
+def _run(gpu, tag, launch=None):
+    safe, calls = CASES[tag]
+    nb, init, dt, en, fin = _fixture(tag)
+    ens = gpu.Ensemble(nb, init, dt, G=1.0, safe_mode=safe)
+    if launch:
+        ens.set_launch(**launch)
+    E = [ens.energy()]
+    for n in calls:
+        ens.steps(n)
+        E.append(ens.energy())
+    st = ens.state()
+    ens.free()
+    return np.stack(E, axis=1), st, en, fin
+
+
+@pytest.mark.parametrize("tag", sorted(CASES))
+def test_ensemble_bitwise(gpu, tag):
+    E, st, en, fin = _run(gpu, tag)
+    assert np.array_equal(E, en)
+    assert np.array_equal(st[:, :6], fin[:, :6])
+    assert np.array_equal(st[:, 9], fin[:, 9])
+
+
+@pytest.mark.parametrize("launch", [{"grid": 3, "chunk": 0}, {"grid": 0, "chunk": 7}, {"grid": 5, "chunk": 16}])
+def test_ensemble_launch_shapes_bitwise(gpu, launch):
+    # fewer persistent workgroups than tasks, and requests split into several
+    # graph launches, give the same bits
+    E, st, en, fin = _run(gpu, "ensemble200_seed99", launch)
+    assert np.array_equal(E, en)
+    assert np.array_equal(st[:, :6], fin[:, :6])
+
+
+@pytest.mark.parametrize("safe", [0, 1])
+def test_ensemble_whfast_dh32(gpu, safe):
+    # the per-simulation fixtures (steps(100) ten times), as one ensemble
+    init = golden("whfast_dh32_init.npy")
+    meta = golden("whfast_dh32_meta.npy")
+    dts = golden("whfast_dh32_dt.npy")
+    fin = golden("whfast_dh32_safe%d_final.npy" % safe)
+    en = golden("whfast_dh32_safe%d_energy.npy" % safe)
+    nb = meta[:, 1]
+    state = np.concatenate([init[s, :nb[s]] for s in range(len(nb))])
+    ens = gpu.Ensemble(nb, state, dts, safe_mode=safe)
+    E = [ens.energy()]
+    for _ in range(10):
+        ens.steps(100)
+        E.append(ens.energy())
+    assert np.array_equal(np.stack(E, axis=1), en)
+    st = ens.state()
+    for s in range(len(nb)):
+        assert np.array_equal(ens.system(s, st)[:, :6], fin[s, :nb[s], :6]), s
+
+
+def test_ensemble_tune_keeps_state(gpu):
+    nb, init, dt, en, fin = _fixture("ensemble200_seed99")
+    ens = gpu.Ensemble(nb, init, dt)
+    ens.steps(37)
+    before = ens.state()
+    best, shape = ens.tune(probe_steps=16)
+    assert best > 0 and shape["grid"] >= 1
+    assert np.array_equal(ens.state(), before)
+    assert ens.steps_done == 37
+    ens.steps(1)
+    ens.steps(64)
+    assert np.array_equal(ens.energy(), en[:, 3])
+    assert np.array_equal(ens.state()[:, :6], fin[:, :6])
+
+
+def test_ensemble_matches_per_simulation_plugin(gpu):
+    # the same systems through reb_simulation_steps one by one
+    nb, init, dt, _, _ = _fixture("ensemble200_seed99")
+    ens = gpu.Ensemble(nb, init, dt, gravity="compensated")
+    ens.steps(50)
+    st = ens.state()
+    first = np.concatenate([[0], np.cumsum(nb)[:-1]])
+    for s in range(0, len(nb), 23):
+        sim = gpu.Sim()
+        sim.set_state(init[first[s]:first[s] + nb[s]])
+        sim.s.dt = dt[s]
+        sim.s.gravity = gpu.GRAVITY["compensated"]
+        wh = sim.set_integrator("whfast")
+        wh.coordinates = gpu.WHFAST_COORDINATES["democraticheliocentric"]
+        wh.safe_mode = 0
+        sim.precision = "exact"
+        sim.steps(50)
+        assert np.array_equal(sim.state()[:, :6], st[first[s]:first[s] + nb[s], :6]), s
+        sim.free()
+
+
+def test_ensemble_errors(gpu):
+    with pytest.raises(gpu.RebError, match="1 to 65 bodies"):
+        gpu.Ensemble([66], np.zeros((66, 11)), 0.1)
+    with pytest.raises(gpu.RebError, match="BASIC or REB_GRAVITY_COMPENSATED"):
+        gpu.Ensemble([2], np.zeros((2, 11)), 0.1, gravity="tree")
+    with pytest.raises(ValueError):
+        gpu.Ensemble([3], np.zeros((2, 11)), 0.1)

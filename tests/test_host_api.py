@@ -178,3 +178,31 @@ def test_from_orbit_bitwise_vs_oracle(amd, orc):
         out = np.zeros(6)
         assert orc.lib().orc_particle_from_orbit(1.0, pr, 1e-3, a, e, el[0] / 2, el[1], el[2], el[3], out) == 0
         assert [p.x, p.y, p.z, p.vx, p.vy, p.vz] == list(out), k
+
+
+def test_merge_resolver_bitwise(amd):
+    """collision.c:674-737 against the compiled reference: merged body,
+    outcome and energy_offset (with and without track_energy_offset)."""
+    L = amd.lib()
+    xin = golden("merge_in.npy")
+    xout = golden("merge_out.npy")
+    meta = golden("merge_meta.npy")
+    res = golden("merge_res.npy")
+    sim = amd.Sim()
+    for _ in range(3):
+        sim.add(m=1.0)
+    P = sim.s.particles
+    names = ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "m", "r")
+    for c in range(xin.shape[0]):
+        for i in range(3):
+            for f, v in zip(names, xin[c, i]):
+                setattr(P[i], f, v)
+        sim.s.track_energy_offset = int(meta[c, 2])
+        sim.s.N_active = int(meta[c, 3]) if meta[c, 3] >= 0 else 2 ** 64 - 1
+        sim.s.energy_offset = 0.25
+        sim.s.G = res[c, 2]
+        out = L.reb_collision_resolve_merge(sim.ptr, amd.Collision(p1=int(meta[c, 0]), p2=int(meta[c, 1])))
+        got = np.array([[getattr(P[i], f) for f in names] for i in range(3)])
+        assert out == int(res[c, 0]), c
+        assert np.array_equal(got, xout[c]), c
+        assert sim.s.energy_offset == res[c, 1], c
