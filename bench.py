@@ -234,6 +234,171 @@ def ensemble_cpu(args):
             "max_rel_energy_error": r["max_rel_energy_error"], "mean_rel_energy_error": r["mean_rel_energy_error"]}
 
 
+def serving_jobs(args):
+    """Poisson arrivals (seeded): 70% planetary systems (star + K planets, K
+    uniform in {2..10}, generated as the ensemble's) for log-uniform 1e3..1e5
+    inner orbits (20 steps per orbit), 30% Plummer clusters N uniform in
+    {256..8192} for 100..2000 leapfrog steps in split precision."""
+    rng = np.random.default_rng(args.serve_seed)
+    t = 0.0
+    jobs = []
+    for k in range(args.serve_jobs):
+        t += rng.exponential(1.0 / args.serve_rate)
+        seed = 100000 + k
+        if rng.random() < 0.7:
+            orbits = float(np.exp(rng.uniform(np.log(1e3), np.log(1e5))))
+            jobs.append({"kind": "whfast", "t": t, "seed": seed, "steps": int(round(20 * orbits))})
+        else:
+            jobs.append({"kind": "nbody", "t": t, "seed": seed, "N": int(rng.integers(256, 8193)),
+                         "steps": int(rng.integers(100, 2001)), "softening": 0.01, "dt": 1e-3})
+    return jobs
+
+
+def serving_leg(amd, args):
+    jobs = serving_jobs(args)
+    # inputs are made before the clock starts
+    for j in jobs:
+        if j["kind"] == "whfast":
+            nb, st, dt = amd.planetary_systems(1, j["seed"], 2, 10)
+            j["state"], j["dt"] = st, float(dt[0])
+        else:
+            sim = amd.Sim()
+            sim.s.rand_seed = j["seed"]
+            sim.add_plummer(j["N"], 1.0, 1.0)
+            sim.move_to_com()
+            j["state"] = sim.state()
+            sim.free()
+    srv = amd.Server(memory_budget=args.serve_budget, max_nbody_jobs=args.serve_workers,
+                     tick_steps=args.serve_tick)
+    t0 = time.perf_counter()
+    ids = []
+    for j in jobs:
+        delay = j["t"] - (time.perf_counter() - t0)
+        if delay > 0:
+            time.sleep(delay)
+        if j["kind"] == "whfast":
+            ids.append(srv.submit_whfast(j["state"], j["dt"], steps=j["steps"]))
+        else:
+            ids.append(srv.submit_nbody(j["state"], softening=j["softening"], dt=j["dt"], steps=j["steps"],
+                                        precision="split"))
+    lat, derr = {"whfast": [], "nbody": []}, {"whfast": [], "nbody": []}
+    t_first, t_last = None, 0.0
+    for j, jid in zip(jobs, ids):
+        if srv.wait(jid, 600) != "done":
+            raise RuntimeError("serving job failed: " + srv.job_error(jid))
+        _, (e0, e1), (ts, ta, td) = srv.result(jid)
+        lat[j["kind"]].append(td - ts)
+        derr[j["kind"]].append(abs((e1 - e0) / e0))
+        t_first = ts if t_first is None else min(t_first, ts)
+        t_last = max(t_last, td)
+    stats = srv.stats()
+    srv.close()
+    all_lat = np.array(lat["whfast"] + lat["nbody"])
+    span = t_last - t_first
+    res = {
+        "metric": "serving jobs/s and p50/p99 latency (70% WHFast K 2..10, 30% Plummer N 256..8192 split)",
+        "jobs": len(jobs), "jobs_per_s": len(jobs) / span, "span_s": span,
+        "arrival_rate": args.serve_rate,
+        "latency_s": {"p50": float(np.quantile(all_lat, 0.5)), "p99": float(np.quantile(all_lat, 0.99)),
+                      "max": float(all_lat.max())},
+        "latency_by_kind_s": {k: {"p50": float(np.quantile(v, 0.5)), "p99": float(np.quantile(v, 0.99)), "n": len(v)}
+                              for k, v in lat.items() if v},
+        "rel_energy_error": {k: {"median": float(np.median(v)), "max": float(np.max(v))} for k, v in derr.items() if v},
+        "device_pool_busy_s": stats["pool_busy_s"], "pool_ticks": int(stats["ticks"]),
+        "peak_bytes": stats["peak_bytes"], "budget": stats["budget"],
+    }
+    log("serving: %d jobs in %.2f s -> %.2f jobs/s; latency p50 %.3f s p99 %.3f s" % (
+        len(jobs), span, res["jobs_per_s"], res["latency_s"]["p50"], res["latency_s"]["p99"]))
+    if not args.no_cpu:
+        res["cpu_baseline"] = serving_cpu(args, jobs)
+        if res["cpu_baseline"]:
+            res["speedup_vs_cpu"] = res["jobs_per_s"] / res["cpu_baseline"]["value"]
+            log("serving cpu: %.3f jobs/s projected on %d threads" % (res["cpu_baseline"]["value"],
+                                                                      res["cpu_baseline"]["cores"]))
+    return res
+
+
+def serving_cpu(args, jobs):
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_bench")
+    if not os.path.exists(exe):
+        return None
+    import tempfile
+    with tempfile.NamedTemporaryFile("w", suffix=".jobs", delete=False) as f:
+        for j in jobs:
+            if j["kind"] == "whfast":
+                f.write("whfast %d %d\n" % (j["seed"], j["steps"]))
+            else:
+                f.write("nbody %d %d %d %r %r\n" % (j["seed"], j["N"], j["steps"], j["softening"], j["dt"]))
+        path = f.name
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    cmd = [exe, "serve", path, repr(float(args.serve_cpu_scale))]
+    log("serving cpu baseline:", " ".join(cmd), "threads", threads)
+    out = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=900)
+    os.unlink(path)
+    if out.returncode != 0:
+        log(out.stderr)
+        return None
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    return {"value": r["jobs_per_s"], "unit": "jobs/s", "cores": r["threads"], "kind": "reference",
+            "sample": "reference OpenMP build, the same %d jobs (N-body in FP64 BASIC: the reference has no split "
+                      "mode), one job per thread, every job for 1/%g of its steps (%.1f s), projected to the full "
+                      "steps" % (r["jobs"], r["scale"], r["seconds"])}
+
+
+def sheet_leg(amd, args):
+    """Tree gravity + tree collisions: shearing sheet of N=200000 hard spheres
+    (10 x 10 root boxes of 136 radii: optical depth ~1), SEI, 2 ghost boxes,
+    opening angle 0.5, restitution 0.5; `sheet_steps` timed steps."""
+    import ctypes
+    L = amd.lib()
+    t_gen = time.perf_counter()
+    sim = amd.shearing_sheet(args.sheet_n, args.sheet_root, args.sheet_nroot, args.sheet_seed)
+    log("sheet: N=%d generated in %.1f s" % (args.sheet_n, time.perf_counter() - t_gen))
+    L.reb_amd_set_resident(sim.ptr, 1)
+    sim.steps(1)                                   # warm-up (allocations, first tree)
+    L.reb_amd_synchronize_device(sim.ptr)
+    n0 = sim.s.collisions_log_n
+    t0 = time.perf_counter()
+    sim.steps(args.sheet_steps)
+    L.reb_amd_synchronize_device(sim.ptr)
+    wall = time.perf_counter() - t0
+    ncoll = sim.s.collisions_log_n - n0
+    inter = L.reb_amd_tree_interactions(sim.ptr)
+    L.reb_amd_set_resident(sim.ptr, 0)
+    sim.check()
+    res = {"metric": "shearing sheet steps/s (tree gravity + tree collisions, N=%d)" % args.sheet_n,
+           "value": args.sheet_steps / wall, "unit": "steps/s", "steps": args.sheet_steps, "seconds": wall,
+           "ms_per_step": 1e3 * wall / args.sheet_steps, "collisions_per_step": ncoll / args.sheet_steps,
+           "tree_interactions_last_step": int(inter),
+           "config": {"N": args.sheet_n, "root_size": args.sheet_root, "n_root": args.sheet_nroot,
+                      "ghost_boxes": 2, "opening_angle2": 0.25, "seed": args.sheet_seed}}
+    log("sheet: %.2f steps/s (%.1f ms/step), %.0f collisions/step, %d tree interactions/step" % (
+        res["value"], res["ms_per_step"], res["collisions_per_step"], inter))
+    sim.free()
+    if not args.no_cpu:
+        exe = os.path.join(ROOT, "oracle", "_ref", "ref_bench")
+        if os.path.exists(exe):
+            threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+            env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+            cmd = [exe, "sheet", str(args.sheet_n), str(args.sheet_cpu_steps), repr(float(args.sheet_root)),
+                   str(args.sheet_nroot), str(args.sheet_seed)]
+            log("sheet cpu baseline:", " ".join(cmd), "threads", threads)
+            out = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=900)
+            if out.returncode == 0:
+                r = json.loads(out.stdout.strip().splitlines()[-1])
+                res["cpu_baseline"] = {
+                    "value": r["steps_per_s"], "unit": "steps/s", "cores": r["threads"], "kind": "reference",
+                    "sample": "reference OpenMP build, the same sheet, %d steps after 1 warm-up step, %.1f s "
+                              "(%.0f collisions/step)" % (r["steps"], r["seconds"], r["collisions_per_step"])}
+                res["speedup_vs_cpu"] = res["value"] / r["steps_per_s"]
+                log("sheet cpu: %.3f steps/s on %d threads -> GPU %.0fx" % (r["steps_per_s"], r["threads"],
+                                                                            res["speedup_vs_cpu"]))
+            else:
+                log(out.stderr)
+    return res
+
+
 def copy_calibration(mib, reps=10):
     """Device-to-device copies of `mib` MiB: the reference reading of the
     FETCH_SIZE / WRITE_SIZE counters for a known byte count."""
@@ -268,7 +433,20 @@ def main():
     ap.add_argument("--ens-seed", type=int, default=1)
     ap.add_argument("--ens-probe", type=int, default=500, help="steps per autotuner probe")
     ap.add_argument("--ens-cpu-steps", type=int, default=500)
-    ap.add_argument("--only", choices=["direct", "ensemble"], default=None)
+    ap.add_argument("--serve-jobs", type=int, default=200)
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="job arrivals per second")
+    ap.add_argument("--serve-seed", type=int, default=7)
+    ap.add_argument("--serve-budget", type=int, default=8 << 30, help="device-memory budget in bytes")
+    ap.add_argument("--serve-workers", type=int, default=8, help="concurrent N-body jobs")
+    ap.add_argument("--serve-tick", type=int, default=2000, help="planetary steps per pool tick")
+    ap.add_argument("--serve-cpu-scale", type=float, default=50.0)
+    ap.add_argument("--sheet-n", type=int, default=200000)
+    ap.add_argument("--sheet-root", type=float, default=136.0)
+    ap.add_argument("--sheet-nroot", type=int, default=10)
+    ap.add_argument("--sheet-seed", type=int, default=99)
+    ap.add_argument("--sheet-steps", type=int, default=500)
+    ap.add_argument("--sheet-cpu-steps", type=int, default=3)
+    ap.add_argument("--only", choices=["direct", "ensemble", "serve", "sheet"], default=None)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device (counter calibration)")
     args = ap.parse_args()
@@ -287,6 +465,12 @@ def main():
     if args.only == "ensemble":
         print(json.dumps({"ensemble": ensemble_leg(amd, args)}))
         return
+    if args.only == "serve":
+        print(json.dumps({"serving": serving_leg(amd, args)}))
+        return
+    if args.only == "sheet":
+        print(json.dumps({"sheet": sheet_leg(amd, args)}))
+        return
     head, _ = run_mode(amd, args, "fp64", "fp64", "basic", args.steps, args.warmup)
     modes = {"fp64": head}
     if not args.quick:
@@ -299,6 +483,8 @@ def main():
             modes[k]["accuracy"] = v
     cpu = None if args.no_cpu else cpu_baseline(args)
     ens = None if (args.quick or args.only == "direct") else ensemble_leg(amd, args)
+    serving = None if (args.quick or args.only == "direct") else serving_leg(amd, args)
+    sheet = None if (args.quick or args.only == "direct") else sheet_leg(amd, args)
 
     achieved = head["algorithmic_tflops"]
     line = {
@@ -324,6 +510,8 @@ def main():
         "cpu_baseline": cpu,
         "modes": modes,
         "ensemble": ens,
+        "serving": serving,
+        "sheet": sheet,
     }
     print(json.dumps(line))
 

@@ -60,6 +60,13 @@ REB_API void reb_amd_profile(const struct reb_simulation* r, double* out4);
  * since profiling was switched on or last read. */
 REB_API int reb_amd_kernel_time(struct reb_simulation* r, double* total_ms, int* launches);
 
+/* ------------------------------------------------------------ collisions
+ * Sets r->coefficient_of_restitution to a C callback that returns `eps` for
+ * every impact velocity (the reference's callback, integrator-agnostic, with
+ * no per-collision call into a scripting language). */
+REB_API void reb_amd_set_coefficient_of_restitution(struct reb_simulation* r, double eps);
+REB_API double reb_amd_coefficient_of_restitution_constant(const struct reb_simulation* const r, double v);
+
 /* ------------------------------------------------------------ residency
  * By default every reb_simulation_steps / integrate / update_acceleration /
  * energy call uploads r->particles first and downloads it on return (host
@@ -102,6 +109,9 @@ REB_API int reb_amd_kepler_solver_batch(double* state6, const double* mu, const 
  * leaf, minus the body count of an internal cell), exactly the fields of
  * struct reb_treecell (tree.h:34-56).  Returns the number of cells or -1. */
 REB_API long reb_amd_tree_dump(struct reb_simulation* r, double* out, long cap);
+/* Particle-node interactions (accepted cells and leaves, over all ghost boxes)
+ * of the last REB_GRAVITY_TREE evaluation; -1 if there was none. */
+REB_API long long reb_amd_tree_interactions(struct reb_simulation* r);
 
 /* ---------------------------------------------------------------------------
  * Batched ensembles: many independent planetary systems integrated together
@@ -145,6 +155,52 @@ REB_API int reb_amd_ensemble_get_launch(const struct reb_amd_ensemble* e, int* g
  * afterwards) and keeps the fastest; best_ms receives its time. */
 REB_API int reb_amd_ensemble_tune(struct reb_amd_ensemble* e, long probe_steps, double* best_ms);
 REB_API const char* reb_amd_ensemble_error(const struct reb_amd_ensemble* e);
+
+/* ---------------------------------------------------------------------------
+ * Serving: a job queue in front of the GPU.  Jobs are admitted in arrival
+ * order while their device-memory footprints fit in the budget (later, smaller
+ * jobs may pass one that does not fit yet).  Planetary jobs share one device
+ * pool stepped in ticks of `whfast_tick_steps` (continuous batching: jobs join
+ * and leave at tick boundaries); N-body jobs run on their own streams from up
+ * to `max_nbody_jobs` worker threads.  A job's result equals that of the same
+ * simulation run alone through reb_simulation_steps.  No counterpart in the
+ * reference, whose unit of work is one reb_simulation per caller.
+ * ------------------------------------------------------------------------- */
+enum REB_AMD_JOB_STATE {
+    REB_AMD_JOB_UNKNOWN = -2,
+    REB_AMD_JOB_FAILED = -1,
+    REB_AMD_JOB_PENDING = 0,
+    REB_AMD_JOB_RUNNING = 1,
+    REB_AMD_JOB_DONE = 2,
+};
+struct reb_amd_server;
+
+/* memory_budget_bytes 0: 8 GiB; max_nbody_jobs 0: 8; whfast_tick_steps 0: 2000. */
+REB_API struct reb_amd_server* reb_amd_server_create(size_t memory_budget_bytes, int max_nbody_jobs,
+                                                     long whfast_tick_steps);
+/* Cancels pending jobs, waits for running ones, frees everything. */
+REB_API void reb_amd_server_free(struct reb_amd_server* s);
+/* WHFast (democratic heliocentric, default kernel) on a system of 1..65 bodies,
+ * central body first, for n_steps steps ending synchronized.  Job id or -1. */
+REB_API long reb_amd_server_submit_whfast(struct reb_amd_server* s, size_t n_bodies, const struct reb_particle* bodies,
+                                          double dt, double G, int safe_mode, uint64_t n_steps);
+/* Leapfrog with direct gravity (REB_GRAVITY_BASIC or _COMPENSATED) in the given
+ * REB_AMD_PRECISION for n_steps steps.  Job id or -1. */
+REB_API long reb_amd_server_submit_nbody(struct reb_amd_server* s, size_t n, const struct reb_particle* bodies,
+                                         double G, double softening, double dt, uint64_t n_steps, int gravity,
+                                         int precision);
+REB_API int reb_amd_server_status(struct reb_amd_server* s, long job);
+/* Waits for completion (timeout_s < 0: no limit); returns the job state. */
+REB_API int reb_amd_server_wait(struct reb_amd_server* s, long job, double timeout_s);
+/* Copies a finished job's bodies (n must equal the submitted count), its
+ * energies before and after, and its submit/admit/done times (seconds since
+ * the server was created), then forgets the job.  0 on success. */
+REB_API int reb_amd_server_result(struct reb_amd_server* s, long job, struct reb_particle* out, size_t n,
+                                  double* energy2, double* times3);
+REB_API const char* reb_amd_server_job_error(struct reb_amd_server* s, long job);
+/* out[0..9]: jobs held, pending, running, done, failed, bytes in use, peak
+ * bytes in use, budget, pool ticks, pool busy seconds. */
+REB_API int reb_amd_server_stats(struct reb_amd_server* s, double* out, int n);
 
 #ifdef __cplusplus
 }

@@ -190,3 +190,5 @@ class Sim:
 
 
 from .ensemble import Ensemble, planetary_systems  # noqa: E402
+from .serve import Server  # noqa: E402
+from .workloads import shearing_sheet  # noqa: E402

@@ -107,6 +107,7 @@ Simulation._fields_ = [
 ]
 
 # enum values (rebound.h:209-233, 372-393; integrator_whfast.h:31-45)
+JOB_STATE = {-2: "unknown", -1: "failed", 0: "pending", 1: "running", 2: "done"}
 GRAVITY = {"none": 0, "basic": 1, "compensated": 2, "tree": 3, "jacobi": 5, "custom": 7}
 COLLISION = {"none": 0, "direct": 1, "tree": 2, "line": 4, "linetree": 5}
 BOUNDARY = {"none": 0, "open": 1, "periodic": 2, "shear": 3}
@@ -165,9 +166,12 @@ def declare(lib):
                                                            c_int, c_int, c_void_p, c_void_p]),
         "reb_amd_kepler_solver_batch": (c_int, [POINTER(c_double), POINTER(c_double), POINTER(c_double), c_size_t]),
         "reb_amd_tree_dump": (ctypes.c_long, [S, POINTER(c_double), ctypes.c_long]),
+        "reb_amd_tree_interactions": (ctypes.c_longlong, [S]),
         "reb_amd_set_profiling": (None, [S, c_int]),
         "reb_amd_kernel_time": (c_int, [S, POINTER(c_double), POINTER(c_int)]),
         "reb_amd_set_resident": (None, [S, c_int]),
+        "reb_amd_set_coefficient_of_restitution": (None, [S, c_double]),
+        "reb_amd_coefficient_of_restitution_constant": (c_double, [S, c_double]),
         "reb_amd_sync_host": (c_int, [S]),
         "reb_amd_mark_host_modified": (None, [S]),
         "reb_amd_synchronize_device": (c_int, [S]),
@@ -187,6 +191,18 @@ def declare(lib):
                                                 POINTER(c_int)]),
         "reb_amd_ensemble_tune": (c_int, [c_void_p, ctypes.c_long, POINTER(c_double)]),
         "reb_amd_ensemble_error": (c_char_p, [c_void_p]),
+        "reb_amd_server_create": (c_void_p, [c_size_t, c_int, ctypes.c_long]),
+        "reb_amd_server_free": (None, [c_void_p]),
+        "reb_amd_server_submit_whfast": (ctypes.c_long, [c_void_p, c_size_t, POINTER(Particle), c_double, c_double,
+                                                         c_int, ctypes.c_uint64]),
+        "reb_amd_server_submit_nbody": (ctypes.c_long, [c_void_p, c_size_t, POINTER(Particle), c_double, c_double,
+                                                        c_double, ctypes.c_uint64, c_int, c_int]),
+        "reb_amd_server_status": (c_int, [c_void_p, ctypes.c_long]),
+        "reb_amd_server_wait": (c_int, [c_void_p, ctypes.c_long, c_double]),
+        "reb_amd_server_result": (c_int, [c_void_p, ctypes.c_long, POINTER(Particle), c_size_t, POINTER(c_double),
+                                          POINTER(c_double)]),
+        "reb_amd_server_job_error": (c_char_p, [c_void_p, ctypes.c_long]),
+        "reb_amd_server_stats": (c_int, [c_void_p, POINTER(c_double), c_int]),
     }
     for name, (res, args) in sig.items():
         if hasattr(lib, name):

@@ -232,6 +232,9 @@ REB_API struct reb_amd_ensemble* reb_amd_ensemble_create(size_t n_systems, const
         sys[s].Nact_grav = (int)n_bodies[s];
         sys[s].tp_type = 0;
         sys[s].steps = -1;
+        sys[s].req = 0;
+        sys[s].pad = 0;
+        sys[s].G = G;
         sys[s].dt = dt[s];
         order[s] = (int)s;
     }

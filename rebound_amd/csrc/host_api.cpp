@@ -277,8 +277,21 @@ REB_API struct reb_simulation* reb_simulation_copy(struct reb_simulation* r) {
         }
     }
     dev_of(c)->precision = dev_of(r)->precision;
+    dev_of(c)->cor_constant = dev_of(r)->cor_constant;
     c->did_modify_particles = 1;
     return c;
+}
+
+// A constant coefficient of restitution as a C callback, so the host resolver
+// (collision.c:573-665) does not call back into an interpreter per collision.
+REB_API double reb_amd_coefficient_of_restitution_constant(const struct reb_simulation* const r, double v) {
+    (void)v;
+    return dev_of(r)->cor_constant;
+}
+
+REB_API void reb_amd_set_coefficient_of_restitution(struct reb_simulation* r, double eps) {
+    dev_of(r)->cor_constant = eps;
+    r->coefficient_of_restitution = reb_amd_coefficient_of_restitution_constant;
 }
 
 // ===================================================================== particles

@@ -100,6 +100,8 @@ struct DeviceState {
     // download r->particles; reb_amd_sync_host / reb_amd_mark_host_modified
     // move the state explicitly
     bool resident = false;
+    // value returned by reb_amd_coefficient_of_restitution_constant
+    double cor_constant = 1.0;
     double* field(int f) const { return soa.as<double>() + (size_t)f * cap; }
 };
 

@@ -1,0 +1,791 @@
+// Serving loop: a job queue in front of the GPU with admission by memory
+// footprint and continuous batching.
+//
+// Jobs.  Two kinds, each the reference's own entry point applied to one
+// simulation: a planetary system integrated by WHFast (democratic
+// heliocentric, reb_simulation_steps: simulation.c:504-513,
+// integrator_whfast.c:1113-1259) and an N-body cluster integrated by leapfrog
+// with direct gravity (integrator_leapfrog.c:97-209, gravity.c:167-531) in a
+// chosen precision.  A job's result is what the reference returns for that
+// simulation alone; it does not depend on which other jobs share the GPU.
+//
+// Admission.  Every job has a device-memory footprint (its state, scratch and
+// metadata).  The scheduler admits pending jobs in arrival order while their
+// footprints fit in the budget; a job that does not fit waits, and smaller
+// later jobs may pass it.  Memory is returned when a job completes.
+//
+// Continuous batching.  Planetary systems live in one device pool (a
+// structure-of-arrays arena, see ensemble.hip for the layout).  The scheduler
+// runs the pool in ticks: each tick is one persistent-kernel launch that
+// advances every active system by min(remaining, tick) steps; systems that
+// reach their horizon stop inside the launch and synchronize at its end.
+// Jobs admitted while a tick runs join at the next tick; finished systems
+// leave, and their slots and bodies are reused.  N-body jobs run on their own
+// streams from worker threads (one simulation each, state resident in HBM),
+// so their kernels overlap with the pool's and with each other on the device.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "internal.h"
+#include "whfast_device.h"
+
+namespace rbamd {
+
+enum JobKind { JOB_WHFAST = 0, JOB_NBODY = 1 };
+
+struct Job {
+    long id = 0;
+    int kind = JOB_WHFAST;
+    std::vector<struct reb_particle> bodies;   // input, then result
+    double dt = 0., G = 1., softening = 0.;
+    int safe_mode = 0, gravity = REB_GRAVITY_BASIC, precision = REB_AMD_PRECISION_FP64;
+    uint64_t n_steps = 0;
+    size_t footprint = 0;
+    int state = REB_AMD_JOB_PENDING;
+    std::string error;
+    double t_submit = 0., t_admit = 0., t_done = 0.;
+    double e0 = 0., e1 = 0.;
+    // pool bookkeeping (scheduler thread only)
+    int slot = -1;
+    size_t body0 = 0;
+    uint64_t remaining = 0;
+    bool fresh = false;
+};
+
+// Pinned staging buffer that grows.
+struct Pinned {
+    void* p = nullptr;
+    size_t bytes = 0;
+    bool ensure(size_t need) {
+        if (need <= bytes && p) return true;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+        size_t want = std::max<size_t>(need, 1 << 16);
+        if (hipHostMalloc(&p, want) != hipSuccess) {
+            p = nullptr;
+            return false;
+        }
+        bytes = want;
+        return true;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
+__global__ void k_pool_scatter(const double* __restrict__ stage, const int* __restrict__ dest, int nb,
+                               double* __restrict__ body, double* __restrict__ ph, size_t stride) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nb) return;
+    const size_t b = (size_t)dest[k];
+    for (int f = 0; f < 10; f++) body[f * stride + b] = stage[10 * (size_t)k + f];
+    for (int f = 0; f < 7; f++) ph[f * stride + b] = 0.;
+}
+
+__global__ void k_pool_set_flags(const int* __restrict__ slots, int ns, unsigned int* __restrict__ flags,
+                                 unsigned int value) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < ns) flags[slots[k]] = value;
+}
+
+__global__ void k_pool_gather(const int* __restrict__ src, int nb, const double* __restrict__ body, size_t stride,
+                              double* __restrict__ out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nb) return;
+    const size_t b = (size_t)src[k];
+    for (int f = 0; f < 10; f++) out[10 * (size_t)k + f] = body[f * stride + b];
+}
+
+// Device arena of planetary systems.
+struct Pool {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t cap = 0;                  // body capacity (= field stride)
+    size_t nslots = 0;               // system-slot capacity
+    DevBuf body, ph;                 // 10 and 7 fields of cap doubles
+    DevBuf sys, flags, energy;       // per slot
+    DevBuf order, tasks, counter;    // per tick
+    DevBuf idx;                      // scatter/gather indices
+    DevBuf stage;                    // device side of scatter/gather data
+    Pinned h_stage, h_meta;
+    std::vector<WHSys> hsys;
+    std::vector<Job*> slot_job;
+    std::vector<int> free_slots;
+    std::map<size_t, size_t> free_ranges;   // start -> length
+    int max_resident = 0;
+    unsigned long long ticks = 0;
+    double busy_s = 0.;
+};
+
+struct Server {
+    int device = 0;
+    size_t budget = 0, used = 0, peak_used = 0;
+    int max_nbody = 4, nbody_running = 0;
+    long tick_steps = 2000;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Job*> pending;
+    std::unordered_map<long, std::unique_ptr<Job>> jobs;
+    long next_id = 1;
+    unsigned long long generation = 0;   // bumped on submit and on completion
+    bool stop = false;
+    std::thread sched;
+    std::vector<std::thread> workers;    // N-body workers, max_nbody of them
+    std::deque<Job*> nbody_ready;        // admitted N-body jobs waiting for a worker
+    std::chrono::steady_clock::time_point t0;
+    std::vector<Job*> pool_active;       // scheduler thread only
+    Pool pool;
+    std::string err;
+    unsigned long long done = 0, failed = 0;
+};
+
+static double since(const Server* S) {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - S->t0).count();
+}
+
+static size_t whfast_footprint(size_t n) { return n * 17 * sizeof(double) + sizeof(WHSys) + 64; }
+
+static size_t nbody_footprint(size_t n, int precision) {
+    size_t cap = 64;
+    while (cap < n) cap *= 2;
+    return NFIELD * cap * sizeof(double) + reb_amd_gravity_workspace_bytes(n, precision) + 16 * n + 4096;
+}
+
+// ------------------------------------------------------------------ pool memory
+static bool pool_fail(Server* S, const char* what, hipError_t e) {
+    S->err = std::string("serving pool: ") + what + ": " + hipGetErrorString(e);
+    set_last_error(S->err);
+    return false;
+}
+#define POOL_HIP(S, call)                                          \
+    do {                                                           \
+        hipError_t _e = (call);                                    \
+        if (_e != hipSuccess) return pool_fail((S), #call, _e);    \
+    } while (0)
+
+static bool pool_grow_bodies(Server* S, size_t need) {
+    Pool& P = S->pool;
+    size_t cap = std::max<size_t>(P.cap ? P.cap * 2 : 4096, 64);
+    while (cap < need) cap *= 2;
+    DevBuf nb, np;
+    if (!nb.ensure(sizeof(double) * 10 * cap) || !np.ensure(sizeof(double) * 7 * cap))
+        return pool_fail(S, "out of device memory for the system pool", hipErrorOutOfMemory);
+    if (P.cap) {
+        for (int f = 0; f < 10; f++)
+            POOL_HIP(S, hipMemcpyAsync(nb.as<double>() + f * cap, P.body.as<double>() + f * P.cap,
+                                       sizeof(double) * P.cap, hipMemcpyDeviceToDevice, P.stream));
+        for (int f = 0; f < 7; f++)
+            POOL_HIP(S, hipMemcpyAsync(np.as<double>() + f * cap, P.ph.as<double>() + f * P.cap,
+                                       sizeof(double) * P.cap, hipMemcpyDeviceToDevice, P.stream));
+        POOL_HIP(S, hipStreamSynchronize(P.stream));
+    }
+    P.body.release();
+    P.ph.release();
+    P.body = nb;
+    P.ph = np;
+    nb.p = np.p = nullptr;
+    // the new tail is free; merge with a free range ending at the old capacity
+    size_t start = P.cap, len = cap - P.cap;
+    auto it = P.free_ranges.lower_bound(start);
+    if (it != P.free_ranges.begin()) {
+        auto pv = std::prev(it);
+        if (pv->first + pv->second == start) {
+            start = pv->first;
+            len += pv->second;
+            P.free_ranges.erase(pv);
+        }
+    }
+    P.free_ranges[start] = len;
+    P.cap = cap;
+    return true;
+}
+
+static bool pool_grow_slots(Server* S, size_t need) {
+    Pool& P = S->pool;
+    size_t n = P.nslots ? P.nslots * 2 : 1024;
+    while (n < need) n *= 2;
+    DevBuf nf;
+    if (!nf.ensure(sizeof(unsigned int) * n)) return pool_fail(S, "out of device memory", hipErrorOutOfMemory);
+    if (P.nslots)
+        POOL_HIP(S, hipMemcpyAsync(nf.p, P.flags.p, sizeof(unsigned int) * P.nslots, hipMemcpyDeviceToDevice,
+                                   P.stream));
+    POOL_HIP(S, hipStreamSynchronize(P.stream));
+    P.flags.release();
+    P.flags = nf;
+    nf.p = nullptr;
+    if (!P.sys.ensure(sizeof(WHSys) * n) || !P.energy.ensure(sizeof(double) * n) ||
+        !P.order.ensure(sizeof(int) * n) || !P.tasks.ensure(sizeof(WHTask) * n))
+        return pool_fail(S, "out of device memory", hipErrorOutOfMemory);
+    WHSys empty{};
+    empty.steps = 0;
+    P.hsys.resize(n, empty);
+    P.slot_job.resize(n, nullptr);
+    for (size_t s = n; s-- > P.nslots;) P.free_slots.push_back((int)s);
+    P.nslots = n;
+    return true;
+}
+
+static bool alloc_range(Pool& P, size_t n, size_t* start) {
+    for (auto it = P.free_ranges.begin(); it != P.free_ranges.end(); ++it) {
+        if (it->second >= n) {
+            *start = it->first;
+            const size_t rest = it->second - n, nstart = it->first + n;
+            P.free_ranges.erase(it);
+            if (rest) P.free_ranges[nstart] = rest;
+            return true;
+        }
+    }
+    return false;
+}
+
+static void free_range(Pool& P, size_t start, size_t n) {
+    auto it = P.free_ranges.lower_bound(start);
+    if (it != P.free_ranges.end() && start + n == it->first) {
+        n += it->second;
+        it = P.free_ranges.erase(it);
+    }
+    if (it != P.free_ranges.begin()) {
+        auto pv = std::prev(it);
+        if (pv->first + pv->second == start) {
+            pv->second += n;
+            return;
+        }
+    }
+    P.free_ranges[start] = n;
+}
+
+static WHParams pool_params(Server* S, int ntasks, long nsteps) {
+    Pool& P = S->pool;
+    WHParams W;
+    W.sys = P.sys.as<WHSys>();
+    W.tasks = P.tasks.as<WHTask>();
+    W.order = P.order.as<int>();
+    W.flags = P.flags.as<unsigned int>();
+    W.counter = P.counter.as<unsigned int>();
+    W.ntasks = ntasks;
+    W.grid = std::max(1, std::min(ntasks, P.max_resident));
+    W.body = P.body.as<double>();
+    W.ph = P.ph.as<double>();
+    W.stride = P.cap;
+    W.G = 1.;   // per system: WHSys::G
+    W.soft2 = 0.;
+    W.compensated = 0;
+    W.nsteps = nsteps;
+    W.safe_mode = 0;
+    W.did_modify = 0;
+    W.sync_at_end = 0;
+    return W;
+}
+
+// Places newly admitted planetary systems into the pool.
+static bool pool_admit(Server* S, const std::vector<Job*>& fresh) {
+    Pool& P = S->pool;
+    size_t nb = 0;
+    for (Job* j : fresh) {
+        const size_t n = j->bodies.size();
+        size_t start;
+        if (!alloc_range(P, n, &start)) {
+            size_t need = P.cap + n;
+            if (!pool_grow_bodies(S, need) || !alloc_range(P, n, &start)) return false;
+        }
+        if (P.free_slots.empty() && !pool_grow_slots(S, P.nslots + 1)) return false;
+        const int slot = P.free_slots.back();
+        P.free_slots.pop_back();
+        j->slot = slot;
+        j->body0 = start;
+        j->remaining = j->n_steps;
+        j->fresh = true;
+        WHSys& w = P.hsys[slot];
+        w.body0 = (int)start;
+        w.N = (int)n;
+        w.Nact = (int)n;
+        w.Nact_grav = (int)n;
+        w.tp_type = 0;
+        w.steps = 0;
+        w.req = 0;
+        w.pad = 0;
+        w.dt = j->dt;
+        w.G = j->G;
+        P.slot_job[slot] = j;
+        nb += n;
+    }
+    // stage bodies (10 doubles each) and destinations, then scatter on the device
+    const size_t data_bytes = sizeof(double) * 10 * nb;
+    const size_t idx_bytes = sizeof(int) * (nb + fresh.size());
+    if (!P.h_stage.ensure(data_bytes + idx_bytes + 256) || !P.stage.ensure(data_bytes + 256) ||
+        !P.idx.ensure(idx_bytes + 256))
+        return pool_fail(S, "staging", hipErrorOutOfMemory);
+    double* hd = static_cast<double*>(P.h_stage.p);
+    int* hi = reinterpret_cast<int*>(static_cast<char*>(P.h_stage.p) + data_bytes);
+    size_t k = 0;
+    for (Job* j : fresh) {
+        for (size_t b = 0; b < j->bodies.size(); b++, k++) {
+            const struct reb_particle& p = j->bodies[b];
+            const double f[10] = {p.x, p.y, p.z, p.vx, p.vy, p.vz, p.ax, p.ay, p.az, p.m};
+            memcpy(hd + 10 * k, f, sizeof(f));
+            hi[k] = (int)(j->body0 + b);
+        }
+    }
+    for (size_t q = 0; q < fresh.size(); q++) hi[nb + q] = fresh[q]->slot;
+    POOL_HIP(S, hipMemcpyAsync(P.stage.p, hd, data_bytes, hipMemcpyHostToDevice, P.stream));
+    POOL_HIP(S, hipMemcpyAsync(P.idx.p, hi, idx_bytes, hipMemcpyHostToDevice, P.stream));
+    hipLaunchKernelGGL(k_pool_scatter, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, P.stream,
+                       P.stage.as<double>(), P.idx.as<int>(), (int)nb, P.body.as<double>(), P.ph.as<double>(), P.cap);
+    hipLaunchKernelGGL(k_pool_set_flags, dim3((unsigned)((fresh.size() + 255) / 256)), dim3(256), 0, P.stream,
+                       P.idx.as<int>() + nb, (int)fresh.size(), P.flags.as<unsigned int>(), 1u);
+    POOL_HIP(S, hipGetLastError());
+    // initial energies (tools.c:108-162) of the new systems
+    POOL_HIP(S, hipMemcpyAsync(P.sys.p, P.hsys.data(), sizeof(WHSys) * P.nslots, hipMemcpyHostToDevice, P.stream));
+    WHParams W = pool_params(S, 0, 0);
+    if (whfast_energy_launch(W, P.energy.as<double>(), (int)P.nslots, P.stream))
+        return pool_fail(S, "energy launch", hipGetLastError());
+    if (!P.h_meta.ensure(sizeof(double) * P.nslots)) return pool_fail(S, "staging", hipErrorOutOfMemory);
+    POOL_HIP(S, hipMemcpyAsync(P.h_meta.p, P.energy.p, sizeof(double) * P.nslots, hipMemcpyDeviceToHost, P.stream));
+    POOL_HIP(S, hipStreamSynchronize(P.stream));
+    for (Job* j : fresh) j->e0 = static_cast<double*>(P.h_meta.p)[j->slot];
+    return true;
+}
+
+static void finish_job(Server* S, Job* j, bool ok, const std::string& msg) {
+    std::lock_guard<std::mutex> g(S->mu);
+    j->t_done = since(S);
+    j->state = ok ? REB_AMD_JOB_DONE : REB_AMD_JOB_FAILED;
+    if (!ok) j->error = msg;
+    S->used -= j->footprint;
+    if (ok)
+        S->done++;
+    else
+        S->failed++;
+    if (j->kind == JOB_NBODY) S->nbody_running--;
+    S->generation++;
+    S->cv_done.notify_all();
+    S->cv_work.notify_all();
+}
+
+// One tick of the planetary pool: every active system advances by
+// min(remaining, tick_steps) steps; finished systems are collected.
+static bool pool_tick(Server* S) {
+    Pool& P = S->pool;
+    auto& act = S->pool_active;
+    if (act.empty()) return true;
+    long tick = 0;
+    for (Job* j : act) {
+        WHSys& w = P.hsys[j->slot];
+        const uint64_t k = std::min<uint64_t>(j->remaining, (uint64_t)S->tick_steps);
+        w.steps = (int)k;
+        w.req = (k == j->remaining ? 1 : 0) | (j->fresh ? 2 : 0) | (j->safe_mode ? 4 : 0);
+        tick = std::max<long>(tick, (long)k);
+    }
+    // tasks over the active systems, bucketed by planet count (largest first)
+    std::vector<int> order(act.size());
+    for (size_t q = 0; q < act.size(); q++) order[q] = act[q]->slot;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return P.hsys[a].N > P.hsys[b].N; });
+    std::vector<WHTask> tasks;
+    for (size_t k = 0; k < order.size();) {
+        const int Gs = whfast_group_size(P.hsys[order[k]].N - 1);
+        const int per = 64 / Gs;
+        size_t m = 0;
+        while (k + m < order.size() && (int)m < per && P.hsys[order[k + m]].N == P.hsys[order[k]].N) m++;
+        tasks.push_back(WHTask{Gs, (int)k, (int)m});
+        k += m;
+    }
+    const size_t sys_bytes = sizeof(WHSys) * P.nslots, ord_bytes = sizeof(int) * order.size(),
+                 task_bytes = sizeof(WHTask) * tasks.size();
+    if (!P.h_meta.ensure(sys_bytes + ord_bytes + task_bytes + 64)) return pool_fail(S, "staging", hipErrorOutOfMemory);
+    char* hm = static_cast<char*>(P.h_meta.p);
+    memcpy(hm, P.hsys.data(), sys_bytes);
+    memcpy(hm + sys_bytes, order.data(), ord_bytes);
+    memcpy(hm + sys_bytes + ord_bytes, tasks.data(), task_bytes);
+    POOL_HIP(S, hipMemcpyAsync(P.sys.p, hm, sys_bytes, hipMemcpyHostToDevice, P.stream));
+    POOL_HIP(S, hipMemcpyAsync(P.order.p, hm + sys_bytes, ord_bytes, hipMemcpyHostToDevice, P.stream));
+    POOL_HIP(S, hipMemcpyAsync(P.tasks.p, hm + sys_bytes + ord_bytes, task_bytes, hipMemcpyHostToDevice, P.stream));
+    const double t_start = since(S);
+    WHParams W = pool_params(S, (int)tasks.size(), tick);
+    if (whfast_launch(W, P.stream)) return pool_fail(S, "tick launch", hipGetLastError());
+    // collect the systems that reached their horizon
+    std::vector<Job*> fin, keep;
+    for (Job* j : act) {
+        j->remaining -= (uint64_t)P.hsys[j->slot].steps;
+        j->fresh = false;
+        (j->remaining == 0 ? fin : keep).push_back(j);
+    }
+    if (!fin.empty()) {
+        size_t nb = 0;
+        for (Job* j : fin) nb += j->bodies.size();
+        const size_t out_bytes = sizeof(double) * 10 * nb;
+        const size_t idx_bytes = sizeof(int) * nb;
+        if (!P.h_stage.ensure(out_bytes + idx_bytes + sizeof(double) * P.nslots + 256) ||
+            !P.stage.ensure(out_bytes + 256) || !P.idx.ensure(idx_bytes + 256))
+            return pool_fail(S, "staging", hipErrorOutOfMemory);
+        int* hi = reinterpret_cast<int*>(static_cast<char*>(P.h_stage.p) + out_bytes);
+        size_t k = 0;
+        for (Job* j : fin)
+            for (size_t b = 0; b < j->bodies.size(); b++) hi[k++] = (int)(j->body0 + b);
+        POOL_HIP(S, hipMemcpyAsync(P.idx.p, hi, idx_bytes, hipMemcpyHostToDevice, P.stream));
+        hipLaunchKernelGGL(k_pool_gather, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, P.stream,
+                           P.idx.as<int>(), (int)nb, P.body.as<double>(), P.cap, P.stage.as<double>());
+        if (whfast_energy_launch(W, P.energy.as<double>(), (int)P.nslots, P.stream))
+            return pool_fail(S, "energy launch", hipGetLastError());
+        double* he = reinterpret_cast<double*>(static_cast<char*>(P.h_stage.p) + out_bytes + idx_bytes + 64);
+        POOL_HIP(S, hipMemcpyAsync(P.h_stage.p, P.stage.p, out_bytes, hipMemcpyDeviceToHost, P.stream));
+        POOL_HIP(S, hipMemcpyAsync(he, P.energy.p, sizeof(double) * P.nslots, hipMemcpyDeviceToHost, P.stream));
+        POOL_HIP(S, hipStreamSynchronize(P.stream));
+        const double* hd = static_cast<const double*>(P.h_stage.p);
+        k = 0;
+        for (Job* j : fin) {
+            for (auto& p : j->bodies) {
+                const double* f = hd + 10 * k++;
+                p.x = f[0]; p.y = f[1]; p.z = f[2];
+                p.vx = f[3]; p.vy = f[4]; p.vz = f[5];
+                p.ax = f[6]; p.ay = f[7]; p.az = f[8];
+                p.m = f[9];
+            }
+            j->e1 = he[j->slot];
+            WHSys& w = P.hsys[j->slot];
+            w = WHSys{};
+            P.slot_job[j->slot] = nullptr;
+            P.free_slots.push_back(j->slot);
+            free_range(P, j->body0, j->bodies.size());
+        }
+    } else {
+        POOL_HIP(S, hipStreamSynchronize(P.stream));
+    }
+    P.busy_s += since(S) - t_start;
+    P.ticks++;
+    act.swap(keep);
+    for (Job* j : fin) finish_job(S, j, true, "");
+    return true;
+}
+
+static void run_nbody(Server* S, Job* j) {
+    (void)hipSetDevice(S->device);
+    struct reb_simulation* r = reb_simulation_create();
+    r->save_messages = 1;
+    r->G = j->G;
+    r->softening = j->softening;
+    r->dt = j->dt;
+    reb_simulation_set_integrator(r, "leapfrog");
+    r->gravity = static_cast<decltype(r->gravity)>(j->gravity);
+    for (const auto& p : j->bodies) reb_simulation_add(r, p);
+    reb_amd_set_precision(r, j->precision);
+    reb_amd_set_resident(r, 1);
+    j->e0 = reb_simulation_energy(r);
+    reb_simulation_steps(r, (size_t)j->n_steps);
+    reb_amd_set_resident(r, 0);
+    j->e1 = reb_simulation_energy(r);
+    std::string msg;
+    bool ok = true;
+    char buf[REB_STRING_SIZE_MAX];
+    while (reb_pop_message(r->messages, buf)) {
+        if (buf[0] == 'e') {
+            ok = false;
+            if (msg.empty()) msg = buf + 1;
+        }
+    }
+    if (ok)
+        for (size_t i = 0; i < j->bodies.size() && i < r->N; i++) j->bodies[i] = r->particles[i];
+    reb_simulation_free(r);
+    finish_job(S, j, ok, msg);
+}
+
+static void nbody_worker(Server* S) {
+    (void)hipSetDevice(S->device);
+    for (;;) {
+        Job* j = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            S->cv_work.wait(lk, [&] { return S->stop || !S->nbody_ready.empty(); });
+            if (S->nbody_ready.empty()) break;   // stop requested and nothing left
+            j = S->nbody_ready.front();
+            S->nbody_ready.pop_front();
+        }
+        run_nbody(S, j);
+    }
+}
+
+static void scheduler(Server* S) {
+    (void)hipSetDevice(S->device);
+    unsigned long long seen = ~0ull;
+    for (;;) {
+        std::vector<Job*> fresh;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            if (S->pool_active.empty())
+                S->cv_work.wait(lk, [&] { return S->stop || S->generation != seen; });
+            if (S->stop) break;
+            seen = S->generation;
+            // admission: arrival order, skipping jobs that do not fit yet
+            for (auto it = S->pending.begin(); it != S->pending.end();) {
+                Job* j = *it;
+                const bool fits = S->used + j->footprint <= S->budget;
+                const bool slot_ok = j->kind != JOB_NBODY || S->nbody_running < S->max_nbody;
+                if (!fits || !slot_ok) {
+                    ++it;
+                    continue;
+                }
+                it = S->pending.erase(it);
+                S->used += j->footprint;
+                S->peak_used = std::max(S->peak_used, S->used);
+                j->state = REB_AMD_JOB_RUNNING;
+                j->t_admit = since(S);
+                if (j->kind == JOB_NBODY) {
+                    S->nbody_running++;
+                    S->nbody_ready.push_back(j);
+                    S->cv_work.notify_all();
+                } else {
+                    fresh.push_back(j);
+                }
+            }
+        }
+        if (!fresh.empty()) {
+            if (pool_admit(S, fresh)) {
+                for (Job* j : fresh) S->pool_active.push_back(j);
+            } else {
+                for (Job* j : fresh) finish_job(S, j, false, S->err);
+            }
+        }
+        if (!S->pool_active.empty() && !pool_tick(S)) {
+            std::vector<Job*> act;
+            act.swap(S->pool_active);
+            for (Job* j : act) finish_job(S, j, false, S->err);
+        }
+    }
+}
+
+}  // namespace rbamd
+
+using namespace rbamd;
+
+struct reb_amd_server {
+    Server s;
+};
+
+extern "C" {
+
+REB_API struct reb_amd_server* reb_amd_server_create(size_t memory_budget_bytes, int max_nbody_jobs,
+                                                     long whfast_tick_steps) {
+    if (reb_amd_device_count() <= 0) {
+        set_last_error("reb_amd_server_create: no HIP device is available");
+        return nullptr;
+    }
+    auto* h = new reb_amd_server();
+    Server* S = &h->s;
+    (void)hipGetDevice(&S->device);
+    S->budget = memory_budget_bytes ? memory_budget_bytes : ((size_t)8 << 30);
+    S->max_nbody = max_nbody_jobs > 0 ? max_nbody_jobs : 8;
+    S->tick_steps = whfast_tick_steps > 0 ? whfast_tick_steps : 2000;
+    S->t0 = std::chrono::steady_clock::now();
+    S->pool.device = S->device;
+    S->pool.max_resident = whfast_max_resident(S->device);
+    if (hipStreamCreateWithFlags(&S->pool.stream, hipStreamNonBlocking) != hipSuccess ||
+        !S->pool.counter.ensure(256) || hipMemset(S->pool.counter.p, 0, 256) != hipSuccess) {
+        set_last_error("reb_amd_server_create: could not set up the device pool");
+        delete h;
+        return nullptr;
+    }
+    S->sched = std::thread(scheduler, S);
+    for (int w = 0; w < S->max_nbody; w++) S->workers.emplace_back(nbody_worker, S);
+    return h;
+}
+
+// Cancels pending jobs, lets running jobs finish, releases everything.
+REB_API void reb_amd_server_free(struct reb_amd_server* h) {
+    if (!h) return;
+    Server* S = &h->s;
+    {
+        std::unique_lock<std::mutex> lk(S->mu);
+        for (Job* j : S->pending) {
+            j->state = REB_AMD_JOB_FAILED;
+            j->error = "server shut down";
+        }
+        S->pending.clear();
+    }
+    // drain the pool and the workers
+    for (;;) {
+        std::unique_lock<std::mutex> lk(S->mu);
+        bool busy = S->nbody_running > 0;
+        for (auto& kv : S->jobs)
+            if (kv.second->state == REB_AMD_JOB_RUNNING) busy = true;
+        if (!busy) break;
+        S->cv_done.wait_for(lk, std::chrono::milliseconds(50));
+    }
+    {
+        std::lock_guard<std::mutex> g(S->mu);
+        S->stop = true;
+        S->cv_work.notify_all();
+    }
+    S->sched.join();
+    for (auto& t : S->workers)
+        if (t.joinable()) t.join();
+    Pool& P = S->pool;
+    (void)hipSetDevice(S->device);
+    (void)hipStreamSynchronize(P.stream);
+    for (DevBuf* b : {&P.body, &P.ph, &P.sys, &P.flags, &P.energy, &P.order, &P.tasks, &P.counter, &P.idx, &P.stage})
+        b->release();
+    P.h_stage.release();
+    P.h_meta.release();
+    (void)hipStreamDestroy(P.stream);
+    delete h;
+}
+
+static long submit(Server* S, std::unique_ptr<Job> j) {
+    std::lock_guard<std::mutex> g(S->mu);
+    if (j->footprint > S->budget) {
+        S->err = "job footprint (" + std::to_string(j->footprint) + " bytes) exceeds the memory budget";
+        set_last_error(S->err);
+        return -1;
+    }
+    j->id = S->next_id++;
+    j->t_submit = since(S);
+    Job* raw = j.get();
+    S->jobs[raw->id] = std::move(j);
+    S->pending.push_back(raw);
+    S->generation++;
+    S->cv_work.notify_all();
+    return raw->id;
+}
+
+// A planetary system (central body first, 1..65 bodies) for n_steps WHFast
+// steps in democratic heliocentric coordinates with the given G and safe_mode,
+// ending synchronized.
+REB_API long reb_amd_server_submit_whfast(struct reb_amd_server* h, size_t n_bodies, const struct reb_particle* bodies,
+                                          double dt, double G, int safe_mode, uint64_t n_steps) {
+    if (n_bodies < 1 || n_bodies > 65 || !bodies || !std::isfinite(dt) || !std::isfinite(G)) {
+        set_last_error("reb_amd_server_submit_whfast: 1 to 65 bodies, a finite dt and a finite G required");
+        return -1;
+    }
+    auto j = std::make_unique<Job>();
+    j->kind = JOB_WHFAST;
+    j->bodies.assign(bodies, bodies + n_bodies);
+    j->dt = dt;
+    j->G = G;
+    j->safe_mode = safe_mode ? 1 : 0;
+    j->n_steps = n_steps;
+    j->footprint = whfast_footprint(n_bodies);
+    return submit(&h->s, std::move(j));
+}
+
+// An N-body simulation: leapfrog (order 2) for n_steps steps with direct
+// gravity (REB_GRAVITY_BASIC or _COMPENSATED) in the given precision.
+REB_API long reb_amd_server_submit_nbody(struct reb_amd_server* h, size_t n, const struct reb_particle* bodies,
+                                         double G, double softening, double dt, uint64_t n_steps, int gravity,
+                                         int precision) {
+    if (n < 1 || !bodies || (gravity != REB_GRAVITY_BASIC && gravity != REB_GRAVITY_COMPENSATED) ||
+        precision < REB_AMD_PRECISION_FP64 || precision > REB_AMD_PRECISION_SPLIT || n > (size_t)1 << 24) {
+        set_last_error("reb_amd_server_submit_nbody: invalid arguments");
+        return -1;
+    }
+    auto j = std::make_unique<Job>();
+    j->kind = JOB_NBODY;
+    j->bodies.assign(bodies, bodies + n);
+    j->G = G;
+    j->softening = softening;
+    j->dt = dt;
+    j->n_steps = n_steps;
+    j->gravity = gravity;
+    j->precision = precision;
+    j->footprint = nbody_footprint(n, precision);
+    return submit(&h->s, std::move(j));
+}
+
+REB_API int reb_amd_server_status(struct reb_amd_server* h, long job) {
+    Server* S = &h->s;
+    std::lock_guard<std::mutex> g(S->mu);
+    auto it = S->jobs.find(job);
+    return it == S->jobs.end() ? REB_AMD_JOB_UNKNOWN : it->second->state;
+}
+
+// Waits until the job is done or failed (timeout_s < 0: no limit).  Returns
+// its state (REB_AMD_JOB_RUNNING or _PENDING after a timeout).
+REB_API int reb_amd_server_wait(struct reb_amd_server* h, long job, double timeout_s) {
+    Server* S = &h->s;
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto it = S->jobs.find(job);
+    if (it == S->jobs.end()) return REB_AMD_JOB_UNKNOWN;
+    Job* j = it->second.get();
+    auto done = [&] { return j->state == REB_AMD_JOB_DONE || j->state == REB_AMD_JOB_FAILED; };
+    if (timeout_s < 0)
+        S->cv_done.wait(lk, done);
+    else
+        S->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), done);
+    return j->state;
+}
+
+// Copies a finished job's bodies (x y z vx vy vz ax ay az m updated; other
+// fields as submitted), energies before and after (tools.c:108-162) and its
+// submit / admit / done times in seconds since the server was created, then
+// forgets the job.  0 on success.
+REB_API int reb_amd_server_result(struct reb_amd_server* h, long job, struct reb_particle* out, size_t n,
+                                  double* energy2, double* times3) {
+    Server* S = &h->s;
+    std::lock_guard<std::mutex> g(S->mu);
+    auto it = S->jobs.find(job);
+    if (it == S->jobs.end()) {
+        set_last_error("reb_amd_server_result: unknown job");
+        return 1;
+    }
+    Job* j = it->second.get();
+    if (j->state != REB_AMD_JOB_DONE) {
+        set_last_error(j->state == REB_AMD_JOB_FAILED ? "job failed: " + j->error : "job not finished");
+        return 1;
+    }
+    if (out) {
+        if (n != j->bodies.size()) {
+            set_last_error("reb_amd_server_result: wrong body count");
+            return 1;
+        }
+        std::copy(j->bodies.begin(), j->bodies.end(), out);
+    }
+    if (energy2) {
+        energy2[0] = j->e0;
+        energy2[1] = j->e1;
+    }
+    if (times3) {
+        times3[0] = j->t_submit;
+        times3[1] = j->t_admit;
+        times3[2] = j->t_done;
+    }
+    S->jobs.erase(it);
+    return 0;
+}
+
+REB_API const char* reb_amd_server_job_error(struct reb_amd_server* h, long job) {
+    Server* S = &h->s;
+    static thread_local std::string copy;
+    std::lock_guard<std::mutex> g(S->mu);
+    auto it = S->jobs.find(job);
+    copy = it == S->jobs.end() ? std::string("unknown job") : it->second->error;
+    return copy.c_str();
+}
+
+// out[0..9]: submitted-but-unreported jobs, pending, running, done, failed,
+// bytes in use, peak bytes in use, budget, pool ticks, pool busy seconds.
+REB_API int reb_amd_server_stats(struct reb_amd_server* h, double* out, int n) {
+    Server* S = &h->s;
+    std::lock_guard<std::mutex> g(S->mu);
+    size_t running = 0;
+    for (auto& kv : S->jobs)
+        if (kv.second->state == REB_AMD_JOB_RUNNING) running++;
+    const double v[10] = {(double)S->jobs.size(), (double)S->pending.size(), (double)running, (double)S->done,
+                          (double)S->failed, (double)S->used, (double)S->peak_used, (double)S->budget,
+                          (double)S->pool.ticks, S->pool.busy_s};
+    for (int i = 0; i < n && i < 10; i++) out[i] = v[i];
+    return 0;
+}
+
+}  // extern "C"

@@ -556,8 +556,6 @@ __global__ __launch_bounds__(64) void k_tree_gravity(const double* __restrict__ 
     }
 }
 
-static unsigned long long g_last_interactions = 0;
-
 bool gravity_tree(struct reb_simulation* r) {
     DeviceState* d = dev_of(r);
     // gravity.c:54-56: wrap into the box before building
@@ -976,6 +974,19 @@ REB_API long reb_amd_tree_dump(struct reb_simulation* r, double* out, long cap) 
     if (!ensure_device(r)) return -1;
     if (!tree_build(r)) return -1;
     return tree_dump_device(r, out, cap);
+}
+
+// Particle-cell and particle-particle interactions summed by the last tree
+// gravity evaluation (over all ghost boxes), or -1 if there was none.
+REB_API long long reb_amd_tree_interactions(struct reb_simulation* r) {
+    using namespace rbamd;
+    DeviceState* d = dev_of(r);
+    if (!d->tree || !d->stream) return -1;
+    unsigned long long v = 0;
+    const auto* counter = reinterpret_cast<const unsigned long long*>(d->tree->info.as<int>() + 8);
+    if (!RB_HIP(r, hipMemcpyAsync(&v, counter, sizeof(v), hipMemcpyDeviceToHost, d->stream))) return -1;
+    if (!stream_sync(r)) return -1;
+    return (long long)v;
 }
 
 }  // extern "C"

@@ -289,7 +289,8 @@ __global__ __launch_bounds__(64) void k_whfast_dh(WHParams P) {
         // launch's last step; its lanes keep taking part in the barriers).
         const long my_steps = !sys_ok ? 0 : (S.steps >= 0 && S.steps < P.nsteps ? (long)S.steps : P.nsteps);
         const double dt = S.dt;
-        const double Gc = P.G;
+        const double Gc = S.G;
+        const bool safe = P.safe_mode || (sys_ok && (S.req & 4));
 
         // Each lane writes its planet's values into LDS; the group then forms
         // the reference's sequential sums over planets 1..NA-1 in index order.
@@ -443,10 +444,10 @@ __global__ __launch_bounds__(64) void k_whfast_dh(WHParams P) {
         };
 
         // integrator_whfast.c:1113-1259 (DH, default kernel, no correctors)
-        bool modified = P.did_modify || ((flags0 >> 4) & 1u);
+        bool modified = P.did_modify || (sys_ok && (S.req & 2));
         for (long s = 0; s < P.nsteps; s++) {
             const bool act = s < my_steps;
-            const bool recalc = act && (P.safe_mode || modified);
+            const bool recalc = act && (safe || modified);
             if (__any(recalc)) {
                 if (recalc && sync == 0) warn |= 2;
                 synchronize(recalc);
@@ -466,10 +467,10 @@ __global__ __launch_bounds__(64) void k_whfast_dh(WHParams P) {
             }
             jump(dt / 2., act);
             if (act) sync = 0;
-            if (P.safe_mode) synchronize(act);
+            if (__any(act && safe)) synchronize(act && safe);
             if (act) modified = false;
         }
-        synchronize(sys_ok && (P.sync_at_end || ((flags0 >> 3) & 1u)));
+        synchronize(sys_ok && (P.sync_at_end || (S.req & 1)));
 
         // ---- store
         if (live) {
@@ -519,7 +520,7 @@ __global__ void k_system_energy(WHParams P, double* __restrict__ out, int nsys) 
             const double dx = B[bi] - B[bj];
             const double dy = B[st + bi] - B[st + bj];
             const double dz = B[2 * st + bi] - B[2 * st + bj];
-            e_pot -= P.G * B[9 * st + bj] * B[9 * st + bi] / sqrt(dx * dx + dy * dy + dz * dz);
+            e_pot -= S.G * B[9 * st + bj] * B[9 * st + bi] / sqrt(dx * dx + dy * dy + dz * dz);
         }
     }
     out[s] = e_kin + e_pot;
@@ -635,6 +636,9 @@ static bool whfast_prepare(struct reb_simulation* r, WHParams& P) {
     sys.Nact_grav = (int)((r->N_active == SIZE_MAX) ? r->N : r->N_active);
     sys.tp_type = r->testparticle_type;
     sys.steps = -1;
+    sys.req = 0;
+    sys.pad = 0;
+    sys.G = r->G;
     sys.dt = r->dt;
     WHTask task{whfast_group_size((int)r->N - 1), 0, 1};
     struct Meta {

@@ -15,7 +15,12 @@ struct WHSys {
     int Nact_grav;   // N_active used by the gravity loop
     int tp_type;     // testparticle_type
     int steps;       // steps in the current launch (-1: the launch's count)
+    int req;         // requests for this launch: bit0 synchronize at its end,
+                     // bit1 particles modified (recompute the heliocentric state),
+                     // bit2 safe mode for this system
+    int pad;
     double dt;
+    double G;
 };
 
 // A wavefront's unit of work: `nsys` systems of one group size G (lanes per
@@ -30,9 +35,7 @@ struct WHParams {
     const WHSys* sys;
     const WHTask* tasks;
     const int* order;
-    unsigned int* flags;      // per system: bit0 synchronized, bit1 period warning, bit2 resync warning;
-                              // set by the host: bit3 synchronize at the end of this launch,
-                              // bit4 particles modified (recompute the heliocentric state)
+    unsigned int* flags;      // per system: bit0 synchronized, bit1 period warning, bit2 resync warning
     unsigned int* counter;    // work counter of the persistent grid
     int ntasks;
     int grid;                 // persistent workgroups (one wavefront each)

@@ -137,3 +137,33 @@ def test_sheet_direct_search_bitwise(gpu):
     assert np.array_equal(got[:, :2].astype(np.int64), pairs)
     assert np.array_equal(got[:, 2:], golden("sheet2000_direct_gb.npy"))
     assert np.array_equal(sim.state()[:, :6], golden("sheet2000_direct_step1_state.npy")[:, :6])
+
+
+def test_sheet_full_size_collision_pair_set(gpu, orc):
+    """The full-size sheet (200000 bodies, 10 x 10 root boxes): the set of
+    detected (p1, p2, ghost box) collisions equals the oracle's tree search
+    (collision.c:197-269, :422-503) on the same state.  dt = 0 and no gravity,
+    so the search sees the initial state."""
+    sim = gpu.shearing_sheet(200000, 136.0, 10, 99)
+    st = sim.state()
+    sim.s.dt = 0.0
+    sim.s.gravity = gpu.GRAVITY["none"]
+    sim.steps(1)
+    n = sim.s.N_collisions
+    dt = np.dtype([("p1", "<u8"), ("p2", "<u8"), ("gb", "<f8", 6), ("ri", "<u8")])
+    raw = ctypes.string_at(ctypes.cast(sim.s.collisions, ctypes.c_void_p), n * dt.itemsize)
+    got = np.frombuffer(raw, dtype=dt)
+    gbs = []
+    for i in (-1, 0, 1):
+        for j in (-1, 0, 1):
+            g = np.zeros(6)
+            orc.lib().orc_ghostbox_shear(1.0, 1360.0, 1360.0, 136.0, 0.0, i, j, 0, g)
+            gbs.append(g)
+    pairs, gb = orc.collisions_tree(st, 136.0, (10, 10, 1), np.array(gbs), cap=1 << 21)
+    assert n == pairs.shape[0] and n > 100000
+    a = np.column_stack([got["p1"].astype(np.float64), got["p2"].astype(np.float64), got["gb"]])
+    b = np.column_stack([pairs.astype(np.float64), gb])
+    a = a[np.lexsort(a.T[::-1])]
+    b = b[np.lexsort(b.T[::-1])]
+    assert np.array_equal(a, b)
+    sim.free()
