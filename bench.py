@@ -208,6 +208,17 @@ def ensemble_leg(amd, args):
                 "%.3e" % r2.mean() == "%.3e" % cpu["mean_rel_energy_error"])
             res["cpu_baseline"] = cpu
             res["speedup_vs_cpu"] = res["value"] / cpu["value"]
+            # algorithmic work per system-step, counted by the CPU restatement
+            # integrating the first 512 systems for 200 steps
+            sys.path.insert(0, ROOT)
+            import oracle
+            k = min(512, len(nb))
+            fl = oracle.whfast_flops_per_system_step(nb[:k], st[:int(nb[:k].sum())], dt[:k], 200)
+            achieved = fl * res["value"] / 1e12
+            res["roofline"] = {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                               "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": "k_whfast_dh",
+                               "flop_per_system_step": fl,
+                               "note": "latency-bound: 1.3 wavefronts per SIMD at 16384 systems"}
             log("ensemble cpu: %.3e system-steps/s on %d threads -> GPU %.0fx; energy errors match: %s" % (
                 cpu["value"], cpu["cores"], res["speedup_vs_cpu"], cpu["energy_error_matches"]))
     ens.free()

@@ -47,12 +47,8 @@ REB_API int reb_amd_device_name(char* buf, size_t len);
 REB_API const char* reb_amd_version(void);
 REB_API const char* reb_amd_last_error(void);
 
-/* Device time (ms) spent in the last reb_simulation_steps/integrate call by
- * category, as REBOUND's PROFILING categories (output.h:28-49):
- * out[0]=gravity, out[1]=integrator, out[2]=boundary, out[3]=collision.
- * Timing is recorded only when enabled with reb_amd_set_profiling(r,1). */
+/* Kernel timing switch for reb_amd_kernel_time (off by default). */
 REB_API void reb_amd_set_profiling(struct reb_simulation* r, int on);
-REB_API void reb_amd_profile(const struct reb_simulation* r, double* out4);
 
 /* With profiling on, each launch of the direct-gravity pair kernel is
  * bracketed by HIP events on the simulation's stream; this waits for the
@@ -160,9 +156,12 @@ REB_API const char* reb_amd_ensemble_error(const struct reb_amd_ensemble* e);
  * Serving: a job queue in front of the GPU.  Jobs are admitted in arrival
  * order while their device-memory footprints fit in the budget (later, smaller
  * jobs may pass one that does not fit yet).  Planetary jobs share one device
- * pool stepped in ticks of `whfast_tick_steps` (continuous batching: jobs join
- * and leave at tick boundaries); N-body jobs run on their own streams from up
- * to `max_nbody_jobs` worker threads.  A job's result equals that of the same
+ * pool stepped in ticks of at most `whfast_tick_steps` steps, each aiming at
+ * about 2 ms of device time; up to `max_nbody_jobs` N-body jobs are batched
+ * on a second stream, all advancing by the same number of leapfrog steps per
+ * round with three launches per step for the whole batch (EXACT-precision
+ * jobs run alone on a third stream).  Jobs join and leave at tick and round
+ * boundaries (continuous batching).  A job's result equals that of the same
  * simulation run alone through reb_simulation_steps.  No counterpart in the
  * reference, whose unit of work is one reb_simulation per caller.
  * ------------------------------------------------------------------------- */
@@ -175,7 +174,7 @@ enum REB_AMD_JOB_STATE {
 };
 struct reb_amd_server;
 
-/* memory_budget_bytes 0: 8 GiB; max_nbody_jobs 0: 8; whfast_tick_steps 0: 2000. */
+/* memory_budget_bytes 0: 8 GiB; max_nbody_jobs 0: 32; whfast_tick_steps 0: 2000. */
 REB_API struct reb_amd_server* reb_amd_server_create(size_t memory_budget_bytes, int max_nbody_jobs,
                                                      long whfast_tick_steps);
 /* Cancels pending jobs, waits for running ones, frees everything. */

@@ -60,6 +60,7 @@ def lib():
             "orc_collision_direct": (c_long, [_dp, c_size_t, _dp, c_int, _ip, _dp, c_long]),
             "orc_collision_shuffle": (None, [_ip, _dp, c_long, U]),
             "orc_hardsphere": (c_double, [_dp, c_int64, c_int64, _dp, c_double, c_double, POINTER(c_long)]),
+            "orc_flops": (ctypes.c_ulonglong, [c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -135,6 +136,19 @@ def whfast_dh(s, G, dt, steps, safe_mode, soft=0.0, ph=None, is_sync=1, did_modi
     flag = c_int(is_sync)
     lib().orc_whfast_dh(s, ph, n, G, soft, dt, steps, safe_mode, ctypes.byref(flag), did_modify)
     return s, ph, flag.value
+
+
+def whfast_flops_per_system_step(n_bodies, state, dts, steps, G=1.0, safe_mode=0):
+    """Algorithmic FLOPs per system-step of WHFast DH on the given systems
+    (counted by the restatement while it integrates them for `steps` steps,
+    first step and final synchronization included)."""
+    first = np.concatenate([[0], np.cumsum(n_bodies)[:-1]])
+    lib().orc_flops(1)
+    for s, n in enumerate(n_bodies):
+        sub = np.zeros((int(n), 11))
+        sub[:, :min(11, state.shape[1])] = state[first[s]:first[s] + n, :11]
+        whfast_dh(sub, G, float(dts[s]), steps, safe_mode)
+    return lib().orc_flops(1) / (len(n_bodies) * float(steps))
 
 
 def tree_dump(s, root_size, nr):

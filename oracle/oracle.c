@@ -284,6 +284,18 @@ void orc_leapfrog(double* s, size_t N, double G, double soft, double dt, long st
     free(acc);
 }
 
+/* Algorithmic floating-point operations of the WHFast path, counted as the
+ * reference's source writes them: + - * / and sqrt are one each (constant
+ * products such as 2.*M_PI folded), comparisons, fabs and floor are free.
+ * Gravity pairs count 20 (gravity.c:222-230). */
+static unsigned long long g_flops = 0;
+#define FL(n) (g_flops += (unsigned long long)(n))
+unsigned long long orc_flops(int reset) {
+    unsigned long long v = g_flops;
+    if (reset) g_flops = 0;
+    return v;
+}
+
 /* ------------------------------------------------------------------ Kepler (integrator_whfast.c:161-427) */
 static const double invfactorial[14] = {1., 1., 1. / 2., 1. / 6., 1. / 24., 1. / 120., 1. / 720., 1. / 5040.,
                                         1. / 40320., 1. / 362880., 1. / 3628800., 1. / 39916800.,
@@ -296,7 +308,9 @@ static void stumpff_cs3(double* cs, double z) {
     while (fabs(z) > 0.1) {
         z = z / 4.;
         n++;
+        FL(1);
     }
+    FL(24 + 9 * n);
     double c_odd = invfactorial[13];
     double c_even = invfactorial[12];
     for (int np = 11; np >= 3; np -= 2) {
@@ -317,6 +331,7 @@ static void stumpff_cs3(double* cs, double z) {
 
 static void stiefel_Gs3(double* Gs, double beta, double X_) {
     double X2 = X_ * X_;
+    FL(6);
     stumpff_cs3(Gs, beta * X2);
     Gs[1] *= X_;
     Gs[2] *= X2;
@@ -332,6 +347,7 @@ int orc_kepler(double* p, double mu, double dt) {
     const double beta = 2. * mu * r0i - v2;
     const double eta0 = p1[0] * p1[3] + p1[1] * p1[4] + p1[2] * p1[5];
     const double zeta0 = mu - beta * r0;
+    FL(22);
     double Xv, Gs[6];
     double invperiod = 0;
     double X_per_period = nan("");
@@ -342,6 +358,7 @@ int orc_kepler(double* p, double mu, double dt) {
         if (fabs(dt) * invperiod > 1.) warn = 1;
         const double dtr0i = dt * r0i;
         Xv = dtr0i * (1. - dtr0i * eta0 * 0.5 * r0i);
+        FL(13);
     } else {
         Xv = 0.;
     }
@@ -351,8 +368,10 @@ int orc_kepler(double* p, double mu, double dt) {
     const double eg = eta0 * Gs[1] + zeta0 * Gs[2];
     double ri = 1. / (r0 + eg);
     Xv = ri * (Xv * eg - eta0 * Gs[2] - zeta0 * Gs[3] + dt);
+    FL(14);
     if (fastabs(Xv - oldX) > 0.01 * X_per_period) {
         Xv = beta * dt / mu;
+        FL(2);
         double prevX[65];
         for (int n_lag = 1; n_lag < 64; n_lag++) {
             stiefel_Gs3(Gs, beta, Xv);
@@ -361,6 +380,7 @@ int orc_kepler(double* p, double mu, double dt) {
             const double fpp = eta0 * Gs[0] + zeta0 * Gs[1];
             const double denom = fp + sqrt(fabs(16. * fp * fp - 20. * f * fpp));
             Xv = (Xv * denom - 5. * f) / denom;
+            FL(24);
             for (int i = 1; i < n_lag; i++) {
                 if (Xv == prevX[i]) {
                     n_lag = 64;
@@ -372,6 +392,7 @@ int orc_kepler(double* p, double mu, double dt) {
         }
         const double e1 = eta0 * Gs[1] + zeta0 * Gs[2];
         ri = 1. / (r0 + e1);
+        FL(5);
     } else {
         double oldX2 = nan("");
         for (int n_hg = 1; n_hg < 32; n_hg++) {
@@ -381,6 +402,7 @@ int orc_kepler(double* p, double mu, double dt) {
             const double e1 = eta0 * Gs[1] + zeta0 * Gs[2];
             ri = 1. / (r0 + e1);
             Xv = ri * (Xv * e1 - eta0 * Gs[2] - zeta0 * Gs[3] + dt);
+            FL(12);
             if (Xv == oldX || Xv == oldX2) {
                 converged = 1;
                 break;
@@ -392,12 +414,14 @@ int orc_kepler(double* p, double mu, double dt) {
         if (beta > 0.) {
             X_min = X_per_period * floor(dt * invperiod);
             X_max = X_min + X_per_period;
+            FL(3);
         } else {
             double h2 = r0 * r0 * v2 - eta0 * eta0;
             double q = h2 / mu / (1. + sqrt(1. - h2 * beta / (mu * mu)));
             double vq = copysign(sqrt(h2) / q, dt);
             X_min = dt / (fastabs(vq * dt) + r0);
             X_max = dt / q;
+            FL(18);
             if (dt < 0.) {
                 double temp = X_min;
                 X_min = X_max;
@@ -405,6 +429,7 @@ int orc_kepler(double* p, double mu, double dt) {
             }
         }
         Xv = (X_max + X_min) / 2.;
+        FL(2);
         do {
             stiefel_Gs3(Gs, beta, Xv);
             double sv = r0 * Xv + eta0 * Gs[2] + zeta0 * Gs[3] - dt;
@@ -414,9 +439,11 @@ int orc_kepler(double* p, double mu, double dt) {
                 X_min = Xv;
             }
             Xv = (X_max + X_min) / 2.;
+            FL(11);
         } while (fastabs((X_max - X_min)) > fastabs((X_max + X_min) * 1e-15));
         const double e1 = eta0 * Gs[1] + zeta0 * Gs[2];
         ri = 1. / (r0 + e1);
+        FL(5);
     }
     if (isnan(ri)) {
         ri = 0.;
@@ -424,6 +451,7 @@ int orc_kepler(double* p, double mu, double dt) {
         Gs[2] = 0.;
         Gs[3] = 0.;
     }
+    FL(33);
     double f = -mu * Gs[2] * r0i;
     double g = dt - mu * Gs[3];
     double fd = -mu * Gs[1] * r0i * ri;
@@ -442,6 +470,7 @@ int orc_kepler(double* p, double mu, double dt) {
 
 static void dh_from_inertial(const double* s, double* ph, size_t N, size_t NA) {
     /* transformations.c:388-424 */
+    FL(13 * NA + 6 + 6 * (N - 1));
     double x0 = 0., y0 = 0., z0 = 0., vx0 = 0., vy0 = 0., vz0 = 0., m0 = 0.;
     for (size_t i = 0; i < NA; i++) {
         double m = COL(s, i, M);
@@ -473,6 +502,7 @@ static void dh_from_inertial(const double* s, double* ph, size_t N, size_t NA) {
 
 static void dh_to_inertial(double* s, const double* ph, size_t N, size_t NA) {
     /* transformations.c:426-477 */
+    FL(18 * (NA - 1) + 6 + 6 * (N - 1));
     const double mtot = PH(ph, 0, 6);
     double x0 = 0., y0 = 0., z0 = 0.;
     for (size_t i = 1; i < NA; i++) {
@@ -510,6 +540,7 @@ static void dh_to_inertial(double* s, const double* ph, size_t N, size_t NA) {
 
 static void dh_kepler_com(const double* s, double* ph, size_t N, double G, double h) {
     const double m0 = COL(s, 0, M);
+    FL(6 + (N - 1));
     for (size_t i = 1; i < N; i++) {
         double st[6];
         for (int k = 0; k < 6; k++) st[k] = PH(ph, i, k);
@@ -524,6 +555,7 @@ static void dh_kepler_com(const double* s, double* ph, size_t N, double G, doubl
 static void dh_jump(const double* s, double* ph, size_t N, size_t NA, double h) {
     /* integrator_whfast.c:537-554 */
     const double m0 = COL(s, 0, M);
+    FL(6 * (NA - 1) + 6 * (N - 1));
     double px = 0, py = 0, pz = 0;
     for (size_t i = 1; i < NA; i++) {
         const double m = COL(s, i, M);
@@ -560,6 +592,7 @@ void orc_whfast_dh(double* s, double* ph, size_t N, double G, double soft, doubl
         dh_jump(s, ph, N, NA, dt / 2.);
         dh_to_inertial(s, ph, N, NA);
         orc_gravity_basic(s, N, N, 0, 2, G, soft, gb, 1, acc);
+        FL(20 * (N - 1) * (N > 1 ? N - 2 : 0) + 6 * (N - 1));
         for (size_t i = 0; i < N; i++) {
             COL(s, i, AX) = acc[3 * i];
             COL(s, i, AY) = acc[3 * i + 1];

@@ -96,4 +96,8 @@ void orc_collision_shuffle(int64_t* pairs, double* gbo, long n, unsigned int* se
 double orc_hardsphere(double* s, int64_t p1, int64_t p2, const double* gb6, double eps, double min_v,
                       long* log_n);
 
+/* Algorithmic floating-point operations counted by the WHFast path since the
+ * last reset (+ - * / sqrt one each, gravity pairs 20). */
+unsigned long long orc_flops(int reset);
+
 #endif

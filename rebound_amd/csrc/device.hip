@@ -86,8 +86,7 @@ bool device_ready(struct reb_simulation* r) {
     d->device = dev;
     if (!RB_HIP(r, hipSetDevice(dev))) return false;
     if (!RB_HIP(r, hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking))) return false;
-    for (auto& e : d->ev)
-        if (!RB_HIP(r, hipEventCreate(&e))) return false;
+    d->owns_stream = true;
     return true;
 }
 
@@ -206,9 +205,11 @@ void device_release(struct reb_simulation* r) {
     d->flags.release();
     if (d->host_stage) (void)hipHostFree(d->host_stage);
     d->host_stage = nullptr;
-    for (auto& e : d->ev)
+    for (auto& e : d->kev)
         if (e) (void)hipEventDestroy(e);
-    if (d->stream) (void)hipStreamDestroy(d->stream);
+    d->kev.clear();
+    d->kev_used = 0;
+    if (d->stream && d->owns_stream) (void)hipStreamDestroy(d->stream);
     d->stream = nullptr;
     d->cap = 0;
 }

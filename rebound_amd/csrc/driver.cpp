@@ -193,7 +193,6 @@ static void begin_call(struct reb_simulation* r) {
     // simulation was declared device-resident).
     DeviceState* d = dev_of(r);
     if (!d->resident) mark_host_newer(r);
-    for (double& v : d->prof_ms) v = 0.;
 }
 
 static void end_call(struct reb_simulation* r) {
@@ -274,11 +273,6 @@ REB_API void reb_amd_set_profiling(struct reb_simulation* r, int on) {
     stream_sync(r);
     d->profiling = on != 0;
     d->kev_used = 0;
-}
-
-REB_API void reb_amd_profile(const struct reb_simulation* r, double* out4) {
-    const DeviceState* d = dev_of(r);
-    for (int k = 0; k < 4; k++) out4[k] = d->prof_ms[k];
 }
 
 REB_API int reb_amd_kernel_time(struct reb_simulation* r, double* total_ms, int* launches) {

@@ -25,7 +25,23 @@
 
 namespace rbamd {
 
-static constexpr int kBlock = 256;     // targets per workgroup = sources per LDS tile
+static constexpr int kBlock = 256;     // targets per workgroup = sources per LDS tile (exact kernels)
+
+// Shape of the fast kernels (tuning knobs, fixed at build time): targets per
+// workgroup (= sources per LDS tile), unroll of the pair loop, and wavefronts
+// per SIMD the register budget is sized for.
+#ifndef RB_FAST_BLOCK
+#define RB_FAST_BLOCK 256
+#endif
+#ifndef RB_FAST_UNROLL
+#define RB_FAST_UNROLL 4
+#endif
+#ifndef RB_FAST_WAVES
+#define RB_FAST_WAVES 4
+#endif
+#define RB_STR(x) #x
+#define RB_UNROLL(n) _Pragma(RB_STR(unroll n))
+static constexpr int kFast = RB_FAST_BLOCK;
 
 struct GB3 { double x, y, z; };
 // Ghost-box offsets travel by value in the kernel arguments (kernarg segment,
@@ -34,6 +50,14 @@ static constexpr int kMaxGB = 125;       // N_ghost up to 2 in all three axes
 struct GBList {
     int n;
     GB3 g[kMaxGB];
+};
+
+// The leapfrog's kick and closing drift, applied by the pass that finishes the
+// accelerations (on == 0: nothing).
+struct KickDrift {
+    double *x, *y, *z, *vx, *vy, *vz;
+    double kick, drift;
+    int on;
 };
 
 __device__ __forceinline__ bool skip_pair(int i, int j, int ignore) {
@@ -53,6 +77,7 @@ __global__ __launch_bounds__(kBlock) void k_basic_exact(
     __shared__ double4 tile[kBlock];
     const int i = blockIdx.x * kBlock + threadIdx.x;
     const bool live = i < N;
+    const int blk_lo = blockIdx.x * kBlock, blk_hi = blk_lo + kBlock;
     const double xi = live ? x[i] : 0., yi = live ? y[i] : 0., zi = live ? z[i] : 0.;
     double a_x = 0., a_y = 0., a_z = 0.;
     for (int g = 0; g < gl.n; g++) {
@@ -68,15 +93,7 @@ __global__ __launch_bounds__(kBlock) void k_basic_exact(
                 __syncthreads();
                 const int n = min(kBlock, j1 - t);
                 if (!live || (pass == 1 && i >= N_active)) continue;
-                for (int k = 0; k < n; k++) {
-                    const int j = t + k;
-                    if (pass == 0) {
-                        if (skip_pair(i, j, ignore)) continue;
-                    } else {
-                        if (ignore == 1 && j == 1 && i == 0) continue;
-                        if (ignore == 2 && (j == 0 || i == 0)) continue;
-                    }
-                    const double4 s = tile[k];
+                auto term = [&](const double4 s) {
                     const double dx = (gb.x + xi) - s.x;
                     const double dy = (gb.y + yi) - s.y;
                     const double dz = (gb.z + zi) - s.z;
@@ -85,6 +102,26 @@ __global__ __launch_bounds__(kBlock) void k_basic_exact(
                     a_x += prefact * dx;
                     a_y += prefact * dy;
                     a_z += prefact * dz;
+                };
+                // Tiles without this block's targets (and, with ignored terms,
+                // without sources 0 and 1) need no per-pair test: the terms
+                // are then independent work the compiler can overlap, while
+                // the sums still add them in ascending j.
+                const bool check = (pass == 0 && t < blk_hi && t + n > blk_lo) || (ignore != 0 && t < 2);
+                if (!check) {
+#pragma unroll 4
+                    for (int k = 0; k < n; k++) term(tile[k]);
+                    continue;
+                }
+                for (int k = 0; k < n; k++) {
+                    const int j = t + k;
+                    if (pass == 0) {
+                        if (skip_pair(i, j, ignore)) continue;
+                    } else {
+                        if (ignore == 1 && j == 1 && i == 0) continue;
+                        if (ignore == 2 && (j == 0 || i == 0)) continue;
+                    }
+                    term(tile[k]);
                 }
             }
         }
@@ -113,6 +150,7 @@ __global__ __launch_bounds__(kBlock) void k_comp_exact(
     __shared__ double4 tile[kBlock];
     const int i = blockIdx.x * kBlock + threadIdx.x;
     const bool live = i < N;
+    const int blk_lo = blockIdx.x * kBlock, blk_hi = blk_lo + kBlock;
     const double xi = live ? x[i] : 0., yi = live ? y[i] : 0., zi = live ? z[i] : 0.;
     double a_x = 0., a_y = 0., a_z = 0., c_x = 0., c_y = 0., c_z = 0.;
     for (int pass = 0; pass < 2; pass++) {
@@ -126,10 +164,7 @@ __global__ __launch_bounds__(kBlock) void k_comp_exact(
             __syncthreads();
             const int n = min(kBlock, j1 - t);
             if (!live || (pass == 1 && i >= N_active)) continue;
-            for (int k = 0; k < n; k++) {
-                const int j = t + k;
-                if (skip_pair(i, j, ignore)) continue;
-                const double4 s = tile[k];
+            auto term = [&](const double4 s) {
                 const double dx = xi - s.x;
                 const double dy = yi - s.y;
                 const double dz = zi - s.z;
@@ -140,6 +175,17 @@ __global__ __launch_bounds__(kBlock) void k_comp_exact(
                 kahan_add(prefactj * dx, a_x, c_x);
                 kahan_add(prefactj * dy, a_y, c_y);
                 kahan_add(prefactj * dz, a_z, c_z);
+            };
+            const bool check = (t < blk_hi && t + n > blk_lo) || (ignore != 0 && t < 2);
+            if (!check) {
+#pragma unroll 4
+                for (int k = 0; k < n; k++) term(tile[k]);
+                continue;
+            }
+            for (int k = 0; k < n; k++) {
+                const int j = t + k;
+                if (skip_pair(i, j, ignore)) continue;
+                term(tile[k]);
             }
         }
     }
@@ -183,7 +229,7 @@ __device__ __forceinline__ void pair_tile(const double4* __restrict__ tile, cons
                                           int jbase, int i, int ignore, double gx, double gy, double gz,
                                           double soft2, double& sx, double& sy, double& sz, double& cx, double& cy,
                                           double& cz, double c15, double c1875) {
-#pragma unroll 4
+    RB_UNROLL(RB_FAST_UNROLL)
     for (int k = 0; k < n; k++) {
         const double4 s = tile[k];
         const double dx = gx - s.x;
@@ -214,30 +260,42 @@ __device__ __forceinline__ void pair_tile(const double4* __restrict__ tile, cons
     }
 }
 
-// Eight wavefronts per SIMD: at N=65536 the grid (256 target blocks x 8
-// source chunks) is then exactly one wave of resident workgroups on 256 CUs.
-template <int MODE>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_direct_fast(
-    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
-    const double* __restrict__ m, int n_recv, int j_begin, int j_end, int chunk, int ignore, double soft2,
-    GBList gl, double* __restrict__ partial, int pstride) {
-    __shared__ double4 tile[kBlock];
-    __shared__ float mtile[kBlock];
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+// Ghost-box offsets of a pass: the list of the simulation (kernel argument) or
+// the single zero offset of a simulation without ghost boxes.
+struct GBFromList {
+    const GBList& gl;
+    __device__ int n() const { return gl.n; }
+    __device__ GB3 get(int g) const { return gl.g[g]; }
+};
+struct GBZero {
+    __device__ int n() const { return 1; }
+    __device__ GB3 get(int) const { return GB3{0., 0., 0.}; }
+};
+
+// One workgroup of the pair kernel: targets bx*kFast.. of n_recv, sources of
+// chunk `by` of [j_begin, j_end).  Writes the chunk's sums and carries to
+// `partial` (pstride doubles per component).
+template <int MODE, class GBS>
+__device__ __forceinline__ void direct_fast_block(const double* __restrict__ x, const double* __restrict__ y,
+                                                  const double* __restrict__ z, const double* __restrict__ m,
+                                                  int n_recv, int j_begin, int j_end, int chunk, int ignore,
+                                                  double soft2, const GBS& gbs, double* __restrict__ partial,
+                                                  int pstride, int bx, int by, double4* tile, float* mtile) {
+    const int i = bx * kFast + threadIdx.x;
     const bool live = i < n_recv;
     const double xi = live ? x[i] : 0., yi = live ? y[i] : 0., zi = live ? z[i] : 0.;
-    const int j0 = j_begin + (int)blockIdx.y * chunk;
+    const int j0 = j_begin + by * chunk;
     const int j1 = min(j_end, j0 + chunk);
-    const int blk_lo = blockIdx.x * kBlock;
-    const int blk_hi = blk_lo + kBlock;
+    const int blk_lo = bx * kFast;
+    const int blk_hi = blk_lo + kFast;
     double sx = 0., sy = 0., sz = 0.;      // running sums
     double cx = 0., cy = 0., cz = 0.;      // Kahan carries (COMP per pair, FP64/SPLIT per tile)
     const double c15 = vreg_const(1.5);
     const double c1875 = vreg_const(1.875);
-    for (int g = 0; g < gl.n; g++) {
-        const GB3 gb = gl.g[g];
+    for (int g = 0; g < gbs.n(); g++) {
+        const GB3 gb = gbs.get(g);
         const double gx = gb.x + xi, gy = gb.y + yi, gz = gb.z + zi;
-        for (int t = j0; t < j1; t += kBlock) {
+        for (int t = j0; t < j1; t += kFast) {
             __syncthreads();
             const int jj = t + (int)threadIdx.x;
             if (jj < j1) {
@@ -246,7 +304,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
                 mtile[threadIdx.x] = (float)mj;
             }
             __syncthreads();
-            const int n = min(kBlock, j1 - t);
+            const int n = min(kFast, j1 - t);
             // Only tiles holding a target of this block, or sources 0/1 when
             // terms are ignored, need the per-pair skip test.
             const bool check = (t < blk_hi && t + n > blk_lo) || (ignore != 0 && t < 2);
@@ -273,7 +331,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
         }
     }
     if (live) {
-        double* p = partial + (size_t)blockIdx.y * 6 * pstride;
+        double* p = partial + (size_t)by * 6 * pstride;
         p[0 * pstride + i] = sx;
         p[1 * pstride + i] = sy;
         p[2 * pstride + i] = sz;
@@ -283,13 +341,28 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
     }
 }
 
+// Four wavefronts per SIMD (128 VGPRs, no spills in the pair loop) measured
+// faster than eight on gfx950; at N=65536 the grid is 256 target blocks x 8
+// source chunks.
+template <int MODE>
+__global__ __launch_bounds__(kFast) __attribute__((amdgpu_waves_per_eu(RB_FAST_WAVES, RB_FAST_WAVES))) void k_direct_fast(
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+    const double* __restrict__ m, int n_recv, int j_begin, int j_end, int chunk, int ignore, double soft2,
+    GBList gl, double* __restrict__ partial, int pstride) {
+    __shared__ double4 tile[kFast];
+    __shared__ float mtile[kFast];
+    direct_fast_block<MODE>(x, y, z, m, n_recv, j_begin, j_end, chunk, ignore, soft2, GBFromList{gl}, partial,
+                            pstride, (int)blockIdx.x, (int)blockIdx.y, tile, mtile);
+}
+
 // Sum the chunk partials in chunk order (Kahan), apply -G and write (or add to)
-// the acceleration.  ignore==2 zeroes particle 0 (it feels nothing).
-__global__ void k_reduce(const double* __restrict__ partial, int pstride, int nchunk, int n_recv, double negG,
-                         int accumulate, int ignore, double* __restrict__ ax, double* __restrict__ ay,
-                         double* __restrict__ az) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_recv) return;
+// the acceleration.  ignore==2 zeroes particle 0 (it feels nothing).  With
+// kd.on, the leapfrog's kick and closing drift follow in the same pass
+// (integrator_leapfrog.c:84-93 then :72-82: v += dt*a, then x += dt/2*v),
+// which is safe because every read of the positions has finished.
+__device__ __forceinline__ void reduce_one(const double* __restrict__ partial, int pstride, int nchunk, int i,
+                                           double negG, int accumulate, int ignore, double* __restrict__ ax,
+                                           double* __restrict__ ay, double* __restrict__ az, const KickDrift& kd) {
     double out[3];
     for (int c = 0; c < 3; c++) {
         double s = 0., k = 0.;
@@ -302,14 +375,48 @@ __global__ void k_reduce(const double* __restrict__ partial, int pstride, int nc
     }
     if (ignore == 2 && i == 0) out[0] = out[1] = out[2] = 0.;
     if (accumulate) {
-        ax[i] += out[0];
-        ay[i] += out[1];
-        az[i] += out[2];
-    } else {
-        ax[i] = out[0];
-        ay[i] = out[1];
-        az[i] = out[2];
+        out[0] = ax[i] + out[0];
+        out[1] = ay[i] + out[1];
+        out[2] = az[i] + out[2];
     }
+    ax[i] = out[0];
+    ay[i] = out[1];
+    az[i] = out[2];
+    if (kd.on) {
+        const double vx = kd.vx[i] + kd.kick * out[0];
+        const double vy = kd.vy[i] + kd.kick * out[1];
+        const double vz = kd.vz[i] + kd.kick * out[2];
+        kd.vx[i] = vx;
+        kd.vy[i] = vy;
+        kd.vz[i] = vz;
+        kd.x[i] = kd.x[i] + kd.drift * vx;
+        kd.y[i] = kd.y[i] + kd.drift * vy;
+        kd.z[i] = kd.z[i] + kd.drift * vz;
+    }
+}
+
+__global__ void k_reduce(const double* __restrict__ partial, int pstride, int nchunk, int n_recv, double negG,
+                         int accumulate, int ignore, double* __restrict__ ax, double* __restrict__ ay,
+                         double* __restrict__ az, KickDrift kd) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_recv) return;
+    reduce_one(partial, pstride, nchunk, i, negG, accumulate, ignore, ax, ay, az, kd);
+}
+
+// The same kick and drift after a kernel that writes the accelerations itself.
+__global__ void k_kick_drift(const double* __restrict__ ax, const double* __restrict__ ay,
+                             const double* __restrict__ az, int n, KickDrift kd) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double vx = kd.vx[i] + kd.kick * ax[i];
+    const double vy = kd.vy[i] + kd.kick * ay[i];
+    const double vz = kd.vz[i] + kd.kick * az[i];
+    kd.vx[i] = vx;
+    kd.vy[i] = vy;
+    kd.vz[i] = vz;
+    kd.x[i] = kd.x[i] + kd.drift * vx;
+    kd.y[i] = kd.y[i] + kd.drift * vy;
+    kd.z[i] = kd.z[i] + kd.drift * vz;
 }
 
 __global__ void k_zero3(double* __restrict__ a, double* __restrict__ b, double* __restrict__ c, int n) {
@@ -317,14 +424,62 @@ __global__ void k_zero3(double* __restrict__ a, double* __restrict__ b, double* 
     if (i < n) a[i] = b[i] = c[i] = 0.;
 }
 
-// Number of source chunks for the fast modes: enough workgroups to put about
-// eight wavefronts on each SIMD of the 256-CU part, chunks no shorter than
-// four LDS tiles.
+// ---------------------------------------------------------------- many simulations per launch
+// One order-2 leapfrog step with direct gravity (no ghost boxes, no test
+// particles, nothing ignored) for a batch of independent simulations in three
+// launches: drift, pair workgroups of every simulation, then reduction with
+// the kick and closing drift.  Each simulation keeps its own chunking
+// (pick_chunks of its own N), so its bits are those of stepping it alone.
+__global__ __launch_bounds__(256) void k_seg_drift(const SegJob* __restrict__ jobs, const SegRange* __restrict__ rg) {
+    const SegRange R = rg[blockIdx.x];
+    const SegJob J = jobs[R.job];
+    const int i = R.first + (int)threadIdx.x;
+    if (i >= J.n) return;
+    J.x[i] += J.drift * J.vx[i];
+    J.y[i] += J.drift * J.vy[i];
+    J.z[i] += J.drift * J.vz[i];
+}
+
+__global__ __launch_bounds__(kFast) __attribute__((amdgpu_waves_per_eu(RB_FAST_WAVES, RB_FAST_WAVES))) void k_seg_direct(
+    const SegJob* __restrict__ jobs, const SegTile* __restrict__ tiles) {
+    __shared__ double4 tile[kFast];
+    __shared__ float mtile[kFast];
+    const SegTile T = tiles[blockIdx.x];
+    const SegJob J = jobs[T.job];
+    switch (J.mode) {
+        case FM_FP64:
+            direct_fast_block<FM_FP64>(J.x, J.y, J.z, J.m, J.n, 0, J.n, J.chunk, 0, J.soft2, GBZero{}, J.partial, J.n,
+                                       T.bx, T.by, tile, mtile);
+            break;
+        case FM_SPLIT:
+            direct_fast_block<FM_SPLIT>(J.x, J.y, J.z, J.m, J.n, 0, J.n, J.chunk, 0, J.soft2, GBZero{}, J.partial,
+                                        J.n, T.bx, T.by, tile, mtile);
+            break;
+        default:
+            direct_fast_block<FM_COMP>(J.x, J.y, J.z, J.m, J.n, 0, J.n, J.chunk, 0, J.soft2, GBZero{}, J.partial, J.n,
+                                       T.bx, T.by, tile, mtile);
+            break;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_seg_reduce(const SegJob* __restrict__ jobs, const SegRange* __restrict__ rg) {
+    const SegRange R = rg[blockIdx.x];
+    const SegJob J = jobs[R.job];
+    const int i = R.first + (int)threadIdx.x;
+    if (i >= J.n) return;
+    const KickDrift kd{J.x, J.y, J.z, J.vx, J.vy, J.vz, J.kick, J.drift, 1};
+    reduce_one(J.partial, J.n, J.nchunk, i, J.negG, 0, 0, J.ax, J.ay, J.az, kd);
+}
+
+// Number of source chunks for the fast modes: enough workgroups to fill every
+// SIMD of the 256-CU part several times over, chunks no shorter than one LDS
+// tile (short chunks keep the time of one step small for N of a few
+// thousand, where a single workgroup's chunk sets the kernel's duration).
 static int pick_chunks(int n_recv, int n_src, int ngb) {
-    const int blocks_x = (n_recv + kBlock - 1) / kBlock;
-    const int target_blocks = 2048;
+    const int blocks_x = (n_recv + kFast - 1) / kFast;
+    const int target_blocks = 2048 * 256 / kFast;
     int s = (target_blocks + blocks_x - 1) / blocks_x;
-    int max_s = (n_src * ngb) / (4 * kBlock);
+    int max_s = (n_src * ngb) / kFast;
     if (max_s < 1) max_s = 1;
     if (s > max_s) s = max_s;
     if (s > 64) s = 64;
@@ -334,34 +489,59 @@ static int pick_chunks(int n_recv, int n_src, int ngb) {
 
 static size_t workspace_bytes(int n, int nchunk) { return sizeof(double) * 6 * (size_t)n * (size_t)nchunk; }
 
+// The chunking a simulation of n bodies uses when stepped alone (launch_fast).
+void seg_plan(int n, int* chunk, int* used, size_t* workspace) {
+    const int nchunk = pick_chunks(n, n, 1);
+    int c = (n + nchunk - 1) / nchunk;
+    c = ((c + kFast - 1) / kFast) * kFast;
+    *chunk = c;
+    *used = n > 0 ? (n + c - 1) / c : 1;
+    *workspace = workspace_bytes(n, nchunk) + 256;
+}
+
+int seg_tiles_of(int n, int used) { return ((n + kFast - 1) / kFast) * used; }
+
+int seg_step_launch(const SegJob* jobs, const SegRange* ranges, int nranges, const SegTile* tiles, int ntiles,
+                    hipStream_t st) {
+    if (nranges == 0) return 0;
+    hipLaunchKernelGGL(k_seg_drift, dim3(nranges), dim3(256), 0, st, jobs, ranges);
+    hipLaunchKernelGGL(k_seg_direct, dim3(ntiles), dim3(kFast), 0, st, jobs, tiles);
+    hipLaunchKernelGGL(k_seg_reduce, dim3(nranges), dim3(256), 0, st, jobs, ranges);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int seg_fast_mode(int precision, int compensated) {
+    return compensated ? FM_COMP : (precision == REB_AMD_PRECISION_SPLIT ? FM_SPLIT : FM_FP64);
+}
+
 // Launch one fast-mode pass: targets [0,n_recv) feel sources [j_begin,j_end).
 static int launch_fast(int mode, const double* x, const double* y, const double* z, const double* m, int n_recv,
                        int j_begin, int j_end, int ignore, double G, double soft2, const GBList& gl,
                        double* partial, int nchunk, int accumulate, double* ax, double* ay, double* az,
-                       hipStream_t st, hipEvent_t* ev) {
+                       hipStream_t st, hipEvent_t* ev, const KickDrift& kd) {
     const int nsrc = j_end - j_begin;
     int chunk = (nsrc + nchunk - 1) / nchunk;
-    chunk = ((chunk + kBlock - 1) / kBlock) * kBlock;
+    chunk = ((chunk + kFast - 1) / kFast) * kFast;
     const int used = nsrc > 0 ? (nsrc + chunk - 1) / chunk : 1;
-    dim3 grid((n_recv + kBlock - 1) / kBlock, used);
+    dim3 grid((n_recv + kFast - 1) / kFast, used);
     if (ev) (void)hipEventRecord(ev[0], st);
     switch (mode) {
         case FM_FP64:
-            hipLaunchKernelGGL(k_direct_fast<FM_FP64>, grid, dim3(kBlock), 0, st, x, y, z, m, n_recv, j_begin, j_end,
+            hipLaunchKernelGGL(k_direct_fast<FM_FP64>, grid, dim3(kFast), 0, st, x, y, z, m, n_recv, j_begin, j_end,
                                chunk, ignore, soft2, gl, partial, n_recv);
             break;
         case FM_SPLIT:
-            hipLaunchKernelGGL(k_direct_fast<FM_SPLIT>, grid, dim3(kBlock), 0, st, x, y, z, m, n_recv, j_begin,
+            hipLaunchKernelGGL(k_direct_fast<FM_SPLIT>, grid, dim3(kFast), 0, st, x, y, z, m, n_recv, j_begin,
                                j_end, chunk, ignore, soft2, gl, partial, n_recv);
             break;
         default:
-            hipLaunchKernelGGL(k_direct_fast<FM_COMP>, grid, dim3(kBlock), 0, st, x, y, z, m, n_recv, j_begin, j_end,
+            hipLaunchKernelGGL(k_direct_fast<FM_COMP>, grid, dim3(kFast), 0, st, x, y, z, m, n_recv, j_begin, j_end,
                                chunk, ignore, soft2, gl, partial, n_recv);
             break;
     }
     if (ev) (void)hipEventRecord(ev[1], st);
     hipLaunchKernelGGL(k_reduce, dim3((n_recv + 255) / 256), dim3(256), 0, st, partial, n_recv, used, n_recv, -G,
-                       accumulate, ignore, ax, ay, az);
+                       accumulate, ignore, ax, ay, az, kd);
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
@@ -369,8 +549,10 @@ static int launch_fast(int mode, const double* x, const double* y, const double*
 static int direct_dispatch(const double* x, const double* y, const double* z, const double* m, double* ax,
                            double* ay, double* az, int N, int N_active, int tp_type, int ignore, double G,
                            double soft2, int precision, int compensated, const GBList& gl,
-                           double* workspace, hipStream_t st, hipEvent_t* ev = nullptr) {
+                           double* workspace, hipStream_t st, hipEvent_t* ev = nullptr,
+                           const KickDrift& kd = KickDrift{}) {
     if (N == 0) return 0;
+    const KickDrift none{};
     if (precision == REB_AMD_PRECISION_EXACT) {
         dim3 grid((N + kBlock - 1) / kBlock);
         if (ev) (void)hipEventRecord(ev[0], st);
@@ -381,23 +563,27 @@ static int direct_dispatch(const double* x, const double* y, const double* z, co
             hipLaunchKernelGGL(k_basic_exact, grid, dim3(kBlock), 0, st, x, y, z, m, ax, ay, az, N, N_active,
                                tp_type, ignore, G, soft2, gl);
         if (ev) (void)hipEventRecord(ev[1], st);
+        if (kd.on) hipLaunchKernelGGL(k_kick_drift, dim3((N + 255) / 256), dim3(256), 0, st, ax, ay, az, N, kd);
         return hipGetLastError() == hipSuccess ? 0 : 1;
     }
     const int mode = compensated ? FM_COMP : (precision == REB_AMD_PRECISION_SPLIT ? FM_SPLIT : FM_FP64);
     const int nchunk = pick_chunks(N, N_active, gl.n);
     int rc = 0;
+    const bool second = tp_type && N_active < N;
     if (N_active > 0) {
         rc |= launch_fast(mode, x, y, z, m, N, 0, N_active, ignore, G, soft2, gl, workspace, nchunk, 0, ax, ay, az,
-                          st, ev);
+                          st, ev, second ? none : kd);
     } else {
         hipLaunchKernelGGL(k_zero3, dim3((N + 255) / 256), dim3(256), 0, st, ax, ay, az, N);
     }
-    if (tp_type && N_active < N) {
+    if (second) {
         // Active targets also feel the test particles (gravity.c:259-276).
         const int nchunk2 = pick_chunks(N_active, N - N_active, gl.n);
         rc |= launch_fast(mode, x, y, z, m, N_active, N_active, N, ignore, G, soft2, gl, workspace,
-                          nchunk2 < nchunk ? nchunk2 : nchunk, 1, ax, ay, az, st, nullptr);
+                          nchunk2 < nchunk ? nchunk2 : nchunk, 1, ax, ay, az, st, nullptr, none);
     }
+    if (kd.on && (second || N_active == 0))
+        hipLaunchKernelGGL(k_kick_drift, dim3((N + 255) / 256), dim3(256), 0, st, ax, ay, az, N, kd);
     return rc;
 }
 
@@ -411,7 +597,9 @@ bool gravity_zero(struct reb_simulation* r) {
     return RB_HIP(r, hipGetLastError());
 }
 
-bool gravity_direct(struct reb_simulation* r, bool compensated) {
+// kick_drift (optional): {kick dt, drift dt} of a leapfrog step, applied in the
+// pass that finishes the accelerations.
+bool gravity_direct(struct reb_simulation* r, bool compensated, const double* kick_drift) {
     DeviceState* d = dev_of(r);
     const int N = (int)r->N;
     if (N == 0) return true;
@@ -448,9 +636,14 @@ bool gravity_direct(struct reb_simulation* r, bool compensated) {
         ev = &d->kev[d->kev_used];
         d->kev_used += 2;
     }
+    KickDrift kd{};
+    if (kick_drift) {
+        kd = KickDrift{d->field(FX), d->field(FY), d->field(FZ), d->field(FVX), d->field(FVY), d->field(FVZ),
+                       kick_drift[0], kick_drift[1], 1};
+    }
     int rc = direct_dispatch(d->field(FX), d->field(FY), d->field(FZ), d->field(FM), d->field(FAX), d->field(FAY),
                              d->field(FAZ), N, N_active, r->testparticle_type, (int)r->gravity_ignore_terms, r->G,
-                             soft2, d->precision, compensated ? 1 : 0, gl, d->partial.as<double>(), d->stream, ev);
+                             soft2, d->precision, compensated ? 1 : 0, gl, d->partial.as<double>(), d->stream, ev, kd);
     if (rc) return RB_HIP(r, hipGetLastError());
     return true;
 }
@@ -560,6 +753,21 @@ __global__ __launch_bounds__(256) void k_energy_fold(const double* __restrict__ 
     }
 }
 
+int energy_launch(const double* x, const double* y, const double* z, const double* vx, const double* vy,
+                  const double* vz, const double* m, int n, int n_active, int n_interact, double G, bool exact,
+                  double* out, hipStream_t st) {
+    if (exact && n <= 8192) {
+        hipLaunchKernelGGL(k_energy_exact, dim3(1), dim3(64), 0, st, x, y, z, vx, vy, vz, m, n_active, n_interact, G,
+                           out);
+    } else {
+        double* rows = out + 16;
+        hipLaunchKernelGGL(k_energy_rows, dim3((n_interact + 255) / 256), dim3(256), 0, st, x, y, z, vx, vy, vz, m,
+                           n_active, n_interact, G, rows);
+        hipLaunchKernelGGL(k_energy_fold, dim3(1), dim3(256), 0, st, rows, n_interact, out);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
 double energy_device(struct reb_simulation* r) {
     DeviceState* d = dev_of(r);
     const size_t N = r->N;
@@ -571,16 +779,11 @@ double energy_device(struct reb_simulation* r) {
         return nan("");
     }
     double* out = d->scratch.as<double>();
-    if (d->precision == REB_AMD_PRECISION_EXACT && N <= 8192) {
-        hipLaunchKernelGGL(k_energy_exact, dim3(1), dim3(64), 0, d->stream, d->field(FX), d->field(FY), d->field(FZ),
-                           d->field(FVX), d->field(FVY), d->field(FVZ), d->field(FM), (int)N_active, (int)N_interact,
-                           r->G, out);
-    } else {
-        double* rows = out + 16;
-        hipLaunchKernelGGL(k_energy_rows, dim3((N_interact + 255) / 256), dim3(256), 0, d->stream, d->field(FX),
-                           d->field(FY), d->field(FZ), d->field(FVX), d->field(FVY), d->field(FVZ), d->field(FM),
-                           (int)N_active, (int)N_interact, r->G, rows);
-        hipLaunchKernelGGL(k_energy_fold, dim3(1), dim3(256), 0, d->stream, rows, (int)N_interact, out);
+    if (energy_launch(d->field(FX), d->field(FY), d->field(FZ), d->field(FVX), d->field(FVY), d->field(FVZ),
+                      d->field(FM), (int)N, (int)N_active, (int)N_interact, r->G,
+                      d->precision == REB_AMD_PRECISION_EXACT, out, d->stream)) {
+        (void)RB_HIP(r, hipGetLastError());
+        return nan("");
     }
     double h[2] = {0, 0};
     if (!RB_HIP(r, hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, d->stream))) return nan("");

@@ -88,9 +88,6 @@ struct DeviceState {
     int gravity_tile = 0;           // autotuned j-split (0 = pick)
     TreeBuffers* tree = nullptr;
     WHFastDevice* whfast = nullptr;
-    // per-category device time of the last steps()/integrate() call (ms)
-    hipEvent_t ev[8] = {nullptr};
-    double prof_ms[4] = {0, 0, 0, 0};
     // kernel timing: when `profiling` is set, every launch of the dominant
     // pair kernel of direct gravity is bracketed by an event pair from `kev`
     bool profiling = false;
@@ -100,6 +97,8 @@ struct DeviceState {
     // download r->particles; reb_amd_sync_host / reb_amd_mark_host_modified
     // move the state explicitly
     bool resident = false;
+    // the stream was handed in by the library's serving loop (not destroyed here)
+    bool owns_stream = true;
     // value returned by reb_amd_coefficient_of_restitution_constant
     double cor_constant = 1.0;
     double* field(int f) const { return soa.as<double>() + (size_t)f * cap; }
@@ -129,7 +128,34 @@ std::vector<struct reb_vec6d> ghostboxes(struct reb_simulation* r, int nx, int n
 
 // ------------------------------------------------------------------ kernels (launchers)
 // Direct O(N^2) gravity (gravity.c:167-531) on the simulation's device state.
-bool gravity_direct(struct reb_simulation* r, bool compensated);
+bool gravity_direct(struct reb_simulation* r, bool compensated, const double* kick_drift = nullptr);
+
+// Batched order-2 leapfrog steps with direct gravity for many independent
+// simulations (gravity.hip k_seg_*).  A SegJob holds one simulation's device
+// arrays, the chunking it would use alone (seg_plan) and its constants.
+struct SegJob {
+    double *x, *y, *z, *vx, *vy, *vz, *ax, *ay, *az;
+    const double* m;
+    double* partial;
+    int n, chunk, nchunk, mode;
+    double negG, soft2, kick, drift;
+};
+struct SegTile {     // one pair workgroup: job, target block, source chunk
+    int job, bx, by;
+};
+struct SegRange {    // 256 bodies of one job
+    int job, first;
+};
+void seg_plan(int n, int* chunk, int* used, size_t* workspace);
+int seg_tiles_of(int n, int used);
+int seg_step_launch(const SegJob* jobs, const SegRange* ranges, int nranges, const SegTile* tiles, int ntiles,
+                    hipStream_t st);
+int seg_fast_mode(int precision, int compensated);
+// Energy (tools.c:108-162) of device arrays into out[0] (kinetic) and out[1]
+// (potential) on the stream; `scratch` holds 2 n + 16 doubles.
+int energy_launch(const double* x, const double* y, const double* z, const double* vx, const double* vy,
+                  const double* vz, const double* m, int n, int n_active, int n_interact, double G, bool exact,
+                  double* scratch, hipStream_t st);
 bool gravity_zero(struct reb_simulation* r);
 bool gravity_tree(struct reb_simulation* r);                 // tree.hip
 bool collision_search_device(struct reb_simulation* r);      // tree.hip / collision

@@ -20,14 +20,19 @@
 // advances every active system by min(remaining, tick) steps; systems that
 // reach their horizon stop inside the launch and synchronize at its end.
 // Jobs admitted while a tick runs join at the next tick; finished systems
-// leave, and their slots and bodies are reused.  N-body jobs run on their own
-// streams from worker threads (one simulation each, state resident in HBM),
-// so their kernels overlap with the pool's and with each other on the device.
+// leave, and their slots and bodies are reused.  N-body jobs are batched the
+// same way on a second stream: every scheduler round advances all of them by
+// the same number of leapfrog steps, three launches per step for the whole
+// batch (gravity.hip k_seg_*), each job with the chunking it would use alone;
+// they join and leave between rounds.  Jobs in EXACT precision run as single
+// simulations from worker threads on a third stream.  The streams overlap on
+// the device.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <limits>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -62,6 +67,9 @@ struct Job {
     size_t body0 = 0;
     uint64_t remaining = 0;
     bool fresh = false;
+    // N-body batch: the job's device arrays (x y z vx vy vz ax ay az m, n apart)
+    DevBuf soa, partial, scratch;
+    int chunk = 0, used = 0;
 };
 
 // Pinned staging buffer that grows.
@@ -146,10 +154,23 @@ struct Server {
     bool stop = false;
     std::thread sched;
     std::vector<std::thread> workers;    // N-body workers, max_nbody of them
+    std::vector<hipStream_t> streams;    // N-body streams (worker w uses w % size)
     std::deque<Job*> nbody_ready;        // admitted N-body jobs waiting for a worker
+    double tick_target_s = 0.002;        // wall time a pool tick aims at
+    double step_s = 0.;                  // measured pool time per step
     std::chrono::steady_clock::time_point t0;
     std::vector<Job*> pool_active;       // scheduler thread only
     Pool pool;
+    // N-body batch (scheduler thread only)
+    hipStream_t nb_stream = nullptr;
+    std::vector<Job*> nb_active;
+    bool nb_dirty = true;
+    int nb_ranges = 0, nb_tiles = 0;
+    DevBuf nb_jobs, nb_rng, nb_til;
+    Pinned nb_meta;
+    double nb_step_s = 0.;               // device time per batch step (events)
+    hipEvent_t nb_ev[2] = {nullptr, nullptr};
+    unsigned long long nb_rounds = 0;
     std::string err;
     unsigned long long done = 0, failed = 0;
 };
@@ -383,10 +404,13 @@ static bool pool_tick(Server* S) {
     Pool& P = S->pool;
     auto& act = S->pool_active;
     if (act.empty()) return true;
+    // Ticks aim at ~2 ms of device time, so jobs join and leave promptly.
+    long tick_max = S->tick_steps;
+    if (S->step_s > 0.) tick_max = std::max(16L, std::min(tick_max, (long)(S->tick_target_s / S->step_s)));
     long tick = 0;
     for (Job* j : act) {
         WHSys& w = P.hsys[j->slot];
-        const uint64_t k = std::min<uint64_t>(j->remaining, (uint64_t)S->tick_steps);
+        const uint64_t k = std::min<uint64_t>(j->remaining, (uint64_t)tick_max);
         w.steps = (int)k;
         w.req = (k == j->remaining ? 1 : 0) | (j->fresh ? 2 : 0) | (j->safe_mode ? 4 : 0);
         tick = std::max<long>(tick, (long)k);
@@ -465,16 +489,22 @@ static bool pool_tick(Server* S) {
     } else {
         POOL_HIP(S, hipStreamSynchronize(P.stream));
     }
-    P.busy_s += since(S) - t_start;
+    const double took = since(S) - t_start;
+    P.busy_s += took;
     P.ticks++;
+    if (tick >= 16) S->step_s = S->step_s > 0. ? 0.7 * S->step_s + 0.3 * took / (double)tick : took / (double)tick;
     act.swap(keep);
     for (Job* j : fin) finish_job(S, j, true, "");
     return true;
 }
 
-static void run_nbody(Server* S, Job* j) {
+static void run_nbody(Server* S, Job* j, hipStream_t stream) {
     (void)hipSetDevice(S->device);
     struct reb_simulation* r = reb_simulation_create();
+    DeviceState* d = dev_of(r);
+    d->device = S->device;
+    d->stream = stream;        // the worker's stream, kept by the server
+    d->owns_stream = false;
     r->save_messages = 1;
     r->G = j->G;
     r->softening = j->softening;
@@ -503,7 +533,163 @@ static void run_nbody(Server* S, Job* j) {
     finish_job(S, j, ok, msg);
 }
 
-static void nbody_worker(Server* S) {
+// ------------------------------------------------------------------ N-body batch
+static bool nb_fail(Server* S, const char* what) {
+    S->err = std::string("N-body batch: ") + what;
+    set_last_error(S->err);
+    return false;
+}
+
+static SegJob seg_of(const Job* j) {
+    SegJob J;
+    const size_t n = j->bodies.size();
+    double* b = j->soa.as<double>();
+    J.x = b; J.y = b + n; J.z = b + 2 * n;
+    J.vx = b + 3 * n; J.vy = b + 4 * n; J.vz = b + 5 * n;
+    J.ax = b + 6 * n; J.ay = b + 7 * n; J.az = b + 8 * n;
+    J.m = b + 9 * n;
+    J.partial = j->partial.as<double>();
+    J.n = (int)n;
+    J.chunk = j->chunk;
+    J.nchunk = j->used;
+    J.mode = seg_fast_mode(j->precision, j->gravity == REB_GRAVITY_COMPENSATED ? 1 : 0);
+    J.negG = -j->G;
+    J.soft2 = j->softening * j->softening;
+    J.kick = j->dt;
+    J.drift = j->dt * 0.5;
+    return J;
+}
+
+// Energy (tools.c:108-162) of a batched job; waits for the stream.
+static bool nb_energy(Server* S, Job* j, double* e) {
+    const SegJob J = seg_of(j);
+    const int n = J.n;
+    if (energy_launch(J.x, J.y, J.z, J.vx, J.vy, J.vz, J.m, n, n, n, j->G, false, j->scratch.as<double>(),
+                      S->nb_stream))
+        return nb_fail(S, "energy launch");
+    double h[2];
+    if (hipMemcpyAsync(h, j->scratch.p, sizeof(h), hipMemcpyDeviceToHost, S->nb_stream) != hipSuccess ||
+        hipStreamSynchronize(S->nb_stream) != hipSuccess)
+        return nb_fail(S, "energy copy");
+    *e = h[0] + h[1];
+    return true;
+}
+
+static bool nb_admit(Server* S, Job* j) {
+    const size_t n = j->bodies.size();
+    size_t ws = 0;
+    seg_plan((int)n, &j->chunk, &j->used, &ws);
+    if (!j->soa.ensure(sizeof(double) * 10 * n) || !j->partial.ensure(ws) ||
+        !j->scratch.ensure(sizeof(double) * (2 * n + 16)))
+        return nb_fail(S, "out of device memory");
+    if (!S->nb_meta.ensure(sizeof(double) * 10 * n + 64)) return nb_fail(S, "staging");
+    double* h = static_cast<double*>(S->nb_meta.p);
+    for (size_t i = 0; i < n; i++) {
+        const struct reb_particle& p = j->bodies[i];
+        const double f[10] = {p.x, p.y, p.z, p.vx, p.vy, p.vz, p.ax, p.ay, p.az, p.m};
+        for (int k = 0; k < 10; k++) h[k * n + i] = f[k];
+    }
+    if (hipMemcpyAsync(j->soa.p, h, sizeof(double) * 10 * n, hipMemcpyHostToDevice, S->nb_stream) != hipSuccess)
+        return nb_fail(S, "upload");
+    if (!nb_energy(S, j, &j->e0)) return false;
+    j->remaining = j->n_steps;
+    S->nb_dirty = true;
+    return true;
+}
+
+// Uploads the job, range and tile tables of the active batch.
+static bool nb_tables(Server* S) {
+    std::vector<SegJob> jobs;
+    std::vector<SegRange> rng;
+    std::vector<SegTile> til;
+    for (size_t q = 0; q < S->nb_active.size(); q++) {
+        const Job* j = S->nb_active[q];
+        jobs.push_back(seg_of(j));
+        const int n = (int)j->bodies.size();
+        for (int f = 0; f < n; f += 256) rng.push_back(SegRange{(int)q, f});
+        const int bxs = seg_tiles_of(n, 1);
+        for (int by = 0; by < j->used; by++)
+            for (int bx = 0; bx < bxs; bx++) til.push_back(SegTile{(int)q, bx, by});
+    }
+    const size_t bj = sizeof(SegJob) * jobs.size(), br = sizeof(SegRange) * rng.size(),
+                 bt = sizeof(SegTile) * til.size();
+    // the tables go through pinned memory and replace tables in use: wait for
+    // the stream first
+    if (hipStreamSynchronize(S->nb_stream) != hipSuccess) return nb_fail(S, "stream");
+    if (!S->nb_jobs.ensure(bj) || !S->nb_rng.ensure(br) || !S->nb_til.ensure(bt) ||
+        !S->nb_meta.ensure(bj + br + bt + 64))
+        return nb_fail(S, "out of device memory for the batch tables");
+    char* h = static_cast<char*>(S->nb_meta.p);
+    memcpy(h, jobs.data(), bj);
+    memcpy(h + bj, rng.data(), br);
+    memcpy(h + bj + br, til.data(), bt);
+    if (hipMemcpyAsync(S->nb_jobs.p, h, bj, hipMemcpyHostToDevice, S->nb_stream) != hipSuccess ||
+        hipMemcpyAsync(S->nb_rng.p, h + bj, br, hipMemcpyHostToDevice, S->nb_stream) != hipSuccess ||
+        hipMemcpyAsync(S->nb_til.p, h + bj + br, bt, hipMemcpyHostToDevice, S->nb_stream) != hipSuccess)
+        return nb_fail(S, "table upload");
+    S->nb_ranges = (int)rng.size();
+    S->nb_tiles = (int)til.size();
+    S->nb_dirty = false;
+    return true;
+}
+
+// Queues one round of the batch: every job advances by the same number of
+// steps (no job past its horizon); returns the step count.
+static long nb_launch_round(Server* S) {
+    if (S->nb_active.empty()) return 0;
+    if (S->nb_dirty && !nb_tables(S)) return -1;
+    uint64_t k = std::numeric_limits<uint64_t>::max();
+    for (Job* j : S->nb_active) k = std::min(k, j->remaining);
+    long target = S->nb_step_s > 0. ? (long)(S->tick_target_s / S->nb_step_s) : 8;
+    target = std::max(1L, std::min(target, 4096L));
+    const long steps = (long)std::min<uint64_t>(k, (uint64_t)target);
+    if (steps > 0 && S->nb_ev[0]) (void)hipEventRecord(S->nb_ev[0], S->nb_stream);
+    for (long q = 0; q < steps; q++)
+        if (seg_step_launch(S->nb_jobs.as<SegJob>(), S->nb_rng.as<SegRange>(), S->nb_ranges,
+                            S->nb_til.as<SegTile>(), S->nb_tiles, S->nb_stream))
+            return -1;
+    if (steps > 0 && S->nb_ev[1]) (void)hipEventRecord(S->nb_ev[1], S->nb_stream);
+    return steps;
+}
+
+// After the round's kernels: collects the jobs that reached their horizon.
+static bool nb_collect(Server* S, long steps) {
+    if (hipStreamSynchronize(S->nb_stream) != hipSuccess) return nb_fail(S, "stream");
+    float ms = 0.f;
+    if (steps > 0 && S->nb_ev[0] && hipEventElapsedTime(&ms, S->nb_ev[0], S->nb_ev[1]) == hipSuccess) {
+        const double per = 1e-3 * ms / (double)steps;
+        S->nb_step_s = S->nb_step_s > 0. ? 0.7 * S->nb_step_s + 0.3 * per : per;
+        S->nb_rounds++;
+    }
+    std::vector<Job*> keep, fin;
+    for (Job* j : S->nb_active) {
+        j->remaining -= (uint64_t)steps;
+        (j->remaining == 0 ? fin : keep).push_back(j);
+    }
+    for (Job* j : fin) {
+        const size_t n = j->bodies.size();
+        if (!nb_energy(S, j, &j->e1)) return false;
+        if (!S->nb_meta.ensure(sizeof(double) * 10 * n + 64)) return nb_fail(S, "staging");
+        double* h = static_cast<double*>(S->nb_meta.p);
+        if (hipMemcpyAsync(h, j->soa.p, sizeof(double) * 10 * n, hipMemcpyDeviceToHost, S->nb_stream) != hipSuccess ||
+            hipStreamSynchronize(S->nb_stream) != hipSuccess)
+            return nb_fail(S, "download");
+        for (size_t i = 0; i < n; i++) {
+            struct reb_particle& p = j->bodies[i];
+            double* f[10] = {&p.x, &p.y, &p.z, &p.vx, &p.vy, &p.vz, &p.ax, &p.ay, &p.az, &p.m};
+            for (int k = 0; k < 10; k++) *f[k] = h[k * n + i];
+        }
+        j->soa.release();
+        j->partial.release();
+        j->scratch.release();
+    }
+    if (!fin.empty()) S->nb_dirty = true;
+    S->nb_active.swap(keep);
+    for (Job* j : fin) finish_job(S, j, true, "");
+    return true;
+}
+
+static void nbody_worker(Server* S, hipStream_t stream) {
     (void)hipSetDevice(S->device);
     for (;;) {
         Job* j = nullptr;
@@ -514,7 +700,7 @@ static void nbody_worker(Server* S) {
             j = S->nbody_ready.front();
             S->nbody_ready.pop_front();
         }
-        run_nbody(S, j);
+        run_nbody(S, j, stream);
     }
 }
 
@@ -522,10 +708,10 @@ static void scheduler(Server* S) {
     (void)hipSetDevice(S->device);
     unsigned long long seen = ~0ull;
     for (;;) {
-        std::vector<Job*> fresh;
+        std::vector<Job*> fresh, nb_fresh;
         {
             std::unique_lock<std::mutex> lk(S->mu);
-            if (S->pool_active.empty())
+            if (S->pool_active.empty() && S->nb_active.empty())
                 S->cv_work.wait(lk, [&] { return S->stop || S->generation != seen; });
             if (S->stop) break;
             seen = S->generation;
@@ -545,11 +731,25 @@ static void scheduler(Server* S) {
                 j->t_admit = since(S);
                 if (j->kind == JOB_NBODY) {
                     S->nbody_running++;
-                    S->nbody_ready.push_back(j);
-                    S->cv_work.notify_all();
+                    if (j->precision == REB_AMD_PRECISION_EXACT) {
+                        S->nbody_ready.push_back(j);
+                        S->cv_work.notify_all();
+                    } else {
+                        nb_fresh.push_back(j);
+                    }
                 } else {
                     fresh.push_back(j);
                 }
+            }
+        }
+        for (Job* j : nb_fresh) {
+            if (nb_admit(S, j)) {
+                S->nb_active.push_back(j);
+            } else {
+                j->soa.release();
+                j->partial.release();
+                j->scratch.release();
+                finish_job(S, j, false, S->err);
             }
         }
         if (!fresh.empty()) {
@@ -559,10 +759,24 @@ static void scheduler(Server* S) {
                 for (Job* j : fresh) finish_job(S, j, false, S->err);
             }
         }
+        // the N-body round is queued first so that it runs beside the pool tick
+        const long nb_steps = nb_launch_round(S);
         if (!S->pool_active.empty() && !pool_tick(S)) {
             std::vector<Job*> act;
             act.swap(S->pool_active);
             for (Job* j : act) finish_job(S, j, false, S->err);
+        }
+        if (nb_steps < 0 || (!S->nb_active.empty() && !nb_collect(S, nb_steps))) {
+            std::vector<Job*> act;
+            act.swap(S->nb_active);
+            (void)hipStreamSynchronize(S->nb_stream);
+            for (Job* j : act) {
+                j->soa.release();
+                j->partial.release();
+                j->scratch.release();
+                finish_job(S, j, false, S->err);
+            }
+            S->nb_dirty = true;
         }
     }
 }
@@ -587,19 +801,31 @@ REB_API struct reb_amd_server* reb_amd_server_create(size_t memory_budget_bytes,
     Server* S = &h->s;
     (void)hipGetDevice(&S->device);
     S->budget = memory_budget_bytes ? memory_budget_bytes : ((size_t)8 << 30);
-    S->max_nbody = max_nbody_jobs > 0 ? max_nbody_jobs : 8;
+    S->max_nbody = max_nbody_jobs > 0 ? max_nbody_jobs : 32;
     S->tick_steps = whfast_tick_steps > 0 ? whfast_tick_steps : 2000;
     S->t0 = std::chrono::steady_clock::now();
     S->pool.device = S->device;
     S->pool.max_resident = whfast_max_resident(S->device);
     if (hipStreamCreateWithFlags(&S->pool.stream, hipStreamNonBlocking) != hipSuccess ||
-        !S->pool.counter.ensure(256) || hipMemset(S->pool.counter.p, 0, 256) != hipSuccess) {
+        !S->pool.counter.ensure(256) || hipMemsetAsync(S->pool.counter.p, 0, 256, S->pool.stream) != hipSuccess ||
+        hipStreamSynchronize(S->pool.stream) != hipSuccess) {
         set_last_error("reb_amd_server_create: could not set up the device pool");
         delete h;
         return nullptr;
     }
     S->sched = std::thread(scheduler, S);
-    for (int w = 0; w < S->max_nbody; w++) S->workers.emplace_back(nbody_worker, S);
+    // Three streams (planetary pool, N-body batch, EXACT N-body jobs): within
+    // the four hardware queues a process gets by default, so no kernel waits
+    // behind another stream's launch.
+    hipStream_t st = nullptr;
+    if (hipStreamCreateWithFlags(&S->nb_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&S->nb_ev[0]) != hipSuccess || hipEventCreate(&S->nb_ev[1]) != hipSuccess) {
+        set_last_error("reb_amd_server_create: could not create a stream");
+    } else {
+        S->streams.push_back(st);
+        for (int w = 0; w < 2; w++) S->workers.emplace_back(nbody_worker, S, st);
+    }
     return h;
 }
 
@@ -640,6 +866,18 @@ REB_API void reb_amd_server_free(struct reb_amd_server* h) {
     P.h_stage.release();
     P.h_meta.release();
     (void)hipStreamDestroy(P.stream);
+    if (S->nb_stream) {
+        (void)hipStreamSynchronize(S->nb_stream);
+        (void)hipStreamDestroy(S->nb_stream);
+    }
+    for (DevBuf* b : {&S->nb_jobs, &S->nb_rng, &S->nb_til}) b->release();
+    for (hipEvent_t e : S->nb_ev)
+        if (e) (void)hipEventDestroy(e);
+    S->nb_meta.release();
+    for (hipStream_t st : S->streams) {
+        (void)hipStreamSynchronize(st);
+        (void)hipStreamDestroy(st);
+    }
     delete h;
 }
 

@@ -38,6 +38,11 @@
 namespace rbamd {
 
 static constexpr int kMaxGBT = 64;   // keeps the by-value argument under 4 KB
+// `info` block: ints 0..7 (build results and flags), bytes 32..39 the
+// interaction counter, bytes 64.. the per-block top-two radii of k_top2
+// (at most kTop2Blocks blocks, two doubles each).
+static constexpr int kTop2Blocks = 64;
+static constexpr size_t kInfoBytes = 64 + 2 * sizeof(double) * kTop2Blocks;
 struct GBT {
     int n;
     double g[kMaxGBT][6];
@@ -335,7 +340,7 @@ static bool tree_alloc(struct reb_simulation* r, TreeBuffers* T, size_t n) {
     while (cap < n) cap *= 2;
     bool ok = T->key0.ensure(8 * cap) && T->key1.ensure(8 * cap) && T->idx.ensure(4 * cap) &&
               T->key0b.ensure(8 * cap) && T->key1b.ensure(8 * cap) && T->idxb.ensure(4 * cap) &&
-              T->lcp.ensure(4 * cap) && T->cnt.ensure(4 * cap) && T->base.ensure(4 * cap) && T->info.ensure(64);
+              T->lcp.ensure(4 * cap) && T->cnt.ensure(4 * cap) && T->base.ensure(4 * cap) && T->info.ensure(kInfoBytes);
     if (!ok) {
         reb_simulation_error(r, "Out of device memory for the tree.");
         return false;
@@ -878,7 +883,7 @@ bool collision_search_device(struct reb_simulation* r) {
             if (!tree_build(r)) return false;
             // second largest radius
             double* top = reinterpret_cast<double*>(T->info.as<int>() + 16);
-            const int nb = std::min(64, (n + 255) / 256);
+            const int nb = std::min(kTop2Blocks, (n + 255) / 256);
             hipLaunchKernelGGL(k_top2, dim3(nb), dim3(256), 0, d->stream, d->field(FR), n, top);
             std::vector<double> h(2 * nb);
             if (!RB_HIP(r, hipMemcpyAsync(h.data(), top, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, d->stream)))

@@ -567,12 +567,64 @@ static void gen_merge(void){
     reb_simulation_free(g);
 }
 
+/* Boundaries during integration (OpenMP build: its per-target sums over
+ * ghost boxes are the order the device follows).  OPEN removes bodies that
+ * leave the box; PERIODIC wraps them, with ghost boxes in the force. */
+static struct reb_simulation* boxed(int boundary, int gravity, size_t N, double root, int nr, unsigned int seed){
+    struct reb_simulation* r = reb_simulation_create();
+    r->rand_seed = seed;
+    r->boundary = boundary;
+    r->gravity = gravity;
+    r->root_size = root; r->N_root_x = nr; r->N_root_y = nr; r->N_root_z = 1;
+    if (boundary == REB_BOUNDARY_PERIODIC){ r->N_ghost_x = 1; r->N_ghost_y = 1; r->N_ghost_z = 0; }
+    r->softening = 0.1;
+    r->opening_angle2 = 0.25;
+    r->dt = 0.01;
+    reb_simulation_set_integrator(r, "leapfrog");
+    double b = root*nr;
+    for (size_t i=0; i<N; i++){
+        struct reb_particle p = {0};
+        p.x = reb_random_uniform(r, -b/2., b/2.);
+        p.y = reb_random_uniform(r, -b/2., b/2.);
+        p.z = reb_random_uniform(r, -root/2., root/2.);
+        p.vx = reb_random_normal(r, 4.);
+        p.vy = reb_random_normal(r, 4.);
+        p.vz = reb_random_normal(r, 1.);
+        p.m = reb_random_uniform(r, 0.5, 1.5)*1e-2;
+        reb_simulation_add(r, p);
+    }
+    return r;
+}
+
+static void gen_boundaries(void){
+    /* OPEN: 200 bodies in a 10-box; the N history every 10 steps and the end state */
+    struct reb_simulation* r = boxed(REB_BOUNDARY_OPEN, REB_GRAVITY_BASIC, 200, 10., 1, 21u);
+    dump_state("open200_init.npy", r);
+    int64_t nh[11]; nh[0] = (int64_t)r->N;
+    for (int c=1; c<=10; c++){ reb_simulation_steps(r, 10); nh[c] = (int64_t)r->N; }
+    wi("open200_nhist.npy", nh, 11, 0);
+    dump_state("open200_final.npy", r);
+    reb_simulation_free(r);
+    /* PERIODIC, direct gravity with one ring of ghost boxes */
+    r = boxed(REB_BOUNDARY_PERIODIC, REB_GRAVITY_BASIC, 300, 4., 3, 22u);
+    dump_state("periodic300_init.npy", r);
+    reb_simulation_steps(r, 50);
+    dump_state("periodic300_final_basic.npy", r);
+    reb_simulation_free(r);
+    /* PERIODIC, tree gravity */
+    r = boxed(REB_BOUNDARY_PERIODIC, REB_GRAVITY_TREE, 300, 4., 3, 22u);
+    reb_simulation_steps(r, 20);
+    dump_state("periodic300_final_tree.npy", r);
+    reb_simulation_free(r);
+}
+
 int main(int argc, char** argv){
     if (argc < 3){ fprintf(stderr, "usage: gen_golden OUTDIR serial|omp\n"); return 1; }
     outdir = argv[1];
     if (strcmp(argv[2], "omp") == 0){
         gen_plummer_omp();
         gen_ghostbox_basic("omp");
+        gen_boundaries();
         return 0;
     }
     gen_rng();

@@ -160,3 +160,30 @@ def test_sei_operator_bitwise_vs_oracle(gpu, orc):
     ref[:, 6:9] = orc.gravity_basic(ref, soft=0.1)
     orc.lib().orc_sei_operator(ref, 500, 1.0, 1.0, dt, 1)
     assert np.array_equal(sim.state()[:, :6], ref[:, :6])
+
+
+@pytest.mark.parametrize("precision", ["exact", "fp64", "split"])
+@pytest.mark.parametrize("gravity", ["basic", "compensated"])
+@pytest.mark.parametrize("testparticles", [False, True])
+def test_leapfrog_fused_kick_drift_bitwise(gpu, precision, gravity, testparticles):
+    """With direct gravity the leapfrog's kick and closing drift run in the
+    pass that finishes the accelerations; an additional_forces callback (here
+    a no-op) keeps them separate.  Both give the same bits."""
+    st = golden("plummer1024_seed1_state.npy")[:700]
+    out = []
+    for split in (False, True):
+        sim = sim_from(gpu, st, softening=0.01, dt=1e-3)
+        sim.set_integrator("leapfrog")
+        sim.s.gravity = gpu.GRAVITY[gravity]
+        sim.precision = precision
+        if testparticles:
+            sim.s.N_active = 600
+            sim.s.testparticle_type = 1
+        if split:
+            cb = gpu._abi.HEARTBEAT(lambda r: None)
+            sim._keep.append(cb)
+            sim.s.additional_forces = ctypes.cast(cb, ctypes.c_void_p)
+        sim.steps(20)
+        out.append(sim.state())
+        sim.free()
+    assert np.array_equal(out[0][:, :9], out[1][:, :9])

@@ -248,3 +248,16 @@ def test_collision_direct_first_step(orc):
     # tree search finds the same overlapping, approaching pairs as the direct search
     assert set(map(tuple, pairs)) == set(map(tuple, pairs_t))
     assert len(pairs) > 100
+
+
+def test_whfast_flop_count(orc, amd):
+    """The restatement's FLOP counter (used to price the ensemble kernel):
+    linear in the step count, and at least the pair forces plus one Kepler
+    solve (22 + 14 + 33 + 30 series FLOPs) per planet and step."""
+    nb, st, dt = amd.planetary_systems(64, 3, 2, 8)
+    f100 = orc.whfast_flops_per_system_step(nb, st, dt, 100)
+    f200 = orc.whfast_flops_per_system_step(nb, st, dt, 200)
+    assert abs(f200 / f100 - 1.0) < 0.05
+    K = (nb - 1).astype(float)
+    floor = (20 * K * (K - 1) + 99 * K).mean()
+    assert floor < f100 < 3 * floor
