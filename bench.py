@@ -357,6 +357,44 @@ def serving_cpu(args, jobs):
                       "steps" % (r["jobs"], r["scale"], r["seconds"])}
 
 
+def small_leg(amd, args):
+    """The plain method at the reference's CPU-sized case: Plummer N=1024,
+    softening 0.01, leapfrog dt=1e-3, 1000 steps, BASIC in EXACT precision (bit-
+    identical to the reference, so the energy drift is the reference's)."""
+    sim = plummer_sim(amd, 1024, 1, 0.01, 1e-3, "exact")
+    e0 = sim.energy()
+    L = amd.lib()
+    L.reb_amd_set_resident(sim.ptr, 1)
+    sim.steps(10)
+    L.reb_amd_synchronize_device(sim.ptr)
+    t0 = time.perf_counter()
+    sim.steps(1000)
+    L.reb_amd_synchronize_device(sim.ptr)
+    wall = time.perf_counter() - t0
+    L.reb_amd_set_resident(sim.ptr, 0)
+    e1 = sim.energy()
+    sim.free()
+    pairs = 1024.0 * 1023.0
+    res = {"metric": "Plummer N=1024 leapfrog steps/s (EXACT, bit-identical)", "value": 1000 / wall,
+           "unit": "steps/s", "us_per_step": 1e6 * wall / 1000, "gpairs_per_s": pairs * 1000 / wall / 1e9,
+           "rel_energy_drift_1010_steps": (e1 - e0) / abs(e0)}
+    log("small: N=1024 %.0f steps/s (%.1f us/step)" % (res["value"], res["us_per_step"]))
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_bench")
+    if not args.no_cpu and os.path.exists(exe):
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+        out = subprocess.run([exe, "plummer", "1024", "1000", "0.01", "0.001", "basic", "1"], env=env,
+                             capture_output=True, text=True, timeout=600)
+        if out.returncode == 0:
+            r = json.loads(out.stdout.strip().splitlines()[-1])
+            res["cpu_baseline"] = {"value": r["steps"] / r["seconds"], "unit": "steps/s", "cores": r["threads"],
+                                   "kind": "reference",
+                                   "sample": "reference OpenMP build, the same case, 1000 steps after 1 warm-up step"}
+            res["speedup_vs_cpu"] = res["value"] / res["cpu_baseline"]["value"]
+            log("small cpu: %.0f steps/s on %d threads" % (res["cpu_baseline"]["value"], r["threads"]))
+    return res
+
+
 def sheet_leg(amd, args):
     """Tree gravity + tree collisions: shearing sheet of N=200000 hard spheres
     (10 x 10 root boxes of 136 radii: optical depth ~1), SEI, 2 ghost boxes,
@@ -457,7 +495,7 @@ def main():
     ap.add_argument("--sheet-seed", type=int, default=99)
     ap.add_argument("--sheet-steps", type=int, default=500)
     ap.add_argument("--sheet-cpu-steps", type=int, default=3)
-    ap.add_argument("--only", choices=["direct", "ensemble", "serve", "sheet"], default=None)
+    ap.add_argument("--only", choices=["direct", "ensemble", "serve", "sheet", "small"], default=None)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device (counter calibration)")
     args = ap.parse_args()
@@ -482,6 +520,9 @@ def main():
     if args.only == "sheet":
         print(json.dumps({"sheet": sheet_leg(amd, args)}))
         return
+    if args.only == "small":
+        print(json.dumps({"small": small_leg(amd, args)}))
+        return
     head, _ = run_mode(amd, args, "fp64", "fp64", "basic", args.steps, args.warmup)
     modes = {"fp64": head}
     if not args.quick:
@@ -496,6 +537,7 @@ def main():
     ens = None if (args.quick or args.only == "direct") else ensemble_leg(amd, args)
     serving = None if (args.quick or args.only == "direct") else serving_leg(amd, args)
     sheet = None if (args.quick or args.only == "direct") else sheet_leg(amd, args)
+    small = None if (args.quick or args.only == "direct") else small_leg(amd, args)
 
     achieved = head["algorithmic_tflops"]
     line = {
@@ -523,6 +565,7 @@ def main():
         "ensemble": ens,
         "serving": serving,
         "sheet": sheet,
+        "small": small,
     }
     print(json.dumps(line))
 

@@ -188,6 +188,9 @@ REB_API long reb_amd_server_submit_whfast(struct reb_amd_server* s, size_t n_bod
 REB_API long reb_amd_server_submit_nbody(struct reb_amd_server* s, size_t n, const struct reb_particle* bodies,
                                          double G, double softening, double dt, uint64_t n_steps, int gravity,
                                          int precision);
+/* Device-memory footprint of a job, as admission counts it. */
+REB_API size_t reb_amd_server_whfast_bytes(size_t n_bodies);
+REB_API size_t reb_amd_server_nbody_bytes(size_t n, int precision);
 REB_API int reb_amd_server_status(struct reb_amd_server* s, long job);
 /* Waits for completion (timeout_s < 0: no limit); returns the job state. */
 REB_API int reb_amd_server_wait(struct reb_amd_server* s, long job, double timeout_s);

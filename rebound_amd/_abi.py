@@ -198,6 +198,8 @@ def declare(lib):
         "reb_amd_server_submit_nbody": (ctypes.c_long, [c_void_p, c_size_t, POINTER(Particle), c_double, c_double,
                                                         c_double, ctypes.c_uint64, c_int, c_int]),
         "reb_amd_server_status": (c_int, [c_void_p, ctypes.c_long]),
+        "reb_amd_server_whfast_bytes": (c_size_t, [c_size_t]),
+        "reb_amd_server_nbody_bytes": (c_size_t, [c_size_t, c_int]),
         "reb_amd_server_wait": (c_int, [c_void_p, ctypes.c_long, c_double]),
         "reb_amd_server_result": (c_int, [c_void_p, ctypes.c_long, POINTER(Particle), c_size_t, POINTER(c_double),
                                           POINTER(c_double)]),

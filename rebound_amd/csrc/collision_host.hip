@@ -10,13 +10,109 @@
 // hard-sphere resolver uses the host libm (atan2/sin/cos) like the reference.
 // Only the bodies named in the collision list travel between device and host
 // when the resolver is one of the built-ins that touch p1 and p2 alone.
+//
+// The built-in hard-sphere resolver with a constant (or no) coefficient of
+// restitution touches nothing but p1, p2 and the two logging fields, so the
+// shuffled list is split into levels: a collision's level is one more than the
+// last level that touched either of its bodies.  The collisions of one level
+// touch disjoint bodies and are resolved in parallel on host threads; every
+// body still sees its collisions in list order, and the pressure log is summed
+// in list order afterwards, so the result is the sequential one, bit for bit.
+#include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
-#include <unordered_map>
+#include <functional>
+#include <mutex>
+#include <thread>
 
 #include "internal.h"
 
 namespace rbamd {
+
+// A small persistent pool of host threads for the level-parallel resolve.
+class HostPool {
+  public:
+    static HostPool& get() {
+        static HostPool pool;
+        return pool;
+    }
+    int size() const { return (int)threads_.size() + 1; }
+    // Runs fn(i) for i in [0, n) on the pool and the calling thread.
+    void run(size_t n, const std::function<void(size_t)>& fn) {
+        if (n < 256 || threads_.empty()) {
+            for (size_t i = 0; i < n; i++) fn(i);
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            fn_ = &fn;
+            n_ = n;
+            next_.store(0);
+            busy_ = (int)threads_.size();
+            gen_++;
+        }
+        cv_.notify_all();
+        work();
+        std::unique_lock<std::mutex> lk(mu_);
+        done_.wait(lk, [&] { return busy_ == 0; });
+        fn_ = nullptr;
+    }
+
+  private:
+    HostPool() {
+        unsigned hw = std::thread::hardware_concurrency();
+        int k = (int)std::min<unsigned>(hw ? hw : 1, 16u) - 1;
+        for (int t = 0; t < k; t++) threads_.emplace_back([this] { loop(); });
+    }
+    ~HostPool() {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            stop_ = true;
+            gen_++;
+        }
+        cv_.notify_all();
+        for (auto& t : threads_) t.join();
+    }
+    void work() {
+        const size_t chunk = 64;
+        for (;;) {
+            const size_t b = next_.fetch_add(chunk);
+            if (b >= n_) break;
+            const size_t e = std::min(n_, b + chunk);
+            for (size_t i = b; i < e; i++) (*fn_)(i);
+        }
+    }
+    void loop() {
+        unsigned long long seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+                if (stop_) return;
+                seen = gen_;
+            }
+            work();
+            std::lock_guard<std::mutex> g(mu_);
+            if (--busy_ == 0) done_.notify_all();
+        }
+    }
+    std::vector<std::thread> threads_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_;
+    const std::function<void(size_t)>* fn_ = nullptr;
+    size_t n_ = 0;
+    std::atomic<size_t> next_{0};
+    int busy_ = 0;
+    unsigned long long gen_ = 0;
+    bool stop_ = false;
+};
+
+// collision.c:573-665 on the host copy: applies one hard-sphere collision to
+// p1 and p2 and returns 1 with the pressure-log term in *plog if it did.
+static int hardsphere_apply(const struct reb_simulation* r, struct reb_particle* particles,
+                            const struct reb_collision& c, double* plog);
 
 // Gathers / scatters the full state of a list of bodies.
 __global__ void k_gather_bodies(const double* __restrict__ soa, size_t cap, const int* __restrict__ ids, int n,
@@ -33,6 +129,27 @@ __global__ void k_scatter_bodies(double* __restrict__ soa, size_t cap, const int
     if (k >= n) return;
     const int i = ids[k];
     for (int f = 0; f < NFIELD; f++) soa[(size_t)f * cap + i] = in[(size_t)k * NFIELD + f];
+}
+
+// The fields the hard-sphere resolver reads (x y z vx vy vz m r) of listed
+// bodies, and the velocities it writes back.
+__global__ void k_gather_hs(const double* __restrict__ soa, size_t cap, const int* __restrict__ ids, int n,
+                            double* __restrict__ out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const size_t i = (size_t)ids[k];
+    const int f[8] = {FX, FY, FZ, FVX, FVY, FVZ, FM, FR};
+    for (int q = 0; q < 8; q++) out[(size_t)q * n + k] = soa[(size_t)f[q] * cap + i];
+}
+
+__global__ void k_scatter_v(double* __restrict__ soa, size_t cap, const int* __restrict__ ids, int n,
+                            const double* __restrict__ in) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const size_t i = (size_t)ids[k];
+    soa[(size_t)FVX * cap + i] = in[k];
+    soa[(size_t)FVY * cap + i] = in[(size_t)n + k];
+    soa[(size_t)FVZ * cap + i] = in[2 * (size_t)n + k];
 }
 
 static void copy_body_in(struct reb_particle& p, const double* s) {
@@ -57,7 +174,17 @@ static bool move_bodies(struct reb_simulation* r, const std::vector<int>& ids, b
     if (!d->aos.ensure(idbytes + databytes + 256)) return false;
     int* dids = d->aos.as<int>();
     double* ddata = reinterpret_cast<double*>(d->aos.as<char>() + ((idbytes + 255) / 256) * 256);
-    std::vector<double> h((size_t)NFIELD * n);
+    // pinned staging: the simulation's host stage holds NFIELD * cap doubles (n <= cap)
+    std::vector<double> fallback;
+    double* hd = d->host_stage;
+    if (!hd || (size_t)n > d->host_cap) {
+        fallback.resize((size_t)NFIELD * n);
+        hd = fallback.data();
+    }
+    struct {
+        double* p;
+        double* data() { return p; }
+    } h{hd};
     if (!RB_HIP(r, hipMemcpyAsync(dids, ids.data(), idbytes, hipMemcpyHostToDevice, d->stream))) return false;
     if (to_host) {
         hipLaunchKernelGGL(k_gather_bodies, dim3((n + 127) / 128), dim3(128), 0, d->stream, d->soa.as<double>(),
@@ -93,6 +220,101 @@ static int rand_r_step(unsigned int* seed) {
     return result;
 }
 
+// The built-in hard-sphere resolver with a constant restitution, on a compact
+// host copy of the involved bodies: only the eight fields it reads come over,
+// only the velocities go back, and collisions of one level run in parallel.
+static void resolve_hardsphere_compact(struct reb_simulation* r) {
+    DeviceState* d = dev_of(r);
+    const size_t nc = r->N_collisions;
+    std::vector<int> loc(r->N, -1), ids;
+    ids.reserve(2 * nc);
+    std::vector<struct reb_collision> cl(r->collisions, r->collisions + nc);
+    for (auto& c : cl) {
+        for (size_t* p : {&c.p1, &c.p2}) {
+            int& l = loc[*p];
+            if (l < 0) {
+                l = (int)ids.size();
+                ids.push_back((int)*p);
+            }
+            *p = (size_t)l;
+        }
+    }
+    const int n = (int)ids.size();
+    // levels: one more than the last level that touched either body
+    std::vector<int> level(nc), last(n, 0);
+    int nlev = 0;
+    for (size_t k = 0; k < nc; k++) {
+        const int l = std::max(last[cl[k].p1], last[cl[k].p2]) + 1;
+        level[k] = l;
+        last[cl[k].p1] = l;
+        last[cl[k].p2] = l;
+        nlev = std::max(nlev, l);
+    }
+    std::vector<size_t> start(nlev + 2, 0), order(nc);
+    for (size_t k = 0; k < nc; k++) start[level[k] + 1]++;
+    for (int l = 1; l <= nlev + 1; l++) start[l] += start[l - 1];
+    std::vector<size_t> fill(start.begin(), start.end());
+    for (size_t k = 0; k < nc; k++) order[fill[level[k]]++] = k;
+    // bring the eight fields over (pinned staging)
+    const size_t idbytes = ((sizeof(int) * n + 255) / 256) * 256;
+    if (!d->aos.ensure(idbytes + sizeof(double) * 8 * n + 256)) {
+        reb_simulation_error(r, "Out of device memory for the collision resolve.");
+        return;
+    }
+    int* dids = d->aos.as<int>();
+    double* ddata = reinterpret_cast<double*>(d->aos.as<char>() + idbytes);
+    std::vector<double> fallback;
+    double* h = d->host_stage;
+    if (!h || (size_t)n * 8 > d->host_cap * NFIELD) {
+        fallback.resize((size_t)8 * n);
+        h = fallback.data();
+    }
+    if (!RB_HIP(r, hipMemcpyAsync(dids, ids.data(), sizeof(int) * n, hipMemcpyHostToDevice, d->stream))) return;
+    hipLaunchKernelGGL(k_gather_hs, dim3((n + 255) / 256), dim3(256), 0, d->stream, d->soa.as<double>(), d->cap,
+                       dids, n, ddata);
+    if (!RB_HIP(r, hipMemcpyAsync(h, ddata, sizeof(double) * 8 * n, hipMemcpyDeviceToHost, d->stream))) return;
+    if (!stream_sync(r)) return;
+    std::vector<struct reb_particle> P(n);
+    for (int k = 0; k < n; k++) {
+        struct reb_particle& p = P[k];
+        memset(&p, 0, sizeof(p));
+        p.x = h[k];
+        p.y = h[(size_t)n + k];
+        p.z = h[2 * (size_t)n + k];
+        p.vx = h[3 * (size_t)n + k];
+        p.vy = h[4 * (size_t)n + k];
+        p.vz = h[5 * (size_t)n + k];
+        p.m = h[6 * (size_t)n + k];
+        p.r = h[7 * (size_t)n + k];
+    }
+    std::vector<double> plog(nc, 0.);
+    std::vector<unsigned char> hit(nc, 0);
+    for (int l = 1; l <= nlev; l++) {
+        const size_t b = start[l], e = start[l + 1];
+        HostPool::get().run(e - b, [&](size_t q) {
+            const size_t k = order[b + q];
+            hit[k] = (unsigned char)hardsphere_apply(r, P.data(), cl[k], &plog[k]);
+        });
+    }
+    // the pressure log in list order (collision.c:655-663)
+    for (size_t k = 0; k < nc; k++) {
+        if (!hit[k]) continue;
+        r->collisions_plog += plog[k];
+        r->collisions_log_n++;
+    }
+    for (int k = 0; k < n; k++) {
+        h[k] = P[k].vx;
+        h[(size_t)n + k] = P[k].vy;
+        h[2 * (size_t)n + k] = P[k].vz;
+    }
+    if (!RB_HIP(r, hipMemcpyAsync(ddata, h, sizeof(double) * 3 * n, hipMemcpyHostToDevice, d->stream))) return;
+    hipLaunchKernelGGL(k_scatter_v, dim3((n + 255) / 256), dim3(256), 0, d->stream, d->soa.as<double>(), d->cap,
+                       dids, n, ddata);
+    (void)RB_HIP(r, hipGetLastError());
+    (void)stream_sync(r);   // the staging buffer is reused by the next transfer
+    mark_device_newer(r);
+}
+
 void collision_resolve_host(struct reb_simulation* r) {
     // shuffle (collision.c:336-342)
     for (size_t i = 0; i < r->N_collisions; i++) {
@@ -105,16 +327,25 @@ void collision_resolve_host(struct reb_simulation* r) {
     enum REB_COLLISION_RESOLVE_OUTCOME (*resolve)(struct reb_simulation* const, struct reb_collision) =
         r->collision_resolve;
     if (resolve == nullptr) resolve = reb_collision_resolve_halt;
+    if (resolve == reb_collision_resolve_hardsphere &&
+        (r->coefficient_of_restitution == nullptr ||
+         r->coefficient_of_restitution == reb_amd_coefficient_of_restitution_constant) &&
+        dev_of(r)->loc != LOC_HOST) {
+        resolve_hardsphere_compact(r);
+        return;
+    }
     // The built-in hard-sphere and halt resolvers touch only p1 and p2: move
     // just those bodies.  Anything else sees the whole host state.
     const bool local = (resolve == reb_collision_resolve_hardsphere || resolve == reb_collision_resolve_halt);
     std::vector<int> ids;
     if (local) {
-        std::unordered_map<size_t, int> seen;
-        seen.reserve(2 * r->N_collisions);
+        std::vector<unsigned char> seen(r->N, 0);
         for (size_t k = 0; k < r->N_collisions; k++) {
             for (size_t p : {r->collisions[k].p1, r->collisions[k].p2})
-                if (seen.emplace(p, 1).second) ids.push_back((int)p);
+                if (p < r->N && !seen[p]) {
+                    seen[p] = 1;
+                    ids.push_back((int)p);
+                }
         }
         if (!move_bodies(r, ids, true)) return;
     } else {
@@ -174,7 +405,20 @@ extern "C" {
 // collision.c:573-665
 REB_API enum REB_COLLISION_RESOLVE_OUTCOME reb_collision_resolve_hardsphere(struct reb_simulation* const r,
                                                                             struct reb_collision c) {
-    struct reb_particle* const particles = r->particles;
+    double plog = 0.;
+    if (rbamd::hardsphere_apply(r, r->particles, c, &plog)) {
+        r->collisions_plog += plog;
+        r->collisions_log_n++;
+    }
+    return REB_COLLISION_RESOLVE_OUTCOME_REMOVE_NONE;
+}
+
+}  // extern "C"
+
+namespace rbamd {
+
+static int hardsphere_apply(const struct reb_simulation* r, struct reb_particle* particles,
+                            const struct reb_collision& c, double* plog) {
     struct reb_particle p1 = particles[c.p1];
     struct reb_particle p2 = particles[c.p2];
     struct reb_vec6d gb = c.gb;
@@ -188,11 +432,11 @@ REB_API enum REB_COLLISION_RESOLVE_OUTCOME reb_collision_resolve_hardsphere(stru
     } else {
         oldvyouter = p2.vy;
     }
-    if (rp * rp < x21 * x21 + y21 * y21 + z21 * z21) return REB_COLLISION_RESOLVE_OUTCOME_REMOVE_NONE;
+    if (rp * rp < x21 * x21 + y21 * y21 + z21 * z21) return 0;
     double vx21 = p1.vx + gb.vx - p2.vx;
     double vy21 = p1.vy + gb.vy - p2.vy;
     double vz21 = p1.vz + gb.vz - p2.vz;
-    if (vx21 * x21 + vy21 * y21 + vz21 * z21 > 0) return REB_COLLISION_RESOLVE_OUTCOME_REMOVE_NONE;
+    if (vx21 * x21 + vy21 * y21 + vz21 * z21 > 0) return 0;
     // rotate the pair into the xy plane, then onto the x axis.  The reference
     // library, built by gcc, evaluates each sin/cos pair with one glibc
     // sincos() call (its results differ from separate sin and cos calls in
@@ -229,14 +473,16 @@ REB_API enum REB_COLLISION_RESOLVE_OUTCOME reb_collision_resolve_hardsphere(stru
     particles[c.p1].vy += p1pf * dvy2nn;
     particles[c.p1].vz += p1pf * dvz2nn;
     if (x21 > 0) {
-        r->collisions_plog += -fabs(x21) * (oldvyouter - particles[c.p1].vy) * p1.m;
-        r->collisions_log_n++;
+        *plog = -fabs(x21) * (oldvyouter - particles[c.p1].vy) * p1.m;
     } else {
-        r->collisions_plog += -fabs(x21) * (oldvyouter - particles[c.p2].vy) * p2.m;
-        r->collisions_log_n++;
+        *plog = -fabs(x21) * (oldvyouter - particles[c.p2].vy) * p2.m;
     }
-    return REB_COLLISION_RESOLVE_OUTCOME_REMOVE_NONE;
+    return 1;
 }
+
+}  // namespace rbamd
+
+extern "C" {
 
 // collision.c:667-672
 REB_API enum REB_COLLISION_RESOLVE_OUTCOME reb_collision_resolve_halt(struct reb_simulation* const r,

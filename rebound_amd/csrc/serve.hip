@@ -941,6 +941,10 @@ REB_API long reb_amd_server_submit_nbody(struct reb_amd_server* h, size_t n, con
     return submit(&h->s, std::move(j));
 }
 
+// Device-memory footprints used for admission.
+REB_API size_t reb_amd_server_whfast_bytes(size_t n_bodies) { return whfast_footprint(n_bodies); }
+REB_API size_t reb_amd_server_nbody_bytes(size_t n, int precision) { return nbody_footprint(n, precision); }
+
 REB_API int reb_amd_server_status(struct reb_amd_server* h, long job) {
     Server* S = &h->s;
     std::lock_guard<std::mutex> g(S->mu);

@@ -56,6 +56,7 @@ struct TreeBuffers {
     DevBuf node_i;                                // 4 ints per cell: next pt level start
     DevBuf info;                                  // total, maxlevel, flags
     DevBuf coll_cnt, coll_off, coll_rec;          // collision search
+    DevBuf grec, crec;                            // 64-byte walk records per cell (gravity, collisions)
     size_t body_cap = 0, node_cap = 0, rec_cap = 0;
     int nnodes = 0;
     int maxlevel = 0;
@@ -333,6 +334,45 @@ __global__ void k_tree_moments(double* __restrict__ nd, int* __restrict__ ni, si
     A.mz[c] = mz;
 }
 
+// One cell as the walks read it: 64 bytes, so that a wavefront's (uniform)
+// cell fetch is one scalar load, and the pre-order successor and the skip
+// target can both be fetched while the current cell is evaluated.
+struct __align__(64) GravRec {
+    double mx, my, mz, m, w;
+    int next, pt;
+    double pad;
+};
+struct __align__(64) CollCell {
+    double cx, cy, cz, w;
+    int next, pt;
+    double pad[3];
+};
+
+__global__ void k_tree_pack(const double* __restrict__ nd, const int* __restrict__ ni, size_t cap, int total,
+                            GravRec* __restrict__ g, CollCell* __restrict__ cc) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= total) return;
+    GravRec a;
+    a.mx = nd[5 * cap + c];
+    a.my = nd[6 * cap + c];
+    a.mz = nd[7 * cap + c];
+    a.m = nd[4 * cap + c];
+    a.w = nd[3 * cap + c];
+    a.next = ni[c];
+    a.pt = ni[cap + c];
+    a.pad = 0.;
+    g[c] = a;
+    CollCell b;
+    b.cx = nd[c];
+    b.cy = nd[cap + c];
+    b.cz = nd[2 * cap + c];
+    b.w = a.w;
+    b.next = a.next;
+    b.pt = a.pt;
+    b.pad[0] = b.pad[1] = b.pad[2] = 0.;
+    cc[c] = b;
+}
+
 // ---------------------------------------------------------------- host: build
 static bool tree_alloc(struct reb_simulation* r, TreeBuffers* T, size_t n) {
     if (n <= T->body_cap && T->key0.p) return true;
@@ -485,26 +525,25 @@ bool tree_build(struct reb_simulation* r) {
     for (int lev = T->maxlevel; lev >= 0; lev--)
         hipLaunchKernelGGL(k_tree_moments, dim3((total + 255) / 256), dim3(256), 0, d->stream,
                            T->node_d.as<double>(), T->node_i.as<int>(), T->node_cap, total, lev);
+    if (!T->grec.ensure(sizeof(GravRec) * (size_t)total) || !T->crec.ensure(sizeof(CollCell) * (size_t)total)) {
+        reb_simulation_error(r, "Out of device memory for the tree cells.");
+        return false;
+    }
+    hipLaunchKernelGGL(k_tree_pack, dim3((total + 255) / 256), dim3(256), 0, d->stream, T->node_d.as<double>(),
+                       T->node_i.as<int>(), T->node_cap, total, T->grec.as<GravRec>(), T->crec.as<CollCell>());
     return RB_HIP(r, hipGetLastError());
 }
 
 // ---------------------------------------------------------------- tree gravity
 // tree.c:275-328 for every ghost box (gravity.c:84-102), in the reference's
 // expressions; 64 bodies (in key order) per wavefront.
-__global__ __launch_bounds__(64) void k_tree_gravity(const double* __restrict__ nd, const int* __restrict__ ni,
-                                                     size_t cap, int total, const int* __restrict__ idx, int n,
+__global__ __launch_bounds__(64) void k_tree_gravity(const GravRec* __restrict__ rec, int total,
+                                                     const int* __restrict__ idx, int n,
                                                      const double* __restrict__ x, const double* __restrict__ y,
                                                      const double* __restrict__ z, GBT gl, double G, double soft2,
                                                      double theta2, double* __restrict__ ax,
                                                      double* __restrict__ ay, double* __restrict__ az,
                                                      unsigned long long* __restrict__ counter) {
-    const double* __restrict__ Cm = nd + 4 * cap;
-    const double* __restrict__ Cmx = nd + 5 * cap;
-    const double* __restrict__ Cmy = nd + 6 * cap;
-    const double* __restrict__ Cmz = nd + 7 * cap;
-    const double* __restrict__ Cw = nd + 3 * cap;
-    const int* __restrict__ Cnext = ni;
-    const int* __restrict__ Cpt = ni + cap;
     const int s = blockIdx.x * 64 + threadIdx.x;
     const bool live = s < n;
     const int me = live ? idx[s] : -1;
@@ -515,38 +554,43 @@ __global__ __launch_bounds__(64) void k_tree_gravity(const double* __restrict__ 
         const double gx = gl.g[g][0] + xi, gy = gl.g[g][1] + yi, gz = gl.g[g][2] + zi;
         int skip = live ? 0 : 0x7fffffff;
         int c = 0;
+        GravRec cur = rec[0];
         while (c < total) {
-            c = __builtin_amdgcn_readfirstlane(c);
-            const int pt = Cpt[c];
-            const int nx = Cnext[c];
+            // both possible successors are fetched before this cell is evaluated
+            const int c_open = c + 1, c_skip = cur.next;
+            const GravRec r_open = rec[c_open < total ? c_open : total - 1];
+            const GravRec r_skip = rec[c_skip < total ? c_skip : total - 1];
+            const int pt = cur.pt;
             bool open = false;
             if (c >= skip) {
-                const double dx = gx - Cmx[c];
-                const double dy = gy - Cmy[c];
-                const double dz = gz - Cmz[c];
+                const double dx = gx - cur.mx;
+                const double dy = gy - cur.my;
+                const double dz = gz - cur.mz;
                 const double r2 = dx * dx + dy * dy + dz * dz;
                 bool use = true;
                 if (pt < 0) {
-                    const double w = Cw[c];
+                    const double w = cur.w;
                     if (w * w > theta2 * r2) {
                         open = true;
                         use = false;
                     } else {
-                        skip = nx;
+                        skip = c_skip;
                     }
                 } else if (pt == me) {
                     use = false;
                 }
                 if (use) {
                     const double _r = sqrt(r2 + soft2);
-                    const double prefact = -G / (_r * _r * _r) * Cm[c];
+                    const double prefact = -G / (_r * _r * _r) * cur.m;
                     a_x += prefact * dx;
                     a_y += prefact * dy;
                     a_z += prefact * dz;
                     inter++;
                 }
             }
-            c = (__ballot(open) != 0ull) ? c + 1 : nx;
+            const bool any = __ballot(open) != 0ull;
+            c = any ? c_open : c_skip;
+            cur = any ? r_open : r_skip;
         }
     }
     if (live) {
@@ -586,8 +630,8 @@ bool gravity_tree(struct reb_simulation* r) {
     }
     unsigned long long* counter = reinterpret_cast<unsigned long long*>(T->info.as<int>() + 8);
     if (!RB_HIP(r, hipMemsetAsync(counter, 0, 8, d->stream))) return false;
-    hipLaunchKernelGGL(k_tree_gravity, dim3((n + 63) / 64), dim3(64), 0, d->stream, T->node_d.as<double>(),
-                       T->node_i.as<int>(), T->node_cap, T->nnodes, T->idx.as<int>(), n, d->field(FX),
+    hipLaunchKernelGGL(k_tree_gravity, dim3((n + 63) / 64), dim3(64), 0, d->stream, T->grec.as<GravRec>(),
+                       T->nnodes, T->idx.as<int>(), n, d->field(FX),
                        d->field(FY), d->field(FZ), gl, r->G, r->softening * r->softening, r->opening_angle2,
                        d->field(FAX), d->field(FAY), d->field(FAZ), counter);
     mark_device_newer(r);
@@ -639,20 +683,14 @@ struct CollRec {
 // collisions of each body, WRITE=true stores them at the body's offset in
 // detection order (ghost box, root box, depth-first).
 template <bool WRITE>
-__global__ __launch_bounds__(64) void k_tree_collide(const double* __restrict__ nd, const int* __restrict__ ni,
-                                                     size_t cap, int total, const int* __restrict__ idx, int n,
+__global__ __launch_bounds__(64) void k_tree_collide(const CollCell* __restrict__ cells, int total,
+                                                     const int* __restrict__ idx, int n,
                                                      const double* __restrict__ x, const double* __restrict__ y,
                                                      const double* __restrict__ z, const double* __restrict__ vx,
                                                      const double* __restrict__ vy, const double* __restrict__ vz,
                                                      const double* __restrict__ rad, GBT gl, double second_r,
                                                      int* __restrict__ counts, const int* __restrict__ offsets,
                                                      CollRec* __restrict__ rec) {
-    const double* __restrict__ Ccx = nd;
-    const double* __restrict__ Ccy = nd + cap;
-    const double* __restrict__ Ccz = nd + 2 * cap;
-    const double* __restrict__ Cw = nd + 3 * cap;
-    const int* __restrict__ Cnext = ni;
-    const int* __restrict__ Cpt = ni + cap;
     const int s = blockIdx.x * 64 + threadIdx.x;
     const bool live = s < n;
     const int me = live ? idx[s] : -1;
@@ -667,10 +705,13 @@ __global__ __launch_bounds__(64) void k_tree_collide(const double* __restrict__ 
         int skip = live ? 0 : 0x7fffffff;
         int c = 0;
         int ri = -1;
+        CollCell cur = cells[0];
         while (c < total) {
-            c = __builtin_amdgcn_readfirstlane(c);
-            const int pt = Cpt[c];
-            const int nx = Cnext[c];
+            // both possible successors are fetched before this cell is evaluated
+            const int c_open = c + 1, nx = cur.next;
+            const CollCell r_open = cells[c_open < total ? c_open : total - 1];
+            const CollCell r_skip = cells[nx < total ? nx : total - 1];
+            const int pt = cur.pt;
             bool open = false;
             if (c >= skip) {
                 if (pt >= 0) {
@@ -691,18 +732,20 @@ __global__ __launch_bounds__(64) void k_tree_collide(const double* __restrict__ 
                         }
                     }
                 } else {
-                    const double dx = gx - Ccx[c];
-                    const double dy = gy - Ccy[c];
-                    const double dz = gz - Ccz[c];
+                    const double dx = gx - cur.cx;
+                    const double dy = gy - cur.cy;
+                    const double dz = gz - cur.cz;
                     const double r2 = dx * dx + dy * dy + dz * dz;
-                    const double rp = p1_r + second_r + 0.86602540378443 * Cw[c];
+                    const double rp = p1_r + second_r + 0.86602540378443 * cur.w;
                     if (r2 < rp * rp)
                         open = true;
                     else
                         skip = nx;
                 }
             }
-            c = (__ballot(open) != 0ull) ? c + 1 : nx;
+            const bool any = __ballot(open) != 0ull;
+            c = any ? c_open : nx;
+            cur = any ? r_open : r_skip;
         }
     }
     if (!WRITE && live) counts[me] = found;
@@ -901,16 +944,16 @@ bool collision_search_device(struct reb_simulation* r) {
             const double second_r = (n >= 2) ? b2 : 0.;
             if (!ensure_coll(r, T, n, 1)) return false;
             const dim3 grid((n + 63) / 64);
-            hipLaunchKernelGGL(k_tree_collide<false>, grid, dim3(64), 0, d->stream, T->node_d.as<double>(),
-                               T->node_i.as<int>(), T->node_cap, T->nnodes, T->idx.as<int>(), n, d->field(FX),
+            hipLaunchKernelGGL(k_tree_collide<false>, grid, dim3(64), 0, d->stream, T->crec.as<CollCell>(),
+                               T->nnodes, T->idx.as<int>(), n, d->field(FX),
                                d->field(FY), d->field(FZ), d->field(FVX), d->field(FVY), d->field(FVZ), d->field(FR),
                                gl, second_r, T->coll_cnt.as<int>(), nullptr, nullptr);
             const long total = scan_counts(r, T, n);
             if (total < 0) return false;
             if (!ensure_coll(r, T, n, (size_t)total + 1)) return false;
             if (total > 0) {
-                hipLaunchKernelGGL(k_tree_collide<true>, grid, dim3(64), 0, d->stream, T->node_d.as<double>(),
-                                   T->node_i.as<int>(), T->node_cap, T->nnodes, T->idx.as<int>(), n, d->field(FX),
+                hipLaunchKernelGGL(k_tree_collide<true>, grid, dim3(64), 0, d->stream, T->crec.as<CollCell>(),
+                                   T->nnodes, T->idx.as<int>(), n, d->field(FX),
                                    d->field(FY), d->field(FZ), d->field(FVX), d->field(FVY), d->field(FVZ),
                                    d->field(FR), gl, second_r, nullptr, T->coll_off.as<int>(),
                                    T->coll_rec.as<CollRec>());
@@ -938,7 +981,8 @@ void tree_release(struct reb_simulation* r) {
     if (!d->tree) return;
     TreeBuffers* T = d->tree;
     for (DevBuf* b : {&T->key0, &T->key1, &T->idx, &T->key0b, &T->key1b, &T->idxb, &T->lcp, &T->cnt, &T->base,
-                      &T->cub, &T->node_d, &T->node_i, &T->info, &T->coll_cnt, &T->coll_off, &T->coll_rec})
+                      &T->cub, &T->node_d, &T->node_i, &T->info, &T->coll_cnt, &T->coll_off, &T->coll_rec,
+                      &T->grec, &T->crec})
         b->release();
     delete T;
     d->tree = nullptr;

@@ -233,7 +233,11 @@ struct Slot {
 // slot values, in index order.  Every lane of the group evaluates the same sum.
 #define GROUP_LOOP(var, lo, hi) for (int var = (lo); var < (hi); var++)
 
-__global__ __launch_bounds__(64) void k_whfast_dh(WHParams P) {
+// One wavefront per workgroup and rarely more than two per SIMD (an ensemble of
+// 16384 systems is 1317 wave tasks): the register budget allows two waves per
+// SIMD, so a step's whole state (about 40 doubles per lane plus the Kepler
+// solver's temporaries) stays in VGPRs without spills.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_whfast_dh(WHParams P) {
     __shared__ Slot slot[64];
     __shared__ int task_id;
     const int lane = threadIdx.x;

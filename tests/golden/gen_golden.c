@@ -432,6 +432,8 @@ static void gen_sheet(void){
     wi("sheet2000_collision_counts.npy", counts, 5, 0);
     int64_t seedafter = r->rand_seed;
     wi("sheet2000_seed_after.npy", &seedafter, 1, 0);
+    double plog[2] = {r->collisions_plog, (double)r->collisions_log_n};
+    wd("sheet2000_plog.npy", plog, 2, 0);
     reb_simulation_free(r);
 
     /* DIRECT search on the same initial state, one step */

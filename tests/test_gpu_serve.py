@@ -110,7 +110,8 @@ def test_nbody_jobs_mix_equals_alone(gpu):
 
 def test_admission_by_memory_footprint(gpu):
     st = _plummer(gpu, 2000, 5)
-    srv = gpu.Server(memory_budget=400_000, max_nbody_jobs=4)    # one N=2000 job (313 kB) at a time
+    one = gpu.lib().reb_amd_server_nbody_bytes(2000, gpu.PRECISION["split"])
+    srv = gpu.Server(memory_budget=int(1.5 * one), max_nbody_jobs=4)    # one N=2000 job at a time
     ids = [srv.submit_nbody(st, softening=0.01, steps=20, precision="split") for _ in range(3)]
     times = []
     for jid in ids:

@@ -91,6 +91,13 @@ def _run_steps(gpu, collision, record):
     sim.s.collision = gpu.COLLISION[collision]
     rec = []
     cor = gpu._abi.COR(lambda r, v: 0.5)
+    if record == "constant":
+        # built-in resolver with the library's constant restitution: the
+        # level-parallel host resolve
+        hs = ctypes.cast(gpu.lib().reb_collision_resolve_hardsphere, ctypes.c_void_p).value
+        sim.s.collision_resolve = ctypes.cast(hs, gpu._abi.RESOLVE)
+        gpu.lib().reb_amd_set_coefficient_of_restitution(sim.ptr, 0.5)
+        return sim, rec
     if record:
         L = gpu.lib()
 
@@ -110,7 +117,7 @@ def _run_steps(gpu, collision, record):
     return sim, rec
 
 
-@pytest.mark.parametrize("record", [True, False])
+@pytest.mark.parametrize("record", [True, False, "constant"])
 def test_sheet_five_steps_bitwise(gpu, record):
     sim, rec = _run_steps(gpu, "tree", record)
     counts = golden("sheet2000_collision_counts.npy")
@@ -119,7 +126,7 @@ def test_sheet_five_steps_bitwise(gpu, record):
         ref = golden("sheet2000_step%d_state.npy" % (s + 1))
         assert np.array_equal(sim.state()[:, :6], ref[:, :6]), s
     assert sim.s.collisions_log_n > 0
-    if record:
+    if record is True:
         pairs = golden("sheet2000_collision_pairs.npy")
         gb = golden("sheet2000_collision_gb.npy")
         got = np.array(rec)
@@ -127,6 +134,9 @@ def test_sheet_five_steps_bitwise(gpu, record):
         assert np.array_equal(got[:, :2].astype(np.int64), pairs)
         assert np.array_equal(got[:, 2:], gb)
     assert sim.s.rand_seed == int(golden("sheet2000_seed_after.npy")[0])
+    # the pressure log is summed in the resolve order (collision.c:655-663)
+    plog = golden("sheet2000_plog.npy")
+    assert sim.s.collisions_plog == plog[0] and sim.s.collisions_log_n == int(plog[1])
 
 
 def test_sheet_direct_search_bitwise(gpu):
