@@ -26,6 +26,11 @@ sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6        # MI355X dense FP64 vector
 FLOP_PER_PAIR = {"fp64": 20, "compensated": 28, "split": 28, "exact": 20}
+# HBM bytes per launch of k_direct_fast<FM_FP64> at N=65536 from rocprofv3
+# --pmc: FETCH_SIZE 8229.9 KiB x 2.0 + WRITE_SIZE 24576 KiB x 1.0, the factors
+# being what the counters read for a device-to-device copy of known size
+# (profiles/direct_fp64_plummer65536.md).
+DIRECT_TRAFFIC_BYTES_N65536 = (8229.9 * 2.0 + 24576.0 * 1.0) * 1024.0
 
 
 def log(*a):
@@ -557,7 +562,9 @@ def main():
             args.n, args.softening, args.dt), "N": args.n, "softening": args.softening, "dt": args.dt,
             "integrator": "leapfrog", "gravity": "basic", "precision": "fp64"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": DIRECT_TRAFFIC_BYTES_N65536 if args.n == 65536 else None,
+                     "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)",
                      "kernel": "k_direct_fast<FM_FP64>", "kernel_ms": head["kernel_ms"],
                      "flop_per_pair": FLOP_PER_PAIR["fp64"], "pairs_per_launch": float(args.n) * (args.n - 1)},
         "cpu_baseline": cpu,

@@ -220,15 +220,41 @@ static int rand_r_step(unsigned int* seed) {
     return result;
 }
 
+}  // namespace rbamd
+
+// Host arrays of the compact resolve, reused from step to step (sized to the
+// largest step so far); `loc` is all -1 between calls.
+struct rbamd::ResolveScratch {
+    std::vector<int> loc, ids, level, last;
+    std::vector<size_t> start, order, fill;
+    std::vector<struct reb_collision> cl;
+    std::vector<struct reb_particle> P;
+    std::vector<double> plog;
+    std::vector<unsigned char> hit;
+};
+
+namespace rbamd {
+
+void resolve_release(struct reb_simulation* r) {
+    DeviceState* d = dev_of(r);
+    delete d->resolve;
+    d->resolve = nullptr;
+}
+
 // The built-in hard-sphere resolver with a constant restitution, on a compact
 // host copy of the involved bodies: only the eight fields it reads come over,
 // only the velocities go back, and collisions of one level run in parallel.
 static void resolve_hardsphere_compact(struct reb_simulation* r) {
     DeviceState* d = dev_of(r);
+    if (!d->resolve) d->resolve = new ResolveScratch();
+    ResolveScratch& W = *d->resolve;
     const size_t nc = r->N_collisions;
-    std::vector<int> loc(r->N, -1), ids;
-    ids.reserve(2 * nc);
-    std::vector<struct reb_collision> cl(r->collisions, r->collisions + nc);
+    if (W.loc.size() < r->N) W.loc.resize(r->N, -1);
+    std::vector<int>& loc = W.loc;
+    std::vector<int>& ids = W.ids;
+    ids.clear();
+    std::vector<struct reb_collision>& cl = W.cl;
+    cl.assign(r->collisions, r->collisions + nc);
     for (auto& c : cl) {
         for (size_t* p : {&c.p1, &c.p2}) {
             int& l = loc[*p];
@@ -240,8 +266,12 @@ static void resolve_hardsphere_compact(struct reb_simulation* r) {
         }
     }
     const int n = (int)ids.size();
+    for (int k = 0; k < n; k++) loc[ids[k]] = -1;   // ready for the next call
     // levels: one more than the last level that touched either body
-    std::vector<int> level(nc), last(n, 0);
+    std::vector<int>& level = W.level;
+    std::vector<int>& last = W.last;
+    level.resize(nc);
+    last.assign(n, 0);
     int nlev = 0;
     for (size_t k = 0; k < nc; k++) {
         const int l = std::max(last[cl[k].p1], last[cl[k].p2]) + 1;
@@ -250,10 +280,14 @@ static void resolve_hardsphere_compact(struct reb_simulation* r) {
         last[cl[k].p2] = l;
         nlev = std::max(nlev, l);
     }
-    std::vector<size_t> start(nlev + 2, 0), order(nc);
+    std::vector<size_t>& start = W.start;
+    std::vector<size_t>& order = W.order;
+    std::vector<size_t>& fill = W.fill;
+    start.assign(nlev + 2, 0);
+    order.resize(nc);
     for (size_t k = 0; k < nc; k++) start[level[k] + 1]++;
     for (int l = 1; l <= nlev + 1; l++) start[l] += start[l - 1];
-    std::vector<size_t> fill(start.begin(), start.end());
+    fill.assign(start.begin(), start.end());
     for (size_t k = 0; k < nc; k++) order[fill[level[k]]++] = k;
     // bring the eight fields over (pinned staging)
     const size_t idbytes = ((sizeof(int) * n + 255) / 256) * 256;
@@ -274,10 +308,10 @@ static void resolve_hardsphere_compact(struct reb_simulation* r) {
                        dids, n, ddata);
     if (!RB_HIP(r, hipMemcpyAsync(h, ddata, sizeof(double) * 8 * n, hipMemcpyDeviceToHost, d->stream))) return;
     if (!stream_sync(r)) return;
-    std::vector<struct reb_particle> P(n);
+    std::vector<struct reb_particle>& P = W.P;
+    if (P.size() < (size_t)n) P.resize(n);
     for (int k = 0; k < n; k++) {
         struct reb_particle& p = P[k];
-        memset(&p, 0, sizeof(p));
         p.x = h[k];
         p.y = h[(size_t)n + k];
         p.z = h[2 * (size_t)n + k];
@@ -287,8 +321,10 @@ static void resolve_hardsphere_compact(struct reb_simulation* r) {
         p.m = h[6 * (size_t)n + k];
         p.r = h[7 * (size_t)n + k];
     }
-    std::vector<double> plog(nc, 0.);
-    std::vector<unsigned char> hit(nc, 0);
+    std::vector<double>& plog = W.plog;
+    std::vector<unsigned char>& hit = W.hit;
+    plog.assign(nc, 0.);
+    hit.assign(nc, 0);
     for (int l = 1; l <= nlev; l++) {
         const size_t b = start[l], e = start[l + 1];
         HostPool::get().run(e - b, [&](size_t q) {

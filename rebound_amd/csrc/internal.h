@@ -68,6 +68,7 @@ enum Field { FX = 0, FY, FZ, FVX, FVY, FVZ, FAX, FAY, FAZ, FM, FR, NFIELD };
 enum Loc { LOC_HOST = 0, LOC_DEVICE = 1, LOC_BOTH = 2 };
 
 struct TreeBuffers;          // tree.hip
+struct ResolveScratch;       // collision_host.hip
 struct WHFastDevice;         // whfast.hip
 
 // Device-side state of one simulation.  Allocated next to the public struct
@@ -101,6 +102,8 @@ struct DeviceState {
     bool owns_stream = true;
     // value returned by reb_amd_coefficient_of_restitution_constant
     double cor_constant = 1.0;
+    // host work arrays of the collision resolve, kept between steps
+    ResolveScratch* resolve = nullptr;
     double* field(int f) const { return soa.as<double>() + (size_t)f * cap; }
 };
 
@@ -174,6 +177,7 @@ void whfast_step_device(struct reb_simulation* r, struct reb_integrator_whfast_s
 void whfast_synchronize_device(struct reb_simulation* r, struct reb_integrator_whfast_state* s);
 void whfast_release(struct reb_simulation* r);
 void tree_release(struct reb_simulation* r);
+void resolve_release(struct reb_simulation* r);
 
 // Host-side driver pieces shared between files
 void collision_resolve_host(struct reb_simulation* r);       // collision.c:336-404
