@@ -366,24 +366,33 @@ def small_leg(amd, args):
     """The plain method at the reference's CPU-sized case: Plummer N=1024,
     softening 0.01, leapfrog dt=1e-3, 1000 steps, BASIC in EXACT precision (bit-
     identical to the reference, so the energy drift is the reference's)."""
-    sim = plummer_sim(amd, 1024, 1, 0.01, 1e-3, "exact")
-    e0 = sim.energy()
     L = amd.lib()
-    L.reb_amd_set_resident(sim.ptr, 1)
-    sim.steps(10)
-    L.reb_amd_synchronize_device(sim.ptr)
-    t0 = time.perf_counter()
-    sim.steps(1000)
-    L.reb_amd_synchronize_device(sim.ptr)
-    wall = time.perf_counter() - t0
-    L.reb_amd_set_resident(sim.ptr, 0)
-    e1 = sim.energy()
-    sim.free()
+
+    def run(precision):
+        sim = plummer_sim(amd, 1024, 1, 0.01, 1e-3, precision)
+        e0 = sim.energy()
+        L.reb_amd_set_resident(sim.ptr, 1)
+        sim.steps(10)
+        L.reb_amd_synchronize_device(sim.ptr)
+        t0 = time.perf_counter()
+        sim.steps(1000)
+        L.reb_amd_synchronize_device(sim.ptr)
+        wall = time.perf_counter() - t0
+        L.reb_amd_set_resident(sim.ptr, 0)
+        e1 = sim.energy()
+        sim.free()
+        return wall, (e1 - e0) / abs(e0)
+
+    wall, drift = run("exact")
+    wall64, drift64 = run("fp64")
     pairs = 1024.0 * 1023.0
     res = {"metric": "Plummer N=1024 leapfrog steps/s (EXACT, bit-identical)", "value": 1000 / wall,
            "unit": "steps/s", "us_per_step": 1e6 * wall / 1000, "gpairs_per_s": pairs * 1000 / wall / 1e9,
-           "rel_energy_drift_1010_steps": (e1 - e0) / abs(e0)}
-    log("small: N=1024 %.0f steps/s (%.1f us/step)" % (res["value"], res["us_per_step"]))
+           "rel_energy_drift_1010_steps": drift,
+           "fp64": {"value": 1000 / wall64, "unit": "steps/s", "us_per_step": 1e6 * wall64 / 1000,
+                    "rel_energy_drift_1010_steps": drift64}}
+    log("small: N=1024 EXACT %.0f steps/s (%.1f us/step), FP64 %.0f steps/s" % (
+        res["value"], res["us_per_step"], 1000 / wall64))
     exe = os.path.join(ROOT, "oracle", "_ref", "ref_bench")
     if not args.no_cpu and os.path.exists(exe):
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
