@@ -111,13 +111,14 @@ REB_API long long reb_amd_tree_interactions(struct reb_simulation* r);
 
 /* ---------------------------------------------------------------------------
  * Batched ensembles: many independent planetary systems integrated together
- * with WHFast in democratic heliocentric coordinates (no correctors, default
- * kernel, gravity_ignore_terms INVOLVING_0), state resident in device memory.
- * Each system evolves bit-identically to a reb_simulation holding the same
- * particles with integrator "whfast", coordinates DEMOCRATICHELIOCENTRIC, the
- * given safe_mode, dt and G, driven by reb_simulation_steps
- * (integrator_whfast.c:1113-1259, simulation.c:504-513).  The reference has no
- * batched form: its equivalent is a loop over reb_simulation_steps calls.
+ * with WHFast (default kernel), state resident in device memory: democratic
+ * heliocentric coordinates unless reb_amd_ensemble_set_whfast selects Jacobi
+ * coordinates and symplectic correctors.  Each system evolves bit-identically
+ * to a reb_simulation holding the same particles with integrator "whfast",
+ * the same coordinates, correctors and safe_mode, dt and G, driven by
+ * reb_simulation_steps (integrator_whfast.c:1051-1259, simulation.c:504-513).
+ * The reference has no batched form: its equivalent is a loop over
+ * reb_simulation_steps calls.
  * ------------------------------------------------------------------------- */
 struct reb_amd_ensemble;
 
@@ -129,6 +130,12 @@ REB_API struct reb_amd_ensemble* reb_amd_ensemble_create(size_t n_systems, const
                                                          const struct reb_particle* bodies, const double* dt,
                                                          double G, int safe_mode, int gravity);
 REB_API void reb_amd_ensemble_free(struct reb_amd_ensemble* e);
+/* Before the first steps: coordinates REB_INTEGRATOR_WHFAST_COORDINATES_JACOBI
+ * or _DEMOCRATICHELIOCENTRIC, first corrector order (0, 3, 5, 7, 11, 17) and
+ * second corrector (0/1), both Jacobi only; the fields of
+ * struct reb_integrator_whfast_state (integrator_whfast.h:28-46). */
+REB_API int reb_amd_ensemble_set_whfast(struct reb_amd_ensemble* e, int coordinates, unsigned int corrector,
+                                        unsigned int corrector2);
 /* Queues n steps of every system, ending synchronized; does not wait. */
 REB_API int reb_amd_ensemble_steps(struct reb_amd_ensemble* e, uint64_t n_steps);
 REB_API int reb_amd_ensemble_wait(struct reb_amd_ensemble* e);

@@ -47,6 +47,8 @@ def lib():
             "orc_kepler": (c_int, [_dp, c_double, c_double]),
             "orc_whfast_dh": (None, [_dp, _dp, c_size_t, c_double, c_double, c_double, c_long, c_int, POINTER(c_int),
                                      c_int]),
+            "orc_whfast_jacobi": (None, [_dp, _dp, c_size_t, c_size_t, c_int, c_double, c_double, c_int, c_double,
+                                         c_long, c_int, c_int, c_int, POINTER(c_int), c_int]),
             "orc_sei_operator": (None, [_dp, c_size_t, c_double, c_double, c_double, c_int]),
             "orc_shear_wrap": (None, [_dp, c_size_t, c_double, c_double, c_double, c_double, c_double]),
             "orc_ghostbox_shear": (None, [c_double, c_double, c_double, c_double, c_double, c_int, c_int, c_int,
@@ -58,6 +60,9 @@ def lib():
             "orc_collision_tree": (c_long, [_dp, c_size_t, c_double, c_int, c_int, c_int, _dp, c_int, _ip, _dp,
                                             c_long]),
             "orc_collision_direct": (c_long, [_dp, c_size_t, _dp, c_int, _ip, _dp, c_long]),
+            "orc_collision_line": (c_long, [_dp, c_size_t, _dp, c_int, c_double, _ip, _dp, c_long]),
+            "orc_collision_linetree": (c_long, [_dp, c_size_t, c_double, c_int, c_int, c_int, _dp, c_int, c_double,
+                                                _ip, _dp, c_long]),
             "orc_collision_shuffle": (None, [_ip, _dp, c_long, U]),
             "orc_hardsphere": (c_double, [_dp, c_int64, c_int64, _dp, c_double, c_double, POINTER(c_long)]),
             "orc_flops": (ctypes.c_ulonglong, [c_int]),
@@ -138,6 +143,17 @@ def whfast_dh(s, G, dt, steps, safe_mode, soft=0.0, ph=None, is_sync=1, did_modi
     return s, ph, flag.value
 
 
+def whfast_jacobi(s, G, dt, steps, safe_mode, soft=0.0, corrector=0, corrector2=0, N_active=None, tp_type=0,
+                  compensated=0, pj=None, is_sync=1, did_modify=1):
+    s = np.ascontiguousarray(s, dtype=np.float64).copy()
+    n = s.shape[0]
+    pj = np.zeros((n, 7)) if pj is None else np.ascontiguousarray(pj, dtype=np.float64).copy()
+    flag = c_int(is_sync)
+    lib().orc_whfast_jacobi(s, pj, n, n if N_active is None else N_active, tp_type, G, soft, compensated, dt, steps,
+                            safe_mode, corrector, corrector2, ctypes.byref(flag), did_modify)
+    return s, pj, flag.value
+
+
 def whfast_flops_per_system_step(n_bodies, state, dts, steps, G=1.0, safe_mode=0):
     """Algorithmic FLOPs per system-step of WHFast DH on the given systems
     (counted by the restatement while it integrates them for `steps` steps,
@@ -191,6 +207,27 @@ def collisions_direct(s, gbs6, cap=1 << 20):
     pairs = np.zeros((cap, 2), dtype=np.int64)
     gbo = np.zeros((cap, 6))
     n = lib().orc_collision_direct(s, s.shape[0], gbs6, gbs6.shape[0], pairs, gbo, cap)
+    return pairs[:n].copy(), gbo[:n].copy()
+
+
+def collisions_line(s, gbs6, dt, cap=1 << 20):
+    s = np.ascontiguousarray(s, dtype=np.float64)
+    gbs6 = np.ascontiguousarray(gbs6, dtype=np.float64).reshape(-1, 6)
+    pairs = np.zeros((cap, 2), dtype=np.int64)
+    gbo = np.zeros((cap, 6))
+    n = lib().orc_collision_line(s, s.shape[0], gbs6, gbs6.shape[0], dt, pairs, gbo, cap)
+    return pairs[:n].copy(), gbo[:n].copy()
+
+
+def collisions_linetree(s, root_size, nr, gbs6, dt, cap=1 << 20):
+    s = np.ascontiguousarray(s, dtype=np.float64)
+    gbs6 = np.ascontiguousarray(gbs6, dtype=np.float64).reshape(-1, 6)
+    pairs = np.zeros((cap, 2), dtype=np.int64)
+    gbo = np.zeros((cap, 6))
+    n = lib().orc_collision_linetree(s, s.shape[0], root_size, nr[0], nr[1], nr[2], gbs6, gbs6.shape[0], dt, pairs,
+                                     gbo, cap)
+    if n < 0:
+        raise ValueError("tree construction error")
     return pairs[:n].copy(), gbo[:n].copy()
 
 

@@ -621,6 +621,361 @@ void orc_whfast_dh(double* s, double* ph, size_t N, double G, double soft, doubl
     free(acc);
 }
 
+/* ------------------------------------------------------------------ WHFast Jacobi */
+typedef struct {
+    double* s;        /* inertial bodies, ORC_NCOL columns */
+    double* pj;       /* Jacobi bodies, 7 columns */
+    double* acc;      /* 3N scratch */
+    double* ja;       /* 3N Jacobi accelerations */
+    size_t N, NA, NAG;
+    int tp, comp;
+    double G, soft, dt;
+} jacctx;
+
+static void jac_from_inertial(jacctx* c) {
+    /* transformations.c:34-81, p_mass = the inertial bodies */
+    const double* s = c->s;
+    double* pj = c->pj;
+    double eta = COL(s, 0, M);
+    double sx = eta * COL(s, 0, X), sy = eta * COL(s, 0, Y), sz = eta * COL(s, 0, Z);
+    double svx = eta * COL(s, 0, VX), svy = eta * COL(s, 0, VY), svz = eta * COL(s, 0, VZ);
+    for (size_t i = 1; i < c->NA; i++) {
+        const double ei = 1. / eta;
+        const double mi = COL(s, i, M);
+        eta += mi;
+        const double pme = eta * ei;
+        PH(pj, i, 6) = mi;
+        PH(pj, i, 0) = COL(s, i, X) - sx * ei;
+        PH(pj, i, 1) = COL(s, i, Y) - sy * ei;
+        PH(pj, i, 2) = COL(s, i, Z) - sz * ei;
+        PH(pj, i, 3) = COL(s, i, VX) - svx * ei;
+        PH(pj, i, 4) = COL(s, i, VY) - svy * ei;
+        PH(pj, i, 5) = COL(s, i, VZ) - svz * ei;
+        sx = sx * pme + mi * PH(pj, i, 0);
+        sy = sy * pme + mi * PH(pj, i, 1);
+        sz = sz * pme + mi * PH(pj, i, 2);
+        svx = svx * pme + mi * PH(pj, i, 3);
+        svy = svy * pme + mi * PH(pj, i, 4);
+        svz = svz * pme + mi * PH(pj, i, 5);
+    }
+    const double ei = 1. / eta;
+    for (size_t i = c->NA; i < c->N; i++) {
+        PH(pj, i, 6) = COL(s, i, M);
+        PH(pj, i, 0) = COL(s, i, X) - sx * ei;
+        PH(pj, i, 1) = COL(s, i, Y) - sy * ei;
+        PH(pj, i, 2) = COL(s, i, Z) - sz * ei;
+        PH(pj, i, 3) = COL(s, i, VX) - svx * ei;
+        PH(pj, i, 4) = COL(s, i, VY) - svy * ei;
+        PH(pj, i, 5) = COL(s, i, VZ) - svz * ei;
+    }
+    const double Mi = 1. / eta;
+    PH(pj, 0, 6) = eta;
+    PH(pj, 0, 0) = sx * Mi;
+    PH(pj, 0, 1) = sy * Mi;
+    PH(pj, 0, 2) = sz * Mi;
+    PH(pj, 0, 3) = svx * Mi;
+    PH(pj, 0, 4) = svy * Mi;
+    PH(pj, 0, 5) = svz * Mi;
+}
+
+static void jac_to_inertial(jacctx* c, int with_vel) {
+    /* transformations.c:178-226 (with_vel) and :228-258 (positions) */
+    double* s = c->s;
+    const double* pj = c->pj;
+    double eta = PH(pj, 0, 6);
+    double sx = PH(pj, 0, 0) * eta, sy = PH(pj, 0, 1) * eta, sz = PH(pj, 0, 2) * eta;
+    double svx = PH(pj, 0, 3) * eta, svy = PH(pj, 0, 4) * eta, svz = PH(pj, 0, 5) * eta;
+    for (size_t i = c->N - 1; i >= c->NA; i--) {
+        const double ei = 1. / eta;
+        COL(s, i, X) = PH(pj, i, 0) + sx * ei;
+        COL(s, i, Y) = PH(pj, i, 1) + sy * ei;
+        COL(s, i, Z) = PH(pj, i, 2) + sz * ei;
+        if (with_vel) {
+            COL(s, i, VX) = PH(pj, i, 3) + svx * ei;
+            COL(s, i, VY) = PH(pj, i, 4) + svy * ei;
+            COL(s, i, VZ) = PH(pj, i, 5) + svz * ei;
+        }
+    }
+    for (size_t i = c->NA - 1; i > 0; i--) {
+        const double ei = 1. / eta;
+        const double mi = COL(s, i, M);
+        sx = (sx - mi * PH(pj, i, 0)) * ei;
+        sy = (sy - mi * PH(pj, i, 1)) * ei;
+        sz = (sz - mi * PH(pj, i, 2)) * ei;
+        COL(s, i, X) = PH(pj, i, 0) + sx;
+        COL(s, i, Y) = PH(pj, i, 1) + sy;
+        COL(s, i, Z) = PH(pj, i, 2) + sz;
+        if (with_vel) {
+            svx = (svx - mi * PH(pj, i, 3)) * ei;
+            svy = (svy - mi * PH(pj, i, 4)) * ei;
+            svz = (svz - mi * PH(pj, i, 5)) * ei;
+            COL(s, i, VX) = PH(pj, i, 3) + svx;
+            COL(s, i, VY) = PH(pj, i, 4) + svy;
+            COL(s, i, VZ) = PH(pj, i, 5) + svz;
+        }
+        eta -= mi;
+        sx *= eta;
+        sy *= eta;
+        sz *= eta;
+        svx *= eta;
+        svy *= eta;
+        svz *= eta;
+    }
+    const double mi = 1. / eta;
+    COL(s, 0, X) = sx * mi;
+    COL(s, 0, Y) = sy * mi;
+    COL(s, 0, Z) = sz * mi;
+    if (with_vel) {
+        COL(s, 0, VX) = svx * mi;
+        COL(s, 0, VY) = svy * mi;
+        COL(s, 0, VZ) = svz * mi;
+    }
+}
+
+static void jac_kepler(jacctx* c, double h) {
+    /* integrator_whfast.c:589-606, the serial form of eta */
+    double eta = COL(c->s, 0, M);
+    for (size_t i = 1; i < c->N; i++) {
+        if (i < c->NA) eta += PH(c->pj, i, 6);
+        double st[6];
+        for (int k = 0; k < 6; k++) st[k] = PH(c->pj, i, k);
+        orc_kepler(st, eta * c->G, h);
+        for (int k = 0; k < 6; k++) PH(c->pj, i, k) = st[k];
+    }
+}
+
+static void jac_com(jacctx* c, double h) {
+    /* integrator_whfast.c:638-641 */
+    PH(c->pj, 0, 0) += h * PH(c->pj, 0, 3);
+    PH(c->pj, 0, 1) += h * PH(c->pj, 0, 4);
+    PH(c->pj, 0, 2) += h * PH(c->pj, 0, 5);
+}
+
+static void jac_gravity(jacctx* c) {
+    /* reb_simulation_update_acceleration with gravity_ignore_terms = BETWEEN_0_AND_1 */
+    const double gb[3] = {0, 0, 0};
+    if (c->comp)
+        orc_gravity_compensated(c->s, c->N, c->NAG, c->tp, 1, c->G, c->soft, c->acc);
+    else
+        orc_gravity_basic(c->s, c->N, c->NAG, c->tp, 1, c->G, c->soft, gb, 1, c->acc);
+    for (size_t i = 0; i < c->N; i++) {
+        COL(c->s, i, AX) = c->acc[3 * i];
+        COL(c->s, i, AY) = c->acc[3 * i + 1];
+        COL(c->s, i, AZ) = c->acc[3 * i + 2];
+    }
+}
+
+static void jac_interaction(jacctx* c, double h) {
+    /* transformations.c:147-176, then integrator_whfast.c:439-483 */
+    const double* s = c->s;
+    double* ja = c->ja;
+    double eta = COL(s, 0, M);
+    double sx = eta * COL(s, 0, AX), sy = eta * COL(s, 0, AY), sz = eta * COL(s, 0, AZ);
+    for (size_t i = 1; i < c->NA; i++) {
+        const double ei = 1. / eta;
+        const double mi = COL(s, i, M);
+        eta += mi;
+        const double pme = eta * ei;
+        ja[3 * i] = COL(s, i, AX) - sx * ei;
+        ja[3 * i + 1] = COL(s, i, AY) - sy * ei;
+        ja[3 * i + 2] = COL(s, i, AZ) - sz * ei;
+        sx = sx * pme + mi * ja[3 * i];
+        sy = sy * pme + mi * ja[3 * i + 1];
+        sz = sz * pme + mi * ja[3 * i + 2];
+    }
+    const double ei = 1. / eta;
+    for (size_t i = c->NA; i < c->N; i++) {
+        ja[3 * i] = COL(s, i, AX) - sx * ei;
+        ja[3 * i + 1] = COL(s, i, AY) - sy * ei;
+        ja[3 * i + 2] = COL(s, i, AZ) - sz * ei;
+    }
+    double et = COL(s, 0, M);
+    for (size_t i = 1; i < c->N; i++) {
+        const double px = PH(c->pj, i, 0), py = PH(c->pj, i, 1), pz = PH(c->pj, i, 2);
+        if (i < c->NA) et += PH(c->pj, i, 6);
+        PH(c->pj, i, 3) += h * ja[3 * i];
+        PH(c->pj, i, 4) += h * ja[3 * i + 1];
+        PH(c->pj, i, 5) += h * ja[3 * i + 2];
+        if (i > 1) {
+            const double rj2i = 1. / (px * px + py * py + pz * pz);
+            const double rji = sqrt(rj2i);
+            const double rj3iM = rji * rj2i * c->G * et;
+            const double prefac1 = h * rj3iM;
+            PH(c->pj, i, 3) += prefac1 * px;
+            PH(c->pj, i, 4) += prefac1 * py;
+            PH(c->pj, i, 5) += prefac1 * pz;
+        }
+    }
+}
+
+/* Symplectic corrector coefficients: Wisdom, Holman & Touma (1996), as
+ * integrator_whfast.c:125-152 spells them. */
+static const double ca1 = 0.41833001326703777398908601289259374469640768464934;
+static const double ca2 = 0.83666002653407554797817202578518748939281536929867;
+static const double ca3 = 1.2549900398011133219672580386777812340892230539480;
+static const double ca4 = 1.6733200530681510959563440515703749787856307385973;
+static const double ca5 = 2.0916500663351888699454300644629687234820384232467;
+static const double ca6 = 2.5099800796022266439345160773555624681784461078960;
+static const double ca7 = 2.9283100928692644179236020902481562128748537925454;
+static const double ca8 = 3.3466401061363021919126881031407499575712614771947;
+static const double cb31 = -0.024900596027799867499350357910273437184309981229127;
+static const double cb51 = -0.0083001986759332891664501193034244790614366604097090;
+static const double cb52 = 0.041500993379666445832250596517122395307183302048545;
+static const double cb71 = 0.0024926811426922105779030593952776964450539008582219;
+static const double cb72 = -0.018270923246702131478062356884535264841652263842597;
+static const double cb73 = 0.053964399093127498721765893493510877532452806339655;
+static const double cb111 = 0.00020361579647854651301632818774633716473696537436847;
+static const double cb112 = -0.0023487215292295354188307328851055489876255097419754;
+static const double cb113 = 0.012309078592019946317544564763237909911330686448336;
+static const double cb114 = -0.038121613681288650508647613260247372125243616270670;
+static const double cb115 = 0.072593394748842738674253180742744961827622366521517;
+static const double cb171 = -0.0000043347415473373580190650223498124944896789841432241;
+static const double cb172 = 0.000076436355227935738363241846979413475106795392377415;
+static const double cb173 = -0.00063599983075817658983166881625078545864140848560259;
+static const double cb174 = 0.0033132577069380655655490196833451994080066801611459;
+static const double cb175 = -0.012071760822342291062449751726959664253913904872527;
+static const double cb176 = 0.032422198864713580293681523029577130832258806467604;
+static const double cb177 = -0.065192863576377893658290760803725762027864651086787;
+static const double cb178 = 0.093056103771425958591541059067553547100903397724386;
+static const double c2b = 0.03486083443891981449909050107438281205803;
+
+static void jac_Z(jacctx* c, double a, double b) {
+    /* integrator_whfast.c:653-678 */
+    jac_kepler(c, a);
+    jac_to_inertial(c, 0);
+    jac_gravity(c);
+    jac_interaction(c, -b);
+    jac_kepler(c, -2. * a);
+    jac_to_inertial(c, 0);
+    jac_gravity(c);
+    jac_interaction(c, b);
+    jac_kepler(c, a);
+}
+
+static void jac_corrector(jacctx* c, double inv, int order) {
+    /* integrator_whfast.c:696-751 */
+    const double dt = c->dt;
+    if (order == 3) {
+        jac_Z(c, ca1 * dt, -inv * cb31 * dt);
+        jac_Z(c, -ca1 * dt, inv * cb31 * dt);
+    }
+    if (order == 5) {
+        jac_Z(c, -ca2 * dt, -inv * cb51 * dt);
+        jac_Z(c, -ca1 * dt, -inv * cb52 * dt);
+        jac_Z(c, ca1 * dt, inv * cb52 * dt);
+        jac_Z(c, ca2 * dt, inv * cb51 * dt);
+    }
+    if (order == 7) {
+        jac_Z(c, -ca3 * dt, -inv * cb71 * dt);
+        jac_Z(c, -ca2 * dt, -inv * cb72 * dt);
+        jac_Z(c, -ca1 * dt, -inv * cb73 * dt);
+        jac_Z(c, ca1 * dt, inv * cb73 * dt);
+        jac_Z(c, ca2 * dt, inv * cb72 * dt);
+        jac_Z(c, ca3 * dt, inv * cb71 * dt);
+    }
+    if (order == 11) {
+        const double A[5] = {ca5, ca4, ca3, ca2, ca1};
+        const double B[5] = {cb111, cb112, cb113, cb114, cb115};
+        for (int k = 0; k < 5; k++) jac_Z(c, -A[k] * dt, -inv * B[k] * dt);
+        for (int k = 4; k >= 0; k--) jac_Z(c, A[k] * dt, inv * B[k] * dt);
+    }
+    if (order == 17) {
+        const double A[8] = {ca8, ca7, ca6, ca5, ca4, ca3, ca2, ca1};
+        const double B[8] = {cb171, cb172, cb173, cb174, cb175, cb176, cb177, cb178};
+        for (int k = 0; k < 8; k++) jac_Z(c, -A[k] * dt, -inv * B[k] * dt);
+        for (int k = 7; k >= 0; k--) jac_Z(c, A[k] * dt, inv * B[k] * dt);
+    }
+}
+
+static void jac_C(jacctx* c, double a, double b) {
+    /* integrator_whfast.c:753-766 */
+    jac_kepler(c, a);
+    jac_to_inertial(c, 0);
+    jac_gravity(c);
+    jac_interaction(c, b);
+    jac_kepler(c, -a);
+}
+
+static void jac_U(jacctx* c, double a, double b) {
+    /* integrator_whfast.c:768-778: U = K(a) Y(a,b) Y(a,-b) K(-a), Y(a,b) = C(a,b) C(-a,-b) */
+    jac_kepler(c, a);
+    jac_C(c, a, b);
+    jac_C(c, -a, -b);
+    jac_C(c, a, -b);
+    jac_C(c, -a, b);
+    jac_kepler(c, -a);
+}
+
+static void jac_corrector2(jacctx* c, double inv) {
+    /* integrator_whfast.c:779-784 */
+    const double a = 0.5 * inv * c->dt;
+    const double b = c2b * inv * c->dt;
+    jac_U(c, a, b);
+    jac_U(c, -a, b);
+}
+
+static void jac_synchronize(jacctx* c, int corrector, int corrector2) {
+    /* integrator_whfast.c:1051-1111 (default kernel) */
+    jac_kepler(c, c->dt / 2.);
+    jac_com(c, c->dt / 2.);
+    if (corrector2) jac_corrector2(c, -1.);
+    if (corrector) jac_corrector(c, -1., corrector);
+    jac_to_inertial(c, 1);
+}
+
+void orc_whfast_jacobi(double* s, double* pj, size_t N, size_t N_active, int tp_type, double G, double soft,
+                       int compensated, double dt, long steps, int safe_mode, int corrector, int corrector2,
+                       int* is_sync, int did_modify) {
+    jacctx c;
+    c.s = s;
+    c.pj = pj;
+    c.acc = malloc(sizeof(double) * 3 * N + 8);
+    c.ja = malloc(sizeof(double) * 3 * N + 8);
+    c.N = N;
+    c.NAG = N_active > N ? N : N_active;
+    c.NA = tp_type == 1 ? N : c.NAG;
+    c.tp = tp_type;
+    c.comp = compensated;
+    c.G = G;
+    c.soft = soft;
+    c.dt = dt;
+    for (long st = 0; st < steps; st++) {
+        /* integrator_whfast.c:1113-1259 (default kernel) */
+        if (safe_mode || did_modify) {
+            if (*is_sync == 0) {
+                jac_synchronize(&c, corrector, corrector2);
+                *is_sync = 1;
+            }
+            jac_from_inertial(&c);
+        }
+        if (*is_sync) {
+            if (corrector) jac_corrector(&c, 1., corrector);
+            if (corrector2) jac_corrector2(&c, 1.);
+            jac_kepler(&c, dt / 2.);
+            jac_com(&c, dt / 2.);
+        } else {
+            jac_kepler(&c, dt);
+            jac_com(&c, dt);
+        }
+        jac_to_inertial(&c, 1);
+        jac_gravity(&c);
+        jac_interaction(&c, dt);
+        *is_sync = 0;
+        if (safe_mode) {
+            jac_synchronize(&c, corrector, corrector2);
+            *is_sync = 1;
+        }
+        did_modify = 0;
+    }
+    /* reb_simulation_steps ends with a synchronize (simulation.c:510) */
+    if (*is_sync == 0) {
+        jac_synchronize(&c, corrector, corrector2);
+        *is_sync = 1;
+    }
+    free(c.acc);
+    free(c.ja);
+}
+
 /* ------------------------------------------------------------------ SEI + shear boundary */
 void orc_sei_operator(double* s, size_t N, double OMEGA, double OMEGAZ, double dt, int with_kick) {
     const double sindt = sin(OMEGA * (-dt / 2.));
@@ -1029,6 +1384,123 @@ long orc_collision_direct(const double* s, size_t N, const double* gbs, int ngb,
         }
     }
     return n;
+}
+
+/* Swept-sphere overlap (collision.c:150-171, :510-534): closest distance of
+ * two bodies on straight lines over the last step, MIN as collision.c:43. */
+#define ORC_MIN(a, b) ((a) > (b) ? (b) : (a))
+static int line_overlap(double dx1, double dy1, double dz1, double dvx1, double dvy1, double dvz1, double dt,
+                        double rsum) {
+    const double r1 = (dx1 * dx1 + dy1 * dy1 + dz1 * dz1);
+    const double dx2 = dx1 - dt * dvx1;
+    const double dy2 = dy1 - dt * dvy1;
+    const double dz2 = dz1 - dt * dvz1;
+    const double r2 = (dx2 * dx2 + dy2 * dy2 + dz2 * dz2);
+    const double t_closest = (dx1 * dvx1 + dy1 * dvy1 + dz1 * dvz1) / (dvx1 * dvx1 + dvy1 * dvy1 + dvz1 * dvz1);
+    double rmin2_ab = ORC_MIN(r1, r2);
+    if (t_closest / dt >= 0. && t_closest / dt <= 1.) {
+        const double dx3 = dx1 - t_closest * dvx1;
+        const double dy3 = dy1 - t_closest * dvy1;
+        const double dz3 = dz1 - t_closest * dvz1;
+        const double r3 = (dx3 * dx3 + dy3 * dy3 + dz3 * dz3);
+        rmin2_ab = ORC_MIN(rmin2_ab, r3);
+    }
+    return !(rmin2_ab > rsum * rsum);
+}
+
+long orc_collision_line(const double* s, size_t N, const double* gbs, int ngb, double dt, int64_t* pairs, double* gbo,
+                        long cap) {
+    /* collision.c:125-196: ghost box, i, j > i */
+    long n = 0;
+    for (int g = 0; g < ngb; g++) {
+        const double* gu = gbs + 6 * g;
+        for (size_t i = 0; i < N; i++) {
+            const double* p1 = s + i * ORC_NCOL;
+            const double gb[6] = {gu[0] + p1[X], gu[1] + p1[Y], gu[2] + p1[Z],
+                                  gu[3] + p1[VX], gu[4] + p1[VY], gu[5] + p1[VZ]};
+            for (size_t j = i + 1; j < N; j++) {
+                const double* p2 = s + j * ORC_NCOL;
+                if (!line_overlap(gb[0] - p2[X], gb[1] - p2[Y], gb[2] - p2[Z], gb[3] - p2[VX], gb[4] - p2[VY],
+                                  gb[5] - p2[VZ], dt, p1[RR] + p2[RR]))
+                    continue;
+                if (n < cap) {
+                    pairs[2 * n] = (int64_t)i;
+                    pairs[2 * n + 1] = (int64_t)j;
+                    memcpy(gbo + 6 * n, gu, 6 * sizeof(double));
+                }
+                n++;
+            }
+        }
+    }
+    return n;
+}
+
+typedef struct {
+    const double* s;
+    double dt, maxdrift;
+    int64_t* pairs;
+    double* gbo;
+    long cap, n;
+} linectx;
+
+static void line_in_cell(linectx* C, const double* gb, const double* gbunmod, size_t p1, double p1_r,
+                         double p1_r_plus_dtv, const cell* c) {
+    /* collision.c:506-566 */
+    const double* s = C->s;
+    if (c->pt >= 0) {
+        if ((size_t)c->pt == p1) return;
+        const double* p2 = s + (size_t)c->pt * ORC_NCOL;
+        if (!line_overlap(gb[0] - p2[X], gb[1] - p2[Y], gb[2] - p2[Z], gb[3] - p2[VX], gb[4] - p2[VY], gb[5] - p2[VZ],
+                          C->dt, p1_r + p2[RR]))
+            return;
+        if (C->n < C->cap) {
+            C->pairs[2 * C->n] = (int64_t)p1;
+            C->pairs[2 * C->n + 1] = c->pt;
+            memcpy(C->gbo + 6 * C->n, gbunmod, 6 * sizeof(double));
+        }
+        C->n++;
+    } else {
+        const double dx = gb[0] - c->x;
+        const double dy = gb[1] - c->y;
+        const double dz = gb[2] - c->z;
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const double rp = p1_r_plus_dtv + C->maxdrift + 0.86602540378443 * c->w;
+        if (r2 < rp * rp) {
+            for (int o = 0; o < 8; o++)
+                if (c->oct[o]) line_in_cell(C, gb, gbunmod, p1, p1_r, p1_r_plus_dtv, c->oct[o]);
+        }
+    }
+}
+
+long orc_collision_linetree(const double* s, size_t N, double root_size, int nrx, int nry, int nrz, const double* gbs,
+                            int ngb, double dt, int64_t* pairs, double* gbo, long cap) {
+    /* collision.c:270-331 */
+    double vmax2 = 0.;
+    for (size_t i = 0; i < N; i++) {
+        const double* p = s + i * ORC_NCOL;
+        const double v2 = p[VX] * p[VX] + p[VY] * p[VY] + p[VZ] * p[VZ];
+        vmax2 = vmax2 > v2 ? vmax2 : v2;
+    }
+    treectx T = {s, N, root_size, nrx, nry, nrz, 0};
+    cell** roots = build(&T);
+    int nroot = nrx * nry * nrz;
+    linectx C = {s, dt, dt * sqrt(vmax2), pairs, gbo, cap, 0};
+    if (!T.err) {
+        for (size_t i = 0; i < N; i++) {
+            const double* p1 = s + i * ORC_NCOL;
+            const double p1_r_plus_dtv = p1[RR] + dt * sqrt(p1[VX] * p1[VX] + p1[VY] * p1[VY] + p1[VZ] * p1[VZ]);
+            for (int g = 0; g < ngb; g++) {
+                const double* gu = gbs + 6 * g;
+                const double gb[6] = {gu[0] + p1[X], gu[1] + p1[Y], gu[2] + p1[Z],
+                                      gu[3] + p1[VX], gu[4] + p1[VY], gu[5] + p1[VZ]};
+                for (int r = 0; r < nroot; r++)
+                    if (roots[r]) line_in_cell(&C, gb, gu, i, p1[RR], p1_r_plus_dtv, roots[r]);
+            }
+        }
+    }
+    for (int r = 0; r < nroot; r++) delete_cell(roots[r]);
+    free(roots);
+    return T.err ? -1 : C.n;
 }
 
 void orc_collision_shuffle(int64_t* pairs, double* gbo, long n, unsigned int* seed) {

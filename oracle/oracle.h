@@ -59,6 +59,16 @@ int orc_kepler(double* state6, double mu, double dt);
 void orc_whfast_dh(double* s, double* ph7, size_t N, double G, double soft, double dt, long steps, int safe_mode,
                    int* is_sync, int did_modify);
 
+/* WHFast in Jacobi coordinates (integrator_whfast.c:876-1259 with the
+ * default kernel): interaction with BASIC or COMPENSATED gravity and
+ * IGNORE_TERMS_BETWEEN_0_AND_1, first symplectic corrector of order
+ * `corrector` (0, 3, 5, 7, 11, 17) and the second corrector when `corrector2`.
+ * N_active and tp_type as the simulation fields.  pj7 holds the Jacobi state
+ * (N x 7) across calls; otherwise as orc_whfast_dh. */
+void orc_whfast_jacobi(double* s, double* pj7, size_t N, size_t N_active, int tp_type, double G, double soft,
+                       int compensated, double dt, long steps, int safe_mode, int corrector, int corrector2,
+                       int* is_sync, int did_modify);
+
 /* SEI operators (integrator_sei.c:126-174): H0 over dt for every body,
  * preceded by the phi1 kick v += a dt when with_kick; the sin/tan constants
  * are computed as integrator_sei.c:91-101 does. */
@@ -89,6 +99,13 @@ long orc_collision_tree(const double* s, size_t N, double root_size, int nrx, in
 /* Direct search (collision.c:64-124) in the reference's order */
 long orc_collision_direct(const double* s, size_t N, const double* gbs, int ngb, int64_t* pairs, double* gbo,
                           long cap);
+/* LINE (collision.c:125-196) and LINETREE (:270-331, :506-566) searches with
+ * the swept-sphere test over the last step dt (dt_last_done), same output
+ * layout as orc_collision_direct. */
+long orc_collision_line(const double* s, size_t N, const double* gbs6, int ngb, double dt, int64_t* pairs,
+                        double* gbo6, long cap);
+long orc_collision_linetree(const double* s, size_t N, double root_size, int nrx, int nry, int nrz,
+                            const double* gbs6, int ngb, double dt, int64_t* pairs, double* gbo6, long cap);
 /* shuffle (collision.c:336-342) */
 void orc_collision_shuffle(int64_t* pairs, double* gbo, long n, unsigned int* seed);
 /* hardsphere (collision.c:573-665) with constant coefficient of restitution

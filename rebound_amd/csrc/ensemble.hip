@@ -46,6 +46,8 @@ struct Ensemble {
     double G = 1.;
     int safe_mode = 0;
     int compensated = 0;
+    int coords = WH_DH;
+    int corrector = 0, corrector2 = 0;
     DevBuf body, ph, sys, tasks, order, flags, counter, energy;
     int ntasks = 0;
     int grid = 0;
@@ -85,6 +87,9 @@ static WHParams ens_params(const Ensemble* e) {
     P.safe_mode = e->safe_mode;
     P.did_modify = 0;
     P.sync_at_end = 0;
+    P.coords = e->coords;
+    P.corrector = e->corrector;
+    P.corrector2 = e->corrector2;
     return P;
 }
 
@@ -287,6 +292,38 @@ REB_API void reb_amd_ensemble_free(struct reb_amd_ensemble* h) {
         b->release();
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete h;
+}
+
+// Coordinates and symplectic correctors of every system (the fields
+// coordinates, corrector and corrector2 of integrator_whfast.h:28-46), before
+// the first steps; democratic heliocentric without correctors by default.
+REB_API int reb_amd_ensemble_set_whfast(struct reb_amd_ensemble* h, int coordinates, unsigned int corrector,
+                                        unsigned int corrector2) {
+    Ensemble* e = &h->e;
+    if (e->first_step_done) {
+        set_last_error("reb_amd_ensemble_set_whfast: the ensemble has already been stepped");
+        return 1;
+    }
+    if (coordinates != REB_INTEGRATOR_WHFAST_COORDINATES_JACOBI &&
+        coordinates != REB_INTEGRATOR_WHFAST_COORDINATES_DEMOCRATICHELIOCENTRIC) {
+        set_last_error("reb_amd_ensemble_set_whfast: coordinates must be JACOBI or DEMOCRATICHELIOCENTRIC");
+        return 1;
+    }
+    const bool jac = coordinates == REB_INTEGRATOR_WHFAST_COORDINATES_JACOBI;
+    if ((corrector || corrector2) && !jac) {
+        set_last_error("reb_amd_ensemble_set_whfast: symplectic correctors are only compatible with Jacobi coordinates");
+        return 1;
+    }
+    if (corrector != 0 && corrector != 3 && corrector != 5 && corrector != 7 && corrector != 11 && corrector != 17) {
+        set_last_error("reb_amd_ensemble_set_whfast: first symplectic correctors are only available in the "
+                       "following orders: 0, 3, 5, 7, 11, 17");
+        return 1;
+    }
+    drop_graphs(e);
+    e->coords = jac ? WH_JACOBI : WH_DH;
+    e->corrector = (int)corrector;
+    e->corrector2 = corrector2 ? 1 : 0;
+    return 0;
 }
 
 // Queues n steps of every system (ending synchronized); returns without

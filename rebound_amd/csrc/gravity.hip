@@ -66,76 +66,6 @@ __device__ __forceinline__ bool skip_pair(int i, int j, int ignore) {
     return i == j;
 }
 
-// ---------------------------------------------------------------- EXACT, BASIC
-// gravity.c:217-231 (targets 0..N-1, sources 0..N_active-1 for every ghost
-// box), then :259-276 (active targets also feel test particles when
-// testparticle_type is set).
-__global__ __launch_bounds__(kBlock) void k_basic_exact(
-    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
-    const double* __restrict__ m, double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
-    int N, int N_active, int tp_type, int ignore, double G, double soft2, GBList gl) {
-    __shared__ double4 tile[kBlock];
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    const bool live = i < N;
-    const int blk_lo = blockIdx.x * kBlock, blk_hi = blk_lo + kBlock;
-    const double xi = live ? x[i] : 0., yi = live ? y[i] : 0., zi = live ? z[i] : 0.;
-    double a_x = 0., a_y = 0., a_z = 0.;
-    for (int g = 0; g < gl.n; g++) {
-        const GB3 gb = gl.g[g];
-        for (int pass = 0; pass < 2; pass++) {
-            int j0 = pass == 0 ? 0 : N_active;
-            int j1 = pass == 0 ? N_active : N;
-            if (pass == 1 && !tp_type) break;
-            for (int t = j0; t < j1; t += kBlock) {
-                __syncthreads();
-                const int jj = t + (int)threadIdx.x;
-                if (jj < j1) tile[threadIdx.x] = make_double4(x[jj], y[jj], z[jj], m[jj]);
-                __syncthreads();
-                const int n = min(kBlock, j1 - t);
-                if (!live || (pass == 1 && i >= N_active)) continue;
-                auto term = [&](const double4 s) {
-                    const double dx = (gb.x + xi) - s.x;
-                    const double dy = (gb.y + yi) - s.y;
-                    const double dz = (gb.z + zi) - s.z;
-                    const double _r = sqrt(dx * dx + dy * dy + dz * dz + soft2);
-                    const double prefact = -G / (_r * _r * _r) * s.w;
-                    a_x += prefact * dx;
-                    a_y += prefact * dy;
-                    a_z += prefact * dz;
-                };
-                // Tiles without this block's targets (and, with ignored terms,
-                // without sources 0 and 1) need no per-pair test: the terms
-                // are then independent work the compiler can overlap, while
-                // the sums still add them in ascending j.
-                const bool check = (pass == 0 && t < blk_hi && t + n > blk_lo) || (ignore != 0 && t < 2);
-                if (!check) {
-#pragma unroll 4
-                    for (int k = 0; k < n; k++) term(tile[k]);
-                    continue;
-                }
-                for (int k = 0; k < n; k++) {
-                    const int j = t + k;
-                    if (pass == 0) {
-                        if (skip_pair(i, j, ignore)) continue;
-                    } else {
-                        if (ignore == 1 && j == 1 && i == 0) continue;
-                        if (ignore == 2 && (j == 0 || i == 0)) continue;
-                    }
-                    term(tile[k]);
-                }
-            }
-        }
-    }
-    if (live) {
-        ax[i] = a_x;
-        ay[i] = a_y;
-        az[i] = a_z;
-    }
-}
-
-// ---------------------------------------------------------------- EXACT, COMPENSATED
-// gravity.c:308-343 (active targets), :346-378 (test-particle targets), :379-410
-// (active targets feel test particles).  No ghost boxes in the reference.
 __device__ __forceinline__ void kahan_add(double term, double& a, double& c) {
     const double yv = term - c;
     const double tv = a + yv;
@@ -143,57 +73,171 @@ __device__ __forceinline__ void kahan_add(double term, double& a, double& c) {
     a = tv;
 }
 
-__global__ __launch_bounds__(kBlock) void k_comp_exact(
-    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
-    const double* __restrict__ m, double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
-    int N, int N_active, int tp_type, int ignore, double G, double soft2) {
-    __shared__ double4 tile[kBlock];
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+// ---------------------------------------------------------------- EXACT
+// BASIC: gravity.c:217-231 (targets 0..N-1, sources 0..N_active-1 for every
+// ghost box), then :259-276 (active targets also feel test particles when
+// testparticle_type is set).  COMPENSATED: gravity.c:308-343 (active
+// targets), :346-378 (test-particle targets), :379-410 (active targets feel
+// test particles), no ghost boxes, Kahan sums.
+//
+// The terms are independent: all 256 threads of a workgroup compute them for
+// TGT targets x TILE sources at a time (IEEE sqrt and division, no
+// contraction: the reference's expressions) into LDS rows, one row per
+// (target, component), and one lane per row then adds its row in ascending
+// source order: the reference's sequential sum, each component its own
+// chain.  The rows are double-buffered, so one barrier per tile suffices and
+// the summing lanes' chains overlap the other waves' terms of the next tile;
+// each thread loads its sources one tile ahead.
+static constexpr int kExThreads = 256;
+
+template <bool COMP, int TGT, int TILE>
+__global__ __launch_bounds__(kExThreads) void k_exact(const double* __restrict__ x, const double* __restrict__ y,
+                                                       const double* __restrict__ z, const double* __restrict__ m,
+                                                       double* __restrict__ ax, double* __restrict__ ay,
+                                                       double* __restrict__ az, int N, int N_active, int tp_type,
+                                                       int ignore, double G, double soft2, GBList gl) {
+    constexpr int PER = TGT * TILE / kExThreads;   // terms per thread and tile
+    constexpr int ROWS = 3 * TGT;
+    constexpr int PITCH = TILE + 2;                // row pitch (doubles): 16-byte aligned rows, spread banks
+    static_assert(PER >= 1 && PER * kExThreads == TGT * TILE, "tile shape");
+    static_assert(ROWS <= kExThreads, "one summing lane per row");
+    __shared__ __align__(16) double T[2][ROWS][PITCH];
+    const int tid = threadIdx.x;
+    const int i0 = blockIdx.x * TGT;
+    // term phase: thread -> (target ti, sources tj0 .. tj0 + PER - 1 of a tile)
+    const int ti = tid % TGT;
+    const int tj0 = (tid / TGT) * PER;
+    const int i = i0 + ti;
     const bool live = i < N;
-    const int blk_lo = blockIdx.x * kBlock, blk_hi = blk_lo + kBlock;
     const double xi = live ? x[i] : 0., yi = live ? y[i] : 0., zi = live ? z[i] : 0.;
-    double a_x = 0., a_y = 0., a_z = 0., c_x = 0., c_y = 0., c_z = 0.;
-    for (int pass = 0; pass < 2; pass++) {
-        int j0 = pass == 0 ? 0 : N_active;
-        int j1 = pass == 0 ? N_active : N;
-        if (pass == 1 && !tp_type) break;
-        for (int t = j0; t < j1; t += kBlock) {
-            __syncthreads();
-            const int jj = t + (int)threadIdx.x;
-            if (jj < j1) tile[threadIdx.x] = make_double4(x[jj], y[jj], z[jj], m[jj]);
-            __syncthreads();
-            const int n = min(kBlock, j1 - t);
-            if (!live || (pass == 1 && i >= N_active)) continue;
-            auto term = [&](const double4 s) {
-                const double dx = xi - s.x;
-                const double dy = yi - s.y;
-                const double dz = zi - s.z;
-                const double r2 = dx * dx + dy * dy + dz * dz + soft2;
-                const double rr = sqrt(r2);
-                const double prefact = G / (r2 * rr);
-                const double prefactj = -prefact * s.w;
-                kahan_add(prefactj * dx, a_x, c_x);
-                kahan_add(prefactj * dy, a_y, c_y);
-                kahan_add(prefactj * dz, a_z, c_z);
+    // sum phase: lanes 0 .. ROWS-1, lane -> (target si, component sc)
+    const int si = i0 + tid / 3;
+    const int sc = tid % 3;
+    const bool summer = tid < ROWS && si < N;
+    double acc = 0., comp = 0.;
+    int buf = 0;
+    const int ngb = COMP ? 1 : gl.n;
+    for (int g = 0; g < ngb; g++) {
+        const GB3 gb = COMP ? GB3{0., 0., 0.} : gl.g[g];
+        for (int pass = 0; pass < 2; pass++) {
+            const int j0 = pass == 0 ? 0 : N_active;
+            const int j1 = pass == 0 ? N_active : N;
+            if (pass == 1 && !tp_type) break;
+            // the second pass only reaches active targets
+            const bool tgt = live && !(pass == 1 && i >= N_active);
+            const bool stgt = summer && !(pass == 1 && si >= N_active);
+            auto skipped = [&](int ii, int j) {
+                if (COMP || pass == 0) return skip_pair(ii, j, ignore);
+                return (ignore == 1 && j == 1 && ii == 0) || (ignore == 2 && (j == 0 || ii == 0));
             };
-            const bool check = (t < blk_hi && t + n > blk_lo) || (ignore != 0 && t < 2);
-            if (!check) {
-#pragma unroll 4
-                for (int k = 0; k < n; k++) term(tile[k]);
-                continue;
-            }
-            for (int k = 0; k < n; k++) {
-                const int j = t + k;
-                if (skip_pair(i, j, ignore)) continue;
-                term(tile[k]);
+            // this thread's sources of the next tile, loaded one tile ahead
+            double nx[PER], ny[PER], nz[PER], nm[PER];
+            auto fetch = [&](int t) {
+#pragma unroll
+                for (int q = 0; q < PER; q++) {
+                    const int j = t + tj0 + q;
+                    if (j < j1) {
+                        nx[q] = x[j]; ny[q] = y[j]; nz[q] = z[j]; nm[q] = m[j];
+                    }
+                }
+            };
+            if (j0 < j1) fetch(j0);
+            for (int t = j0; t < j1; t += TILE, buf ^= 1) {
+                const int n = min(TILE, j1 - t);
+                double cx[PER], cy[PER], cz[PER], cm[PER];
+#pragma unroll
+                for (int q = 0; q < PER; q++) {
+                    cx[q] = nx[q]; cy[q] = ny[q]; cz[q] = nz[q]; cm[q] = nm[q];
+                }
+                if (t + TILE < j1) fetch(t + TILE);
+#pragma unroll
+                for (int q = 0; q < PER; q++) {
+                    const int k = tj0 + q;
+                    const int j = t + k;
+                    if (!tgt || k >= n || skipped(i, j)) continue;
+                    double tx, ty, tz;
+                    if (COMP) {
+                        const double dx = xi - cx[q];
+                        const double dy = yi - cy[q];
+                        const double dz = zi - cz[q];
+                        const double r2 = dx * dx + dy * dy + dz * dz + soft2;
+                        const double rr = sqrt(r2);
+                        const double prefact = G / (r2 * rr);
+                        const double prefactj = -prefact * cm[q];
+                        tx = prefactj * dx;
+                        ty = prefactj * dy;
+                        tz = prefactj * dz;
+                    } else {
+                        const double dx = (gb.x + xi) - cx[q];
+                        const double dy = (gb.y + yi) - cy[q];
+                        const double dz = (gb.z + zi) - cz[q];
+                        const double _r = sqrt(dx * dx + dy * dy + dz * dz + soft2);
+                        const double prefact = -G / (_r * _r * _r) * cm[q];
+                        tx = prefact * dx;
+                        ty = prefact * dy;
+                        tz = prefact * dz;
+                    }
+                    T[buf][3 * ti + 0][k] = tx;
+                    T[buf][3 * ti + 1][k] = ty;
+                    T[buf][3 * ti + 2][k] = tz;
+                }
+                // After this barrier the summers read T[buf]; the next writes
+                // to T[buf] come two tiles later, after the next barrier, which
+                // the summers reach only once this sum is done.
+                __syncthreads();
+                if (stgt) {
+                    const double* row = T[buf][tid];
+                    auto add = [&](double term) {
+                        if (COMP) {
+                            const double yv = term - comp;
+                            const double tv = acc + yv;
+                            comp = (tv - acc) - yv;
+                            acc = tv;
+                        } else {
+                            acc += term;
+                        }
+                    };
+                    // Pairs the reference leaves out: the target itself and,
+                    // with ignored terms, pairs with bodies 0 and 1.
+                    const bool check = (t <= si && si < t + n) || (ignore != 0 && (t < 2 || si < 2));
+                    if (check) {
+                        for (int k = 0; k < n; k++)
+                            if (!skipped(si, t + k)) add(row[k]);
+                    } else if (n == TILE) {
+#pragma unroll 16
+                        for (int k = 0; k < TILE; k++) add(row[k]);
+                    } else {
+                        for (int k = 0; k < n; k++) add(row[k]);
+                    }
+                }
             }
         }
     }
-    if (live) {
-        ax[i] = a_x;
-        ay[i] = a_y;
-        az[i] = a_z;
+    if (summer) {
+        double* out = sc == 0 ? ax : (sc == 1 ? ay : az);
+        out[si] = acc;
     }
+}
+
+template <bool COMP>
+static void launch_exact(hipStream_t st, const double* x, const double* y, const double* z, const double* m,
+                         double* ax, double* ay, double* az, int N, int N_active, int tp_type, int ignore, double G,
+                         double soft2, const GBList& gl) {
+    // Targets per workgroup: as many as keep >= 1024 workgroups (four per
+    // CU), so that small N still fills the chip; the source tile widens to
+    // keep one term per thread.
+#define RB_EXACT(TGT, TILE)                                                                                           \
+    hipLaunchKernelGGL((k_exact<COMP, TGT, TILE>), dim3((N + TGT - 1) / TGT), dim3(kExThreads), 0, st, x, y, z, m,   \
+                       ax, ay, az, N, N_active, tp_type, ignore, G, soft2, gl)
+    if (N >= 16 * 1024)
+        RB_EXACT(16, 64);
+    else if (N >= 4 * 1024)
+        RB_EXACT(4, 64);
+    else if (N >= 2 * 1024)
+        RB_EXACT(2, 128);
+    else
+        RB_EXACT(1, 256);
+#undef RB_EXACT
 }
 
 // ---------------------------------------------------------------- FP64 / SPLIT / COMP-FP64
@@ -554,14 +598,11 @@ static int direct_dispatch(const double* x, const double* y, const double* z, co
     if (N == 0) return 0;
     const KickDrift none{};
     if (precision == REB_AMD_PRECISION_EXACT) {
-        dim3 grid((N + kBlock - 1) / kBlock);
         if (ev) (void)hipEventRecord(ev[0], st);
         if (compensated)
-            hipLaunchKernelGGL(k_comp_exact, grid, dim3(kBlock), 0, st, x, y, z, m, ax, ay, az, N, N_active, tp_type,
-                               ignore, G, soft2);
+            launch_exact<true>(st, x, y, z, m, ax, ay, az, N, N_active, tp_type, ignore, G, soft2, gl);
         else
-            hipLaunchKernelGGL(k_basic_exact, grid, dim3(kBlock), 0, st, x, y, z, m, ax, ay, az, N, N_active,
-                               tp_type, ignore, G, soft2, gl);
+            launch_exact<false>(st, x, y, z, m, ax, ay, az, N, N_active, tp_type, ignore, G, soft2, gl);
         if (ev) (void)hipEventRecord(ev[1], st);
         if (kd.on) hipLaunchKernelGGL(k_kick_drift, dim3((N + 255) / 256), dim3(256), 0, st, ax, ay, az, N, kd);
         return hipGetLastError() == hipSuccess ? 0 : 1;
@@ -650,17 +691,18 @@ bool gravity_direct(struct reb_simulation* r, bool compensated, const double* ki
 
 // ---------------------------------------------------------------- energy
 // tools.c:108-162.  EXACT: the kinetic and potential sums are accumulated in
-// the reference's sequential order by one wavefront (lanes evaluate 64 terms,
-// lane 0 folds them in order), so the value is bit-identical.  Otherwise each
-// target's row sum over j>i is compensated and rows are folded with Kahan.
-__global__ __launch_bounds__(64) void k_energy_exact(const double* __restrict__ x, const double* __restrict__ y,
-                                                     const double* __restrict__ z, const double* __restrict__ vx,
-                                                     const double* __restrict__ vy, const double* __restrict__ vz,
-                                                     const double* __restrict__ m, int N_active, int N_interact,
-                                                     double G, double* __restrict__ out) {
+// the reference's sequential order, so the value is bit-identical: the
+// kinetic terms by one wavefront (lanes evaluate 64 terms, lane 0 folds them
+// in order); the pair terms of a batch of rows are written in the
+// reference's (i, j > i) order by a grid, then one lane folds them in order
+// while its wavefront streams the next chunk in.  Otherwise each target's row
+// sum over j>i is compensated and rows are folded with Kahan.
+__global__ __launch_bounds__(64) void k_energy_kin(const double* __restrict__ vx, const double* __restrict__ vy,
+                                                   const double* __restrict__ vz, const double* __restrict__ m,
+                                                   int N_interact, double* __restrict__ out) {
     __shared__ double terms[64];
     const int lane = threadIdx.x;
-    double e_kin = 0., e_pot = 0.;
+    double e_kin = 0.;
     for (int b = 0; b < N_interact; b += 64) {
         const int i = b + lane;
         if (i < N_interact) {
@@ -674,28 +716,72 @@ __global__ __launch_bounds__(64) void k_energy_exact(const double* __restrict__ 
         }
         __syncthreads();
     }
-    for (int i = 0; i < N_active; i++) {
-        const double xi = x[i], yi = y[i], zi = z[i], mi = m[i];
-        for (int b = i + 1; b < N_interact; b += 64) {
-            const int j = b + lane;
-            if (j < N_interact) {
-                const double dx = xi - x[j];
-                const double dy = yi - y[j];
-                const double dz = zi - z[j];
-                terms[lane] = G * m[j] * mi / sqrt(dx * dx + dy * dy + dz * dz);
-            }
-            __syncthreads();
-            if (lane == 0) {
-                const int n = min(64, N_interact - b);
-                for (int k = 0; k < n; k++) e_pot -= terms[k];
-            }
-            __syncthreads();
-        }
-    }
     if (lane == 0) {
         out[0] = e_kin;
-        out[1] = e_pot;
+        out[1] = 0.;
     }
+}
+
+// Pair terms G m_j m_i / r_ij of rows i0 .. i0 + gridDim.x - 1, j = i+1 ..
+// N_interact-1, row after row (the reference's loop order).
+__device__ __forceinline__ long energy_row_start(long i, long NI) { return i * (NI - 1) - i * (i - 1) / 2; }
+
+__global__ __launch_bounds__(256) void k_energy_pair_terms(const double* __restrict__ x, const double* __restrict__ y,
+                                                           const double* __restrict__ z, const double* __restrict__ m,
+                                                           int i0, int N_interact, double G,
+                                                           double* __restrict__ buf) {
+    const int i = i0 + blockIdx.x;
+    const long off = energy_row_start(i, N_interact) - energy_row_start(i0, N_interact) - (i + 1);
+    const double xi = x[i], yi = y[i], zi = z[i], mi = m[i];
+    for (int j = i + 1 + (int)threadIdx.x; j < N_interact; j += blockDim.x) {
+        const double dx = xi - x[j];
+        const double dy = yi - y[j];
+        const double dz = zi - z[j];
+        buf[off + j] = G * m[j] * mi / sqrt(dx * dx + dy * dy + dz * dz);
+    }
+}
+
+// e_pot -= buf[0 .. n-1] in order, continuing out[1].
+__global__ __launch_bounds__(64) void k_energy_fold_terms(const double* __restrict__ buf, long n,
+                                                          double* __restrict__ out) {
+    constexpr int PER = 16;
+    constexpr int CH = 64 * PER;
+    __shared__ __align__(16) double stage[2][CH];
+    const int lane = threadIdx.x;
+    double e = out[1];
+    const long nch = (n + CH - 1) / CH;
+    double r[PER];
+    auto load = [&](long c) {
+        const long base = c * CH + (long)lane * PER;
+#pragma unroll
+        for (int q = 0; q < PER; q++) r[q] = base + q < n ? buf[base + q] : 0.;
+    };
+    auto store = [&](int b) {
+#pragma unroll
+        for (int q = 0; q < PER; q++) stage[b][lane * PER + q] = r[q];
+    };
+    if (nch > 0) {
+        load(0);
+        store(0);
+    }
+    __syncthreads();
+    for (long c = 0; c < nch; c++) {
+        const int b = (int)(c & 1);
+        if (c + 1 < nch) load(c + 1);   // in flight while lane 0 folds chunk c
+        if (lane == 0) {
+            const long cnt = n - c * CH;
+            if (cnt >= CH) {
+#pragma unroll 16
+                for (int k = 0; k < CH; k++) e -= stage[b][k];
+            } else {
+                for (int k = 0; k < (int)cnt; k++) e -= stage[b][k];
+            }
+        }
+        // stage[b ^ 1] was last read in the previous iteration, before its barrier
+        if (c + 1 < nch) store(b ^ 1);
+        __syncthreads();
+    }
+    if (lane == 0) out[1] = e;
 }
 
 __global__ __launch_bounds__(256) void k_energy_rows(const double* __restrict__ x, const double* __restrict__ y,
@@ -755,10 +841,27 @@ __global__ __launch_bounds__(256) void k_energy_fold(const double* __restrict__ 
 
 int energy_launch(const double* x, const double* y, const double* z, const double* vx, const double* vy,
                   const double* vz, const double* m, int n, int n_active, int n_interact, double G, bool exact,
-                  double* out, hipStream_t st) {
-    if (exact && n <= 8192) {
-        hipLaunchKernelGGL(k_energy_exact, dim3(1), dim3(64), 0, st, x, y, z, vx, vy, vz, m, n_active, n_interact, G,
-                           out);
+                  double* out, hipStream_t st, double* terms, size_t terms_cap) {
+    if (exact && !(terms && terms_cap > 0)) return 1;
+    if (exact) {
+        hipLaunchKernelGGL(k_energy_kin, dim3(1), dim3(64), 0, st, vx, vy, vz, m, n_interact, out);
+        // rows in batches whose terms fit the buffer (a row has at most n_interact - 1)
+        const long NI = n_interact;
+        for (int i0 = 0; i0 < n_active;) {
+            int i1 = i0;
+            long cnt = 0;
+            while (i1 < n_active && cnt + (NI - 1 - i1) <= (long)terms_cap) {
+                cnt += NI - 1 - i1;
+                i1++;
+            }
+            if (i1 == i0) return 1;   // a single row larger than the buffer
+            if (cnt > 0) {
+                hipLaunchKernelGGL(k_energy_pair_terms, dim3(i1 - i0), dim3(256), 0, st, x, y, z, m, i0, n_interact, G,
+                                   terms);
+                hipLaunchKernelGGL(k_energy_fold_terms, dim3(1), dim3(64), 0, st, terms, cnt, out);
+            }
+            i0 = i1;
+        }
     } else {
         double* rows = out + 16;
         hipLaunchKernelGGL(k_energy_rows, dim3((n_interact + 255) / 256), dim3(256), 0, st, x, y, z, vx, vy, vz, m,
@@ -768,20 +871,28 @@ int energy_launch(const double* x, const double* y, const double* z, const doubl
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
+static constexpr size_t kEnergyExactMax = 16384;             // bodies
+static constexpr size_t kEnergyTermsMax = size_t(4) << 20;    // pair terms per batch (32 MB)
+
 double energy_device(struct reb_simulation* r) {
     DeviceState* d = dev_of(r);
     const size_t N = r->N;
     const size_t N_active = (r->N_active == SIZE_MAX) ? N : r->N_active;
     const size_t N_interact = (r->testparticle_type == 0) ? N_active : N;
     if (N == 0) return r->energy_offset;
-    if (!d->scratch.ensure(sizeof(double) * (2 * N + 16))) {
+    // EXACT: the reference's sequential sum, bit-identical, up to kEnergyExactMax
+    // bodies (a serial chain of N^2/2 additions); above, the compensated sum.
+    const bool exact = d->precision == REB_AMD_PRECISION_EXACT && N_interact <= kEnergyExactMax;
+    const size_t terms_cap = exact ? std::min<size_t>(kEnergyTermsMax, N_active * N_interact / 2 + N_interact) : 0;
+    const size_t head = 2 * N + 16;
+    if (!d->scratch.ensure(sizeof(double) * (head + terms_cap))) {
         reb_simulation_error(r, "Out of device memory for the energy reduction.");
         return nan("");
     }
     double* out = d->scratch.as<double>();
     if (energy_launch(d->field(FX), d->field(FY), d->field(FZ), d->field(FVX), d->field(FVY), d->field(FVZ),
-                      d->field(FM), (int)N, (int)N_active, (int)N_interact, r->G,
-                      d->precision == REB_AMD_PRECISION_EXACT, out, d->stream)) {
+                      d->field(FM), (int)N, (int)N_active, (int)N_interact, r->G, exact, out, d->stream,
+                      exact ? out + head : nullptr, terms_cap)) {
         (void)RB_HIP(r, hipGetLastError());
         return nan("");
     }

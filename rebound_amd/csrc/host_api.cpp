@@ -112,7 +112,7 @@ static const struct reb_integrator rb_integrator_sei = {
     (char*)"Symplectic Epicycle Integrator for the shearing sheet, on the GPU.",
     rb_sei_step, nullptr, rb_sei_create, rb_plain_free, nullptr, nullptr, nullptr};
 static const struct reb_integrator rb_integrator_whfast = {
-    (char*)"WHFast (democratic heliocentric coordinates), Kepler drifts on the GPU.",
+    (char*)"WHFast (Jacobi or democratic heliocentric coordinates, symplectic correctors), on the GPU.",
     rb_whfast_step, rb_whfast_synchronize, rb_whfast_create, rb_whfast_free, nullptr, nullptr, nullptr};
 static const struct reb_integrator rb_integrator_unsupported = {
     (char*)"Not available in this engine.", rb_unsupported_step, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};

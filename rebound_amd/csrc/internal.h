@@ -158,7 +158,7 @@ int seg_fast_mode(int precision, int compensated);
 // (potential) on the stream; `scratch` holds 2 n + 16 doubles.
 int energy_launch(const double* x, const double* y, const double* z, const double* vx, const double* vy,
                   const double* vz, const double* m, int n, int n_active, int n_interact, double G, bool exact,
-                  double* scratch, hipStream_t st);
+                  double* scratch, hipStream_t st, double* terms = nullptr, size_t terms_cap = 0);
 bool gravity_zero(struct reb_simulation* r);
 bool gravity_tree(struct reb_simulation* r);                 // tree.hip
 bool collision_search_device(struct reb_simulation* r);      // tree.hip / collision

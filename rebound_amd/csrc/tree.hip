@@ -691,29 +691,78 @@ __global__ __launch_bounds__(256) void k_top2(const double* __restrict__ rad, in
     }
 }
 
+// Swept-sphere test of LINE and LINETREE (collision.c:150-171, :510-534):
+// the closest distance of two bodies moving on straight lines over the last
+// step (dt_last_done), from the separation d1 and relative velocity dv now,
+// against the sum of their radii.  MIN as collision.c:43 spells it.
+__device__ __forceinline__ bool line_overlap(double dx1, double dy1, double dz1, double dvx1, double dvy1,
+                                             double dvz1, double dt, double rsum) {
+    const double r1 = (dx1 * dx1 + dy1 * dy1 + dz1 * dz1);
+    const double dx2 = dx1 - dt * dvx1;
+    const double dy2 = dy1 - dt * dvy1;
+    const double dz2 = dz1 - dt * dvz1;
+    const double r2 = (dx2 * dx2 + dy2 * dy2 + dz2 * dz2);
+    const double t_closest = (dx1 * dvx1 + dy1 * dvy1 + dz1 * dvz1) / (dvx1 * dvx1 + dvy1 * dvy1 + dvz1 * dvz1);
+    double rmin2_ab = r1 > r2 ? r2 : r1;
+    if (t_closest / dt >= 0. && t_closest / dt <= 1.) {
+        const double dx3 = dx1 - t_closest * dvx1;
+        const double dy3 = dy1 - t_closest * dvy1;
+        const double dz3 = dz1 - t_closest * dvz1;
+        const double r3 = (dx3 * dx3 + dy3 * dy3 + dz3 * dz3);
+        rmin2_ab = rmin2_ab > r3 ? r3 : rmin2_ab;
+    }
+    return !(rmin2_ab > rsum * rsum);
+}
+
+// Largest |v|^2 of each block's bodies (collision.c:273-277); the host takes
+// the maximum over blocks (a maximum is exact in any order).
+__global__ __launch_bounds__(256) void k_vmax2(const double* __restrict__ vx, const double* __restrict__ vy,
+                                               const double* __restrict__ vz, int n, double* __restrict__ out) {
+    __shared__ double red[256];
+    double best = 0.;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const double v2 = vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i];
+        best = best > v2 ? best : v2;
+    }
+    red[threadIdx.x] = best;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            const double o = red[threadIdx.x + w];
+            red[threadIdx.x] = red[threadIdx.x] > o ? red[threadIdx.x] : o;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
 // Record of one detected collision: p1, p2, ghost box index, root box index.
 struct CollRec {
     int p1, p2, gb, ri;
 };
 
-// Tree search of collision.c:197-269 + :422-503.  WRITE=false counts the
-// collisions of each body, WRITE=true stores them at the body's offset in
-// detection order (ghost box, root box, depth-first).
-template <bool WRITE>
+// Tree search of collision.c:197-269 + :422-503 (TREE) or :270-331 +
+// :506-566 (LINETREE, LINE = true).  WRITE=false counts the collisions of
+// each body, WRITE=true stores them at the body's offset in detection order
+// (ghost box, root box, depth-first).  TREE passes the second largest radius
+// as `reach`; LINETREE the largest drift of the last step, dt_last_done*|v|max.
+template <bool WRITE, bool LINE = false>
 __global__ __launch_bounds__(64) void k_tree_collide(const CollCell* __restrict__ cells, int total,
                                                      const int* __restrict__ idx, int n,
                                                      const double* __restrict__ x, const double* __restrict__ y,
                                                      const double* __restrict__ z, const double* __restrict__ vx,
                                                      const double* __restrict__ vy, const double* __restrict__ vz,
-                                                     const double* __restrict__ rad, GBT gl, double second_r,
+                                                     const double* __restrict__ rad, GBT gl, double reach,
                                                      int* __restrict__ counts, const int* __restrict__ offsets,
-                                                     CollRec* __restrict__ rec) {
+                                                     CollRec* __restrict__ rec, double dt_last = 0.) {
     const int s = blockIdx.x * 64 + threadIdx.x;
     const bool live = s < n;
     const int me = live ? idx[s] : -1;
     const double px = live ? x[me] : 0., py = live ? y[me] : 0., pz = live ? z[me] : 0.;
     const double pvx = live ? vx[me] : 0., pvy = live ? vy[me] : 0., pvz = live ? vz[me] : 0.;
     const double p1_r = live ? rad[me] : 0.;
+    // LINETREE: the radius grown by the body's own drift (collision.c:293)
+    const double p1_r_plus_dtv = LINE ? p1_r + dt_last * sqrt(pvx * pvx + pvy * pvy + pvz * pvz) : 0.;
     int found = 0;
     int out = (WRITE && live) ? offsets[me] : 0;
     for (int g = 0; g < gl.n; g++) {
@@ -736,15 +785,23 @@ __global__ __launch_bounds__(64) void k_tree_collide(const CollCell* __restrict_
                         const double dx = gx - x[pt];
                         const double dy = gy - y[pt];
                         const double dz = gz - z[pt];
-                        const double r2 = dx * dx + dy * dy + dz * dz;
-                        const double rp = p1_r + rad[pt];
-                        if (!(r2 > rp * rp)) {
-                            const double dvx = gvx - vx[pt];
-                            const double dvy = gvy - vy[pt];
-                            const double dvz = gvz - vz[pt];
-                            if (!(dvx * dx + dvy * dy + dvz * dz > 0)) {
+                        if (LINE) {
+                            if (line_overlap(dx, dy, dz, gvx - vx[pt], gvy - vy[pt], gvz - vz[pt], dt_last,
+                                             p1_r + rad[pt])) {
                                 if (WRITE) rec[out++] = CollRec{me, pt, g, ri};
                                 found++;
+                            }
+                        } else {
+                            const double r2 = dx * dx + dy * dy + dz * dz;
+                            const double rp = p1_r + rad[pt];
+                            if (!(r2 > rp * rp)) {
+                                const double dvx = gvx - vx[pt];
+                                const double dvy = gvy - vy[pt];
+                                const double dvz = gvz - vz[pt];
+                                if (!(dvx * dx + dvy * dy + dvz * dz > 0)) {
+                                    if (WRITE) rec[out++] = CollRec{me, pt, g, ri};
+                                    found++;
+                                }
                             }
                         }
                     }
@@ -753,7 +810,8 @@ __global__ __launch_bounds__(64) void k_tree_collide(const CollCell* __restrict_
                     const double dy = gy - cur.cy;
                     const double dz = gz - cur.cz;
                     const double r2 = dx * dx + dy * dy + dz * dz;
-                    const double rp = p1_r + second_r + 0.86602540378443 * cur.w;
+                    const double rp = LINE ? p1_r_plus_dtv + reach + 0.86602540378443 * cur.w
+                                           : p1_r + reach + 0.86602540378443 * cur.w;
                     if (r2 < rp * rp)
                         open = true;
                     else
@@ -785,14 +843,17 @@ __global__ void k_inverse_perm(const int* __restrict__ idx, int* __restrict__ po
 }
 
 // ---------------------------------------------------------------- direct search
-// collision.c:64-124: ghost box outer, projectile i, target j ascending.
-template <bool WRITE>
+// collision.c:64-124 (DIRECT): ghost box outer, projectile i, target j
+// ascending.  collision.c:125-196 (LINE = true): the same loops over the
+// pairs j > i with the swept-sphere test.
+template <bool WRITE, bool LINE = false>
 __global__ __launch_bounds__(256) void k_direct_collide(const double* __restrict__ x, const double* __restrict__ y,
                                                         const double* __restrict__ z, const double* __restrict__ vx,
                                                         const double* __restrict__ vy, const double* __restrict__ vz,
                                                         const double* __restrict__ rad, int n, int n_targets,
                                                         GBT gl, int* __restrict__ counts,
-                                                        const int* __restrict__ offsets, CollRec* __restrict__ rec) {
+                                                        const int* __restrict__ offsets, CollRec* __restrict__ rec,
+                                                        double dt_last = 0.) {
     __shared__ double4 tp[256];
     __shared__ double4 tv[256];
     const int g = blockIdx.y;
@@ -817,9 +878,17 @@ __global__ __launch_bounds__(256) void k_direct_collide(const double* __restrict
         const int m = min(256, n_targets - t);
         for (int k = 0; k < m; k++) {
             const int j = t + k;
-            if (i == j) continue;
+            if (LINE ? j <= i : i == j) continue;
             const double4 q = tp[k];
             const double dx = gx - q.x, dy = gy - q.y, dz = gz - q.z;
+            if (LINE) {
+                const double4 qv = tv[k];
+                if (line_overlap(dx, dy, dz, gvx - qv.x, gvy - qv.y, gvz - qv.z, dt_last, p1_r + q.w)) {
+                    if (WRITE) rec[out++] = CollRec{i, j, g, 0};
+                    found++;
+                }
+                continue;
+            }
             const double sr = p1_r + q.w;
             const double r2 = dx * dx + dy * dy + dz * dz;
             if (r2 > sr * sr) continue;
@@ -984,10 +1053,66 @@ bool collision_search_device(struct reb_simulation* r) {
             }
             return fetch_collisions(r, T, total, gl);
         }
-        case REB_COLLISION_LINE:
-        case REB_COLLISION_LINETREE:
-            reb_simulation_error(r, "LINE and LINETREE collision searches are not part of this engine.");
-            return false;
+        case REB_COLLISION_LINE: {
+            // collision.c:125-196: pairs j > i of all bodies (no N_targets)
+            const int nslots = gl.n * n;
+            const double dt_last = r->dt_last_done;
+            if (!ensure_coll(r, T, nslots, 1)) return false;
+            dim3 grid((n + 255) / 256, gl.n);
+            hipLaunchKernelGGL((k_direct_collide<false, true>), grid, dim3(256), 0, d->stream, d->field(FX),
+                               d->field(FY), d->field(FZ), d->field(FVX), d->field(FVY), d->field(FVZ), d->field(FR), n,
+                               n, gl, T->coll_cnt.as<int>(), nullptr, nullptr, dt_last);
+            const long total = scan_counts(r, T, nslots);
+            if (total < 0) return false;
+            if (!ensure_coll(r, T, nslots, (size_t)total + 1)) return false;
+            if (total > 0)
+                hipLaunchKernelGGL((k_direct_collide<true, true>), grid, dim3(256), 0, d->stream, d->field(FX),
+                                   d->field(FY), d->field(FZ), d->field(FVX), d->field(FVY), d->field(FVZ),
+                                   d->field(FR), n, n, gl, nullptr, T->coll_off.as<int>(), T->coll_rec.as<CollRec>(),
+                                   dt_last);
+            return fetch_collisions(r, T, total, gl);
+        }
+        case REB_COLLISION_LINETREE: {
+            // collision.c:270-331: the largest drift of the last step widens
+            // every cell test
+            const double dt_last = r->dt_last_done;
+            const int nb = std::min(kTop2Blocks, (n + 255) / 256);
+            if (!T->info.ensure(kInfoBytes)) return false;
+            double* top = reinterpret_cast<double*>(T->info.as<int>() + 16);
+            hipLaunchKernelGGL(k_vmax2, dim3(nb), dim3(256), 0, d->stream, d->field(FVX), d->field(FVY),
+                               d->field(FVZ), n, top);
+            std::vector<double> h(nb);
+            if (!RB_HIP(r, hipMemcpyAsync(h.data(), top, sizeof(double) * nb, hipMemcpyDeviceToHost, d->stream)))
+                return false;
+            if (!stream_sync(r)) return false;
+            double vmax2 = 0.;
+            for (double v : h) vmax2 = vmax2 > v ? vmax2 : v;
+            const double maxdrift = dt_last * sqrt(vmax2);
+            if (!tree_build(r)) return false;
+            if (!ensure_coll(r, T, n, 1)) return false;
+            const dim3 grid((n + 63) / 64);
+            hipLaunchKernelGGL((k_tree_collide<false, true>), grid, dim3(64), 0, d->stream, T->crec.as<CollCell>(),
+                               T->nnodes, T->idx.as<int>(), n, d->field(FX), d->field(FY), d->field(FZ),
+                               d->field(FVX), d->field(FVY), d->field(FVZ), d->field(FR), gl, maxdrift,
+                               T->coll_cnt.as<int>(), nullptr, nullptr, dt_last);
+            const long total = scan_counts(r, T, n);
+            if (total < 0) return false;
+            if (!ensure_coll(r, T, n, (size_t)total + 1)) return false;
+            if (total > 0) {
+                hipLaunchKernelGGL((k_tree_collide<true, true>), grid, dim3(64), 0, d->stream,
+                                   T->crec.as<CollCell>(), T->nnodes, T->idx.as<int>(), n, d->field(FX),
+                                   d->field(FY), d->field(FZ), d->field(FVX), d->field(FVY), d->field(FVZ),
+                                   d->field(FR), gl, maxdrift, nullptr, T->coll_off.as<int>(),
+                                   T->coll_rec.as<CollRec>(), dt_last);
+                int* pos_of = T->lcp.as<int>();
+                hipLaunchKernelGGL(k_inverse_perm, dim3((n + 255) / 256), dim3(256), 0, d->stream, T->idx.as<int>(),
+                                   pos_of, n);
+                hipLaunchKernelGGL(k_coll_root, dim3((total + 255) / 256), dim3(256), 0, d->stream,
+                                   T->coll_rec.as<CollRec>(), (int)total, T->key0.as<unsigned long long>(), pos_of,
+                                   T->L1);
+            }
+            return fetch_collisions(r, T, total, gl);
+        }
         default:
             return true;
     }

@@ -233,11 +233,91 @@ struct Slot {
 // slot values, in index order.  Every lane of the group evaluates the same sum.
 #define GROUP_LOOP(var, lo, hi) for (int var = (lo); var < (hi); var++)
 
+// Symplectic corrector coefficients of Wisdom, Holman & Touma (1996), the
+// values integrator_whfast.c:125-152 spells: a_k = k * sqrt(7/40) and the
+// b coefficients of the 3rd to 17th order correctors; kCorr2B is the second
+// corrector's coefficient (Wisdom 2006).
+__constant__ double kCorrA[9] = {0.,
+                                 0.41833001326703777398908601289259374469640768464934,
+                                 0.83666002653407554797817202578518748939281536929867,
+                                 1.2549900398011133219672580386777812340892230539480,
+                                 1.6733200530681510959563440515703749787856307385973,
+                                 2.0916500663351888699454300644629687234820384232467,
+                                 2.5099800796022266439345160773555624681784461078960,
+                                 2.9283100928692644179236020902481562128748537925454,
+                                 3.3466401061363021919126881031407499575712614771947};
+static constexpr double kCorr2B = 0.03486083443891981449909050107438281205803;
+// One corrector of order n is a chain of Z(a, b) operators.  An entry holds
+// the signed index of a (a = sign * a_|k| * dt) and the b coefficient for
+// inv = +1 (b = inv * coef * dt); the chains of integrator_whfast.c:696-751.
+struct CorrZ {
+    signed char ka;
+    double kb;
+};
+__constant__ CorrZ kCorr3[2] = {{1, 0.024900596027799867499350357910273437184309981229127},
+                                {-1, -0.024900596027799867499350357910273437184309981229127}};
+__constant__ CorrZ kCorr5[4] = {{-2, 0.0083001986759332891664501193034244790614366604097090},
+                                {-1, -0.041500993379666445832250596517122395307183302048545},
+                                {1, 0.041500993379666445832250596517122395307183302048545},
+                                {2, -0.0083001986759332891664501193034244790614366604097090}};
+__constant__ CorrZ kCorr7[6] = {{-3, -0.0024926811426922105779030593952776964450539008582219},
+                                {-2, 0.018270923246702131478062356884535264841652263842597},
+                                {-1, -0.053964399093127498721765893493510877532452806339655},
+                                {1, 0.053964399093127498721765893493510877532452806339655},
+                                {2, -0.018270923246702131478062356884535264841652263842597},
+                                {3, 0.0024926811426922105779030593952776964450539008582219}};
+__constant__ CorrZ kCorr11[10] = {{-5, -0.00020361579647854651301632818774633716473696537436847},
+                                  {-4, 0.0023487215292295354188307328851055489876255097419754},
+                                  {-3, -0.012309078592019946317544564763237909911330686448336},
+                                  {-2, 0.038121613681288650508647613260247372125243616270670},
+                                  {-1, -0.072593394748842738674253180742744961827622366521517},
+                                  {1, 0.072593394748842738674253180742744961827622366521517},
+                                  {2, -0.038121613681288650508647613260247372125243616270670},
+                                  {3, 0.012309078592019946317544564763237909911330686448336},
+                                  {4, -0.0023487215292295354188307328851055489876255097419754},
+                                  {5, 0.00020361579647854651301632818774633716473696537436847}};
+__constant__ CorrZ kCorr17[16] = {{-8, 0.0000043347415473373580190650223498124944896789841432241},
+                                  {-7, -0.000076436355227935738363241846979413475106795392377415},
+                                  {-6, 0.00063599983075817658983166881625078545864140848560259},
+                                  {-5, -0.0033132577069380655655490196833451994080066801611459},
+                                  {-4, 0.012071760822342291062449751726959664253913904872527},
+                                  {-3, -0.032422198864713580293681523029577130832258806467604},
+                                  {-2, 0.065192863576377893658290760803725762027864651086787},
+                                  {-1, -0.093056103771425958591541059067553547100903397724386},
+                                  {1, 0.093056103771425958591541059067553547100903397724386},
+                                  {2, -0.065192863576377893658290760803725762027864651086787},
+                                  {3, 0.032422198864713580293681523029577130832258806467604},
+                                  {4, -0.012071760822342291062449751726959664253913904872527},
+                                  {5, 0.0033132577069380655655490196833451994080066801611459},
+                                  {6, -0.00063599983075817658983166881625078545864140848560259},
+                                  {7, 0.000076436355227935738363241846979413475106795392377415},
+                                  {8, -0.0000043347415473373580190650223498124944896789841432241}};
+
+__device__ __forceinline__ const CorrZ* corrector_chain(int order, int& n) {
+    switch (order) {
+        case 3: n = 2; return kCorr3;
+        case 5: n = 4; return kCorr5;
+        case 7: n = 6; return kCorr7;
+        case 11: n = 10; return kCorr11;
+        case 17: n = 16; return kCorr17;
+        default: n = 0; return nullptr;
+    }
+}
+
 // One wavefront per workgroup and rarely more than two per SIMD (an ensemble of
 // 16384 systems is 1317 wave tasks): the register budget allows two waves per
 // SIMD, so a step's whole state (about 40 doubles per lane plus the Kepler
 // solver's temporaries) stays in VGPRs without spills.
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_whfast_dh(WHParams P) {
+//
+// COORD selects the reference's coordinate system: WH_DH (democratic
+// heliocentric) or WH_JACOBI.  In Jacobi coordinates the star takes part in
+// the interaction (gravity_ignore_terms = BETWEEN_0_AND_1), the transforms are
+// the sequential scans of transformations.c:34-81,147-176,178-258, which every
+// lane of a group evaluates over the published bodies in the reference's order,
+// and the symplectic correctors run at the synchronisation points.
+template <int COORD>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_whfast(WHParams P) {
+    constexpr bool JAC = COORD == WH_JACOBI;
     __shared__ Slot slot[64];
     __shared__ int task_id;
     const int lane = threadIdx.x;
@@ -264,13 +344,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
         const size_t b0 = (size_t)S.body0;
         // ---- load
         double ix = 0, iy = 0, iz = 0, ivx = 0, ivy = 0, ivz = 0, im = 0;   // inertial planet
-        double hx = 0, hy = 0, hz = 0, hvx = 0, hvy = 0, hvz = 0;           // heliocentric planet
+        double hx = 0, hy = 0, hz = 0, hvx = 0, hvy = 0, hvz = 0;           // heliocentric / Jacobi planet
         double sx = 0, sy = 0, sz = 0, svx = 0, svy = 0, svz = 0, sm = 0;   // inertial star
-        double cx = 0, cy = 0, cz = 0, cvx = 0, cvy = 0, cvz = 0, cm = 0;   // p_h[0] (barycentre)
+        double cx = 0, cy = 0, cz = 0, cvx = 0, cvy = 0, cvz = 0, cm = 0;   // p_jh[0] (barycentre)
         double ax = 0, ay = 0, az = 0;
+        double sax = 0, say = 0, saz = 0;                                   // star (Jacobi only)
         if (sys_ok) {
             sx = P.body[0 * st + b0]; sy = P.body[1 * st + b0]; sz = P.body[2 * st + b0];
             svx = P.body[3 * st + b0]; svy = P.body[4 * st + b0]; svz = P.body[5 * st + b0];
+            if (JAC) {
+                sax = P.body[6 * st + b0]; say = P.body[7 * st + b0]; saz = P.body[8 * st + b0];
+            }
             sm = P.body[9 * st + b0];
             cx = P.ph[0 * st + b0]; cy = P.ph[1 * st + b0]; cz = P.ph[2 * st + b0];
             cvx = P.ph[3 * st + b0]; cvy = P.ph[4 * st + b0]; cvz = P.ph[5 * st + b0];
@@ -285,7 +369,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
             hx = P.ph[0 * st + b]; hy = P.ph[1 * st + b]; hz = P.ph[2 * st + b];
             hvx = P.ph[3 * st + b]; hvy = P.ph[4 * st + b]; hvz = P.ph[5 * st + b];
         }
-        double hm = im;   // p_h[i].m
+        double hm = im;   // p_jh[i].m (DH: equal to the inertial mass on this engine)
+        if (JAC && live) hm = P.ph[6 * st + b0 + me];
         const unsigned int flags0 = sys_ok ? P.flags[sidx] : 1u;
         int sync = flags0 & 1u;
         int warn = (flags0 >> 1) & 3u;        // warnings stay set until the host clears them
@@ -305,134 +390,398 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
             slot[lane] = v;
             __syncthreads();
         };
-        auto publish_inertial = [&]() { publish(Slot{ix, iy, iz, ivx, ivy, ivz, im}); };
-        // transformations.c:388-424 (inertial -> democratic heliocentric)
-        auto from_inertial = [&]() {
-            publish(Slot{ix * im, iy * im, iz * im, ivx * im, ivy * im, ivz * im, im});
-            double x0 = 0., y0 = 0., z0 = 0., vx0 = 0., vy0 = 0., vz0 = 0., m0 = 0.;
-            if (0 < NA) {
-                const double m = sm;
-                x0 += sx * m; y0 += sy * m; z0 += sz * m;
-                vx0 += svx * m; vy0 += svy * m; vz0 += svz * m;
-                m0 += m;
-            }
+        // Jacobi: eta of my Kepler problem and of my interaction term, m0 plus
+        // the masses p_jh[1..min(me, NA-1)].m added in index order
+        // (integrator_whfast.c:595-605 serial form, :446-451).
+        double eta_me = 0.;
+        auto interior_mass = [&]() {
+            publish(Slot{0., 0., 0., 0., 0., 0., hm});
+            double eta = sm;
             GROUP_LOOP(k, 1, NA) {
-                const Slot q = slot[gbase + k - 1];
-                x0 += q.x; y0 += q.y; z0 += q.z;
-                vx0 += q.vx; vy0 += q.vy; vz0 += q.vz;
-                m0 += q.m;
+                if (k <= me) eta += slot[gbase + k - 1].m;
             }
-            cx = x0 / m0; cy = y0 / m0; cz = z0 / m0;
-            cvx = vx0 / m0; cvy = vy0 / m0; cvz = vz0 / m0;
-            cm = m0;
-            hx = ix - sx; hy = iy - sy; hz = iz - sz;
-            hvx = ivx - cvx; hvy = ivy - cvy; hvz = ivz - cvz;
-            hm = im;
+            eta_me = eta;
         };
-        // transformations.c:426-477 (democratic heliocentric -> inertial,
-        // positions then velocities).  Barriers are executed by every lane;
-        // lanes whose system does not take part (en == false) keep their state.
-        // The velocity pass is skipped (with_vel == false) where nothing reads
-        // the inertial velocities before they are recomputed.
-        auto to_inertial = [&](bool en, bool with_vel) {
-            const double mtot = cm;
-            const double m0 = sm;   // particles[0].m; particles[i].m == p_h[i].m after the position pass
-            Slot t{hx * hm / mtot, hy * hm / mtot, hz * hm / mtot, 0., 0., 0., 0.};
-            if (with_vel) {
-                t.vx = hvx * hm / m0;
-                t.vy = hvy * hm / m0;
-                t.vz = hvz * hm / m0;
+        if (JAC) interior_mass();
+
+        // ------------------------------------------------ coordinate transforms
+        // DH: transformations.c:388-424.  Jacobi: transformations.c:34-81.
+        auto from_inertial = [&]() {
+            if constexpr (JAC) {
+                publish(Slot{ix, iy, iz, ivx, ivy, ivz, im});
+                double eta = sm;
+                double s_x = eta * sx, s_y = eta * sy, s_z = eta * sz;
+                double s_vx = eta * svx, s_vy = eta * svy, s_vz = eta * svz;
+                GROUP_LOOP(k, 1, NA) {
+                    const Slot q = slot[gbase + k - 1];
+                    const double ei = 1. / eta;
+                    eta += q.m;
+                    const double pme = eta * ei;
+                    const double jx = q.x - s_x * ei, jy = q.y - s_y * ei, jz = q.z - s_z * ei;
+                    const double jvx = q.vx - s_vx * ei, jvy = q.vy - s_vy * ei, jvz = q.vz - s_vz * ei;
+                    if (k == me) {
+                        hx = jx; hy = jy; hz = jz;
+                        hvx = jvx; hvy = jvy; hvz = jvz;
+                        eta_me = eta;
+                    }
+                    s_x = s_x * pme + q.m * jx;
+                    s_y = s_y * pme + q.m * jy;
+                    s_z = s_z * pme + q.m * jz;
+                    s_vx = s_vx * pme + q.m * jvx;
+                    s_vy = s_vy * pme + q.m * jvy;
+                    s_vz = s_vz * pme + q.m * jvz;
+                }
+                const double ei = 1. / eta;
+                if (me >= NA) {
+                    hx = ix - s_x * ei; hy = iy - s_y * ei; hz = iz - s_z * ei;
+                    hvx = ivx - s_vx * ei; hvy = ivy - s_vy * ei; hvz = ivz - s_vz * ei;
+                    eta_me = eta;
+                }
+                const double Mtotali = 1. / eta;
+                cm = eta;
+                cx = s_x * Mtotali; cy = s_y * Mtotali; cz = s_z * Mtotali;
+                cvx = s_vx * Mtotali; cvy = s_vy * Mtotali; cvz = s_vz * Mtotali;
+                hm = im;
+            } else {
+                publish(Slot{ix * im, iy * im, iz * im, ivx * im, ivy * im, ivz * im, im});
+                double x0 = 0., y0 = 0., z0 = 0., vx0 = 0., vy0 = 0., vz0 = 0., m0 = 0.;
+                if (0 < NA) {
+                    const double m = sm;
+                    x0 += sx * m; y0 += sy * m; z0 += sz * m;
+                    vx0 += svx * m; vy0 += svy * m; vz0 += svz * m;
+                    m0 += m;
+                }
+                GROUP_LOOP(k, 1, NA) {
+                    const Slot q = slot[gbase + k - 1];
+                    x0 += q.x; y0 += q.y; z0 += q.z;
+                    vx0 += q.vx; vy0 += q.vy; vz0 += q.vz;
+                    m0 += q.m;
+                }
+                cx = x0 / m0; cy = y0 / m0; cz = z0 / m0;
+                cvx = vx0 / m0; cvy = vy0 / m0; cvz = vz0 / m0;
+                cm = m0;
+                hx = ix - sx; hy = iy - sy; hz = iz - sz;
+                hvx = ivx - cvx; hvy = ivy - cvy; hvz = ivz - cvz;
+                hm = im;
             }
-            publish(t);
-            double x0 = 0., y0 = 0., z0 = 0.;
-            double vx0 = 0., vy0 = 0., vz0 = 0.;
-            GROUP_LOOP(k, 1, NA) {
-                const Slot q = slot[gbase + k - 1];
-                x0 += q.x;
-                y0 += q.y;
-                z0 += q.z;
+        };
+        // DH: transformations.c:426-477 (positions then velocities).  Jacobi:
+        // transformations.c:178-226 (posvel) and :228-258 (positions only).
+        // Barriers are executed by every lane; lanes whose system does not
+        // take part (en == false) keep their state.  The velocity pass is
+        // skipped (with_vel == false) where nothing reads the inertial
+        // velocities before they are recomputed.
+        auto to_inertial = [&](bool en, bool with_vel) {
+            if constexpr (JAC) {
+                Slot t{hx, hy, hz, 0., 0., 0., im};
                 if (with_vel) {
-                    vx0 += q.vx;
-                    vy0 += q.vy;
-                    vz0 += q.vz;
+                    t.vx = hvx; t.vy = hvy; t.vz = hvz;
+                }
+                publish(t);
+                double eta = cm;
+                double s_x = cx * eta, s_y = cy * eta, s_z = cz * eta;
+                double s_vx = 0., s_vy = 0., s_vz = 0.;
+                if (with_vel) {
+                    s_vx = cvx * eta; s_vy = cvy * eta; s_vz = cvz * eta;
+                }
+                double nx = ix, ny = iy, nz = iz, nvx = ivx, nvy = ivy, nvz = ivz;
+                if (me >= NA) {
+                    const double ei = 1. / eta;
+                    nx = hx + s_x * ei; ny = hy + s_y * ei; nz = hz + s_z * ei;
+                    if (with_vel) {
+                        nvx = hvx + s_vx * ei; nvy = hvy + s_vy * ei; nvz = hvz + s_vz * ei;
+                    }
+                }
+                for (int k = NA - 1; k > 0; k--) {
+                    const Slot q = slot[gbase + k - 1];
+                    const double ei = 1. / eta;
+                    s_x = (s_x - q.m * q.x) * ei;
+                    s_y = (s_y - q.m * q.y) * ei;
+                    s_z = (s_z - q.m * q.z) * ei;
+                    if (with_vel) {
+                        s_vx = (s_vx - q.m * q.vx) * ei;
+                        s_vy = (s_vy - q.m * q.vy) * ei;
+                        s_vz = (s_vz - q.m * q.vz) * ei;
+                    }
+                    if (k == me) {
+                        nx = q.x + s_x; ny = q.y + s_y; nz = q.z + s_z;
+                        if (with_vel) {
+                            nvx = q.vx + s_vx; nvy = q.vy + s_vy; nvz = q.vz + s_vz;
+                        }
+                    }
+                    eta -= q.m;
+                    s_x *= eta; s_y *= eta; s_z *= eta;
+                    if (with_vel) {
+                        s_vx *= eta; s_vy *= eta; s_vz *= eta;
+                    }
+                }
+                if (!en) return;
+                const double mi = 1. / eta;
+                sx = s_x * mi; sy = s_y * mi; sz = s_z * mi;
+                if (live) {
+                    ix = nx; iy = ny; iz = nz;
+                }
+                if (with_vel) {
+                    svx = s_vx * mi; svy = s_vy * mi; svz = s_vz * mi;
+                    if (live) {
+                        ivx = nvx; ivy = nvy; ivz = nvz;
+                    }
+                }
+            } else {
+                const double mtot = cm;
+                const double m0 = sm;   // particles[0].m; particles[i].m == p_h[i].m after the position pass
+                Slot t{hx * hm / mtot, hy * hm / mtot, hz * hm / mtot, 0., 0., 0., 0.};
+                if (with_vel) {
+                    t.vx = hvx * hm / m0;
+                    t.vy = hvy * hm / m0;
+                    t.vz = hvz * hm / m0;
+                }
+                publish(t);
+                double x0 = 0., y0 = 0., z0 = 0.;
+                double vx0 = 0., vy0 = 0., vz0 = 0.;
+                GROUP_LOOP(k, 1, NA) {
+                    const Slot q = slot[gbase + k - 1];
+                    x0 += q.x;
+                    y0 += q.y;
+                    z0 += q.z;
+                    if (with_vel) {
+                        vx0 += q.vx;
+                        vy0 += q.vy;
+                        vz0 += q.vz;
+                    }
+                }
+                if (!en) return;
+                if (me < NA) im = hm;
+                sx = cx - x0; sy = cy - y0; sz = cz - z0;
+                ix = hx + sx; iy = hy + sy; iz = hz + sz;
+                if (with_vel) {
+                    ivx = hvx + cvx; ivy = hvy + cvy; ivz = hvz + cvz;
+                    svx = cvx - vx0; svy = cvy - vy0; svz = cvz - vz0;
                 }
             }
-            if (!en) return;
-            if (me < NA) im = hm;
-            sx = cx - x0; sy = cy - y0; sz = cz - z0;
-            ix = hx + sx; iy = hy + sy; iz = hz + sz;
-            if (with_vel) {
-                ivx = hvx + cvx; ivy = hvy + cvy; ivz = hvz + cvz;
-                svx = cvx - vx0; svy = cvy - vy0; svz = cvz - vz0;
-            }
         };
-        // integrator_whfast.c:612-617 and :638-651
-        auto kepler_com = [&](double h, bool en) {
+        // integrator_whfast.c:589-636 (DH mu = G m0; Jacobi mu = G eta_i)
+        auto kepler = [&](double h, bool en) {
+            if (!en || !live) return;
+            const double mu = JAC ? eta_me * Gc : sm * Gc;
+            warn |= kepler_solver(hx, hy, hz, hvx, hvy, hvz, mu, h);
+        };
+        // integrator_whfast.c:638-651
+        auto com = [&](double h, bool en) {
             if (!en) return;
-            if (live) warn |= kepler_solver(hx, hy, hz, hvx, hvy, hvz, sm * Gc, h);
             cx += h * cvx;
             cy += h * cvy;
             cz += h * cvz;
         };
-        // integrator_whfast.c:537-554 (masses are the inertial ones, equal to
-        // p_h[i].m here: the engine changes no masses)
+        // integrator_whfast.c:537-554, DH only (masses are the inertial ones,
+        // equal to p_h[i].m here: the engine changes no masses)
         auto jump = [&](double h, bool en) {
-            publish(Slot{hm * hvx, hm * hvy, hm * hvz, 0., 0., 0., 0.});
-            double qx = 0, qy = 0, qz = 0;
-            GROUP_LOOP(k, 1, NA) {
-                const Slot q = slot[gbase + k - 1];
-                qx += q.x;
-                qy += q.y;
-                qz += q.z;
-            }
-            const double m0 = sm;
-            if (live && en) {
-                hx += h * (qx / m0);
-                hy += h * (qy / m0);
-                hz += h * (qz / m0);
+            if constexpr (!JAC) {
+                publish(Slot{hm * hvx, hm * hvy, hm * hvz, 0., 0., 0., 0.});
+                double qx = 0, qy = 0, qz = 0;
+                GROUP_LOOP(k, 1, NA) {
+                    const Slot q = slot[gbase + k - 1];
+                    qx += q.x;
+                    qy += q.y;
+                    qz += q.z;
+                }
+                const double m0 = sm;
+                if (live && en) {
+                    hx += h * (qx / m0);
+                    hy += h * (qy / m0);
+                    hz += h * (qz / m0);
+                }
             }
         };
-        // BASIC (gravity.c:215-232) or COMPENSATED (:308-410) with
-        // gravity_ignore_terms = INVOLVING_0: planets only, ascending j.
+        // One pair term of BASIC (gravity.c:215-232) or COMPENSATED (:308-410),
+        // added to (a, k) in the reference's form.
+        // One pair term of BASIC (gravity.c:215-232) or COMPENSATED (:308-410)
+        // in the reference's form, and its addition to (a, k).
+        auto pair_term = [&](double dx, double dy, double dz, double mj, double& t_x, double& t_y, double& t_z) {
+            if (P.compensated) {
+                const double r2 = dx * dx + dy * dy + dz * dz + P.soft2;
+                const double rr = sqrt(r2);
+                const double prefact = Gc / (r2 * rr);
+                const double prefactj = -prefact * mj;
+                t_x = prefactj * dx;
+                t_y = prefactj * dy;
+                t_z = prefactj * dz;
+            } else {
+                const double _r = sqrt(dx * dx + dy * dy + dz * dz + P.soft2);
+                const double prefact = -Gc / (_r * _r * _r) * mj;
+                t_x = prefact * dx;
+                t_y = prefact * dy;
+                t_z = prefact * dz;
+            }
+        };
+        auto pair_add = [&](double t_x, double t_y, double t_z, double& a_x, double& a_y, double& a_z, double& k_x,
+                            double& k_y, double& k_z) {
+            if (P.compensated) {
+                double tv, yv;
+                yv = t_x - k_x; tv = a_x + yv; k_x = (tv - a_x) - yv; a_x = tv;
+                yv = t_y - k_y; tv = a_y + yv; k_y = (tv - a_y) - yv; a_y = tv;
+                yv = t_z - k_z; tv = a_z + yv; k_z = (tv - a_z) - yv; a_z = tv;
+            } else {
+                a_x += t_x;
+                a_y += t_y;
+                a_z += t_z;
+            }
+        };
+        // Accelerations of BASIC or COMPENSATED gravity, ascending j.  DH
+        // (gravity_ignore_terms = INVOLVING_0): planets only.  Jacobi
+        // (BETWEEN_0_AND_1): the star is a source for planets 2.. and a
+        // target; its sum is ordered over owner-lane terms.  The terms of
+        // four sources at a time are independent and evaluated together;
+        // they are added in ascending j.
         auto gravity = [&](bool en) {
-            publish_inertial();
+            Slot pub{ix, iy, iz, 0., 0., 0., im};
+            if constexpr (JAC) {
+                // star target, my planet as source (pass 0 for me < NAG, pass 1
+                // with test particles of type 1); pair 0-1 ignored
+                if (live && me >= 2 && (me < NAG || S.tp_type))
+                    pair_term(sx - ix, sy - iy, sz - iz, im, pub.vx, pub.vy, pub.vz);
+            }
+            publish(pub);
+            if constexpr (JAC) {
+                if (en) {
+                    double a_x = 0., a_y = 0., a_z = 0., k_x = 0., k_y = 0., k_z = 0.;
+                    for (int j = 2; j < N; j++) {
+                        if (!(j < NAG || S.tp_type)) break;
+                        const Slot q = slot[gbase + j - 1];
+                        pair_add(q.vx, q.vy, q.vz, a_x, a_y, a_z, k_x, k_y, k_z);
+                    }
+                    sax = a_x; say = a_y; saz = a_z;
+                }
+            }
             if (!live || !en) return;
             ax = 0.; ay = 0.; az = 0.;
             double kx = 0., ky = 0., kz = 0.;
+            if (JAC && me != 1 && NAG > 0) {
+                double tx, ty, tz;
+                pair_term(ix - sx, iy - sy, iz - sz, sm, tx, ty, tz);
+                pair_add(tx, ty, tz, ax, ay, az, kx, ky, kz);
+            }
             for (int pass = 0; pass < 2; pass++) {
                 const int j0 = pass == 0 ? 1 : (NAG > 1 ? NAG : 1);
                 const int j1 = pass == 0 ? NAG : N;
                 if (pass == 1 && !(S.tp_type && me < NAG)) break;
-                for (int j = j0; j < j1; j++) {
-                    if (j == me) continue;
-                    const Slot q = slot[gbase + j - 1];
-                    const double dx = ix - q.x;
-                    const double dy = iy - q.y;
-                    const double dz = iz - q.z;
-                    if (P.compensated) {
-                        const double r2 = dx * dx + dy * dy + dz * dz + P.soft2;
-                        const double rr = sqrt(r2);
-                        const double prefact = Gc / (r2 * rr);
-                        const double prefactj = -prefact * q.m;
-                        double t, yv;
-                        yv = prefactj * dx - kx; t = ax + yv; kx = (t - ax) - yv; ax = t;
-                        yv = prefactj * dy - ky; t = ay + yv; ky = (t - ay) - yv; ay = t;
-                        yv = prefactj * dz - kz; t = az + yv; kz = (t - az) - yv; az = t;
-                    } else {
-                        const double _r = sqrt(dx * dx + dy * dy + dz * dz + P.soft2);
-                        const double prefact = -Gc / (_r * _r * _r) * q.m;
-                        ax += prefact * dx;
-                        ay += prefact * dy;
-                        az += prefact * dz;
+                for (int jb = j0; jb < j1; jb += 4) {
+                    double tx[4], ty[4], tz[4];
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const int j = jb + q;
+                        if (j >= j1 || j == me) continue;
+                        const Slot o = slot[gbase + j - 1];
+                        pair_term(ix - o.x, iy - o.y, iz - o.z, o.m, tx[q], ty[q], tz[q]);
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const int j = jb + q;
+                        if (j >= j1 || j == me) continue;
+                        pair_add(tx[q], ty[q], tz[q], ax, ay, az, kx, ky, kz);
                     }
                 }
             }
         };
-        // integrator_whfast.c:1051-1111 (default kernel, no correctors).  Every
-        // lane executes the barriers; `want` selects the systems that take part.
+        // DH: integrator_whfast.c:494-501.  Jacobi: :439-483 (accelerations to
+        // Jacobi form, transformations.c:147-176, then Eq. 132 of the WHFast
+        // paper with the Kepler terms of the interior bodies).
+        auto interaction = [&](double h, bool en) {
+            if constexpr (JAC) {
+                publish(Slot{ax, ay, az, 0., 0., 0., im});
+                double eta = sm;
+                double s_x = eta * sax, s_y = eta * say, s_z = eta * saz;
+                double jax = 0., jay = 0., jaz = 0.;
+                GROUP_LOOP(k, 1, NA) {
+                    const Slot q = slot[gbase + k - 1];
+                    const double ei = 1. / eta;
+                    eta += q.m;
+                    const double pme = eta * ei;
+                    const double jx = q.x - s_x * ei, jy = q.y - s_y * ei, jz = q.z - s_z * ei;
+                    if (k == me) {
+                        jax = jx; jay = jy; jaz = jz;
+                    }
+                    s_x = s_x * pme + q.m * jx;
+                    s_y = s_y * pme + q.m * jy;
+                    s_z = s_z * pme + q.m * jz;
+                }
+                if (me >= NA) {
+                    const double ei = 1. / eta;
+                    jax = ax - s_x * ei; jay = ay - s_y * ei; jaz = az - s_z * ei;
+                }
+                if (!live || !en) return;
+                const double px = hx, py = hy, pz = hz;
+                hvx += h * jax;
+                hvy += h * jay;
+                hvz += h * jaz;
+                if (me > 1) {
+                    const double rj2i = 1. / (px * px + py * py + pz * pz);
+                    const double rji = sqrt(rj2i);
+                    const double rj3iM = rji * rj2i * Gc * eta_me;
+                    const double prefac1 = h * rj3iM;
+                    hvx += prefac1 * px;
+                    hvy += prefac1 * py;
+                    hvz += prefac1 * pz;
+                }
+            } else {
+                if (!live || !en) return;
+                hvx += h * ax;
+                hvy += h * ay;
+                hvz += h * az;
+            }
+        };
+        // Symplectic correctors (Jacobi): integrator_whfast.c:653-751 and
+        // :753-784.  Every lane runs the chain; en selects the systems.
+        auto corrector_Z = [&](double a, double b, bool en) {
+            kepler(a, en);
+            to_inertial(en, false);
+            gravity(en);
+            interaction(-b, en);
+            kepler(-2. * a, en);
+            to_inertial(en, false);
+            gravity(en);
+            interaction(b, en);
+            kepler(a, en);
+        };
+        auto apply_corrector = [&](double inv, bool en) {
+            int n = 0;
+            const CorrZ* c = corrector_chain(P.corrector, n);
+            for (int z = 0; z < n; z++) {
+                const int k = c[z].ka;
+                const double a = (k < 0 ? -kCorrA[-k] : kCorrA[k]) * dt;
+                const double b = (c[z].kb * inv) * dt;
+                corrector_Z(a, b, en);
+            }
+        };
+        auto operator_C = [&](double a, double b, bool en) {
+            kepler(a, en);
+            to_inertial(en, false);
+            gravity(en);
+            interaction(b, en);
+            kepler(-a, en);
+        };
+        auto operator_U = [&](double a, double b, bool en) {
+            kepler(a, en);
+            operator_C(a, b, en);        // Y(a, b)
+            operator_C(-a, -b, en);
+            operator_C(a, -b, en);       // Y(a, -b)
+            operator_C(-a, b, en);
+            kepler(-a, en);
+        };
+        auto apply_corrector2 = [&](double inv, bool en) {
+            const double a = 0.5 * inv * dt;
+            const double b = kCorr2B * inv * dt;
+            operator_U(a, b, en);
+            operator_U(-a, b, en);
+        };
+        // integrator_whfast.c:1051-1111 (default kernel).  Every lane executes
+        // the barriers; `want` selects the systems that take part.
         auto synchronize = [&](bool want) {
             const bool en = want && !sync;
-            kepler_com(dt / 2., en);
+            kepler(dt / 2., en);
+            com(dt / 2., en);
+            if (JAC && P.corrector2 && __any(en)) apply_corrector2(-1., en);
+            if (JAC && P.corrector && __any(en)) apply_corrector(-1., en);
             to_inertial(en, true);
             if (want) sync = 1;
         };
@@ -440,14 +789,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
             // the inertial -> heliocentric transform writes only when en
             const double sx0 = cx, sy0 = cy, sz0 = cz, svx0 = cvx, svy0 = cvy, svz0 = cvz, sm0 = cm;
             const double hx0 = hx, hy0 = hy, hz0 = hz, hvx0 = hvx, hvy0 = hvy, hvz0 = hvz, hm0 = hm;
+            const double eta0 = eta_me;
             from_inertial();
             if (!en) {
                 cx = sx0; cy = sy0; cz = sz0; cvx = svx0; cvy = svy0; cvz = svz0; cm = sm0;
                 hx = hx0; hy = hy0; hz = hz0; hvx = hvx0; hvy = hvy0; hvz = hvz0; hm = hm0;
+                eta_me = eta0;
             }
         };
 
-        // integrator_whfast.c:1113-1259 (DH, default kernel, no correctors)
+        // integrator_whfast.c:1113-1259 (default kernel)
         bool modified = P.did_modify || (sys_ok && (S.req & 2));
         for (long s = 0; s < P.nsteps; s++) {
             const bool act = s < my_steps;
@@ -457,18 +808,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                 synchronize(recalc);
                 from_inertial_if(recalc);
             }
-            kepler_com(sync ? dt / 2. : dt, act);
+            if (JAC && (P.corrector || P.corrector2)) {
+                const bool first = act && sync;
+                if (P.corrector && __any(first)) apply_corrector(1., first);
+                if (P.corrector2 && __any(first)) apply_corrector2(1., first);
+            }
+            kepler(sync ? dt / 2. : dt, act);
+            com(sync ? dt / 2. : dt, act);
             jump(dt / 2., act);
             // Inertial velocities of the middle of a step are visible only
             // after the system's last step of the launch (or are replaced by a
             // synchronization before anything reads them).
             to_inertial(act, act && s == my_steps - 1);
             gravity(act);
-            if (live && act) {
-                hvx += dt * ax;
-                hvy += dt * ay;
-                hvz += dt * az;
-            }
+            interaction(dt, act);
             jump(dt / 2., act);
             if (act) sync = 0;
             if (__any(act && safe)) synchronize(act && safe);
@@ -490,7 +843,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
         if (sys_ok && gl == 0) {
             P.body[0 * st + b0] = sx; P.body[1 * st + b0] = sy; P.body[2 * st + b0] = sz;
             P.body[3 * st + b0] = svx; P.body[4 * st + b0] = svy; P.body[5 * st + b0] = svz;
-            P.body[6 * st + b0] = 0.; P.body[7 * st + b0] = 0.; P.body[8 * st + b0] = 0.;
+            P.body[6 * st + b0] = sax; P.body[7 * st + b0] = say; P.body[8 * st + b0] = saz;
             P.ph[0 * st + b0] = cx; P.ph[1 * st + b0] = cy; P.ph[2 * st + b0] = cz;
             P.ph[3 * st + b0] = cvx; P.ph[4 * st + b0] = cvy; P.ph[5 * st + b0] = cvz;
             P.ph[6 * st + b0] = cm;
@@ -534,7 +887,10 @@ __global__ void k_reset_counter(unsigned int* c) { *c = 0u; }
 
 int whfast_launch(const WHParams& P, hipStream_t st) {
     hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, st, P.counter);
-    hipLaunchKernelGGL(k_whfast_dh, dim3(P.grid), dim3(64), 0, st, P);
+    if (P.coords == WH_JACOBI)
+        hipLaunchKernelGGL(k_whfast<WH_JACOBI>, dim3(P.grid), dim3(64), 0, st, P);
+    else
+        hipLaunchKernelGGL(k_whfast<WH_DH>, dim3(P.grid), dim3(64), 0, st, P);
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
@@ -550,7 +906,7 @@ int whfast_group_size(int K) { return K < 1 ? 1 : (K > 64 ? 64 : K); }
 // Persistent workgroups (one wavefront each) that fit on the device at once.
 int whfast_max_resident(int device) {
     int per_cu = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_whfast_dh, 64, 0) != hipSuccess || per_cu < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_whfast<WH_DH>, 64, 0) != hipSuccess || per_cu < 1)
         per_cu = 8;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1)
         cus = 256;
@@ -580,13 +936,31 @@ static bool whfast_check(struct reb_simulation* r, struct reb_integrator_whfast_
         reb_simulation_error(r, "Variational particles are not supported by this engine.");
         return false;
     }
-    if (s->coordinates != REB_INTEGRATOR_WHFAST_COORDINATES_DEMOCRATICHELIOCENTRIC) {
-        reb_simulation_error(r, "This engine implements WHFast in democratic heliocentric coordinates "
-                                "(set coordinates = REB_INTEGRATOR_WHFAST_COORDINATES_DEMOCRATICHELIOCENTRIC).");
+    if (s->coordinates != REB_INTEGRATOR_WHFAST_COORDINATES_DEMOCRATICHELIOCENTRIC &&
+        s->coordinates != REB_INTEGRATOR_WHFAST_COORDINATES_JACOBI) {
+        reb_simulation_error(r, "This engine implements WHFast in Jacobi and democratic heliocentric coordinates.");
         return false;
     }
-    if (s->kernel != REB_INTEGRATOR_WHFAST_KERNEL_DEFAULT || s->corrector != 0 || s->corrector2 != 0) {
-        reb_simulation_error(r, "WHFast kernels and symplectic correctors are not supported by this engine.");
+    if (s->kernel != REB_INTEGRATOR_WHFAST_KERNEL_DEFAULT && s->coordinates != REB_INTEGRATOR_WHFAST_COORDINATES_JACOBI) {
+        reb_simulation_error(r, "Non-standard kernel requires Jacobi coordinates.");
+        return false;
+    }
+    if (s->kernel != REB_INTEGRATOR_WHFAST_KERNEL_DEFAULT) {
+        reb_simulation_error(r, "WHFast kernels other than the default are not supported by this engine.");
+        return false;
+    }
+    if (s->corrector != 0 && s->coordinates != REB_INTEGRATOR_WHFAST_COORDINATES_JACOBI) {
+        reb_simulation_error(r, "Symplectic correctors are only compatible with Jacobi and Barycentric coordinates.");
+        return false;
+    }
+    if (s->corrector != 0 && s->corrector != 3 && s->corrector != 5 && s->corrector != 7 && s->corrector != 11 &&
+        s->corrector != 17) {
+        reb_simulation_error(r, "First symplectic correctors are only available in the following orders: 0, 3, 5, 7, 11, 17.");
+        return false;
+    }
+    if (s->corrector2 != 0 && s->coordinates != REB_INTEGRATOR_WHFAST_COORDINATES_JACOBI) {
+        // the reference's second corrector runs the Jacobi operators whatever the coordinates
+        reb_simulation_error(r, "The second symplectic corrector is supported with Jacobi coordinates only.");
         return false;
     }
     if (s->keep_unsynchronized == 1 && s->safe_mode == 1)
@@ -599,6 +973,11 @@ static bool whfast_check(struct reb_simulation* r, struct reb_integrator_whfast_
         reb_simulation_error(r, "WHFast on this engine evaluates the interaction with BASIC or COMPENSATED gravity.");
         return false;
     }
+    if (r->additional_forces) {
+        // the interaction runs inside the group kernel, which has no host callbacks
+        reb_simulation_error(r, "additional_forces are not supported with WHFast on this engine.");
+        return false;
+    }
     if (r->N < 1) return false;
     if (r->N - 1 > 64) {
         reb_simulation_error(r, "WHFast on this engine handles up to 64 bodies besides the central one per simulation.");
@@ -608,7 +987,9 @@ static bool whfast_check(struct reb_simulation* r, struct reb_integrator_whfast_
         reb_simulation_error(r, "Ghost boxes are not supported with WHFast.");
         return false;
     }
-    r->gravity_ignore_terms = REB_GRAVITY_IGNORE_TERMS_INVOLVING_0;
+    r->gravity_ignore_terms = s->coordinates == REB_INTEGRATOR_WHFAST_COORDINATES_JACOBI
+                                  ? REB_GRAVITY_IGNORE_TERMS_BETWEEN_0_AND_1
+                                  : REB_GRAVITY_IGNORE_TERMS_INVOLVING_0;
     if (s->N_allocated != r->N) {
         s->N_allocated = r->N;
         s->p_jh = (struct reb_particle*)realloc(s->p_jh, sizeof(struct reb_particle) * r->N);
@@ -726,6 +1107,9 @@ bool whfast_run(struct reb_simulation* r, struct reb_integrator_whfast_state* s,
     P.safe_mode = s->safe_mode ? 1 : 0;
     P.did_modify = r->did_modify_particles ? 1 : 0;
     P.sync_at_end = sync_at_end;
+    P.coords = s->coordinates == REB_INTEGRATOR_WHFAST_COORDINATES_JACOBI ? WH_JACOBI : WH_DH;
+    P.corrector = (int)s->corrector;
+    P.corrector2 = s->corrector2 ? 1 : 0;
     if (whfast_launch(P, dev_of(r)->stream)) return RB_HIP(r, hipGetLastError());
     return whfast_finish(r, s, P);
 }

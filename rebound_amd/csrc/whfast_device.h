@@ -31,6 +31,9 @@ struct WHTask {
     int nsys;
 };
 
+// Coordinate systems of the group kernel (integrator_whfast.h:39-46).
+enum { WH_DH = 0, WH_JACOBI = 1 };
+
 struct WHParams {
     const WHSys* sys;
     const WHTask* tasks;
@@ -49,6 +52,9 @@ struct WHParams {
     int safe_mode;
     int did_modify;
     int sync_at_end;
+    int coords = WH_DH;       // WH_DH or WH_JACOBI
+    int corrector = 0;        // first symplectic corrector order (Jacobi): 0, 3, 5, 7, 11, 17
+    int corrector2 = 0;       // second corrector (Jacobi): 0 or 1
 };
 
 int whfast_launch(const WHParams& P, hipStream_t st);

@@ -89,9 +89,11 @@ class Ensemble:
 
     n_bodies: bodies per system (1..65, central body first); state: rows of
     x y z vx vy vz ax ay az m [r] for all bodies, system after system; dt: one
-    timestep per system (or a scalar)."""
+    timestep per system (or a scalar); coordinates "democraticheliocentric" or
+    "jacobi", the latter with optional symplectic correctors."""
 
-    def __init__(self, n_bodies, state, dt, G=1.0, safe_mode=0, gravity="basic"):
+    def __init__(self, n_bodies, state, dt, G=1.0, safe_mode=0, gravity="basic", coordinates="democraticheliocentric",
+                 corrector=0, corrector2=0):
         L = lib()
         n_bodies = np.ascontiguousarray(n_bodies, dtype=np.uint64)
         state = np.asarray(state, dtype=np.float64)
@@ -113,6 +115,12 @@ class Ensemble:
                                             _abi.GRAVITY[gravity])
         if not self._h:
             raise RebError("reb_amd_ensemble_create failed: " + L.reb_amd_last_error().decode())
+        if coordinates != "democraticheliocentric" or corrector or corrector2:
+            if L.reb_amd_ensemble_set_whfast(self._h, _abi.WHFAST_COORDINATES[coordinates], int(corrector),
+                                             int(corrector2)):
+                msg = L.reb_amd_last_error().decode()
+                self.free()
+                raise RebError(msg)
 
     def free(self):
         if getattr(self, "_h", None):

@@ -13,8 +13,9 @@
  *   leapfrog + energy                    integrator_leapfrog.c:97-209, tools.c:108-162
  *   whfast kepler solver                 integrator_whfast.c:241-427
  *   whfast DH integration                integrator_whfast.c:1113-1259
+ *   whfast Jacobi + symplectic correctors integrator_whfast.c:653-784, 1051-1259
  *   tree build / moments / walk          tree.c:64-328, gravity.c:47-106
- *   collision search + hardsphere        collision.c:49-405, 573-665
+ *   collision search + hardsphere        collision.c:49-405, 573-665 (DIRECT, TREE, LINE, LINETREE)
  *   SEI + shear boundary                 integrator_sei.c:86-174, boundary.c:35-201
  */
 #include <stdio.h>
@@ -519,6 +520,142 @@ static void gen_ensemble(const char* tag, unsigned int seed, int NS, int kmin, i
     reb_simulation_free(g);
 }
 
+/* LINE and LINETREE searches (collision.c:125-196, :270-331, :506-566) on
+ * the small sheet: collisions in resolve order (hard spheres applied) and the
+ * bodies after each step. */
+/* The search input of a step: the bodies, time and seed at the first resolve
+ * call of the step (no collision has been resolved yet). */
+static int pre_pending = 0;
+static double* pre_state = NULL;
+static double pre_t = 0.;
+static enum REB_COLLISION_RESOLVE_OUTCOME rec_resolve_pre(struct reb_simulation* const r, struct reb_collision c){
+    if (pre_pending){
+        pre_pending = 0;
+        pre_state = state_of(r);
+        pre_t = r->t;
+    }
+    return rec_resolve(r, c);
+}
+
+static void gen_line(void){
+    const int kinds[2] = {REB_COLLISION_LINE, REB_COLLISION_LINETREE};
+    const char* tags[2] = {"line", "linetree"};
+    const int nsteps[2] = {1, 3};
+    for (int k=0; k<2; k++){
+        rec_n = 0;
+        struct reb_simulation* r = sheet(2000, 50., 2, 2, 99u);
+        r->collision = kinds[k];
+        r->collision_resolve = rec_resolve_pre;
+        int64_t counts[3] = {0, 0, 0};
+        char name[128];
+        for (int st=0; st<nsteps[k]; st++){
+            size_t before = rec_n;
+            if (st == 0){
+                pre_pending = 1;
+                /* the shuffle draws from the seed the search starts with */
+                int64_t seed = (int64_t)r->rand_seed;
+                snprintf(name, sizeof(name), "sheet2000_%s_seed_before.npy", tags[k]); wi(name, &seed, 1, 0);
+            }
+            reb_simulation_steps(r, 1);
+            if (st == 0 && pre_state){
+                snprintf(name, sizeof(name), "sheet2000_%s_search_state.npy", tags[k]);
+                wd(name, pre_state, r->N, NCOL);
+                double meta[2] = {pre_t, r->dt_last_done};
+                snprintf(name, sizeof(name), "sheet2000_%s_search_t_dt.npy", tags[k]); wd(name, meta, 2, 0);
+                free(pre_state);
+                pre_state = NULL;
+            }
+            counts[st] = (int64_t)(rec_n - before);
+            snprintf(name, sizeof(name), "sheet2000_%s_step%d_state.npy", tags[k], st+1);
+            dump_state(name, r);
+        }
+        snprintf(name, sizeof(name), "sheet2000_%s_pairs.npy", tags[k]); wi(name, rec_pairs, rec_n, 2);
+        snprintf(name, sizeof(name), "sheet2000_%s_gb.npy", tags[k]); wd(name, rec_gb, rec_n, 6);
+        snprintf(name, sizeof(name), "sheet2000_%s_counts.npy", tags[k]); wi(name, counts, nsteps[k], 0);
+        reb_simulation_free(r);
+    }
+}
+
+/* WHFast in Jacobi coordinates (the reference's default): NS systems of
+ * 1..8 planets, each integrated under every mode below for three
+ * reb_simulation_steps calls of `nsteps` steps.  Per mode: energies after 0,
+ * 1, 2, 3 calls and the final bodies; the modes as rows of an int table
+ * (safe_mode, corrector, corrector2, compensated, N_active offset from N
+ * (0: all active), testparticle_type). */
+static void gen_whfast_jacobi(void){
+    const int NS = 16;
+    const int nsteps = 100;
+    enum { NMODE = 8 };
+    static const int64_t modes[NMODE][6] = {
+        {1, 0, 0, 0, 0, 0},
+        {0, 0, 0, 0, 0, 0},
+        {0, 11, 0, 0, 0, 0},
+        {1, 17, 1, 0, 0, 0},
+        {0, 3, 1, 1, 0, 0},
+        {0, 5, 0, 0, 2, 0},
+        {1, 7, 0, 0, 2, 1},
+        {0, 0, 0, 1, 3, 1},
+    };
+    struct reb_simulation* g = reb_simulation_create();
+    g->rand_seed = 4242u;
+    struct reb_simulation* sys[NS];
+    size_t rows = 0;
+    int64_t nb[NS];
+    double dts[NS];
+    for (int s=0; s<NS; s++){
+        sys[s] = ens_system(g, 1 + s % 8, 1 + s % 8);
+        nb[s] = (int64_t)sys[s]->N;
+        dts[s] = 2.*M_PI*sqrt(1./(1.+sys[s]->particles[1].m))/20.;
+        rows += sys[s]->N;
+    }
+    double* init = calloc(rows*NCOL, sizeof(double));
+    size_t row = 0;
+    for (int s=0; s<NS; s++){
+        double* st = state_of(sys[s]);
+        memcpy(init+row*NCOL, st, sizeof(double)*NCOL*sys[s]->N);
+        free(st);
+        row += sys[s]->N;
+    }
+    wd("whfast_jac16_init.npy", init, rows, NCOL);
+    wi("whfast_jac16_nbodies.npy", nb, NS, 0);
+    wd("whfast_jac16_dt.npy", dts, NS, 0);
+    wi("whfast_jac16_modes.npy", &modes[0][0], NMODE, 6);
+    for (int md=0; md<NMODE; md++){
+        double* fin = calloc(rows*NCOL, sizeof(double));
+        double* en = calloc(NS*4, sizeof(double));
+        row = 0;
+        for (int s=0; s<NS; s++){
+            struct reb_simulation* r = reb_simulation_copy(sys[s]);
+            struct reb_integrator_whfast_state* wh = reb_simulation_set_integrator(r, "whfast");
+            wh->coordinates = REB_INTEGRATOR_WHFAST_COORDINATES_JACOBI;
+            wh->safe_mode = (unsigned int)modes[md][0];
+            wh->corrector = (unsigned int)modes[md][1];
+            wh->corrector2 = (unsigned int)modes[md][2];
+            r->gravity = modes[md][3] ? REB_GRAVITY_COMPENSATED : REB_GRAVITY_BASIC;
+            if (modes[md][4] && (int64_t)r->N > modes[md][4]) r->N_active = r->N - (size_t)modes[md][4];
+            r->testparticle_type = (int)modes[md][5];
+            r->dt = dts[s];
+            en[s*4] = reb_simulation_energy(r);
+            for (int c=0; c<3; c++){
+                reb_simulation_steps(r, (size_t)nsteps);
+                en[s*4+1+c] = reb_simulation_energy(r);
+            }
+            double* st = state_of(r);
+            memcpy(fin+row*NCOL, st, sizeof(double)*NCOL*r->N);
+            free(st);
+            row += r->N;
+            reb_simulation_free(r);
+        }
+        char name[128];
+        snprintf(name, sizeof(name), "whfast_jac16_mode%d_final.npy", md); wd(name, fin, rows, NCOL);
+        snprintf(name, sizeof(name), "whfast_jac16_mode%d_energy.npy", md); wd(name, en, NS, 4);
+        free(fin); free(en);
+    }
+    for (int s=0; s<NS; s++) reb_simulation_free(sys[s]);
+    free(init);
+    reb_simulation_free(g);
+}
+
 /* reb_collision_resolve_merge on random three-body states: inputs, the
  * collision (p1, p2), track_energy_offset and N_active; outputs the bodies,
  * the outcome, energy_offset and G. */
@@ -642,5 +779,7 @@ int main(int argc, char** argv){
     }
     gen_sheet();
     gen_merge();
+    gen_whfast_jacobi();
+    gen_line();
     return 0;
 }

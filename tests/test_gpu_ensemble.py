@@ -1,8 +1,9 @@
 """Batched WHFast ensembles against fixtures written by the reference library.
 
 Every system of an ensemble must evolve bit-identically to a reference
-reb_simulation integrating it alone with WHFast (democratic heliocentric,
-integrator_whfast.c:1113-1259) through reb_simulation_steps; the fixtures hold
+reb_simulation integrating it alone with WHFast (democratic heliocentric or
+Jacobi coordinates, integrator_whfast.c:1051-1259) through
+reb_simulation_steps; the fixtures hold
 the reference's energies after each steps() call and its final bodies."""
 import numpy as np
 import pytest
@@ -77,6 +78,26 @@ def test_ensemble_whfast_dh32(gpu, safe):
         assert np.array_equal(ens.system(s, st)[:, :6], fin[s, :nb[s], :6]), s
 
 
+@pytest.mark.parametrize("mode", range(5))
+def test_ensemble_whfast_jacobi(gpu, mode):
+    """Jacobi coordinates with symplectic correctors: the per-simulation
+    fixtures (three steps(100) calls), all sixteen systems as one ensemble."""
+    safe, c1, c2, comp, off, tp = (int(v) for v in golden("whfast_jac16_modes.npy")[mode])
+    assert off == 0 and tp == 0
+    nb = golden("whfast_jac16_nbodies.npy")
+    ens = gpu.Ensemble(nb, golden("whfast_jac16_init.npy"), golden("whfast_jac16_dt.npy"), safe_mode=safe,
+                       gravity="compensated" if comp else "basic", coordinates="jacobi", corrector=c1,
+                       corrector2=c2)
+    E = [ens.energy()]
+    for _ in range(3):
+        ens.steps(100)
+        E.append(ens.energy())
+    assert np.array_equal(np.stack(E, axis=1), golden("whfast_jac16_mode%d_energy.npy" % mode))
+    st = ens.state()
+    assert np.array_equal(st[:, :6], golden("whfast_jac16_mode%d_final.npy" % mode)[:, :6])
+    ens.free()
+
+
 def test_ensemble_tune_keeps_state(gpu):
     nb, init, dt, en, fin = _fixture("ensemble200_seed99")
     ens = gpu.Ensemble(nb, init, dt)
@@ -120,3 +141,10 @@ def test_ensemble_errors(gpu):
         gpu.Ensemble([2], np.zeros((2, 11)), 0.1, gravity="tree")
     with pytest.raises(ValueError):
         gpu.Ensemble([3], np.zeros((2, 11)), 0.1)
+    st = np.zeros((2, 11))
+    st[:, 9] = 1.0
+    st[1, 0] = 1.0
+    with pytest.raises(gpu.RebError, match="following orders"):
+        gpu.Ensemble([2], st, 0.1, coordinates="jacobi", corrector=4)
+    with pytest.raises(gpu.RebError, match="only compatible with Jacobi"):
+        gpu.Ensemble([2], st, 0.1, corrector=3)

@@ -60,6 +60,25 @@ def test_exact_bitwise_testparticles(gpu, which, grav):
     assert np.array_equal(acc, golden("plummer1024_seed1_acc_%s_tp1.npy" % which))
 
 
+@pytest.mark.parametrize("grav", ["basic", "compensated"])
+@pytest.mark.parametrize("n", [3001, 5003, 16500])
+def test_exact_bitwise_larger_n(gpu, orc, grav, n):
+    """The exact kernel's workgroup shapes by size (the 1024-body cases above
+    take 1 target x 256 sources; 3001: 2 x 128; 5003: 4 x 64; 16500: 16 x 64),
+    with ragged last tiles and workgroups, test particles and ignored terms."""
+    st = orc.plummer(n, 7)[0]
+    f = orc.gravity_basic if grav == "basic" else orc.gravity_compensated
+    for N_active, tp, ign in ((n, 0, 0), (n - 499, 1, 1), (n * 3 // 4, 1, 2)):
+        sim = make_sim(gpu, st, 0.005, "exact")
+        sim.s.gravity = gpu.GRAVITY[grav]
+        sim.s.N_active = N_active
+        sim.s.testparticle_type = tp
+        sim.s.gravity_ignore_terms = ign
+        acc = sim.update_acceleration()
+        ref = f(st, soft=0.005, N_active=N_active, tp_type=tp, ignore=ign)
+        assert np.array_equal(acc, ref), (grav, N_active, tp, ign)
+
+
 def test_exact_bitwise_ghostboxes(gpu):
     st = golden("periodic256_state.npy")
     sim = make_sim(gpu, st, 0.05, "exact")
@@ -122,6 +141,17 @@ def test_energy_exact_bitwise(gpu):
     st = golden("plummer1024_seed1_state.npy")
     sim = make_sim(gpu, st, 0.01, "exact")
     assert sim.energy() == golden("plummer1024_seed1_lf_energy.npy")[0]
+
+
+def test_energy_exact_bitwise_batched(gpu, orc):
+    """N=3001: the pair terms (4.5M) exceed one 4M-term batch of the exact
+    energy, with and without test particles."""
+    st = orc.plummer(3001, 11)[0]
+    for N_active, tp in ((3001, 0), (2500, 0), (2500, 1)):
+        sim = make_sim(gpu, st, 0.0, "exact")
+        sim.s.N_active = N_active
+        sim.s.testparticle_type = tp
+        assert sim.energy() == orc.energy(st, 1.0, N_active, tp), (N_active, tp)
 
 
 def test_energy_parallel(gpu, orc):

@@ -1,5 +1,6 @@
 """GPU parity of the integrators: leapfrog (integrator_leapfrog.c), WHFast
-democratic heliocentric (integrator_whfast.c) and its Kepler solver, SEI
+Jacobi and democratic heliocentric coordinates with the symplectic correctors
+(integrator_whfast.c) and its Kepler solver, SEI
 (integrator_sei.c) with the shear boundary (boundary.c)."""
 import ctypes
 import math
@@ -112,11 +113,63 @@ def test_whfast_dh_bitwise(gpu, safe):
         assert np.array_equal(np.array(E), en[k]), k
 
 
+def _jacobi_cases():
+    init = golden("whfast_jac16_init.npy")
+    nb = golden("whfast_jac16_nbodies.npy")
+    dts = golden("whfast_jac16_dt.npy")
+    first = np.concatenate([[0], np.cumsum(nb)[:-1]])
+    return [(init[f:f + n], float(dt)) for f, n, dt in zip(first, nb, dts)]
+
+
+@pytest.mark.parametrize("mode", range(8))
+def test_whfast_jacobi_bitwise(gpu, mode):
+    """Jacobi coordinates (the reference's default) with the symplectic
+    correctors, COMPENSATED gravity and test particles, per the mode table of
+    the fixtures: bit-identical bodies and energies after three
+    reb_simulation_steps calls of 100 steps."""
+    safe, c1, c2, comp, off, tp = (int(v) for v in golden("whfast_jac16_modes.npy")[mode])
+    fin = golden("whfast_jac16_mode%d_final.npy" % mode)
+    en = golden("whfast_jac16_mode%d_energy.npy" % mode)
+    row = 0
+    for k, (s0, dt) in enumerate(_jacobi_cases()):
+        n = s0.shape[0]
+        sim = sim_from(gpu, s0, dt=dt)
+        wh = sim.set_integrator("whfast")
+        assert wh.coordinates == gpu.WHFAST_COORDINATES["jacobi"]
+        wh.safe_mode = safe
+        wh.corrector = c1
+        wh.corrector2 = c2
+        sim.s.gravity = gpu.GRAVITY["compensated" if comp else "basic"]
+        if off and n > off:
+            sim.s.N_active = n - off
+        sim.s.testparticle_type = tp
+        sim.precision = "exact"
+        E = [sim.energy()]
+        for _ in range(3):
+            sim.steps(100)
+            E.append(sim.energy())
+        assert np.array_equal(sim.state()[:, :6], fin[row:row + n, :6]), (mode, k)
+        assert np.array_equal(np.array(E), en[k]), (mode, k)
+        row += n
+
+
 def test_whfast_rejects_unsupported(gpu):
     s0, dt = _systems()[0]
     sim = sim_from(gpu, s0, dt=dt)
-    sim.set_integrator("whfast")      # Jacobi by default
-    with pytest.raises(gpu.RebError, match="democratic heliocentric"):
+    wh = sim.set_integrator("whfast")
+    wh.coordinates = gpu.WHFAST_COORDINATES["whds"]
+    with pytest.raises(gpu.RebError, match="Jacobi and democratic heliocentric"):
+        sim.steps(1)
+    sim = sim_from(gpu, s0, dt=dt)
+    wh = sim.set_integrator("whfast")
+    wh.corrector = 4
+    with pytest.raises(gpu.RebError, match="following orders"):
+        sim.steps(1)
+    sim = sim_from(gpu, s0, dt=dt)
+    wh = sim.set_integrator("whfast")
+    wh.coordinates = gpu.WHFAST_COORDINATES["democraticheliocentric"]
+    wh.corrector = 3
+    with pytest.raises(gpu.RebError, match="only compatible with Jacobi"):
         sim.steps(1)
 
 

@@ -139,6 +139,23 @@ def test_sheet_five_steps_bitwise(gpu, record):
     assert sim.s.collisions_plog == plog[0] and sim.s.collisions_log_n == int(plog[1])
 
 
+@pytest.mark.parametrize("kind,steps", [("line", 1), ("linetree", 3)])
+def test_sheet_line_searches_bitwise(gpu, kind, steps):
+    """LINE (collision.c:125-196) and LINETREE (:270-331): the swept-sphere
+    searches give the reference's collision lists in resolve order and its
+    bodies after every step."""
+    sim, rec = _run_steps(gpu, kind, True)
+    counts = golden("sheet2000_%s_counts.npy" % kind)
+    for s in range(steps):
+        sim.steps(1)
+        ref = golden("sheet2000_%s_step%d_state.npy" % (kind, s + 1))
+        assert np.array_equal(sim.state()[:, :6], ref[:, :6]), s
+    got = np.array(rec)
+    assert got.shape[0] == counts.sum()
+    assert np.array_equal(got[:, :2].astype(np.int64), golden("sheet2000_%s_pairs.npy" % kind))
+    assert np.array_equal(got[:, 2:], golden("sheet2000_%s_gb.npy" % kind))
+
+
 def test_sheet_direct_search_bitwise(gpu):
     sim, rec = _run_steps(gpu, "direct", True)
     sim.steps(1)

@@ -190,6 +190,47 @@ def test_whfast_dh_bitwise(orc, safe):
         assert np.array_equal(np.array(E), en[k]), k
 
 
+@pytest.mark.parametrize("mode", range(8))
+def test_whfast_jacobi_bitwise(orc, mode):
+    """The Jacobi restatement with correctors, COMPENSATED gravity and test
+    particles reproduces the reference's bodies and energies bit for bit."""
+    safe, c1, c2, comp, off, tp = (int(v) for v in golden("whfast_jac16_modes.npy")[mode])
+    init = golden("whfast_jac16_init.npy")
+    nb = golden("whfast_jac16_nbodies.npy")
+    dts = golden("whfast_jac16_dt.npy")
+    fin = golden("whfast_jac16_mode%d_final.npy" % mode)
+    en = golden("whfast_jac16_mode%d_energy.npy" % mode)
+    row = 0
+    for k, n in enumerate(nb):
+        n = int(n)
+        s = init[row:row + n].copy()
+        NA = n - off if off and n > off else n
+        pj, sync = None, 1
+        E = [orc.energy(s, 1.0, NA, tp)]
+        for c in range(3):
+            s, pj, sync = orc.whfast_jacobi(s, 1.0, float(dts[k]), 100, safe, corrector=c1, corrector2=c2,
+                                            N_active=NA, tp_type=tp, compensated=comp, pj=pj, is_sync=sync,
+                                            did_modify=1 if c == 0 else 0)
+            E.append(orc.energy(s, 1.0, NA, tp))
+        assert np.array_equal(s[:, :6], fin[row:row + n, :6]), (mode, k)
+        assert np.array_equal(np.array(E), en[k]), (mode, k)
+        row += n
+
+
+def test_whfast_jacobi_corrector_improves_energy(orc):
+    """Independent of the fixtures: on a two-planet system the 11th-order
+    corrector brings the energy error of the same run closer to zero than the
+    plain kernel (Wisdom, Holman & Touma 1996), and the two runs differ."""
+    nb = golden("whfast_jac16_nbodies.npy")
+    assert nb[0] == 2 and nb[1] == 3
+    s0 = golden("whfast_jac16_init.npy")[2:5]      # system 1: the star and two planets
+    errs = []
+    for c1 in (0, 11):
+        s, _, _ = orc.whfast_jacobi(s0, 1.0, 0.05, 2000, 0, corrector=c1)
+        errs.append(abs((orc.energy(s) - orc.energy(s0)) / orc.energy(s0)))
+    assert errs[1] < 0.5 * errs[0], errs
+
+
 # ------------------------------------------------------------------ tree and collisions
 SHEET = dict(root_size=50.0, nr=(2, 2, 1))
 
@@ -248,6 +289,44 @@ def test_collision_direct_first_step(orc):
     # tree search finds the same overlapping, approaching pairs as the direct search
     assert set(map(tuple, pairs)) == set(map(tuple, pairs_t))
     assert len(pairs) > 100
+
+
+@pytest.mark.parametrize("kind", ["line", "linetree"])
+def test_collision_line_search_bitwise(orc, kind):
+    """LINE and LINETREE on the reference's own search input of a sheet step:
+    after the shuffle, the pairs and ghost boxes equal the reference's resolve
+    order bit for bit."""
+    import ctypes
+    from oracle import lib
+    s = golden("sheet2000_%s_search_state.npy" % kind)
+    t, dt = golden("sheet2000_%s_search_t_dt.npy" % kind)
+    gbs = _sheet_gbs(t, ng=1, six=True)
+    if kind == "line":
+        pairs, gbo = orc.collisions_line(s, gbs, dt)
+    else:
+        pairs, gbo = orc.collisions_linetree(s, SHEET["root_size"], SHEET["nr"], gbs, dt)
+    seed = ctypes.c_uint(int(golden("sheet2000_%s_seed_before.npy" % kind)[0]))
+    lib().orc_collision_shuffle(pairs, gbo, len(pairs), ctypes.byref(seed))
+    n0 = int(golden("sheet2000_%s_counts.npy" % kind)[0])
+    assert len(pairs) == n0 > 100
+    assert np.array_equal(pairs, golden("sheet2000_%s_pairs.npy" % kind)[:n0])
+    assert np.array_equal(gbo, golden("sheet2000_%s_gb.npy" % kind)[:n0])
+
+
+def test_collision_linetree_within_line(orc):
+    """Independent of the fixtures: every LINETREE pair of the central box is
+    a LINE pair in one of its two orders (the tree only prunes), and LINE
+    finds each pair once with j > i."""
+    s = golden("sheet2000_line_search_state.npy")
+    t, dt = golden("sheet2000_line_search_t_dt.npy")
+    g0 = np.zeros((1, 6))
+    lp, _ = orc.collisions_line(s, g0, dt)
+    tp, _ = orc.collisions_linetree(s, SHEET["root_size"], SHEET["nr"], g0, dt)
+    assert np.all(lp[:, 1] > lp[:, 0])
+    line_set = set(map(tuple, lp)) | set(map(tuple, lp[:, ::-1]))
+    tree_set = set(map(tuple, tp))
+    assert tree_set <= line_set
+    assert len(tree_set) > 0.9 * len(line_set)
 
 
 def test_whfast_flop_count(orc, amd):
