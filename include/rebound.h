@@ -293,6 +293,14 @@ struct reb_simulation {
 REB_API struct reb_simulation* reb_simulation_create(void);
 REB_API void reb_simulation_free(struct reb_simulation* const r);
 REB_API struct reb_simulation* reb_simulation_copy(struct reb_simulation* r);
+/* Binary files and simulation archives in REBOUND's format (rebound.h:425,
+ * :466-472): a new file holds the simulation; an existing archive gets a
+ * snapshot appended.  snapshot counts from 0; negative values from the end. */
+REB_API struct reb_simulation* reb_simulation_create_from_file(char* filename, int64_t snapshot);
+REB_API void reb_simulation_save_to_file(struct reb_simulation* r, const char* filename);
+REB_API void reb_simulation_save_to_file_interval(struct reb_simulation* const r, const char* filename, double interval);
+REB_API void reb_simulation_save_to_file_walltime(struct reb_simulation* const r, const char* filename, double walltime);
+REB_API void reb_simulation_save_to_file_step(struct reb_simulation* const r, const char* filename, uint64_t step);
 REB_API size_t reb_simulation_struct_size(void);                     /* simulation.c:65-67 */
 
 /* Build, version and hash strings (rebound.c:41-43); the reference's Python

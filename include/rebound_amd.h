@@ -56,6 +56,40 @@ REB_API void reb_amd_set_profiling(struct reb_simulation* r, int on);
  * since profiling was switched on or last read. */
 REB_API int reb_amd_kernel_time(struct reb_simulation* r, double* total_ms, int* launches);
 
+/* ------------------------------------------------------------ device forces
+ * The reference's additional_forces callback (rebound.h:403, called at the end
+ * of reb_simulation_update_acceleration, simulation.c:640-689) runs on the
+ * host, so every force evaluation would move the bodies to the host and back.
+ * A device force is the same hook without that round trip: the callback gets
+ * device pointers to the bodies (structure of arrays) and the simulation's HIP
+ * stream, and enqueues its own kernels there, adding to ax, ay, az.  It runs
+ * after gravity and before additional_forces when both are set.  No
+ * counterpart in the reference; for callers whose extra forces are GPU code
+ * (REBOUNDx-style physics). */
+struct reb_amd_device_bodies {
+    double* x;
+    double* y;
+    double* z;
+    double* vx;
+    double* vy;
+    double* vz;
+    double* ax;
+    double* ay;
+    double* az;
+    double* m;
+    double* r;
+    size_t N;
+    void* stream;      /* hipStream_t of the simulation: enqueue work here, do not synchronize */
+};
+typedef void (*reb_amd_device_forces_fn)(struct reb_simulation* r, const struct reb_amd_device_bodies* b,
+                                         void* user);
+/* fn == NULL removes the device force.  Not supported with WHFast (its
+ * interaction runs inside one kernel). */
+REB_API void reb_amd_set_device_forces(struct reb_simulation* r, reb_amd_device_forces_fn fn, void* user);
+/* A built-in device force: linear drag a -= gamma v on every body, with
+ * `user` pointing to gamma (a double that must outlive the simulation's use). */
+REB_API void reb_amd_device_force_drag(struct reb_simulation* r, const struct reb_amd_device_bodies* b, void* user);
+
 /* ------------------------------------------------------------ collisions
  * Sets r->coefficient_of_restitution to a C callback that returns `eps` for
  * every impact velocity (the reference's callback, integrator-agnostic, with

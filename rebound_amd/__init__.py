@@ -62,12 +62,26 @@ def device_name():
 class Sim:
     """Owns one `struct reb_simulation*` of the library."""
 
-    def __init__(self, save_messages=True):
+    def __init__(self, save_messages=True, _ptr=None):
         self.lib = lib()
-        self.ptr = self.lib.reb_simulation_create()
+        self.ptr = _ptr if _ptr is not None else self.lib.reb_simulation_create()
         self._keep = []          # ctypes callbacks must outlive the simulation
         if save_messages:
             self.s.save_messages = 1
+
+    @classmethod
+    def from_file(cls, path, snapshot=0):
+        """reb_simulation_create_from_file: a snapshot of a binary file or
+        simulation archive (negative snapshots count from the end)."""
+        ptr = lib().reb_simulation_create_from_file(os.fsencode(path), int(snapshot))
+        if not ptr:
+            raise RebError("cannot read %s (%s)" % (path, lib().reb_amd_last_error().decode()))
+        return cls(save_messages=True, _ptr=ptr)
+
+    def save(self, path):
+        """reb_simulation_save_to_file: a new file, or a snapshot appended to an archive."""
+        self.lib.reb_simulation_save_to_file(self.ptr, os.fsencode(path))
+        self.check()
 
     # ------------------------------------------------------------ plumbing
     @property

@@ -126,6 +126,11 @@ def declare(lib):
         "reb_simulation_create": (S, []),
         "reb_simulation_free": (None, [S]),
         "reb_simulation_copy": (S, [S]),
+        "reb_simulation_create_from_file": (S, [c_char_p, c_int64]),
+        "reb_simulation_save_to_file": (None, [S, c_char_p]),
+        "reb_simulation_save_to_file_interval": (None, [S, c_char_p, c_double]),
+        "reb_simulation_save_to_file_walltime": (None, [S, c_char_p, c_double]),
+        "reb_simulation_save_to_file_step": (None, [S, c_char_p, ctypes.c_uint64]),
         "reb_simulation_struct_size": (c_size_t, []),
         "reb_simulation_set_integrator": (c_void_p, [S, c_char_p]),
         "reb_simulation_add": (None, [S, Particle]),
@@ -178,6 +183,7 @@ def declare(lib):
         "reb_amd_ensemble_create": (c_void_p, [c_size_t, POINTER(c_size_t), POINTER(Particle), POINTER(c_double),
                                                c_double, c_int, c_int]),
         "reb_amd_ensemble_free": (None, [c_void_p]),
+        "reb_amd_set_device_forces": (None, [S, c_void_p, c_void_p]),
         "reb_amd_ensemble_set_whfast": (c_int, [c_void_p, c_int, ctypes.c_uint, ctypes.c_uint]),
         "reb_amd_ensemble_steps": (c_int, [c_void_p, ctypes.c_uint64]),
         "reb_amd_ensemble_wait": (c_int, [c_void_p]),

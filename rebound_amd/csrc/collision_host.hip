@@ -48,6 +48,7 @@ class HostPool {
         {
             std::lock_guard<std::mutex> g(mu_);
             fn_ = &fn;
+            all_ = nullptr;
             n_ = n;
             next_.store(0);
             busy_ = (int)threads_.size();
@@ -59,12 +60,62 @@ class HostPool {
         done_.wait(lk, [&] { return busy_ == 0; });
         fn_ = nullptr;
     }
+    // Runs fn(t) once on every thread of the pool, t = 0 .. size()-1 (0 is
+    // the calling thread), and returns when all have finished.
+    void run_all(const std::function<void(int)>& fn) {
+        if (threads_.empty()) {
+            fn(0);
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            fn_ = nullptr;
+            all_ = &fn;
+            busy_ = (int)threads_.size();
+            gen_++;
+        }
+        cv_.notify_all();
+        fn(0);
+        std::unique_lock<std::mutex> lk(mu_);
+        done_.wait(lk, [&] { return busy_ == 0; });
+        all_ = nullptr;
+    }
+    // fn(order[q]) for the items q of level 1, then of level 2, ... (level l
+    // holds q in [start[l], start[l+1])): items of one level in parallel, a
+    // barrier between levels, all levels in one dispatch of the pool.
+    void run_levels(int nlev, const size_t* start, const size_t* order, const std::function<void(size_t)>& fn) {
+        const size_t total = start[nlev + 1] - start[1];
+        if (threads_.empty() || total < 512) {
+            for (size_t q = start[1]; q < start[nlev + 1]; q++) fn(order[q]);
+            return;
+        }
+        const int P = size();
+        std::vector<std::atomic<size_t>> next(nlev + 2);
+        for (int l = 1; l <= nlev; l++) next[l].store(start[l]);
+        std::atomic<long> arrived{0};
+        run_all([&](int) {
+            for (int l = 1; l <= nlev; l++) {
+                const size_t e = start[l + 1];
+                const size_t chunk = (e - start[l]) / (size_t)(4 * P) + 1;
+                for (;;) {
+                    const size_t b = next[l].fetch_add(chunk);
+                    if (b >= e) break;
+                    const size_t end = std::min(e, b + chunk);
+                    for (size_t q = b; q < end; q++) fn(order[q]);
+                }
+                // barrier: every thread has finished level l
+                arrived.fetch_add(1);
+                const long want = (long)l * P;
+                while (arrived.load(std::memory_order_acquire) < want) std::this_thread::yield();
+            }
+        });
+    }
 
   private:
     HostPool() {
         unsigned hw = std::thread::hardware_concurrency();
         int k = (int)std::min<unsigned>(hw ? hw : 1, 16u) - 1;
-        for (int t = 0; t < k; t++) threads_.emplace_back([this] { loop(); });
+        for (int t = 0; t < k; t++) threads_.emplace_back([this, t] { loop(t + 1); });
     }
     ~HostPool() {
         {
@@ -84,16 +135,21 @@ class HostPool {
             for (size_t i = b; i < e; i++) (*fn_)(i);
         }
     }
-    void loop() {
+    void loop(int index) {
         unsigned long long seen = 0;
         for (;;) {
+            const std::function<void(int)>* all = nullptr;
             {
                 std::unique_lock<std::mutex> lk(mu_);
                 cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
                 if (stop_) return;
                 seen = gen_;
+                all = all_;
             }
-            work();
+            if (all)
+                (*all)(index);
+            else
+                work();
             std::lock_guard<std::mutex> g(mu_);
             if (--busy_ == 0) done_.notify_all();
         }
@@ -102,6 +158,7 @@ class HostPool {
     std::mutex mu_;
     std::condition_variable cv_, done_;
     const std::function<void(size_t)>* fn_ = nullptr;
+    const std::function<void(int)>* all_ = nullptr;
     size_t n_ = 0;
     std::atomic<size_t> next_{0};
     int busy_ = 0;
@@ -225,9 +282,12 @@ static int rand_r_step(unsigned int* seed) {
 // Host arrays of the compact resolve, reused from step to step (sized to the
 // largest step so far); `loc` is all -1 between calls.
 struct rbamd::ResolveScratch {
-    std::vector<int> loc, ids, level, last;
+    struct Pair {
+        int p1, p2, gb;   // compact body indices, ghost-box index
+    };
+    std::vector<int> loc, ids, level, last, perm;
     std::vector<size_t> start, order, fill;
-    std::vector<struct reb_collision> cl;
+    std::vector<Pair> cl;
     std::vector<struct reb_particle> P;
     std::vector<double> plog;
     std::vector<unsigned char> hit;
@@ -248,22 +308,35 @@ static void resolve_hardsphere_compact(struct reb_simulation* r) {
     DeviceState* d = dev_of(r);
     if (!d->resolve) d->resolve = new ResolveScratch();
     ResolveScratch& W = *d->resolve;
-    const size_t nc = r->N_collisions;
+    const CollList& L = *d->coll;
+    const size_t nc = L.n;
+    // shuffle (collision.c:336-342) as a permutation of the detection order:
+    // the same swaps applied to indices instead of 72-byte records
+    std::vector<int>& perm = W.perm;
+    perm.resize(nc);
+    for (size_t k = 0; k < nc; k++) perm[k] = (int)k;
+    for (size_t i = 0; i < nc; i++) {
+        const size_t nw = (size_t)rand_r_step(&r->rand_seed) % nc;
+        std::swap(perm[i], perm[nw]);
+    }
     if (W.loc.size() < r->N) W.loc.resize(r->N, -1);
     std::vector<int>& loc = W.loc;
     std::vector<int>& ids = W.ids;
     ids.clear();
-    std::vector<struct reb_collision>& cl = W.cl;
-    cl.assign(r->collisions, r->collisions + nc);
-    for (auto& c : cl) {
-        for (size_t* p : {&c.p1, &c.p2}) {
-            int& l = loc[*p];
+    auto& cl = W.cl;
+    cl.resize(nc);
+    for (size_t k = 0; k < nc; k++) {
+        const CollRecH& h = L.rec[perm[k]];
+        int c2[2] = {h.p1, h.p2};
+        for (int& p : c2) {
+            int& l = loc[p];
             if (l < 0) {
                 l = (int)ids.size();
-                ids.push_back((int)*p);
+                ids.push_back(p);
             }
-            *p = (size_t)l;
+            p = l;
         }
+        cl[k] = ResolveScratch::Pair{c2[0], c2[1], h.gb};
     }
     const int n = (int)ids.size();
     for (int k = 0; k < n; k++) loc[ids[k]] = -1;   // ready for the next call
@@ -325,13 +398,14 @@ static void resolve_hardsphere_compact(struct reb_simulation* r) {
     std::vector<unsigned char>& hit = W.hit;
     plog.assign(nc, 0.);
     hit.assign(nc, 0);
-    for (int l = 1; l <= nlev; l++) {
-        const size_t b = start[l], e = start[l + 1];
-        HostPool::get().run(e - b, [&](size_t q) {
-            const size_t k = order[b + q];
-            hit[k] = (unsigned char)hardsphere_apply(r, P.data(), cl[k], &plog[k]);
-        });
-    }
+    HostPool::get().run_levels(nlev, start.data(), order.data(), [&](size_t k) {
+        struct reb_collision c;
+        c.p1 = (size_t)cl[k].p1;
+        c.p2 = (size_t)cl[k].p2;
+        c.gb = L.gb[cl[k].gb];
+        c.ri = 0;
+        hit[k] = (unsigned char)hardsphere_apply(r, P.data(), c, &plog[k]);
+    });
     // the pressure log in list order (collision.c:655-663)
     for (size_t k = 0; k < nc; k++) {
         if (!hit[k]) continue;
@@ -347,28 +421,52 @@ static void resolve_hardsphere_compact(struct reb_simulation* r) {
     hipLaunchKernelGGL(k_scatter_v, dim3((n + 255) / 256), dim3(256), 0, d->stream, d->soa.as<double>(), d->cap,
                        dids, n, ddata);
     (void)RB_HIP(r, hipGetLastError());
+    // r->collisions as the reference leaves it: the shuffled list
+    if (r->N_allocated_collisions < nc) {
+        size_t cap = r->N_allocated_collisions ? r->N_allocated_collisions : 32;
+        while (cap < nc) cap *= 2;
+        auto* pc = (struct reb_collision*)realloc(r->collisions, sizeof(struct reb_collision) * cap);
+        if (!pc) {
+            reb_simulation_error(r, "Out of memory for the collision list.");
+            return;
+        }
+        r->collisions = pc;
+        r->N_allocated_collisions = cap;
+    }
+    HostPool::get().run(nc, [&](size_t k) {
+        const CollRecH& hr = L.rec[perm[k]];
+        struct reb_collision c;
+        c.p1 = (size_t)hr.p1;
+        c.p2 = (size_t)hr.p2;
+        c.gb = L.gb[hr.gb];
+        c.ri = (size_t)hr.ri;
+        r->collisions[k] = c;
+    });
+    d->coll->pending = false;
     (void)stream_sync(r);   // the staging buffer is reused by the next transfer
     mark_device_newer(r);
 }
 
 void collision_resolve_host(struct reb_simulation* r) {
+    if (r->N_collisions == 0) return;
+    enum REB_COLLISION_RESOLVE_OUTCOME (*resolve)(struct reb_simulation* const, struct reb_collision) =
+        r->collision_resolve;
+    if (resolve == nullptr) resolve = reb_collision_resolve_halt;
+    DeviceState* d = dev_of(r);
+    if (resolve == reb_collision_resolve_hardsphere &&
+        (r->coefficient_of_restitution == nullptr ||
+         r->coefficient_of_restitution == reb_amd_coefficient_of_restitution_constant) &&
+        d->loc != LOC_HOST && d->coll && d->coll->pending && d->coll->n == r->N_collisions) {
+        resolve_hardsphere_compact(r);   // shuffles, resolves and writes r->collisions
+        return;
+    }
+    if (!materialize_collisions(r)) return;
     // shuffle (collision.c:336-342)
     for (size_t i = 0; i < r->N_collisions; i++) {
         size_t nw = (size_t)rand_r_step(&r->rand_seed) % r->N_collisions;
         struct reb_collision c1 = r->collisions[i];
         r->collisions[i] = r->collisions[nw];
         r->collisions[nw] = c1;
-    }
-    if (r->N_collisions == 0) return;
-    enum REB_COLLISION_RESOLVE_OUTCOME (*resolve)(struct reb_simulation* const, struct reb_collision) =
-        r->collision_resolve;
-    if (resolve == nullptr) resolve = reb_collision_resolve_halt;
-    if (resolve == reb_collision_resolve_hardsphere &&
-        (r->coefficient_of_restitution == nullptr ||
-         r->coefficient_of_restitution == reb_amd_coefficient_of_restitution_constant) &&
-        dev_of(r)->loc != LOC_HOST) {
-        resolve_hardsphere_compact(r);
-        return;
     }
     // The built-in hard-sphere and halt resolvers touch only p1 and p2: move
     // just those bodies.  Anything else sees the whole host state.

@@ -199,6 +199,7 @@ void device_release(struct reb_simulation* r) {
     whfast_release(r);
     tree_release(r);
     resolve_release(r);
+    coll_release(r);
     d->soa.release();
     d->partial.release();
     d->scratch.release();

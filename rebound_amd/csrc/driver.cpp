@@ -58,6 +58,15 @@ void update_acceleration_internal(struct reb_simulation* r) {
         reb_simulation_error(r, "Variational particles are not supported by this engine.");
         return;
     }
+    DeviceState* d = dev_of(r);
+    if (d->dev_forces) {
+        // the callback enqueues on the simulation's stream: no host round trip
+        const struct reb_amd_device_bodies b = {d->field(FX), d->field(FY), d->field(FZ), d->field(FVX),
+                                                d->field(FVY), d->field(FVZ), d->field(FAX), d->field(FAY),
+                                                d->field(FAZ), d->field(FM), d->field(FR), r->N, (void*)d->stream};
+        d->dev_forces(r, &b, d->dev_forces_user);
+        mark_device_newer(r);
+    }
     if (r->additional_forces) run_host_callback(r, r->additional_forces);
 }
 
@@ -218,10 +227,12 @@ REB_API void reb_simulation_steps(struct reb_simulation* const r, size_t N_steps
     begin_call(r);
     run_heartbeat(r);
     for (size_t i = 0; i < N_steps; i++) {
+        if (r->simulationarchive_filename) archive_heartbeat(r);
         reb_simulation_step(r);
         run_heartbeat(r);
     }
     reb_simulation_synchronize(r);
+    if (r->simulationarchive_filename) archive_heartbeat(r);
     end_call(r);
 }
 
@@ -248,6 +259,7 @@ REB_API enum REB_STATUS reb_simulation_integrate(struct reb_simulation* const r,
     if (r->status != REB_STATUS_PAUSED && r->status != REB_STATUS_SCREENSHOT) r->status = REB_STATUS_RUNNING;
     run_heartbeat(r);
     while (reb_check_exit(r, tmax, &last_full_dt) < 0) {
+        if (r->simulationarchive_filename) archive_heartbeat(r);
         reb_simulation_step(r);
         run_heartbeat(r);
         if (g_sigint) r->status = REB_STATUS_SIGINT;
@@ -255,6 +267,7 @@ REB_API enum REB_STATUS reb_simulation_integrate(struct reb_simulation* const r,
     }
     reb_simulation_synchronize(r);
     if (r->exact_finish_time == 1) r->dt = last_full_dt;
+    if (r->simulationarchive_filename) archive_heartbeat(r);
     signal(SIGINT, old);
     end_call(r);
     return r->status;

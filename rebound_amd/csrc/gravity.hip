@@ -516,14 +516,17 @@ __global__ __launch_bounds__(256) void k_seg_reduce(const SegJob* __restrict__ j
 }
 
 // Number of source chunks for the fast modes: enough workgroups to fill every
-// SIMD of the 256-CU part several times over, chunks no shorter than one LDS
-// tile (short chunks keep the time of one step small for N of a few
-// thousand, where a single workgroup's chunk sets the kernel's duration).
+// SIMD of the 256-CU part several times over, chunks no shorter than
+// kMinChunk sources (short chunks keep the time of one step small for N of a
+// few thousand, where a single workgroup's chunk sets the kernel's duration).
+static constexpr int kMinChunk = 32;
+static_assert(kFast % kMinChunk == 0, "chunks tile the LDS tile");
+static int round_chunk(int c) { return ((c + kMinChunk - 1) / kMinChunk) * kMinChunk; }
 static int pick_chunks(int n_recv, int n_src, int ngb) {
     const int blocks_x = (n_recv + kFast - 1) / kFast;
     const int target_blocks = 2048 * 256 / kFast;
     int s = (target_blocks + blocks_x - 1) / blocks_x;
-    int max_s = (n_src * ngb) / kFast;
+    int max_s = (n_src * ngb) / kMinChunk;
     if (max_s < 1) max_s = 1;
     if (s > max_s) s = max_s;
     if (s > 64) s = 64;
@@ -536,8 +539,7 @@ static size_t workspace_bytes(int n, int nchunk) { return sizeof(double) * 6 * (
 // The chunking a simulation of n bodies uses when stepped alone (launch_fast).
 void seg_plan(int n, int* chunk, int* used, size_t* workspace) {
     const int nchunk = pick_chunks(n, n, 1);
-    int c = (n + nchunk - 1) / nchunk;
-    c = ((c + kFast - 1) / kFast) * kFast;
+    const int c = round_chunk((n + nchunk - 1) / nchunk);
     *chunk = c;
     *used = n > 0 ? (n + c - 1) / c : 1;
     *workspace = workspace_bytes(n, nchunk) + 256;
@@ -564,8 +566,7 @@ static int launch_fast(int mode, const double* x, const double* y, const double*
                        double* partial, int nchunk, int accumulate, double* ax, double* ay, double* az,
                        hipStream_t st, hipEvent_t* ev, const KickDrift& kd) {
     const int nsrc = j_end - j_begin;
-    int chunk = (nsrc + nchunk - 1) / nchunk;
-    chunk = ((chunk + kFast - 1) / kFast) * kFast;
+    const int chunk = round_chunk((nsrc + nchunk - 1) / nchunk);
     const int used = nsrc > 0 ? (nsrc + chunk - 1) / chunk : 1;
     dim3 grid((n_recv + kFast - 1) / kFast, used);
     if (ev) (void)hipEventRecord(ev[0], st);

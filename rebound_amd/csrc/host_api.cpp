@@ -278,6 +278,8 @@ REB_API struct reb_simulation* reb_simulation_copy(struct reb_simulation* r) {
     }
     dev_of(c)->precision = dev_of(r)->precision;
     dev_of(c)->cor_constant = dev_of(r)->cor_constant;
+    dev_of(c)->dev_forces = dev_of(r)->dev_forces;
+    dev_of(c)->dev_forces_user = dev_of(r)->dev_forces_user;
     c->did_modify_particles = 1;
     return c;
 }
@@ -292,6 +294,11 @@ REB_API double reb_amd_coefficient_of_restitution_constant(const struct reb_simu
 REB_API void reb_amd_set_coefficient_of_restitution(struct reb_simulation* r, double eps) {
     dev_of(r)->cor_constant = eps;
     r->coefficient_of_restitution = reb_amd_coefficient_of_restitution_constant;
+}
+
+REB_API void reb_amd_set_device_forces(struct reb_simulation* r, reb_amd_device_forces_fn fn, void* user) {
+    dev_of(r)->dev_forces = fn;
+    dev_of(r)->dev_forces_user = user;
 }
 
 // ===================================================================== particles

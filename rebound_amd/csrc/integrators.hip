@@ -321,7 +321,7 @@ void rb_leapfrog_step(struct reb_simulation* r, void* state) {
     // the closing drift run in the pass that finishes the accelerations (same
     // arithmetic, two launches fewer per step).
     const bool fuse = (r->gravity == REB_GRAVITY_BASIC || r->gravity == REB_GRAVITY_COMPENSATED) &&
-                      !r->additional_forces && !r->N_var && r->N > 0;
+                      !r->additional_forces && !dev_of(r)->dev_forces && !r->N_var && r->N > 0;
     switch (lf->order) {
         case 2:
             D(dt * 0.5);
@@ -467,3 +467,28 @@ bool exit_max_distance_device(struct reb_simulation* r, double max2) { return fl
 bool exit_min_distance_device(struct reb_simulation* r, double min2) { return flag_kernel(r, true, min2); }
 
 }  // namespace rbamd
+
+namespace rbamd {
+__global__ void k_drag(const double* __restrict__ vx, const double* __restrict__ vy, const double* __restrict__ vz,
+                       double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az, double gamma,
+                       int N) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    ax[i] -= gamma * vx[i];
+    ay[i] -= gamma * vy[i];
+    az[i] -= gamma * vz[i];
+}
+}  // namespace rbamd
+
+extern "C" REB_API void reb_amd_device_force_drag(struct reb_simulation* r, const struct reb_amd_device_bodies* b,
+                                                  void* user) {
+    using namespace rbamd;
+    if (!user) {
+        reb_simulation_error(r, "reb_amd_device_force_drag needs a pointer to gamma.");
+        return;
+    }
+    if (b->N == 0) return;
+    const double gamma = *static_cast<const double*>(user);
+    hipLaunchKernelGGL(k_drag, grid_for(b->N), dim3(256), 0, (hipStream_t)b->stream, b->vx, b->vy, b->vz, b->ax,
+                       b->ay, b->az, gamma, (int)b->N);
+}

@@ -69,6 +69,20 @@ enum Loc { LOC_HOST = 0, LOC_DEVICE = 1, LOC_BOTH = 2 };
 
 struct TreeBuffers;          // tree.hip
 struct ResolveScratch;       // collision_host.hip
+
+// Collisions of the last device search in detection order, in pinned host
+// memory: bodies, ghost-box index, root box (the device record's layout),
+// and the ghost boxes of that search.  r->collisions is filled from it when
+// the resolve needs the reference's struct list.
+struct CollRecH {
+    int p1, p2, gb, ri;
+};
+struct CollList {
+    CollRecH* rec = nullptr;
+    size_t cap = 0, n = 0;
+    std::vector<reb_vec6d> gb;
+    bool pending = false;     // r->collisions not yet written from rec
+};
 struct WHFastDevice;         // whfast.hip
 
 // Device-side state of one simulation.  Allocated next to the public struct
@@ -104,6 +118,11 @@ struct DeviceState {
     double cor_constant = 1.0;
     // host work arrays of the collision resolve, kept between steps
     ResolveScratch* resolve = nullptr;
+    // collisions of the last device search (pinned host copy)
+    CollList* coll = nullptr;
+    // device-side additional forces (reb_amd_set_device_forces)
+    reb_amd_device_forces_fn dev_forces = nullptr;
+    void* dev_forces_user = nullptr;
     double* field(int f) const { return soa.as<double>() + (size_t)f * cap; }
 };
 
@@ -178,6 +197,12 @@ void whfast_synchronize_device(struct reb_simulation* r, struct reb_integrator_w
 void whfast_release(struct reb_simulation* r);
 void tree_release(struct reb_simulation* r);
 void resolve_release(struct reb_simulation* r);
+// Writes r->collisions from the last device search (detection order) when it
+// is still pending; false on an allocation failure.
+bool materialize_collisions(struct reb_simulation* r);
+void coll_release(struct reb_simulation* r);
+// simulationarchive.c:377-409: automatic snapshots by interval, step or walltime
+void archive_heartbeat(struct reb_simulation* r);
 
 // Host-side driver pieces shared between files
 void collision_resolve_host(struct reb_simulation* r);       // collision.c:336-404

@@ -56,6 +56,7 @@ struct TreeBuffers {
     DevBuf node_i;                                // 4 ints per cell: next pt level start
     DevBuf info;                                  // total, maxlevel, flags
     DevBuf coll_cnt, coll_off, coll_rec;          // collision search
+    DevBuf coll_slot;                             // kCollSlots records per body (single-pass search)
     DevBuf grec, crec;                            // 64-byte walk records per cell (gravity, collisions)
     size_t body_cap = 0, node_cap = 0, rec_cap = 0;
     int nnodes = 0;
@@ -741,12 +742,18 @@ struct CollRec {
     int p1, p2, gb, ri;
 };
 
+// Records kept per body by the single-pass search; a body with more
+// collisions sends the search through a second, writing pass.
+static constexpr int kCollSlots = 8;
+
 // Tree search of collision.c:197-269 + :422-503 (TREE) or :270-331 +
-// :506-566 (LINETREE, LINE = true).  WRITE=false counts the collisions of
-// each body, WRITE=true stores them at the body's offset in detection order
-// (ghost box, root box, depth-first).  TREE passes the second largest radius
-// as `reach`; LINETREE the largest drift of the last step, dt_last_done*|v|max.
-template <bool WRITE, bool LINE = false>
+// :506-566 (LINETREE, LINE = true), in detection order per body (ghost box,
+// root box, depth-first).  MODE 0 counts the collisions of each body; MODE 1
+// stores them at the body's offset; MODE 2 counts them and keeps the first
+// kCollSlots in the body's slots, raising *overflow when a body has more.
+// TREE passes the second largest radius as `reach`; LINETREE the largest
+// drift of the last step, dt_last_done*|v|max.
+template <int MODE, bool LINE = false>
 __global__ __launch_bounds__(64) void k_tree_collide(const CollCell* __restrict__ cells, int total,
                                                      const int* __restrict__ idx, int n,
                                                      const double* __restrict__ x, const double* __restrict__ y,
@@ -754,7 +761,9 @@ __global__ __launch_bounds__(64) void k_tree_collide(const CollCell* __restrict_
                                                      const double* __restrict__ vy, const double* __restrict__ vz,
                                                      const double* __restrict__ rad, GBT gl, double reach,
                                                      int* __restrict__ counts, const int* __restrict__ offsets,
-                                                     CollRec* __restrict__ rec, double dt_last = 0.) {
+                                                     CollRec* __restrict__ rec, double dt_last = 0.,
+                                                     int* __restrict__ overflow = nullptr) {
+    constexpr bool WRITE = MODE == 1;
     const int s = blockIdx.x * 64 + threadIdx.x;
     const bool live = s < n;
     const int me = live ? idx[s] : -1;
@@ -765,6 +774,12 @@ __global__ __launch_bounds__(64) void k_tree_collide(const CollCell* __restrict_
     const double p1_r_plus_dtv = LINE ? p1_r + dt_last * sqrt(pvx * pvx + pvy * pvy + pvz * pvz) : 0.;
     int found = 0;
     int out = (WRITE && live) ? offsets[me] : 0;
+    CollRec* const slot = MODE == 2 && live ? rec + (size_t)me * kCollSlots : nullptr;
+    auto emit = [&](int pt, int g, int ri) {
+        if (MODE == 1) rec[out++] = CollRec{me, pt, g, ri};
+        if (MODE == 2 && found < kCollSlots) slot[found] = CollRec{me, pt, g, ri};
+        found++;
+    };
     for (int g = 0; g < gl.n; g++) {
         const double gx = gl.g[g][0] + px, gy = gl.g[g][1] + py, gz = gl.g[g][2] + pz;
         const double gvx = gl.g[g][3] + pvx, gvy = gl.g[g][4] + pvy, gvz = gl.g[g][5] + pvz;
@@ -787,10 +802,8 @@ __global__ __launch_bounds__(64) void k_tree_collide(const CollCell* __restrict_
                         const double dz = gz - z[pt];
                         if (LINE) {
                             if (line_overlap(dx, dy, dz, gvx - vx[pt], gvy - vy[pt], gvz - vz[pt], dt_last,
-                                             p1_r + rad[pt])) {
-                                if (WRITE) rec[out++] = CollRec{me, pt, g, ri};
-                                found++;
-                            }
+                                             p1_r + rad[pt]))
+                                emit(pt, g, ri);
                         } else {
                             const double r2 = dx * dx + dy * dy + dz * dz;
                             const double rp = p1_r + rad[pt];
@@ -798,10 +811,7 @@ __global__ __launch_bounds__(64) void k_tree_collide(const CollCell* __restrict_
                                 const double dvx = gvx - vx[pt];
                                 const double dvy = gvy - vy[pt];
                                 const double dvz = gvz - vz[pt];
-                                if (!(dvx * dx + dvy * dy + dvz * dz > 0)) {
-                                    if (WRITE) rec[out++] = CollRec{me, pt, g, ri};
-                                    found++;
-                                }
+                                if (!(dvx * dx + dvy * dy + dvz * dz > 0)) emit(pt, g, ri);
                             }
                         }
                     }
@@ -824,6 +834,17 @@ __global__ __launch_bounds__(64) void k_tree_collide(const CollCell* __restrict_
         }
     }
     if (!WRITE && live) counts[me] = found;
+    if (MODE == 2 && live && found > kCollSlots) *overflow = 1;   // same value from every writer
+}
+
+// The slot records of every body to its offset (bodies with at most
+// kCollSlots collisions: the single-pass case).
+__global__ void k_coll_compact(const int* __restrict__ counts, const int* __restrict__ offsets,
+                               const CollRec* __restrict__ slots, int n, CollRec* __restrict__ rec) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = counts[i], o = offsets[i];
+    for (int k = 0; k < c; k++) rec[o + k] = slots[(size_t)i * kCollSlots + k];
 }
 
 // Root box of each record: the reference stores the root box being searched.
@@ -934,33 +955,114 @@ static long scan_counts(struct reb_simulation* r, TreeBuffers* T, int nslots) {
     return total;
 }
 
-// Copies the device records into r->collisions in detection order.
+// The device records of a search into pinned host memory; r->collisions is
+// written from them by materialize_collisions when a resolve needs it.
 static bool fetch_collisions(struct reb_simulation* r, TreeBuffers* T, long total, const GBT& gl) {
     DeviceState* d = dev_of(r);
     r->N_collisions = 0;
+    if (!d->coll) d->coll = new CollList();
+    CollList& L = *d->coll;
+    L.n = 0;
+    L.pending = false;
     if (total <= 0) return true;
-    std::vector<CollRec> h((size_t)total);
-    if (!RB_HIP(r, hipMemcpyAsync(h.data(), T->coll_rec.p, sizeof(CollRec) * total, hipMemcpyDeviceToHost,
-                                  d->stream)))
-        return false;
-    if (!stream_sync(r)) return false;
-    if (r->N_allocated_collisions < (size_t)total) {
-        size_t cap = r->N_allocated_collisions ? r->N_allocated_collisions : 32;
+    if ((size_t)total > L.cap) {
+        size_t cap = L.cap ? L.cap : 4096;
         while (cap < (size_t)total) cap *= 2;
-        r->collisions = (struct reb_collision*)realloc(r->collisions, sizeof(struct reb_collision) * cap);
-        r->N_allocated_collisions = cap;
+        if (L.rec) (void)hipHostFree(L.rec);
+        L.rec = nullptr;
+        L.cap = 0;
+        if (!RB_HIP(r, hipHostMalloc((void**)&L.rec, sizeof(CollRecH) * cap, hipHostMallocDefault))) return false;
+        L.cap = cap;
     }
-    for (long k = 0; k < total; k++) {
-        struct reb_collision c;
-        c.p1 = (size_t)h[k].p1;
-        c.p2 = (size_t)h[k].p2;
-        const double* g = gl.g[h[k].gb];
-        c.gb = reb_vec6d{g[0], g[1], g[2], g[3], g[4], g[5]};
-        c.ri = (size_t)h[k].ri;
-        r->collisions[k] = c;
-    }
+    static_assert(sizeof(CollRecH) == sizeof(CollRec), "host and device collision records");
+    if (!RB_HIP(r, hipMemcpyAsync(L.rec, T->coll_rec.p, sizeof(CollRec) * total, hipMemcpyDeviceToHost, d->stream)))
+        return false;
+    L.gb.resize(gl.n);
+    for (int k = 0; k < gl.n; k++)
+        L.gb[k] = reb_vec6d{gl.g[k][0], gl.g[k][1], gl.g[k][2], gl.g[k][3], gl.g[k][4], gl.g[k][5]};
+    if (!stream_sync(r)) return false;
+    L.n = (size_t)total;
+    L.pending = true;
     r->N_collisions = (size_t)total;
     return true;
+}
+
+bool materialize_collisions(struct reb_simulation* r) {
+    DeviceState* d = dev_of(r);
+    if (!d->coll || !d->coll->pending) return true;
+    CollList& L = *d->coll;
+    if (r->N_allocated_collisions < L.n) {
+        size_t cap = r->N_allocated_collisions ? r->N_allocated_collisions : 32;
+        while (cap < L.n) cap *= 2;
+        auto* p = (struct reb_collision*)realloc(r->collisions, sizeof(struct reb_collision) * cap);
+        if (!p) {
+            reb_simulation_error(r, "Out of memory for the collision list.");
+            return false;
+        }
+        r->collisions = p;
+        r->N_allocated_collisions = cap;
+    }
+    for (size_t k = 0; k < L.n; k++) {
+        const CollRecH& h = L.rec[k];
+        struct reb_collision c;
+        c.p1 = (size_t)h.p1;
+        c.p2 = (size_t)h.p2;
+        c.gb = L.gb[h.gb];
+        c.ri = (size_t)h.ri;
+        r->collisions[k] = c;
+    }
+    L.pending = false;
+    return true;
+}
+
+void coll_release(struct reb_simulation* r) {
+    DeviceState* d = dev_of(r);
+    if (!d->coll) return;
+    if (d->coll->rec) (void)hipHostFree(d->coll->rec);
+    delete d->coll;
+    d->coll = nullptr;
+}
+
+// TREE / LINETREE search into T->coll_rec in detection order; returns the
+// number of collisions or -1.  One pass keeps up to kCollSlots records per
+// body; only when a body has more does a second pass write them all.
+template <bool LINE>
+static long tree_collide_run(struct reb_simulation* r, TreeBuffers* T, int n, const GBT& gl, double reach,
+                             double dt_last) {
+    DeviceState* d = dev_of(r);
+    if (!ensure_coll(r, T, n, 1)) return -1;
+    if (!T->coll_slot.ensure(sizeof(CollRec) * kCollSlots * (size_t)n)) {
+        reb_simulation_error(r, "Out of device memory for the collision search.");
+        return -1;
+    }
+    int* overflow = T->info.as<int>() + 12;
+    if (!RB_HIP(r, hipMemsetAsync(overflow, 0, sizeof(int), d->stream))) return -1;
+    const dim3 grid((n + 63) / 64);
+    hipLaunchKernelGGL((k_tree_collide<2, LINE>), grid, dim3(64), 0, d->stream, T->crec.as<CollCell>(), T->nnodes,
+                       T->idx.as<int>(), n, d->field(FX), d->field(FY), d->field(FZ), d->field(FVX), d->field(FVY),
+                       d->field(FVZ), d->field(FR), gl, reach, T->coll_cnt.as<int>(), nullptr,
+                       T->coll_slot.as<CollRec>(), dt_last, overflow);
+    int hover = 0;
+    if (!RB_HIP(r, hipMemcpyAsync(&hover, overflow, sizeof(int), hipMemcpyDeviceToHost, d->stream))) return -1;
+    const long total = scan_counts(r, T, n);   // synchronizes: hover is current
+    if (total < 0) return -1;
+    if (!ensure_coll(r, T, n, (size_t)total + 1)) return -1;
+    if (total == 0) return 0;
+    if (!hover) {
+        hipLaunchKernelGGL(k_coll_compact, dim3((n + 255) / 256), dim3(256), 0, d->stream, T->coll_cnt.as<int>(),
+                           T->coll_off.as<int>(), T->coll_slot.as<CollRec>(), n, T->coll_rec.as<CollRec>());
+    } else {
+        hipLaunchKernelGGL((k_tree_collide<1, LINE>), grid, dim3(64), 0, d->stream, T->crec.as<CollCell>(),
+                           T->nnodes, T->idx.as<int>(), n, d->field(FX), d->field(FY), d->field(FZ), d->field(FVX),
+                           d->field(FVY), d->field(FVZ), d->field(FR), gl, reach, nullptr, T->coll_off.as<int>(),
+                           T->coll_rec.as<CollRec>(), dt_last, nullptr);
+    }
+    // root index of each record (the lcp buffer is free now: reuse as pos_of)
+    int* pos_of = T->lcp.as<int>();
+    hipLaunchKernelGGL(k_inverse_perm, dim3((n + 255) / 256), dim3(256), 0, d->stream, T->idx.as<int>(), pos_of, n);
+    hipLaunchKernelGGL(k_coll_root, dim3((total + 255) / 256), dim3(256), 0, d->stream, T->coll_rec.as<CollRec>(),
+                       (int)total, T->key0.as<unsigned long long>(), pos_of, T->L1);
+    return total;
 }
 
 bool collision_search_device(struct reb_simulation* r) {
@@ -1028,29 +1130,8 @@ bool collision_search_device(struct reb_simulation* r) {
                 }
             }
             const double second_r = (n >= 2) ? b2 : 0.;
-            if (!ensure_coll(r, T, n, 1)) return false;
-            const dim3 grid((n + 63) / 64);
-            hipLaunchKernelGGL(k_tree_collide<false>, grid, dim3(64), 0, d->stream, T->crec.as<CollCell>(),
-                               T->nnodes, T->idx.as<int>(), n, d->field(FX),
-                               d->field(FY), d->field(FZ), d->field(FVX), d->field(FVY), d->field(FVZ), d->field(FR),
-                               gl, second_r, T->coll_cnt.as<int>(), nullptr, nullptr);
-            const long total = scan_counts(r, T, n);
+            const long total = tree_collide_run<false>(r, T, n, gl, second_r, 0.);
             if (total < 0) return false;
-            if (!ensure_coll(r, T, n, (size_t)total + 1)) return false;
-            if (total > 0) {
-                hipLaunchKernelGGL(k_tree_collide<true>, grid, dim3(64), 0, d->stream, T->crec.as<CollCell>(),
-                                   T->nnodes, T->idx.as<int>(), n, d->field(FX),
-                                   d->field(FY), d->field(FZ), d->field(FVX), d->field(FVY), d->field(FVZ),
-                                   d->field(FR), gl, second_r, nullptr, T->coll_off.as<int>(),
-                                   T->coll_rec.as<CollRec>());
-                // root index of each record (the lcp buffer is free now: reuse as pos_of)
-                int* pos_of = T->lcp.as<int>();
-                hipLaunchKernelGGL(k_inverse_perm, dim3((n + 255) / 256), dim3(256), 0, d->stream, T->idx.as<int>(),
-                                   pos_of, n);
-                hipLaunchKernelGGL(k_coll_root, dim3((total + 255) / 256), dim3(256), 0, d->stream,
-                                   T->coll_rec.as<CollRec>(), (int)total, T->key0.as<unsigned long long>(), pos_of,
-                                   T->L1);
-            }
             return fetch_collisions(r, T, total, gl);
         }
         case REB_COLLISION_LINE: {
@@ -1089,28 +1170,8 @@ bool collision_search_device(struct reb_simulation* r) {
             for (double v : h) vmax2 = vmax2 > v ? vmax2 : v;
             const double maxdrift = dt_last * sqrt(vmax2);
             if (!tree_build(r)) return false;
-            if (!ensure_coll(r, T, n, 1)) return false;
-            const dim3 grid((n + 63) / 64);
-            hipLaunchKernelGGL((k_tree_collide<false, true>), grid, dim3(64), 0, d->stream, T->crec.as<CollCell>(),
-                               T->nnodes, T->idx.as<int>(), n, d->field(FX), d->field(FY), d->field(FZ),
-                               d->field(FVX), d->field(FVY), d->field(FVZ), d->field(FR), gl, maxdrift,
-                               T->coll_cnt.as<int>(), nullptr, nullptr, dt_last);
-            const long total = scan_counts(r, T, n);
+            const long total = tree_collide_run<true>(r, T, n, gl, maxdrift, dt_last);
             if (total < 0) return false;
-            if (!ensure_coll(r, T, n, (size_t)total + 1)) return false;
-            if (total > 0) {
-                hipLaunchKernelGGL((k_tree_collide<true, true>), grid, dim3(64), 0, d->stream,
-                                   T->crec.as<CollCell>(), T->nnodes, T->idx.as<int>(), n, d->field(FX),
-                                   d->field(FY), d->field(FZ), d->field(FVX), d->field(FVY), d->field(FVZ),
-                                   d->field(FR), gl, maxdrift, nullptr, T->coll_off.as<int>(),
-                                   T->coll_rec.as<CollRec>(), dt_last);
-                int* pos_of = T->lcp.as<int>();
-                hipLaunchKernelGGL(k_inverse_perm, dim3((n + 255) / 256), dim3(256), 0, d->stream, T->idx.as<int>(),
-                                   pos_of, n);
-                hipLaunchKernelGGL(k_coll_root, dim3((total + 255) / 256), dim3(256), 0, d->stream,
-                                   T->coll_rec.as<CollRec>(), (int)total, T->key0.as<unsigned long long>(), pos_of,
-                                   T->L1);
-            }
             return fetch_collisions(r, T, total, gl);
         }
         default:
@@ -1123,7 +1184,7 @@ void tree_release(struct reb_simulation* r) {
     if (!d->tree) return;
     TreeBuffers* T = d->tree;
     for (DevBuf* b : {&T->key0, &T->key1, &T->idx, &T->key0b, &T->key1b, &T->idxb, &T->lcp, &T->cnt, &T->base,
-                      &T->cub, &T->node_d, &T->node_i, &T->info, &T->coll_cnt, &T->coll_off, &T->coll_rec,
+                      &T->cub, &T->node_d, &T->node_i, &T->info, &T->coll_cnt, &T->coll_off, &T->coll_rec, &T->coll_slot,
                       &T->grec, &T->crec})
         b->release();
     delete T;

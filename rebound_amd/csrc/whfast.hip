@@ -973,9 +973,9 @@ static bool whfast_check(struct reb_simulation* r, struct reb_integrator_whfast_
         reb_simulation_error(r, "WHFast on this engine evaluates the interaction with BASIC or COMPENSATED gravity.");
         return false;
     }
-    if (r->additional_forces) {
+    if (r->additional_forces || dev_of(r)->dev_forces) {
         // the interaction runs inside the group kernel, which has no host callbacks
-        reb_simulation_error(r, "additional_forces are not supported with WHFast on this engine.");
+        reb_simulation_error(r, "additional_forces and device forces are not supported with WHFast on this engine.");
         return false;
     }
     if (r->N < 1) return false;
@@ -1005,10 +1005,28 @@ static bool whfast_prepare(struct reb_simulation* r, WHParams& P) {
     if (!d->whfast) d->whfast = new WHFastDevice();
     WHFastDevice* w = d->whfast;
     if (w->cap != d->cap) {
+        const bool fresh = w->cap == 0;
         if (!w->ph.ensure(sizeof(double) * 7 * d->cap)) return false;
         if (!RB_HIP(r, hipMemsetAsync(w->ph.p, 0, sizeof(double) * 7 * d->cap, d->stream))) return false;
         w->cap = d->cap;
-        r->did_modify_particles = 1;
+        auto* s = (struct reb_integrator_whfast_state*)r->integrator.state;
+        if (fresh && s && s->p_jh && s->N_allocated == r->N && r->N > 0 && !r->did_modify_particles) {
+            // The integrator state holds p_jh of its own (a simulation read
+            // from a binary file): continue from it, as the reference does.
+            std::vector<double> h(7 * (size_t)r->N);
+            for (size_t i = 0; i < r->N; i++) {
+                const struct reb_particle& q = s->p_jh[i];
+                const double f[7] = {q.x, q.y, q.z, q.vx, q.vy, q.vz, q.m};
+                for (int k = 0; k < 7; k++) h[(size_t)k * r->N + i] = f[k];
+            }
+            for (int k = 0; k < 7; k++)
+                if (!RB_HIP(r, hipMemcpyAsync(w->ph.as<double>() + (size_t)k * d->cap, h.data() + (size_t)k * r->N,
+                                              sizeof(double) * r->N, hipMemcpyHostToDevice, d->stream)))
+                    return false;
+            if (!stream_sync(r)) return false;
+        } else {
+            r->did_modify_particles = 1;
+        }
     }
     const size_t meta_bytes = sizeof(WHSys) + sizeof(WHTask) + 2 * sizeof(int) + 2 * sizeof(unsigned int) + 64;
     if (!w->meta.ensure(meta_bytes)) return false;
@@ -1096,6 +1114,14 @@ static void whfast_mirror_pjh(struct reb_simulation* r, struct reb_integrator_wh
         s->p_jh[i].vz = h[5 * r->N + i];
         s->p_jh[i].m = h[6 * r->N + i];
     }
+}
+
+// The host copy of p_jh made current (before a binary snapshot is written).
+void whfast_refresh_host(struct reb_simulation* r) {
+    auto* s = (struct reb_integrator_whfast_state*)r->integrator.state;
+    DeviceState* d = dev_of(r);
+    if (!s || !d->whfast || !d->whfast->cap) return;
+    whfast_mirror_pjh(r, s);
 }
 
 // n steps in one launch (n == 1 for the per-step plugin call).

@@ -17,6 +17,7 @@
  *   tree build / moments / walk          tree.c:64-328, gravity.c:47-106
  *   collision search + hardsphere        collision.c:49-405, 573-665 (DIRECT, TREE, LINE, LINETREE)
  *   SEI + shear boundary                 integrator_sei.c:86-174, boundary.c:35-201
+ *   binary files / simulation archives   binarydata.c:42-932, simulationarchive.c:45-666
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -576,6 +577,52 @@ static void gen_line(void){
     }
 }
 
+/* Binary files and archives (binarydata.c, simulationarchive.c): a leapfrog
+ * simulation saved before any step, and a WHFast archive with a snapshot
+ * every 7 steps (unsynchronized, safe_mode 0) over 30 steps; the bodies at
+ * each snapshot's step and after the 30 steps. */
+static void gen_archive(void){
+    char path[512];
+    struct reb_simulation* r = reb_simulation_create();
+    r->rand_seed = 77u;
+    r->G = 1.; r->softening = 0.01; r->dt = 0.01;
+    reb_simulation_add_plummer(r, 64, 1., 1.);
+    reb_simulation_move_to_com(r);
+    struct reb_integrator_leapfrog_state* lf = reb_simulation_set_integrator(r, "leapfrog");
+    lf->order = 4;
+    snprintf(path, sizeof(path), "%s/archive_leapfrog.bin", outdir);
+    remove(path);
+    reb_simulation_save_to_file(r, path);
+    dump_state("archive_leapfrog_state.npy", r);
+    reb_simulation_free(r);
+
+    struct reb_simulation* g = reb_simulation_create();
+    g->rand_seed = 515u;
+    r = ens_system(g, 6, 6);
+    reb_simulation_free(g);
+    struct reb_integrator_whfast_state* wh = reb_simulation_set_integrator(r, "whfast");
+    wh->safe_mode = 0;
+    wh->corrector = 5;
+    r->dt = 2.*M_PI*sqrt(1./(1.+r->particles[1].m))/20.;
+    snprintf(path, sizeof(path), "%s/archive_whfast.bin", outdir);
+    remove(path);
+    reb_simulation_save_to_file_step(r, path, 7);
+    reb_simulation_steps(r, 30);
+    dump_state("archive_whfast_final_state.npy", r);
+    reb_simulation_free(r);
+    /* every snapshot as the reference reads it back */
+    for (int k=0; k<5; k++){
+        struct reb_simulation* q = reb_simulation_create_from_file(path, k);
+        char name[128];
+        snprintf(name, sizeof(name), "archive_whfast_snap%d_state.npy", k);
+        dump_state(name, q);
+        int64_t meta[3] = {(int64_t)q->steps_done, (int64_t)q->is_synchronized, (int64_t)q->N};
+        snprintf(name, sizeof(name), "archive_whfast_snap%d_meta.npy", k);
+        wi(name, meta, 3, 0);
+        reb_simulation_free(q);
+    }
+}
+
 /* WHFast in Jacobi coordinates (the reference's default): NS systems of
  * 1..8 planets, each integrated under every mode below for three
  * reb_simulation_steps calls of `nsteps` steps.  Per mode: energies after 0,
@@ -781,5 +828,6 @@ int main(int argc, char** argv){
     gen_merge();
     gen_whfast_jacobi();
     gen_line();
+    gen_archive();
     return 0;
 }

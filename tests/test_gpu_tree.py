@@ -156,6 +156,57 @@ def test_sheet_line_searches_bitwise(gpu, kind, steps):
     assert np.array_equal(got[:, 2:], golden("sheet2000_%s_gb.npy" % kind))
 
 
+@pytest.mark.parametrize("kind", ["tree", "linetree"])
+def test_search_bodies_with_many_collisions(gpu, orc, kind):
+    """A body overlapping 40 approaching bodies: more collisions than the
+    single-pass search keeps per body, so the search takes its second pass;
+    the list in resolve order equals the oracle's search after the reference's
+    shuffle."""
+    from oracle import lib as olib
+    rng = np.random.default_rng(3)
+    n = 400
+    st = np.zeros((n, 11))
+    st[:, 0:2] = rng.uniform(-40, 40, (n, 2))
+    st[:, 2] = rng.uniform(-2, 2, n)
+    st[:, 9] = 1.0
+    st[:, 10] = 0.3
+    st[0, :3] = 0.0
+    st[0, 10] = 6.0
+    d = rng.normal(size=(40, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    st[1:41, :3] = d * rng.uniform(2.0, 5.5, (40, 1))
+    st[1:41, 3:6] = -0.2 * st[1:41, :3]
+    sim = gpu.Sim()
+    sim.set_state(st)
+    sim.s.rand_seed = 1234
+    sim.s.root_size = 100.0
+    sim.s.N_root_x = sim.s.N_root_y = sim.s.N_root_z = 1
+    sim.s.gravity = gpu.GRAVITY["none"]
+    sim.s.collision = gpu.COLLISION[kind]
+    sim.s.dt = 0.05
+    sim.set_integrator("leapfrog")
+    rec = []
+
+    def resolve(r, c):
+        rec.append((c.p1, c.p2))
+        return 0
+
+    cb = gpu._abi.RESOLVE(resolve)
+    sim._keep.append(cb)
+    sim.s.collision_resolve = cb
+    sim.steps(1)
+    s1 = sim.state()
+    g0 = np.zeros((1, 6))
+    if kind == "tree":
+        pairs, gbo = orc.collisions_tree(s1, 100.0, (1, 1, 1), g0)
+    else:
+        pairs, gbo = orc.collisions_linetree(s1, 100.0, (1, 1, 1), g0, 0.05)
+    seed = ctypes.c_uint(1234)
+    olib().orc_collision_shuffle(pairs, gbo, len(pairs), ctypes.byref(seed))
+    assert (pairs[:, 0] == 0).sum() > 8
+    assert np.array_equal(np.array(rec, dtype=np.int64), pairs)
+
+
 def test_sheet_direct_search_bitwise(gpu):
     sim, rec = _run_steps(gpu, "direct", True)
     sim.steps(1)
