@@ -41,6 +41,8 @@ def lib():
                                                 c_double, c_double, _dp]),
             "orc_gravity_basic": (None, [_dp, c_size_t, c_size_t, c_int, c_int, c_double, c_double, _dp, c_int, _dp]),
             "orc_gravity_compensated": (None, [_dp, c_size_t, c_size_t, c_int, c_int, c_double, c_double, _dp]),
+            "orc_gravity_rows": (None, [_dp, c_size_t, c_size_t, c_int, c_int, c_double, c_double, _dp, c_int, c_int,
+                                        _ip, c_size_t, _dp]),
             "orc_energy": (c_double, [_dp, c_size_t, c_size_t, c_int, c_double]),
             "orc_leapfrog": (None, [_dp, c_size_t, c_double, c_double, c_double, c_long, POINTER(c_double), c_long,
                                     ctypes.c_void_p]),
@@ -110,6 +112,17 @@ def gravity_compensated(s, G=1.0, soft=0.0, N_active=None, tp_type=0, ignore=0):
     n = s.shape[0]
     acc = np.zeros((n, 3))
     lib().orc_gravity_compensated(s, n, n if N_active is None else N_active, tp_type, ignore, G, soft, acc)
+    return acc
+
+
+def gravity_rows(s, rows, G=1.0, soft=0.0, N_active=None, tp_type=0, ignore=0, gbs=None, compensated=0):
+    s = np.ascontiguousarray(s, dtype=np.float64)
+    n = s.shape[0]
+    rows = np.ascontiguousarray(rows, dtype=np.int64)
+    gbs = np.zeros((1, 3)) if gbs is None else np.ascontiguousarray(gbs, dtype=np.float64)
+    acc = np.zeros((len(rows), 3))
+    lib().orc_gravity_rows(s, n, n if N_active is None else N_active, tp_type, ignore, G, soft, gbs, gbs.shape[0],
+                           compensated, rows, len(rows), acc)
     return acc
 
 

@@ -184,6 +184,61 @@ static void kahan(double term, double* a, double* c) {
     *a = tv;
 }
 
+/* The accelerations of the target rows rows[0..nrows) alone, with the
+ * arithmetic and order of orc_gravity_basic (compensated = 0; ghost boxes
+ * gbs) or orc_gravity_compensated (compensated = 1; no ghost boxes): for
+ * checking large N against a sample of targets. */
+void orc_gravity_rows(const double* s, size_t N, size_t N_active, int tp_type, int ignore, double G, double soft,
+                      const double* gbs, int ngb, int compensated, const int64_t* rows, size_t nrows, double* acc) {
+    const double softening2 = soft * soft;
+    if (N_active > N) N_active = N;
+    if (compensated) ngb = 1;
+    for (size_t q = 0; q < nrows; q++) {
+        const size_t i = (size_t)rows[q];
+        double a[3] = {0., 0., 0.}, c[3] = {0., 0., 0.};
+        for (int g = 0; g < ngb; g++) {
+            const double gx = compensated ? 0. : gbs[3 * g], gy = compensated ? 0. : gbs[3 * g + 1],
+                         gz = compensated ? 0. : gbs[3 * g + 2];
+            for (int pass = 0; pass < 2; pass++) {
+                const size_t j0 = pass == 0 ? 0 : N_active, j1 = pass == 0 ? N_active : N;
+                if (pass == 1 && (!tp_type || i >= N_active)) break;
+                for (size_t j = j0; j < j1; j++) {
+                    if (pass == 0 || compensated) {
+                        if (skip_pair(i, j, ignore)) continue;
+                    } else {
+                        if (ignore == 1 && (j == 1 && i == 0)) continue;
+                        if (ignore == 2 && (j == 0 || i == 0)) continue;
+                    }
+                    if (compensated) {
+                        const double dx = COL(s, i, X) - COL(s, j, X);
+                        const double dy = COL(s, i, Y) - COL(s, j, Y);
+                        const double dz = COL(s, i, Z) - COL(s, j, Z);
+                        const double r2 = dx * dx + dy * dy + dz * dz + softening2;
+                        const double r = sqrt(r2);
+                        const double prefact = G / (r2 * r);
+                        const double prefactj = -prefact * COL(s, j, M);
+                        kahan(prefactj * dx, &a[0], &c[0]);
+                        kahan(prefactj * dy, &a[1], &c[1]);
+                        kahan(prefactj * dz, &a[2], &c[2]);
+                    } else {
+                        const double dx = (gx + COL(s, i, X)) - COL(s, j, X);
+                        const double dy = (gy + COL(s, i, Y)) - COL(s, j, Y);
+                        const double dz = (gz + COL(s, i, Z)) - COL(s, j, Z);
+                        const double _r = sqrt(dx * dx + dy * dy + dz * dz + softening2);
+                        const double prefact = -G / (_r * _r * _r) * COL(s, j, M);
+                        a[0] += prefact * dx;
+                        a[1] += prefact * dy;
+                        a[2] += prefact * dz;
+                    }
+                }
+            }
+        }
+        acc[3 * q] = a[0];
+        acc[3 * q + 1] = a[1];
+        acc[3 * q + 2] = a[2];
+    }
+}
+
 void orc_gravity_compensated(const double* s, size_t N, size_t N_active, int tp_type, int ignore, double G,
                              double soft, double* acc) {
     const double softening2 = soft * soft;

@@ -35,6 +35,10 @@ int orc_particle_from_orbit(double G, const double* primary7 /* x y z vx vy vz m
 void orc_gravity_basic(const double* s, size_t N, size_t N_active, int tp_type, int ignore, double G, double soft,
                        const double* gbs, int ngb, double* acc3);
 /* gravity.c:308-410 (OpenMP form) */
+/* Target rows rows[0..nrows) of orc_gravity_basic (compensated = 0) or
+ * orc_gravity_compensated (compensated = 1), same arithmetic and order. */
+void orc_gravity_rows(const double* s, size_t N, size_t N_active, int tp_type, int ignore, double G, double soft,
+                      const double* gbs, int ngb, int compensated, const int64_t* rows, size_t nrows, double* acc);
 void orc_gravity_compensated(const double* s, size_t N, size_t N_active, int tp_type, int ignore, double G,
                              double soft, double* acc3);
 /* tools.c:108-162 */

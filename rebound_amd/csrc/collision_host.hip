@@ -18,6 +18,13 @@
 // touch disjoint bodies and are resolved in parallel on host threads; every
 // body still sees its collisions in list order, and the pressure log is summed
 // in list order afterwards, so the result is the sequential one, bit for bit.
+// Right after the search the device numbers the colliding bodies, rewrites
+// the records with those numbers and packs the eight fields the resolver
+// reads into 64-byte records; both travel to pinned host memory while the
+// host draws the shuffle, the resolver works on the records in place, and
+// they travel back for a scatter of the velocities.
+#include <hipcub/hipcub.hpp>
+
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -166,10 +173,12 @@ class HostPool {
     bool stop_ = false;
 };
 
-// collision.c:573-665 on the host copy: applies one hard-sphere collision to
-// p1 and p2 and returns 1 with the pressure-log term in *plog if it did.
-static int hardsphere_apply(const struct reb_simulation* r, struct reb_particle* particles,
-                            const struct reb_collision& c, double* plog);
+// collision.c:573-665 on a host copy: applies one hard-sphere collision to
+// bodies i1 and i2 (reb_particle or HSBody records) and returns 1 with the
+// pressure-log term in *plog if it did.
+template <class Body>
+static int hardsphere_apply(const struct reb_simulation* r, Body* particles, size_t i1, size_t i2,
+                            const struct reb_vec6d& gb, double* plog);
 
 // Gathers / scatters the full state of a list of bodies.
 __global__ void k_gather_bodies(const double* __restrict__ soa, size_t cap, const int* __restrict__ ids, int n,
@@ -188,25 +197,52 @@ __global__ void k_scatter_bodies(double* __restrict__ soa, size_t cap, const int
     for (int f = 0; f < NFIELD; f++) soa[(size_t)f * cap + i] = in[(size_t)k * NFIELD + f];
 }
 
-// The fields the hard-sphere resolver reads (x y z vx vy vz m r) of listed
-// bodies, and the velocities it writes back.
-__global__ void k_gather_hs(const double* __restrict__ soa, size_t cap, const int* __restrict__ ids, int n,
-                            double* __restrict__ out) {
+// The eight fields the hard-sphere resolver reads, one 64-byte record per
+// colliding body; it writes back vx, vy, vz.
+struct HSBody {
+    double x, y, z, vx, vy, vz, m, r;
+};
+// A collision in compact body numbers, with its ghost-box index.
+struct HSPair {
+    int p1, p2, gb;
+};
+
+// flag[p] = 1 for every body named by a record.
+__global__ void k_coll_mark(const CollRecH* __restrict__ rec, int nc, int* __restrict__ flag) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const size_t i = (size_t)ids[k];
-    const int f[8] = {FX, FY, FZ, FVX, FVY, FVZ, FM, FR};
-    for (int q = 0; q < 8; q++) out[(size_t)q * n + k] = soa[(size_t)f[q] * cap + i];
+    if (k >= nc) return;
+    flag[rec[k].p1] = 1;
+    flag[rec[k].p2] = 1;
 }
 
-__global__ void k_scatter_v(double* __restrict__ soa, size_t cap, const int* __restrict__ ids, int n,
-                            const double* __restrict__ in) {
+// Compact number cidx[p] (exclusive scan of flag) -> the body's fields.
+__global__ void k_coll_gather(const double* __restrict__ soa, size_t cap, const int* __restrict__ flag,
+                              const int* __restrict__ cidx, int N, int* __restrict__ ids, HSBody* __restrict__ out) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N || !flag[p]) return;
+    const int c = cidx[p];
+    ids[c] = p;
+    out[c] = HSBody{soa[(size_t)FX * cap + p],  soa[(size_t)FY * cap + p],  soa[(size_t)FZ * cap + p],
+                    soa[(size_t)FVX * cap + p], soa[(size_t)FVY * cap + p], soa[(size_t)FVZ * cap + p],
+                    soa[(size_t)FM * cap + p],  soa[(size_t)FR * cap + p]};
+}
+
+__global__ void k_coll_pairs(const CollRecH* __restrict__ rec, int nc, const int* __restrict__ cidx,
+                             HSPair* __restrict__ out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nc) return;
+    const CollRecH h = rec[k];
+    out[k] = HSPair{cidx[h.p1], cidx[h.p2], h.gb};
+}
+
+__global__ void k_coll_scatter_v(double* __restrict__ soa, size_t cap, const int* __restrict__ ids, int n,
+                                 const HSBody* __restrict__ in) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const size_t i = (size_t)ids[k];
-    soa[(size_t)FVX * cap + i] = in[k];
-    soa[(size_t)FVY * cap + i] = in[(size_t)n + k];
-    soa[(size_t)FVZ * cap + i] = in[2 * (size_t)n + k];
+    soa[(size_t)FVX * cap + i] = in[k].vx;
+    soa[(size_t)FVY * cap + i] = in[k].vy;
+    soa[(size_t)FVZ * cap + i] = in[k].vz;
 }
 
 static void copy_body_in(struct reb_particle& p, const double* s) {
@@ -279,39 +315,125 @@ static int rand_r_step(unsigned int* seed) {
 
 }  // namespace rbamd
 
-// Host arrays of the compact resolve, reused from step to step (sized to the
-// largest step so far); `loc` is all -1 between calls.
+// Buffers of the compact resolve, reused from step to step (sized to the
+// largest step so far).  `ready` is set when the device has prepared the
+// compact records of the last search.
 struct rbamd::ResolveScratch {
-    struct Pair {
-        int p1, p2, gb;   // compact body indices, ghost-box index
-    };
-    std::vector<int> loc, ids, level, last, perm;
+    DevBuf flag, cidx, ids, body, pair, cub;      // device
+    HSBody* body_h = nullptr;                     // pinned host
+    HSPair* pair_h = nullptr;
+    int* n_h = nullptr;
+    size_t body_cap = 0, pair_cap = 0;
+    int n = 0;
+    bool ready = false;
+    std::vector<int> level, last, perm;
     std::vector<size_t> start, order, fill;
-    std::vector<Pair> cl;
-    std::vector<struct reb_particle> P;
+    std::vector<HSPair> cl;
     std::vector<double> plog;
     std::vector<unsigned char> hit;
+    void release() {
+        for (DevBuf* b : {&flag, &cidx, &ids, &body, &pair, &cub}) b->release();
+        if (body_h) (void)hipHostFree(body_h);
+        if (pair_h) (void)hipHostFree(pair_h);
+        if (n_h) (void)hipHostFree(n_h);
+        body_h = nullptr;
+        pair_h = nullptr;
+        n_h = nullptr;
+        body_cap = pair_cap = 0;
+    }
 };
 
 namespace rbamd {
 
 void resolve_release(struct reb_simulation* r) {
     DeviceState* d = dev_of(r);
+    if (d->resolve) d->resolve->release();
     delete d->resolve;
     d->resolve = nullptr;
 }
 
-// The built-in hard-sphere resolver with a constant restitution, on a compact
-// host copy of the involved bodies: only the eight fields it reads come over,
-// only the velocities go back, and collisions of one level run in parallel.
-static void resolve_hardsphere_compact(struct reb_simulation* r) {
+bool compact_resolve_eligible(const struct reb_simulation* r) {
+    const DeviceState* d = dev_of(r);
+    return r->collision_resolve == reb_collision_resolve_hardsphere &&
+           (r->coefficient_of_restitution == nullptr ||
+            r->coefficient_of_restitution == reb_amd_coefficient_of_restitution_constant) &&
+           d->loc != LOC_HOST;
+}
+
+// Grows a pinned host array of T to at least `need` elements (contents lost).
+template <class T>
+static bool pinned_ensure(struct reb_simulation* r, T*& p, size_t& cap, size_t need) {
+    if (need <= cap && p) return true;
+    size_t c = cap ? cap : 4096;
+    while (c < need) c *= 2;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    if (!RB_HIP(r, hipHostMalloc((void**)&p, sizeof(T) * c, hipHostMallocDefault))) return false;
+    cap = c;
+    return true;
+}
+
+bool compact_resolve_mark(struct reb_simulation* r, const CollRecH* rec, size_t nc) {
     DeviceState* d = dev_of(r);
     if (!d->resolve) d->resolve = new ResolveScratch();
     ResolveScratch& W = *d->resolve;
+    W.ready = false;
+    const int N = (int)r->N;
+    size_t tmp = 0;
+    hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, W.flag.as<int>(), W.cidx.as<int>(), N + 1, d->stream);
+    if (!W.flag.ensure(sizeof(int) * (N + 1)) || !W.cidx.ensure(sizeof(int) * (N + 1)) || !W.cub.ensure(tmp + 256)) {
+        reb_simulation_error(r, "Out of device memory for the collision resolve.");
+        return false;
+    }
+    if (!W.n_h && !RB_HIP(r, hipHostMalloc((void**)&W.n_h, sizeof(int), hipHostMallocDefault))) return false;
+    if (!RB_HIP(r, hipMemsetAsync(W.flag.p, 0, sizeof(int) * (N + 1), d->stream))) return false;
+    hipLaunchKernelGGL(k_coll_mark, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, d->stream, rec, (int)nc,
+                       W.flag.as<int>());
+    hipcub::DeviceScan::ExclusiveSum(W.cub.p, tmp, W.flag.as<int>(), W.cidx.as<int>(), N + 1, d->stream);
+    return RB_HIP(r, hipMemcpyAsync(W.n_h, W.cidx.as<int>() + N, sizeof(int), hipMemcpyDeviceToHost, d->stream));
+}
+
+bool compact_resolve_gather(struct reb_simulation* r, const CollRecH* rec, size_t nc) {
+    DeviceState* d = dev_of(r);
+    ResolveScratch& W = *d->resolve;
+    const int N = (int)r->N;
+    const int n = *W.n_h;   // the stream has been synchronized since compact_resolve_mark
+    if (!W.ids.ensure(sizeof(int) * n + 256) || !W.body.ensure(sizeof(HSBody) * n + 256) ||
+        !W.pair.ensure(sizeof(HSPair) * nc + 256)) {
+        reb_simulation_error(r, "Out of device memory for the collision resolve.");
+        return false;
+    }
+    if (!pinned_ensure(r, W.body_h, W.body_cap, (size_t)n) || !pinned_ensure(r, W.pair_h, W.pair_cap, nc))
+        return false;
+    hipLaunchKernelGGL(k_coll_gather, dim3((N + 255) / 256), dim3(256), 0, d->stream, d->soa.as<double>(), d->cap,
+                       W.flag.as<int>(), W.cidx.as<int>(), N, W.ids.as<int>(), W.body.as<HSBody>());
+    hipLaunchKernelGGL(k_coll_pairs, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, d->stream, rec, (int)nc,
+                       W.cidx.as<int>(), W.pair.as<HSPair>());
+    if (!RB_HIP(r, hipGetLastError())) return false;
+    if (!RB_HIP(r, hipMemcpyAsync(W.pair_h, W.pair.p, sizeof(HSPair) * nc, hipMemcpyDeviceToHost, d->stream)))
+        return false;
+    if (!RB_HIP(r, hipMemcpyAsync(W.body_h, W.body.p, sizeof(HSBody) * n, hipMemcpyDeviceToHost, d->stream)))
+        return false;
+    W.n = n;
+    W.ready = true;
+    return true;
+}
+
+// The built-in hard-sphere resolver with a constant restitution, on the
+// compact records prepared by compact_resolve_gather: only the eight fields it
+// reads come over, only the velocities go back, and collisions of one level
+// run in parallel.
+static void resolve_hardsphere_compact(struct reb_simulation* r) {
+    DeviceState* d = dev_of(r);
+    ResolveScratch& W = *d->resolve;
+    W.ready = false;
     const CollList& L = *d->coll;
     const size_t nc = L.n;
-    // shuffle (collision.c:336-342) as a permutation of the detection order:
-    // the same swaps applied to indices instead of 72-byte records
+    const int n = W.n;
+    // shuffle (collision.c:336-342) as a permutation of the detection order,
+    // the same swaps applied to indices instead of 72-byte records; drawn
+    // while the compact records are on their way
     std::vector<int>& perm = W.perm;
     perm.resize(nc);
     for (size_t k = 0; k < nc; k++) perm[k] = (int)k;
@@ -319,38 +441,23 @@ static void resolve_hardsphere_compact(struct reb_simulation* r) {
         const size_t nw = (size_t)rand_r_step(&r->rand_seed) % nc;
         std::swap(perm[i], perm[nw]);
     }
-    if (W.loc.size() < r->N) W.loc.resize(r->N, -1);
-    std::vector<int>& loc = W.loc;
-    std::vector<int>& ids = W.ids;
-    ids.clear();
+    if (!stream_sync(r)) return;
+    // the shuffled list in compact numbers, and its levels: one more than the
+    // last level that touched either body
     auto& cl = W.cl;
-    cl.resize(nc);
-    for (size_t k = 0; k < nc; k++) {
-        const CollRecH& h = L.rec[perm[k]];
-        int c2[2] = {h.p1, h.p2};
-        for (int& p : c2) {
-            int& l = loc[p];
-            if (l < 0) {
-                l = (int)ids.size();
-                ids.push_back(p);
-            }
-            p = l;
-        }
-        cl[k] = ResolveScratch::Pair{c2[0], c2[1], h.gb};
-    }
-    const int n = (int)ids.size();
-    for (int k = 0; k < n; k++) loc[ids[k]] = -1;   // ready for the next call
-    // levels: one more than the last level that touched either body
     std::vector<int>& level = W.level;
     std::vector<int>& last = W.last;
+    cl.resize(nc);
     level.resize(nc);
     last.assign(n, 0);
     int nlev = 0;
     for (size_t k = 0; k < nc; k++) {
-        const int l = std::max(last[cl[k].p1], last[cl[k].p2]) + 1;
+        const HSPair q = W.pair_h[perm[k]];
+        cl[k] = q;
+        const int l = std::max(last[q.p1], last[q.p2]) + 1;
         level[k] = l;
-        last[cl[k].p1] = l;
-        last[cl[k].p2] = l;
+        last[q.p1] = l;
+        last[q.p2] = l;
         nlev = std::max(nlev, l);
     }
     std::vector<size_t>& start = W.start;
@@ -362,66 +469,8 @@ static void resolve_hardsphere_compact(struct reb_simulation* r) {
     for (int l = 1; l <= nlev + 1; l++) start[l] += start[l - 1];
     fill.assign(start.begin(), start.end());
     for (size_t k = 0; k < nc; k++) order[fill[level[k]]++] = k;
-    // bring the eight fields over (pinned staging)
-    const size_t idbytes = ((sizeof(int) * n + 255) / 256) * 256;
-    if (!d->aos.ensure(idbytes + sizeof(double) * 8 * n + 256)) {
-        reb_simulation_error(r, "Out of device memory for the collision resolve.");
-        return;
-    }
-    int* dids = d->aos.as<int>();
-    double* ddata = reinterpret_cast<double*>(d->aos.as<char>() + idbytes);
-    std::vector<double> fallback;
-    double* h = d->host_stage;
-    if (!h || (size_t)n * 8 > d->host_cap * NFIELD) {
-        fallback.resize((size_t)8 * n);
-        h = fallback.data();
-    }
-    if (!RB_HIP(r, hipMemcpyAsync(dids, ids.data(), sizeof(int) * n, hipMemcpyHostToDevice, d->stream))) return;
-    hipLaunchKernelGGL(k_gather_hs, dim3((n + 255) / 256), dim3(256), 0, d->stream, d->soa.as<double>(), d->cap,
-                       dids, n, ddata);
-    if (!RB_HIP(r, hipMemcpyAsync(h, ddata, sizeof(double) * 8 * n, hipMemcpyDeviceToHost, d->stream))) return;
-    if (!stream_sync(r)) return;
-    std::vector<struct reb_particle>& P = W.P;
-    if (P.size() < (size_t)n) P.resize(n);
-    for (int k = 0; k < n; k++) {
-        struct reb_particle& p = P[k];
-        p.x = h[k];
-        p.y = h[(size_t)n + k];
-        p.z = h[2 * (size_t)n + k];
-        p.vx = h[3 * (size_t)n + k];
-        p.vy = h[4 * (size_t)n + k];
-        p.vz = h[5 * (size_t)n + k];
-        p.m = h[6 * (size_t)n + k];
-        p.r = h[7 * (size_t)n + k];
-    }
-    std::vector<double>& plog = W.plog;
-    std::vector<unsigned char>& hit = W.hit;
-    plog.assign(nc, 0.);
-    hit.assign(nc, 0);
-    HostPool::get().run_levels(nlev, start.data(), order.data(), [&](size_t k) {
-        struct reb_collision c;
-        c.p1 = (size_t)cl[k].p1;
-        c.p2 = (size_t)cl[k].p2;
-        c.gb = L.gb[cl[k].gb];
-        c.ri = 0;
-        hit[k] = (unsigned char)hardsphere_apply(r, P.data(), c, &plog[k]);
-    });
-    // the pressure log in list order (collision.c:655-663)
-    for (size_t k = 0; k < nc; k++) {
-        if (!hit[k]) continue;
-        r->collisions_plog += plog[k];
-        r->collisions_log_n++;
-    }
-    for (int k = 0; k < n; k++) {
-        h[k] = P[k].vx;
-        h[(size_t)n + k] = P[k].vy;
-        h[2 * (size_t)n + k] = P[k].vz;
-    }
-    if (!RB_HIP(r, hipMemcpyAsync(ddata, h, sizeof(double) * 3 * n, hipMemcpyHostToDevice, d->stream))) return;
-    hipLaunchKernelGGL(k_scatter_v, dim3((n + 255) / 256), dim3(256), 0, d->stream, d->soa.as<double>(), d->cap,
-                       dids, n, ddata);
-    (void)RB_HIP(r, hipGetLastError());
-    // r->collisions as the reference leaves it: the shuffled list
+    // r->collisions as the reference leaves it: the shuffled list, written
+    // by the same pass that resolves
     if (r->N_allocated_collisions < nc) {
         size_t cap = r->N_allocated_collisions ? r->N_allocated_collisions : 32;
         while (cap < nc) cap *= 2;
@@ -433,7 +482,13 @@ static void resolve_hardsphere_compact(struct reb_simulation* r) {
         r->collisions = pc;
         r->N_allocated_collisions = cap;
     }
-    HostPool::get().run(nc, [&](size_t k) {
+    std::vector<double>& plog = W.plog;
+    std::vector<unsigned char>& hit = W.hit;
+    plog.assign(nc, 0.);
+    hit.assign(nc, 0);
+    HSBody* B = W.body_h;
+    HostPool::get().run_levels(nlev, start.data(), order.data(), [&](size_t k) {
+        hit[k] = (unsigned char)hardsphere_apply(r, B, (size_t)cl[k].p1, (size_t)cl[k].p2, L.gb[cl[k].gb], &plog[k]);
         const CollRecH& hr = L.rec[perm[k]];
         struct reb_collision c;
         c.p1 = (size_t)hr.p1;
@@ -442,8 +497,19 @@ static void resolve_hardsphere_compact(struct reb_simulation* r) {
         c.ri = (size_t)hr.ri;
         r->collisions[k] = c;
     });
+    // the pressure log in list order (collision.c:655-663)
+    for (size_t k = 0; k < nc; k++) {
+        if (!hit[k]) continue;
+        r->collisions_plog += plog[k];
+        r->collisions_log_n++;
+    }
+    // the records go back whole; the scatter takes the velocities.  The next
+    // search's copy into body_h is ordered after this one on the stream.
+    if (!RB_HIP(r, hipMemcpyAsync(W.body.p, B, sizeof(HSBody) * n, hipMemcpyHostToDevice, d->stream))) return;
+    hipLaunchKernelGGL(k_coll_scatter_v, dim3((n + 255) / 256), dim3(256), 0, d->stream, d->soa.as<double>(), d->cap,
+                       W.ids.as<int>(), n, W.body.as<HSBody>());
+    (void)RB_HIP(r, hipGetLastError());
     d->coll->pending = false;
-    (void)stream_sync(r);   // the staging buffer is reused by the next transfer
     mark_device_newer(r);
 }
 
@@ -453,13 +519,12 @@ void collision_resolve_host(struct reb_simulation* r) {
         r->collision_resolve;
     if (resolve == nullptr) resolve = reb_collision_resolve_halt;
     DeviceState* d = dev_of(r);
-    if (resolve == reb_collision_resolve_hardsphere &&
-        (r->coefficient_of_restitution == nullptr ||
-         r->coefficient_of_restitution == reb_amd_coefficient_of_restitution_constant) &&
-        d->loc != LOC_HOST && d->coll && d->coll->pending && d->coll->n == r->N_collisions) {
+    if (d->resolve && d->resolve->ready && compact_resolve_eligible(r) && d->coll && d->coll->pending &&
+        d->coll->n == r->N_collisions) {
         resolve_hardsphere_compact(r);   // shuffles, resolves and writes r->collisions
         return;
     }
+    if (d->resolve) d->resolve->ready = false;
     if (!materialize_collisions(r)) return;
     // shuffle (collision.c:336-342)
     for (size_t i = 0; i < r->N_collisions; i++) {
@@ -540,7 +605,7 @@ extern "C" {
 REB_API enum REB_COLLISION_RESOLVE_OUTCOME reb_collision_resolve_hardsphere(struct reb_simulation* const r,
                                                                             struct reb_collision c) {
     double plog = 0.;
-    if (rbamd::hardsphere_apply(r, r->particles, c, &plog)) {
+    if (rbamd::hardsphere_apply(r, r->particles, c.p1, c.p2, c.gb, &plog)) {
         r->collisions_plog += plog;
         r->collisions_log_n++;
     }
@@ -551,11 +616,11 @@ REB_API enum REB_COLLISION_RESOLVE_OUTCOME reb_collision_resolve_hardsphere(stru
 
 namespace rbamd {
 
-static int hardsphere_apply(const struct reb_simulation* r, struct reb_particle* particles,
-                            const struct reb_collision& c, double* plog) {
-    struct reb_particle p1 = particles[c.p1];
-    struct reb_particle p2 = particles[c.p2];
-    struct reb_vec6d gb = c.gb;
+template <class Body>
+static int hardsphere_apply(const struct reb_simulation* r, Body* particles, size_t i1, size_t i2,
+                            const struct reb_vec6d& gb, double* plog) {
+    const Body p1 = particles[i1];
+    const Body p2 = particles[i2];
     double x21 = p1.x + gb.x - p2.x;
     double y21 = p1.y + gb.y - p2.y;
     double z21 = p1.z + gb.z - p2.z;
@@ -599,17 +664,17 @@ static int hardsphere_apply(const struct reb_simulation* r, struct reb_particle*
     double dvy2nn = ctheta * dvy2n;
     double dvz2nn = stheta * dvy2n;
     const double p2pf = p1.m / (p1.m + p2.m);
-    particles[c.p2].vx -= p2pf * dvx2n;
-    particles[c.p2].vy -= p2pf * dvy2nn;
-    particles[c.p2].vz -= p2pf * dvz2nn;
+    particles[i2].vx -= p2pf * dvx2n;
+    particles[i2].vy -= p2pf * dvy2nn;
+    particles[i2].vz -= p2pf * dvz2nn;
     const double p1pf = p2.m / (p1.m + p2.m);
-    particles[c.p1].vx += p1pf * dvx2n;
-    particles[c.p1].vy += p1pf * dvy2nn;
-    particles[c.p1].vz += p1pf * dvz2nn;
+    particles[i1].vx += p1pf * dvx2n;
+    particles[i1].vy += p1pf * dvy2nn;
+    particles[i1].vz += p1pf * dvz2nn;
     if (x21 > 0) {
-        *plog = -fabs(x21) * (oldvyouter - particles[c.p1].vy) * p1.m;
+        *plog = -fabs(x21) * (oldvyouter - particles[i1].vy) * p1.m;
     } else {
-        *plog = -fabs(x21) * (oldvyouter - particles[c.p2].vy) * p2.m;
+        *plog = -fabs(x21) * (oldvyouter - particles[i2].vy) * p2.m;
     }
     return 1;
 }

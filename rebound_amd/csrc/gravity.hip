@@ -89,6 +89,7 @@ __device__ __forceinline__ void kahan_add(double term, double& a, double& c) {
 // the summing lanes' chains overlap the other waves' terms of the next tile;
 // each thread loads its sources one tile ahead.
 static constexpr int kExThreads = 256;
+static constexpr int kExactRowwiseN = 49152;   // from here on the row-wise kernels
 
 template <bool COMP, int TGT, int TILE>
 __global__ __launch_bounds__(kExThreads) void k_exact(const double* __restrict__ x, const double* __restrict__ y,
@@ -219,24 +220,157 @@ __global__ __launch_bounds__(kExThreads) void k_exact(const double* __restrict__
     }
 }
 
+// ---------------------------------------------------------------- EXACT, row-wise (large N)
+// One thread per target adds its terms in ascending source order; the terms
+// of tiles away from the target are independent and overlap in flight.  At
+// large N this beats the LDS-term kernel above, whose per-term LDS traffic
+// and barriers cost more than the serial chain hides.
+//
+// gravity.c:217-231 (targets 0..N-1, sources 0..N_active-1 for every ghost
+// box), then :259-276 (active targets also feel test particles when
+// testparticle_type is set).
+__global__ __launch_bounds__(kBlock) void k_basic_exact(
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+    const double* __restrict__ m, double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
+    int N, int N_active, int tp_type, int ignore, double G, double soft2, GBList gl) {
+    __shared__ double4 tile[kBlock];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const bool live = i < N;
+    const int blk_lo = blockIdx.x * kBlock, blk_hi = blk_lo + kBlock;
+    const double xi = live ? x[i] : 0., yi = live ? y[i] : 0., zi = live ? z[i] : 0.;
+    double a_x = 0., a_y = 0., a_z = 0.;
+    for (int g = 0; g < gl.n; g++) {
+        const GB3 gb = gl.g[g];
+        for (int pass = 0; pass < 2; pass++) {
+            int j0 = pass == 0 ? 0 : N_active;
+            int j1 = pass == 0 ? N_active : N;
+            if (pass == 1 && !tp_type) break;
+            for (int t = j0; t < j1; t += kBlock) {
+                __syncthreads();
+                const int jj = t + (int)threadIdx.x;
+                if (jj < j1) tile[threadIdx.x] = make_double4(x[jj], y[jj], z[jj], m[jj]);
+                __syncthreads();
+                const int n = min(kBlock, j1 - t);
+                if (!live || (pass == 1 && i >= N_active)) continue;
+                auto term = [&](const double4 s) {
+                    const double dx = (gb.x + xi) - s.x;
+                    const double dy = (gb.y + yi) - s.y;
+                    const double dz = (gb.z + zi) - s.z;
+                    const double _r = sqrt(dx * dx + dy * dy + dz * dz + soft2);
+                    const double prefact = -G / (_r * _r * _r) * s.w;
+                    a_x += prefact * dx;
+                    a_y += prefact * dy;
+                    a_z += prefact * dz;
+                };
+                // Tiles without this block's targets (and, with ignored terms,
+                // without bodies 0 and 1 as sources or among the block's
+                // targets) need no per-pair test: the terms are then
+                // independent work the compiler can overlap, while the sums
+                // still add them in ascending j.
+                const bool check =
+                    (pass == 0 && t < blk_hi && t + n > blk_lo) || (ignore != 0 && (t < 2 || blk_lo < 2));
+                if (!check) {
+#pragma unroll 4
+                    for (int k = 0; k < n; k++) term(tile[k]);
+                    continue;
+                }
+                for (int k = 0; k < n; k++) {
+                    const int j = t + k;
+                    if (pass == 0) {
+                        if (skip_pair(i, j, ignore)) continue;
+                    } else {
+                        if (ignore == 1 && j == 1 && i == 0) continue;
+                        if (ignore == 2 && (j == 0 || i == 0)) continue;
+                    }
+                    term(tile[k]);
+                }
+            }
+        }
+    }
+    if (live) {
+        ax[i] = a_x;
+        ay[i] = a_y;
+        az[i] = a_z;
+    }
+}
+
+// COMPENSATED, row-wise
+// gravity.c:308-343 (active targets), :346-378 (test-particle targets), :379-410
+// (active targets feel test particles).  No ghost boxes in the reference.
+
+__global__ __launch_bounds__(kBlock) void k_comp_exact(
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+    const double* __restrict__ m, double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
+    int N, int N_active, int tp_type, int ignore, double G, double soft2) {
+    __shared__ double4 tile[kBlock];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const bool live = i < N;
+    const int blk_lo = blockIdx.x * kBlock, blk_hi = blk_lo + kBlock;
+    const double xi = live ? x[i] : 0., yi = live ? y[i] : 0., zi = live ? z[i] : 0.;
+    double a_x = 0., a_y = 0., a_z = 0., c_x = 0., c_y = 0., c_z = 0.;
+    for (int pass = 0; pass < 2; pass++) {
+        int j0 = pass == 0 ? 0 : N_active;
+        int j1 = pass == 0 ? N_active : N;
+        if (pass == 1 && !tp_type) break;
+        for (int t = j0; t < j1; t += kBlock) {
+            __syncthreads();
+            const int jj = t + (int)threadIdx.x;
+            if (jj < j1) tile[threadIdx.x] = make_double4(x[jj], y[jj], z[jj], m[jj]);
+            __syncthreads();
+            const int n = min(kBlock, j1 - t);
+            if (!live || (pass == 1 && i >= N_active)) continue;
+            auto term = [&](const double4 s) {
+                const double dx = xi - s.x;
+                const double dy = yi - s.y;
+                const double dz = zi - s.z;
+                const double r2 = dx * dx + dy * dy + dz * dz + soft2;
+                const double rr = sqrt(r2);
+                const double prefact = G / (r2 * rr);
+                const double prefactj = -prefact * s.w;
+                kahan_add(prefactj * dx, a_x, c_x);
+                kahan_add(prefactj * dy, a_y, c_y);
+                kahan_add(prefactj * dz, a_z, c_z);
+            };
+            const bool check = (t < blk_hi && t + n > blk_lo) || (ignore != 0 && (t < 2 || blk_lo < 2));
+            if (!check) {
+#pragma unroll 4
+                for (int k = 0; k < n; k++) term(tile[k]);
+                continue;
+            }
+            for (int k = 0; k < n; k++) {
+                const int j = t + k;
+                if (skip_pair(i, j, ignore)) continue;
+                term(tile[k]);
+            }
+        }
+    }
+    if (live) {
+        ax[i] = a_x;
+        ay[i] = a_y;
+        az[i] = a_z;
+    }
+}
+
 template <bool COMP>
 static void launch_exact(hipStream_t st, const double* x, const double* y, const double* z, const double* m,
                          double* ax, double* ay, double* az, int N, int N_active, int tp_type, int ignore, double G,
                          double soft2, const GBList& gl) {
-    // Targets per workgroup: as many as keep >= 1024 workgroups (four per
-    // CU), so that small N still fills the chip; the source tile widens to
-    // keep one term per thread.
+    // Measured on gfx950 (shapes 16x16 .. 1x256 targets x sources, N = 256
+    // .. 65536): 8 targets x 64 sources is within 10 % of the best shape at
+    // every N below ~50k; the row-wise kernel wins above.
 #define RB_EXACT(TGT, TILE)                                                                                           \
     hipLaunchKernelGGL((k_exact<COMP, TGT, TILE>), dim3((N + TGT - 1) / TGT), dim3(kExThreads), 0, st, x, y, z, m,   \
                        ax, ay, az, N, N_active, tp_type, ignore, G, soft2, gl)
-    if (N >= 16 * 1024)
-        RB_EXACT(16, 64);
-    else if (N >= 4 * 1024)
-        RB_EXACT(4, 64);
-    else if (N >= 2 * 1024)
-        RB_EXACT(2, 128);
-    else
-        RB_EXACT(1, 256);
+    if (N >= kExactRowwiseN) {
+        if (COMP)
+            hipLaunchKernelGGL(k_comp_exact, dim3((N + kBlock - 1) / kBlock), dim3(kBlock), 0, st, x, y, z, m, ax,
+                               ay, az, N, N_active, tp_type, ignore, G, soft2);
+        else
+            hipLaunchKernelGGL(k_basic_exact, dim3((N + kBlock - 1) / kBlock), dim3(kBlock), 0, st, x, y, z, m, ax,
+                               ay, az, N, N_active, tp_type, ignore, G, soft2, gl);
+    } else {
+        RB_EXACT(8, 64);
+    }
 #undef RB_EXACT
 }
 
@@ -350,7 +484,8 @@ __device__ __forceinline__ void direct_fast_block(const double* __restrict__ x, 
             __syncthreads();
             const int n = min(kFast, j1 - t);
             // Only tiles holding a target of this block, or sources 0/1 when
-            // terms are ignored, need the per-pair skip test.
+            // terms are ignored, need the per-pair skip test (target 0 with
+            // ignore == 2 is zeroed by the reduction).
             const bool check = (t < blk_hi && t + n > blk_lo) || (ignore != 0 && t < 2);
             if (MODE != FM_COMP) {
                 // plain FP64 sum over the tile, folded into the running sum with Kahan

@@ -197,6 +197,14 @@ void whfast_synchronize_device(struct reb_simulation* r, struct reb_integrator_w
 void whfast_release(struct reb_simulation* r);
 void tree_release(struct reb_simulation* r);
 void resolve_release(struct reb_simulation* r);
+// The built-in hard-sphere resolve with a constant restitution runs on compact
+// records the device prepares right after a search (collision_host.hip):
+// compact_resolve_mark numbers the bodies of the nc device records; after the
+// next stream synchronization compact_resolve_gather packs them and starts
+// their copy to the host.  False on an allocation failure.
+bool compact_resolve_eligible(const struct reb_simulation* r);
+bool compact_resolve_mark(struct reb_simulation* r, const CollRecH* rec, size_t nc);
+bool compact_resolve_gather(struct reb_simulation* r, const CollRecH* rec, size_t nc);
 // Writes r->collisions from the last device search (detection order) when it
 // is still pending; false on an allocation failure.
 bool materialize_collisions(struct reb_simulation* r);

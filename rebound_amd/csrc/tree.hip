@@ -742,15 +742,17 @@ struct CollRec {
     int p1, p2, gb, ri;
 };
 
-// Records kept per body by the single-pass search; a body with more
-// collisions sends the search through a second, writing pass.
+// Records kept per body by the single-pass search; the bodies with more
+// collisions are searched again by a pass that walks only them.
 static constexpr int kCollSlots = 8;
 
 // Tree search of collision.c:197-269 + :422-503 (TREE) or :270-331 +
 // :506-566 (LINETREE, LINE = true), in detection order per body (ghost box,
 // root box, depth-first).  MODE 0 counts the collisions of each body; MODE 1
 // stores them at the body's offset; MODE 2 counts them and keeps the first
-// kCollSlots in the body's slots, raising *overflow when a body has more.
+// kCollSlots in the body's slots, raising *overflow when a body has more;
+// MODE 3 stores at their offsets the collisions of the bodies whose count
+// exceeds kCollSlots, and a wave without such a body returns at once.
 // TREE passes the second largest radius as `reach`; LINETREE the largest
 // drift of the last step, dt_last_done*|v|max.
 template <int MODE, bool LINE = false>
@@ -763,10 +765,11 @@ __global__ __launch_bounds__(64) void k_tree_collide(const CollCell* __restrict_
                                                      int* __restrict__ counts, const int* __restrict__ offsets,
                                                      CollRec* __restrict__ rec, double dt_last = 0.,
                                                      int* __restrict__ overflow = nullptr) {
-    constexpr bool WRITE = MODE == 1;
+    constexpr bool WRITE = MODE == 1 || MODE == 3;
     const int s = blockIdx.x * 64 + threadIdx.x;
-    const bool live = s < n;
-    const int me = live ? idx[s] : -1;
+    const int me = s < n ? idx[s] : -1;
+    const bool live = s < n && (MODE != 3 || counts[me] > kCollSlots);
+    if (MODE == 3 && __ballot(live) == 0ull) return;   // one wave per block: uniform exit
     const double px = live ? x[me] : 0., py = live ? y[me] : 0., pz = live ? z[me] : 0.;
     const double pvx = live ? vx[me] : 0., pvy = live ? vy[me] : 0., pvz = live ? vz[me] : 0.;
     const double p1_r = live ? rad[me] : 0.;
@@ -776,7 +779,7 @@ __global__ __launch_bounds__(64) void k_tree_collide(const CollCell* __restrict_
     int out = (WRITE && live) ? offsets[me] : 0;
     CollRec* const slot = MODE == 2 && live ? rec + (size_t)me * kCollSlots : nullptr;
     auto emit = [&](int pt, int g, int ri) {
-        if (MODE == 1) rec[out++] = CollRec{me, pt, g, ri};
+        if (WRITE) rec[out++] = CollRec{me, pt, g, ri};
         if (MODE == 2 && found < kCollSlots) slot[found] = CollRec{me, pt, g, ri};
         found++;
     };
@@ -833,17 +836,18 @@ __global__ __launch_bounds__(64) void k_tree_collide(const CollCell* __restrict_
             cur = any ? r_open : r_skip;
         }
     }
-    if (!WRITE && live) counts[me] = found;
+    if ((MODE == 0 || MODE == 2) && live) counts[me] = found;
     if (MODE == 2 && live && found > kCollSlots) *overflow = 1;   // same value from every writer
 }
 
-// The slot records of every body to its offset (bodies with at most
-// kCollSlots collisions: the single-pass case).
+// The slot records of every body with at most kCollSlots collisions to its
+// offset; the others are written by k_tree_collide<3>.
 __global__ void k_coll_compact(const int* __restrict__ counts, const int* __restrict__ offsets,
                                const CollRec* __restrict__ slots, int n, CollRec* __restrict__ rec) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int c = counts[i], o = offsets[i];
+    if (c > kCollSlots) return;
     for (int k = 0; k < c; k++) rec[o + k] = slots[(size_t)i * kCollSlots + k];
 }
 
@@ -977,6 +981,9 @@ static bool fetch_collisions(struct reb_simulation* r, TreeBuffers* T, long tota
     static_assert(sizeof(CollRecH) == sizeof(CollRec), "host and device collision records");
     if (!RB_HIP(r, hipMemcpyAsync(L.rec, T->coll_rec.p, sizeof(CollRec) * total, hipMemcpyDeviceToHost, d->stream)))
         return false;
+    const CollRecH* drec = T->coll_rec.as<CollRecH>();
+    const bool compact = compact_resolve_eligible(r);
+    if (compact && !compact_resolve_mark(r, drec, (size_t)total)) return false;
     L.gb.resize(gl.n);
     for (int k = 0; k < gl.n; k++)
         L.gb[k] = reb_vec6d{gl.g[k][0], gl.g[k][1], gl.g[k][2], gl.g[k][3], gl.g[k][4], gl.g[k][5]};
@@ -984,6 +991,7 @@ static bool fetch_collisions(struct reb_simulation* r, TreeBuffers* T, long tota
     L.n = (size_t)total;
     L.pending = true;
     r->N_collisions = (size_t)total;
+    if (compact && !compact_resolve_gather(r, drec, (size_t)total)) return false;
     return true;
 }
 
@@ -1025,7 +1033,7 @@ void coll_release(struct reb_simulation* r) {
 
 // TREE / LINETREE search into T->coll_rec in detection order; returns the
 // number of collisions or -1.  One pass keeps up to kCollSlots records per
-// body; only when a body has more does a second pass write them all.
+// body; when a body has more, a second pass searches again for those bodies.
 template <bool LINE>
 static long tree_collide_run(struct reb_simulation* r, TreeBuffers* T, int n, const GBT& gl, double reach,
                              double dt_last) {
@@ -1048,15 +1056,13 @@ static long tree_collide_run(struct reb_simulation* r, TreeBuffers* T, int n, co
     if (total < 0) return -1;
     if (!ensure_coll(r, T, n, (size_t)total + 1)) return -1;
     if (total == 0) return 0;
-    if (!hover) {
-        hipLaunchKernelGGL(k_coll_compact, dim3((n + 255) / 256), dim3(256), 0, d->stream, T->coll_cnt.as<int>(),
-                           T->coll_off.as<int>(), T->coll_slot.as<CollRec>(), n, T->coll_rec.as<CollRec>());
-    } else {
-        hipLaunchKernelGGL((k_tree_collide<1, LINE>), grid, dim3(64), 0, d->stream, T->crec.as<CollCell>(),
+    hipLaunchKernelGGL(k_coll_compact, dim3((n + 255) / 256), dim3(256), 0, d->stream, T->coll_cnt.as<int>(),
+                       T->coll_off.as<int>(), T->coll_slot.as<CollRec>(), n, T->coll_rec.as<CollRec>());
+    if (hover)
+        hipLaunchKernelGGL((k_tree_collide<3, LINE>), grid, dim3(64), 0, d->stream, T->crec.as<CollCell>(),
                            T->nnodes, T->idx.as<int>(), n, d->field(FX), d->field(FY), d->field(FZ), d->field(FVX),
-                           d->field(FVY), d->field(FVZ), d->field(FR), gl, reach, nullptr, T->coll_off.as<int>(),
-                           T->coll_rec.as<CollRec>(), dt_last, nullptr);
-    }
+                           d->field(FVY), d->field(FVZ), d->field(FR), gl, reach, T->coll_cnt.as<int>(),
+                           T->coll_off.as<int>(), T->coll_rec.as<CollRec>(), dt_last, nullptr);
     // root index of each record (the lcp buffer is free now: reuse as pos_of)
     int* pos_of = T->lcp.as<int>();
     hipLaunchKernelGGL(k_inverse_perm, dim3((n + 255) / 256), dim3(256), 0, d->stream, T->idx.as<int>(), pos_of, n);

@@ -79,6 +79,27 @@ def test_exact_bitwise_larger_n(gpu, orc, grav, n):
         assert np.array_equal(acc, ref), (grav, N_active, tp, ign)
 
 
+@pytest.mark.parametrize("grav", ["basic", "compensated"])
+def test_exact_bitwise_row_kernel_large_n(gpu, orc, grav):
+    """N=50000 takes the row-wise exact kernels; sampled target rows against
+    the restatement, with test particles and ignored terms."""
+    n = 50000
+    st = orc.plummer(n, 9)[0]
+    rng = np.random.default_rng(4)
+    for N_active, tp, ign in ((n, 0, 0), (n - 777, 1, 1), (n // 2, 1, 2)):
+        rows = np.unique(np.concatenate([[0, 1, 2, N_active - 1, min(N_active, n - 1), n - 1],
+                                         rng.integers(0, n, 40)]))
+        sim = make_sim(gpu, st, 0.005, "exact")
+        sim.s.gravity = gpu.GRAVITY[grav]
+        sim.s.N_active = N_active
+        sim.s.testparticle_type = tp
+        sim.s.gravity_ignore_terms = ign
+        acc = sim.update_acceleration()
+        ref = orc.gravity_rows(st, rows, soft=0.005, N_active=N_active, tp_type=tp, ignore=ign,
+                               compensated=1 if grav == "compensated" else 0)
+        assert np.array_equal(acc[rows], ref), (grav, N_active, tp, ign)
+
+
 def test_exact_bitwise_ghostboxes(gpu):
     st = golden("periodic256_state.npy")
     sim = make_sim(gpu, st, 0.05, "exact")

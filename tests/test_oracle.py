@@ -231,6 +231,19 @@ def test_whfast_jacobi_corrector_improves_energy(orc):
     assert errs[1] < 0.5 * errs[0], errs
 
 
+@pytest.mark.parametrize("comp", [0, 1])
+def test_gravity_rows_equal_full_sums(orc, comp):
+    """The row-sampled restatement gives the full restatement's rows bit for
+    bit, with test particles and ignored terms (it checks large N on the GPU)."""
+    st = orc.plummer(3001, 7)[0]
+    rows = [0, 1, 5, 2249, 2250, 2501, 2502, 2999, 3000]
+    f = orc.gravity_compensated if comp else orc.gravity_basic
+    for NA, tp, ign in ((3001, 0, 0), (2502, 1, 1), (2250, 1, 2), (2250, 0, 1)):
+        full = f(st, soft=0.005, N_active=NA, tp_type=tp, ignore=ign)
+        got = orc.gravity_rows(st, rows, soft=0.005, N_active=NA, tp_type=tp, ignore=ign, compensated=comp)
+        assert np.array_equal(full[rows], got), (NA, tp, ign)
+
+
 # ------------------------------------------------------------------ tree and collisions
 SHEET = dict(root_size=50.0, nr=(2, 2, 1))
 
