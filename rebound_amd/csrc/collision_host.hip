@@ -18,11 +18,14 @@
 // touch disjoint bodies and are resolved in parallel on host threads; every
 // body still sees its collisions in list order, and the pressure log is summed
 // in list order afterwards, so the result is the sequential one, bit for bit.
-// Right after the search the device numbers the colliding bodies, rewrites
-// the records with those numbers and packs the eight fields the resolver
-// reads into 64-byte records; both travel to pinned host memory while the
-// host draws the shuffle, the resolver works on the records in place, and
-// they travel back for a scatter of the velocities.
+// Right after the search the device numbers the colliding bodies and packs
+// the eight fields the resolver reads into 64-byte records, which travel to
+// pinned host memory while the host draws the shuffle.  The shuffle goes to
+// the device, which writes the shuffled list twice: in compact body numbers
+// for the resolve, and as the reference's struct list straight into
+// r->collisions (pinned), a copy that runs while the host resolves.  The
+// resolver works on the body records in place; they travel back for a
+// scatter of the velocities.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -227,12 +230,28 @@ __global__ void k_coll_gather(const double* __restrict__ soa, size_t cap, const 
                     soa[(size_t)FM * cap + p],  soa[(size_t)FR * cap + p]};
 }
 
-__global__ void k_coll_pairs(const CollRecH* __restrict__ rec, int nc, const int* __restrict__ cidx,
-                             HSPair* __restrict__ out) {
+// Ghost boxes of a collision search (the inner ring: at most 3 x 3 x 3).
+static constexpr int kCollGB = 27;
+struct CollGB {
+    int n;
+    struct reb_vec6d g[kCollGB];
+};
+
+// The shuffled list: entry k is detection record perm[k], in compact body
+// numbers and as the reference's struct.
+__global__ void k_coll_shuffled(const CollRecH* __restrict__ rec, const int* __restrict__ perm, int nc,
+                                const int* __restrict__ cidx, CollGB gb, HSPair* __restrict__ cl,
+                                struct reb_collision* __restrict__ out) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nc) return;
-    const CollRecH h = rec[k];
-    out[k] = HSPair{cidx[h.p1], cidx[h.p2], h.gb};
+    const CollRecH h = rec[perm[k]];
+    cl[k] = HSPair{cidx[h.p1], cidx[h.p2], h.gb};
+    struct reb_collision c;
+    c.p1 = (size_t)h.p1;
+    c.p2 = (size_t)h.p2;
+    c.gb = gb.g[h.gb];
+    c.ri = (size_t)h.ri;
+    out[k] = c;
 }
 
 __global__ void k_coll_scatter_v(double* __restrict__ soa, size_t cap, const int* __restrict__ ids, int n,
@@ -319,27 +338,32 @@ static int rand_r_step(unsigned int* seed) {
 // largest step so far).  `ready` is set when the device has prepared the
 // compact records of the last search.
 struct rbamd::ResolveScratch {
-    DevBuf flag, cidx, ids, body, pair, cub;      // device
-    HSBody* body_h = nullptr;                     // pinned host
+    DevBuf flag, cidx, ids, body, pair, perm, coll, cub;   // device
+    HSBody* body_h = nullptr;                              // pinned host
     HSPair* pair_h = nullptr;
+    int* perm_h = nullptr;
     int* n_h = nullptr;
-    size_t body_cap = 0, pair_cap = 0;
+    size_t body_cap = 0, pair_cap = 0, perm_cap = 0;
+    hipEvent_t ev_pairs = nullptr, ev_coll = nullptr;
+    const CollRecH* drec = nullptr;   // device records of the prepared search
     int n = 0;
     bool ready = false;
-    std::vector<int> level, last, perm;
-    std::vector<size_t> start, order, fill;
-    std::vector<HSPair> cl;
+    std::vector<int> level, last;
+    std::vector<size_t> start, order;
     std::vector<double> plog;
     std::vector<unsigned char> hit;
     void release() {
-        for (DevBuf* b : {&flag, &cidx, &ids, &body, &pair, &cub}) b->release();
-        if (body_h) (void)hipHostFree(body_h);
-        if (pair_h) (void)hipHostFree(pair_h);
-        if (n_h) (void)hipHostFree(n_h);
+        for (DevBuf* b : {&flag, &cidx, &ids, &body, &pair, &perm, &coll, &cub}) b->release();
+        for (void* p : {(void*)body_h, (void*)pair_h, (void*)perm_h, (void*)n_h})
+            if (p) (void)hipHostFree(p);
+        if (ev_pairs) (void)hipEventDestroy(ev_pairs);
+        if (ev_coll) (void)hipEventDestroy(ev_coll);
         body_h = nullptr;
         pair_h = nullptr;
+        perm_h = nullptr;
         n_h = nullptr;
-        body_cap = pair_cap = 0;
+        ev_pairs = ev_coll = nullptr;
+        body_cap = pair_cap = perm_cap = 0;
     }
 };
 
@@ -352,9 +376,9 @@ void resolve_release(struct reb_simulation* r) {
     d->resolve = nullptr;
 }
 
-bool compact_resolve_eligible(const struct reb_simulation* r) {
+bool compact_resolve_eligible(const struct reb_simulation* r, int n_ghostboxes) {
     const DeviceState* d = dev_of(r);
-    return r->collision_resolve == reb_collision_resolve_hardsphere &&
+    return n_ghostboxes <= kCollGB && r->collision_resolve == reb_collision_resolve_hardsphere &&
            (r->coefficient_of_restitution == nullptr ||
             r->coefficient_of_restitution == reb_amd_coefficient_of_restitution_constant) &&
            d->loc != LOC_HOST;
@@ -400,24 +424,46 @@ bool compact_resolve_gather(struct reb_simulation* r, const CollRecH* rec, size_
     const int N = (int)r->N;
     const int n = *W.n_h;   // the stream has been synchronized since compact_resolve_mark
     if (!W.ids.ensure(sizeof(int) * n + 256) || !W.body.ensure(sizeof(HSBody) * n + 256) ||
-        !W.pair.ensure(sizeof(HSPair) * nc + 256)) {
+        !W.pair.ensure(sizeof(HSPair) * nc + 256) || !W.perm.ensure(sizeof(int) * nc + 256) ||
+        !W.coll.ensure(sizeof(struct reb_collision) * nc + 256)) {
         reb_simulation_error(r, "Out of device memory for the collision resolve.");
         return false;
     }
-    if (!pinned_ensure(r, W.body_h, W.body_cap, (size_t)n) || !pinned_ensure(r, W.pair_h, W.pair_cap, nc))
+    if (!pinned_ensure(r, W.body_h, W.body_cap, (size_t)n) || !pinned_ensure(r, W.pair_h, W.pair_cap, nc) ||
+        !pinned_ensure(r, W.perm_h, W.perm_cap, nc) || !collisions_reserve(r, nc))
         return false;
+    if (!W.ev_pairs && !RB_HIP(r, hipEventCreateWithFlags(&W.ev_pairs, hipEventDisableTiming))) return false;
+    if (!W.ev_coll && !RB_HIP(r, hipEventCreateWithFlags(&W.ev_coll, hipEventDisableTiming))) return false;
     hipLaunchKernelGGL(k_coll_gather, dim3((N + 255) / 256), dim3(256), 0, d->stream, d->soa.as<double>(), d->cap,
                        W.flag.as<int>(), W.cidx.as<int>(), N, W.ids.as<int>(), W.body.as<HSBody>());
-    hipLaunchKernelGGL(k_coll_pairs, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, d->stream, rec, (int)nc,
-                       W.cidx.as<int>(), W.pair.as<HSPair>());
     if (!RB_HIP(r, hipGetLastError())) return false;
-    if (!RB_HIP(r, hipMemcpyAsync(W.pair_h, W.pair.p, sizeof(HSPair) * nc, hipMemcpyDeviceToHost, d->stream)))
-        return false;
     if (!RB_HIP(r, hipMemcpyAsync(W.body_h, W.body.p, sizeof(HSBody) * n, hipMemcpyDeviceToHost, d->stream)))
         return false;
+    W.drec = rec;
     W.n = n;
     W.ready = true;
     return true;
+}
+
+bool collisions_reserve(struct reb_simulation* r, size_t nc) {
+    if (r->N_allocated_collisions >= nc && r->collisions) return true;
+    size_t cap = r->N_allocated_collisions ? r->N_allocated_collisions : 32;
+    while (cap < nc) cap *= 2;
+    struct reb_collision* p = nullptr;
+    if (hipHostMalloc((void**)&p, sizeof(struct reb_collision) * cap, hipHostMallocDefault) != hipSuccess || !p) {
+        reb_simulation_error(r, "Out of memory for the collision list.");
+        return false;
+    }
+    if (r->collisions) (void)hipHostFree(r->collisions);
+    r->collisions = p;
+    r->N_allocated_collisions = cap;
+    return true;
+}
+
+void collisions_free(struct reb_simulation* r) {
+    if (r->collisions) (void)hipHostFree(r->collisions);
+    r->collisions = nullptr;
+    r->N_allocated_collisions = 0;
 }
 
 // The built-in hard-sphere resolver with a constant restitution, on the
@@ -433,82 +479,89 @@ static void resolve_hardsphere_compact(struct reb_simulation* r) {
     const int n = W.n;
     // shuffle (collision.c:336-342) as a permutation of the detection order,
     // the same swaps applied to indices instead of 72-byte records; drawn
-    // while the compact records are on their way
-    std::vector<int>& perm = W.perm;
-    perm.resize(nc);
+    // while the body records are on their way
+    int* perm = W.perm_h;
     for (size_t k = 0; k < nc; k++) perm[k] = (int)k;
     for (size_t i = 0; i < nc; i++) {
         const size_t nw = (size_t)rand_r_step(&r->rand_seed) % nc;
         std::swap(perm[i], perm[nw]);
     }
-    if (!stream_sync(r)) return;
-    // the shuffled list in compact numbers, and its levels: one more than the
-    // last level that touched either body
-    auto& cl = W.cl;
+    CollGB gb;
+    gb.n = (int)L.gb.size();
+    for (int k = 0; k < gb.n; k++) gb.g[k] = L.gb[k];
+    if (!RB_HIP(r, hipMemcpyAsync(W.perm.p, perm, sizeof(int) * nc, hipMemcpyHostToDevice, d->stream))) return;
+    hipLaunchKernelGGL(k_coll_shuffled, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, d->stream, W.drec,
+                       W.perm.as<int>(), (int)nc, W.cidx.as<int>(), gb, W.pair.as<HSPair>(),
+                       W.coll.as<struct reb_collision>());
+    if (!RB_HIP(r, hipGetLastError())) return;
+    if (!RB_HIP(r, hipMemcpyAsync(W.pair_h, W.pair.p, sizeof(HSPair) * nc, hipMemcpyDeviceToHost, d->stream)) ||
+        !RB_HIP(r, hipEventRecord(W.ev_pairs, d->stream)))
+        return;
+    // r->collisions as the reference leaves it (the shuffled list), copied
+    // while the host resolves
+    if (!RB_HIP(r, hipMemcpyAsync(r->collisions, W.coll.p, sizeof(struct reb_collision) * nc, hipMemcpyDeviceToHost,
+                                  d->stream)) ||
+        !RB_HIP(r, hipEventRecord(W.ev_coll, d->stream)))
+        return;
+    if (!RB_HIP(r, hipEventSynchronize(W.ev_pairs))) return;
+    // levels: one more than the last level that touched either body, counted
+    // per level on the way
+    const HSPair* cl = W.pair_h;
     std::vector<int>& level = W.level;
     std::vector<int>& last = W.last;
-    cl.resize(nc);
+    std::vector<size_t>& start = W.start;
     level.resize(nc);
     last.assign(n, 0);
+    start.assign(64, 0);
     int nlev = 0;
     for (size_t k = 0; k < nc; k++) {
-        const HSPair q = W.pair_h[perm[k]];
-        cl[k] = q;
+        if (k + 16 < nc) {
+            __builtin_prefetch(&last[cl[k + 16].p1]);
+            __builtin_prefetch(&last[cl[k + 16].p2]);
+        }
+        const HSPair q = cl[k];
         const int l = std::max(last[q.p1], last[q.p2]) + 1;
         level[k] = l;
         last[q.p1] = l;
         last[q.p2] = l;
         nlev = std::max(nlev, l);
+        if ((size_t)l + 1 >= start.size()) start.resize(2 * ((size_t)l + 1), 0);
+        start[l + 1]++;
     }
-    std::vector<size_t>& start = W.start;
-    std::vector<size_t>& order = W.order;
-    std::vector<size_t>& fill = W.fill;
-    start.assign(nlev + 2, 0);
-    order.resize(nc);
-    for (size_t k = 0; k < nc; k++) start[level[k] + 1]++;
+    start.resize(nlev + 2);
     for (int l = 1; l <= nlev + 1; l++) start[l] += start[l - 1];
-    fill.assign(start.begin(), start.end());
-    for (size_t k = 0; k < nc; k++) order[fill[level[k]]++] = k;
-    // r->collisions as the reference leaves it: the shuffled list, written
-    // by the same pass that resolves
-    if (r->N_allocated_collisions < nc) {
-        size_t cap = r->N_allocated_collisions ? r->N_allocated_collisions : 32;
-        while (cap < nc) cap *= 2;
-        auto* pc = (struct reb_collision*)realloc(r->collisions, sizeof(struct reb_collision) * cap);
-        if (!pc) {
-            reb_simulation_error(r, "Out of memory for the collision list.");
-            return;
-        }
-        r->collisions = pc;
-        r->N_allocated_collisions = cap;
+    std::vector<size_t>& order = W.order;
+    order.resize(nc);
+    {
+        std::vector<size_t> fill(start.begin(), start.end());
+        for (size_t k = 0; k < nc; k++) order[fill[level[k]]++] = k;
     }
+    // plog starts at -0.0, the exact identity of the sum below, so that the
+    // collisions that do not bounce need no branch
     std::vector<double>& plog = W.plog;
     std::vector<unsigned char>& hit = W.hit;
-    plog.assign(nc, 0.);
+    plog.assign(nc, -0.);
     hit.assign(nc, 0);
     HSBody* B = W.body_h;
     HostPool::get().run_levels(nlev, start.data(), order.data(), [&](size_t k) {
         hit[k] = (unsigned char)hardsphere_apply(r, B, (size_t)cl[k].p1, (size_t)cl[k].p2, L.gb[cl[k].gb], &plog[k]);
-        const CollRecH& hr = L.rec[perm[k]];
-        struct reb_collision c;
-        c.p1 = (size_t)hr.p1;
-        c.p2 = (size_t)hr.p2;
-        c.gb = L.gb[hr.gb];
-        c.ri = (size_t)hr.ri;
-        r->collisions[k] = c;
     });
     // the pressure log in list order (collision.c:655-663)
+    double sum = r->collisions_plog;
+    long nhit = 0;
     for (size_t k = 0; k < nc; k++) {
-        if (!hit[k]) continue;
-        r->collisions_plog += plog[k];
-        r->collisions_log_n++;
+        sum += plog[k];
+        nhit += hit[k];
     }
+    r->collisions_plog = sum;
+    r->collisions_log_n += nhit;
     // the records go back whole; the scatter takes the velocities.  The next
     // search's copy into body_h is ordered after this one on the stream.
     if (!RB_HIP(r, hipMemcpyAsync(W.body.p, B, sizeof(HSBody) * n, hipMemcpyHostToDevice, d->stream))) return;
     hipLaunchKernelGGL(k_coll_scatter_v, dim3((n + 255) / 256), dim3(256), 0, d->stream, d->soa.as<double>(), d->cap,
                        W.ids.as<int>(), n, W.body.as<HSBody>());
     (void)RB_HIP(r, hipGetLastError());
+    (void)RB_HIP(r, hipEventSynchronize(W.ev_coll));   // r->collisions is complete
     d->coll->pending = false;
     mark_device_newer(r);
 }
@@ -519,8 +572,8 @@ void collision_resolve_host(struct reb_simulation* r) {
         r->collision_resolve;
     if (resolve == nullptr) resolve = reb_collision_resolve_halt;
     DeviceState* d = dev_of(r);
-    if (d->resolve && d->resolve->ready && compact_resolve_eligible(r) && d->coll && d->coll->pending &&
-        d->coll->n == r->N_collisions) {
+    if (d->resolve && d->resolve->ready && d->coll && d->coll->pending && d->coll->n == r->N_collisions &&
+        compact_resolve_eligible(r, (int)d->coll->gb.size())) {
         resolve_hardsphere_compact(r);   // shuffles, resolves and writes r->collisions
         return;
     }

@@ -232,7 +232,7 @@ REB_API void reb_simulation_free(struct reb_simulation* const r) {
     if (r->free_particle_ap)
         for (size_t i = 0; i < r->N; i++) r->free_particle_ap(&r->particles[i]);
     free(r->particles);
-    free(r->collisions);
+    collisions_free(r);
     free(r->gravity_cs);
     free(r->simulationarchive_filename);
     SimBlock* b = reinterpret_cast<SimBlock*>(r);

@@ -82,6 +82,7 @@ struct CollList {
     size_t cap = 0, n = 0;
     std::vector<reb_vec6d> gb;
     bool pending = false;     // r->collisions not yet written from rec
+    bool rec_host = false;    // rec holds the records (else only the device does)
 };
 struct WHFastDevice;         // whfast.hip
 
@@ -202,9 +203,13 @@ void resolve_release(struct reb_simulation* r);
 // compact_resolve_mark numbers the bodies of the nc device records; after the
 // next stream synchronization compact_resolve_gather packs them and starts
 // their copy to the host.  False on an allocation failure.
-bool compact_resolve_eligible(const struct reb_simulation* r);
+bool compact_resolve_eligible(const struct reb_simulation* r, int n_ghostboxes);
 bool compact_resolve_mark(struct reb_simulation* r, const CollRecH* rec, size_t nc);
 bool compact_resolve_gather(struct reb_simulation* r, const CollRecH* rec, size_t nc);
+// r->collisions lives in pinned host memory (the device copies the shuffled
+// list into it): grown to at least nc entries (contents not kept), and freed.
+bool collisions_reserve(struct reb_simulation* r, size_t nc);
+void collisions_free(struct reb_simulation* r);
 // Writes r->collisions from the last device search (detection order) when it
 // is still pending; false on an allocation failure.
 bool materialize_collisions(struct reb_simulation* r);

@@ -979,10 +979,14 @@ static bool fetch_collisions(struct reb_simulation* r, TreeBuffers* T, long tota
         L.cap = cap;
     }
     static_assert(sizeof(CollRecH) == sizeof(CollRec), "host and device collision records");
-    if (!RB_HIP(r, hipMemcpyAsync(L.rec, T->coll_rec.p, sizeof(CollRec) * total, hipMemcpyDeviceToHost, d->stream)))
-        return false;
+    // the compact resolve reads the records on the device; any other resolve
+    // reads them on the host
     const CollRecH* drec = T->coll_rec.as<CollRecH>();
-    const bool compact = compact_resolve_eligible(r);
+    const bool compact = compact_resolve_eligible(r, gl.n);
+    L.rec_host = !compact;
+    if (!compact &&
+        !RB_HIP(r, hipMemcpyAsync(L.rec, T->coll_rec.p, sizeof(CollRec) * total, hipMemcpyDeviceToHost, d->stream)))
+        return false;
     if (compact && !compact_resolve_mark(r, drec, (size_t)total)) return false;
     L.gb.resize(gl.n);
     for (int k = 0; k < gl.n; k++)
@@ -999,17 +1003,14 @@ bool materialize_collisions(struct reb_simulation* r) {
     DeviceState* d = dev_of(r);
     if (!d->coll || !d->coll->pending) return true;
     CollList& L = *d->coll;
-    if (r->N_allocated_collisions < L.n) {
-        size_t cap = r->N_allocated_collisions ? r->N_allocated_collisions : 32;
-        while (cap < L.n) cap *= 2;
-        auto* p = (struct reb_collision*)realloc(r->collisions, sizeof(struct reb_collision) * cap);
-        if (!p) {
-            reb_simulation_error(r, "Out of memory for the collision list.");
+    if (!L.rec_host) {   // prepared for the compact resolve: the records are still on the device
+        if (!RB_HIP(r, hipMemcpyAsync(L.rec, d->tree->coll_rec.p, sizeof(CollRecH) * L.n, hipMemcpyDeviceToHost,
+                                      d->stream)) ||
+            !stream_sync(r))
             return false;
-        }
-        r->collisions = p;
-        r->N_allocated_collisions = cap;
+        L.rec_host = true;
     }
+    if (!collisions_reserve(r, L.n)) return false;
     for (size_t k = 0; k < L.n; k++) {
         const CollRecH& h = L.rec[k];
         struct reb_collision c;

@@ -121,14 +121,25 @@ def _run_steps(gpu, collision, record):
 def test_sheet_five_steps_bitwise(gpu, record):
     sim, rec = _run_steps(gpu, "tree", record)
     counts = golden("sheet2000_collision_counts.npy")
+    pairs = golden("sheet2000_collision_pairs.npy")
+    gb = golden("sheet2000_collision_gb.npy")
+    off = np.concatenate([[0], np.cumsum(counts)])
     for s in range(5):
         sim.steps(1)
         ref = golden("sheet2000_step%d_state.npy" % (s + 1))
         assert np.array_equal(sim.state()[:, :6], ref[:, :6]), s
+        # r->collisions holds the step's list in resolve order, as the
+        # reference leaves it
+        nc = sim.s.N_collisions
+        assert nc == counts[s]
+        c = sim.s.collisions
+        got = np.array([(c[k].p1, c[k].p2) for k in range(nc)], dtype=np.int64).reshape(-1, 2)
+        ggb = np.array([(c[k].gb.x, c[k].gb.y, c[k].gb.z, c[k].gb.vx, c[k].gb.vy, c[k].gb.vz)
+                        for k in range(nc)]).reshape(-1, 6)
+        assert np.array_equal(got, pairs[off[s]:off[s + 1]]), s
+        assert np.array_equal(ggb, gb[off[s]:off[s + 1]]), s
     assert sim.s.collisions_log_n > 0
     if record is True:
-        pairs = golden("sheet2000_collision_pairs.npy")
-        gb = golden("sheet2000_collision_gb.npy")
         got = np.array(rec)
         assert got.shape[0] == counts.sum()
         assert np.array_equal(got[:, :2].astype(np.int64), pairs)
