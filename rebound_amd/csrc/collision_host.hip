@@ -92,32 +92,44 @@ class HostPool {
     }
     // fn(order[q]) for the items q of level 1, then of level 2, ... (level l
     // holds q in [start[l], start[l+1])): items of one level in parallel, a
-    // barrier between levels, all levels in one dispatch of the pool.
-    void run_levels(int nlev, const size_t* start, const size_t* order, const std::function<void(size_t)>& fn) {
-        const size_t total = start[nlev + 1] - start[1];
-        if (threads_.empty() || total < 512) {
-            for (size_t q = start[1]; q < start[nlev + 1]; q++) fn(order[q]);
+    // barrier between levels, all levels in one dispatch of the pool.  The
+    // last levels, once fewer than kTail items remain, run on one thread in
+    // level order (a barrier costs more than their items).
+    void run_levels(int nlev, const int* start, const int* order, const std::function<void(size_t)>& fn) {
+        constexpr int kTail = 1024;
+        const int total = start[nlev + 1] - start[1];
+        if (threads_.empty() || total < 2 * kTail) {
+            for (int q = start[1]; q < start[nlev + 1]; q++) fn((size_t)order[q]);
             return;
         }
+        int par = nlev;   // levels 1..par in parallel
+        while (par > 1 && start[nlev + 1] - start[par] < kTail) par--;
         const int P = size();
-        std::vector<std::atomic<size_t>> next(nlev + 2);
-        for (int l = 1; l <= nlev; l++) next[l].store(start[l]);
+        std::vector<std::atomic<int>> next(par + 2);
+        for (int l = 1; l <= par; l++) next[l].store(start[l]);
         std::atomic<long> arrived{0};
-        run_all([&](int) {
-            for (int l = 1; l <= nlev; l++) {
-                const size_t e = start[l + 1];
-                const size_t chunk = (e - start[l]) / (size_t)(4 * P) + 1;
+        run_all([&](int t) {
+            for (int l = 1; l <= par; l++) {
+                const int e = start[l + 1];
+                const int chunk = (e - start[l]) / (4 * P) + 1;
                 for (;;) {
-                    const size_t b = next[l].fetch_add(chunk);
+                    const int b = next[l].fetch_add(chunk);
                     if (b >= e) break;
-                    const size_t end = std::min(e, b + chunk);
-                    for (size_t q = b; q < end; q++) fn(order[q]);
+                    const int end = std::min(e, b + chunk);
+                    for (int q = b; q < end; q++) fn((size_t)order[q]);
                 }
                 // barrier: every thread has finished level l
                 arrived.fetch_add(1);
                 const long want = (long)l * P;
-                while (arrived.load(std::memory_order_acquire) < want) std::this_thread::yield();
+                for (long spin = 0; arrived.load(std::memory_order_acquire) < want; spin++) {
+                    if (spin < 20000)
+                        __builtin_ia32_pause();
+                    else
+                        std::this_thread::yield();
+                }
             }
+            if (t == 0)
+                for (int q = start[par + 1]; q < start[nlev + 1]; q++) fn((size_t)order[q]);
         });
     }
 
@@ -264,6 +276,116 @@ __global__ void k_coll_scatter_v(double* __restrict__ soa, size_t cap, const int
     soa[(size_t)FVZ * cap + i] = in[k].vz;
 }
 
+// Levels of the shuffled list on the device.  The level of collision k is one
+// more than the largest level of the collisions before it that share a body.
+// prev holds, for each of the two bodies of k, the nearest earlier collision
+// of that body (from each body's list of collisions); rounds then give a
+// level to every collision whose predecessors had one after the round
+// before.  Each round reads one buffer and writes the other, so it sees only
+// what earlier launches wrote: round l settles exactly the collisions of
+// level l.  Levels are at most kMaxLevels, so level starts fit kLevBins.
+static constexpr int kLevBins = 1024;
+static constexpr int kMaxLevels = kLevBins - 2;
+static constexpr int kLevBlocks = 64;   // blocks of the histogram / placement passes
+
+// Collisions per body (both bodies of every collision).
+__global__ void k_lev_body_count(const HSPair* __restrict__ cl, int nc, int* __restrict__ cnt) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nc) return;
+    atomicAdd(&cnt[cl[k].p1], 1);
+    atomicAdd(&cnt[cl[k].p2], 1);
+}
+
+// Each body's collisions at its offset (in any order).
+__global__ void k_lev_body_fill(const HSPair* __restrict__ cl, int nc, const int* __restrict__ off,
+                                int* __restrict__ cursor, int* __restrict__ list) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nc) return;
+    const int a = cl[k].p1, b = cl[k].p2;
+    list[off[a] + atomicAdd(&cursor[a], 1)] = k;
+    list[off[b] + atomicAdd(&cursor[b], 1)] = k;
+}
+
+// prev[2k + s]: the latest collision before k of body s of k (-1: none).
+__global__ void k_lev_links(const HSPair* __restrict__ cl, int nc, const int* __restrict__ off,
+                            const int* __restrict__ list, int* __restrict__ prev) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= 2 * nc) return;
+    const int k = e >> 1;
+    const int body = (e & 1) ? cl[k].p2 : cl[k].p1;
+    int best = -1;
+    for (int j = off[body]; j < off[body + 1]; j++) {
+        const int q = list[j];
+        if (q < k && q > best) best = q;
+    }
+    prev[e] = best;
+}
+
+__global__ void k_lev_round(const int* __restrict__ prev, const int* __restrict__ in, int* __restrict__ out, int nc,
+                            int* __restrict__ pending) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nc) return;
+    int v = in[k];
+    if (v == 0) {
+        const int q1 = prev[2 * k], q2 = prev[2 * k + 1];
+        const int a = q1 < 0 ? 0 : in[q1];
+        const int b = q2 < 0 ? 0 : in[q2];
+        if ((q1 < 0 || a > 0) && (q2 < 0 || b > 0))
+            v = max(a, b) + 1;
+        else
+            *pending = 1;
+    }
+    out[k] = v;
+}
+
+// Block b of kLevBlocks takes collisions [b*per, (b+1)*per): its count per
+// level into hist[b][*].
+__global__ __launch_bounds__(256) void k_lev_hist(const int* __restrict__ level, int nc, int per,
+                                                  int* __restrict__ hist) {
+    __shared__ int h[kLevBins];
+    for (int i = threadIdx.x; i < kLevBins; i += 256) h[i] = 0;
+    __syncthreads();
+    const int k0 = blockIdx.x * per, k1 = min(nc, k0 + per);
+    for (int k = k0 + (int)threadIdx.x; k < k1; k += 256) atomicAdd(&h[level[k]], 1);
+    __syncthreads();
+    for (int i = threadIdx.x; i < kLevBins; i += 256) hist[blockIdx.x * kLevBins + i] = h[i];
+}
+
+// One block of kLevBins threads: start[l] (first slot of level l) and the
+// first slot of each histogram block in each level, offs[b][l].
+__global__ __launch_bounds__(kLevBins) void k_lev_offsets(const int* __restrict__ hist, int* __restrict__ start,
+                                                          int* __restrict__ offs) {
+    __shared__ int sc[kLevBins];
+    const int l = threadIdx.x;
+    int tot = 0;
+    for (int b = 0; b < kLevBlocks; b++) tot += hist[b * kLevBins + l];
+    sc[l] = tot;
+    __syncthreads();
+    for (int d = 1; d < kLevBins; d <<= 1) {   // inclusive scan
+        const int v = l >= d ? sc[l - d] : 0;
+        __syncthreads();
+        sc[l] += v;
+        __syncthreads();
+    }
+    int o = sc[l] - tot;   // exclusive
+    start[l] = o;
+    for (int b = 0; b < kLevBlocks; b++) {
+        offs[b * kLevBins + l] = o;
+        o += hist[b * kLevBins + l];
+    }
+}
+
+// Collision k to a slot of its level (any order within a level: the
+// collisions of one level touch disjoint bodies).
+__global__ __launch_bounds__(256) void k_lev_place(const int* __restrict__ level, int nc, int per,
+                                                   const int* __restrict__ offs, int* __restrict__ order) {
+    __shared__ int cur[kLevBins];
+    for (int i = threadIdx.x; i < kLevBins; i += 256) cur[i] = offs[blockIdx.x * kLevBins + i];
+    __syncthreads();
+    const int k0 = blockIdx.x * per, k1 = min(nc, k0 + per);
+    for (int k = k0 + (int)threadIdx.x; k < k1; k += 256) order[atomicAdd(&cur[level[k]], 1)] = k;
+}
+
 static void copy_body_in(struct reb_particle& p, const double* s) {
     p.x = s[FX]; p.y = s[FY]; p.z = s[FZ];
     p.vx = s[FVX]; p.vy = s[FVY]; p.vz = s[FVZ];
@@ -339,23 +461,28 @@ static int rand_r_step(unsigned int* seed) {
 // compact records of the last search.
 struct rbamd::ResolveScratch {
     DevBuf flag, cidx, ids, body, pair, perm, coll, cub;   // device
+    DevBuf bcnt, boff, bcur, blist, prev, level_d, lmisc, lcub;
     HSBody* body_h = nullptr;                              // pinned host
     HSPair* pair_h = nullptr;
-    int* perm_h = nullptr;
+    int* perm_h = nullptr;    // the permutation, then the level order
     int* n_h = nullptr;
+    int* lev_h = nullptr;     // round flags, then level starts (kLevBins)
+    int last_nlev = 0;
     size_t body_cap = 0, pair_cap = 0, perm_cap = 0;
     hipEvent_t ev_pairs = nullptr, ev_coll = nullptr;
     const CollRecH* drec = nullptr;   // device records of the prepared search
     int n = 0;
     bool ready = false;
-    std::vector<int> level, last;
-    std::vector<size_t> start, order;
+    std::vector<int> level, last, hstart, horder;
     std::vector<double> plog;
     std::vector<unsigned char> hit;
     void release() {
-        for (DevBuf* b : {&flag, &cidx, &ids, &body, &pair, &perm, &coll, &cub}) b->release();
-        for (void* p : {(void*)body_h, (void*)pair_h, (void*)perm_h, (void*)n_h})
+        for (DevBuf* b : {&flag, &cidx, &ids, &body, &pair, &perm, &coll, &cub, &bcnt, &boff, &bcur, &blist, &prev,
+                          &level_d, &lmisc, &lcub})
+            b->release();
+        for (void* p : {(void*)body_h, (void*)pair_h, (void*)perm_h, (void*)n_h, (void*)lev_h})
             if (p) (void)hipHostFree(p);
+        lev_h = nullptr;
         if (ev_pairs) (void)hipEventDestroy(ev_pairs);
         if (ev_coll) (void)hipEventDestroy(ev_coll);
         body_h = nullptr;
@@ -466,6 +593,112 @@ void collisions_free(struct reb_simulation* r) {
     r->N_allocated_collisions = 0;
 }
 
+// Levels of the shuffled list in W.pair on the device (see k_lev_round).
+// Returns 1 with the level starts (kLevBins entries) and the collisions
+// ordered by level enqueued for copying into W.lev_h and W.perm_h and *nlev
+// an upper bound of the levels, 0 if the list needs more than kMaxLevels
+// levels, -1 on a HIP error.
+static int device_levels(struct reb_simulation* r, ResolveScratch& W, int nc, int n, int* nlev) {
+    DeviceState* d = dev_of(r);
+    const size_t m = 2 * (size_t)nc;
+    // lmisc: round flags | level starts | per-block histograms | per-block offsets
+    const size_t misc = (size_t)kLevBins * (2 + 2 * kLevBlocks);
+    if (!W.bcnt.ensure(sizeof(int) * (n + 1)) || !W.boff.ensure(sizeof(int) * (n + 1)) ||
+        !W.bcur.ensure(sizeof(int) * n) || !W.blist.ensure(sizeof(int) * m) || !W.prev.ensure(sizeof(int) * m) ||
+        !W.level_d.ensure(sizeof(int) * 2 * (size_t)nc) || !W.lmisc.ensure(sizeof(int) * misc)) {
+        reb_simulation_error(r, "Out of device memory for the collision resolve.");
+        return -1;
+    }
+    if (!W.lev_h && !RB_HIP(r, hipHostMalloc((void**)&W.lev_h, sizeof(int) * kLevBins, hipHostMallocDefault)))
+        return -1;
+    int* flags = W.lmisc.as<int>();
+    int* start = flags + kLevBins;
+    int* hist = start + kLevBins;
+    int* offs = hist + (size_t)kLevBins * kLevBlocks;
+    size_t t_scan = 0;
+    hipcub::DeviceScan::ExclusiveSum(nullptr, t_scan, W.bcnt.as<int>(), W.boff.as<int>(), n + 1, d->stream);
+    if (!W.lcub.ensure(t_scan + 256)) {
+        reb_simulation_error(r, "Out of device memory for the collision resolve.");
+        return -1;
+    }
+    const dim3 gk((unsigned)((nc + 255) / 256)), gm((unsigned)((m + 255) / 256)), blk(256);
+    const HSPair* cl = W.pair.as<HSPair>();
+    // each body's collisions, then the predecessors
+    if (!RB_HIP(r, hipMemsetAsync(W.bcnt.p, 0, sizeof(int) * (n + 1), d->stream)) ||
+        !RB_HIP(r, hipMemsetAsync(W.bcur.p, 0, sizeof(int) * n, d->stream)))
+        return -1;
+    hipLaunchKernelGGL(k_lev_body_count, gk, blk, 0, d->stream, cl, nc, W.bcnt.as<int>());
+    hipcub::DeviceScan::ExclusiveSum(W.lcub.p, t_scan, W.bcnt.as<int>(), W.boff.as<int>(), n + 1, d->stream);
+    hipLaunchKernelGGL(k_lev_body_fill, gk, blk, 0, d->stream, cl, nc, W.boff.as<int>(), W.bcur.as<int>(),
+                       W.blist.as<int>());
+    hipLaunchKernelGGL(k_lev_links, gm, blk, 0, d->stream, cl, nc, W.boff.as<int>(), W.blist.as<int>(),
+                       W.prev.as<int>());
+    if (!RB_HIP(r, hipMemsetAsync(W.level_d.p, 0, sizeof(int) * nc, d->stream)) ||
+        !RB_HIP(r, hipMemsetAsync(flags, 0, sizeof(int) * kLevBins, d->stream)))
+        return -1;
+    // rounds in batches; after each batch the last round's flag says whether
+    // any collision still had no level when it ran.  The levels alternate
+    // between the two halves of level_d; `level` is the latest.
+    int* lv_buf[2] = {W.level_d.as<int>(), W.level_d.as<int>() + nc};
+    int* level = lv_buf[0];
+    int rounds = 0, batch = std::max(8, W.last_nlev + 2);
+    for (;;) {
+        const int b = std::min(batch, kMaxLevels - rounds);
+        for (int i = 0; i < b; i++, rounds++) {
+            int* next = lv_buf[(rounds + 1) & 1];
+            hipLaunchKernelGGL(k_lev_round, gk, blk, 0, d->stream, W.prev.as<int>(), level, next, nc,
+                               flags + rounds);
+            level = next;
+        }
+        if (!RB_HIP(r, hipGetLastError()) ||
+            !RB_HIP(r, hipMemcpyAsync(W.lev_h, flags + rounds - 1, sizeof(int), hipMemcpyDeviceToHost, d->stream)) ||
+            !stream_sync(r))
+            return -1;
+        if (W.lev_h[0] == 0) break;
+        if (rounds >= kMaxLevels) return 0;
+        batch = 8;
+    }
+    // collisions ordered by level
+    const int per = (nc + kLevBlocks - 1) / kLevBlocks;
+    hipLaunchKernelGGL(k_lev_hist, dim3(kLevBlocks), blk, 0, d->stream, level, nc, per, hist);
+    hipLaunchKernelGGL(k_lev_offsets, dim3(1), dim3(kLevBins), 0, d->stream, hist, start, offs);
+    hipLaunchKernelGGL(k_lev_place, dim3(kLevBlocks), blk, 0, d->stream, level, nc, per, offs, W.perm.as<int>());
+    if (!RB_HIP(r, hipGetLastError()) ||
+        !RB_HIP(r, hipMemcpyAsync(W.lev_h, start, sizeof(int) * kLevBins, hipMemcpyDeviceToHost, d->stream)) ||
+        !RB_HIP(r, hipMemcpyAsync(W.perm_h, W.perm.p, sizeof(int) * nc, hipMemcpyDeviceToHost, d->stream)))
+        return -1;
+    *nlev = rounds;
+    return 1;
+}
+
+// The same levels on the host (a list with more than kMaxLevels levels).
+static void host_levels(ResolveScratch& W, const HSPair* cl, int nc, int n, int* nlev) {
+    std::vector<int>& level = W.level;
+    std::vector<int>& last = W.last;
+    std::vector<int>& start = W.hstart;
+    level.resize(nc);
+    last.assign(n, 0);
+    start.assign(64, 0);
+    int top = 0;
+    for (int k = 0; k < nc; k++) {
+        const HSPair q = cl[k];
+        const int l = std::max(last[q.p1], last[q.p2]) + 1;
+        level[k] = l;
+        last[q.p1] = l;
+        last[q.p2] = l;
+        top = std::max(top, l);
+        if ((size_t)l + 1 >= start.size()) start.resize(2 * ((size_t)l + 1), 0);
+        start[l + 1]++;
+    }
+    start.resize(top + 2);
+    for (int l = 1; l <= top + 1; l++) start[l] += start[l - 1];
+    std::vector<int>& order = W.horder;
+    order.resize(nc);
+    std::vector<int> fill(start.begin(), start.end());
+    for (int k = 0; k < nc; k++) order[fill[level[k]]++] = k;
+    *nlev = top;
+}
+
 // The built-in hard-sphere resolver with a constant restitution, on the
 // compact records prepared by compact_resolve_gather: only the eight fields it
 // reads come over, only the velocities go back, and collisions of one level
@@ -475,15 +708,15 @@ static void resolve_hardsphere_compact(struct reb_simulation* r) {
     ResolveScratch& W = *d->resolve;
     W.ready = false;
     const CollList& L = *d->coll;
-    const size_t nc = L.n;
+    const int nc = (int)L.n;
     const int n = W.n;
     // shuffle (collision.c:336-342) as a permutation of the detection order,
     // the same swaps applied to indices instead of 72-byte records; drawn
     // while the body records are on their way
     int* perm = W.perm_h;
-    for (size_t k = 0; k < nc; k++) perm[k] = (int)k;
-    for (size_t i = 0; i < nc; i++) {
-        const size_t nw = (size_t)rand_r_step(&r->rand_seed) % nc;
+    for (int k = 0; k < nc; k++) perm[k] = k;
+    for (int i = 0; i < nc; i++) {
+        const int nw = (int)((size_t)rand_r_step(&r->rand_seed) % (size_t)nc);
         std::swap(perm[i], perm[nw]);
     }
     CollGB gb;
@@ -491,9 +724,12 @@ static void resolve_hardsphere_compact(struct reb_simulation* r) {
     for (int k = 0; k < gb.n; k++) gb.g[k] = L.gb[k];
     if (!RB_HIP(r, hipMemcpyAsync(W.perm.p, perm, sizeof(int) * nc, hipMemcpyHostToDevice, d->stream))) return;
     hipLaunchKernelGGL(k_coll_shuffled, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, d->stream, W.drec,
-                       W.perm.as<int>(), (int)nc, W.cidx.as<int>(), gb, W.pair.as<HSPair>(),
+                       W.perm.as<int>(), nc, W.cidx.as<int>(), gb, W.pair.as<HSPair>(),
                        W.coll.as<struct reb_collision>());
     if (!RB_HIP(r, hipGetLastError())) return;
+    int nlev = 0;
+    const int lv = device_levels(r, W, nc, n, &nlev);
+    if (lv < 0) return;
     if (!RB_HIP(r, hipMemcpyAsync(W.pair_h, W.pair.p, sizeof(HSPair) * nc, hipMemcpyDeviceToHost, d->stream)) ||
         !RB_HIP(r, hipEventRecord(W.ev_pairs, d->stream)))
         return;
@@ -504,38 +740,19 @@ static void resolve_hardsphere_compact(struct reb_simulation* r) {
         !RB_HIP(r, hipEventRecord(W.ev_coll, d->stream)))
         return;
     if (!RB_HIP(r, hipEventSynchronize(W.ev_pairs))) return;
-    // levels: one more than the last level that touched either body, counted
-    // per level on the way
     const HSPair* cl = W.pair_h;
-    std::vector<int>& level = W.level;
-    std::vector<int>& last = W.last;
-    std::vector<size_t>& start = W.start;
-    level.resize(nc);
-    last.assign(n, 0);
-    start.assign(64, 0);
-    int nlev = 0;
-    for (size_t k = 0; k < nc; k++) {
-        if (k + 16 < nc) {
-            __builtin_prefetch(&last[cl[k + 16].p1]);
-            __builtin_prefetch(&last[cl[k + 16].p2]);
-        }
-        const HSPair q = cl[k];
-        const int l = std::max(last[q.p1], last[q.p2]) + 1;
-        level[k] = l;
-        last[q.p1] = l;
-        last[q.p2] = l;
-        nlev = std::max(nlev, l);
-        if ((size_t)l + 1 >= start.size()) start.resize(2 * ((size_t)l + 1), 0);
-        start[l + 1]++;
+    const int* start;
+    const int* order;
+    if (lv == 1) {
+        start = W.lev_h;
+        order = W.perm_h;
+        while (nlev > 0 && start[nlev + 1] == start[nlev]) nlev--;
+    } else {
+        host_levels(W, cl, nc, n, &nlev);
+        start = W.hstart.data();
+        order = W.horder.data();
     }
-    start.resize(nlev + 2);
-    for (int l = 1; l <= nlev + 1; l++) start[l] += start[l - 1];
-    std::vector<size_t>& order = W.order;
-    order.resize(nc);
-    {
-        std::vector<size_t> fill(start.begin(), start.end());
-        for (size_t k = 0; k < nc; k++) order[fill[level[k]]++] = k;
-    }
+    W.last_nlev = nlev;
     // plog starts at -0.0, the exact identity of the sum below, so that the
     // collisions that do not bounce need no branch
     std::vector<double>& plog = W.plog;
@@ -543,13 +760,13 @@ static void resolve_hardsphere_compact(struct reb_simulation* r) {
     plog.assign(nc, -0.);
     hit.assign(nc, 0);
     HSBody* B = W.body_h;
-    HostPool::get().run_levels(nlev, start.data(), order.data(), [&](size_t k) {
+    HostPool::get().run_levels(nlev, start, order, [&](size_t k) {
         hit[k] = (unsigned char)hardsphere_apply(r, B, (size_t)cl[k].p1, (size_t)cl[k].p2, L.gb[cl[k].gb], &plog[k]);
     });
     // the pressure log in list order (collision.c:655-663)
     double sum = r->collisions_plog;
     long nhit = 0;
-    for (size_t k = 0; k < nc; k++) {
+    for (int k = 0; k < nc; k++) {
         sum += plog[k];
         nhit += hit[k];
     }

@@ -744,7 +744,7 @@ struct CollRec {
 
 // Records kept per body by the single-pass search; the bodies with more
 // collisions are searched again by a pass that walks only them.
-static constexpr int kCollSlots = 8;
+static constexpr int kCollSlots = 32;
 
 // Tree search of collision.c:197-269 + :422-503 (TREE) or :270-331 +
 // :506-566 (LINETREE, LINE = true), in detection order per body (ghost box,
