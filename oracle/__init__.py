@@ -50,7 +50,8 @@ def lib():
             "orc_whfast_dh": (None, [_dp, _dp, c_size_t, c_double, c_double, c_double, c_long, c_int, POINTER(c_int),
                                      c_int]),
             "orc_whfast_jacobi": (None, [_dp, _dp, c_size_t, c_size_t, c_int, c_double, c_double, c_int, c_double,
-                                         c_long, c_int, c_int, c_int, POINTER(c_int), c_int]),
+                                         c_long, c_int, c_int, c_int, c_int, POINTER(c_int), c_int]),
+            "orc_gravity_jacobi": (None, [_dp, c_size_t, c_double]),
             "orc_sei_operator": (None, [_dp, c_size_t, c_double, c_double, c_double, c_int]),
             "orc_shear_wrap": (None, [_dp, c_size_t, c_double, c_double, c_double, c_double, c_double]),
             "orc_ghostbox_shear": (None, [c_double, c_double, c_double, c_double, c_double, c_int, c_int, c_int,
@@ -156,14 +157,21 @@ def whfast_dh(s, G, dt, steps, safe_mode, soft=0.0, ph=None, is_sync=1, did_modi
     return s, ph, flag.value
 
 
+def gravity_jacobi(s, G=1.0):
+    """Accelerations of the Jacobi gravity (gravity.c:108-163), (N, 3)."""
+    s = np.ascontiguousarray(s, dtype=np.float64).copy()
+    lib().orc_gravity_jacobi(s, s.shape[0], G)
+    return s[:, 6:9].copy()
+
+
 def whfast_jacobi(s, G, dt, steps, safe_mode, soft=0.0, corrector=0, corrector2=0, N_active=None, tp_type=0,
-                  compensated=0, pj=None, is_sync=1, did_modify=1):
+                  compensated=0, pj=None, is_sync=1, did_modify=1, kernel=0):
     s = np.ascontiguousarray(s, dtype=np.float64).copy()
     n = s.shape[0]
     pj = np.zeros((n, 7)) if pj is None else np.ascontiguousarray(pj, dtype=np.float64).copy()
     flag = c_int(is_sync)
     lib().orc_whfast_jacobi(s, pj, n, n if N_active is None else N_active, tp_type, G, soft, compensated, dt, steps,
-                            safe_mode, corrector, corrector2, ctypes.byref(flag), did_modify)
+                            safe_mode, corrector, corrector2, kernel, ctypes.byref(flag), did_modify)
     return s, pj, flag.value
 
 

@@ -682,8 +682,11 @@ typedef struct {
     double* pj;       /* Jacobi bodies, 7 columns */
     double* acc;      /* 3N scratch */
     double* ja;       /* 3N Jacobi accelerations */
+    double* jerk;     /* 3N jerk (modified kick kernel) */
     size_t N, NA, NAG;
     int tp, comp;
+    int kernel;       /* 0 default, 1 modified kick, 2 composition, 3 lazy */
+    int jacgrav;      /* r->gravity == REB_GRAVITY_JACOBI (kernels 1 and 3) */
     double G, soft, dt;
 } jacctx;
 
@@ -806,8 +809,137 @@ static void jac_com(jacctx* c, double h) {
     PH(c->pj, 0, 2) += h * PH(c->pj, 0, 5);
 }
 
+static void jac_gravity_jacobi(jacctx* c) {
+    /* gravity.c:108-163: Jacobi and direct terms, each pair visited once in
+     * the order j = 0..N-1, i = 0..j, both bodies updated; no softening */
+    double* s = c->s;
+    const size_t N = c->N;
+    const double G = c->G;
+    double Rx = 0., Ry = 0., Rz = 0., Mt = 0.;
+    for (size_t j = 0; j < N; j++) {
+        COL(s, j, AX) = 0.;
+        COL(s, j, AY) = 0.;
+        COL(s, j, AZ) = 0.;
+        for (size_t i = 0; i < j + 1; i++) {
+            if (j > 1) {
+                const double qx = COL(s, j, X) - Rx / Mt;
+                const double qy = COL(s, j, Y) - Ry / Mt;
+                const double qz = COL(s, j, Z) - Rz / Mt;
+                const double dr = sqrt(qx * qx + qy * qy + qz * qz);
+                const double w = i < j ? -COL(s, j, M) : Mt;
+                const double pf = G * w / (dr * dr * dr);
+                COL(s, i, AX) += pf * qx;
+                COL(s, i, AY) += pf * qy;
+                COL(s, i, AZ) += pf * qz;
+            }
+            if (i != j && (i != 0 || j != 1)) {
+                const double dx = COL(s, i, X) - COL(s, j, X);
+                const double dy = COL(s, i, Y) - COL(s, j, Y);
+                const double dz = COL(s, i, Z) - COL(s, j, Z);
+                const double dr = sqrt(dx * dx + dy * dy + dz * dz);
+                const double pf = G / (dr * dr * dr);
+                const double pfi = pf * COL(s, i, M);
+                const double pfj = pf * COL(s, j, M);
+                COL(s, i, AX) -= pfj * dx;
+                COL(s, i, AY) -= pfj * dy;
+                COL(s, i, AZ) -= pfj * dz;
+                COL(s, j, AX) += pfi * dx;
+                COL(s, j, AY) += pfi * dy;
+                COL(s, j, AZ) += pfi * dz;
+            }
+        }
+        Rx += COL(s, j, M) * COL(s, j, X);
+        Ry += COL(s, j, M) * COL(s, j, Y);
+        Rz += COL(s, j, M) * COL(s, j, Z);
+        Mt += COL(s, j, M);
+    }
+}
+
+void orc_gravity_jacobi(double* s, size_t N, double G) {
+    jacctx c;
+    memset(&c, 0, sizeof(c));
+    c.s = s;
+    c.N = N;
+    c.G = G;
+    jac_gravity_jacobi(&c);
+}
+
+static void jac_jerk(jacctx* c) {
+    /* integrator_whfast.c:786-873: time derivative of the Jacobi-gravity
+     * accelerations, same visiting order as the gravity above */
+    const double* s = c->s;
+    double* jk = c->jerk;
+    const size_t N = c->N;
+    const double G = c->G;
+    double Rx = 0., Ry = 0., Rz = 0., Mt = 0., Ax = 0., Ay = 0., Az = 0.;
+    for (size_t j = 0; j < N; j++) {
+        jk[3 * j] = jk[3 * j + 1] = jk[3 * j + 2] = 0.;
+        for (size_t i = 0; i < j + 1; i++) {
+            if (j > 1) {
+                const double w = i < j ? -COL(s, j, M) : Mt;
+                const double qx = COL(s, j, X) - Rx / Mt;
+                const double qy = COL(s, j, Y) - Ry / Mt;
+                const double qz = COL(s, j, Z) - Rz / Mt;
+                const double dax = COL(s, j, AX) - Ax / Mt;
+                const double day = COL(s, j, AY) - Ay / Mt;
+                const double daz = COL(s, j, AZ) - Az / Mt;
+                const double dr = sqrt(qx * qx + qy * qy + qz * qz);
+                const double p2 = G * w / (dr * dr * dr);
+                jk[3 * i] += p2 * dax;
+                jk[3 * i + 1] += p2 * day;
+                jk[3 * i + 2] += p2 * daz;
+                const double al = dax * qx + day * qy + daz * qz;
+                const double p1 = 3. * al * p2 / (dr * dr);
+                jk[3 * i] -= p1 * qx;
+                jk[3 * i + 1] -= p1 * qy;
+                jk[3 * i + 2] -= p1 * qz;
+            }
+            if (j != i && (i != 0 || j != 1)) {
+                const double dx = COL(s, j, X) - COL(s, i, X);
+                const double dy = COL(s, j, Y) - COL(s, i, Y);
+                const double dz = COL(s, j, Z) - COL(s, i, Z);
+                const double dax = COL(s, j, AX) - COL(s, i, AX);
+                const double day = COL(s, j, AY) - COL(s, i, AY);
+                const double daz = COL(s, j, AZ) - COL(s, i, AZ);
+                const double dr = sqrt(dx * dx + dy * dy + dz * dz);
+                const double al = dax * dx + day * dy + daz * dz;
+                const double p2 = G / (dr * dr * dr);
+                const double p2i = p2 * COL(s, i, M);
+                const double p2j = p2 * COL(s, j, M);
+                jk[3 * j] -= dax * p2i;
+                jk[3 * j + 1] -= day * p2i;
+                jk[3 * j + 2] -= daz * p2i;
+                jk[3 * i] += dax * p2j;
+                jk[3 * i + 1] += day * p2j;
+                jk[3 * i + 2] += daz * p2j;
+                const double p1 = 3. * al * p2 / (dr * dr);
+                const double p1i = p1 * COL(s, i, M);
+                const double p1j = p1 * COL(s, j, M);
+                jk[3 * j] += dx * p1i;
+                jk[3 * j + 1] += dy * p1i;
+                jk[3 * j + 2] += dz * p1i;
+                jk[3 * i] -= dx * p1j;
+                jk[3 * i + 1] -= dy * p1j;
+                jk[3 * i + 2] -= dz * p1j;
+            }
+        }
+        Ax += COL(s, j, AX) * COL(s, j, M);
+        Ay += COL(s, j, AY) * COL(s, j, M);
+        Az += COL(s, j, AZ) * COL(s, j, M);
+        Rx += COL(s, j, X) * COL(s, j, M);
+        Ry += COL(s, j, Y) * COL(s, j, M);
+        Rz += COL(s, j, Z) * COL(s, j, M);
+        Mt += COL(s, j, M);
+    }
+}
+
 static void jac_gravity(jacctx* c) {
-    /* reb_simulation_update_acceleration with gravity_ignore_terms = BETWEEN_0_AND_1 */
+    /* reb_simulation_update_acceleration: the Jacobi gravity for kernels 1
+     * and 3, else gravity_ignore_terms = BETWEEN_0_AND_1 */
+    if (c->jacgrav) {
+        jac_gravity_jacobi(c);
+        return;
+    }
     const double gb[3] = {0, 0, 0};
     if (c->comp)
         orc_gravity_compensated(c->s, c->N, c->NAG, c->tp, 1, c->G, c->soft, c->acc);
@@ -851,7 +983,7 @@ static void jac_interaction(jacctx* c, double h) {
         PH(c->pj, i, 3) += h * ja[3 * i];
         PH(c->pj, i, 4) += h * ja[3 * i + 1];
         PH(c->pj, i, 5) += h * ja[3 * i + 2];
-        if (i > 1) {
+        if (i > 1 && !c->jacgrav) {   /* Jacobi gravity already holds these terms */
             const double rj2i = 1. / (px * px + py * py + pz * pz);
             const double rji = sqrt(rj2i);
             const double rj3iM = rji * rj2i * c->G * et;
@@ -970,9 +1102,11 @@ static void jac_corrector2(jacctx* c, double inv) {
 }
 
 static void jac_synchronize(jacctx* c, int corrector, int corrector2) {
-    /* integrator_whfast.c:1051-1111 (default kernel) */
-    jac_kepler(c, c->dt / 2.);
-    jac_com(c, c->dt / 2.);
+    /* integrator_whfast.c:1051-1111: a half drift, 3/8 of one for the
+     * composition kernel */
+    const double h = c->kernel == 2 ? 3. * c->dt / 8. : c->dt / 2.;
+    jac_kepler(c, h);
+    jac_com(c, h);
     if (corrector2) jac_corrector2(c, -1.);
     if (corrector) jac_corrector(c, -1., corrector);
     jac_to_inertial(c, 1);
@@ -980,12 +1114,15 @@ static void jac_synchronize(jacctx* c, int corrector, int corrector2) {
 
 void orc_whfast_jacobi(double* s, double* pj, size_t N, size_t N_active, int tp_type, double G, double soft,
                        int compensated, double dt, long steps, int safe_mode, int corrector, int corrector2,
-                       int* is_sync, int did_modify) {
+                       int kernel, int* is_sync, int did_modify) {
     jacctx c;
     c.s = s;
     c.pj = pj;
     c.acc = malloc(sizeof(double) * 3 * N + 8);
     c.ja = malloc(sizeof(double) * 3 * N + 8);
+    c.jerk = malloc(sizeof(double) * 3 * N + 8);
+    c.kernel = kernel;
+    c.jacgrav = kernel == 1 || kernel == 3;
     c.N = N;
     c.NAG = N_active > N ? N : N_active;
     c.NA = tp_type == 1 ? N : c.NAG;
@@ -1006,15 +1143,54 @@ void orc_whfast_jacobi(double* s, double* pj, size_t N, size_t N_active, int tp_
         if (*is_sync) {
             if (corrector) jac_corrector(&c, 1., corrector);
             if (corrector2) jac_corrector2(&c, 1.);
-            jac_kepler(&c, dt / 2.);
-            jac_com(&c, dt / 2.);
+            const double h = kernel == 2 ? 5. * dt / 8. : dt / 2.;
+            jac_kepler(&c, h);
+            jac_com(&c, h);
         } else {
             jac_kepler(&c, dt);
             jac_com(&c, dt);
         }
         jac_to_inertial(&c, 1);
         jac_gravity(&c);
-        jac_interaction(&c, dt);
+        if (kernel == 1) {
+            /* integrator_whfast.c:1178-1187 */
+            jac_jerk(&c);
+            for (size_t i = 0; i < N; i++) {
+                const double pf = dt * dt / 12.;
+                COL(s, i, AX) += pf * c.jerk[3 * i];
+                COL(s, i, AY) += pf * c.jerk[3 * i + 1];
+                COL(s, i, AZ) += pf * c.jerk[3 * i + 2];
+            }
+            jac_interaction(&c, dt);
+        } else if (kernel == 2) {
+            /* integrator_whfast.c:1189-1218: interaction and drift stages of
+             * the composition kernel, a gravity evaluation before each kick */
+            const double kick[5] = {-dt / 6., dt / 6., dt, -dt / 6., dt / 6.};
+            const double drift[4] = {-dt / 4., dt / 8., -dt / 8., dt / 4.};
+            for (int q = 0; q < 5; q++) {
+                if (q > 0) {
+                    jac_to_inertial(&c, 0);
+                    jac_gravity(&c);
+                }
+                jac_interaction(&c, kick[q]);
+                if (q < 4) {
+                    jac_kepler(&c, drift[q]);
+                    jac_com(&c, drift[q]);
+                }
+            }
+        } else if (kernel == 3) {
+            /* integrator_whfast.c:1220-1235 (WHT Eq. 10.6) */
+            for (size_t i = 1; i < N; i++) {
+                const double pf = dt * dt / 12.;
+                COL(s, i, X) += pf * COL(s, i, AX);
+                COL(s, i, Y) += pf * COL(s, i, AY);
+                COL(s, i, Z) += pf * COL(s, i, AZ);
+            }
+            jac_gravity(&c);
+            jac_interaction(&c, dt);
+        } else {
+            jac_interaction(&c, dt);
+        }
         *is_sync = 0;
         if (safe_mode) {
             jac_synchronize(&c, corrector, corrector2);
@@ -1029,6 +1205,7 @@ void orc_whfast_jacobi(double* s, double* pj, size_t N, size_t N_active, int tp_
     }
     free(c.acc);
     free(c.ja);
+    free(c.jerk);
 }
 
 /* ------------------------------------------------------------------ SEI + shear boundary */

@@ -69,9 +69,11 @@ void orc_whfast_dh(double* s, double* ph7, size_t N, double G, double soft, doub
  * `corrector` (0, 3, 5, 7, 11, 17) and the second corrector when `corrector2`.
  * N_active and tp_type as the simulation fields.  pj7 holds the Jacobi state
  * (N x 7) across calls; otherwise as orc_whfast_dh. */
+/* gravity.c:108-163 into the acceleration columns of s */
+void orc_gravity_jacobi(double* s, size_t N, double G);
 void orc_whfast_jacobi(double* s, double* pj7, size_t N, size_t N_active, int tp_type, double G, double soft,
                        int compensated, double dt, long steps, int safe_mode, int corrector, int corrector2,
-                       int* is_sync, int did_modify);
+                       int kernel, int* is_sync, int did_modify);
 
 /* SEI operators (integrator_sei.c:126-174): H0 over dt for every body,
  * preceded by the phi1 kick v += a dt when with_kick; the sin/tan constants

@@ -46,8 +46,8 @@ void update_acceleration_internal(struct reb_simulation* r) {
             ok = gravity_direct(r, true);
             break;
         case REB_GRAVITY_JACOBI:
-            reb_simulation_error(r, "REB_GRAVITY_JACOBI is not part of this engine.");
-            return;
+            ok = gravity_jacobi(r);
+            break;
         default:
             reb_simulation_error(r, "Gravity module not found.");
             return;

@@ -764,7 +764,68 @@ static int direct_dispatch(const double* x, const double* y, const double* z, co
     return rc;
 }
 
+// ---------------------------------------------------------------- JACOBI
+// gravity.c:108-163, the gravity of the WHFast modified-kick and lazy kernels:
+// Jacobi and direct terms with every pair visited once (j = 0..N-1, i = 0..j)
+// and added to both bodies, without softening.  The order of the additions to
+// each body is the whole definition, so one thread walks the pairs; the
+// systems it serves have a few bodies (the WHFast group kernel evaluates the
+// same sums for its own systems).
+__global__ void k_gravity_jacobi(const double* __restrict__ x, const double* __restrict__ y,
+                                 const double* __restrict__ z, const double* __restrict__ m, double* ax, double* ay,
+                                 double* az, int N, double G) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double Rx = 0., Ry = 0., Rz = 0., Mt = 0.;
+    for (int j = 0; j < N; j++) {
+        ax[j] = 0.;
+        ay[j] = 0.;
+        az[j] = 0.;
+        const double xj = x[j], yj = y[j], zj = z[j], mj = m[j];
+        for (int i = 0; i <= j; i++) {
+            if (j > 1) {
+                const double qx = xj - Rx / Mt;
+                const double qy = yj - Ry / Mt;
+                const double qz = zj - Rz / Mt;
+                const double dr = sqrt(qx * qx + qy * qy + qz * qz);
+                const double w = i < j ? -mj : Mt;
+                const double pf = G * w / (dr * dr * dr);
+                ax[i] += pf * qx;
+                ay[i] += pf * qy;
+                az[i] += pf * qz;
+            }
+            if (i != j && (i != 0 || j != 1)) {
+                const double dx = x[i] - xj;
+                const double dy = y[i] - yj;
+                const double dz = z[i] - zj;
+                const double dr = sqrt(dx * dx + dy * dy + dz * dz);
+                const double pf = G / (dr * dr * dr);
+                const double pfi = pf * m[i];
+                const double pfj = pf * mj;
+                ax[i] -= pfj * dx;
+                ay[i] -= pfj * dy;
+                az[i] -= pfj * dz;
+                ax[j] += pfi * dx;
+                ay[j] += pfi * dy;
+                az[j] += pfi * dz;
+            }
+        }
+        Rx += mj * xj;
+        Ry += mj * yj;
+        Rz += mj * zj;
+        Mt += mj;
+    }
+}
+
 // ---------------------------------------------------------------- simulation path
+bool gravity_jacobi(struct reb_simulation* r) {
+    DeviceState* d = dev_of(r);
+    const int N = (int)r->N;
+    if (N == 0) return true;
+    hipLaunchKernelGGL(k_gravity_jacobi, dim3(1), dim3(64), 0, d->stream, d->field(FX), d->field(FY), d->field(FZ),
+                       d->field(FM), d->field(FAX), d->field(FAY), d->field(FAZ), N, r->G);
+    return RB_HIP(r, hipGetLastError());
+}
+
 bool gravity_zero(struct reb_simulation* r) {
     DeviceState* d = dev_of(r);
     const int N = (int)r->N;

@@ -180,6 +180,7 @@ int energy_launch(const double* x, const double* y, const double* z, const doubl
                   const double* vz, const double* m, int n, int n_active, int n_interact, double G, bool exact,
                   double* scratch, hipStream_t st, double* terms = nullptr, size_t terms_cap = 0);
 bool gravity_zero(struct reb_simulation* r);
+bool gravity_jacobi(struct reb_simulation* r);               // gravity.c:108-163
 bool gravity_tree(struct reb_simulation* r);                 // tree.hip
 bool collision_search_device(struct reb_simulation* r);      // tree.hip / collision
 double energy_device(struct reb_simulation* r);              // tools.c:108-162

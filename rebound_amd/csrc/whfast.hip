@@ -319,6 +319,8 @@ template <int COORD>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_whfast(WHParams P) {
     constexpr bool JAC = COORD == WH_JACOBI;
     __shared__ Slot slot[64];
+    __shared__ double aux[64][3];    // per-planet sums of the serial Jacobi gravity / jerk
+    __shared__ double aux0[64][3];   // the star's, at the group's first lane
     __shared__ int task_id;
     const int lane = threadIdx.x;
     for (;;) {
@@ -683,6 +685,158 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                 }
             }
         };
+        // The Jacobi gravity (gravity.c:108-163) of the modified-kick and lazy
+        // kernels visits every pair once, j = 0..N-1 and i = 0..j, and adds
+        // to both bodies; its sums are formed by the group's first lane in
+        // that order over the published bodies (the star from registers).
+        // Body k of the system: 0 the star, k >= 1 the slot of planet k.
+        auto bx = [&](int k) { return k == 0 ? sx : slot[gbase + k - 1].x; };
+        auto by = [&](int k) { return k == 0 ? sy : slot[gbase + k - 1].y; };
+        auto bz = [&](int k) { return k == 0 ? sz : slot[gbase + k - 1].z; };
+        auto bm = [&](int k) { return k == 0 ? sm : slot[gbase + k - 1].m; };
+        auto acc_of = [&](int k) -> double* { return k == 0 ? aux0[gbase] : aux[gbase + k - 1]; };
+        auto gravity_jacobi = [&](bool en) {
+            publish(Slot{ix, iy, iz, 0., 0., 0., im});
+            if (gl == 0 && sys_ok) {
+                double Rx = 0., Ry = 0., Rz = 0., Mt = 0.;
+                for (int j = 0; j < N; j++) {
+                    double* aj = acc_of(j);
+                    aj[0] = 0.; aj[1] = 0.; aj[2] = 0.;
+                    const double xj = bx(j), yj = by(j), zj = bz(j), mj = bm(j);
+                    for (int i = 0; i <= j; i++) {
+                        double* ai = acc_of(i);
+                        if (j > 1) {
+                            const double qx = xj - Rx / Mt;
+                            const double qy = yj - Ry / Mt;
+                            const double qz = zj - Rz / Mt;
+                            const double dr = sqrt(qx * qx + qy * qy + qz * qz);
+                            const double w = i < j ? -mj : Mt;
+                            const double pf = Gc * w / (dr * dr * dr);
+                            ai[0] += pf * qx;
+                            ai[1] += pf * qy;
+                            ai[2] += pf * qz;
+                        }
+                        if (i != j && (i != 0 || j != 1)) {
+                            const double dx = bx(i) - xj;
+                            const double dy = by(i) - yj;
+                            const double dz = bz(i) - zj;
+                            const double dr = sqrt(dx * dx + dy * dy + dz * dz);
+                            const double pf = Gc / (dr * dr * dr);
+                            const double pfi = pf * bm(i);
+                            const double pfj = pf * mj;
+                            ai[0] -= pfj * dx;
+                            ai[1] -= pfj * dy;
+                            ai[2] -= pfj * dz;
+                            aj[0] += pfi * dx;
+                            aj[1] += pfi * dy;
+                            aj[2] += pfi * dz;
+                        }
+                    }
+                    Rx += mj * xj;
+                    Ry += mj * yj;
+                    Rz += mj * zj;
+                    Mt += mj;
+                }
+            }
+            __syncthreads();
+            if (!en) return;
+            sax = aux0[gbase][0]; say = aux0[gbase][1]; saz = aux0[gbase][2];
+            if (live) {
+                ax = aux[lane][0]; ay = aux[lane][1]; az = aux[lane][2];
+            }
+        };
+        // integrator_whfast.c:786-873 (jerk of the Jacobi gravity, same pair
+        // order) and :1178-1186: a += dt^2/12 jerk for every body.
+        auto modified_kick = [&](bool en) {
+            publish(Slot{ix, iy, iz, ax, ay, az, im});
+            if (gl == 0 && sys_ok) {
+                auto bax = [&](int k) { return k == 0 ? sax : slot[gbase + k - 1].vx; };
+                auto bay = [&](int k) { return k == 0 ? say : slot[gbase + k - 1].vy; };
+                auto baz = [&](int k) { return k == 0 ? saz : slot[gbase + k - 1].vz; };
+                double Rx = 0., Ry = 0., Rz = 0., Mt = 0., Ax = 0., Ay = 0., Az = 0.;
+                for (int j = 0; j < N; j++) {
+                    double* kj = acc_of(j);
+                    kj[0] = 0.; kj[1] = 0.; kj[2] = 0.;
+                    const double xj = bx(j), yj = by(j), zj = bz(j), mj = bm(j);
+                    const double axj = bax(j), ayj = bay(j), azj = baz(j);
+                    for (int i = 0; i <= j; i++) {
+                        double* ki = acc_of(i);
+                        if (j > 1) {
+                            const double w = i < j ? -mj : Mt;
+                            const double qx = xj - Rx / Mt;
+                            const double qy = yj - Ry / Mt;
+                            const double qz = zj - Rz / Mt;
+                            const double dax = axj - Ax / Mt;
+                            const double day = ayj - Ay / Mt;
+                            const double daz = azj - Az / Mt;
+                            const double dr = sqrt(qx * qx + qy * qy + qz * qz);
+                            const double p2 = Gc * w / (dr * dr * dr);
+                            ki[0] += p2 * dax;
+                            ki[1] += p2 * day;
+                            ki[2] += p2 * daz;
+                            const double al = dax * qx + day * qy + daz * qz;
+                            const double p1 = 3. * al * p2 / (dr * dr);
+                            ki[0] -= p1 * qx;
+                            ki[1] -= p1 * qy;
+                            ki[2] -= p1 * qz;
+                        }
+                        if (j != i && (i != 0 || j != 1)) {
+                            const double dx = xj - bx(i);
+                            const double dy = yj - by(i);
+                            const double dz = zj - bz(i);
+                            const double dax = axj - bax(i);
+                            const double day = ayj - bay(i);
+                            const double daz = azj - baz(i);
+                            const double dr = sqrt(dx * dx + dy * dy + dz * dz);
+                            const double al = dax * dx + day * dy + daz * dz;
+                            const double p2 = Gc / (dr * dr * dr);
+                            const double p2i = p2 * bm(i);
+                            const double p2j = p2 * mj;
+                            kj[0] -= dax * p2i;
+                            kj[1] -= day * p2i;
+                            kj[2] -= daz * p2i;
+                            ki[0] += dax * p2j;
+                            ki[1] += day * p2j;
+                            ki[2] += daz * p2j;
+                            const double p1 = 3. * al * p2 / (dr * dr);
+                            const double p1i = p1 * bm(i);
+                            const double p1j = p1 * mj;
+                            kj[0] += dx * p1i;
+                            kj[1] += dy * p1i;
+                            kj[2] += dz * p1i;
+                            ki[0] -= dx * p1j;
+                            ki[1] -= dy * p1j;
+                            ki[2] -= dz * p1j;
+                        }
+                    }
+                    Ax += axj * mj;
+                    Ay += ayj * mj;
+                    Az += azj * mj;
+                    Rx += xj * mj;
+                    Ry += yj * mj;
+                    Rz += zj * mj;
+                    Mt += mj;
+                }
+            }
+            __syncthreads();
+            if (!en) return;
+            const double pf = dt * dt / 12.;
+            sax += pf * aux0[gbase][0];
+            say += pf * aux0[gbase][1];
+            saz += pf * aux0[gbase][2];
+            if (live) {
+                ax += pf * aux[lane][0];
+                ay += pf * aux[lane][1];
+                az += pf * aux[lane][2];
+            }
+        };
+        // reb_simulation_update_acceleration for the kernel in use
+        auto grav = [&](bool en) {
+            if (JAC && P.jacgrav)
+                gravity_jacobi(en);
+            else
+                gravity(en);
+        };
         // DH: integrator_whfast.c:494-501.  Jacobi: :439-483 (accelerations to
         // Jacobi form, transformations.c:147-176, then Eq. 132 of the WHFast
         // paper with the Kepler terms of the interior bodies).
@@ -714,7 +868,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                 hvx += h * jax;
                 hvy += h * jay;
                 hvz += h * jaz;
-                if (me > 1) {
+                if (me > 1 && !P.jacgrav) {   // the Jacobi gravity holds these terms already
                     const double rj2i = 1. / (px * px + py * py + pz * pz);
                     const double rji = sqrt(rj2i);
                     const double rj3iM = rji * rj2i * Gc * eta_me;
@@ -735,11 +889,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
         auto corrector_Z = [&](double a, double b, bool en) {
             kepler(a, en);
             to_inertial(en, false);
-            gravity(en);
+            grav(en);
             interaction(-b, en);
             kepler(-2. * a, en);
             to_inertial(en, false);
-            gravity(en);
+            grav(en);
             interaction(b, en);
             kepler(a, en);
         };
@@ -756,7 +910,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
         auto operator_C = [&](double a, double b, bool en) {
             kepler(a, en);
             to_inertial(en, false);
-            gravity(en);
+            grav(en);
             interaction(b, en);
             kepler(-a, en);
         };
@@ -774,12 +928,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
             operator_U(a, b, en);
             operator_U(-a, b, en);
         };
-        // integrator_whfast.c:1051-1111 (default kernel).  Every lane executes
-        // the barriers; `want` selects the systems that take part.
+        // integrator_whfast.c:1051-1111.  Every lane executes the barriers;
+        // `want` selects the systems that take part.  The composition kernel
+        // drifts 3/8 of a step, the others half a step.
         auto synchronize = [&](bool want) {
             const bool en = want && !sync;
-            kepler(dt / 2., en);
-            com(dt / 2., en);
+            const double hs = P.kernel == 2 ? 3. * dt / 8. : dt / 2.;
+            kepler(hs, en);
+            com(hs, en);
             if (JAC && P.corrector2 && __any(en)) apply_corrector2(-1., en);
             if (JAC && P.corrector && __any(en)) apply_corrector(-1., en);
             to_inertial(en, true);
@@ -798,7 +954,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
             }
         };
 
-        // integrator_whfast.c:1113-1259 (default kernel)
+        // integrator_whfast.c:1113-1259
+        const double h_first = P.kernel == 2 ? 5. * dt / 8. : dt / 2.;
         bool modified = P.did_modify || (sys_ok && (S.req & 2));
         for (long s = 0; s < P.nsteps; s++) {
             const bool act = s < my_steps;
@@ -813,16 +970,50 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                 if (P.corrector && __any(first)) apply_corrector(1., first);
                 if (P.corrector2 && __any(first)) apply_corrector2(1., first);
             }
-            kepler(sync ? dt / 2. : dt, act);
-            com(sync ? dt / 2. : dt, act);
+            kepler(sync ? h_first : dt, act);
+            com(sync ? h_first : dt, act);
             jump(dt / 2., act);
             // Inertial velocities of the middle of a step are visible only
             // after the system's last step of the launch (or are replaced by a
             // synchronization before anything reads them).
             to_inertial(act, act && s == my_steps - 1);
-            gravity(act);
-            interaction(dt, act);
-            jump(dt / 2., act);
+            grav(act);
+            if (!JAC || P.kernel == 0) {
+                interaction(dt, act);
+                jump(dt / 2., act);
+            } else if (P.kernel == 1) {
+                // integrator_whfast.c:1178-1187
+                modified_kick(act);
+                interaction(dt, act);
+            } else if (P.kernel == 2) {
+                // integrator_whfast.c:1189-1218: kicks of -dt/6, dt/6, dt,
+                // -dt/6, dt/6 between drifts of -dt/4, dt/8, -dt/8, dt/4,
+                // positions and accelerations refreshed before each kick
+                const double kick[5] = {-dt / 6., dt / 6., dt, -dt / 6., dt / 6.};
+                const double drift[4] = {-dt / 4., dt / 8., -dt / 8., dt / 4.};
+                for (int q = 0; q < 5; q++) {
+                    if (q > 0) {
+                        to_inertial(act, false);
+                        grav(act);
+                    }
+                    interaction(kick[q], act);
+                    if (q < 4) {
+                        kepler(drift[q], act);
+                        com(drift[q], act);
+                    }
+                }
+            } else {
+                // integrator_whfast.c:1220-1235 (WHT Eq. 10.6): positions
+                // moved by dt^2/12 a, accelerations there, then the kick
+                if (act && live) {
+                    const double pf = dt * dt / 12.;
+                    ix += pf * ax;
+                    iy += pf * ay;
+                    iz += pf * az;
+                }
+                grav(act);
+                interaction(dt, act);
+            }
             if (act) sync = 0;
             if (__any(act && safe)) synchronize(act && safe);
             if (act) modified = false;
@@ -945,8 +1136,8 @@ static bool whfast_check(struct reb_simulation* r, struct reb_integrator_whfast_
         reb_simulation_error(r, "Non-standard kernel requires Jacobi coordinates.");
         return false;
     }
-    if (s->kernel != REB_INTEGRATOR_WHFAST_KERNEL_DEFAULT) {
-        reb_simulation_error(r, "WHFast kernels other than the default are not supported by this engine.");
+    if (s->kernel > 3) {
+        reb_simulation_error(r, "Kernel method must be 0 (default), 1 (exact modified kick), 2 (composition kernel), or 3 (lazy implementer's modified kick). ");
         return false;
     }
     if (s->corrector != 0 && s->coordinates != REB_INTEGRATOR_WHFAST_COORDINATES_JACOBI) {
@@ -969,8 +1160,14 @@ static bool whfast_check(struct reb_simulation* r, struct reb_integrator_whfast_
         reb_simulation_error(r, "keep_unsynchronized is not supported by this engine.");
         return false;
     }
-    if (r->gravity != REB_GRAVITY_BASIC && r->gravity != REB_GRAVITY_COMPENSATED) {
-        reb_simulation_error(r, "WHFast on this engine evaluates the interaction with BASIC or COMPENSATED gravity.");
+    const bool jac = s->coordinates == REB_INTEGRATOR_WHFAST_COORDINATES_JACOBI;
+    // integrator_whfast.c:929-939: the modified-kick and lazy kernels switch
+    // the simulation to the Jacobi gravity
+    if (s->kernel == 1 || s->kernel == 3) r->gravity = REB_GRAVITY_JACOBI;
+    if (r->gravity != REB_GRAVITY_BASIC && r->gravity != REB_GRAVITY_COMPENSATED &&
+        !(jac && r->gravity == REB_GRAVITY_JACOBI)) {
+        reb_simulation_error(r, "WHFast on this engine evaluates the interaction with BASIC or COMPENSATED gravity "
+                                "(or the Jacobi gravity in Jacobi coordinates).");
         return false;
     }
     if (r->additional_forces || dev_of(r)->dev_forces) {
@@ -987,9 +1184,8 @@ static bool whfast_check(struct reb_simulation* r, struct reb_integrator_whfast_
         reb_simulation_error(r, "Ghost boxes are not supported with WHFast.");
         return false;
     }
-    r->gravity_ignore_terms = s->coordinates == REB_INTEGRATOR_WHFAST_COORDINATES_JACOBI
-                                  ? REB_GRAVITY_IGNORE_TERMS_BETWEEN_0_AND_1
-                                  : REB_GRAVITY_IGNORE_TERMS_INVOLVING_0;
+    if (s->kernel != 1 && s->kernel != 3)
+        r->gravity_ignore_terms = jac ? REB_GRAVITY_IGNORE_TERMS_BETWEEN_0_AND_1 : REB_GRAVITY_IGNORE_TERMS_INVOLVING_0;
     if (s->N_allocated != r->N) {
         s->N_allocated = r->N;
         s->p_jh = (struct reb_particle*)realloc(s->p_jh, sizeof(struct reb_particle) * r->N);
@@ -1136,6 +1332,8 @@ bool whfast_run(struct reb_simulation* r, struct reb_integrator_whfast_state* s,
     P.coords = s->coordinates == REB_INTEGRATOR_WHFAST_COORDINATES_JACOBI ? WH_JACOBI : WH_DH;
     P.corrector = (int)s->corrector;
     P.corrector2 = s->corrector2 ? 1 : 0;
+    P.kernel = (int)s->kernel;
+    P.jacgrav = r->gravity == REB_GRAVITY_JACOBI ? 1 : 0;
     if (whfast_launch(P, dev_of(r)->stream)) return RB_HIP(r, hipGetLastError());
     return whfast_finish(r, s, P);
 }

@@ -55,6 +55,8 @@ struct WHParams {
     int coords = WH_DH;       // WH_DH or WH_JACOBI
     int corrector = 0;        // first symplectic corrector order (Jacobi): 0, 3, 5, 7, 11, 17
     int corrector2 = 0;       // second corrector (Jacobi): 0 or 1
+    int kernel = 0;           // 0 default, 1 modified kick, 2 composition, 3 lazy (Jacobi)
+    int jacgrav = 0;          // accelerations by the Jacobi gravity (kernels 1 and 3)
 };
 
 int whfast_launch(const WHParams& P, hipStream_t st);

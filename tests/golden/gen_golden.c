@@ -703,6 +703,70 @@ static void gen_whfast_jacobi(void){
     reb_simulation_free(g);
 }
 
+/* The WHFast kernels (modified kick, composition, lazy implementer's) in
+ * Jacobi coordinates on the same 16 systems as the Jacobi fixtures: modes
+ * {kernel, safe_mode, corrector, corrector2, compensated}; bodies after three
+ * reb_simulation_steps calls of 100 steps, energies, and r->gravity after. */
+static void gen_whfast_kernels(void){
+    const int NS = 16;
+    const int nsteps = 100;
+    enum { NMODE = 6 };
+    static const int64_t modes[NMODE][5] = {
+        {1, 0, 0, 0, 0},
+        {1, 1, 5, 1, 0},
+        {2, 0, 0, 0, 0},
+        {2, 1, 3, 0, 1},
+        {3, 0, 0, 0, 0},
+        {3, 0, 7, 1, 0},
+    };
+    struct reb_simulation* g = reb_simulation_create();
+    g->rand_seed = 4242u;
+    struct reb_simulation* sys[NS];
+    size_t rows = 0;
+    double dts[NS];
+    for (int s=0; s<NS; s++){
+        sys[s] = ens_system(g, 1 + s % 8, 1 + s % 8);
+        dts[s] = 2.*M_PI*sqrt(1./(1.+sys[s]->particles[1].m))/20.;
+        rows += sys[s]->N;
+    }
+    wi("whfast_kern16_modes.npy", &modes[0][0], NMODE, 5);
+    for (int md=0; md<NMODE; md++){
+        double* fin = calloc(rows*NCOL, sizeof(double));
+        double* en = calloc(NS*4, sizeof(double));
+        int64_t grav[NS];
+        size_t row = 0;
+        for (int s=0; s<NS; s++){
+            struct reb_simulation* r = reb_simulation_copy(sys[s]);
+            struct reb_integrator_whfast_state* wh = reb_simulation_set_integrator(r, "whfast");
+            wh->coordinates = REB_INTEGRATOR_WHFAST_COORDINATES_JACOBI;
+            wh->kernel = (unsigned int)modes[md][0];
+            wh->safe_mode = (unsigned int)modes[md][1];
+            wh->corrector = (unsigned int)modes[md][2];
+            wh->corrector2 = (unsigned int)modes[md][3];
+            r->gravity = modes[md][4] ? REB_GRAVITY_COMPENSATED : REB_GRAVITY_BASIC;
+            r->dt = dts[s];
+            en[s*4] = reb_simulation_energy(r);
+            for (int c=0; c<3; c++){
+                reb_simulation_steps(r, (size_t)nsteps);
+                en[s*4+1+c] = reb_simulation_energy(r);
+            }
+            grav[s] = (int64_t)r->gravity;
+            double* st = state_of(r);
+            memcpy(fin+row*NCOL, st, sizeof(double)*NCOL*r->N);
+            free(st);
+            row += r->N;
+            reb_simulation_free(r);
+        }
+        char name[128];
+        snprintf(name, sizeof(name), "whfast_kern16_mode%d_final.npy", md); wd(name, fin, rows, NCOL);
+        snprintf(name, sizeof(name), "whfast_kern16_mode%d_energy.npy", md); wd(name, en, NS, 4);
+        snprintf(name, sizeof(name), "whfast_kern16_mode%d_gravity.npy", md); wi(name, grav, NS, 0);
+        free(fin); free(en);
+    }
+    for (int s=0; s<NS; s++) reb_simulation_free(sys[s]);
+    reb_simulation_free(g);
+}
+
 /* reb_collision_resolve_merge on random three-body states: inputs, the
  * collision (p1, p2), track_energy_offset and N_active; outputs the bodies,
  * the outcome, energy_offset and G. */
@@ -827,6 +891,7 @@ int main(int argc, char** argv){
     gen_sheet();
     gen_merge();
     gen_whfast_jacobi();
+    gen_whfast_kernels();
     gen_line();
     gen_archive();
     return 0;

@@ -153,6 +153,49 @@ def test_whfast_jacobi_bitwise(gpu, mode):
         row += n
 
 
+@pytest.mark.parametrize("mode", range(6))
+def test_whfast_kernels_bitwise(gpu, mode):
+    """The modified-kick, composition and lazy implementer's kernels in
+    Jacobi coordinates (with correctors, safe mode and COMPENSATED gravity per
+    the mode table): bodies with their accelerations, energies after three
+    reb_simulation_steps calls of 100 steps, and r->gravity afterwards (the
+    Jacobi gravity for kernels 1 and 3) as the reference leaves them."""
+    kern, safe, c1, c2, comp = (int(v) for v in golden("whfast_kern16_modes.npy")[mode])
+    fin = golden("whfast_kern16_mode%d_final.npy" % mode)
+    en = golden("whfast_kern16_mode%d_energy.npy" % mode)
+    grav = golden("whfast_kern16_mode%d_gravity.npy" % mode)
+    row = 0
+    for k, (s0, dt) in enumerate(_jacobi_cases()):
+        n = s0.shape[0]
+        sim = sim_from(gpu, s0, dt=dt)
+        wh = sim.set_integrator("whfast")
+        wh.kernel = kern
+        wh.safe_mode = safe
+        wh.corrector = c1
+        wh.corrector2 = c2
+        sim.s.gravity = gpu.GRAVITY["compensated" if comp else "basic"]
+        sim.precision = "exact"
+        E = [sim.energy()]
+        for _ in range(3):
+            sim.steps(100)
+            E.append(sim.energy())
+        assert np.array_equal(sim.state()[:, :9], fin[row:row + n, :9]), (mode, k)
+        assert np.array_equal(np.array(E), en[k]), (mode, k)
+        assert sim.s.gravity == grav[k]
+        row += n
+
+
+def test_gravity_jacobi_update_acceleration(gpu, orc):
+    """reb_simulation_update_acceleration with REB_GRAVITY_JACOBI
+    (gravity.c:108-163) equals the restatement bit for bit."""
+    for s0, _ in _jacobi_cases()[:6]:
+        sim = sim_from(gpu, s0)
+        sim.s.G = 1.3
+        sim.s.gravity = gpu.GRAVITY["jacobi"]
+        acc = sim.update_acceleration()
+        assert np.array_equal(acc, orc.gravity_jacobi(s0, 1.3))
+
+
 def test_whfast_rejects_unsupported(gpu):
     s0, dt = _systems()[0]
     sim = sim_from(gpu, s0, dt=dt)
@@ -170,6 +213,17 @@ def test_whfast_rejects_unsupported(gpu):
     wh.coordinates = gpu.WHFAST_COORDINATES["democraticheliocentric"]
     wh.corrector = 3
     with pytest.raises(gpu.RebError, match="only compatible with Jacobi"):
+        sim.steps(1)
+    sim = sim_from(gpu, s0, dt=dt)
+    wh = sim.set_integrator("whfast")
+    wh.kernel = 4
+    with pytest.raises(gpu.RebError, match="Kernel method must be 0"):
+        sim.steps(1)
+    sim = sim_from(gpu, s0, dt=dt)
+    wh = sim.set_integrator("whfast")
+    wh.coordinates = gpu.WHFAST_COORDINATES["democraticheliocentric"]
+    wh.kernel = 2
+    with pytest.raises(gpu.RebError, match="Non-standard kernel requires Jacobi"):
         sim.steps(1)
 
 
