@@ -170,6 +170,11 @@ REB_API void reb_amd_ensemble_free(struct reb_amd_ensemble* e);
  * struct reb_integrator_whfast_state (integrator_whfast.h:28-46). */
 REB_API int reb_amd_ensemble_set_whfast(struct reb_amd_ensemble* e, int coordinates, unsigned int corrector,
                                         unsigned int corrector2);
+/* Before the first steps, after Jacobi coordinates are selected: the WHFast
+ * kernel (field kernel of integrator_whfast.h:28-46): 0 default, 1 modified
+ * kick, 2 composition, 3 lazy implementer's; 1 and 3 use the Jacobi gravity
+ * (gravity.c:108-163).  Replaces setting the field on each reb_simulation. */
+REB_API int reb_amd_ensemble_set_whfast_kernel(struct reb_amd_ensemble* e, unsigned int kernel);
 /* Queues n steps of every system, ending synchronized; does not wait. */
 REB_API int reb_amd_ensemble_steps(struct reb_amd_ensemble* e, uint64_t n_steps);
 REB_API int reb_amd_ensemble_wait(struct reb_amd_ensemble* e);

@@ -185,6 +185,7 @@ def declare(lib):
         "reb_amd_ensemble_free": (None, [c_void_p]),
         "reb_amd_set_device_forces": (None, [S, c_void_p, c_void_p]),
         "reb_amd_ensemble_set_whfast": (c_int, [c_void_p, c_int, ctypes.c_uint, ctypes.c_uint]),
+        "reb_amd_ensemble_set_whfast_kernel": (c_int, [c_void_p, ctypes.c_uint]),
         "reb_amd_ensemble_steps": (c_int, [c_void_p, ctypes.c_uint64]),
         "reb_amd_ensemble_wait": (c_int, [c_void_p]),
         "reb_amd_ensemble_steps_done": (ctypes.c_uint64, [c_void_p]),

@@ -551,21 +551,6 @@ __global__ __launch_bounds__(64) void k_tree_gravity(const GravRec* __restrict__
     const double xi = live ? x[me] : 0., yi = live ? y[me] : 0., zi = live ? z[me] : 0.;
     double a_x = 0., a_y = 0., a_z = 0.;
     unsigned long long inter = 0;
-    // Software pipeline: the walk decides cell k while the force term of the
-    // lane's previous accepted cell (sqrt, division) is finished; terms are
-    // still added in the walk's order, one by one.
-    bool pend = false;
-    double pdx = 0., pdy = 0., pdz = 0., pr2 = 0., pm = 0.;
-    auto finish = [&]() {
-        if (pend) {
-            const double _r = sqrt(pr2 + soft2);
-            const double prefact = -G / (_r * _r * _r) * pm;
-            a_x += prefact * pdx;
-            a_y += prefact * pdy;
-            a_z += prefact * pdz;
-            inter++;
-        }
-    };
     for (int g = 0; g < gl.n; g++) {
         const double gx = gl.g[g][0] + xi, gy = gl.g[g][1] + yi, gz = gl.g[g][2] + zi;
         int skip = live ? 0 : 0x7fffffff;
@@ -577,14 +562,13 @@ __global__ __launch_bounds__(64) void k_tree_gravity(const GravRec* __restrict__
             const GravRec r_open = rec[c_open < total ? c_open : total - 1];
             const GravRec r_skip = rec[c_skip < total ? c_skip : total - 1];
             const int pt = cur.pt;
-            bool open = false, use = false;
-            double dx = 0., dy = 0., dz = 0., r2 = 0.;
+            bool open = false;
             if (c >= skip) {
-                dx = gx - cur.mx;
-                dy = gy - cur.my;
-                dz = gz - cur.mz;
-                r2 = dx * dx + dy * dy + dz * dz;
-                use = true;
+                const double dx = gx - cur.mx;
+                const double dy = gy - cur.my;
+                const double dz = gz - cur.mz;
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                bool use = true;
                 if (pt < 0) {
                     const double w = cur.w;
                     if (w * w > theta2 * r2) {
@@ -596,20 +580,19 @@ __global__ __launch_bounds__(64) void k_tree_gravity(const GravRec* __restrict__
                 } else if (pt == me) {
                     use = false;
                 }
+                if (use) {
+                    const double _r = sqrt(r2 + soft2);
+                    const double prefact = -G / (_r * _r * _r) * cur.m;
+                    a_x += prefact * dx;
+                    a_y += prefact * dy;
+                    a_z += prefact * dz;
+                    inter++;
+                }
             }
             const bool any = __ballot(open) != 0ull;
-            finish();
-            pend = use;
-            pdx = dx;
-            pdy = dy;
-            pdz = dz;
-            pr2 = r2;
-            pm = cur.m;
             c = any ? c_open : c_skip;
             cur = any ? r_open : r_skip;
         }
-        finish();
-        pend = false;
     }
     if (live) {
         ax[me] = a_x;

@@ -93,7 +93,7 @@ class Ensemble:
     "jacobi", the latter with optional symplectic correctors."""
 
     def __init__(self, n_bodies, state, dt, G=1.0, safe_mode=0, gravity="basic", coordinates="democraticheliocentric",
-                 corrector=0, corrector2=0):
+                 corrector=0, corrector2=0, kernel=0):
         L = lib()
         n_bodies = np.ascontiguousarray(n_bodies, dtype=np.uint64)
         state = np.asarray(state, dtype=np.float64)
@@ -118,6 +118,11 @@ class Ensemble:
         if coordinates != "democraticheliocentric" or corrector or corrector2:
             if L.reb_amd_ensemble_set_whfast(self._h, _abi.WHFAST_COORDINATES[coordinates], int(corrector),
                                              int(corrector2)):
+                msg = L.reb_amd_last_error().decode()
+                self.free()
+                raise RebError(msg)
+        if kernel:
+            if L.reb_amd_ensemble_set_whfast_kernel(self._h, int(kernel)):
                 msg = L.reb_amd_last_error().decode()
                 self.free()
                 raise RebError(msg)

@@ -98,6 +98,33 @@ def test_ensemble_whfast_jacobi(gpu, mode):
     ens.free()
 
 
+@pytest.mark.parametrize("mode", range(6))
+def test_ensemble_whfast_kernels(gpu, mode):
+    """The modified-kick, composition and lazy implementer's kernels: the
+    per-simulation fixtures (three steps(100) calls), all sixteen systems as
+    one ensemble, bodies with accelerations and energies bit for bit."""
+    kern, safe, c1, c2, comp = (int(v) for v in golden("whfast_kern16_modes.npy")[mode])
+    nb = golden("whfast_jac16_nbodies.npy")
+    ens = gpu.Ensemble(nb, golden("whfast_jac16_init.npy"), golden("whfast_jac16_dt.npy"), safe_mode=safe,
+                       gravity="compensated" if comp else "basic", coordinates="jacobi", corrector=c1,
+                       corrector2=c2, kernel=kern)
+    E = [ens.energy()]
+    for _ in range(3):
+        ens.steps(100)
+        E.append(ens.energy())
+    assert np.array_equal(np.stack(E, axis=1), golden("whfast_kern16_mode%d_energy.npy" % mode))
+    assert np.array_equal(ens.state()[:, :9], golden("whfast_kern16_mode%d_final.npy" % mode)[:, :9])
+    ens.free()
+
+
+def test_ensemble_whfast_kernel_errors(gpu):
+    nb, init, dt, en, fin = _fixture("ensemble200_seed99")
+    with pytest.raises(gpu.RebError, match="requires Jacobi"):
+        gpu.Ensemble(nb, init, dt, kernel=2)
+    with pytest.raises(gpu.RebError, match="kernel method must be"):
+        gpu.Ensemble(nb, init, dt, coordinates="jacobi", kernel=4)
+
+
 def test_ensemble_tune_keeps_state(gpu):
     nb, init, dt, en, fin = _fixture("ensemble200_seed99")
     ens = gpu.Ensemble(nb, init, dt)
