@@ -40,9 +40,10 @@ namespace rbamd {
 static constexpr int kMaxGBT = 64;   // keeps the by-value argument under 4 KB
 // `info` block: ints 0..7 (build results and flags), bytes 32..39 the
 // interaction counter, bytes 64.. the per-block top-two radii of k_top2
-// (at most kTop2Blocks blocks, two doubles each).
+// (at most kTop2Blocks blocks, two doubles each), then the second largest
+// radius of all bodies.
 static constexpr int kTop2Blocks = 64;
-static constexpr size_t kInfoBytes = 64 + 2 * sizeof(double) * kTop2Blocks;
+static constexpr size_t kInfoBytes = 64 + 2 * sizeof(double) * kTop2Blocks + 16;
 struct GBT {
     int n;
     double g[kMaxGBT][6];
@@ -459,6 +460,7 @@ bool tree_build(struct reb_simulation* r) {
     int* info = T->info.as<int>();
     bool two = false;
     int hinfo[4] = {0, 0, 0, 0};
+    int last[2] = {0, 0};
     for (int attempt = 0; attempt < 2; attempt++) {
         two = attempt == 1;
         if (!RB_HIP(r, hipMemsetAsync(info, 0, 16, d->stream))) return false;
@@ -471,7 +473,18 @@ bool tree_build(struct reb_simulation* r) {
                            T->key0.as<unsigned long long>(), T->key1.as<unsigned long long>(), n, g.L1, g.L2,
                            two ? 1 : 0, T->idx.as<int>(), d->field(FX), d->field(FY), d->field(FZ),
                            T->lcp.as<int>(), info);
-        if (!RB_HIP(r, hipMemcpyAsync(hinfo, info, 16, hipMemcpyDeviceToHost, d->stream))) return false;
+        // cells per body and their offsets, read back with the build flags in
+        // one synchronization (used only when the flags report no error)
+        hipLaunchKernelGGL(k_tree_count, dim3((n + 255) / 256), dim3(256), 0, d->stream, T->lcp.as<int>(), n,
+                           T->cnt.as<int>(), info);
+        size_t tmp = 0;
+        hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, T->cnt.as<int>(), T->base.as<int>(), n, d->stream);
+        if (!T->cub.ensure(tmp + 256)) return false;
+        hipcub::DeviceScan::ExclusiveSum(T->cub.p, tmp, T->cnt.as<int>(), T->base.as<int>(), n, d->stream);
+        if (!RB_HIP(r, hipMemcpyAsync(&last[0], T->base.as<int>() + (n - 1), 4, hipMemcpyDeviceToHost, d->stream)) ||
+            !RB_HIP(r, hipMemcpyAsync(&last[1], T->cnt.as<int>() + (n - 1), 4, hipMemcpyDeviceToHost, d->stream)) ||
+            !RB_HIP(r, hipMemcpyAsync(hinfo, info, 16, hipMemcpyDeviceToHost, d->stream)))
+            return false;
         if (!stream_sync(r)) return false;
         if (hinfo[2] & 1) {
             reb_simulation_error(r, "Particle is outside of simulation box. Cannot add to tree.");
@@ -494,19 +507,6 @@ bool tree_build(struct reb_simulation* r) {
     T->two_words = two;
     T->nnodes = 0;
     if (n == 0) return true;
-    hipLaunchKernelGGL(k_tree_count, dim3((n + 255) / 256), dim3(256), 0, d->stream, T->lcp.as<int>(), n,
-                       T->cnt.as<int>(), info);
-    size_t tmp = 0;
-    hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, T->cnt.as<int>(), T->base.as<int>(), n, d->stream);
-    if (!T->cub.ensure(tmp + 256)) return false;
-    hipcub::DeviceScan::ExclusiveSum(T->cub.p, tmp, T->cnt.as<int>(), T->base.as<int>(), n, d->stream);
-    int last[2] = {0, 0};
-    if (!RB_HIP(r, hipMemcpyAsync(&last[0], T->base.as<int>() + (n - 1), 4, hipMemcpyDeviceToHost, d->stream)))
-        return false;
-    if (!RB_HIP(r, hipMemcpyAsync(&last[1], T->cnt.as<int>() + (n - 1), 4, hipMemcpyDeviceToHost, d->stream)))
-        return false;
-    if (!RB_HIP(r, hipMemcpyAsync(hinfo, info, 16, hipMemcpyDeviceToHost, d->stream))) return false;
-    if (!stream_sync(r)) return false;
     const int total = last[0] + last[1];
     T->maxlevel = hinfo[1];
     if ((size_t)total > T->node_cap) {
@@ -675,6 +675,23 @@ __global__ __launch_bounds__(256) void k_top2(const double* __restrict__ rad, in
     }
 }
 
+// The second largest radius over the blocks' top two (simulation.c:718-799),
+// left on the device for the search (0 for fewer than two bodies).
+__global__ void k_top2_final(const double* __restrict__ top, int nb, int n, double* __restrict__ second) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double b1 = -1.0, b2 = -1.0;
+    for (int k = 0; k < 2 * nb; k++) {
+        const double v = top[k];
+        if (v > b1) {
+            b2 = b1;
+            b1 = v;
+        } else if (v > b2) {
+            b2 = v;
+        }
+    }
+    *second = n >= 2 ? b2 : 0.;
+}
+
 // Swept-sphere test of LINE and LINETREE (collision.c:150-171, :510-534):
 // the closest distance of two bodies moving on straight lines over the last
 // step (dt_last_done), from the separation d1 and relative velocity dv now,
@@ -736,7 +753,8 @@ static constexpr int kCollSlots = 32;
 // kCollSlots in the body's slots, raising *overflow when a body has more;
 // MODE 3 stores at their offsets the collisions of the bodies whose count
 // exceeds kCollSlots, and a wave without such a body returns at once.
-// TREE passes the second largest radius as `reach`; LINETREE the largest
+// TREE passes the second largest radius in device memory (`reach_d`); LINETREE
+// passes `reach`, the largest
 // drift of the last step, dt_last_done*|v|max.
 template <int MODE, bool LINE = false>
 __global__ __launch_bounds__(64) void k_tree_collide(const CollCell* __restrict__ cells, int total,
@@ -747,7 +765,9 @@ __global__ __launch_bounds__(64) void k_tree_collide(const CollCell* __restrict_
                                                      const double* __restrict__ rad, GBT gl, double reach,
                                                      int* __restrict__ counts, const int* __restrict__ offsets,
                                                      CollRec* __restrict__ rec, double dt_last = 0.,
-                                                     int* __restrict__ overflow = nullptr) {
+                                                     int* __restrict__ overflow = nullptr,
+                                                     const double* __restrict__ reach_d = nullptr) {
+    if (reach_d) reach = *reach_d;
     constexpr bool WRITE = MODE == 1 || MODE == 3;
     const int s = blockIdx.x * 64 + threadIdx.x;
     const int me = s < n ? idx[s] : -1;
@@ -1020,7 +1040,7 @@ void coll_release(struct reb_simulation* r) {
 // body; when a body has more, a second pass searches again for those bodies.
 template <bool LINE>
 static long tree_collide_run(struct reb_simulation* r, TreeBuffers* T, int n, const GBT& gl, double reach,
-                             double dt_last) {
+                             double dt_last, const double* reach_d = nullptr) {
     DeviceState* d = dev_of(r);
     if (!ensure_coll(r, T, n, 1)) return -1;
     if (!T->coll_slot.ensure(sizeof(CollRec) * kCollSlots * (size_t)n)) {
@@ -1033,7 +1053,7 @@ static long tree_collide_run(struct reb_simulation* r, TreeBuffers* T, int n, co
     hipLaunchKernelGGL((k_tree_collide<2, LINE>), grid, dim3(64), 0, d->stream, T->crec.as<CollCell>(), T->nnodes,
                        T->idx.as<int>(), n, d->field(FX), d->field(FY), d->field(FZ), d->field(FVX), d->field(FVY),
                        d->field(FVZ), d->field(FR), gl, reach, T->coll_cnt.as<int>(), nullptr,
-                       T->coll_slot.as<CollRec>(), dt_last, overflow);
+                       T->coll_slot.as<CollRec>(), dt_last, overflow, reach_d);
     int hover = 0;
     if (!RB_HIP(r, hipMemcpyAsync(&hover, overflow, sizeof(int), hipMemcpyDeviceToHost, d->stream))) return -1;
     const long total = scan_counts(r, T, n);   // synchronizes: hover is current
@@ -1046,7 +1066,7 @@ static long tree_collide_run(struct reb_simulation* r, TreeBuffers* T, int n, co
         hipLaunchKernelGGL((k_tree_collide<3, LINE>), grid, dim3(64), 0, d->stream, T->crec.as<CollCell>(),
                            T->nnodes, T->idx.as<int>(), n, d->field(FX), d->field(FY), d->field(FZ), d->field(FVX),
                            d->field(FVY), d->field(FVZ), d->field(FR), gl, reach, T->coll_cnt.as<int>(),
-                           T->coll_off.as<int>(), T->coll_rec.as<CollRec>(), dt_last, nullptr);
+                           T->coll_off.as<int>(), T->coll_rec.as<CollRec>(), dt_last, nullptr, reach_d);
     // root index of each record (the lcp buffer is free now: reuse as pos_of)
     int* pos_of = T->lcp.as<int>();
     hipLaunchKernelGGL(k_inverse_perm, dim3((n + 255) / 256), dim3(256), 0, d->stream, T->idx.as<int>(), pos_of, n);
@@ -1102,25 +1122,13 @@ bool collision_search_device(struct reb_simulation* r) {
         }
         case REB_COLLISION_TREE: {
             if (!tree_build(r)) return false;
-            // second largest radius
+            // second largest radius, left on the device for the search
             double* top = reinterpret_cast<double*>(T->info.as<int>() + 16);
+            double* second_r = top + 2 * kTop2Blocks;
             const int nb = std::min(kTop2Blocks, (n + 255) / 256);
             hipLaunchKernelGGL(k_top2, dim3(nb), dim3(256), 0, d->stream, d->field(FR), n, top);
-            std::vector<double> h(2 * nb);
-            if (!RB_HIP(r, hipMemcpyAsync(h.data(), top, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, d->stream)))
-                return false;
-            if (!stream_sync(r)) return false;
-            double b1 = -1.0, b2 = -1.0;
-            for (double v : h) {
-                if (v > b1) {
-                    b2 = b1;
-                    b1 = v;
-                } else if (v > b2) {
-                    b2 = v;
-                }
-            }
-            const double second_r = (n >= 2) ? b2 : 0.;
-            const long total = tree_collide_run<false>(r, T, n, gl, second_r, 0.);
+            hipLaunchKernelGGL(k_top2_final, dim3(1), dim3(64), 0, d->stream, top, nb, n, second_r);
+            const long total = tree_collide_run<false>(r, T, n, gl, 0., 0., second_r);
             if (total < 0) return false;
             return fetch_collisions(r, T, total, gl);
         }
