@@ -542,16 +542,38 @@ __global__ __launch_bounds__(kFast) __attribute__((amdgpu_waves_per_eu(RB_FAST_W
 __device__ __forceinline__ void reduce_one(const double* __restrict__ partial, int pstride, int nchunk, int i,
                                            double negG, int accumulate, int ignore, double* __restrict__ ax,
                                            double* __restrict__ ay, double* __restrict__ az, const KickDrift& kd) {
-    double out[3];
-    for (int c = 0; c < 3; c++) {
-        double s = 0., k = 0.;
-        for (int q = 0; q < nchunk; q++) {
-            const double* p = partial + (size_t)q * 6 * pstride;
-            kahan_add(p[c * pstride + i], s, k);
-            kahan_add(-p[(c + 3) * pstride + i], s, k);
+    // Per component, chunk by chunk: the chunk's sum, then its carry with the
+    // sign flipped, Kahan-added in chunk order.  The loads of eight chunks are
+    // issued before their additions, so their latencies overlap.
+    double s[3] = {0., 0., 0.}, k[3] = {0., 0., 0.};
+    constexpr int B = 8;
+    int q = 0;
+    for (; q + B <= nchunk; q += B) {
+        double v[B][6];
+#pragma unroll
+        for (int u = 0; u < B; u++) {
+            const double* p = partial + (size_t)(q + u) * 6 * pstride + i;
+#pragma unroll
+            for (int f = 0; f < 6; f++) v[u][f] = p[(size_t)f * pstride];
         }
-        out[c] = negG * s;
+#pragma unroll
+        for (int u = 0; u < B; u++) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                kahan_add(v[u][c], s[c], k[c]);
+                kahan_add(-v[u][c + 3], s[c], k[c]);
+            }
+        }
     }
+    for (; q < nchunk; q++) {
+        const double* p = partial + (size_t)q * 6 * pstride + i;
+        for (int c = 0; c < 3; c++) {
+            kahan_add(p[(size_t)c * pstride], s[c], k[c]);
+            kahan_add(-p[(size_t)(c + 3) * pstride], s[c], k[c]);
+        }
+    }
+    double out[3];
+    for (int c = 0; c < 3; c++) out[c] = negG * s[c];
     if (ignore == 2 && i == 0) out[0] = out[1] = out[2] = 0.;
     if (accumulate) {
         out[0] = ax[i] + out[0];
