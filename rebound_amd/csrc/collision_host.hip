@@ -474,6 +474,11 @@ struct rbamd::ResolveScratch {
     int n = 0;
     bool ready = false;
     std::vector<int> level, last, hstart, horder;
+    // rand_r draws made ahead from the seed `draw_seed` while the device
+    // searched: draws[i] is the i-th result, seeds[i] the seed after it
+    std::vector<int> draws;
+    std::vector<unsigned int> seeds;
+    unsigned int draw_seed = 0;
     std::vector<double> plog;
     std::vector<unsigned char> hit;
     void release() {
@@ -501,6 +506,20 @@ void resolve_release(struct reb_simulation* r) {
     if (d->resolve) d->resolve->release();
     delete d->resolve;
     d->resolve = nullptr;
+}
+
+void compact_resolve_draw_ahead(struct reb_simulation* r, size_t expect) {
+    DeviceState* d = dev_of(r);
+    if (!d->resolve) d->resolve = new ResolveScratch();
+    ResolveScratch& W = *d->resolve;
+    W.draws.resize(expect);
+    W.seeds.resize(expect);
+    unsigned int seed = r->rand_seed;
+    W.draw_seed = seed;
+    for (size_t i = 0; i < expect; i++) {
+        W.draws[i] = rand_r_step(&seed);
+        W.seeds[i] = seed;
+    }
 }
 
 bool compact_resolve_eligible(const struct reb_simulation* r, int n_ghostboxes) {
@@ -715,10 +734,18 @@ static void resolve_hardsphere_compact(struct reb_simulation* r) {
     // while the body records are on their way
     int* perm = W.perm_h;
     for (int k = 0; k < nc; k++) perm[k] = k;
-    for (int i = 0; i < nc; i++) {
+    // the draws made ahead, if they start from the current seed
+    const int ahead = W.draw_seed == r->rand_seed ? std::min(nc, (int)W.draws.size()) : 0;
+    for (int i = 0; i < ahead; i++) {
+        const int nw = (int)((size_t)W.draws[i] % (size_t)nc);
+        std::swap(perm[i], perm[nw]);
+    }
+    if (ahead > 0) r->rand_seed = W.seeds[ahead - 1];
+    for (int i = ahead; i < nc; i++) {
         const int nw = (int)((size_t)rand_r_step(&r->rand_seed) % (size_t)nc);
         std::swap(perm[i], perm[nw]);
     }
+    W.draws.clear();
     CollGB gb;
     gb.n = (int)L.gb.size();
     for (int k = 0; k < gb.n; k++) gb.g[k] = L.gb[k];

@@ -205,6 +205,9 @@ void resolve_release(struct reb_simulation* r);
 // next stream synchronization compact_resolve_gather packs them and starts
 // their copy to the host.  False on an allocation failure.
 bool compact_resolve_eligible(const struct reb_simulation* r, int n_ghostboxes);
+// Draws `expect` rand_r results from r->rand_seed ahead of the shuffle (while
+// the device searches), without advancing r->rand_seed.
+void compact_resolve_draw_ahead(struct reb_simulation* r, size_t expect);
 bool compact_resolve_mark(struct reb_simulation* r, const CollRecH* rec, size_t nc);
 bool compact_resolve_gather(struct reb_simulation* r, const CollRecH* rec, size_t nc);
 // r->collisions lives in pinned host memory (the device copies the shuffled

@@ -50,6 +50,7 @@ struct GBT {
 };
 
 struct TreeBuffers {
+    size_t last_total = 0;                        // collisions found by the last tree search
     DevBuf key0, key1, idx, key0b, key1b, idxb;   // keys and body permutation
     DevBuf lcp, cnt, base;                        // per body
     DevBuf cub;                                   // hipCUB temporary storage
@@ -1054,10 +1055,14 @@ static long tree_collide_run(struct reb_simulation* r, TreeBuffers* T, int n, co
                        T->idx.as<int>(), n, d->field(FX), d->field(FY), d->field(FZ), d->field(FVX), d->field(FVY),
                        d->field(FVZ), d->field(FR), gl, reach, T->coll_cnt.as<int>(), nullptr,
                        T->coll_slot.as<CollRec>(), dt_last, overflow, reach_d);
+    // while the device searches: the shuffle's random draws, sized by the
+    // last search (the resolve draws any further ones itself)
+    if (compact_resolve_eligible(r, gl.n)) compact_resolve_draw_ahead(r, T->last_total + T->last_total / 4 + 64);
     int hover = 0;
     if (!RB_HIP(r, hipMemcpyAsync(&hover, overflow, sizeof(int), hipMemcpyDeviceToHost, d->stream))) return -1;
     const long total = scan_counts(r, T, n);   // synchronizes: hover is current
     if (total < 0) return -1;
+    T->last_total = (size_t)total;
     if (!ensure_coll(r, T, n, (size_t)total + 1)) return -1;
     if (total == 0) return 0;
     hipLaunchKernelGGL(k_coll_compact, dim3((n + 255) / 256), dim3(256), 0, d->stream, T->coll_cnt.as<int>(),
