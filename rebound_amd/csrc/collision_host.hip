@@ -754,6 +754,13 @@ static void resolve_hardsphere_compact(struct reb_simulation* r) {
                        W.perm.as<int>(), nc, W.cidx.as<int>(), gb, W.pair.as<HSPair>(),
                        W.coll.as<struct reb_collision>());
     if (!RB_HIP(r, hipGetLastError())) return;
+    // plog starts at -0.0, the exact identity of the sum after the resolve,
+    // so that the collisions that do not bounce need no branch (cleared here,
+    // while the device works on the levels)
+    std::vector<double>& plog = W.plog;
+    std::vector<unsigned char>& hit = W.hit;
+    plog.assign(nc, -0.);
+    hit.assign(nc, 0);
     int nlev = 0;
     const int lv = device_levels(r, W, nc, n, &nlev);
     if (lv < 0) return;
@@ -780,12 +787,6 @@ static void resolve_hardsphere_compact(struct reb_simulation* r) {
         order = W.horder.data();
     }
     W.last_nlev = nlev;
-    // plog starts at -0.0, the exact identity of the sum below, so that the
-    // collisions that do not bounce need no branch
-    std::vector<double>& plog = W.plog;
-    std::vector<unsigned char>& hit = W.hit;
-    plog.assign(nc, -0.);
-    hit.assign(nc, 0);
     HSBody* B = W.body_h;
     HostPool::get().run_levels(nlev, start, order, [&](size_t k) {
         hit[k] = (unsigned char)hardsphere_apply(r, B, (size_t)cl[k].p1, (size_t)cl[k].p2, L.gb[cl[k].gb], &plog[k]);
