@@ -617,7 +617,10 @@ void collisions_free(struct reb_simulation* r) {
 // ordered by level enqueued for copying into W.lev_h and W.perm_h and *nlev
 // an upper bound of the levels, 0 if the list needs more than kMaxLevels
 // levels, -1 on a HIP error.
-static int device_levels(struct reb_simulation* r, ResolveScratch& W, int nc, int n, int* nlev) {
+// `host_work` runs once, after the first rounds are enqueued and before the
+// host waits for them.
+static int device_levels(struct reb_simulation* r, ResolveScratch& W, int nc, int n, int* nlev,
+                         const std::function<void()>& host_work) {
     DeviceState* d = dev_of(r);
     const size_t m = 2 * (size_t)nc;
     // lmisc: round flags | level starts | per-block histograms | per-block offsets
@@ -670,9 +673,10 @@ static int device_levels(struct reb_simulation* r, ResolveScratch& W, int nc, in
             level = next;
         }
         if (!RB_HIP(r, hipGetLastError()) ||
-            !RB_HIP(r, hipMemcpyAsync(W.lev_h, flags + rounds - 1, sizeof(int), hipMemcpyDeviceToHost, d->stream)) ||
-            !stream_sync(r))
+            !RB_HIP(r, hipMemcpyAsync(W.lev_h, flags + rounds - 1, sizeof(int), hipMemcpyDeviceToHost, d->stream)))
             return -1;
+        if (host_work && rounds == b) host_work();
+        if (!stream_sync(r)) return -1;
         if (W.lev_h[0] == 0) break;
         if (rounds >= kMaxLevels) return 0;
         batch = 8;
@@ -755,14 +759,16 @@ static void resolve_hardsphere_compact(struct reb_simulation* r) {
                        W.coll.as<struct reb_collision>());
     if (!RB_HIP(r, hipGetLastError())) return;
     // plog starts at -0.0, the exact identity of the sum after the resolve,
-    // so that the collisions that do not bounce need no branch (cleared here,
-    // while the device works on the levels)
+    // so that the collisions that do not bounce need no branch (cleared while
+    // the device works on the levels)
     std::vector<double>& plog = W.plog;
     std::vector<unsigned char>& hit = W.hit;
-    plog.assign(nc, -0.);
-    hit.assign(nc, 0);
+    auto clear = [&]() {
+        plog.assign(nc, -0.);
+        hit.assign(nc, 0);
+    };
     int nlev = 0;
-    const int lv = device_levels(r, W, nc, n, &nlev);
+    const int lv = device_levels(r, W, nc, n, &nlev, clear);
     if (lv < 0) return;
     if (!RB_HIP(r, hipMemcpyAsync(W.pair_h, W.pair.p, sizeof(HSPair) * nc, hipMemcpyDeviceToHost, d->stream)) ||
         !RB_HIP(r, hipEventRecord(W.ev_pairs, d->stream)))
