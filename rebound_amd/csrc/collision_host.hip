@@ -617,10 +617,7 @@ void collisions_free(struct reb_simulation* r) {
 // ordered by level enqueued for copying into W.lev_h and W.perm_h and *nlev
 // an upper bound of the levels, 0 if the list needs more than kMaxLevels
 // levels, -1 on a HIP error.
-// `host_work` runs once, after the first rounds are enqueued and before the
-// host waits for them.
-static int device_levels(struct reb_simulation* r, ResolveScratch& W, int nc, int n, int* nlev,
-                         const std::function<void()>& host_work) {
+static int device_levels(struct reb_simulation* r, ResolveScratch& W, int nc, int n, int* nlev) {
     DeviceState* d = dev_of(r);
     const size_t m = 2 * (size_t)nc;
     // lmisc: round flags | level starts | per-block histograms | per-block offsets
@@ -663,7 +660,9 @@ static int device_levels(struct reb_simulation* r, ResolveScratch& W, int nc, in
     // between the two halves of level_d; `level` is the latest.
     int* lv_buf[2] = {W.level_d.as<int>(), W.level_d.as<int>() + nc};
     int* level = lv_buf[0];
-    int rounds = 0, batch = std::max(8, W.last_nlev + 2);
+    // the first batch has a margin over the last list's level count: an empty
+    // round costs a few microseconds, another batch a host round trip
+    int rounds = 0, batch = std::max(8, W.last_nlev + 6);
     for (;;) {
         const int b = std::min(batch, kMaxLevels - rounds);
         for (int i = 0; i < b; i++, rounds++) {
@@ -675,7 +674,6 @@ static int device_levels(struct reb_simulation* r, ResolveScratch& W, int nc, in
         if (!RB_HIP(r, hipGetLastError()) ||
             !RB_HIP(r, hipMemcpyAsync(W.lev_h, flags + rounds - 1, sizeof(int), hipMemcpyDeviceToHost, d->stream)))
             return -1;
-        if (host_work && rounds == b) host_work();
         if (!stream_sync(r)) return -1;
         if (W.lev_h[0] == 0) break;
         if (rounds >= kMaxLevels) return 0;
@@ -758,17 +756,13 @@ static void resolve_hardsphere_compact(struct reb_simulation* r) {
                        W.perm.as<int>(), nc, W.cidx.as<int>(), gb, W.pair.as<HSPair>(),
                        W.coll.as<struct reb_collision>());
     if (!RB_HIP(r, hipGetLastError())) return;
-    // plog starts at -0.0, the exact identity of the sum after the resolve,
-    // so that the collisions that do not bounce need no branch (cleared while
-    // the device works on the levels)
+    // every entry of plog and hit is written by the resolve below
     std::vector<double>& plog = W.plog;
     std::vector<unsigned char>& hit = W.hit;
-    auto clear = [&]() {
-        plog.assign(nc, -0.);
-        hit.assign(nc, 0);
-    };
+    plog.resize(nc);
+    hit.resize(nc);
     int nlev = 0;
-    const int lv = device_levels(r, W, nc, n, &nlev, clear);
+    const int lv = device_levels(r, W, nc, n, &nlev);
     if (lv < 0) return;
     if (!RB_HIP(r, hipMemcpyAsync(W.pair_h, W.pair.p, sizeof(HSPair) * nc, hipMemcpyDeviceToHost, d->stream)) ||
         !RB_HIP(r, hipEventRecord(W.ev_pairs, d->stream)))
@@ -794,8 +788,12 @@ static void resolve_hardsphere_compact(struct reb_simulation* r) {
     }
     W.last_nlev = nlev;
     HSBody* B = W.body_h;
+    // a collision that does not bounce logs -0.0, the exact identity of the
+    // sum below, so the sum needs no branch
     HostPool::get().run_levels(nlev, start, order, [&](size_t k) {
-        hit[k] = (unsigned char)hardsphere_apply(r, B, (size_t)cl[k].p1, (size_t)cl[k].p2, L.gb[cl[k].gb], &plog[k]);
+        double pl = -0.;
+        hit[k] = (unsigned char)hardsphere_apply(r, B, (size_t)cl[k].p1, (size_t)cl[k].p2, L.gb[cl[k].gb], &pl);
+        plog[k] = pl;
     });
     // the pressure log in list order (collision.c:655-663)
     double sum = r->collisions_plog;
