@@ -121,8 +121,10 @@ class HostPool {
                 // barrier: every thread has finished level l
                 arrived.fetch_add(1);
                 const long want = (long)l * P;
+                // spin briefly, then yield: a thread of the pool that the OS
+                // has preempted must get its CPU back quickly
                 for (long spin = 0; arrived.load(std::memory_order_acquire) < want; spin++) {
-                    if (spin < 20000)
+                    if (spin < 2000)
                         __builtin_ia32_pause();
                     else
                         std::this_thread::yield();
@@ -134,9 +136,13 @@ class HostPool {
     }
 
   private:
+    // One thread fewer than the CPUs (at most 16) besides the caller: the
+    // HIP runtime's own threads keep a CPU, so the level barriers do not wait
+    // for a preempted pool thread.
     HostPool() {
         unsigned hw = std::thread::hardware_concurrency();
-        int k = (int)std::min<unsigned>(hw ? hw : 1, 16u) - 1;
+        int k = (int)std::min<unsigned>(hw ? hw : 1, 16u) - 2;
+        if (k < 0) k = 0;
         for (int t = 0; t < k; t++) threads_.emplace_back([this, t] { loop(t + 1); });
     }
     ~HostPool() {
