@@ -485,8 +485,6 @@ struct rbamd::ResolveScratch {
     std::vector<int> draws;
     std::vector<unsigned int> seeds;
     unsigned int draw_seed = 0;
-    bool perm_done = false;           // perm_h holds this search's shuffle
-    unsigned int seed_before = 0;     // r->rand_seed before that shuffle
     std::vector<double> plog;
     std::vector<unsigned char> hit;
     void release() {
@@ -728,34 +726,6 @@ static void host_levels(ResolveScratch& W, const HSPair* cl, int nc, int n, int*
     *nlev = top;
 }
 
-// shuffle (collision.c:336-342) as a permutation of the detection order into
-// W.perm_h: the same swaps applied to indices instead of 72-byte records,
-// with the draws made ahead when they start from the current seed.
-static void shuffle_into(struct reb_simulation* r, ResolveScratch& W, int nc) {
-    W.seed_before = r->rand_seed;
-    int* perm = W.perm_h;
-    for (int k = 0; k < nc; k++) perm[k] = k;
-    const int ahead = W.draw_seed == r->rand_seed ? std::min(nc, (int)W.draws.size()) : 0;
-    for (int i = 0; i < ahead; i++) {
-        const int nw = (int)((size_t)W.draws[i] % (size_t)nc);
-        std::swap(perm[i], perm[nw]);
-    }
-    if (ahead > 0) r->rand_seed = W.seeds[ahead - 1];
-    for (int i = ahead; i < nc; i++) {
-        const int nw = (int)((size_t)rand_r_step(&r->rand_seed) % (size_t)nc);
-        std::swap(perm[i], perm[nw]);
-    }
-    W.draws.clear();
-}
-
-bool compact_resolve_shuffle(struct reb_simulation* r, size_t nc) {
-    ResolveScratch& W = *dev_of(r)->resolve;
-    if (!pinned_ensure(r, W.perm_h, W.perm_cap, nc)) return false;
-    shuffle_into(r, W, (int)nc);
-    W.perm_done = true;
-    return true;
-}
-
 // The built-in hard-sphere resolver with a constant restitution, on the
 // compact records prepared by compact_resolve_gather: only the eight fields it
 // reads come over, only the velocities go back, and collisions of one level
@@ -770,9 +740,20 @@ static void resolve_hardsphere_compact(struct reb_simulation* r) {
     // shuffle (collision.c:336-342) as a permutation of the detection order,
     // the same swaps applied to indices instead of 72-byte records; drawn
     // while the body records are on their way
-    if (!W.perm_done) shuffle_into(r, W, nc);
-    W.perm_done = false;
     int* perm = W.perm_h;
+    for (int k = 0; k < nc; k++) perm[k] = k;
+    // the draws made ahead, if they start from the current seed
+    const int ahead = W.draw_seed == r->rand_seed ? std::min(nc, (int)W.draws.size()) : 0;
+    for (int i = 0; i < ahead; i++) {
+        const int nw = (int)((size_t)W.draws[i] % (size_t)nc);
+        std::swap(perm[i], perm[nw]);
+    }
+    if (ahead > 0) r->rand_seed = W.seeds[ahead - 1];
+    for (int i = ahead; i < nc; i++) {
+        const int nw = (int)((size_t)rand_r_step(&r->rand_seed) % (size_t)nc);
+        std::swap(perm[i], perm[nw]);
+    }
+    W.draws.clear();
     CollGB gb;
     gb.n = (int)L.gb.size();
     for (int k = 0; k < gb.n; k++) gb.g[k] = L.gb[k];
@@ -852,11 +833,6 @@ void collision_resolve_host(struct reb_simulation* r) {
         return;
     }
     if (d->resolve) d->resolve->ready = false;
-    if (d->resolve && d->resolve->perm_done) {
-        // a shuffle was drawn for the compact path: this path draws its own
-        r->rand_seed = d->resolve->seed_before;
-        d->resolve->perm_done = false;
-    }
     if (!materialize_collisions(r)) return;
     // shuffle (collision.c:336-342)
     for (size_t i = 0; i < r->N_collisions; i++) {

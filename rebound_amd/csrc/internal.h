@@ -208,9 +208,6 @@ bool compact_resolve_eligible(const struct reb_simulation* r, int n_ghostboxes);
 // Draws `expect` rand_r results from r->rand_seed ahead of the shuffle (while
 // the device searches), without advancing r->rand_seed.
 void compact_resolve_draw_ahead(struct reb_simulation* r, size_t expect);
-// The shuffle of the nc records (advancing r->rand_seed), drawn on the host
-// while the device prepares the compact records.
-bool compact_resolve_shuffle(struct reb_simulation* r, size_t nc);
 bool compact_resolve_mark(struct reb_simulation* r, const CollRecH* rec, size_t nc);
 bool compact_resolve_gather(struct reb_simulation* r, const CollRecH* rec, size_t nc);
 // r->collisions lives in pinned host memory (the device copies the shuffled

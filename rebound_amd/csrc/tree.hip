@@ -991,8 +991,7 @@ static bool fetch_collisions(struct reb_simulation* r, TreeBuffers* T, long tota
     if (!compact &&
         !RB_HIP(r, hipMemcpyAsync(L.rec, T->coll_rec.p, sizeof(CollRec) * total, hipMemcpyDeviceToHost, d->stream)))
         return false;
-    if (compact && (!compact_resolve_mark(r, drec, (size_t)total) || !compact_resolve_shuffle(r, (size_t)total)))
-        return false;
+    if (compact && !compact_resolve_mark(r, drec, (size_t)total)) return false;
     L.gb.resize(gl.n);
     for (int k = 0; k < gl.n; k++)
         L.gb[k] = reb_vec6d{gl.g[k][0], gl.g[k][1], gl.g[k][2], gl.g[k][3], gl.g[k][4], gl.g[k][5]};
