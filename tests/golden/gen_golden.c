@@ -387,6 +387,35 @@ static struct reb_simulation* sheet(size_t N, double root_size, int nrx, int nry
     return r;
 }
 
+/* The benchmark's full-size sheet (200000 bodies, 10 x 10 root boxes of 136,
+ * seed 99, tree gravity and tree collisions with the hard-sphere resolver):
+ * after each of three reb_simulation_steps(r, 1) calls, the XOR of the bit
+ * patterns of every body's x y z vx vy vz, collisions_log_n, collisions_plog
+ * and rand_seed. */
+static void gen_sheet_full(void){
+    struct reb_simulation* r = sheet(200000, 136., 10, 10, 99u);
+    r->collision = REB_COLLISION_TREE;
+    r->collision_resolve = reb_collision_resolve_hardsphere;
+    int64_t meta[3][3];
+    double plog[3];
+    for (int s=0; s<3; s++){
+        reb_simulation_steps(r, 1);
+        uint64_t x = 0;
+        for (size_t i=0; i<r->N; i++){
+            const double f[6] = {r->particles[i].x, r->particles[i].y, r->particles[i].z,
+                                 r->particles[i].vx, r->particles[i].vy, r->particles[i].vz};
+            for (int k=0; k<6; k++){ uint64_t u; memcpy(&u, &f[k], 8); x ^= u; }
+        }
+        meta[s][0] = (int64_t)x;
+        meta[s][1] = (int64_t)r->collisions_log_n;
+        meta[s][2] = (int64_t)r->rand_seed;
+        plog[s] = r->collisions_plog;
+    }
+    wi("sheet200k_meta.npy", &meta[0][0], 3, 3);
+    wd("sheet200k_plog.npy", plog, 3, 0);
+    reb_simulation_free(r);
+}
+
 /* pre-order dump of the reference tree: center, w, m, com, pt */
 static void dump_cell(struct reb_treecell* c, double** buf, size_t* n, size_t* cap){
     if (!c) return;
@@ -892,6 +921,7 @@ int main(int argc, char** argv){
     gen_merge();
     gen_whfast_jacobi();
     gen_whfast_kernels();
+    gen_sheet_full();
     gen_line();
     gen_archive();
     return 0;

@@ -284,3 +284,22 @@ def test_level_parallel_resolve_equals_sequential_at_scale(gpu):
         assert np.array_equal(ca, cb), step
     a.free()
     b.free()
+
+
+def test_sheet_full_size_three_steps_bitwise(gpu):
+    """The benchmark's sheet at full size (200000 bodies): after each of three
+    steps, the XOR of the bit patterns of every body's position and velocity,
+    the bounce count, the pressure log and the random seed equal the reference
+    library's (serial build) on the same construction."""
+    meta = golden("sheet200k_meta.npy")
+    plog = golden("sheet200k_plog.npy")
+    sim = gpu.shearing_sheet(200000, 136.0, 10, 99)
+    for s in range(3):
+        sim.steps(1)
+        st = np.ascontiguousarray(sim.state()[:, :6])
+        x = np.bitwise_xor.reduce(st.view(np.uint64).ravel())
+        assert int(x) == int(np.int64(meta[s, 0]).view(np.uint64)), s
+        assert sim.s.collisions_log_n == meta[s, 1], s
+        assert sim.s.rand_seed == int(np.int64(meta[s, 2]).view(np.uint64)) & 0xffffffff, s
+        assert sim.s.collisions_plog == plog[s], s
+    sim.free()
