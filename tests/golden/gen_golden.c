@@ -416,6 +416,35 @@ static void gen_sheet_full(void){
     reb_simulation_free(r);
 }
 
+/* XOR of the bit patterns of every body's x y z vx vy vz */
+static uint64_t state_xor(const struct reb_simulation* r){
+    uint64_t x = 0;
+    for (size_t i=0; i<r->N; i++){
+        const double f[6] = {r->particles[i].x, r->particles[i].y, r->particles[i].z,
+                             r->particles[i].vx, r->particles[i].vy, r->particles[i].vz};
+        for (int k=0; k<6; k++){ uint64_t u; memcpy(&u, &f[k], 8); x ^= u; }
+    }
+    return x;
+}
+
+/* The direct-summation benchmark at full size: Plummer sphere of 65536
+ * bodies (seed 42, softening 0.005, G = 1), leapfrog with dt 5e-4 and BASIC
+ * gravity.  State XOR after construction and after each of two steps. */
+static void gen_plummer_full(void){
+    struct reb_simulation* r = plummer(65536, 42u, 0.005);
+    reb_simulation_set_integrator(r, "leapfrog");
+    r->gravity = REB_GRAVITY_BASIC;
+    r->dt = 5e-4;
+    int64_t h[3];
+    h[0] = (int64_t)state_xor(r);
+    for (int s=1; s<3; s++){
+        reb_simulation_steps(r, 1);
+        h[s] = (int64_t)state_xor(r);
+    }
+    wi("plummer65536_seed42_xor.npy", h, 3, 0);
+    reb_simulation_free(r);
+}
+
 /* pre-order dump of the reference tree: center, w, m, com, pt */
 static void dump_cell(struct reb_treecell* c, double** buf, size_t* n, size_t* cap){
     if (!c) return;
@@ -922,6 +951,7 @@ int main(int argc, char** argv){
     gen_whfast_jacobi();
     gen_whfast_kernels();
     gen_sheet_full();
+    gen_plummer_full();
     gen_line();
     gen_archive();
     return 0;

@@ -260,3 +260,28 @@ def test_fp64_large_n_sampled_rows(gpu, orc):
     rows = np.random.default_rng(1).choice(16384, 512, replace=False)
     err = np.sqrt(((acc[rows] - ref[rows]) ** 2).sum(1)) / abs_sum(s, 0.005, rows)
     assert err.max() <= TOL_FP64
+
+
+def test_exact_leapfrog_full_size_two_steps_bitwise(gpu):
+    """The direct-summation benchmark at full size (Plummer sphere of 65536
+    bodies, seed 42, softening 0.005, leapfrog with dt 5e-4, BASIC gravity)
+    in exact mode: the XOR of the bit patterns of every body's position and
+    velocity after construction and after each of two steps equals the
+    reference library's (serial build) on the same construction."""
+    want = golden("plummer65536_seed42_xor.npy").view(np.uint64)
+    sim = gpu.Sim()
+    sim.s.rand_seed = 42
+    sim.s.softening = 0.005
+    sim.s.G = 1.0
+    sim.add_plummer(65536, 1.0, 1.0)
+    sim.move_to_com()
+    sim.set_integrator("leapfrog")
+    sim.s.dt = 5e-4
+    sim.s.gravity = gpu.GRAVITY["basic"]
+    sim.precision = "exact"
+    for s in range(3):
+        if s:
+            sim.steps(1)
+        st = np.ascontiguousarray(sim.state()[:, :6])
+        assert int(np.bitwise_xor.reduce(st.view(np.uint64).ravel())) == int(want[s]), s
+    sim.free()
