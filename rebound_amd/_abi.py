@@ -59,6 +59,7 @@ class Simulation(Structure):
 
 
 HEARTBEAT = CFUNCTYPE(None, POINTER(Simulation))
+STEP = CFUNCTYPE(None, POINTER(Simulation), c_void_p)
 COR = CFUNCTYPE(c_double, POINTER(Simulation), c_double)
 RESOLVE = CFUNCTYPE(c_int, POINTER(Simulation), Collision)
 
@@ -159,6 +160,8 @@ def declare(lib):
         "reb_boundary_get_ghostbox": (Vec6d, [S, c_int, c_int, c_int]),
         "reb_collision_resolve_hardsphere": (c_int, [S, Collision]),
         "reb_collision_resolve_halt": (c_int, [S, Collision]),
+        "reb_collision_resolve_merge": (c_int, [S, Collision]),
+        "reb_integrator_register": (None, [Integrator, c_char_p]),
         "reb_integrator_whfast_kepler_solver": (None, [POINTER(Particle), c_double, c_double, S]),
         "reb_amd_set_precision": (c_int, [S, c_int]),
         "reb_amd_get_precision": (c_int, [S]),

@@ -265,16 +265,20 @@ struct Span {
 };
 
 // Reads the field at *pos of buf (after the header); false at the end of buf.
+// Sizes are checked against the bytes that remain (a crafted size cannot wrap
+// the position), and the name must end with its NUL inside its span.
 bool next_field(const std::string& buf, size_t* pos, Span* s) {
-    if (*pos + sizeof(BinField) > buf.size()) return false;
+    if (*pos > buf.size() || buf.size() - *pos < sizeof(BinField)) return false;
     memcpy(&s->f, buf.data() + *pos, sizeof(BinField));
-    *pos += sizeof(BinField);
-    if (s->f.size_name == 0 || *pos + s->f.size_name > buf.size()) return false;
-    s->name = buf.data() + *pos;
-    *pos += s->f.size_name;
-    s->data = buf.data() + *pos;
-    *pos += s->f.size_data;
-    return *pos <= buf.size();
+    const size_t at = *pos + sizeof(BinField);
+    const size_t left = buf.size() - at;
+    if (s->f.size_name == 0 || s->f.size_name > left) return false;
+    if (memchr(buf.data() + at, 0, s->f.size_name) == nullptr) return false;
+    if (s->f.size_data > left - s->f.size_name) return false;
+    s->name = buf.data() + at;
+    s->data = s->name + s->f.size_name;
+    *pos = at + s->f.size_name + s->f.size_data;
+    return true;
 }
 
 bool particles_differ(const char* a, const char* b, size_t bytes) {
@@ -483,8 +487,15 @@ void input_fields(struct reb_simulation* r, const std::string& buf, size_t pos, 
                 char*** list = reinterpret_cast<char***>(at);
                 size_t q = 0;
                 while (q < s.f.size_data) {
+                    // each entry: a NUL-terminated name, then the pointer it had
                     const char* str = s.data + q;
-                    const size_t len = strlen(str) + 1 + sizeof(char*);
+                    const size_t room = s.f.size_data - q;
+                    const size_t slen = strnlen(str, room);
+                    if (slen == room || room - slen - 1 < sizeof(char*)) {
+                        *warnings |= W_CORRUPT;
+                        return;
+                    }
+                    const size_t len = slen + 1 + sizeof(char*);
                     (*n)++;
                     *list = static_cast<char**>(realloc(*list, sizeof(char*) * (*n)));
                     (*list)[*n - 1] = static_cast<char*>(malloc(len));
@@ -501,7 +512,12 @@ void input_fields(struct reb_simulation* r, const std::string& buf, size_t pos, 
 }
 
 // binarydata.c:851-878: the state the fields leave behind.
-void finish_input(struct reb_simulation* r) {
+void finish_input(struct reb_simulation* r, unsigned* warnings) {
+    if (r->N && !r->particles) {
+        // a damaged file gave the count but not the bodies
+        *warnings |= W_CORRUPT;
+        r->N = 0;
+    }
     r->N_allocated = r->N;
     for (size_t l = 0; l < r->N; l++) {
         r->particles[l].ap = nullptr;
@@ -684,7 +700,7 @@ REB_API struct reb_simulation* reb_simulation_create_from_file(char* filename, i
     r->simulationarchive_filename = nullptr;
     input_fields(r, file, 0, &w);
     if (snapshot > 0) input_fields(r, file, off[(size_t)snapshot], &w);
-    finish_input(r);
+    finish_input(r, &w);
     report(r, w);
     return r;
 }

@@ -49,9 +49,12 @@ class HostPool {
         return pool;
     }
     int size() const { return (int)threads_.size() + 1; }
-    // Runs fn(i) for i in [0, n) on the pool and the calling thread.
+    // Runs fn(i) for i in [0, n) on the pool and the calling thread.  The
+    // pool serves one caller at a time: a caller that finds it busy (another
+    // simulation resolving on another host thread) runs its items itself.
     void run(size_t n, const std::function<void(size_t)>& fn) {
-        if (n < 256 || threads_.empty()) {
+        std::unique_lock<std::mutex> owner(dispatch_, std::try_to_lock);
+        if (n < 256 || threads_.empty() || !owner.owns_lock()) {
             for (size_t i = 0; i < n; i++) fn(i);
             return;
         }
@@ -71,7 +74,8 @@ class HostPool {
         fn_ = nullptr;
     }
     // Runs fn(t) once on every thread of the pool, t = 0 .. size()-1 (0 is
-    // the calling thread), and returns when all have finished.
+    // the calling thread), and returns when all have finished.  The caller
+    // holds dispatch_.
     void run_all(const std::function<void(int)>& fn) {
         if (threads_.empty()) {
             fn(0);
@@ -98,7 +102,10 @@ class HostPool {
     void run_levels(int nlev, const int* start, const int* order, const std::function<void(size_t)>& fn) {
         constexpr int kTail = 1024;
         const int total = start[nlev + 1] - start[1];
-        if (threads_.empty() || total < 2 * kTail) {
+        // levels in order on this thread give the same result, so a busy
+        // pool (another caller) is not waited for
+        std::unique_lock<std::mutex> owner(dispatch_, std::try_to_lock);
+        if (threads_.empty() || total < 2 * kTail || !owner.owns_lock()) {
             for (int q = start[1]; q < start[nlev + 1]; q++) fn((size_t)order[q]);
             return;
         }
@@ -183,6 +190,7 @@ class HostPool {
         }
     }
     std::vector<std::thread> threads_;
+    std::mutex dispatch_;   // held by the caller that owns the pool's threads
     std::mutex mu_;
     std::condition_variable cv_, done_;
     const std::function<void(size_t)>* fn_ = nullptr;

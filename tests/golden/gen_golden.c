@@ -26,8 +26,12 @@
 #include <stdint.h>
 #include "rebound.h"
 #include "tree.h"
+#include "boundary.h"
 
 static const char* outdir = ".";
+/* Raw state dumps (x y z vx vy vz of body 0..N-1, in order) go to this
+ * folder; make_golden.py hashes each with SHA-256 and deletes it. */
+static const char* rawdir = NULL;
 
 /* ---------------------------------------------------------------- npy out */
 static void write_npy(const char* name, const char* descr, size_t elsize,
@@ -387,11 +391,26 @@ static struct reb_simulation* sheet(size_t N, double root_size, int nrx, int nry
     return r;
 }
 
+/* x y z vx vy vz of body 0..N-1, in order, appended to the raw file `name`
+ * (one record per call): make_golden.py hashes each record with SHA-256. */
+static void dump_raw6(const char* name, const struct reb_simulation* r){
+    if (!rawdir) return;
+    char path[1024];
+    snprintf(path, sizeof(path), "%s/%s", rawdir, name);
+    FILE* f = fopen(path, "ab");
+    if (!f){ perror(path); exit(1); }
+    for (size_t i=0; i<r->N; i++){
+        const double v[6] = {r->particles[i].x, r->particles[i].y, r->particles[i].z,
+                             r->particles[i].vx, r->particles[i].vy, r->particles[i].vz};
+        fwrite(v, 8, 6, f);
+    }
+    fclose(f);
+}
+
 /* The benchmark's full-size sheet (200000 bodies, 10 x 10 root boxes of 136,
  * seed 99, tree gravity and tree collisions with the hard-sphere resolver):
- * after each of three reb_simulation_steps(r, 1) calls, the XOR of the bit
- * patterns of every body's x y z vx vy vz, collisions_log_n, collisions_plog
- * and rand_seed. */
+ * after each of three reb_simulation_steps(r, 1) calls, the bodies in order
+ * (hashed), collisions_log_n, collisions_plog and rand_seed. */
 static void gen_sheet_full(void){
     struct reb_simulation* r = sheet(200000, 136., 10, 10, 99u);
     r->collision = REB_COLLISION_TREE;
@@ -400,13 +419,8 @@ static void gen_sheet_full(void){
     double plog[3];
     for (int s=0; s<3; s++){
         reb_simulation_steps(r, 1);
-        uint64_t x = 0;
-        for (size_t i=0; i<r->N; i++){
-            const double f[6] = {r->particles[i].x, r->particles[i].y, r->particles[i].z,
-                                 r->particles[i].vx, r->particles[i].vy, r->particles[i].vz};
-            for (int k=0; k<6; k++){ uint64_t u; memcpy(&u, &f[k], 8); x ^= u; }
-        }
-        meta[s][0] = (int64_t)x;
+        dump_raw6("sheet200k.raw", r);
+        meta[s][0] = 0;
         meta[s][1] = (int64_t)r->collisions_log_n;
         meta[s][2] = (int64_t)r->rand_seed;
         plog[s] = r->collisions_plog;
@@ -416,32 +430,20 @@ static void gen_sheet_full(void){
     reb_simulation_free(r);
 }
 
-/* XOR of the bit patterns of every body's x y z vx vy vz */
-static uint64_t state_xor(const struct reb_simulation* r){
-    uint64_t x = 0;
-    for (size_t i=0; i<r->N; i++){
-        const double f[6] = {r->particles[i].x, r->particles[i].y, r->particles[i].z,
-                             r->particles[i].vx, r->particles[i].vy, r->particles[i].vz};
-        for (int k=0; k<6; k++){ uint64_t u; memcpy(&u, &f[k], 8); x ^= u; }
-    }
-    return x;
-}
-
 /* The direct-summation benchmark at full size: Plummer sphere of 65536
  * bodies (seed 42, softening 0.005, G = 1), leapfrog with dt 5e-4 and BASIC
- * gravity.  State XOR after construction and after each of two steps. */
+ * gravity.  The bodies in order (hashed) after construction and after each
+ * of two steps. */
 static void gen_plummer_full(void){
     struct reb_simulation* r = plummer(65536, 42u, 0.005);
     reb_simulation_set_integrator(r, "leapfrog");
     r->gravity = REB_GRAVITY_BASIC;
     r->dt = 5e-4;
-    int64_t h[3];
-    h[0] = (int64_t)state_xor(r);
+    dump_raw6("plummer65536_seed42.raw", r);
     for (int s=1; s<3; s++){
         reb_simulation_steps(r, 1);
-        h[s] = (int64_t)state_xor(r);
+        dump_raw6("plummer65536_seed42.raw", r);
     }
-    wi("plummer65536_seed42_xor.npy", h, 3, 0);
     reb_simulation_free(r);
 }
 
@@ -926,9 +928,319 @@ static void gen_boundaries(void){
     reb_simulation_free(r);
 }
 
+/* ---------------------------------------------------------------- integrate loop */
+/* Scenarios of reb_simulation_integrate (simulation.c:241-502) and of the
+ * collision outcomes it drives (collision.c:336-404).  Each writes the bodies
+ * (11 columns) and a meta row
+ *   {t, dt, dt_last_done, steps_done, status, N, collisions_log_n, energy_offset}. */
+static void dump_meta(const char* name, const struct reb_simulation* r, int status){
+    const double m[8] = {r->t, r->dt, r->dt_last_done, (double)r->steps_done, (double)status, (double)r->N,
+                         (double)r->collisions_log_n, r->energy_offset};
+    wd(name, m, 8, 0);
+}
+
+/* heartbeat log: {t, N, particles[k].x, particles[k].vy, energy_offset} per call */
+static double hb_log[5*8192];
+static size_t hb_n = 0;
+static size_t hb_body = 0;
+static void hb_record(struct reb_simulation* r){
+    if (hb_n >= 8192) return;
+    double* o = hb_log + 5*hb_n;
+    o[0] = r->t;
+    o[1] = (double)r->N;
+    o[2] = r->N > hb_body ? r->particles[hb_body].x : 0.;
+    o[3] = r->N > hb_body ? r->particles[hb_body].vy : 0.;
+    o[4] = r->energy_offset;
+    hb_n++;
+}
+static void pre_kick(struct reb_simulation* r){ r->particles[0].vx += 1e-7; }
+static void post_damp(struct reb_simulation* r){
+    for (size_t i=0; i<r->N; i++){
+        r->particles[i].vx *= 0.9999;
+        r->particles[i].vy *= 0.9999;
+        r->particles[i].vz *= 0.9999;
+    }
+}
+
+/* collision callback log: {p1, p2, outcome} per call.  Mode 0 removes both
+ * bodies; mode 1 cycles the outcomes 1, 2, 3 (remove p1, p2, both). */
+static int64_t cb_log[3*8192];
+static size_t cb_n = 0;
+static int cb_mode = 0;
+static enum REB_COLLISION_RESOLVE_OUTCOME cb_resolve(struct reb_simulation* const r, struct reb_collision c){
+    r->collisions_log_n++;
+    const int out = cb_mode == 0 ? 3 : 1 + (int)(cb_n % 3);
+    if (cb_n < 8192){
+        cb_log[3*cb_n] = (int64_t)c.p1;
+        cb_log[3*cb_n+1] = (int64_t)c.p2;
+        cb_log[3*cb_n+2] = out;
+    }
+    cb_n++;
+    return (enum REB_COLLISION_RESOLVE_OUTCOME)out;
+}
+
+static struct reb_simulation* two_planets(double m1, double a1, double e1, double m2, double a2, double e2){
+    struct reb_simulation* r = reb_simulation_create();
+    struct reb_particle star = {0};
+    star.m = 1.;
+    reb_simulation_add(r, star);
+    reb_simulation_add(r, reb_particle_from_orbit(r->G, r->particles[0], m1, a1, e1, 0.02, 0.3, 0.1, 0.5));
+    if (m2 > 0.) reb_simulation_add(r, reb_particle_from_orbit(r->G, r->particles[0], m2, a2, e2, 0.03, 1.3, 2.1, 4.));
+    reb_simulation_move_to_com(r);
+    return r;
+}
+
+static void gen_integrate(void){
+    /* leapfrog, exact finish time: tmax not a multiple of dt, then past
+     * tmax without exact finish, then backwards with exact finish */
+    struct reb_simulation* r = plummer(256, 3u, 0.01);
+    reb_simulation_set_integrator(r, "leapfrog");
+    r->gravity = REB_GRAVITY_BASIC;
+    r->dt = 1e-3;
+    dump_state("integ_lf_init.npy", r);
+    int st = reb_simulation_integrate(r, 0.0123456);
+    dump_state("integ_lf_a_state.npy", r); dump_meta("integ_lf_a_meta.npy", r, st);
+    r->exact_finish_time = 0;
+    st = reb_simulation_integrate(r, 0.05);
+    dump_state("integ_lf_b_state.npy", r); dump_meta("integ_lf_b_meta.npy", r, st);
+    r->exact_finish_time = 1;
+    st = reb_simulation_integrate(r, 0.0301);
+    dump_state("integ_lf_c_state.npy", r); dump_meta("integ_lf_c_meta.npy", r, st);
+    reb_simulation_free(r);
+
+    /* WHFast (Jacobi, safe mode off), exact finish time, then without */
+    r = two_planets(1e-3, 1., 0.05, 3e-4, 1.6, 0.1);
+    reb_simulation_add(r, reb_particle_from_orbit(r->G, r->particles[0], 1e-5, 2.5, 0.02, 0.01, 2.3, 0.7, 1.));
+    reb_simulation_move_to_com(r);
+    struct reb_integrator_whfast_state* wh = reb_simulation_set_integrator(r, "whfast");
+    wh->safe_mode = 0;
+    r->dt = 0.0314;
+    dump_state("integ_wh_init.npy", r);
+    st = reb_simulation_integrate(r, 7.77);
+    dump_state("integ_wh_a_state.npy", r); dump_meta("integ_wh_a_meta.npy", r, st);
+    r->exact_finish_time = 0;
+    st = reb_simulation_integrate(r, 20.);
+    dump_state("integ_wh_b_state.npy", r); dump_meta("integ_wh_b_meta.npy", r, st);
+    reb_simulation_free(r);
+
+    /* exit_max_distance: an eccentric planet passes 1.4 from the origin */
+    r = two_planets(1e-3, 1., 0.5, 0., 0., 0.);
+    reb_simulation_set_integrator(r, "leapfrog");
+    r->gravity = REB_GRAVITY_BASIC;
+    r->dt = 1e-3;
+    r->exit_max_distance = 1.4;
+    dump_state("integ_escape_init.npy", r);
+    st = reb_simulation_integrate(r, 100.);
+    dump_state("integ_escape_state.npy", r); dump_meta("integ_escape_meta.npy", r, st);
+    reb_simulation_free(r);
+
+    /* exit_min_distance: crossing orbits */
+    r = two_planets(1e-3, 1., 0.0, 1e-3, 1.5, 0.4);
+    reb_simulation_set_integrator(r, "leapfrog");
+    r->gravity = REB_GRAVITY_BASIC;
+    r->dt = 1e-3;
+    r->exit_min_distance = 0.25;
+    dump_state("integ_encounter_init.npy", r);
+    st = reb_simulation_integrate(r, 200.);
+    dump_state("integ_encounter_state.npy", r); dump_meta("integ_encounter_meta.npy", r, st);
+    reb_simulation_free(r);
+
+    /* hooks: heartbeat log, pre-timestep kick of body 0, post-timestep damping */
+    r = plummer(128, 4u, 0.01);
+    reb_simulation_set_integrator(r, "leapfrog");
+    r->gravity = REB_GRAVITY_BASIC;
+    r->dt = 1e-3;
+    dump_state("integ_hooks_init.npy", r);
+    hb_n = 0; hb_body = 5;
+    r->heartbeat = hb_record;
+    r->pre_timestep_modifications = pre_kick;
+    r->post_timestep_modifications = post_damp;
+    st = reb_simulation_integrate(r, 0.0205);
+    dump_state("integ_hooks_state.npy", r); dump_meta("integ_hooks_meta.npy", r, st);
+    wd("integ_hooks_log.npy", hb_log, hb_n, 5);
+    reb_simulation_free(r);
+
+    /* halt: the default resolver (none set) with DIRECT search, and the halt
+     * resolver set explicitly with TREE search in a periodic box */
+    for (int v=0; v<2; v++){
+        r = reb_simulation_create();
+        r->rand_seed = 31u + (unsigned)v;
+        reb_simulation_set_integrator(r, "leapfrog");
+        r->gravity = REB_GRAVITY_BASIC;
+        r->dt = 0.01;
+        r->root_size = 10.;
+        r->N_root_x = r->N_root_y = r->N_root_z = 1;
+        r->boundary = REB_BOUNDARY_PERIODIC;
+        r->collision = v == 0 ? REB_COLLISION_DIRECT : REB_COLLISION_TREE;
+        if (v == 1) r->collision_resolve = reb_collision_resolve_halt;
+        for (int i=0; i<100; i++){
+            struct reb_particle p = {0};
+            p.x = reb_random_uniform(r, -5., 5.);
+            p.y = reb_random_uniform(r, -5., 5.);
+            p.z = reb_random_uniform(r, -5., 5.);
+            p.vx = reb_random_normal(r, 1.);
+            p.vy = reb_random_normal(r, 1.);
+            p.vz = reb_random_normal(r, 1.);
+            p.m = 1e-3;
+            p.r = 0.2;
+            reb_simulation_add(r, p);
+        }
+        char name[64];
+        snprintf(name, sizeof(name), "integ_halt%d_init.npy", v); dump_state(name, r);
+        st = reb_simulation_integrate(r, 10.);
+        snprintf(name, sizeof(name), "integ_halt%d_state.npy", v); dump_state(name, r);
+        snprintf(name, sizeof(name), "integ_halt%d_meta.npy", v); dump_meta(name, r, st);
+        reb_simulation_free(r);
+    }
+
+    /* collision outcomes that remove bodies (test_collisions.py:104-190):
+     * remove both until no body is left, and the cycled outcomes 1, 2, 3,
+     * for TREE and DIRECT searches (TREE with tree gravity) */
+    for (int v=0; v<4; v++){
+        const int tree = v & 1, one = v >> 1;
+        r = reb_simulation_create();
+        r->rand_seed = 77u + (unsigned)v;
+        r->root_size = 50000.;
+        reb_simulation_set_integrator(r, "leapfrog");
+        r->boundary = REB_BOUNDARY_OPEN;
+        r->gravity = tree ? REB_GRAVITY_TREE : REB_GRAVITY_BASIC;
+        r->collision = tree ? REB_COLLISION_TREE : REB_COLLISION_DIRECT;
+        r->collision_resolve = cb_resolve;
+        const int n = one ? 50 : 10;
+        const double w = one ? 2. : 20.;
+        for (int i=0; i<n; i++){
+            struct reb_particle p = {0};
+            p.m = 1.;
+            p.r = 100.;
+            p.x = reb_random_uniform(r, -w, w);
+            p.y = reb_random_uniform(r, -w, w);
+            p.z = reb_random_uniform(r, -w, w);
+            reb_simulation_add(r, p);
+        }
+        r->dt = 0.001;
+        char name[64];
+        const char* tag = one ? (tree ? "one_tree" : "one_direct") : (tree ? "both_tree" : "both_direct");
+        snprintf(name, sizeof(name), "integ_%s_init.npy", tag); dump_state(name, r);
+        cb_mode = one; cb_n = 0;
+        if (one){
+            st = reb_simulation_integrate(r, r->dt);
+            snprintf(name, sizeof(name), "integ_%s_a_meta.npy", tag); dump_meta(name, r, st);
+            st = reb_simulation_integrate(r, 2.*r->dt);
+        }else{
+            st = reb_simulation_integrate(r, 1000.);
+        }
+        snprintf(name, sizeof(name), "integ_%s_meta.npy", tag); dump_meta(name, r, st);
+        if (r->N){ snprintf(name, sizeof(name), "integ_%s_state.npy", tag); dump_state(name, r); }
+        snprintf(name, sizeof(name), "integ_%s_log.npy", tag); wi(name, cb_log, cb_n, 3);
+        reb_simulation_free(r);
+    }
+
+    /* the merge resolver on a stepping simulation, DIRECT and TREE searches,
+     * periodic box with one ring of ghost boxes, energy offset tracked */
+    for (int v=0; v<2; v++){
+        r = reb_simulation_create();
+        r->rand_seed = 41u;
+        reb_simulation_set_integrator(r, "leapfrog");
+        r->gravity = REB_GRAVITY_BASIC;
+        r->dt = 0.01;
+        r->root_size = 4.;
+        r->N_root_x = 2; r->N_root_y = 2; r->N_root_z = 1;
+        r->N_ghost_x = 1; r->N_ghost_y = 1; r->N_ghost_z = 0;
+        r->boundary = REB_BOUNDARY_PERIODIC;
+        r->collision = v == 0 ? REB_COLLISION_DIRECT : REB_COLLISION_TREE;
+        r->collision_resolve = reb_collision_resolve_merge;
+        r->track_energy_offset = 1;
+        for (int i=0; i<300; i++){
+            struct reb_particle p = {0};
+            p.x = reb_random_uniform(r, -4., 4.);
+            p.y = reb_random_uniform(r, -4., 4.);
+            p.z = reb_random_uniform(r, -2., 2.);
+            p.vx = reb_random_normal(r, 0.5);
+            p.vy = reb_random_normal(r, 0.5);
+            p.vz = reb_random_normal(r, 0.5);
+            p.m = reb_random_uniform(r, 0.5, 1.5)*1e-3;
+            p.r = reb_random_uniform(r, 0.05, 0.15);
+            reb_simulation_add(r, p);
+        }
+        char name[64];
+        snprintf(name, sizeof(name), "integ_merge%d_init.npy", v); dump_state(name, r);
+        hb_n = 0; hb_body = 0;
+        r->heartbeat = hb_record;
+        st = reb_simulation_integrate(r, 0.3);
+        snprintf(name, sizeof(name), "integ_merge%d_state.npy", v); dump_state(name, r);
+        snprintf(name, sizeof(name), "integ_merge%d_meta.npy", v); dump_meta(name, r, st);
+        snprintf(name, sizeof(name), "integ_merge%d_log.npy", v); wd(name, hb_log, hb_n, 5);
+        reb_simulation_free(r);
+    }
+}
+
+/* A host integrator registered by name (rebound.c:86-113): symplectic Euler
+ * through reb_simulation_update_acceleration. */
+static void euler_step(struct reb_simulation* r, void* state){
+    (void)state;
+    reb_simulation_update_acceleration(r);
+    const double dt = r->dt;
+    for (size_t i=0; i<r->N; i++){
+        struct reb_particle* p = &r->particles[i];
+        p->vx += dt*p->ax; p->vy += dt*p->ay; p->vz += dt*p->az;
+        p->x += dt*p->vx; p->y += dt*p->vy; p->z += dt*p->vz;
+    }
+    r->t += dt;
+    r->dt_last_done = dt;
+}
+
+static void gen_custom_integrator(void){
+    struct reb_integrator euler = {0};
+    euler.step = euler_step;
+    reb_integrator_register(euler, "test_symplectic_euler");
+    struct reb_simulation* r = plummer(64, 6u, 0.01);
+    reb_simulation_set_integrator(r, "test_symplectic_euler");
+    r->gravity = REB_GRAVITY_BASIC;
+    r->dt = 1e-3;
+    r->exact_finish_time = 0;
+    dump_state("integ_custom_init.npy", r);
+    int st = reb_simulation_integrate(r, 0.01);
+    dump_state("integ_custom_state.npy", r); dump_meta("integ_custom_meta.npy", r, st);
+    reb_simulation_free(r);
+}
+
+/* The shearing-sheet wrap (boundary.c:78-114) on bodies placed up to 2.7 box
+ * widths outside the box, at three times. */
+static void gen_shear_wrap(void){
+    struct reb_simulation* r = sheet(600, 50., 2, 2, 17u);
+    const double bx = 100., by = 100., bz = 50.;
+    for (size_t i=0; i<r->N; i++){
+        r->particles[i].x = reb_random_uniform(r, -2.7*bx, 2.7*bx);
+        r->particles[i].y = reb_random_uniform(r, -2.7*by, 2.7*by);
+        r->particles[i].z = reb_random_uniform(r, -2.7*bz, 2.7*bz);
+    }
+    dump_state("shear_wrap_in.npy", r);
+    const double ts[3] = {0., 3.7, 1234.56789};
+    double* out = malloc(sizeof(double)*3*NCOL*r->N);
+    double* in = state_of(r);
+    for (int k=0; k<3; k++){
+        for (size_t i=0; i<r->N; i++){
+            struct reb_particle* p = &r->particles[i];
+            const double* row = in + NCOL*i;
+            p->x = row[0]; p->y = row[1]; p->z = row[2]; p->vx = row[3]; p->vy = row[4]; p->vz = row[5];
+        }
+        r->t = ts[k];
+        reb_boundary_check(r);
+        double* st = state_of(r);
+        memcpy(out + (size_t)k*NCOL*r->N, st, sizeof(double)*NCOL*r->N);
+        free(st);
+    }
+    size_t shape[3] = {3, r->N, NCOL};
+    write_npy("shear_wrap_out.npy", "<f8", 8, out, 3, shape);
+    wd("shear_wrap_t.npy", ts, 3, 0);
+    free(out); free(in);
+    reb_simulation_free(r);
+}
+
 int main(int argc, char** argv){
     if (argc < 3){ fprintf(stderr, "usage: gen_golden OUTDIR serial|omp\n"); return 1; }
     outdir = argv[1];
+    rawdir = getenv("GEN_GOLDEN_RAWDIR");
     if (strcmp(argv[2], "omp") == 0){
         gen_plummer_omp();
         gen_ghostbox_basic("omp");
@@ -954,5 +1266,8 @@ int main(int argc, char** argv){
     gen_plummer_full();
     gen_line();
     gen_archive();
+    gen_integrate();
+    gen_custom_integrator();
+    gen_shear_wrap();
     return 0;
 }

@@ -5,14 +5,35 @@ The reference library is compiled from the reference's own C sources by
 against the reference's header, links it to the serial and the OpenMP builds,
 and runs both.  It needs the reference source tree and is run by hand, in the
 container that has it; the fixtures it writes are committed.
+
+Full-size states are too large to commit: the serial run writes them as raw
+records (x y z vx vy vz of body 0..N-1, in order) and this script stores the
+SHA-256 of each record instead.
 """
+import hashlib
 import os
 import subprocess
 import sys
 
+import numpy as np
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 REFSRC = os.environ.get("REFSRC", "/root/reference/src")
+
+
+# raw file -> (fixture, bodies per record)
+RAW = {"plummer65536_seed42.raw": ("plummer65536_seed42_sha256.npy", 65536),
+       "sheet200k.raw": ("sheet200k_sha256.npy", 200000)}
+
+
+def hash_records(rawdir):
+    for raw, (fixture, n) in RAW.items():
+        data = np.fromfile(os.path.join(rawdir, raw), dtype="<f8").reshape(-1, n, 6)
+        digests = np.array([np.frombuffer(hashlib.sha256(np.ascontiguousarray(rec).tobytes()).digest(),
+                                          dtype=np.uint8) for rec in data])
+        np.save(os.path.join(HERE, fixture), digests)
+        os.remove(os.path.join(rawdir, raw))
 
 
 def main():
@@ -28,7 +49,15 @@ def main():
             cmd.append("-fopenmp")
         subprocess.check_call(cmd)
         env = dict(os.environ, OMP_NUM_THREADS="4")
+        rawdir = os.path.join(refdir, "raw")
+        if flavour == "serial":
+            os.makedirs(rawdir, exist_ok=True)
+            for f in os.listdir(rawdir):
+                os.remove(os.path.join(rawdir, f))
+            env["GEN_GOLDEN_RAWDIR"] = rawdir
         subprocess.check_call([exe, HERE, flavour], env=env)
+        if flavour == "serial":
+            hash_records(rawdir)
     print("fixtures written to", HERE)
 
 

@@ -158,3 +158,49 @@ def test_fresh_simulation_writes_reference_bytes(amd, tmp_path):
             assert np.array_equal(particle_doubles(a), particle_doubles(b))
         else:
             assert a == b, n
+
+
+def _field_offsets(buf, pos=64):
+    out = {}
+    while pos + FIELD.size <= len(buf):
+        size_name, size_data = FIELD.unpack_from(buf, pos)
+        name = bytes(buf[pos + FIELD.size:pos + FIELD.size + size_name - 1]).decode()
+        out[name] = pos
+        pos += FIELD.size + size_name + size_data
+        if name == "end":
+            break
+    return out
+
+
+@pytest.mark.parametrize("damage", ["truncated", "huge_data", "huge_name", "unterminated_name", "bad_name_list"])
+def test_damaged_file_read_safely(amd, tmp_path, damage):
+    """A truncated file, field sizes that would wrap the read position, a
+    field name without its terminating NUL and a name list whose last entry
+    runs past its field: the reader stops at the damage (NULL, or a
+    simulation with a warning) instead of reading past the buffer."""
+    import ctypes
+    buf = bytearray(open(os.path.join(GOLDEN, "archive_leapfrog.bin"), "rb").read())
+    off = _field_offsets(buf)
+    p = off["particles"]
+    if damage == "truncated":
+        buf = buf[:p + FIELD.size + 40]
+    elif damage == "huge_data":
+        struct.pack_into("<Q", buf, p + 8, 2 ** 64 - 16)
+    elif damage == "huge_name":
+        struct.pack_into("<Q", buf, p, 2 ** 64 - 8)
+    elif damage == "unterminated_name":
+        size_name, _ = FIELD.unpack_from(buf, p)
+        buf[p + FIELD.size + size_name - 1] = ord("x")
+    else:
+        # a name_list field holding one name with no room for its pointer
+        entry = FIELD.pack(len(b"name_list\0"), 6) + b"name_list\0" + b"abcde\0"
+        buf = buf[:p] + entry + buf[p:]
+    path = tmp_path / ("damaged_%s.bin" % damage)
+    path.write_bytes(bytes(buf))
+    ptr = amd.lib().reb_simulation_create_from_file(os.fsencode(str(path)), 0)
+    if ptr:
+        sim = amd.Sim(save_messages=False, _ptr=ptr)
+        assert sim.s.N <= 64
+        assert sim.s.N == 0 or bool(sim.s.particles)
+        sim.free()
+    assert ctypes.sizeof(amd._abi.Simulation) > 0

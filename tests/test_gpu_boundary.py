@@ -46,3 +46,26 @@ def test_periodic_leapfrog_bitwise(gpu, gravity, steps):
     sim = boxed(gpu, "periodic", gravity, 4.0, 3, golden("periodic300_init.npy"))
     sim.steps(steps)
     assert np.array_equal(sim.state()[:, :6], golden("periodic300_final_%s.npy" % gravity)[:, :6])
+
+
+@pytest.mark.parametrize("k", [0, 1, 2])
+def test_shear_wrap_bitwise(gpu, k):
+    """The SHEAR wrap on the device (boundary.c:78-114) against the
+    reference's reb_boundary_check on bodies up to 2.7 box widths outside the
+    box, at three times: the integrator "none" with dt = 0 leaves t in place,
+    so one step applies the wrap alone."""
+    t = float(golden("shear_wrap_t.npy")[k])
+    sim = gpu.Sim()
+    s = sim.s
+    s.boundary = gpu.BOUNDARY["shear"]
+    s.OMEGA = 1.0
+    s.root_size = 50.0
+    s.N_root_x, s.N_root_y, s.N_root_z = 2, 2, 1
+    sim.set_integrator("none")
+    s.t = t
+    s.dt = 0.0
+    sim.set_state(golden("shear_wrap_in.npy"))
+    sim.steps(1)
+    assert s.t == t
+    assert np.array_equal(sim.state()[:, :6], golden("shear_wrap_out.npy")[k][:, :6])
+    sim.free()

@@ -1,7 +1,12 @@
 """GPU parity of direct gravity (gravity.c:167-531) and the energy (tools.c:108-162).
 
 EXACT is bit-identical to the reference; FP64 within 1e-14 of the summed
-magnitudes sum_j |a_ij| per body; SPLIT within 1e-6 of it (FP32 r^-3)."""
+magnitudes sum_j |a_ij| per body; SPLIT within 1e-7 of it (FP32 r^-3), with
+the 99th percentile of the plain relative error within 1e-7 as well.  At the
+benchmark's full size (65536 bodies) the FP64 modes are within 1e-14 of the
+exact force (terms and sum in extended precision), relative to |a_i|."""
+import hashlib
+
 import numpy as np
 import pytest
 
@@ -10,7 +15,7 @@ from conftest import golden
 pytestmark = pytest.mark.gpu
 
 TOL_FP64 = 1e-14
-TOL_SPLIT = 1e-6
+TOL_SPLIT = 1e-7
 
 
 def make_sim(amd, state, soft, precision, G=1.0):
@@ -119,10 +124,13 @@ def test_fast_modes_plummer4096(gpu, grav, prec, tol):
     sim = make_sim(gpu, st, 0.005, prec)
     sim.s.gravity = gpu.GRAVITY[grav]
     acc = sim.update_acceleration()
-    err = np.sqrt(((acc - ref) ** 2).sum(1)) / abs_sum(st, 0.005)
-    print("%s/%s max normalised error %.3e, median relative %.3e" % (
-        grav, prec, err.max(), np.median(np.sqrt(((acc - ref) ** 2).sum(1)) / np.sqrt((ref ** 2).sum(1)))))
+    da = np.sqrt(((acc - ref) ** 2).sum(1))
+    err = da / abs_sum(st, 0.005)
+    rel = da / np.sqrt((ref ** 2).sum(1))
+    print("%s/%s max normalised error %.3e, relative p99 %.3e max %.3e" % (
+        grav, prec, err.max(), np.quantile(rel, 0.99), rel.max()))
     assert err.max() <= tol
+    assert np.quantile(rel, 0.99) <= tol
 
 
 @pytest.mark.parametrize("prec", ["fp64", "split"])
@@ -262,19 +270,92 @@ def test_fp64_large_n_sampled_rows(gpu, orc):
     assert err.max() <= TOL_FP64
 
 
-def test_exact_leapfrog_full_size_two_steps_bitwise(gpu):
-    """The direct-summation benchmark at full size (Plummer sphere of 65536
-    bodies, seed 42, softening 0.005, leapfrog with dt 5e-4, BASIC gravity)
-    in exact mode: the XOR of the bit patterns of every body's position and
-    velocity after construction and after each of two steps equals the
-    reference library's (serial build) on the same construction."""
-    want = golden("plummer65536_seed42_xor.npy").view(np.uint64)
+def full_size_plummer(gpu):
+    """The direct-summation benchmark's bodies: Plummer sphere of 65536
+    bodies, seed 42, moved to the centre of mass (the product's generator,
+    bit-identical to the reference's, see the leapfrog test below)."""
     sim = gpu.Sim()
     sim.s.rand_seed = 42
     sim.s.softening = 0.005
     sim.s.G = 1.0
     sim.add_plummer(65536, 1.0, 1.0)
     sim.move_to_com()
+    return sim
+
+
+def exact_rows(state, soft, rows):
+    """The exact force on the target rows (every term and the sum in x87
+    extended precision, 64-bit significand) and sum_j |a_ij|."""
+    x = state[:, :3].astype(np.longdouble)
+    m = state[:, 9].astype(np.longdouble)
+    soft2 = np.longdouble(soft) * np.longdouble(soft)
+    truth = np.zeros((len(rows), 3))
+    cond = np.zeros(len(rows))
+    for b in range(0, len(rows), 16):
+        r = rows[b:b + 16]
+        d = x[r, None, :] - x[None, :, :]
+        r2 = (d * d).sum(2) + soft2
+        w = m[None, :] / (r2 * np.sqrt(r2))
+        w[np.arange(len(r)), r] = 0
+        t = w[..., None] * d
+        truth[b:b + 16] = (-t.sum(1)).astype(np.float64)
+        cond[b:b + 16] = np.sqrt((t * t).sum(2)).sum(1).astype(np.float64)
+    return truth, cond
+
+
+@pytest.fixture(scope="module")
+def full_size(gpu, orc):
+    """Bodies, 512 sampled rows, the exact force on them and the oracle's
+    (the reference's per-target sum, gravity.c:215-232)."""
+    sim = full_size_plummer(gpu)
+    st = sim.state()
+    sim.free()
+    rows = np.sort(np.random.default_rng(2024).choice(65536, 512, replace=False))
+    rows[:4] = (0, 1, 2, 65535)
+    rows = np.unique(rows)
+    truth, cond = exact_rows(st, 0.005, rows)
+    ref = orc.gravity_rows(st, rows, soft=0.005)
+    return st, rows, truth, cond, ref
+
+
+@pytest.mark.parametrize("grav,prec", [("basic", "fp64"), ("compensated", "fp64"), ("basic", "split")])
+def test_fast_modes_full_size(gpu, full_size, grav, prec):
+    """The three timed modes of the benchmark at N=65536 (softening 0.005) on
+    512 sampled rows.  FP64 and COMPENSATED: plain relative error against the
+    exact force <= 1e-14 on every row.  The reference's own double-precision
+    sum is itself about 2e-14 from the exact force on the worst rows, so the
+    comparison with the oracle allows that distance plus 1e-14.  SPLIT: error
+    <= 1e-7 of sum_j |a_ij| on every row and 99th percentile of the plain
+    relative error <= 1e-7."""
+    st, rows, truth, cond, ref = full_size
+    sim = full_size_plummer(gpu)
+    sim.s.gravity = gpu.GRAVITY[grav]
+    sim.precision = prec
+    acc = sim.update_acceleration()[rows]
+    sim.free()
+    tn = np.sqrt((truth ** 2).sum(1))
+    da = np.sqrt(((acc - truth) ** 2).sum(1))
+    rel = da / tn
+    ref_rel = np.sqrt(((ref - truth) ** 2).sum(1)) / tn
+    vs_ref = np.sqrt(((acc - ref) ** 2).sum(1)) / tn
+    print("%s/%s vs exact: rel max %.3e p99 %.3e normalised max %.3e | oracle vs exact: rel max %.3e" % (
+        grav, prec, rel.max(), np.quantile(rel, 0.99), (da / cond).max(), ref_rel.max()))
+    if prec == "fp64":
+        assert rel.max() <= TOL_FP64
+        assert np.all(vs_ref <= ref_rel + TOL_FP64)
+    else:
+        assert (da / cond).max() <= TOL_SPLIT
+        assert np.quantile(rel, 0.99) <= TOL_SPLIT
+
+
+def test_exact_leapfrog_full_size_two_steps_bitwise(gpu):
+    """The direct-summation benchmark at full size (Plummer sphere of 65536
+    bodies, seed 42, softening 0.005, leapfrog with dt 5e-4, BASIC gravity)
+    in exact mode: the SHA-256 of every body's position and velocity, in body
+    order, after construction and after each of two steps equals the
+    reference library's (serial build) on the same construction."""
+    want = golden("plummer65536_seed42_sha256.npy")
+    sim = full_size_plummer(gpu)
     sim.set_integrator("leapfrog")
     sim.s.dt = 5e-4
     sim.s.gravity = gpu.GRAVITY["basic"]
@@ -283,5 +364,5 @@ def test_exact_leapfrog_full_size_two_steps_bitwise(gpu):
         if s:
             sim.steps(1)
         st = np.ascontiguousarray(sim.state()[:, :6])
-        assert int(np.bitwise_xor.reduce(st.view(np.uint64).ravel())) == int(want[s]), s
+        assert hashlib.sha256(st.tobytes()).digest() == want[s].tobytes(), s
     sim.free()

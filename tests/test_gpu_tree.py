@@ -3,6 +3,7 @@ collision searches with the shearing-sheet ghost boxes (collision.c), on the
 small sheet written by the compiled reference (2000 bodies, 2x2 root boxes,
 SEI, hard spheres with restitution 0.5)."""
 import ctypes
+import hashlib
 import math
 
 import numpy as np
@@ -288,18 +289,55 @@ def test_level_parallel_resolve_equals_sequential_at_scale(gpu):
 
 def test_sheet_full_size_three_steps_bitwise(gpu):
     """The benchmark's sheet at full size (200000 bodies): after each of three
-    steps, the XOR of the bit patterns of every body's position and velocity,
-    the bounce count, the pressure log and the random seed equal the reference
-    library's (serial build) on the same construction."""
+    steps, the SHA-256 of every body's position and velocity in body order,
+    the bounce count, the pressure log and the random seed equal the
+    reference library's (serial build) on the same construction."""
     meta = golden("sheet200k_meta.npy")
     plog = golden("sheet200k_plog.npy")
+    digests = golden("sheet200k_sha256.npy")
     sim = gpu.shearing_sheet(200000, 136.0, 10, 99)
     for s in range(3):
         sim.steps(1)
         st = np.ascontiguousarray(sim.state()[:, :6])
-        x = np.bitwise_xor.reduce(st.view(np.uint64).ravel())
-        assert int(x) == int(np.int64(meta[s, 0]).view(np.uint64)), s
+        assert hashlib.sha256(st.tobytes()).digest() == digests[s].tobytes(), s
         assert sim.s.collisions_log_n == meta[s, 1], s
         assert sim.s.rand_seed == int(np.int64(meta[s, 2]).view(np.uint64)) & 0xffffffff, s
         assert sim.s.collisions_plog == plog[s], s
     sim.free()
+
+
+def test_two_simulations_resolving_from_two_threads(gpu):
+    """Two sheets of 60000 bodies stepped at the same time from two host
+    threads (tens of thousands of collisions a step each, through the
+    level-parallel resolve and its shared pool of host threads) end bit for
+    bit where the same sheets end when stepped one after the other."""
+    import threading
+    seeds = (5, 6)
+    want = []
+    for sd in seeds:
+        sim = gpu.shearing_sheet(60000, 124.0, 6, sd)
+        sim.steps(3)
+        want.append((sim.state()[:, :6], sim.s.collisions_plog, sim.s.collisions_log_n, sim.s.rand_seed))
+        sim.free()
+    sims = [gpu.shearing_sheet(60000, 124.0, 6, sd) for sd in seeds]
+    errors = []
+
+    def run(sim):
+        try:
+            for _ in range(3):
+                sim.lib.reb_simulation_steps(sim.ptr, 1)
+        except Exception as e:  # reported below
+            errors.append(e)
+
+    threads = [threading.Thread(target=run, args=(s,)) for s in sims]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(240)
+        assert not t.is_alive()
+    assert not errors
+    for sim, (st, plog, logn, seed) in zip(sims, want):
+        sim.check()
+        assert np.array_equal(sim.state()[:, :6], st)
+        assert (sim.s.collisions_plog, sim.s.collisions_log_n, sim.s.rand_seed) == (plog, logn, seed)
+        sim.free()

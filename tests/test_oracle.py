@@ -273,6 +273,22 @@ def _sheet_gbs(t, ng=2, six=False):
     return np.array(out)
 
 
+def test_shear_wrap_bitwise(orc):
+    """orc_shear_wrap against the reference's reb_boundary_check (SHEAR,
+    boundary.c:78-114) on 600 bodies placed up to 2.7 box widths outside a
+    100 x 100 x 50 box, at t = 0, 3.7 and 1234.56789: the shifted y and vy of
+    radial wraps and the repeated wraps agree bit for bit."""
+    inp = golden("shear_wrap_in.npy")
+    out = golden("shear_wrap_out.npy")
+    moved = 0
+    for k, t in enumerate(golden("shear_wrap_t.npy")):
+        s = np.ascontiguousarray(inp.copy())
+        orc.lib().orc_shear_wrap(s, s.shape[0], 1.0, 100.0, 100.0, 50.0, float(t))
+        assert np.array_equal(s[:, :6], out[k][:, :6]), t
+        moved += int((out[k][:, 4] != inp[:, 4]).sum())
+    assert moved > 0
+
+
 def test_tree_gravity_bitwise(orc):
     s = golden("sheet2000_init.npy")
     acc, inter = orc.tree_gravity(s, SHEET["root_size"], SHEET["nr"], 1.546 / 400.0, 0.1, 0.25, _sheet_gbs(0.0))
