@@ -303,6 +303,39 @@ REB_API void reb_simulation_save_to_file_walltime(struct reb_simulation* const r
 REB_API void reb_simulation_save_to_file_step(struct reb_simulation* const r, const char* filename, uint64_t step);
 REB_API size_t reb_simulation_struct_size(void);                     /* simulation.c:65-67 */
 
+/* Archive objects (rebound.h:427,624-626; simulationarchive.c:45-375): an
+ * index of the snapshots of an archive file, kept open, from which any
+ * snapshot is read.  A missing file gives NULL. */
+REB_API struct reb_simulationarchive* reb_simulationarchive_create_from_file(const char* filename);
+REB_API void reb_simulationarchive_free(struct reb_simulationarchive* sa);
+REB_API struct reb_simulation* reb_simulation_create_from_simulationarchive(struct reb_simulationarchive* sa,
+                                                                           int64_t snapshot);
+/* The same with the reference's REB_BINARYDATA_* codes (binarydata.h:31-45)
+ * OR-ed into *warnings instead of messages (simulationarchive.h:58-60, the
+ * calls its Python package makes). */
+REB_API struct reb_simulationarchive* reb_simulationarchive_create_from_file_with_messages(const char* filename,
+                                                                                         int* warnings);
+REB_API struct reb_simulationarchive* reb_simulationarchive_create_from_buffer_with_messages(char* buffer,
+                                                                                           size_t size,
+                                                                                           int* warnings);
+REB_API void reb_simulation_init_from_simulationarchive_with_messages(struct reb_simulation* r,
+                                                                     struct reb_simulationarchive* sa,
+                                                                     int64_t snapshot, int* warnings);
+/* Comparison through the binary format (simulation.c:618-709): 0 when equal
+ * (fields named walltime* aside), 1 when not; output_option 1 prints the
+ * differing fields.  diff_char returns them as a string to free with
+ * reb_free. */
+REB_API int reb_simulation_diff(struct reb_simulation* r1, struct reb_simulation* r2, int output_option);
+REB_API char* reb_simulation_diff_char(struct reb_simulation* r1, struct reb_simulation* r2);
+/* The simulation as one binary snapshot in a malloc'd buffer
+ * (binarydata.c:582-606), and a copy made through that format into a freshly
+ * created simulation (simulation.c:711-717). */
+REB_API void reb_binarydata_simulation_to_stream(struct reb_simulation* r, char** bufp, size_t* sizep);
+REB_API void reb_simulation_copy_with_messages(struct reb_simulation* r_copy, struct reb_simulation* r,
+                                               int* warnings);
+/* Frees memory the library returned (rebound.c:215-217). */
+REB_API void reb_free(void* p);
+
 /* Build, version and hash strings (rebound.c:41-43); the reference's Python
  * package reads reb_version_str and reb_build_str with ctypes in_dll. */
 REB_API extern const char* reb_build_str;
@@ -354,6 +387,49 @@ REB_API struct reb_particle reb_particle_nan(void);
 REB_API void reb_simulation_remove_all_particles(struct reb_simulation* const r);
 REB_API int reb_simulation_remove_particle(struct reb_simulation* const r, size_t index);
 REB_API int reb_particle_cmp(struct reb_particle p1, struct reb_particle p2);
+/* particle.c:128-136 (-1 when p is not one of its simulation's particles),
+ * :375-410; tools.c:454-461, :1524-1569 (-1 when the counts differ) */
+REB_API int reb_simulation_particle_index(struct reb_particle* p);
+REB_API struct reb_particle reb_simulation_jacobi_com(struct reb_particle* p);
+REB_API void reb_particle_iadd(struct reb_particle* p1, struct reb_particle* p2);
+REB_API void reb_particle_isub(struct reb_particle* p1, struct reb_particle* p2);
+REB_API void reb_particle_imul(struct reb_particle* p1, double value);
+REB_API double reb_particle_distance(struct reb_particle* p1, struct reb_particle* p2);
+REB_API void reb_simulation_imul(struct reb_simulation* r, double scalar_pos, double scalar_vel);
+REB_API int reb_simulation_iadd(struct reb_simulation* r, struct reb_simulation* r2);
+REB_API int reb_simulation_isub(struct reb_simulation* r, struct reb_simulation* r2);
+/* simulation.c:801-861: bulk copies; any of the arrays may be NULL */
+REB_API void reb_simulation_get_serialized_particle_data(struct reb_simulation* r, double* m, double* radius,
+                                                         double (*xyz)[3], double (*vxvyvz)[3],
+                                                         double (*xyzvxvyvz)[6]);
+REB_API void reb_simulation_set_serialized_particle_data(struct reb_simulation* r, double* m, double* radius,
+                                                         double (*xyz)[3], double (*vxvyvz)[3],
+                                                         double (*xyzvxvyvz)[6]);
+
+/* ---------------------------------------------------------------- orbits
+ * rebound.h:632-660 (layout), tools.c:504-560 (anomalies), :1004-1208
+ * (elements of a particle about a primary; err 1: massless primary, 2: the
+ * particle sits on the primary), :1571-1583 (spherical coordinates) */
+struct reb_orbit {
+    double d, v, h, P, n, a, e, inc, Omega, omega, pomega, f, M, l, theta, T, rhill;
+    double pal_h, pal_k, pal_ix, pal_iy;
+    struct reb_vec3d hvec;
+    struct reb_vec3d evec;
+};
+REB_API struct reb_orbit reb_orbit_from_particle(double G, struct reb_particle p, struct reb_particle primary);
+REB_API struct reb_orbit reb_orbit_from_particle_err(double G, struct reb_particle p, struct reb_particle primary,
+                                                     int* err);
+REB_API double reb_mod2pi(double f);
+REB_API double reb_M_to_E(double e, double M);
+REB_API double reb_E_to_f(double e, double E);
+REB_API double reb_M_to_f(double e, double M);
+REB_API struct reb_vec3d reb_tools_spherical_to_xyz(const double magnitude, const double theta, const double phi);
+REB_API void reb_tools_xyz_to_spherical(const struct reb_vec3d xyz, double* magnitude, double* theta, double* phi);
+/* tools.c:48-56: djb2 hash of a string (unit names) */
+REB_API uint32_t reb_hash(const char* str);
+/* The reference's OpenMP thread count; the engine's force and step run on the
+ * GPU, so this is accepted and has no effect. */
+REB_API void reb_omp_set_num_threads(int num_threads);
 
 /* ---------------------------------------------------------------- collisions
  * collision.c:573-737 */

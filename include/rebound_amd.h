@@ -27,9 +27,13 @@ extern "C" {
  *  EXACT  IEEE-754 sqrt and division, no contraction, each target summing its
  *         sources in ascending index order exactly as the reference's
  *         OpenMP loop: accelerations are bit-identical to the reference.
- *  SPLIT  Displacement and r^2 in FP64, r^-3 * m_j in FP32 on the vector
- *         units, accumulation in FP64 with Kahan compensation applied per
- *         tile of sources.
+ *  SPLIT  Displacement and r^2 in FP64, r^-1 from the FP32 reciprocal square
+ *         root on the vector units (about 2^-23 relative), cubed and scaled by
+ *         m_j in FP64, accumulation in FP64 with Kahan compensation applied
+ *         per tile of sources.  Per particle |a - a_ref| <= 1e-7 *
+ *         sum_j |a_ij|.  r^2 + softening^2 must lie in FP32's normal range:
+ *         about 1e-19 < r < 1.8e19 in simulation units (SI distances of the
+ *         Solar System are well inside it).
  */
 enum REB_AMD_PRECISION {
     REB_AMD_PRECISION_FP64 = 0,
@@ -186,7 +190,10 @@ REB_API int reb_amd_ensemble_get_particles(struct reb_amd_ensemble* e, struct re
 /* reb_simulation_energy of every system (tools.c:108-162). */
 REB_API int reb_amd_ensemble_energy(struct reb_amd_ensemble* e, double* out);
 /* Per system: bit 0 synchronized, bit 1 Kepler step longer than an orbital
- * period, bit 2 particles modified while not synchronized. */
+ * period, bit 2 particles modified while not synchronized, bit 3 a Kepler
+ * solve took the quartic (Laguerre-Conway) iteration, bit 4 a Kepler solve
+ * fell back to bisection (integrator_whfast.c:289-369).  Bits 1-4 stay set
+ * from launch to launch. */
 REB_API int reb_amd_ensemble_flags(struct reb_amd_ensemble* e, unsigned int* out);
 /* Launch shape: persistent workgroups (0 = default) and steps per launch
  * (0 = a whole request in one launch). */

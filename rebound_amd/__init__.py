@@ -78,6 +78,15 @@ class Sim:
             raise RebError("cannot read %s (%s)" % (path, lib().reb_amd_last_error().decode()))
         return cls(save_messages=True, _ptr=ptr)
 
+    @classmethod
+    def wrap(cls, address):
+        """A Sim owning a simulation the library created elsewhere (an address
+        or c_void_p), e.g. by reb_simulation_create_from_simulationarchive."""
+        addr = address.value if isinstance(address, ctypes.c_void_p) else address
+        if not addr:
+            raise RebError("NULL simulation (%s)" % lib().reb_amd_last_error().decode())
+        return cls(save_messages=True, _ptr=ctypes.cast(ctypes.c_void_p(addr), ctypes.POINTER(_abi.Simulation)))
+
     def save(self, path):
         """reb_simulation_save_to_file: a new file, or a snapshot appended to an archive."""
         self.lib.reb_simulation_save_to_file(self.ptr, os.fsencode(path))

@@ -163,6 +163,11 @@ def declare(lib):
         "reb_collision_resolve_merge": (c_int, [S, Collision]),
         "reb_integrator_register": (None, [Integrator, c_char_p]),
         "reb_integrator_whfast_kepler_solver": (None, [POINTER(Particle), c_double, c_double, S]),
+        "reb_simulation_diff": (c_int, [S, S, c_int]),
+        "reb_simulation_diff_char": (c_void_p, [S, S]),
+        "reb_binarydata_simulation_to_stream": (None, [S, POINTER(c_void_p), POINTER(c_size_t)]),
+        "reb_simulation_copy_with_messages": (None, [S, S, POINTER(c_int)]),
+        "reb_free": (None, [c_void_p]),
         "reb_amd_set_precision": (c_int, [S, c_int]),
         "reb_amd_get_precision": (c_int, [S]),
         "reb_amd_device_count": (c_int, []),

@@ -25,6 +25,23 @@
 
 #include "internal.h"
 
+// simulationarchive.h:30-43: the archive index, in the reference's layout
+// (its Python package mirrors the struct field by field).
+struct reb_simulationarchive {
+    FILE* inf;                 // the archive's stream, kept open
+    char* filename;            // NULL for an archive read from memory
+    int version;
+    int reb_version_major;
+    int reb_version_minor;
+    int reb_version_patch;
+    double auto_interval;
+    double auto_walltime;
+    uint64_t auto_step;
+    int64_t nblobs;            // snapshots, the first one included
+    uint64_t* offset;          // byte offset of each snapshot
+    double* t;                 // simulation time of each snapshot
+};
+
 namespace rbamd {
 
 void whfast_refresh_host(struct reb_simulation* r);   // whfast.hip
@@ -550,6 +567,147 @@ void report(struct reb_simulation* r, unsigned w) {
         reb_simulation_warning(r, "The binary file seems to be corrupted. An attempt has been made to read the uncorrupted parts of it.");
 }
 
+// The reference's REB_BINARYDATA_* codes (binarydata.h:31-45), which the
+// *_with_messages entry points report (its Python package maps them to
+// exceptions and warnings).
+enum RefCode : int {
+    RC_NOFILE = 1,
+    RC_VERSION = 2,
+    RC_POINTERS = 4,
+    RC_FILENOTOPEN = 16,
+    RC_OUTOFRANGE = 32,
+    RC_SEEK = 64,
+    RC_FIELD_UNKNOWN = 128,
+    RC_CORRUPTFILE = 512,
+    RC_OLD = 1024,
+};
+
+int ref_codes(unsigned w) {
+    int c = 0;
+    if (w & W_VERSION) c |= RC_VERSION;
+    if (w & W_POINTERS) c |= RC_POINTERS;
+    if (w & W_UNKNOWN) c |= RC_FIELD_UNKNOWN;
+    if (w & W_CORRUPT) c |= RC_CORRUPTFILE;
+    return c;
+}
+
+// binarydata.c:881-930: messages for the codes; NULL when an error ends the
+// read (the simulation is freed).
+struct reb_simulation* process_codes(struct reb_simulation* r, int c) {
+    auto fail = [&](const char* msg) -> struct reb_simulation* {
+        reb_simulation_error(r, msg);
+        if (r) reb_simulation_free(r);
+        return nullptr;
+    };
+    if (c & RC_NOFILE) return fail("Cannot read binary file. Check filename and file contents.");
+    if (c & RC_VERSION)
+        reb_simulation_warning(r, "Binary file was saved with a different version of REBOUND. Binary format might have changed.");
+    if (c & RC_POINTERS)
+        reb_simulation_warning(r, "You have to reset function pointers after creating a reb_simulation struct with a binary file.");
+    if (c & RC_FILENOTOPEN) return fail("Error while reading binary file (file was not open).");
+    if (c & RC_OUTOFRANGE) return fail("Index out of range.");
+    if (c & RC_SEEK) return fail("Error while trying to seek file.");
+    if (c & RC_FIELD_UNKNOWN) reb_simulation_warning(r, "Unknown field found in binary file.");
+    if (c & RC_OLD)
+        return fail("Reading old Simulationarchives (version < 2) is no longer supported. If you need to read such an "
+                    "archive, use a REBOUND version <= 3.26.3");
+    if (c & RC_CORRUPTFILE)
+        reb_simulation_warning(r, "The binary file seems to be corrupted. An attempt has been made to read the uncorrupted parts of it.");
+    return r;
+}
+
+// Readable value of a field for the diff printout: doubles and integers by
+// value, other data by size.
+std::string field_text(const Span& s) {
+    char tmp[64];
+    if (strcmp(s.name, "particles") == 0) {
+        std::string out;
+        for (size_t off = 0; off + sizeof(struct reb_particle) <= s.f.size_data; off += sizeof(struct reb_particle)) {
+            struct reb_particle p;
+            memcpy(&p, s.data + off, sizeof(p));
+            snprintf(tmp, sizeof(tmp), "%s(%.17g %.17g %.17g", off ? " " : "", p.x, p.y, p.z);
+            out += tmp;
+            snprintf(tmp, sizeof(tmp), " %.17g %.17g %.17g m %.17g)", p.vx, p.vy, p.vz, p.m);
+            out += tmp;
+        }
+        return out;
+    }
+    const Desc* d = nullptr;
+    for (const Desc& e : kSimFields)
+        if (strcmp(e.name, s.name) == 0) d = &e;
+    if (d && d->type == T::DOUBLE && s.f.size_data == sizeof(double)) {
+        double v;
+        memcpy(&v, s.data, sizeof(v));
+        snprintf(tmp, sizeof(tmp), "%.17g", v);
+        return tmp;
+    }
+    if (d && (d->type == T::INT || d->type == T::UINT || d->type == T::UINT32) && s.f.size_data == 4) {
+        int32_t v;
+        memcpy(&v, s.data, sizeof(v));
+        snprintf(tmp, sizeof(tmp), "%d", v);
+        return tmp;
+    }
+    if (d && (d->type == T::INT64 || d->type == T::UINT64 || d->type == T::SIZE_T) && s.f.size_data == 8) {
+        int64_t v;
+        memcpy(&v, s.data, sizeof(v));
+        snprintf(tmp, sizeof(tmp), "%lld", (long long)v);
+        return tmp;
+    }
+    snprintf(tmp, sizeof(tmp), "(%llu bytes)", (unsigned long long)s.f.size_data);
+    return tmp;
+}
+
+// binarydata.c:259-488 (print and no-output modes): whether the two
+// snapshots differ, ignoring fields whose names start with "walltime", and
+// the differences as text.
+bool compare(const std::string& buf1, const std::string& buf2, std::string* text) {
+    bool different = false;
+    auto find = [](const std::string& buf, const char* name, Span* sp) {
+        size_t pos = 64;
+        Span t;
+        while (next_field(buf, &pos, &t)) {
+            if (strcmp(t.name, "end") == 0) return false;
+            if (strcmp(t.name, name) == 0) {
+                *sp = t;
+                return true;
+            }
+        }
+        return false;
+    };
+    if (buf1.size() >= 64 && buf2.size() >= 64 && memcmp(buf1.data(), buf2.data(), 64) != 0)
+        *text += "Header in binary files are different.\n";
+    size_t pos1 = 64;
+    Span a;
+    while (next_field(buf1, &pos1, &a) && strcmp(a.name, "end") != 0) {
+        Span b;
+        if (!find(buf2, a.name, &b)) {
+            different = true;
+            *text += std::string(a.name) + ":\n< " + field_text(a) + "\n";
+            continue;
+        }
+        bool differ = a.f.size_data != b.f.size_data;
+        if (!differ) {
+            if (strcmp(a.name, "particles") == 0)
+                differ = particles_differ(a.data, b.data, a.f.size_data);
+            else
+                differ = memcmp(a.data, b.data, a.f.size_data) != 0;
+        }
+        if (differ) {
+            if (strncmp(a.name, "walltime", 8) != 0) different = true;
+            *text += std::string(a.name) + ":\n< " + field_text(a) + "\n---\n> " + field_text(b) + "\n";
+        }
+    }
+    size_t pos2 = 64;
+    Span b;
+    while (next_field(buf2, &pos2, &b) && strcmp(b.name, "end") != 0) {
+        Span t;
+        if (find(buf1, b.name, &t)) continue;
+        different = true;
+        *text += std::string(b.name) + ":\n> " + field_text(b) + "\n";
+    }
+    return different;
+}
+
 }  // namespace
 
 // The host state a snapshot must see: particles (and WHFast's p_jh) current.
@@ -741,6 +899,225 @@ REB_API void reb_simulation_save_to_file_step(struct reb_simulation* const r, co
         r->simulationarchive_auto_step = step;
         r->simulationarchive_next_step = r->steps_done;
     }
+}
+
+// ------------------------------------------------------------ comparison and streams
+// simulation.c:691-709
+REB_API int reb_simulation_diff(struct reb_simulation* r1, struct reb_simulation* r2, int output_option) {
+    if (output_option != 0 && output_option != 1) {
+        reb_simulation_error(r1, "Invalid output_option in reb_simulation_diff().");
+        return -1;
+    }
+    if (!prepare_host(r1) || !prepare_host(r2)) return -1;
+    std::string text;
+    const bool different = compare(serialize(r1), serialize(r2), &text);
+    if (output_option == 1) {
+        fputs(text.c_str(), stdout);
+        fflush(stdout);
+    }
+    return different ? 1 : 0;
+}
+
+// simulation.c:618-634: the differences as a string the caller frees with reb_free.
+REB_API char* reb_simulation_diff_char(struct reb_simulation* r1, struct reb_simulation* r2) {
+    std::string text;
+    if (prepare_host(r1) && prepare_host(r2)) compare(serialize(r1), serialize(r2), &text);
+    return strdup(text.c_str());
+}
+
+// binarydata.c:582-606: the simulation as one snapshot in memory (malloc'd).
+REB_API void reb_binarydata_simulation_to_stream(struct reb_simulation* r, char** bufp, size_t* sizep) {
+    *bufp = nullptr;
+    *sizep = 0;
+    if (!prepare_host(r)) return;
+    const std::string buf = serialize(r);
+    *bufp = static_cast<char*>(malloc(buf.size()));
+    if (!*bufp) return;
+    memcpy(*bufp, buf.data(), buf.size());
+    *sizep = buf.size();
+}
+
+// simulation.c:711-717 through the binary format (reb_simulation_copy keeps
+// its direct struct copy): r_copy is a freshly created simulation.
+REB_API void reb_simulation_copy_with_messages(struct reb_simulation* r_copy, struct reb_simulation* r,
+                                               int* warnings) {
+    unsigned w = 0;
+    if (prepare_host(r)) {
+        input_fields(r_copy, serialize(r), 0, &w);
+        finish_input(r_copy, &w);
+        dev_of(r_copy)->precision = dev_of(r)->precision;
+        dev_of(r_copy)->cor_constant = dev_of(r)->cor_constant;
+    }
+    if (warnings) *warnings |= ref_codes(w);
+}
+
+// ------------------------------------------------------------ archive objects
+// simulationarchive.c:115-375.  The index is built once from the whole file
+// (the snapshot offsets and each snapshot's time); the stream stays open and
+// each snapshot read takes only its own bytes and the first snapshot's.
+
+static void archive_index(struct reb_simulationarchive* sa, const std::string& file, int* codes) {
+    bool corrupt = false;
+    const std::vector<size_t> off = snapshot_offsets(file, &corrupt);
+    // header "REBOUND Binary File. Version: major.minor.patch"
+    const char* colon = static_cast<const char*>(memchr(file.data(), ':', std::min<size_t>(file.size(), 64)));
+    if (colon && sscanf(colon + 1, "%d.%d.%d", &sa->reb_version_major, &sa->reb_version_minor,
+                        &sa->reb_version_patch) != 3)
+        *codes |= RC_CORRUPTFILE;
+    if (file.size() >= 64) {
+        char cur[64];
+        snprintf(cur, sizeof(cur), "REBOUND Binary File. Version: %s", reb_version_str);
+        if (strncmp(file.data(), cur, strlen(cur) + 1) != 0) *codes |= RC_VERSION;
+    }
+    size_t pos = 64;
+    Span f;
+    while (next_field(file, &pos, &f) && strcmp(f.name, "end") != 0) {
+        auto get = [&](void* dst, size_t n) { memcpy(dst, f.data, std::min<size_t>(n, f.f.size_data)); };
+        if (strcmp(f.name, "simulationarchive_version") == 0) get(&sa->version, sizeof(sa->version));
+        if (strcmp(f.name, "simulationarchive_auto_interval") == 0) get(&sa->auto_interval, sizeof(double));
+        if (strcmp(f.name, "simulationarchive_auto_walltime") == 0) get(&sa->auto_walltime, sizeof(double));
+        if (strcmp(f.name, "simulationarchive_auto_step") == 0) get(&sa->auto_step, sizeof(uint64_t));
+    }
+    if (sa->version < 5) {
+        *codes |= RC_OLD;
+        return;
+    }
+    if (off.empty()) {
+        *codes |= RC_SEEK;
+        return;
+    }
+    if (corrupt) *codes |= RC_CORRUPTFILE;
+    sa->nblobs = (int64_t)off.size();
+    sa->offset = static_cast<uint64_t*>(calloc(off.size(), sizeof(uint64_t)));
+    sa->t = static_cast<double*>(calloc(off.size(), sizeof(double)));
+    double t = 0.;
+    for (size_t k = 0; k < off.size(); k++) {
+        sa->offset[k] = off[k];
+        size_t p = k == 0 ? 64 : off[k];
+        while (next_field(file, &p, &f) && strcmp(f.name, "end") != 0)
+            if (strcmp(f.name, "t") == 0 && f.f.size_data == sizeof(double)) memcpy(&t, f.data, sizeof(double));
+        sa->t[k] = t;
+    }
+}
+
+static bool stream_bytes(FILE* inf, std::string* out) {
+    out->clear();
+    if (fseek(inf, 0, SEEK_SET) != 0) return false;
+    char chunk[1 << 16];
+    size_t n;
+    while ((n = fread(chunk, 1, sizeof(chunk), inf)) > 0) out->append(chunk, n);
+    return true;
+}
+
+REB_API struct reb_simulationarchive* reb_simulationarchive_create_from_file_with_messages(const char* filename,
+                                                                                         int* warnings) {
+    auto* sa = static_cast<struct reb_simulationarchive*>(calloc(1, sizeof(struct reb_simulationarchive)));
+    int codes = 0;
+    sa->inf = filename ? fopen(filename, "rb") : nullptr;
+    sa->filename = filename ? strdup(filename) : nullptr;
+    std::string file;
+    if (!sa->inf || !stream_bytes(sa->inf, &file))
+        codes |= RC_NOFILE;
+    else
+        archive_index(sa, file, &codes);
+    if ((codes & (RC_OLD | RC_SEEK)) && sa->inf) {
+        fclose(sa->inf);
+        sa->inf = nullptr;
+    }
+    if (warnings) *warnings |= codes;
+    return sa;
+}
+
+REB_API struct reb_simulationarchive* reb_simulationarchive_create_from_buffer_with_messages(char* buffer,
+                                                                                           size_t size,
+                                                                                           int* warnings) {
+    auto* sa = static_cast<struct reb_simulationarchive*>(calloc(1, sizeof(struct reb_simulationarchive)));
+    int codes = 0;
+    sa->inf = (buffer && size) ? fmemopen(buffer, size, "rb") : nullptr;
+    if (!sa->inf)
+        codes |= RC_NOFILE;
+    else
+        archive_index(sa, std::string(buffer, size), &codes);
+    if ((codes & (RC_OLD | RC_SEEK)) && sa->inf) {
+        fclose(sa->inf);
+        sa->inf = nullptr;
+    }
+    if (warnings) *warnings |= codes;
+    return sa;
+}
+
+REB_API struct reb_simulationarchive* reb_simulationarchive_create_from_file(const char* filename) {
+    int codes = 0;
+    struct reb_simulationarchive* sa = reb_simulationarchive_create_from_file_with_messages(filename, &codes);
+    if (codes & RC_NOFILE) {
+        // a missing file gives NULL without a message (simulationarchive.c:345-356)
+        reb_simulationarchive_free(sa);
+        return nullptr;
+    }
+    if (codes & (RC_OLD | RC_SEEK)) {
+        process_codes(nullptr, codes);
+        reb_simulationarchive_free(sa);
+        return nullptr;
+    }
+    process_codes(nullptr, codes);
+    return sa;
+}
+
+REB_API void reb_simulationarchive_free(struct reb_simulationarchive* sa) {
+    if (!sa) return;
+    if (sa->inf) fclose(sa->inf);
+    free(sa->filename);
+    free(sa->t);
+    free(sa->offset);
+    free(sa);
+}
+
+// simulationarchive.c:45-84: r is a freshly created simulation.
+REB_API void reb_simulation_init_from_simulationarchive_with_messages(struct reb_simulation* r,
+                                                                     struct reb_simulationarchive* sa,
+                                                                     int64_t snapshot, int* warnings) {
+    int codes = 0;
+    if (!sa || !sa->inf) {
+        codes |= RC_FILENOTOPEN;
+    } else {
+        if (snapshot < 0) snapshot += sa->nblobs;
+        if (snapshot < 0 || snapshot >= sa->nblobs) codes |= RC_OUTOFRANGE;
+    }
+    if (codes) {
+        if (warnings) *warnings |= codes;
+        return;
+    }
+    std::string file;
+    if (!stream_bytes(sa->inf, &file)) {
+        if (warnings) *warnings |= RC_SEEK;
+        return;
+    }
+    free(r->simulationarchive_filename);
+    r->simulationarchive_filename = nullptr;
+    const size_t end0 = sa->nblobs > 1 ? (size_t)sa->offset[1] : file.size();
+    unsigned w = 0;
+    input_fields(r, file.substr(0, end0), 0, &w);
+    if (snapshot > 0) {
+        const size_t a = (size_t)sa->offset[snapshot];
+        const size_t b = snapshot + 1 < sa->nblobs ? (size_t)sa->offset[snapshot + 1] : file.size();
+        if (a > file.size() || b > file.size() || b < a) {
+            if (warnings) *warnings |= RC_SEEK;
+            return;
+        }
+        input_fields(r, file.substr(a, b - a), 0, &w);
+    }
+    finish_input(r, &w);
+    if (warnings) *warnings |= ref_codes(w);
+}
+
+// simulationarchive.c:86-93
+REB_API struct reb_simulation* reb_simulation_create_from_simulationarchive(struct reb_simulationarchive* sa,
+                                                                           int64_t snapshot) {
+    if (!sa) return nullptr;
+    struct reb_simulation* r = reb_simulation_create();
+    int codes = 0;
+    reb_simulation_init_from_simulationarchive_with_messages(r, sa, snapshot, &codes);
+    return process_codes(r, codes);
 }
 
 }  // extern "C"

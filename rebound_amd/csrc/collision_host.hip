@@ -26,7 +26,8 @@
 // r->collisions (pinned), a copy that runs while the host resolves.  The
 // resolver works on the body records in place; they travel back for a
 // scatter of the velocities.
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 #include <atomic>
@@ -40,6 +41,12 @@
 #include "internal.h"
 
 namespace rbamd {
+
+// Exclusive prefix sum of n ints with rocPRIM's device scan; with tmp ==
+// nullptr it only reports the temporary storage in `bytes`.
+static hipError_t exclusive_sum(void* tmp, size_t& bytes, const int* in, int* out, size_t n, hipStream_t st) {
+    return rocprim::exclusive_scan(tmp, bytes, in, out, 0, n, rocprim::plus<int>(), st);
+}
 
 // A small persistent pool of host threads for the level-parallel resolve.
 class HostPool {
@@ -474,8 +481,8 @@ static int rand_r_step(unsigned int* seed) {
 // largest step so far).  `ready` is set when the device has prepared the
 // compact records of the last search.
 struct rbamd::ResolveScratch {
-    DevBuf flag, cidx, ids, body, pair, perm, coll, cub;   // device
-    DevBuf bcnt, boff, bcur, blist, prev, level_d, lmisc, lcub;
+    DevBuf flag, cidx, ids, body, pair, perm, coll, prim_tmp;   // device
+    DevBuf bcnt, boff, bcur, blist, prev, level_d, lmisc, lprim_tmp;
     HSBody* body_h = nullptr;                              // pinned host
     HSPair* pair_h = nullptr;
     int* perm_h = nullptr;    // the permutation, then the level order
@@ -496,8 +503,8 @@ struct rbamd::ResolveScratch {
     std::vector<double> plog;
     std::vector<unsigned char> hit;
     void release() {
-        for (DevBuf* b : {&flag, &cidx, &ids, &body, &pair, &perm, &coll, &cub, &bcnt, &boff, &bcur, &blist, &prev,
-                          &level_d, &lmisc, &lcub})
+        for (DevBuf* b : {&flag, &cidx, &ids, &body, &pair, &perm, &coll, &prim_tmp, &bcnt, &boff, &bcur, &blist,
+                          &prev, &level_d, &lmisc, &lprim_tmp})
             b->release();
         for (void* p : {(void*)body_h, (void*)pair_h, (void*)perm_h, (void*)n_h, (void*)lev_h})
             if (p) (void)hipHostFree(p);
@@ -565,8 +572,9 @@ bool compact_resolve_mark(struct reb_simulation* r, const CollRecH* rec, size_t 
     W.ready = false;
     const int N = (int)r->N;
     size_t tmp = 0;
-    hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, W.flag.as<int>(), W.cidx.as<int>(), N + 1, d->stream);
-    if (!W.flag.ensure(sizeof(int) * (N + 1)) || !W.cidx.ensure(sizeof(int) * (N + 1)) || !W.cub.ensure(tmp + 256)) {
+    exclusive_sum(nullptr, tmp, W.flag.as<int>(), W.cidx.as<int>(), N + 1, d->stream);
+    if (!W.flag.ensure(sizeof(int) * (N + 1)) || !W.cidx.ensure(sizeof(int) * (N + 1)) ||
+        !W.prim_tmp.ensure(tmp + 256)) {
         reb_simulation_error(r, "Out of device memory for the collision resolve.");
         return false;
     }
@@ -574,7 +582,7 @@ bool compact_resolve_mark(struct reb_simulation* r, const CollRecH* rec, size_t 
     if (!RB_HIP(r, hipMemsetAsync(W.flag.p, 0, sizeof(int) * (N + 1), d->stream))) return false;
     hipLaunchKernelGGL(k_coll_mark, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, d->stream, rec, (int)nc,
                        W.flag.as<int>());
-    hipcub::DeviceScan::ExclusiveSum(W.cub.p, tmp, W.flag.as<int>(), W.cidx.as<int>(), N + 1, d->stream);
+    exclusive_sum(W.prim_tmp.p, tmp, W.flag.as<int>(), W.cidx.as<int>(), N + 1, d->stream);
     return RB_HIP(r, hipMemcpyAsync(W.n_h, W.cidx.as<int>() + N, sizeof(int), hipMemcpyDeviceToHost, d->stream));
 }
 
@@ -649,8 +657,8 @@ static int device_levels(struct reb_simulation* r, ResolveScratch& W, int nc, in
     int* hist = start + kLevBins;
     int* offs = hist + (size_t)kLevBins * kLevBlocks;
     size_t t_scan = 0;
-    hipcub::DeviceScan::ExclusiveSum(nullptr, t_scan, W.bcnt.as<int>(), W.boff.as<int>(), n + 1, d->stream);
-    if (!W.lcub.ensure(t_scan + 256)) {
+    exclusive_sum(nullptr, t_scan, W.bcnt.as<int>(), W.boff.as<int>(), n + 1, d->stream);
+    if (!W.lprim_tmp.ensure(t_scan + 256)) {
         reb_simulation_error(r, "Out of device memory for the collision resolve.");
         return -1;
     }
@@ -661,7 +669,7 @@ static int device_levels(struct reb_simulation* r, ResolveScratch& W, int nc, in
         !RB_HIP(r, hipMemsetAsync(W.bcur.p, 0, sizeof(int) * n, d->stream)))
         return -1;
     hipLaunchKernelGGL(k_lev_body_count, gk, blk, 0, d->stream, cl, nc, W.bcnt.as<int>());
-    hipcub::DeviceScan::ExclusiveSum(W.lcub.p, t_scan, W.bcnt.as<int>(), W.boff.as<int>(), n + 1, d->stream);
+    exclusive_sum(W.lprim_tmp.p, t_scan, W.bcnt.as<int>(), W.boff.as<int>(), n + 1, d->stream);
     hipLaunchKernelGGL(k_lev_body_fill, gk, blk, 0, d->stream, cl, nc, W.boff.as<int>(), W.bcur.as<int>(),
                        W.blist.as<int>());
     hipLaunchKernelGGL(k_lev_links, gm, blk, 0, d->stream, cl, nc, W.boff.as<int>(), W.blist.as<int>(),

@@ -12,7 +12,9 @@
 //          with Kahan compensation, and the source range split over
 //          blockIdx.y so that N=65536 fills every SIMD; the partial sums are
 //          added in a fixed order by a second kernel (deterministic).
-//   SPLIT  displacement and r^2 in FP64, r^-3 * m_j in FP32 (v_rsq_f32),
+//   SPLIT  displacement and r^2 in FP64, r^-1 from the FP32 reciprocal
+//          square root (v_rsq_f32, about 2^-23 relative), cubed and
+//          multiplied by m_j in FP64 (no FP32 exponent limit on r^-3 or m),
 //          accumulation in FP64; each LDS tile's FP64 partial is folded into
 //          the running sum with Kahan compensation.
 //
@@ -403,7 +405,7 @@ __device__ __forceinline__ double vreg_const(double v) {
 }
 
 template <int MODE, bool CHECK>
-__device__ __forceinline__ void pair_tile(const double4* __restrict__ tile, const float* __restrict__ mf, int n,
+__device__ __forceinline__ void pair_tile(const double4* __restrict__ tile, int n,
                                           int jbase, int i, int ignore, double gx, double gy, double gz,
                                           double soft2, double& sx, double& sy, double& sz, double& cx, double& cy,
                                           double& cz, double c15, double c1875) {
@@ -416,9 +418,11 @@ __device__ __forceinline__ void pair_tile(const double4* __restrict__ tile, cons
         const double r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, soft2)));
         double w;
         if (MODE == FM_SPLIT) {
-            const float rf = (float)r2;
-            const float yf = __builtin_amdgcn_rsqf(rf);
-            w = (double)(yf * yf * yf * mf[k]);
+            // r^-1 from the FP32 reciprocal square root; its cube and the
+            // mass factor in FP64, so only r^2 itself must lie in FP32's
+            // normal range (about 1e-19 < r < 1.8e19 in simulation units)
+            const double yv = (double)__builtin_amdgcn_rsqf((float)r2);
+            w = yv * yv * yv * s.w;
         } else {
             w = s.w * rinv3_fp64(r2, c15, c1875);
         }
@@ -458,7 +462,7 @@ __device__ __forceinline__ void direct_fast_block(const double* __restrict__ x, 
                                                   const double* __restrict__ z, const double* __restrict__ m,
                                                   int n_recv, int j_begin, int j_end, int chunk, int ignore,
                                                   double soft2, const GBS& gbs, double* __restrict__ partial,
-                                                  int pstride, int bx, int by, double4* tile, float* mtile) {
+                                                  int pstride, int bx, int by, double4* tile) {
     const int i = bx * kFast + threadIdx.x;
     const bool live = i < n_recv;
     const double xi = live ? x[i] : 0., yi = live ? y[i] : 0., zi = live ? z[i] : 0.;
@@ -477,9 +481,7 @@ __device__ __forceinline__ void direct_fast_block(const double* __restrict__ x, 
             __syncthreads();
             const int jj = t + (int)threadIdx.x;
             if (jj < j1) {
-                const double mj = m[jj];
-                tile[threadIdx.x] = make_double4(x[jj], y[jj], z[jj], mj);
-                mtile[threadIdx.x] = (float)mj;
+                tile[threadIdx.x] = make_double4(x[jj], y[jj], z[jj], m[jj]);
             }
             __syncthreads();
             const int n = min(kFast, j1 - t);
@@ -491,20 +493,20 @@ __device__ __forceinline__ void direct_fast_block(const double* __restrict__ x, 
                 // plain FP64 sum over the tile, folded into the running sum with Kahan
                 double tx = 0., ty = 0., tz = 0., d0 = 0., d1 = 0., d2 = 0.;
                 if (check)
-                    pair_tile<MODE, true>(tile, mtile, n, t, i, ignore, gx, gy, gz, soft2, tx, ty, tz, d0, d1, d2,
+                    pair_tile<MODE, true>(tile, n, t, i, ignore, gx, gy, gz, soft2, tx, ty, tz, d0, d1, d2,
                                           c15, c1875);
                 else
-                    pair_tile<MODE, false>(tile, mtile, n, t, i, ignore, gx, gy, gz, soft2, tx, ty, tz, d0, d1, d2,
+                    pair_tile<MODE, false>(tile, n, t, i, ignore, gx, gy, gz, soft2, tx, ty, tz, d0, d1, d2,
                                            c15, c1875);
                 kahan_add(tx, sx, cx);
                 kahan_add(ty, sy, cy);
                 kahan_add(tz, sz, cz);
             } else {
                 if (check)
-                    pair_tile<MODE, true>(tile, mtile, n, t, i, ignore, gx, gy, gz, soft2, sx, sy, sz, cx, cy, cz,
+                    pair_tile<MODE, true>(tile, n, t, i, ignore, gx, gy, gz, soft2, sx, sy, sz, cx, cy, cz,
                                           c15, c1875);
                 else
-                    pair_tile<MODE, false>(tile, mtile, n, t, i, ignore, gx, gy, gz, soft2, sx, sy, sz, cx, cy, cz,
+                    pair_tile<MODE, false>(tile, n, t, i, ignore, gx, gy, gz, soft2, sx, sy, sz, cx, cy, cz,
                                            c15, c1875);
             }
         }
@@ -529,9 +531,8 @@ __global__ __launch_bounds__(kFast) __attribute__((amdgpu_waves_per_eu(RB_FAST_W
     const double* __restrict__ m, int n_recv, int j_begin, int j_end, int chunk, int ignore, double soft2,
     GBList gl, double* __restrict__ partial, int pstride) {
     __shared__ double4 tile[kFast];
-    __shared__ float mtile[kFast];
     direct_fast_block<MODE>(x, y, z, m, n_recv, j_begin, j_end, chunk, ignore, soft2, GBFromList{gl}, partial,
-                            pstride, (int)blockIdx.x, (int)blockIdx.y, tile, mtile);
+                            pstride, (int)blockIdx.x, (int)blockIdx.y, tile);
 }
 
 // Sum the chunk partials in chunk order (Kahan), apply -G and write (or add to)
@@ -644,21 +645,20 @@ __global__ __launch_bounds__(256) void k_seg_drift(const SegJob* __restrict__ jo
 __global__ __launch_bounds__(kFast) __attribute__((amdgpu_waves_per_eu(RB_FAST_WAVES, RB_FAST_WAVES))) void k_seg_direct(
     const SegJob* __restrict__ jobs, const SegTile* __restrict__ tiles) {
     __shared__ double4 tile[kFast];
-    __shared__ float mtile[kFast];
     const SegTile T = tiles[blockIdx.x];
     const SegJob J = jobs[T.job];
     switch (J.mode) {
         case FM_FP64:
             direct_fast_block<FM_FP64>(J.x, J.y, J.z, J.m, J.n, 0, J.n, J.chunk, 0, J.soft2, GBZero{}, J.partial, J.n,
-                                       T.bx, T.by, tile, mtile);
+                                       T.bx, T.by, tile);
             break;
         case FM_SPLIT:
             direct_fast_block<FM_SPLIT>(J.x, J.y, J.z, J.m, J.n, 0, J.n, J.chunk, 0, J.soft2, GBZero{}, J.partial,
-                                        J.n, T.bx, T.by, tile, mtile);
+                                        J.n, T.bx, T.by, tile);
             break;
         default:
             direct_fast_block<FM_COMP>(J.x, J.y, J.z, J.m, J.n, 0, J.n, J.chunk, 0, J.soft2, GBZero{}, J.partial, J.n,
-                                       T.bx, T.by, tile, mtile);
+                                       T.bx, T.by, tile);
             break;
     }
 }
@@ -1102,6 +1102,13 @@ double energy_device(struct reb_simulation* r) {
     // EXACT: the reference's sequential sum, bit-identical, up to kEnergyExactMax
     // bodies (a serial chain of N^2/2 additions); above, the compensated sum.
     const bool exact = d->precision == REB_AMD_PRECISION_EXACT && N_interact <= kEnergyExactMax;
+    if (d->precision == REB_AMD_PRECISION_EXACT && !exact && !d->warned_energy_exact) {
+        d->warned_energy_exact = true;
+        reb_simulation_warning(r,
+                               "EXACT precision: above 16384 interacting bodies the energy is summed with Kahan "
+                               "compensation in parallel, not in the reference's serial order, so its last bits "
+                               "can differ from the reference's.");
+    }
     const size_t terms_cap = exact ? std::min<size_t>(kEnergyTermsMax, N_active * N_interact / 2 + N_interact) : 0;
     const size_t head = 2 * N + 16;
     if (!d->scratch.ensure(sizeof(double) * (head + terms_cap))) {

@@ -71,6 +71,9 @@ REB_API void reb_exit(const char* const msg) {
 
 REB_API void reb_simulation_stop(struct reb_simulation* const r) { r->status = REB_STATUS_USER; }
 
+// rebound.c:215-217: memory the library handed out (strings, streams).
+REB_API void reb_free(void* p) { free(p); }
+
 // ===================================================================== registry
 // Names and behaviour follow rebound.c:49-113 and simulation.c:201-238.  The
 // built-in names that are not part of this engine stay selectable so existing

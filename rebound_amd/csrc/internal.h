@@ -124,6 +124,8 @@ struct DeviceState {
     // device-side additional forces (reb_amd_set_device_forces)
     reb_amd_device_forces_fn dev_forces = nullptr;
     void* dev_forces_user = nullptr;
+    // the EXACT energy fell back to the compensated sum once (warned)
+    bool warned_energy_exact = false;
     double* field(int f) const { return soa.as<double>() + (size_t)f * cap; }
 };
 

@@ -27,7 +27,8 @@
 // the cell (or has accepted an ancestor) skips c's subtree; the wave descends
 // into c if any lane opens it.  Each lane therefore sees its own accepted
 // cells in the reference's depth-first order and sums them in that order.
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -36,6 +37,12 @@
 #include "internal.h"
 
 namespace rbamd {
+
+// Exclusive prefix sum of n ints with rocPRIM's device scan; with tmp ==
+// nullptr it only reports the temporary storage in `bytes`.
+static hipError_t exclusive_sum(void* tmp, size_t& bytes, const int* in, int* out, size_t n, hipStream_t st) {
+    return rocprim::exclusive_scan(tmp, bytes, in, out, 0, n, rocprim::plus<int>(), st);
+}
 
 static constexpr int kMaxGBT = 64;   // keeps the by-value argument under 4 KB
 // `info` block: ints 0..7 (build results and flags), bytes 32..39 the
@@ -53,7 +60,7 @@ struct TreeBuffers {
     size_t last_total = 0;                        // collisions found by the last tree search
     DevBuf key0, key1, idx, key0b, key1b, idxb;   // keys and body permutation
     DevBuf lcp, cnt, base;                        // per body
-    DevBuf cub;                                   // hipCUB temporary storage
+    DevBuf prim_tmp;                              // rocPRIM temporary storage
     DevBuf node_d;                                // 8 doubles per cell: cx cy cz w m mx my mz
     DevBuf node_i;                                // 4 ints per cell: next pt level start
     DevBuf info;                                  // total, maxlevel, flags
@@ -421,13 +428,13 @@ static bool tree_sort(struct reb_simulation* r, TreeBuffers* T, int n, bool two)
     int* Ib = T->idxb.as<int>();
     const int keybits = 3 * T->L1 + T->rbits;
     size_t tmp = 0;
-    hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, K0, K0b, I, Ib, n, 0, keybits, d->stream);
+    rocprim::radix_sort_pairs(nullptr, tmp, K0, K0b, I, Ib, n, 0, keybits, d->stream);
     size_t tmp2 = 0;
-    hipcub::DeviceRadixSort::SortPairs(nullptr, tmp2, K1, K1b, I, Ib, n, 0, 3 * T->L2, d->stream);
+    rocprim::radix_sort_pairs(nullptr, tmp2, K1, K1b, I, Ib, n, 0, 3 * T->L2, d->stream);
     tmp = std::max(tmp, tmp2);
-    if (!T->cub.ensure(tmp + 256)) return false;
+    if (!T->prim_tmp.ensure(tmp + 256)) return false;
     if (!two) {
-        if (hipcub::DeviceRadixSort::SortPairs(T->cub.p, tmp, K0, K0b, I, Ib, n, 0, keybits, d->stream) !=
+        if (rocprim::radix_sort_pairs(T->prim_tmp.p, tmp, K0, K0b, I, Ib, n, 0, keybits, d->stream) !=
             hipSuccess)
             return RB_HIP(r, hipGetLastError());
         std::swap(T->key0, T->key0b);
@@ -435,11 +442,11 @@ static bool tree_sort(struct reb_simulation* r, TreeBuffers* T, int n, bool two)
         return true;
     }
     // least significant word first, then a stable sort on the most significant
-    if (hipcub::DeviceRadixSort::SortPairs(T->cub.p, tmp, K1, K1b, I, Ib, n, 0, 3 * T->L2, d->stream) != hipSuccess)
+    if (rocprim::radix_sort_pairs(T->prim_tmp.p, tmp, K1, K1b, I, Ib, n, 0, 3 * T->L2, d->stream) != hipSuccess)
         return RB_HIP(r, hipGetLastError());
     hipLaunchKernelGGL(k_gather_u64, dim3((n + 255) / 256), dim3(256), 0, d->stream, K0, Ib, K0b, n);
     // K0b: key0 in key1 order; Ib: permutation; sort (K0b, Ib) -> (K0, I)
-    if (hipcub::DeviceRadixSort::SortPairs(T->cub.p, tmp, K0b, K0, Ib, I, n, 0, keybits, d->stream) != hipSuccess)
+    if (rocprim::radix_sort_pairs(T->prim_tmp.p, tmp, K0b, K0, Ib, I, n, 0, keybits, d->stream) != hipSuccess)
         return RB_HIP(r, hipGetLastError());
     hipLaunchKernelGGL(k_gather_u64, dim3((n + 255) / 256), dim3(256), 0, d->stream, K1, I, K1b, n);
     std::swap(T->key1, T->key1b);   // key1 now in final order
@@ -479,9 +486,9 @@ bool tree_build(struct reb_simulation* r) {
         hipLaunchKernelGGL(k_tree_count, dim3((n + 255) / 256), dim3(256), 0, d->stream, T->lcp.as<int>(), n,
                            T->cnt.as<int>(), info);
         size_t tmp = 0;
-        hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, T->cnt.as<int>(), T->base.as<int>(), n, d->stream);
-        if (!T->cub.ensure(tmp + 256)) return false;
-        hipcub::DeviceScan::ExclusiveSum(T->cub.p, tmp, T->cnt.as<int>(), T->base.as<int>(), n, d->stream);
+        exclusive_sum(nullptr, tmp, T->cnt.as<int>(), T->base.as<int>(), n, d->stream);
+        if (!T->prim_tmp.ensure(tmp + 256)) return false;
+        exclusive_sum(T->prim_tmp.p, tmp, T->cnt.as<int>(), T->base.as<int>(), n, d->stream);
         if (!RB_HIP(r, hipMemcpyAsync(&last[0], T->base.as<int>() + (n - 1), 4, hipMemcpyDeviceToHost, d->stream)) ||
             !RB_HIP(r, hipMemcpyAsync(&last[1], T->cnt.as<int>() + (n - 1), 4, hipMemcpyDeviceToHost, d->stream)) ||
             !RB_HIP(r, hipMemcpyAsync(hinfo, info, 16, hipMemcpyDeviceToHost, d->stream)))
@@ -949,13 +956,11 @@ static bool ensure_coll(struct reb_simulation* r, TreeBuffers* T, size_t nslots,
 static long scan_counts(struct reb_simulation* r, TreeBuffers* T, int nslots) {
     DeviceState* d = dev_of(r);
     size_t tmp = 0;
-    hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, T->coll_cnt.as<int>(), T->coll_off.as<int>(), nslots + 1,
-                                     d->stream);
-    if (!T->cub.ensure(tmp + 256)) return -1;
+    exclusive_sum(nullptr, tmp, T->coll_cnt.as<int>(), T->coll_off.as<int>(), nslots + 1, d->stream);
+    if (!T->prim_tmp.ensure(tmp + 256)) return -1;
     // counts[nslots] = 0 so that offsets[nslots] is the total
     if (!RB_HIP(r, hipMemsetAsync(T->coll_cnt.as<int>() + nslots, 0, 4, d->stream))) return -1;
-    hipcub::DeviceScan::ExclusiveSum(T->cub.p, tmp, T->coll_cnt.as<int>(), T->coll_off.as<int>(), nslots + 1,
-                                     d->stream);
+    exclusive_sum(T->prim_tmp.p, tmp, T->coll_cnt.as<int>(), T->coll_off.as<int>(), nslots + 1, d->stream);
     int total = 0;
     if (!RB_HIP(r, hipMemcpyAsync(&total, T->coll_off.as<int>() + nslots, 4, hipMemcpyDeviceToHost, d->stream)))
         return -1;
@@ -1187,8 +1192,8 @@ void tree_release(struct reb_simulation* r) {
     if (!d->tree) return;
     TreeBuffers* T = d->tree;
     for (DevBuf* b : {&T->key0, &T->key1, &T->idx, &T->key0b, &T->key1b, &T->idxb, &T->lcp, &T->cnt, &T->base,
-                      &T->cub, &T->node_d, &T->node_i, &T->info, &T->coll_cnt, &T->coll_off, &T->coll_rec, &T->coll_slot,
-                      &T->grec, &T->crec})
+                      &T->prim_tmp, &T->node_d, &T->node_i, &T->info, &T->coll_cnt, &T->coll_off, &T->coll_rec,
+                      &T->coll_slot, &T->grec, &T->crec})
         b->release();
     delete T;
     d->tree = nullptr;

@@ -81,9 +81,59 @@ __device__ __forceinline__ void stiefel_Gs3(double* Gs, double beta, double X) {
     Gs[3] *= X2 * X;
 }
 
+// One quartic (Laguerre-Conway) update of X (integrator_whfast.c:292-300).
+// Gs receives the Stiefel functions of the X the update started from.
+__device__ __forceinline__ double quartic_next(double X, double beta, double r0, double eta0, double zeta0,
+                                               double dt, double* Gs) {
+    stiefel_Gs3(Gs, beta, X);
+    const double f = r0 * X + eta0 * Gs[2] + zeta0 * Gs[3] - dt;
+    const double fp = r0 + eta0 * Gs[1] + zeta0 * Gs[2];
+    const double fpp = eta0 * Gs[0] + zeta0 * Gs[1];
+    const double denom = fp + sqrt(fabs(16. * fp * fp - 20. * f * fpp));
+    return (X * denom - 5. * f) / denom;
+}
+
+// The quartic iterates X_1, X_2, ... (X_k the update of X_{k-1}) are a
+// deterministic sequence, so the first k with X_k equal to an earlier iterate
+// is mu + lambda (tail length plus cycle length), the earlier one X_mu.  This
+// finds both with Brent's cycle detection in O(1) memory and returns mu +
+// lambda when it is at most 63 (the reference's last quartic iteration), else
+// 0.  With tortoise positions 1, 2, 4, .., 64 and mu, lambda <= 62 the cycle
+// is found before the hare passes iterate 126, hence the bound of 128.
+__device__ __noinline__ int quartic_first_repeat(double X0, double beta, double r0, double eta0, double zeta0,
+                                                 double dt) {
+    double Gs[4];
+    const double a1 = quartic_next(X0, beta, r0, eta0, zeta0, dt, Gs);
+    double tort = a1;
+    double hare = quartic_next(a1, beta, r0, eta0, zeta0, dt, Gs);
+    int power = 1, lam = 1, hidx = 2;
+    while (!(tort == hare)) {
+        if (hidx >= 128) return 0;
+        if (power == lam) {
+            tort = hare;
+            power *= 2;
+            lam = 0;
+        }
+        hare = quartic_next(hare, beta, r0, eta0, zeta0, dt, Gs);
+        hidx++;
+        lam++;
+    }
+    tort = a1;
+    hare = a1;
+    for (int k = 0; k < lam; k++) hare = quartic_next(hare, beta, r0, eta0, zeta0, dt, Gs);
+    int mu = 0;
+    while (!(tort == hare)) {
+        tort = quartic_next(tort, beta, r0, eta0, zeta0, dt, Gs);
+        hare = quartic_next(hare, beta, r0, eta0, zeta0, dt, Gs);
+        if (1 + ++mu + lam > 63) return 0;
+    }
+    return 1 + mu + lam;
+}
+
 // Universal-variable Kepler drift of one body (integrator_whfast.c:241-390),
 // in place.  Returns a bit mask: 1 = the step exceeded one orbital period
-// (the reference warns once).
+// (the reference warns once), 4 = the quartic iteration ran, 8 = the
+// bisection fallback ran.
 __device__ int kepler_solver(double& px, double& py, double& pz, double& pvx, double& pvy, double& pvz, double mu,
                              double dt) {
     const double x0 = px, y0 = py, z0 = pz, vx0 = pvx, vy0 = pvy, vz0 = pvz;
@@ -116,24 +166,35 @@ __device__ int kepler_solver(double& px, double& py, double& pz, double& pvx, do
     X = ri * (X * e0 - eta0 * Gs[2] - zeta0 * Gs[3] + dt);
 
     if (fastabs(X - oldX) > 0.01 * X_per_period) {
-        // Quartic (Laguerre-Conway) iteration with cycle detection (:289-312).
+        // Quartic (Laguerre-Conway) iteration with cycle detection (:289-312):
+        // converged at the first iterate equal to any earlier one.  The last
+        // four iterates are kept in registers (NaN: empty; it equals
+        // nothing), which finds that first repeat exactly when the cycle is
+        // at most four long (the iterates are a deterministic sequence, so
+        // every repeat spans a multiple of the cycle length); a longer cycle
+        // is located by quartic_first_repeat and replayed.  No history array.
+        flags |= 4;
         X = beta * dt / mu;
-        double prevX[64 + 1];
+        const double Xq0 = X;
+        double w0 = __builtin_nan(""), w1 = w0, w2 = w0, w3 = w0;
         for (int n_lag = 1; n_lag < 64; n_lag++) {
-            stiefel_Gs3(Gs, beta, X);
-            const double f = r0 * X + eta0 * Gs[2] + zeta0 * Gs[3] - dt;
-            const double fp = r0 + eta0 * Gs[1] + zeta0 * Gs[2];
-            const double fpp = eta0 * Gs[0] + zeta0 * Gs[1];
-            const double denom = fp + sqrt(fabs(16. * fp * fp - 20. * f * fpp));
-            X = (X * denom - 5. * f) / denom;
-            for (int i = 1; i < n_lag; i++) {
-                if (X == prevX[i]) {
-                    n_lag = 64;
-                    converged = 1;
-                    break;
-                }
+            X = quartic_next(X, beta, r0, eta0, zeta0, dt, Gs);
+            if (X == w0 || X == w1 || X == w2 || X == w3) {
+                converged = 1;
+                break;
             }
-            prevX[n_lag] = X;
+            w3 = w2;
+            w2 = w1;
+            w1 = w0;
+            w0 = X;
+        }
+        if (!converged) {
+            const int n = quartic_first_repeat(Xq0, beta, r0, eta0, zeta0, dt);
+            if (n > 0) {
+                X = Xq0;
+                for (int k = 0; k < n; k++) X = quartic_next(X, beta, r0, eta0, zeta0, dt, Gs);
+                converged = 1;
+            }
         }
         const double e1 = eta0 * Gs[1] + zeta0 * Gs[2];
         ri = 1. / (r0 + e1);
@@ -155,6 +216,7 @@ __device__ int kepler_solver(double& px, double& py, double& pz, double& pvx, do
     }
     if (converged == 0) {
         // Bisection fallback (:333-369).
+        flags |= 8;
         double X_min, X_max;
         if (beta > 0.) {
             X_min = X_per_period * floor(dt * invperiod);
@@ -316,7 +378,7 @@ __device__ __forceinline__ const CorrZ* corrector_chain(int order, int& n) {
 // lane of a group evaluates over the published bodies in the reference's order,
 // and the symplectic correctors run at the synchronisation points.
 template <int COORD>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_whfast(WHParams P) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_whfast(WHParams P) {
     constexpr bool JAC = COORD == WH_JACOBI;
     __shared__ Slot slot[64];
     __shared__ double aux[64][3];    // per-planet sums of the serial Jacobi gravity / jerk
@@ -375,7 +437,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
         if (JAC && live) hm = P.ph[6 * st + b0 + me];
         const unsigned int flags0 = sys_ok ? P.flags[sidx] : 1u;
         int sync = flags0 & 1u;
-        int warn = (flags0 >> 1) & 3u;        // warnings stay set until the host clears them
+        int warn = (flags0 >> 1) & 15u;       // warnings and branch bits stay set until the host clears them
         // Steps of this system in this launch (a system may stop before the
         // launch's last step; its lanes keep taking part in the barriers).
         const long my_steps = !sys_ok ? 0 : (S.steps >= 0 && S.steps < P.nsteps ? (long)S.steps : P.nsteps);
@@ -1019,6 +1081,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
             if (act) modified = false;
         }
         synchronize(sys_ok && (P.sync_at_end || (S.req & 1)));
+        // the system's flags: any of its lanes' warnings and solver branches
+        for (int o = 1; o < G; o <<= 1) warn |= __shfl_xor(warn, o);
 
         // ---- store
         if (live) {
@@ -1038,7 +1102,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
             P.ph[0 * st + b0] = cx; P.ph[1 * st + b0] = cy; P.ph[2 * st + b0] = cz;
             P.ph[3 * st + b0] = cvx; P.ph[4 * st + b0] = cvy; P.ph[5 * st + b0] = cvz;
             P.ph[6 * st + b0] = cm;
-            P.flags[sidx] = (unsigned)(sync & 1) | (unsigned)((warn & 3) << 1);
+            P.flags[sidx] = (unsigned)(sync & 1) | (unsigned)((warn & 15) << 1);
         }
         __syncthreads();
     }

@@ -1135,8 +1135,16 @@ static void gen_integrate(void){
         reb_simulation_free(r);
     }
 
-    /* the merge resolver on a stepping simulation, DIRECT and TREE searches,
-     * periodic box with one ring of ghost boxes, energy offset tracked */
+}
+
+/* The merge resolver on a stepping simulation, DIRECT and TREE searches,
+ * periodic box with one ring of ghost boxes, energy offset tracked.  Written
+ * by the OpenMP build: with ghost boxes its per-target force sums are the
+ * order the device follows (the serial build's half-matrix loop adds the
+ * ghost-box terms in another order). */
+static void gen_integrate_merge(void){
+    struct reb_simulation* r;
+    int st;
     for (int v=0; v<2; v++){
         r = reb_simulation_create();
         r->rand_seed = 41u;
@@ -1245,6 +1253,7 @@ int main(int argc, char** argv){
         gen_plummer_omp();
         gen_ghostbox_basic("omp");
         gen_boundaries();
+        gen_integrate_merge();
         return 0;
     }
     gen_rng();

@@ -204,3 +204,152 @@ def test_damaged_file_read_safely(amd, tmp_path, damage):
         assert sim.s.N == 0 or bool(sim.s.particles)
         sim.free()
     assert ctypes.sizeof(amd._abi.Simulation) > 0
+
+
+class _Archive:
+    """The archive index in the reference's layout (simulationarchive.h:30-43),
+    as its Python package declares it."""
+
+    @staticmethod
+    def cls():
+        import ctypes as c
+
+        class SA(c.Structure):
+            _fields_ = [("inf", c.c_void_p), ("filename", c.c_char_p), ("version", c.c_int),
+                        ("major", c.c_int), ("minor", c.c_int), ("patch", c.c_int),
+                        ("auto_interval", c.c_double), ("auto_walltime", c.c_double),
+                        ("auto_step", c.c_uint64), ("nblobs", c.c_int64),
+                        ("offset", c.POINTER(c.c_uint64)), ("t", c.POINTER(c.c_double))]
+        return SA
+
+
+def _archive_lib(amd):
+    import ctypes as c
+    L = amd.lib()
+    SA = _Archive.cls()
+    L.reb_simulationarchive_create_from_file.restype = c.POINTER(SA)
+    L.reb_simulationarchive_create_from_file.argtypes = [c.c_char_p]
+    L.reb_simulationarchive_create_from_file_with_messages.restype = c.POINTER(SA)
+    L.reb_simulationarchive_create_from_file_with_messages.argtypes = [c.c_char_p, c.POINTER(c.c_int)]
+    L.reb_simulationarchive_create_from_buffer_with_messages.restype = c.POINTER(SA)
+    L.reb_simulationarchive_create_from_buffer_with_messages.argtypes = [c.c_char_p, c.c_size_t,
+                                                                          c.POINTER(c.c_int)]
+    L.reb_simulationarchive_free.argtypes = [c.POINTER(SA)]
+    L.reb_simulation_create_from_simulationarchive.restype = c.c_void_p
+    L.reb_simulation_create_from_simulationarchive.argtypes = [c.POINTER(SA), c.c_int64]
+    L.reb_simulation_init_from_simulationarchive_with_messages.argtypes = [c.c_void_p, c.POINTER(SA), c.c_int64,
+                                                                            c.POINTER(c.c_int)]
+    return L, SA
+
+
+def test_archive_object_index_and_snapshots(amd):
+    """reb_simulationarchive_create_from_file on the reference's archive (five
+    snapshots, one every 7 steps): the index holds the reference's version,
+    the save interval, each snapshot's offset and time, and every snapshot read
+    through the object equals the one read by reb_simulation_create_from_file."""
+    import ctypes as c
+    L, SA = _archive_lib(amd)
+    path = os.path.join(GOLDEN, "archive_whfast.bin")
+    sa = L.reb_simulationarchive_create_from_file(path.encode())
+    assert sa
+    a = sa.contents
+    assert a.nblobs == 5 and a.version >= 5 and a.auto_step == 7
+    assert (a.major, a.minor, a.patch) == (5, 0, 0)
+    assert a.filename == path.encode()
+    offs = [a.offset[k] for k in range(5)]
+    assert offs[0] == 0 and offs == sorted(offs)
+    for k in range(5):
+        direct = amd.Sim.from_file(path, k)
+        assert a.t[k] == direct.s.t
+        ptr = L.reb_simulation_create_from_simulationarchive(sa, k)
+        assert ptr
+        via = amd.Sim.wrap(ptr)
+        assert via.s.steps_done == direct.s.steps_done and via.s.N == direct.s.N
+        assert np.array_equal(via.state(), golden("archive_whfast_snap%d_state.npy" % k))
+        assert amd.lib().reb_simulation_diff(via.ptr, direct.ptr, 0) == 0
+        via.free()
+        direct.free()
+    # negative snapshots count from the end; past the end is an error (NULL)
+    last = amd.Sim.wrap(L.reb_simulation_create_from_simulationarchive(sa, -1))
+    assert np.array_equal(last.state(), golden("archive_whfast_snap4_state.npy"))
+    last.free()
+    assert not L.reb_simulation_create_from_simulationarchive(sa, 5)
+    L.reb_simulationarchive_free(sa)
+
+
+def test_archive_object_with_messages_and_buffer(amd, tmp_path):
+    """The entry points the reference's Python package calls: codes instead of
+    messages (binarydata.h:31-45), from a file and from memory."""
+    import ctypes as c
+    L, SA = _archive_lib(amd)
+    w = c.c_int(0)
+    sa = L.reb_simulationarchive_create_from_file_with_messages(str(tmp_path / "none.bin").encode(), c.byref(w))
+    assert w.value & 1                      # REB_BINARYDATA_ERROR_NOFILE
+    L.reb_simulationarchive_free(sa)
+    assert not L.reb_simulationarchive_create_from_file(str(tmp_path / "none.bin").encode())
+    buf = open(os.path.join(GOLDEN, "archive_whfast.bin"), "rb").read()
+    w = c.c_int(0)
+    sa = L.reb_simulationarchive_create_from_buffer_with_messages(buf, len(buf), c.byref(w))
+    assert w.value == 0 and sa.contents.nblobs == 5 and not sa.contents.filename
+    sim = amd.Sim()
+    w = c.c_int(0)
+    L.reb_simulation_init_from_simulationarchive_with_messages(sim.ptr, sa, 3, c.byref(w))
+    assert w.value == 0
+    assert np.array_equal(sim.state(), golden("archive_whfast_snap3_state.npy"))
+    w = c.c_int(0)
+    L.reb_simulation_init_from_simulationarchive_with_messages(sim.ptr, sa, 9, c.byref(w))
+    assert w.value & 32                     # REB_BINARYDATA_ERROR_OUTOFRANGE
+    sim.free()
+    L.reb_simulationarchive_free(sa)
+    # a damaged second snapshot: the first is kept, with the corrupt-file warning
+    _, end0 = fields(buf)
+    cut = buf[:end0 + BLOB.size + 30]
+    w = c.c_int(0)
+    sa = L.reb_simulationarchive_create_from_buffer_with_messages(cut, len(cut), c.byref(w))
+    assert sa.contents.nblobs == 1 and (w.value & 512)
+    L.reb_simulationarchive_free(sa)
+
+
+def test_diff_copy_and_stream(amd, tmp_path, capfd):
+    """reb_simulation_diff (simulation.c:691-709): 0 for equal simulations, 1
+    after a change, walltime ignored; the differing field printed with option
+    1 and returned by reb_simulation_diff_char.  A copy through the binary
+    format and the stream of a simulation read back as the same simulation."""
+    import ctypes as c
+    L = amd.lib()
+    path = os.path.join(GOLDEN, "archive_whfast.bin")
+    a = amd.Sim.from_file(path, 2)
+    b = amd.Sim.from_file(path, 2)
+    assert L.reb_simulation_diff(a.ptr, b.ptr, 0) == 0
+    b.s.walltime = 123.0
+    assert L.reb_simulation_diff(a.ptr, b.ptr, 0) == 0
+    b.s.dt = 0.5
+    assert L.reb_simulation_diff(a.ptr, b.ptr, 0) == 1
+    capfd.readouterr()
+    assert L.reb_simulation_diff(a.ptr, b.ptr, 1) == 1
+    out = capfd.readouterr().out
+    assert "dt:" in out and "0.5" in out and "walltime:" in out
+    assert L.reb_simulation_diff(a.ptr, b.ptr, 7) == -1
+    L.reb_simulation_diff_char.restype = c.c_void_p
+    p = L.reb_simulation_diff_char(a.ptr, b.ptr)
+    assert b"dt:" in c.string_at(p)
+    L.reb_free(c.c_void_p(p))
+    # stream: the simulation as a binary snapshot in memory
+    bufp = c.c_void_p()
+    size = c.c_size_t()
+    L.reb_binarydata_simulation_to_stream(a.ptr, c.byref(bufp), c.byref(size))
+    data = c.string_at(bufp, size.value)
+    L.reb_free(bufp)
+    f = tmp_path / "stream.bin"
+    f.write_bytes(data)
+    back = amd.Sim.from_file(f, 0)
+    assert L.reb_simulation_diff(a.ptr, back.ptr, 0) == 0
+    # copy through the binary format into a fresh simulation
+    cp = amd.Sim()
+    w = c.c_int(0)
+    L.reb_simulation_copy_with_messages(cp.ptr, a.ptr, c.byref(w))
+    assert w.value == 0
+    assert L.reb_simulation_diff(a.ptr, cp.ptr, 0) == 0
+    assert np.array_equal(cp.state(), a.state())
+    for s in (a, b, back, cp):
+        s.free()

@@ -64,7 +64,17 @@ def test_shear_wrap_bitwise(gpu, k):
     sim.set_integrator("none")
     s.t = t
     s.dt = 0.0
-    sim.set_state(golden("shear_wrap_in.npy"))
+    # The generator moved the bodies outside the box by writing r->particles
+    # directly (reb_simulation_add refuses such bodies): add them inside,
+    # then write the same positions into the host array.
+    st = golden("shear_wrap_in.npy")
+    inside = st.copy()
+    inside[:, :3] = 0.0
+    sim.set_state(inside)
+    p = sim.particles()
+    for k_col, f in enumerate(("x", "y", "z", "vx", "vy", "vz")):
+        p[f][:] = st[:, k_col]
+    sim.lib.reb_amd_mark_host_modified(sim.ptr)
     sim.steps(1)
     assert s.t == t
     assert np.array_equal(sim.state()[:, :6], golden("shear_wrap_out.npy")[k][:, :6])

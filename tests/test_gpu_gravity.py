@@ -16,6 +16,11 @@ pytestmark = pytest.mark.gpu
 
 TOL_FP64 = 1e-14
 TOL_SPLIT = 1e-7
+# One pair's r^-3 in SPLIT carries the FP32 reciprocal square root's error
+# (1 ulp of FP32, 2^-23) three times over (the cube): a force dominated by a
+# single term, such as a planet's by its star, is good to 3 * 2^-23 = 3.6e-7
+# of that term; many-term sums average this down to the 1e-7 above.
+TOL_SPLIT_TERM = 4e-7
 
 
 def make_sim(amd, state, soft, precision, G=1.0):
@@ -149,6 +154,39 @@ def test_fast_modes_testparticles_and_ignore(gpu, orc, prec):
         assert err.max() <= tol, (N_active, tp, ign, err.max())
         if ign == 2:
             assert np.all(acc[0] == 0)
+
+
+def test_split_si_units(gpu, orc):
+    """SPLIT in SI units: a planetary system in metres, kilograms and seconds
+    (G = 6.674e-11, separations 1e9 .. 1e13 m, masses 1e20 .. 2e30 kg).  r^-3
+    (down to 1e-39) and the masses (up to 2e30) lie outside FP32's normal
+    range; the force must keep the per-term bound of the split arithmetic
+    (the star's term dominates every planet's force)."""
+    rng = np.random.default_rng(17)
+    n = 600
+    st = np.zeros((n, 11))
+    au = 1.495978707e11
+    r = au * np.exp(rng.uniform(np.log(0.05), np.log(60.0), n))
+    phi = rng.uniform(0, 2 * np.pi, n)
+    st[:, 0] = r * np.cos(phi)
+    st[:, 1] = r * np.sin(phi)
+    st[:, 2] = r * rng.normal(0, 0.02, n)
+    st[:, 9] = np.exp(rng.uniform(np.log(1e20), np.log(2e27), n))
+    st[0, :3] = 0.0
+    st[0, 9] = 1.989e30
+    G = 6.674e-11
+    ref = orc.gravity_basic(st, G=G, soft=0.0)
+    scale = G * abs_sum(st, 0.0)
+    for prec, tol in (("split", TOL_SPLIT_TERM), ("fp64", TOL_FP64)):
+        sim = make_sim(gpu, st, 0.0, prec, G=G)
+        acc = sim.update_acceleration()
+        sim.free()
+        assert np.isfinite(acc).all()
+        da = np.sqrt(((acc - ref) ** 2).sum(1))
+        rel = da / np.sqrt((ref ** 2).sum(1))
+        print("%s SI: normalised max %.3e relative p99 %.3e" % (prec, (da / scale).max(), np.quantile(rel, 0.99)))
+        assert (da / scale).max() <= tol
+        assert np.quantile(rel, 0.99) <= tol
 
 
 def test_zero_softening_self_term(gpu, orc):

@@ -36,6 +36,22 @@ def make(gpu, init, integrator, dt, gravity="basic", soft=0.0):
     return sim
 
 
+def replay_seed(gpu, sim, seed, per_body, n):
+    """r->rand_seed as the fixture generator left it after drawing the bodies:
+    `seed`, advanced by the same sequence of reb_random_* calls (per body, the
+    kinds in `per_body`: u uniform, n normal), each a glibc rand_r stream on
+    r->rand_seed.  The collision shuffle (collision.c:336-343) continues from
+    there."""
+    L = gpu.lib()
+    sim.s.rand_seed = seed
+    for _ in range(n):
+        for kind in per_body:
+            if kind == "u":
+                L.reb_random_uniform(sim.ptr, 0.0, 1.0)
+            else:
+                L.reb_random_normal(sim.ptr, 1.0)
+
+
 def check_meta(sim, status, name):
     want = golden(name)
     s = sim.s
@@ -161,6 +177,7 @@ def test_halt(gpu, variant):
     variant 1: TREE search with reb_collision_resolve_halt.  The integration
     stops with REB_STATUS_COLLISION after the step that found a collision."""
     sim = make(gpu, "integ_halt%d_init.npy" % variant, "leapfrog", 0.01)
+    replay_seed(gpu, sim, 31 + variant, "uuunnn", 100)
     s = sim.s
     s.root_size = 10.0
     s.N_root_x = s.N_root_y = s.N_root_z = 1
@@ -185,6 +202,7 @@ def test_collision_outcomes_remove_bodies(gpu, tag):
     one = tag.startswith("one")
     tree = tag.endswith("tree")
     sim = make(gpu, "integ_%s_init.npy" % tag, "leapfrog", 0.001, gravity="tree" if tree else "basic")
+    replay_seed(gpu, sim, 77 + (1 if tree else 0) + (2 if one else 0), "uuu", 50 if one else 10)
     s = sim.s
     s.root_size = 50000.0
     s.boundary = gpu.BOUNDARY["open"]
@@ -219,8 +237,10 @@ def test_merge_while_stepping(gpu, variant):
     """reb_collision_resolve_merge on 300 bodies in a periodic box with one
     ring of ghost boxes, DIRECT (0) and TREE (1) searches, energy offset
     tracked: N after every step (heartbeat log), the merged bodies and the
-    energy offset."""
+    energy offset.  The fixtures come from the reference's OpenMP build, whose
+    per-target force sums over ghost boxes are the EXACT kernel's order."""
     sim = make(gpu, "integ_merge%d_init.npy" % variant, "leapfrog", 0.01)
+    replay_seed(gpu, sim, 41, "uuunnnuu", 300)
     s = sim.s
     s.root_size = 4.0
     s.N_root_x, s.N_root_y, s.N_root_z = 2, 2, 1
