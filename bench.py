@@ -25,7 +25,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6        # MI355X dense FP64 vector
+# Algorithmic FLOP per ordered pair of the one-sided kernels (the reference's
+# per-target loop, gravity.c:215-232 and :315-341; sqrt = div = 1).
 FLOP_PER_PAIR = {"fp64": 20, "compensated": 28, "split": 28, "exact": 20}
+# Algorithmic FLOP per unordered pair of the pair-symmetric kernel: the
+# reference's serial loop, gravity.c:195-214 (3 sub, 3 mul + 3 add for r^2
+# with softening, sqrt, 2 mul for r^3, div, 2 mul by the masses, 6 mul + 6 add
+# into both bodies) = 27; split precision 27 + 8 (the FP32 part counted apart
+# as in the one-sided kernel's 28 = 15 FP64 + 13 FP32 per ordered pair).
+FLOP_PER_SYM_PAIR = {"fp64": 27, "split": 35}
 # HBM bytes per launch of k_direct_fast<FM_FP64> at N=65536 from rocprofv3
 # --pmc: FETCH_SIZE 8229.9 KiB x 2.0 + WRITE_SIZE 24576 KiB x 1.0, the factors
 # being what the counters read for a device-to-device copy of known size
@@ -67,6 +75,7 @@ def run_mode(amd, args, name, precision, gravity, steps, warmup):
     ktot = ctypes.c_double(0.0)
     kn = ctypes.c_int(0)
     L.reb_amd_kernel_time(sim.ptr, ctypes.byref(ktot), ctypes.byref(kn))
+    kpairs = L.reb_amd_kernel_pairs(sim.ptr)
     L.reb_amd_set_profiling(sim.ptr, 0)
     L.reb_amd_set_resident(sim.ptr, 0)
     e1 = sim.energy()
@@ -75,13 +84,27 @@ def run_mode(amd, args, name, precision, gravity, steps, warmup):
     pairs = float(n) * float(n - 1)
     wall = t1 - t0
     kernel_ms = ktot.value / max(kn.value, 1)
+    # ordered pairs the timed kernel evaluates per launch: N(N-1) for the
+    # one-sided kernels; for the pair-symmetric one the pairs between
+    # superblocks, each unordered pair once (counted here as two ordered)
+    kp = kpairs / max(kn.value, 1)
+    sym = kp < pairs
+    if sym:
+        units, flop_unit, unit = kp / 2.0, FLOP_PER_SYM_PAIR[name], "unordered pair"
+        kernel = "k_direct_sym<%s>" % ("FM_SPLIT" if name == "split" else "FM_FP64")
+    else:
+        units, flop_unit, unit = kp, FLOP_PER_PAIR[name], "ordered pair"
+        kernel = "k_exact" if name == "exact" else "k_direct_fast<%s>" % (
+            {"fp64": "FM_FP64", "split": "FM_SPLIT", "compensated": "FM_COMP"}[name])
     res = {
         "precision": precision, "gravity": gravity,
         "steps": steps, "seconds": wall, "ms_per_step": 1e3 * wall / steps,
         "gpairs_per_s": pairs * steps / wall / 1e9,
-        "kernel_ms": kernel_ms, "kernel_launches": kn.value,
-        "kernel_gpairs_per_s": pairs / (kernel_ms * 1e-3) / 1e9,
-        "algorithmic_tflops": FLOP_PER_PAIR[name] * pairs / (kernel_ms * 1e-3) / 1e12,
+        "kernel": kernel, "kernel_ms": kernel_ms, "kernel_launches": kn.value,
+        "kernel_ordered_pairs_per_launch": kp,
+        "kernel_gpairs_per_s": kp / (kernel_ms * 1e-3) / 1e9,
+        "work_unit": unit, "units_per_launch": units, "flop_per_unit": flop_unit,
+        "algorithmic_tflops": flop_unit * units / (kernel_ms * 1e-3) / 1e12,
         "rel_energy_drift": (e1 - e0) / abs(e0),
     }
     log("%-12s %8.2f ms/step  %8.1f Gpair/s (kernel %.3f ms, %.1f TFLOP/s algorithmic)" % (
@@ -194,9 +217,15 @@ def ensemble_leg(amd, args):
         "rel_energy_error": {"median": float(np.median(rel)), "p99": float(np.quantile(rel, 0.99)),
                              "max": float(rel.max()), "mean": float(rel.mean())},
         "kepler_period_warnings": int(((flags >> 1) & 1).sum()),
+        # systems whose Kepler solves took the quartic iteration / fell back
+        # to bisection at least once during the request
+        "kepler_quartic_systems": int(((flags >> 3) & 1).sum()),
+        "kepler_bisection_systems": int(((flags >> 4) & 1).sum()),
     }
-    log("ensemble: %.3e system-steps/s (%.2f us/step, %.2f s), rel energy error median %.2e mean %.2e" % (
-        res["value"], res["us_per_step"], wall, np.median(rel), rel.mean()))
+    log("ensemble: %.3e system-steps/s (%.2f us/step, %.2f s), rel energy error median %.2e mean %.2e; "
+        "systems with quartic Kepler solves %d, with bisection %d" % (
+            res["value"], res["us_per_step"], wall, np.median(rel), rel.mean(), res["kepler_quartic_systems"],
+            res["kepler_bisection_systems"]))
     del hip, L
     if not args.no_cpu:
         cpu = ensemble_cpu(args)
@@ -574,8 +603,9 @@ def main():
                      "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": DIRECT_TRAFFIC_BYTES_N65536 if args.n == 65536 else None,
                      "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)",
-                     "kernel": "k_direct_fast<FM_FP64>", "kernel_ms": head["kernel_ms"],
-                     "flop_per_pair": FLOP_PER_PAIR["fp64"], "pairs_per_launch": float(args.n) * (args.n - 1)},
+                     "kernel": head["kernel"], "kernel_ms": head["kernel_ms"],
+                     "work_unit": head["work_unit"], "flop_per_unit": head["flop_per_unit"],
+                     "units_per_launch": head["units_per_launch"]},
         "cpu_baseline": cpu,
         "modes": modes,
         "ensemble": ens,

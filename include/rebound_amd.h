@@ -59,6 +59,12 @@ REB_API void reb_amd_set_profiling(struct reb_simulation* r, int on);
  * stream and returns the summed kernel time (ms) and the number of launches
  * since profiling was switched on or last read. */
 REB_API int reb_amd_kernel_time(struct reb_simulation* r, double* total_ms, int* launches);
+/* With profiling on: the ordered body pairs the timed direct-gravity kernels
+ * evaluated since profiling was switched on or last read (then reset).  On
+ * the symmetric path (every body active, no ghost boxes, nothing ignored,
+ * N >= 8192) the timed kernel covers the pairs between superblocks of 1024
+ * bodies, N^2 minus the pairs inside superblocks; otherwise N(N-1). */
+REB_API double reb_amd_kernel_pairs(struct reb_simulation* r);
 
 /* ------------------------------------------------------------ device forces
  * The reference's additional_forces callback (rebound.h:403, called at the end

@@ -182,6 +182,7 @@ def declare(lib):
         "reb_amd_tree_interactions": (ctypes.c_longlong, [S]),
         "reb_amd_set_profiling": (None, [S, c_int]),
         "reb_amd_kernel_time": (c_int, [S, POINTER(c_double), POINTER(c_int)]),
+        "reb_amd_kernel_pairs": (c_double, [S]),
         "reb_amd_set_resident": (None, [S, c_int]),
         "reb_amd_set_coefficient_of_restitution": (None, [S, c_double]),
         "reb_amd_coefficient_of_restitution_constant": (c_double, [S, c_double]),

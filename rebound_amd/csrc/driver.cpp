@@ -305,6 +305,13 @@ REB_API int reb_amd_kernel_time(struct reb_simulation* r, double* total_ms, int*
     return 0;
 }
 
+REB_API double reb_amd_kernel_pairs(struct reb_simulation* r) {
+    DeviceState* d = dev_of(r);
+    const double p = d->kpairs;
+    d->kpairs = 0.;
+    return p;
+}
+
 REB_API void reb_amd_set_resident(struct reb_simulation* r, int on) {
     DeviceState* d = dev_of(r);
     if (!on && d->resident) ensure_host(r);

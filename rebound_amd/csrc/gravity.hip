@@ -672,6 +672,262 @@ __global__ __launch_bounds__(256) void k_seg_reduce(const SegJob* __restrict__ j
     reduce_one(J.partial, J.n, J.nchunk, i, J.negG, 0, 0, J.ax, J.ay, J.az, kd);
 }
 
+// ---------------------------------------------------------------- symmetric pair kernel
+// Newton's third law, as the reference's serial build uses it (gravity.c:195-214
+// visits each pair once and adds the term to both bodies): each unordered pair
+// of bodies in different superblocks is evaluated once.  The bodies are cut
+// into superblocks of kSymSB; a workgroup takes one pair of superblocks A < B
+// (a task).  Its 4 wavefronts hold the targets of A, SYM_TPL per lane, in
+// registers; the sources of B stream through LDS 64 at a time (a J-block),
+// and lane k evaluates source (k - s) & 63 at step s, so the 64 lanes see 64
+// different sources at once.  The source-side sum travels with its source:
+// after each step the three accumulators are rotated one lane over the whole
+// wavefront (DPP wave_ror:1), and after 64 steps every lane again holds the
+// sum of the source it started with.  Per unordered pair: the FP64 r^-3 of
+// the fast mode (or the split-precision one) once, two mass products, three
+// fused multiply-adds on each side; per step the rotation adds six moves.
+//
+// Outputs, all in fixed order (the results are deterministic):
+//   ipart[t][c][SB]  target-side sums of task t (targets of A, sources of B)
+//   jpart[t][c][SB]  source-side sums of task t (bodies of B, all targets of A)
+//                    with the sign of the term for A's bodies: S_j = -jpart
+//   dpart[6][N]      the one-sided sums within each superblock (direct_fast_block)
+// and k_sym_reduce adds, per body, the partials of every superblock in
+// ascending order with Kahan compensation.
+#ifndef RB_SYM_TPL
+#define RB_SYM_TPL 4
+#endif
+static constexpr int kSymTPL = RB_SYM_TPL;           // targets per lane
+static constexpr int kSymSB = 4 * 64 * kSymTPL;       // bodies per superblock
+static constexpr int kSymJB = 64;                     // sources per J-block
+static constexpr int kSymMinN = 8192;                 // below: the one-sided kernel
+
+// Lane k receives lane k-1's value, lane 0 lane 63's (DPP wave_ror:1).
+__device__ __forceinline__ double dpp_ror1(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x13C, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x13C, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+__host__ __device__ __forceinline__ int sym_task(int A, int B, int nsb) {
+    return A * nsb - A * (A + 1) / 2 + (B - A - 1);
+}
+
+template <int MODE, bool CHECK>
+__global__ __launch_bounds__(256) void k_direct_sym(const double* __restrict__ x, const double* __restrict__ y,
+                                                    const double* __restrict__ z, const double* __restrict__ m,
+                                                    int N, int nsb, double soft2, const int2* __restrict__ tasks,
+                                                    int ntasks_done, double* __restrict__ ipart,
+                                                    double* __restrict__ jpart) {
+    __shared__ double4 jt[2][kSymJB];              // J-block sources (x, y, z, m), double-buffered
+    __shared__ double jred[2][4][3][kSymJB];       // per-wave source sums of a J-block
+    const int2 tk = tasks[ntasks_done + blockIdx.x];
+    const int A = tk.x, B = tk.y;
+    const int t = sym_task(A, B, nsb);
+    const int tid = threadIdx.x;
+    const int w = tid >> 6, k = tid & 63;
+    // targets of this lane
+    double tx[kSymTPL], ty[kSymTPL], tz[kSymTPL], tm[kSymTPL];
+    double sx[kSymTPL], sy[kSymTPL], sz[kSymTPL];
+#pragma unroll
+    for (int q = 0; q < kSymTPL; q++) {
+        const int i = A * kSymSB + w * 64 * kSymTPL + q * 64 + k;
+        const bool ok = !CHECK || i < N;
+        tx[q] = ok ? x[i] : 0.;
+        ty[q] = ok ? y[i] : 0.;
+        tz[q] = ok ? z[i] : 0.;
+        tm[q] = ok ? m[i] : 0.;
+        sx[q] = sy[q] = sz[q] = 0.;
+    }
+    const int jbase = B * kSymSB;
+    constexpr int NJB = kSymSB / kSymJB;
+    // prologue: J-block 0 into LDS, J-block 1 into registers
+    double4 nxt = make_double4(0., 0., 0., 0.);
+    if (tid < kSymJB) {
+        const int j0 = jbase + tid;
+        const bool ok0 = !CHECK || j0 < N;
+        jt[0][tid] = ok0 ? make_double4(x[j0], y[j0], z[j0], m[j0]) : make_double4(0., 0., 0., 0.);
+        const int j1 = jbase + kSymJB + tid;
+        const bool ok1 = !CHECK || j1 < N;
+        if (ok1) nxt = make_double4(x[j1], y[j1], z[j1], m[j1]);
+    }
+    const double c15 = vreg_const(1.5);
+    const double c1875 = vreg_const(1.875);
+    for (int jb = 0; jb < NJB; jb++) {
+        __syncthreads();
+        if (jb > 0 && w < 3) {
+            // source sums of J-block jb-1: the four waves' in order
+            const double* r = &jred[(jb - 1) & 1][0][w][k];
+            const double v = ((r[0] + r[3 * kSymJB]) + r[6 * kSymJB]) + r[9 * kSymJB];
+            jpart[((size_t)t * 3 + w) * kSymSB + (jb - 1) * kSymJB + k] = v;
+        }
+        if (tid < kSymJB) {
+            if (jb + 1 < NJB) jt[(jb + 1) & 1][tid] = nxt;
+            const int jn = jbase + (jb + 2) * kSymJB + tid;
+            const bool okn = jb + 2 < NJB && (!CHECK || jn < N);
+            nxt = okn ? make_double4(x[jn], y[jn], z[jn], m[jn]) : make_double4(0., 0., 0., 0.);
+        }
+        const double4* src = jt[jb & 1];
+        double bx[kSymTPL], by[kSymTPL], bz[kSymTPL];
+#pragma unroll
+        for (int q = 0; q < kSymTPL; q++) bx[q] = by[q] = bz[q] = 0.;
+        double jx = 0., jy = 0., jz = 0.;
+        const int jlim = CHECK ? N - (jbase + jb * kSymJB) : kSymJB;   // valid sources of this J-block
+#pragma unroll 2
+        for (int s = 0; s < kSymJB; s++) {
+            const int idx = (k - s) & (kSymJB - 1);
+            const double4 sj = src[idx];
+#pragma unroll
+            for (int q = 0; q < kSymTPL; q++) {
+                const double dx = tx[q] - sj.x;
+                const double dy = ty[q] - sj.y;
+                const double dz = tz[q] - sj.z;
+                const double r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, soft2)));
+                double wr;
+                if (MODE == FM_SPLIT) {
+                    const double yv = (double)__builtin_amdgcn_rsqf((float)r2);
+                    wr = yv * yv * yv;
+                } else {
+                    wr = rinv3_fp64(r2, c15, c1875);
+                }
+                if (CHECK) {
+                    const int i = A * kSymSB + w * 64 * kSymTPL + q * 64 + k;
+                    if (i >= N || idx >= jlim) wr = 0.;
+                }
+                const double wj = wr * sj.w;
+                const double wi = wr * tm[q];
+                bx[q] = fma(wj, dx, bx[q]);
+                by[q] = fma(wj, dy, by[q]);
+                bz[q] = fma(wj, dz, bz[q]);
+                jx = fma(wi, dx, jx);
+                jy = fma(wi, dy, jy);
+                jz = fma(wi, dz, jz);
+            }
+            jx = dpp_ror1(jx);
+            jy = dpp_ror1(jy);
+            jz = dpp_ror1(jz);
+        }
+#pragma unroll
+        for (int q = 0; q < kSymTPL; q++) {
+            sx[q] += bx[q];
+            sy[q] += by[q];
+            sz[q] += bz[q];
+        }
+        double* jr = &jred[jb & 1][w][0][k];
+        jr[0] = jx;
+        jr[kSymJB] = jy;
+        jr[2 * kSymJB] = jz;
+    }
+    __syncthreads();
+    if (w < 3) {
+        const double* r = &jred[(NJB - 1) & 1][0][w][k];
+        const double v = ((r[0] + r[3 * kSymJB]) + r[6 * kSymJB]) + r[9 * kSymJB];
+        jpart[((size_t)t * 3 + w) * kSymSB + (NJB - 1) * kSymJB + k] = v;
+    }
+#pragma unroll
+    for (int q = 0; q < kSymTPL; q++) {
+        const int li = w * 64 * kSymTPL + q * 64 + k;
+        double* p = ipart + (size_t)t * 3 * kSymSB + li;
+        p[0] = sx[q];
+        p[kSymSB] = sy[q];
+        p[2 * kSymSB] = sz[q];
+    }
+}
+
+// The one-sided sums inside each superblock (pairs i != j of one superblock):
+// the fast pair block with targets and sources limited to the superblock.
+template <int MODE>
+__global__ __launch_bounds__(kFast) __attribute__((amdgpu_waves_per_eu(RB_FAST_WAVES, RB_FAST_WAVES))) void k_direct_diag(
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+    const double* __restrict__ m, int N, double soft2, double* __restrict__ dpart) {
+    __shared__ double4 tile[kFast];
+    constexpr int per = kSymSB / kFast;                 // target blocks per superblock
+    const int A = (int)blockIdx.x / per;
+    const int j0 = A * kSymSB;
+    const int j1 = min(N, j0 + kSymSB);
+    direct_fast_block<MODE>(x, y, z, m, N, j0, j1, kSymSB, 0, soft2, GBZero{}, dpart, N, (int)blockIdx.x, 0, tile);
+}
+
+// Per body: the partials of every superblock in ascending order (Kahan), then
+// -G and the fused kick and drift as in the one-sided reduction.
+__global__ void k_sym_reduce(const double* __restrict__ ipart, const double* __restrict__ jpart,
+                             const double* __restrict__ dpart, int N, int nsb, double negG, double* __restrict__ ax,
+                             double* __restrict__ ay, double* __restrict__ az, KickDrift kd) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int X = i / kSymSB, loc = i % kSymSB;
+    double s[3] = {0., 0., 0.}, c[3] = {0., 0., 0.};
+    for (int Y = 0; Y < nsb; Y++) {
+        if (Y < X) {
+            const double* p = jpart + (size_t)sym_task(Y, X, nsb) * 3 * kSymSB + loc;
+#pragma unroll
+            for (int q = 0; q < 3; q++) kahan_add(-p[(size_t)q * kSymSB], s[q], c[q]);
+        } else if (Y == X) {
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                kahan_add(dpart[(size_t)q * N + i], s[q], c[q]);
+                kahan_add(-dpart[(size_t)(q + 3) * N + i], s[q], c[q]);
+            }
+        } else {
+            const double* p = ipart + (size_t)sym_task(X, Y, nsb) * 3 * kSymSB + loc;
+#pragma unroll
+            for (int q = 0; q < 3; q++) kahan_add(p[(size_t)q * kSymSB], s[q], c[q]);
+        }
+    }
+    const double o0 = negG * s[0], o1 = negG * s[1], o2 = negG * s[2];
+    ax[i] = o0;
+    ay[i] = o1;
+    az[i] = o2;
+    if (kd.on) {
+        const double vx = kd.vx[i] + kd.kick * o0;
+        const double vy = kd.vy[i] + kd.kick * o1;
+        const double vz = kd.vz[i] + kd.kick * o2;
+        kd.vx[i] = vx;
+        kd.vy[i] = vy;
+        kd.vz[i] = vz;
+        kd.x[i] = kd.x[i] + kd.drift * vx;
+        kd.y[i] = kd.y[i] + kd.drift * vy;
+        kd.z[i] = kd.z[i] + kd.drift * vz;
+    }
+}
+
+// Superblock layout of the symmetric path for N bodies.
+struct SymPlan {
+    int nsb = 0;          // superblocks
+    int ntasks = 0;       // superblock pairs A < B
+    int nfull = 0;        // of them, pairs without a partial superblock (launched first)
+    size_t ipart_off = 0, jpart_off = 0, dpart_off = 0, tasks_off = 0, bytes = 0;   // workspace layout (bytes)
+};
+
+static SymPlan sym_plan(int N) {
+    SymPlan p;
+    p.nsb = (N + kSymSB - 1) / kSymSB;
+    p.ntasks = p.nsb * (p.nsb - 1) / 2;
+    const bool ragged = N % kSymSB != 0;
+    p.nfull = ragged ? (p.nsb - 1) * (p.nsb - 2) / 2 : p.ntasks;
+    const size_t part = sizeof(double) * 3 * (size_t)kSymSB * (size_t)p.ntasks;
+    auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    p.ipart_off = 0;
+    p.jpart_off = up(part);
+    p.dpart_off = p.jpart_off + up(part);
+    p.tasks_off = p.dpart_off + up(sizeof(double) * 6 * (size_t)N);
+    p.bytes = p.tasks_off + up(sizeof(int2) * (size_t)(p.ntasks > 0 ? p.ntasks : 1));
+    return p;
+}
+
+// The task list (A, B), A < B: pairs without the ragged last superblock first,
+// then those with it (launched with the bounds checks).
+static std::vector<int2> sym_tasks(const SymPlan& p) {
+    std::vector<int2> t;
+    t.reserve(p.ntasks);
+    const int last = (p.nfull == p.ntasks) ? p.nsb : p.nsb - 1;
+    for (int A = 0; A < last; A++)
+        for (int B = A + 1; B < last; B++) t.push_back(make_int2(A, B));
+    if (last < p.nsb)
+        for (int A = 0; A < last; A++) t.push_back(make_int2(A, last));
+    return t;
+}
+
 // Number of source chunks for the fast modes: enough workgroups to fill every
 // SIMD of the 256-CU part several times over, chunks no shorter than
 // kMinChunk sources (short chunks keep the time of one step small for N of a
@@ -715,6 +971,60 @@ int seg_step_launch(const SegJob* jobs, const SegRange* ranges, int nranges, con
 
 int seg_fast_mode(int precision, int compensated) {
     return compensated ? FM_COMP : (precision == REB_AMD_PRECISION_SPLIT ? FM_SPLIT : FM_FP64);
+}
+
+// Whether a simulation of N bodies, all active, without ghost boxes or
+// ignored terms, takes the symmetric path in this precision and gravity, and
+// the workspace it then needs.
+bool gravity_symmetric_path(size_t N, int precision, bool compensated) {
+    return !compensated && precision != REB_AMD_PRECISION_EXACT && N >= (size_t)kSymMinN;
+}
+size_t gravity_symmetric_bytes(size_t N) { return sym_plan((int)N).bytes; }
+
+// Ordered pairs inside the superblocks (evaluated by k_direct_diag, not by
+// the symmetric kernel): sum over superblocks of n_A^2, self pairs included.
+static double sym_diag_pairs(int N) {
+    const int full = N / kSymSB, rest = N % kSymSB;
+    return (double)full * kSymSB * kSymSB + (double)rest * rest;
+}
+
+// One force evaluation on the symmetric path: the pairs inside superblocks,
+// the superblock pairs (events around these), then the per-body reduction
+// with the fused kick and drift.
+static int sym_launch(int mode, const double* x, const double* y, const double* z, const double* m, double* ax,
+                      double* ay, double* az, int N, double G, double soft2, const SymPlan& plan, char* base,
+                      hipStream_t st, hipEvent_t* ev, const KickDrift& kd) {
+    double* ipart = reinterpret_cast<double*>(base + plan.ipart_off);
+    double* jpart = reinterpret_cast<double*>(base + plan.jpart_off);
+    double* dpart = reinterpret_cast<double*>(base + plan.dpart_off);
+    const int2* tasks = reinterpret_cast<const int2*>(base + plan.tasks_off);
+    const dim3 gdiag(plan.nsb * (kSymSB / kFast));
+    if (mode == FM_SPLIT)
+        hipLaunchKernelGGL(k_direct_diag<FM_SPLIT>, gdiag, dim3(kFast), 0, st, x, y, z, m, N, soft2, dpart);
+    else
+        hipLaunchKernelGGL(k_direct_diag<FM_FP64>, gdiag, dim3(kFast), 0, st, x, y, z, m, N, soft2, dpart);
+    if (ev) (void)hipEventRecord(ev[0], st);
+    const int nrag = plan.ntasks - plan.nfull;
+    if (plan.nfull > 0) {
+        if (mode == FM_SPLIT)
+            hipLaunchKernelGGL((k_direct_sym<FM_SPLIT, false>), dim3(plan.nfull), dim3(256), 0, st, x, y, z, m, N,
+                               plan.nsb, soft2, tasks, 0, ipart, jpart);
+        else
+            hipLaunchKernelGGL((k_direct_sym<FM_FP64, false>), dim3(plan.nfull), dim3(256), 0, st, x, y, z, m, N,
+                               plan.nsb, soft2, tasks, 0, ipart, jpart);
+    }
+    if (nrag > 0) {
+        if (mode == FM_SPLIT)
+            hipLaunchKernelGGL((k_direct_sym<FM_SPLIT, true>), dim3(nrag), dim3(256), 0, st, x, y, z, m, N,
+                               plan.nsb, soft2, tasks, plan.nfull, ipart, jpart);
+        else
+            hipLaunchKernelGGL((k_direct_sym<FM_FP64, true>), dim3(nrag), dim3(256), 0, st, x, y, z, m, N,
+                               plan.nsb, soft2, tasks, plan.nfull, ipart, jpart);
+    }
+    if (ev) (void)hipEventRecord(ev[1], st);
+    hipLaunchKernelGGL(k_sym_reduce, dim3((N + 255) / 256), dim3(256), 0, st, ipart, jpart, dpart, N, plan.nsb, -G,
+                       ax, ay, az, kd);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
 // Launch one fast-mode pass: targets [0,n_recv) feel sources [j_begin,j_end).
@@ -879,11 +1189,6 @@ bool gravity_direct(struct reb_simulation* r, bool compensated, const double* ki
     GBList gl;
     gl.n = ngb;
     for (int g = 0; g < ngb; g++) gl.g[g] = GB3{gbv[g].x, gbv[g].y, gbv[g].z};
-    const int nchunk = pick_chunks(N, N_active, ngb);
-    if (!d->partial.ensure(workspace_bytes(N, nchunk) + 256)) {
-        reb_simulation_error(r, "Out of device memory for the gravity workspace.");
-        return false;
-    }
     hipEvent_t* ev = nullptr;
     if (d->profiling) {
         if (d->kev_used + 2 > d->kev.size()) {
@@ -901,6 +1206,38 @@ bool gravity_direct(struct reb_simulation* r, bool compensated, const double* ki
         kd = KickDrift{d->field(FX), d->field(FY), d->field(FZ), d->field(FVX), d->field(FVY), d->field(FVZ),
                        kick_drift[0], kick_drift[1], 1};
     }
+    // Every body active, no ghost boxes, nothing ignored: each pair once
+    // (Newton's third law); otherwise the one-sided kernels.
+    if (ngb == 1 && N_active == N && r->gravity_ignore_terms == 0 &&
+        gravity_symmetric_path((size_t)N, d->precision, compensated)) {
+        const SymPlan plan = sym_plan(N);
+        if (!d->partial.ensure(plan.bytes)) {
+            reb_simulation_error(r, "Out of device memory for the gravity workspace.");
+            return false;
+        }
+        char* base = static_cast<char*>(d->partial.p);
+        int2* tasks = reinterpret_cast<int2*>(base + plan.tasks_off);
+        if (d->sym_tasks_n != N || d->sym_tasks_dev != (const void*)tasks) {
+            d->sym_tasks_host = sym_tasks(plan);
+            if (!RB_HIP(r, hipMemcpyAsync(tasks, d->sym_tasks_host.data(), sizeof(int2) * d->sym_tasks_host.size(),
+                                          hipMemcpyHostToDevice, d->stream)))
+                return false;
+            d->sym_tasks_n = N;
+            d->sym_tasks_dev = tasks;
+        }
+        if (d->profiling) d->kpairs += (double)N * (double)N - sym_diag_pairs(N);
+        const int mode = d->precision == REB_AMD_PRECISION_SPLIT ? FM_SPLIT : FM_FP64;
+        const int rc = sym_launch(mode, d->field(FX), d->field(FY), d->field(FZ), d->field(FM), d->field(FAX),
+                                  d->field(FAY), d->field(FAZ), N, r->G, soft2, plan, base, d->stream, ev, kd);
+        if (rc) return RB_HIP(r, hipGetLastError());
+        return true;
+    }
+    const int nchunk = pick_chunks(N, N_active, ngb);
+    if (!d->partial.ensure(workspace_bytes(N, nchunk) + 256)) {
+        reb_simulation_error(r, "Out of device memory for the gravity workspace.");
+        return false;
+    }
+    if (d->profiling) d->kpairs += (double)N * (double)(N - 1);
     int rc = direct_dispatch(d->field(FX), d->field(FY), d->field(FZ), d->field(FM), d->field(FAX), d->field(FAY),
                              d->field(FAZ), N, N_active, r->testparticle_type, (int)r->gravity_ignore_terms, r->G,
                              soft2, d->precision, compensated ? 1 : 0, gl, d->partial.as<double>(), d->stream, ev, kd);

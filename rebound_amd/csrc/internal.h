@@ -126,6 +126,13 @@ struct DeviceState {
     void* dev_forces_user = nullptr;
     // the EXACT energy fell back to the compensated sum once (warned)
     bool warned_energy_exact = false;
+    // task list of the symmetric direct-gravity path: for N bodies, at this
+    // device address inside `partial`
+    std::vector<int2> sym_tasks_host;
+    const void* sym_tasks_dev = nullptr;
+    int sym_tasks_n = -1;
+    // ordered pairs evaluated by the timed pair kernels since the last read
+    double kpairs = 0.;
     double* field(int f) const { return soa.as<double>() + (size_t)f * cap; }
 };
 
@@ -154,6 +161,10 @@ std::vector<struct reb_vec6d> ghostboxes(struct reb_simulation* r, int nx, int n
 // ------------------------------------------------------------------ kernels (launchers)
 // Direct O(N^2) gravity (gravity.c:167-531) on the simulation's device state.
 bool gravity_direct(struct reb_simulation* r, bool compensated, const double* kick_drift = nullptr);
+// The pair-symmetric direct-gravity path (gravity.hip) for N bodies, all
+// active, no ghost boxes, nothing ignored; its workspace in bytes.
+bool gravity_symmetric_path(size_t N, int precision, bool compensated);
+size_t gravity_symmetric_bytes(size_t N);
 
 // Batched order-2 leapfrog steps with direct gravity for many independent
 // simulations (gravity.hip k_seg_*).  A SegJob holds one simulation's device

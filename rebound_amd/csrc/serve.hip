@@ -24,9 +24,10 @@
 // same way on a second stream: every scheduler round advances all of them by
 // the same number of leapfrog steps, three launches per step for the whole
 // batch (gravity.hip k_seg_*), each job with the chunking it would use alone;
-// they join and leave between rounds.  Jobs in EXACT precision run as single
-// simulations from worker threads on a third stream.  The streams overlap on
-// the device.
+// they join and leave between rounds.  Jobs in EXACT precision, and jobs
+// large enough for the pair-symmetric force path (gravity.hip), run as single
+// simulations from worker threads on a third stream, so that every job's bits
+// are those of stepping it alone.  The streams overlap on the device.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -181,10 +182,21 @@ static double since(const Server* S) {
 
 static size_t whfast_footprint(size_t n) { return n * 17 * sizeof(double) + sizeof(WHSys) + 64; }
 
-static size_t nbody_footprint(size_t n, int precision) {
+static size_t nbody_footprint(size_t n, int precision, int gravity) {
     size_t cap = 64;
     while (cap < n) cap *= 2;
-    return NFIELD * cap * sizeof(double) + reb_amd_gravity_workspace_bytes(n, precision) + 16 * n + 4096;
+    size_t ws = reb_amd_gravity_workspace_bytes(n, precision);
+    if (gravity_symmetric_path(n, precision, gravity == REB_GRAVITY_COMPENSATED))
+        ws = std::max(ws, gravity_symmetric_bytes(n));
+    return NFIELD * cap * sizeof(double) + ws + 16 * n + 4096;
+}
+
+// Jobs that run as single simulations on the worker threads: EXACT, and those
+// whose force takes the pair-symmetric path when stepped alone (the batch
+// kernels are one-sided, so a job's bits would otherwise depend on the route).
+static bool nbody_single(const Job* j) {
+    return j->precision == REB_AMD_PRECISION_EXACT ||
+           gravity_symmetric_path(j->bodies.size(), j->precision, j->gravity == REB_GRAVITY_COMPENSATED);
 }
 
 // ------------------------------------------------------------------ pool memory
@@ -731,7 +743,7 @@ static void scheduler(Server* S) {
                 j->t_admit = since(S);
                 if (j->kind == JOB_NBODY) {
                     S->nbody_running++;
-                    if (j->precision == REB_AMD_PRECISION_EXACT) {
+                    if (nbody_single(j)) {
                         S->nbody_ready.push_back(j);
                         S->cv_work.notify_all();
                     } else {
@@ -814,7 +826,7 @@ REB_API struct reb_amd_server* reb_amd_server_create(size_t memory_budget_bytes,
         return nullptr;
     }
     S->sched = std::thread(scheduler, S);
-    // Three streams (planetary pool, N-body batch, EXACT N-body jobs): within
+    // Three streams (planetary pool, N-body batch, single N-body jobs): within
     // the four hardware queues a process gets by default, so no kernel waits
     // behind another stream's launch.
     hipStream_t st = nullptr;
@@ -937,13 +949,15 @@ REB_API long reb_amd_server_submit_nbody(struct reb_amd_server* h, size_t n, con
     j->n_steps = n_steps;
     j->gravity = gravity;
     j->precision = precision;
-    j->footprint = nbody_footprint(n, precision);
+    j->footprint = nbody_footprint(n, precision, gravity);
     return submit(&h->s, std::move(j));
 }
 
 // Device-memory footprints used for admission.
 REB_API size_t reb_amd_server_whfast_bytes(size_t n_bodies) { return whfast_footprint(n_bodies); }
-REB_API size_t reb_amd_server_nbody_bytes(size_t n, int precision) { return nbody_footprint(n, precision); }
+REB_API size_t reb_amd_server_nbody_bytes(size_t n, int precision) {
+    return nbody_footprint(n, precision, REB_GRAVITY_BASIC);
+}
 
 REB_API int reb_amd_server_status(struct reb_amd_server* h, long job) {
     Server* S = &h->s;

@@ -47,10 +47,15 @@ __device__ __forceinline__ double fastabs(double x) { return (x > 0.) ? x : -x; 
 
 // c_0..c_3 Stumpff functions of z by series + quartering (integrator_whfast.c:189-213).
 __device__ __forceinline__ void stumpff_cs3(double* cs, double z) {
-    // A finite z needs at most ~520 quarterings; the cap only stops a
-    // non-finite z from spinning forever (the result is then NaN anyway).
+    // A finite z needs at most ~520 quarterings.  An infinite z never comes
+    // below 0.1 (the reference's loop would not end): the functions are
+    // returned as NaN at once.
+    if (isinf(z)) {
+        cs[0] = cs[1] = cs[2] = cs[3] = __builtin_nan("");
+        return;
+    }
     unsigned int n = 0;
-    while (fabs(z) > 0.1 && n < 1100) {
+    while (fabs(z) > 0.1) {
         z = z / 4.;
         n++;
     }
@@ -95,46 +100,46 @@ __device__ __forceinline__ double quartic_next(double X, double beta, double r0,
 
 // The quartic iterates X_1, X_2, ... (X_k the update of X_{k-1}) are a
 // deterministic sequence, so the first k with X_k equal to an earlier iterate
-// is mu + lambda (tail length plus cycle length), the earlier one X_mu.  This
-// finds both with Brent's cycle detection in O(1) memory and returns mu +
-// lambda when it is at most 63 (the reference's last quartic iteration), else
-// 0.  With tortoise positions 1, 2, 4, .., 64 and mu, lambda <= 62 the cycle
-// is found before the hare passes iterate 126, hence the bound of 128.
-__device__ __noinline__ int quartic_first_repeat(double X0, double beta, double r0, double eta0, double zeta0,
-                                                 double dt) {
+// is mu + lambda (tail length plus cycle length), the earlier one X_mu; every
+// later repeat spans a multiple of lambda.  Given X_63, the last iterate the
+// reference's loop produces, this returns mu + lambda when it is at most 63,
+// else 0, without a history: if the first repeat is at most 63, X_63 lies on
+// the cycle, so the cycle length is the first h in 64..125 with X_h == X_63
+// (lambda <= 62); mu is then the first i with X_i == X_{i+lambda}.  A NaN
+// iterate repeats nothing and every later iterate is NaN, so a NaN X_63, or a
+// NaN after it, means no repeat.
+__device__ __noinline__ int quartic_first_repeat(double X0, double X63, double beta, double r0, double eta0,
+                                                 double zeta0, double dt) {
+    if (X63 != X63) return 0;
     double Gs[4];
-    const double a1 = quartic_next(X0, beta, r0, eta0, zeta0, dt, Gs);
-    double tort = a1;
-    double hare = quartic_next(a1, beta, r0, eta0, zeta0, dt, Gs);
-    int power = 1, lam = 1, hidx = 2;
-    while (!(tort == hare)) {
-        if (hidx >= 128) return 0;
-        if (power == lam) {
-            tort = hare;
-            power *= 2;
-            lam = 0;
+    double h = X63;
+    int lam = 0;
+    for (int k = 64; k < 126; k++) {
+        h = quartic_next(h, beta, r0, eta0, zeta0, dt, Gs);
+        if (h != h) return 0;
+        if (h == X63) {
+            lam = k - 63;
+            break;
         }
-        hare = quartic_next(hare, beta, r0, eta0, zeta0, dt, Gs);
-        hidx++;
-        lam++;
     }
-    tort = a1;
-    hare = a1;
+    if (lam == 0) return 0;
+    double tort = quartic_next(X0, beta, r0, eta0, zeta0, dt, Gs);
+    double hare = tort;
     for (int k = 0; k < lam; k++) hare = quartic_next(hare, beta, r0, eta0, zeta0, dt, Gs);
-    int mu = 0;
+    int mu = 1;
     while (!(tort == hare)) {
         tort = quartic_next(tort, beta, r0, eta0, zeta0, dt, Gs);
         hare = quartic_next(hare, beta, r0, eta0, zeta0, dt, Gs);
-        if (1 + ++mu + lam > 63) return 0;
+        if (++mu + lam > 63) return 0;
     }
-    return 1 + mu + lam;
+    return mu + lam;
 }
 
 // Universal-variable Kepler drift of one body (integrator_whfast.c:241-390),
 // in place.  Returns a bit mask: 1 = the step exceeded one orbital period
 // (the reference warns once), 4 = the quartic iteration ran, 8 = the
 // bisection fallback ran.
-__device__ int kepler_solver(double& px, double& py, double& pz, double& pvx, double& pvy, double& pvz, double mu,
+__device__ __forceinline__ int kepler_solver(double& px, double& py, double& pz, double& pvx, double& pvy, double& pvz, double mu,
                              double dt) {
     const double x0 = px, y0 = py, z0 = pz, vx0 = pvx, vy0 = pvy, vz0 = pvz;
     int flags = 0;
@@ -173,6 +178,7 @@ __device__ int kepler_solver(double& px, double& py, double& pz, double& pvx, do
         // at most four long (the iterates are a deterministic sequence, so
         // every repeat spans a multiple of the cycle length); a longer cycle
         // is located by quartic_first_repeat and replayed.  No history array.
+        // (X is X_63 when the loop ends without a repeat.)
         flags |= 4;
         X = beta * dt / mu;
         const double Xq0 = X;
@@ -189,7 +195,7 @@ __device__ int kepler_solver(double& px, double& py, double& pz, double& pvx, do
             w0 = X;
         }
         if (!converged) {
-            const int n = quartic_first_repeat(Xq0, beta, r0, eta0, zeta0, dt);
+            const int n = quartic_first_repeat(Xq0, X, beta, r0, eta0, zeta0, dt);
             if (n > 0) {
                 X = Xq0;
                 for (int k = 0; k < n; k++) X = quartic_next(X, beta, r0, eta0, zeta0, dt, Gs);
@@ -1053,6 +1059,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
                 // positions and accelerations refreshed before each kick
                 const double kick[5] = {-dt / 6., dt / 6., dt, -dt / 6., dt / 6.};
                 const double drift[4] = {-dt / 4., dt / 8., -dt / 8., dt / 4.};
+#pragma unroll
                 for (int q = 0; q < 5; q++) {
                     if (q > 0) {
                         to_inertial(act, false);

@@ -27,6 +27,9 @@
 #include "rebound.h"
 #include "tree.h"
 #include "boundary.h"
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 static const char* outdir = ".";
 /* Raw state dumps (x y z vx vy vz of body 0..N-1, in order) go to this
@@ -1139,9 +1142,9 @@ static void gen_integrate(void){
 
 /* The merge resolver on a stepping simulation, DIRECT and TREE searches,
  * periodic box with one ring of ghost boxes, energy offset tracked.  Written
- * by the OpenMP build: with ghost boxes its per-target force sums are the
- * order the device follows (the serial build's half-matrix loop adds the
- * ghost-box terms in another order). */
+ * by the OpenMP build on one thread: with ghost boxes its per-target force
+ * sums are the order the device follows (the serial build's half-matrix loop
+ * adds the ghost-box terms in another order). */
 static void gen_integrate_merge(void){
     struct reb_simulation* r;
     int st;
@@ -1253,6 +1256,12 @@ int main(int argc, char** argv){
         gen_plummer_omp();
         gen_ghostbox_basic("omp");
         gen_boundaries();
+#ifdef _OPENMP
+        /* the TREE collision search appends collisions from every thread in
+         * arrival order (collision.c:228,475); one thread gives the serial
+         * order, which the device search reproduces */
+        omp_set_num_threads(1);
+#endif
         gen_integrate_merge();
         return 0;
     }
