@@ -137,6 +137,23 @@ REB_API int reb_amd_gravity_direct(const double* x, const double* y, const doubl
                                    double G, double softening, int precision, int compensated,
                                    void* workspace, void* stream);
 
+/* reb_amd_gravity_direct without a workspace and without any allocation:
+ * the fast modes' partial sums go to a fixed device array of
+ * reb_amd_gravity_noalloc_max_bytes() bytes (32 MiB, one per device, held
+ * until the process ends; enough for N up to about 650000 in one call).  No
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments, kernels and results as
+ * reb_amd_gravity_direct; returns 4 (with reb_amd_last_error set) when N needs
+ * more partial-sum space than the array holds.  Every call on a device shares
+ * that device's array, so calls that overlap on one device race. */
+REB_API size_t reb_amd_gravity_noalloc_max_bytes(void);
+REB_API int reb_amd_gravity_direct_noalloc(const double* x, const double* y, const double* z, const double* m,
+                                           double* ax, double* ay, double* az,
+                                           size_t N, size_t N_active, int testparticle_type, int ignore_terms,
+                                           double G, double softening, int precision, int compensated,
+                                           void* stream);
+
 /* Kepler drift of n independent bodies (integrator_whfast.c:241-427): state
  * is n rows of (x,y,z,vx,vy,vz) in host memory, updated in place; mu and dt
  * per body.  Runs on the GPU and waits for it. */

@@ -177,6 +177,9 @@ def declare(lib):
         "reb_amd_gravity_workspace_bytes": (c_size_t, [c_size_t, c_int]),
         "reb_amd_gravity_direct": (c_int, [c_void_p] * 7 + [c_size_t, c_size_t, c_int, c_int, c_double, c_double,
                                                            c_int, c_int, c_void_p, c_void_p]),
+        "reb_amd_gravity_noalloc_max_bytes": (c_size_t, []),
+        "reb_amd_gravity_direct_noalloc": (c_int, [c_void_p] * 7 + [c_size_t, c_size_t, c_int, c_int, c_double,
+                                                                   c_double, c_int, c_int, c_void_p]),
         "reb_amd_kepler_solver_batch": (c_int, [POINTER(c_double), POINTER(c_double), POINTER(c_double), c_size_t]),
         "reb_amd_tree_dump": (ctypes.c_long, [S, POINTER(c_double), ctypes.c_long]),
         "reb_amd_tree_interactions": (ctypes.c_longlong, [S]),

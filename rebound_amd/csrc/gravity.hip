@@ -22,6 +22,8 @@
 // Sources are staged per 256-wide tile into LDS as double4 (x, y, z, m) and
 // read back as broadcasts (every lane of a wavefront reads the same entry).
 #include <cmath>
+#include <mutex>
+#include <string>
 
 #include "internal.h"
 
@@ -1495,6 +1497,77 @@ REB_API int reb_amd_gravity_direct(const double* x, const double* y, const doubl
     gl.g[0] = GB3{0., 0., 0.};
     return direct_dispatch(x, y, z, m, ax, ay, az, (int)N, (int)N_active, testparticle_type, ignore_terms, G,
                            softening * softening, precision, compensated, gl, static_cast<double*>(workspace), st);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- no-allocation variant
+// The chunk partial sums of the fast modes live in a fixed device array
+// instead of a caller workspace.  The name `scratch` is a host-side stand-in
+// in host code; its device address is looked up once per device
+// (hipGetSymbolAddress with that device current) and only that address is
+// passed to kernels.  Every partial-sum entry the reduction reads is written
+// by the pair kernel of the same call first, so nothing needs clearing.
+// Calls that overlap on one device share the array and race.
+namespace rbamd {
+
+static constexpr size_t kScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices];
+
+static void* scratch_on_current_device() {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        set_last_error("reb_amd_gravity_direct_noalloc: no current HIP device");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr) {
+            set_last_error(std::string("reb_amd_gravity_direct_noalloc: hipGetSymbolAddress failed: ") +
+                           hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
+}
+
+}  // namespace rbamd
+
+extern "C" {
+
+REB_API size_t reb_amd_gravity_noalloc_max_bytes(void) { return kScratchBytes; }
+
+REB_API int reb_amd_gravity_direct_noalloc(const double* x, const double* y, const double* z, const double* m,
+                                           double* ax, double* ay, double* az, size_t N, size_t N_active,
+                                           int testparticle_type, int ignore_terms, double G, double softening,
+                                           int precision, int compensated, void* stream) {
+    if (N > (size_t)INT32_MAX / 8) {
+        set_last_error("reb_amd_gravity_direct_noalloc: N too large");
+        return 2;
+    }
+    if (N_active == SIZE_MAX || N_active > N) N_active = N;
+    const size_t need = reb_amd_gravity_workspace_bytes(N, precision);
+    if (need > kScratchBytes) {
+        set_last_error("reb_amd_gravity_direct_noalloc: N=" + std::to_string(N) + " needs " + std::to_string(need) +
+                       " bytes of partial sums, above the fixed " + std::to_string(kScratchBytes) +
+                       "-byte device array; use reb_amd_gravity_direct with a workspace");
+        return 4;
+    }
+    void* ws = scratch_on_current_device();
+    if (ws == nullptr) return 5;
+    hipStream_t st = (hipStream_t)stream;
+    GBList gl;
+    gl.n = 1;
+    gl.g[0] = GB3{0., 0., 0.};
+    return direct_dispatch(x, y, z, m, ax, ay, az, (int)N, (int)N_active, testparticle_type, ignore_terms, G,
+                           softening * softening, precision, compensated, gl, static_cast<double*>(ws), st);
 }
 
 }  // extern "C"
