@@ -34,11 +34,25 @@ FLOP_PER_PAIR = {"fp64": 20, "compensated": 28, "split": 28, "exact": 20}
 # into both bodies) = 27; split precision 27 + 8 (the FP32 part counted apart
 # as in the one-sided kernel's 28 = 15 FP64 + 13 FP32 per ordered pair).
 FLOP_PER_SYM_PAIR = {"fp64": 27, "split": 35}
-# HBM bytes per launch of k_direct_fast<FM_FP64> at N=65536 from rocprofv3
-# --pmc: FETCH_SIZE 8229.9 KiB x 2.0 + WRITE_SIZE 24576 KiB x 1.0, the factors
-# being what the counters read for a device-to-device copy of known size
-# (profiles/direct_fp64_plummer65536.md).
-DIRECT_TRAFFIC_BYTES_N65536 = (8229.9 * 2.0 + 24576.0 * 1.0) * 1024.0
+
+
+def profiled(what, kernel, workload_tag):
+    """A per-launch figure of `kernel` on this workload recorded in the
+    committed rocprofv3 summary profiles/<workload_tag>.md on a line
+    `<what> <kernel>: <value>`: `traffic_bytes_per_launch` (FETCH_SIZE and
+    WRITE_SIZE, each scaled by what the counter reads for a device-to-device
+    copy of known size) or `hw_flops_per_launch` (FP64 VALU FLOPs, FMA = 2).
+    (None, None) when not recorded."""
+    path = os.path.join(ROOT, "profiles", workload_tag + ".md")
+    if not os.path.exists(path):
+        return None, None
+    key = "%s %s:" % (what, kernel)
+    with open(path) as f:
+        for line in f:
+            line = line.strip().strip("`")
+            if line.startswith(key):
+                return float(line[len(key):].split()[0]), os.path.relpath(path, ROOT)
+    return None, None
 
 
 def log(*a):
@@ -169,6 +183,35 @@ def accuracy(amd, args):
     return out
 
 
+def host_cpu():
+    """Model name, physical cores and logical CPUs of this host (/proc/cpuinfo),
+    and the CPUs this process may run on."""
+    model, cores, logical = None, set(), 0
+    try:
+        with open("/proc/cpuinfo") as f:
+            phys = core = None
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model is None:
+                    model = v
+                elif k == "processor":
+                    logical += 1
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                    cores.add((phys, core))
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"cpu_model": model, "physical_cores": len(cores) or None, "logical_cpus": logical or None,
+            "usable_cpus": usable}
+
+
 def cpu_baseline(args):
     exe = os.path.join(ROOT, "oracle", "_ref", "ref_bench")
     if not os.path.exists(exe):
@@ -184,6 +227,7 @@ def cpu_baseline(args):
         return None
     res = json.loads(out.stdout.strip().splitlines()[-1])
     return {"value": res["pairs_per_s"] / 1e9, "unit": "Gpair/s", "cores": res["threads"], "kind": "reference",
+            "host": host_cpu(),
             "sample": "reference OpenMP build (gravity.c:215-232 per-target loop), Plummer N=%d, %d leapfrog "
                       "steps after 1 warm-up step, %.1f s" % (args.n, steps, res["seconds"])}
 
@@ -242,6 +286,19 @@ def ensemble_leg(amd, args):
                 "%.3e" % r2.mean() == "%.3e" % cpu["mean_rel_energy_error"])
             res["cpu_baseline"] = cpu
             res["speedup_vs_cpu"] = res["value"] / cpu["value"]
+            # the full horizon on a sample: the first `ens_full_systems`
+            # systems, integrated for all `ens_steps` steps by the reference
+            full = ensemble_cpu(args, systems=args.ens_full_systems, steps=args.ens_steps)
+            if full is not None:
+                k = args.ens_full_systems
+                gmax, gmean = float(rel[:k].max()), float(rel[:k].mean())
+                full["gpu"] = {"max_rel_energy_error": gmax, "mean_rel_energy_error": gmean}
+                full["energy_error_matches"] = bool("%.3e" % gmax == "%.3e" % full["max_rel_energy_error"] and
+                                                    "%.3e" % gmean == "%.3e" % full["mean_rel_energy_error"])
+                res["full_horizon_check"] = full
+                log("ensemble full horizon (%d systems, %d steps): cpu max %.3e mean %.3e, gpu max %.3e mean "
+                    "%.3e; match: %s" % (k, args.ens_steps, full["max_rel_energy_error"],
+                                         full["mean_rel_energy_error"], gmax, gmean, full["energy_error_matches"]))
             # algorithmic work per system-step, counted by the CPU restatement
             # integrating the first 512 systems for 200 steps
             sys.path.insert(0, ROOT)
@@ -259,13 +316,15 @@ def ensemble_leg(amd, args):
     return res
 
 
-def ensemble_cpu(args):
+def ensemble_cpu(args, systems=None, steps=None):
     exe = os.path.join(ROOT, "oracle", "_ref", "ref_bench")
     if not os.path.exists(exe):
         return None
+    systems = args.ens_systems if systems is None else systems
+    steps = args.ens_cpu_steps if steps is None else steps
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
     env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
-    cmd = [exe, "ensemble", str(args.ens_systems), str(args.ens_cpu_steps), str(args.ens_seed), "0", "2", "8"]
+    cmd = [exe, "ensemble", str(systems), str(steps), str(args.ens_seed), "0", "2", "8"]
     log("ensemble cpu baseline:", " ".join(cmd), "threads", threads)
     out = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=900)
     if out.returncode != 0:
@@ -274,8 +333,7 @@ def ensemble_cpu(args):
     r = json.loads(out.stdout.strip().splitlines()[-1])
     return {"value": r["system_steps_per_s"], "unit": "system-steps/s", "cores": r["threads"], "kind": "reference",
             "sample": "reference OpenMP build, omp parallel for over the same %d systems (one reb_simulation "
-                      "each, reb_simulation_steps), %d steps, %.1f s" % (args.ens_systems, args.ens_cpu_steps,
-                                                                          r["seconds"]),
+                      "each, reb_simulation_steps), %d steps, %.1f s" % (systems, steps, r["seconds"]),
             "max_rel_energy_error": r["max_rel_energy_error"], "mean_rel_energy_error": r["mean_rel_energy_error"]}
 
 
@@ -450,9 +508,15 @@ def sheet_leg(amd, args):
     L.reb_amd_set_resident(sim.ptr, 1)
     sim.steps(1)                                   # warm-up (allocations, first tree)
     L.reb_amd_synchronize_device(sim.ptr)
+    # the CPU baseline's window: the steps right after the warm-up step (the
+    # densest collision steps), timed alone, then the rest of the run
     n0 = sim.s.collisions_log_n
     t0 = time.perf_counter()
-    sim.steps(args.sheet_steps)
+    sim.steps(args.sheet_cpu_steps)
+    L.reb_amd_synchronize_device(sim.ptr)
+    wall_window = time.perf_counter() - t0
+    ncoll_window = sim.s.collisions_log_n - n0
+    sim.steps(args.sheet_steps - args.sheet_cpu_steps)
     L.reb_amd_synchronize_device(sim.ptr)
     wall = time.perf_counter() - t0
     ncoll = sim.s.collisions_log_n - n0
@@ -463,6 +527,9 @@ def sheet_leg(amd, args):
            "value": args.sheet_steps / wall, "unit": "steps/s", "steps": args.sheet_steps, "seconds": wall,
            "ms_per_step": 1e3 * wall / args.sheet_steps, "collisions_per_step": ncoll / args.sheet_steps,
            "tree_interactions_last_step": int(inter),
+           "window": {"steps": args.sheet_cpu_steps, "steps_per_s": args.sheet_cpu_steps / wall_window,
+                      "collisions_per_step": ncoll_window / args.sheet_cpu_steps,
+                      "note": "steps 2..%d, the CPU baseline's window" % (1 + args.sheet_cpu_steps)},
            "config": {"N": args.sheet_n, "root_size": args.sheet_root, "n_root": args.sheet_nroot,
                       "ghost_boxes": 2, "opening_angle2": 0.25, "seed": args.sheet_seed}}
     log("sheet: %.2f steps/s (%.1f ms/step), %.0f collisions/step, %d tree interactions/step" % (
@@ -482,10 +549,13 @@ def sheet_leg(amd, args):
                 res["cpu_baseline"] = {
                     "value": r["steps_per_s"], "unit": "steps/s", "cores": r["threads"], "kind": "reference",
                     "sample": "reference OpenMP build, the same sheet, %d steps after 1 warm-up step, %.1f s "
-                              "(%.0f collisions/step)" % (r["steps"], r["seconds"], r["collisions_per_step"])}
-                res["speedup_vs_cpu"] = res["value"] / r["steps_per_s"]
-                log("sheet cpu: %.3f steps/s on %d threads -> GPU %.0fx" % (r["steps_per_s"], r["threads"],
-                                                                            res["speedup_vs_cpu"]))
+                              "(%.0f collisions/step resolved, %.0f pairs/step found)" % (
+                                  r["steps"], r["seconds"], r["collisions_per_step"],
+                                  r.get("resolver_calls_per_step", float("nan")))}
+                res["cpu_baseline"]["collisions_per_step"] = r["collisions_per_step"]
+                res["speedup_vs_cpu"] = res["window"]["steps_per_s"] / r["steps_per_s"]
+                log("sheet cpu: %.3f steps/s on %d threads; GPU on the same steps %.1f steps/s -> %.0fx" % (
+                    r["steps_per_s"], r["threads"], res["window"]["steps_per_s"], res["speedup_vs_cpu"]))
             else:
                 log(out.stderr)
     return res
@@ -525,6 +595,8 @@ def main():
     ap.add_argument("--ens-seed", type=int, default=1)
     ap.add_argument("--ens-probe", type=int, default=500, help="steps per autotuner probe")
     ap.add_argument("--ens-cpu-steps", type=int, default=500)
+    ap.add_argument("--ens-full-systems", type=int, default=256,
+                    help="systems the reference integrates for the whole request (energy-error check)")
     ap.add_argument("--serve-jobs", type=int, default=200)
     ap.add_argument("--serve-rate", type=float, default=20.0, help="job arrivals per second")
     ap.add_argument("--serve-seed", type=int, default=7)
@@ -583,6 +655,9 @@ def main():
     small = None if (args.quick or args.only == "direct") else small_leg(amd, args)
 
     achieved = head["algorithmic_tflops"]
+    tag = "direct_fp64_plummer65536"
+    traffic, traffic_src = profiled("traffic_bytes_per_launch", head["kernel"], tag) if args.n == 65536 else (None, None)
+    hw_flops, _ = profiled("hw_flops_per_launch", head["kernel"], tag) if args.n == 65536 else (None, None)
     line = {
         "metric": "Gpair-interactions/s at N=65536 (direct gravity, FP64)",
         "value": head["gpairs_per_s"],
@@ -601,8 +676,12 @@ def main():
             "integrator": "leapfrog", "gravity": "basic", "precision": "fp64"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS,
-                     "traffic": DIRECT_TRAFFIC_BYTES_N65536 if args.n == 65536 else None,
+                     "traffic": traffic,
                      "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)",
+                     "traffic_source": traffic_src,
+                     "hw_tflops": (hw_flops / (1e-3 * head["kernel_ms"]) / 1e12) if hw_flops else None,
+                     "hw_tflops_note": "FP64 VALU FLOPs per launch counted by rocprofv3 (SQ_INSTS_VALU_FLOPS_FP64, "
+                                       "FMA = 2) over this run's kernel time",
                      "kernel": head["kernel"], "kernel_ms": head["kernel_ms"],
                      "work_unit": head["work_unit"], "flop_per_unit": head["flop_per_unit"],
                      "units_per_launch": head["units_per_launch"]},

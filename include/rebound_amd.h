@@ -221,10 +221,15 @@ REB_API int reb_amd_ensemble_flags(struct reb_amd_ensemble* e, unsigned int* out
 /* Launch shape: persistent workgroups (0 = default) and steps per launch
  * (0 = a whole request in one launch). */
 REB_API int reb_amd_ensemble_set_launch(struct reb_amd_ensemble* e, int grid, long chunk);
+/* Wave packing: about `tasks` wave tasks (0: the densest, 64 lanes filled per
+ * wavefront).  More tasks put fewer systems in each wavefront and more
+ * wavefronts on the chip; results are unchanged.  Returns the task count or -1. */
+REB_API int reb_amd_ensemble_set_packing(struct reb_amd_ensemble* e, int tasks);
 REB_API int reb_amd_ensemble_get_launch(const struct reb_amd_ensemble* e, int* grid, long* chunk, int* ntasks,
                                         int* max_resident);
-/* Times candidate launch shapes for probe_steps steps each (state restored
- * afterwards) and keeps the fastest; best_ms receives its time. */
+/* Times candidate launch shapes (packing, grid, chunk) for probe_steps steps
+ * each (state restored afterwards) and keeps the fastest; best_ms receives
+ * its time. */
 REB_API int reb_amd_ensemble_tune(struct reb_amd_ensemble* e, long probe_steps, double* best_ms);
 REB_API const char* reb_amd_ensemble_error(const struct reb_amd_ensemble* e);
 
@@ -264,6 +269,26 @@ REB_API long reb_amd_server_submit_whfast(struct reb_amd_server* s, size_t n_bod
 REB_API long reb_amd_server_submit_nbody(struct reb_amd_server* s, size_t n, const struct reb_particle* bodies,
                                          double G, double softening, double dt, uint64_t n_steps, int gravity,
                                          int precision);
+/* A simulation as a job that continues from its state, time and step count
+ * for n_steps steps: WHFast in democratic heliocentric coordinates (1..65
+ * bodies, no correctors, default kernel, BASIC gravity, no softening) or
+ * order-2 leapfrog with BASIC or COMPENSATED direct gravity in the
+ * simulation's REB_AMD_PRECISION.  A WHFast simulation that carries its
+ * heliocentric state (p_jh, e.g. one read from a checkpoint) continues from
+ * it.  The simulation is not modified.  Job id or -1.  Serves resumption from
+ * reb_amd_server_checkpoint; no counterpart in the reference. */
+REB_API long reb_amd_server_submit_simulation(struct reb_amd_server* s, struct reb_simulation* r, uint64_t n_steps);
+/* Writes the job's current state to `filename` as one Simulationarchive
+ * snapshot (appended when the file exists; simulationarchive.c:411-615
+ * format, WHFast's p_jh and synchronization flag included): a pending job as
+ * submitted, a running one at its next pool tick, batch round or step-segment
+ * boundary, a finished one as it ended.  Blocks until written.  Stepping a
+ * simulation read from the snapshot (reb_simulation_create_from_file) for the
+ * job's remaining steps, or submitting it with
+ * reb_amd_server_submit_simulation, reproduces the job's result bit for bit.
+ * 0 on success, 1 (message in reb_amd_last_error) for an unknown or failed
+ * job or a file that cannot be written.  No counterpart in the reference. */
+REB_API int reb_amd_server_checkpoint(struct reb_amd_server* s, long job, const char* filename);
 /* Device-memory footprint of a job, as admission counts it. */
 REB_API size_t reb_amd_server_whfast_bytes(size_t n_bodies);
 REB_API size_t reb_amd_server_nbody_bytes(size_t n, int precision);
@@ -275,6 +300,10 @@ REB_API int reb_amd_server_wait(struct reb_amd_server* s, long job, double timeo
  * the server was created), then forgets the job.  0 on success. */
 REB_API int reb_amd_server_result(struct reb_amd_server* s, long job, struct reb_particle* out, size_t n,
                                   double* energy2, double* times3);
+/* Why a job failed: an error the simulation reported, a device failure of
+ * the pool or batch it ran in, or "non-finite coordinates after N steps"
+ * (a job whose bodies are not all finite at its end fails; other jobs are
+ * not affected). */
 REB_API const char* reb_amd_server_job_error(struct reb_amd_server* s, long job);
 /* out[0..9]: jobs held, pending, running, done, failed, bytes in use, peak
  * bytes in use, budget, pool ticks, pool busy seconds. */

@@ -207,6 +207,7 @@ def declare(lib):
         "reb_amd_ensemble_energy": (c_int, [c_void_p, POINTER(c_double)]),
         "reb_amd_ensemble_flags": (c_int, [c_void_p, POINTER(ctypes.c_uint)]),
         "reb_amd_ensemble_set_launch": (c_int, [c_void_p, c_int, ctypes.c_long]),
+        "reb_amd_ensemble_set_packing": (c_int, [c_void_p, c_int]),
         "reb_amd_ensemble_get_launch": (c_int, [c_void_p, POINTER(c_int), POINTER(ctypes.c_long), POINTER(c_int),
                                                 POINTER(c_int)]),
         "reb_amd_ensemble_tune": (c_int, [c_void_p, ctypes.c_long, POINTER(c_double)]),
@@ -218,6 +219,8 @@ def declare(lib):
         "reb_amd_server_submit_nbody": (ctypes.c_long, [c_void_p, c_size_t, POINTER(Particle), c_double, c_double,
                                                         c_double, ctypes.c_uint64, c_int, c_int]),
         "reb_amd_server_status": (c_int, [c_void_p, ctypes.c_long]),
+        "reb_amd_server_submit_simulation": (ctypes.c_long, [c_void_p, S, ctypes.c_uint64]),
+        "reb_amd_server_checkpoint": (c_int, [c_void_p, ctypes.c_long, c_char_p]),
         "reb_amd_server_whfast_bytes": (c_size_t, [c_size_t]),
         "reb_amd_server_nbody_bytes": (c_size_t, [c_size_t, c_int]),
         "reb_amd_server_wait": (c_int, [c_void_p, ctypes.c_long, c_double]),

@@ -711,7 +711,7 @@ bool compare(const std::string& buf1, const std::string& buf2, std::string* text
 }  // namespace
 
 // The host state a snapshot must see: particles (and WHFast's p_jh) current.
-static bool prepare_host(struct reb_simulation* r) {
+bool prepare_host(struct reb_simulation* r) {
     if (!ensure_host(r)) return false;
     if (r->integrator.name && strcmp(r->integrator.name, "whfast") == 0) whfast_refresh_host(r);
     return true;

@@ -51,6 +51,9 @@ struct Ensemble {
     int kernel = 0;
     DevBuf body, ph, sys, tasks, order, flags, counter, energy;
     int ntasks = 0;
+    int ntasks_dense = 0;            // wave tasks at the densest packing
+    int packing = 0;                 // requested task count (0: densest)
+    std::vector<int> order_h, nb_h;  // host copies: task order, bodies per system
     int grid = 0;
     long chunk = 0;
     bool first_step_done = false;
@@ -141,6 +144,55 @@ static GraphEntry* graph_for(Ensemble* e, long nsteps, int did_modify, int sync_
     slot->sync_at_end = sync_at_end;
     slot->last_use = ++e->use_clock;
     return slot;
+}
+
+// Wave tasks over the systems in `order` (bucketed by body count).  At the
+// densest packing a task holds 64 / (lanes per system) systems of one bucket;
+// with target > 0 every bucket is cut into about target / densest times as
+// many tasks, fewer systems each, so that more wavefronts share the chip
+// (latency hiding when there are few wavefronts per SIMD).  The systems and
+// their arithmetic are the same either way.
+static std::vector<WHTask> make_tasks(const std::vector<int>& order, const std::vector<int>& nb, int target,
+                                      int* dense_count) {
+    struct Bucket {
+        size_t k0, n;
+        int Gs;
+    };
+    std::vector<Bucket> buckets;
+    for (size_t k = 0; k < order.size();) {
+        size_t m = 0;
+        while (k + m < order.size() && nb[order[k + m]] == nb[order[k]]) m++;
+        buckets.push_back(Bucket{k, m, whfast_group_size(nb[order[k]] - 1)});
+        k += m;
+    }
+    size_t dense = 0;
+    for (const Bucket& b : buckets) dense += (b.n + (64 / b.Gs) - 1) / (64 / b.Gs);
+    if (dense_count) *dense_count = (int)dense;
+    std::vector<WHTask> tasks;
+    for (const Bucket& b : buckets) {
+        const size_t per = (size_t)(64 / b.Gs);
+        size_t nt = (b.n + per - 1) / per;
+        if (target > 0 && (size_t)target > dense) nt = std::min(b.n, (nt * (size_t)target + dense - 1) / dense);
+        const size_t each = (b.n + nt - 1) / nt;
+        for (size_t q = 0; q < b.n; q += each)
+            tasks.push_back(WHTask{b.Gs, (int)(b.k0 + q), (int)std::min(each, b.n - q)});
+    }
+    return tasks;
+}
+
+// Re-packs the wave tasks (target 0: densest); drops the captured graphs.
+static bool set_packing(Ensemble* e, int target) {
+    if (target == e->packing && e->ntasks > 0) return true;
+    const std::vector<WHTask> tasks = make_tasks(e->order_h, e->nb_h, target, &e->ntasks_dense);
+    if (!ENS_HIP(e, hipMemcpyAsync(e->tasks.p, tasks.data(), sizeof(WHTask) * tasks.size(), hipMemcpyHostToDevice,
+                                   e->stream)) ||
+        !ENS_HIP(e, hipStreamSynchronize(e->stream)))
+        return false;
+    drop_graphs(e);
+    e->packing = target;
+    e->ntasks = (int)tasks.size();
+    e->grid = std::min(e->ntasks, e->max_resident);
+    return true;
 }
 
 static bool launch(Ensemble* e, long nsteps, int did_modify, int sync_at_end) {
@@ -248,19 +300,15 @@ REB_API struct reb_amd_ensemble* reb_amd_ensemble_create(size_t n_systems, const
     }
     // Bucket by planet count, largest first (the longest tasks start first).
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return sys[a].N > sys[b].N; });
-    std::vector<WHTask> tasks;
-    for (size_t k = 0; k < n_systems;) {
-        const int Gs = whfast_group_size(sys[order[k]].N - 1);
-        const int per = 64 / Gs;
-        size_t m = 0;
-        while (k + m < n_systems && (int)m < per && sys[order[k + m]].N == sys[order[k]].N) m++;
-        tasks.push_back(WHTask{Gs, (int)k, (int)m});
-        k += m;
-    }
+    e->order_h = order;
+    e->nb_h.resize(n_systems);
+    for (size_t q = 0; q < n_systems; q++) e->nb_h[q] = sys[q].N;
+    const std::vector<WHTask> tasks = make_tasks(order, e->nb_h, 0, &e->ntasks_dense);
     e->ntasks = (int)tasks.size();
+    e->packing = 0;
     std::vector<unsigned int> flags(n_systems, 1u);   // freshly added: synchronized
     bool ok = e->body.ensure(sizeof(double) * hb.size()) && e->ph.ensure(sizeof(double) * hp.size()) &&
-              e->sys.ensure(sizeof(WHSys) * n_systems) && e->tasks.ensure(sizeof(WHTask) * tasks.size()) &&
+              e->sys.ensure(sizeof(WHSys) * n_systems) && e->tasks.ensure(sizeof(WHTask) * n_systems) &&
               e->order.ensure(sizeof(int) * n_systems) && e->flags.ensure(sizeof(unsigned int) * n_systems) &&
               e->counter.ensure(256) && e->energy.ensure(sizeof(double) * n_systems);
     if (!ok) {
@@ -431,6 +479,14 @@ REB_API int reb_amd_ensemble_set_launch(struct reb_amd_ensemble* h, int grid, lo
     return 0;
 }
 
+// Wave packing: about `tasks` wave tasks (0: the densest, every wavefront
+// filled).  Returns the resulting task count, or -1.
+REB_API int reb_amd_ensemble_set_packing(struct reb_amd_ensemble* h, int tasks) {
+    Ensemble* e = &h->e;
+    if (tasks < 0 || !ENS_HIP(e, hipSetDevice(e->device)) || !set_packing(e, tasks)) return -1;
+    return e->ntasks;
+}
+
 REB_API int reb_amd_ensemble_get_launch(const struct reb_amd_ensemble* h, int* grid, long* chunk, int* ntasks,
                                         int* max_resident) {
     if (grid) *grid = h->e.grid;
@@ -440,9 +496,9 @@ REB_API int reb_amd_ensemble_get_launch(const struct reb_amd_ensemble* h, int* g
     return 0;
 }
 
-// Autotuner: times `probe_steps` steps of every candidate (grid, chunk) on the
-// ensemble's own state, restoring the state after each probe, and keeps the
-// fastest shape.  The ensemble is left exactly as it was before the call.
+// Autotuner: times `probe_steps` steps of every candidate (packing, grid,
+// chunk) on the ensemble's own state, restoring the state after each probe,
+// and keeps the fastest shape.  The ensemble is left exactly as it was before the call.
 REB_API int reb_amd_ensemble_tune(struct reb_amd_ensemble* h, long probe_steps, double* best_ms) {
     Ensemble* e = &h->e;
     if (probe_steps < 4) probe_steps = 4;
@@ -464,39 +520,52 @@ REB_API int reb_amd_ensemble_tune(struct reb_amd_ensemble* h, long probe_steps, 
     const uint64_t done = e->steps_done;
     const int grid0 = e->grid;
     const long chunk0 = e->chunk;
-    std::vector<int> grids = {std::min(e->ntasks, e->max_resident)};
-    for (int g : {e->ntasks / 2, e->ntasks / 4, 1024, 512})
-        if (g > 0 && g < grids[0] && std::find(grids.begin(), grids.end(), g) == grids.end()) grids.push_back(g);
+    const int pack0 = e->packing;
+    // packings: the densest, and as many tasks as wavefronts fit at once
+    std::vector<int> packs = {0};
+    if (e->max_resident > e->ntasks_dense + e->ntasks_dense / 8) packs.push_back(e->max_resident);
     const std::vector<long> chunks = {0, std::max(1L, probe_steps / 4), std::max(1L, probe_steps / 16)};
     hipEvent_t a = nullptr, b = nullptr;
     ok = ok && ENS_HIP(e, hipEventCreate(&a)) && ENS_HIP(e, hipEventCreate(&b));
     double best = 1e300, best_raw = 0.;
-    int bg = grid0;
+    int bg = grid0, bp = pack0;
     long bc = chunk0;
-    for (int g : grids) {
-        for (long c : chunks) {
-            if (!ok) break;
-            reb_amd_ensemble_set_launch(h, g, c);
-            ok = ens_steps(e, (uint64_t)probe_steps);     // captures the graphs, warms up
-            ok = ok && copy_state(e->body.p, e->ph.p, e->flags.p, sb.p, sp.p, sf.p);
-            e->first_step_done = first_done;
-            ok = ok && ENS_HIP(e, hipEventRecord(a, e->stream)) && ens_steps(e, (uint64_t)probe_steps) &&
-                 ENS_HIP(e, hipEventRecord(b, e->stream)) && ENS_HIP(e, hipEventSynchronize(b));
-            float ms = 0.f;
-            ok = ok && ENS_HIP(e, hipEventElapsedTime(&ms, a, b));
-            // A chunked request waits at every chunk for its slowest system;
-            // over a long request the slow systems differ from chunk to chunk,
-            // which a short probe cannot see, so chunking must win clearly.
-            const double score = c > 0 ? ms / 0.85 : ms;
-            if (ok && score < best) {
-                best = score;
-                best_raw = ms;
-                bg = g;
-                bc = c;
+    // one candidate: warm-up run (captures the graphs), restore, timed run, restore
+    auto probe = [&](int g, long c, float* ms) {
+        reb_amd_ensemble_set_launch(h, g, c);
+        bool fine = ens_steps(e, (uint64_t)probe_steps);
+        fine = fine && copy_state(e->body.p, e->ph.p, e->flags.p, sb.p, sp.p, sf.p);
+        e->first_step_done = first_done;
+        fine = fine && ENS_HIP(e, hipEventRecord(a, e->stream)) && ens_steps(e, (uint64_t)probe_steps) &&
+               ENS_HIP(e, hipEventRecord(b, e->stream)) && ENS_HIP(e, hipEventSynchronize(b)) &&
+               ENS_HIP(e, hipEventElapsedTime(ms, a, b));
+        fine = fine && copy_state(e->body.p, e->ph.p, e->flags.p, sb.p, sp.p, sf.p);
+        e->first_step_done = first_done;
+        e->steps_done = done;
+        return fine;
+    };
+    for (int pk : packs) {
+        ok = ok && set_packing(e, pk);
+        std::vector<int> grids = {std::min(e->ntasks, e->max_resident)};
+        for (int g : {e->ntasks / 2, e->ntasks / 4, 1024, 512})
+            if (g > 0 && g < grids[0] && std::find(grids.begin(), grids.end(), g) == grids.end()) grids.push_back(g);
+        for (int g : grids) {
+            for (long c : chunks) {
+                float ms = 0.f;
+                ok = ok && probe(g, c, &ms);
+                // A chunked request waits at every chunk for its slowest system;
+                // over a long request the slow systems differ from chunk to
+                // chunk, which a short probe cannot see, so chunking must win
+                // clearly.
+                const double score = c > 0 ? ms / 0.85 : ms;
+                if (ok && score < best) {
+                    best = score;
+                    best_raw = ms;
+                    bg = g;
+                    bc = c;
+                    bp = pk;
+                }
             }
-            ok = ok && copy_state(e->body.p, e->ph.p, e->flags.p, sb.p, sp.p, sf.p);
-            e->first_step_done = first_done;
-            e->steps_done = done;
         }
     }
     ok = ENS_HIP(e, hipStreamSynchronize(e->stream)) && ok;
@@ -505,6 +574,7 @@ REB_API int reb_amd_ensemble_tune(struct reb_amd_ensemble* h, long probe_steps, 
     sb.release();
     sp.release();
     sf.release();
+    ok = set_packing(e, bp) && ok;
     reb_amd_ensemble_set_launch(h, bg, bc);
     if (best_ms) *best_ms = best_raw;
     return ok ? 0 : 1;

@@ -721,7 +721,9 @@ __global__ __launch_bounds__(256) void k_direct_sym(const double* __restrict__ x
                                                     int N, int nsb, double soft2, const int2* __restrict__ tasks,
                                                     int ntasks_done, double* __restrict__ ipart,
                                                     double* __restrict__ jpart) {
-    __shared__ double4 jt[2][kSymJB];              // J-block sources (x, y, z, m), double-buffered
+    // J-block sources (x, y, z, m), double-buffered, each stored twice in a
+    // row so that lane k reads entry k + 64 - s at step s without wrapping
+    __shared__ double4 jt[2][2 * kSymJB];
     __shared__ double jred[2][4][3][kSymJB];       // per-wave source sums of a J-block
     const int2 tk = tasks[ntasks_done + blockIdx.x];
     const int A = tk.x, B = tk.y;
@@ -748,7 +750,9 @@ __global__ __launch_bounds__(256) void k_direct_sym(const double* __restrict__ x
     if (tid < kSymJB) {
         const int j0 = jbase + tid;
         const bool ok0 = !CHECK || j0 < N;
-        jt[0][tid] = ok0 ? make_double4(x[j0], y[j0], z[j0], m[j0]) : make_double4(0., 0., 0., 0.);
+        const double4 v0 = ok0 ? make_double4(x[j0], y[j0], z[j0], m[j0]) : make_double4(0., 0., 0., 0.);
+        jt[0][tid] = v0;
+        jt[0][tid + kSymJB] = v0;
         const int j1 = jbase + kSymJB + tid;
         const bool ok1 = !CHECK || j1 < N;
         if (ok1) nxt = make_double4(x[j1], y[j1], z[j1], m[j1]);
@@ -764,7 +768,10 @@ __global__ __launch_bounds__(256) void k_direct_sym(const double* __restrict__ x
             jpart[((size_t)t * 3 + w) * kSymSB + (jb - 1) * kSymJB + k] = v;
         }
         if (tid < kSymJB) {
-            if (jb + 1 < NJB) jt[(jb + 1) & 1][tid] = nxt;
+            if (jb + 1 < NJB) {
+                jt[(jb + 1) & 1][tid] = nxt;
+                jt[(jb + 1) & 1][tid + kSymJB] = nxt;
+            }
             const int jn = jbase + (jb + 2) * kSymJB + tid;
             const bool okn = jb + 2 < NJB && (!CHECK || jn < N);
             nxt = okn ? make_double4(x[jn], y[jn], z[jn], m[jn]) : make_double4(0., 0., 0., 0.);
@@ -775,10 +782,10 @@ __global__ __launch_bounds__(256) void k_direct_sym(const double* __restrict__ x
         for (int q = 0; q < kSymTPL; q++) bx[q] = by[q] = bz[q] = 0.;
         double jx = 0., jy = 0., jz = 0.;
         const int jlim = CHECK ? N - (jbase + jb * kSymJB) : kSymJB;   // valid sources of this J-block
-#pragma unroll 2
+#pragma unroll 4
         for (int s = 0; s < kSymJB; s++) {
-            const int idx = (k - s) & (kSymJB - 1);
-            const double4 sj = src[idx];
+            const int idx = (k - s) & (kSymJB - 1);     // the source's index in the J-block
+            const double4 sj = src[k + kSymJB - s];
 #pragma unroll
             for (int q = 0; q < kSymTPL; q++) {
                 const double dx = tx[q] - sj.x;

@@ -148,6 +148,7 @@ inline const DeviceState* dev_of(const struct reb_simulation* r) { return &reint
 bool device_ready(struct reb_simulation* r);            // pick device, create stream
 bool ensure_device(struct reb_simulation* r);            // host -> device if host is newer
 bool ensure_host(struct reb_simulation* r);              // device -> host if device is newer
+bool prepare_host(struct reb_simulation* r);             // ensure_host, and WHFast's p_jh current
 void mark_host_newer(struct reb_simulation* r);
 void mark_device_newer(struct reb_simulation* r);
 bool stream_sync(struct reb_simulation* r);

@@ -51,6 +51,15 @@ namespace rbamd {
 
 enum JobKind { JOB_WHFAST = 0, JOB_NBODY = 1 };
 
+// A request to write a running job's state to an archive file; served by the
+// thread that steps the job, at the next point where its state is whole.
+struct Checkpoint {
+    std::string file;
+    bool done = false;   // Server::mu
+    int rc = 0;
+    std::string error;
+};
+
 struct Job {
     long id = 0;
     int kind = JOB_WHFAST;
@@ -63,6 +72,15 @@ struct Job {
     std::string error;
     double t_submit = 0., t_admit = 0., t_done = 0.;
     double e0 = 0., e1 = 0.;
+    // where the job starts: time and step count of the submitted simulation,
+    // and for WHFast its heliocentric state (x y z vx vy vz m, rows of n;
+    // empty: computed from the bodies) with the synchronization flag
+    double t0 = 0.;
+    uint64_t steps0 = 0;
+    bool resumed = false;
+    std::vector<double> pjh;
+    int synchronized = 1;
+    std::vector<std::shared_ptr<Checkpoint>> ckpt;   // requests not yet served (Server::mu)
     // pool bookkeeping (scheduler thread only)
     int slot = -1;
     size_t body0 = 0;
@@ -112,12 +130,14 @@ __global__ void k_pool_set_flags(const int* __restrict__ slots, int ns, unsigned
     if (k < ns) flags[slots[k]] = value;
 }
 
-__global__ void k_pool_gather(const int* __restrict__ src, int nb, const double* __restrict__ body, size_t stride,
-                              double* __restrict__ out) {
+// Bodies (10 fields) and heliocentric state (7 fields) of finished systems.
+__global__ void k_pool_gather(const int* __restrict__ src, int nb, const double* __restrict__ body,
+                              const double* __restrict__ ph, size_t stride, double* __restrict__ out) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nb) return;
     const size_t b = (size_t)src[k];
-    for (int f = 0; f < 10; f++) out[10 * (size_t)k + f] = body[f * stride + b];
+    for (int f = 0; f < 10; f++) out[17 * (size_t)k + f] = body[f * stride + b];
+    for (int f = 0; f < 7; f++) out[17 * (size_t)k + 10 + f] = ph[f * stride + b];
 }
 
 // Device arena of planetary systems.
@@ -382,6 +402,20 @@ static bool pool_admit(Server* S, const std::vector<Job*>& fresh) {
     hipLaunchKernelGGL(k_pool_set_flags, dim3((unsigned)((fresh.size() + 255) / 256)), dim3(256), 0, P.stream,
                        P.idx.as<int>() + nb, (int)fresh.size(), P.flags.as<unsigned int>(), 1u);
     POOL_HIP(S, hipGetLastError());
+    // systems resumed from a simulation that carries its heliocentric state
+    // (integrator_whfast.c:71-99, p_jh) continue from it, synchronized or not
+    std::vector<unsigned int> sync_flag(fresh.size());
+    for (size_t q = 0; q < fresh.size(); q++) {
+        const Job* j = fresh[q];
+        if (j->pjh.empty()) continue;
+        const size_t n = j->bodies.size();
+        for (int f = 0; f < 7; f++)
+            POOL_HIP(S, hipMemcpyAsync(P.ph.as<double>() + f * P.cap + j->body0, j->pjh.data() + f * n,
+                                       sizeof(double) * n, hipMemcpyHostToDevice, P.stream));
+        sync_flag[q] = j->synchronized ? 1u : 0u;
+        POOL_HIP(S, hipMemcpyAsync(P.flags.as<unsigned int>() + j->slot, &sync_flag[q], sizeof(unsigned int),
+                                   hipMemcpyHostToDevice, P.stream));
+    }
     // initial energies (tools.c:108-162) of the new systems
     POOL_HIP(S, hipMemcpyAsync(P.sys.p, P.hsys.data(), sizeof(WHSys) * P.nslots, hipMemcpyHostToDevice, P.stream));
     WHParams W = pool_params(S, 0, 0);
@@ -394,20 +428,192 @@ static bool pool_admit(Server* S, const std::vector<Job*>& fresh) {
     return true;
 }
 
-static void finish_job(Server* S, Job* j, bool ok, const std::string& msg) {
+// ------------------------------------------------------------------ checkpoints
+// One Simulationarchive snapshot (simulationarchive.c:411-615) of a job after
+// `steps` of its steps: the bodies, time, step count and the integrator with
+// its settings.  `pjh` (7 rows of n) is WHFast's heliocentric state
+// (integrator_whfast.c:71-99, p_jh), written with the synchronization flag so
+// that a simulation read from the file continues from it.  A submission that
+// has not stepped is written as freshly added particles.
+static bool write_snapshot(const Job& j, const struct reb_particle* bodies, const double* pjh, int synchronized,
+                           uint64_t steps, const std::string& file, std::string* err) {
+    const size_t n = j.bodies.size();
+    struct reb_simulation* r = reb_simulation_create();
+    if (!r) {
+        *err = "could not create a simulation";
+        return false;
+    }
+    r->save_messages = 1;
+    r->G = j.G;
+    r->dt = j.dt;
+    r->softening = j.softening;
+    if (j.kind == JOB_WHFAST) {
+        reb_simulation_set_integrator(r, "whfast");
+        auto* s = (struct reb_integrator_whfast_state*)r->integrator.state;
+        s->safe_mode = (unsigned int)j.safe_mode;
+        s->coordinates = REB_INTEGRATOR_WHFAST_COORDINATES_DEMOCRATICHELIOCENTRIC;
+    } else {
+        reb_simulation_set_integrator(r, "leapfrog");
+        r->gravity = static_cast<decltype(r->gravity)>(j.gravity);
+        reb_amd_set_precision(r, j.precision);
+    }
+    for (size_t i = 0; i < n; i++) reb_simulation_add(r, bodies[i]);
+    // the time as reb_simulation_steps sums it: two half steps per step
+    // (integrator_leapfrog.c:81, integrator_whfast.c:1193-1257)
+    double t = j.t0;
+    for (uint64_t k = 0; k < steps; k++) {
+        t += j.dt / 2.;
+        t += j.dt / 2.;
+    }
+    r->t = t;
+    r->steps_done = j.steps0 + steps;
+    r->did_modify_particles = (steps == 0 && !j.resumed) ? 1 : 0;
+    r->is_synchronized = synchronized ? 1 : 0;
+    if (pjh && j.kind == JOB_WHFAST) {
+        auto* s = (struct reb_integrator_whfast_state*)r->integrator.state;
+        s->p_jh = (struct reb_particle*)calloc(n, sizeof(struct reb_particle));
+        if (!s->p_jh) {
+            reb_simulation_free(r);
+            *err = "out of host memory";
+            return false;
+        }
+        s->N_allocated = n;
+        for (size_t i = 0; i < n; i++) {
+            struct reb_particle& q = s->p_jh[i];
+            q.x = pjh[0 * n + i]; q.y = pjh[1 * n + i]; q.z = pjh[2 * n + i];
+            q.vx = pjh[3 * n + i]; q.vy = pjh[4 * n + i]; q.vz = pjh[5 * n + i];
+            q.m = pjh[6 * n + i];
+        }
+    }
+    reb_simulation_save_to_file(r, file.c_str());
+    bool ok = true;
+    char buf[REB_STRING_SIZE_MAX];
+    while (reb_pop_message(r->messages, buf)) {
+        if (buf[0] == 'e' && ok) {
+            ok = false;
+            *err = buf + 1;
+        }
+    }
+    reb_simulation_free(r);
+    return ok;
+}
+
+// The state a job holds on the host: as submitted (nothing stepped) or final.
+static bool write_host_state(const Job& j, bool final, const std::string& file, std::string* err) {
+    return write_snapshot(j, j.bodies.data(), j.pjh.empty() ? nullptr : j.pjh.data(), j.synchronized,
+                          final ? j.n_steps : 0, file, err);
+}
+
+// Takes a job's outstanding checkpoint requests (Server::mu held by nobody).
+static std::vector<std::shared_ptr<Checkpoint>> take_requests(Server* S, Job* j) {
+    std::vector<std::shared_ptr<Checkpoint>> req;
     std::lock_guard<std::mutex> g(S->mu);
-    j->t_done = since(S);
-    j->state = ok ? REB_AMD_JOB_DONE : REB_AMD_JOB_FAILED;
-    if (!ok) j->error = msg;
-    S->used -= j->footprint;
-    if (ok)
-        S->done++;
-    else
-        S->failed++;
-    if (j->kind == JOB_NBODY) S->nbody_running--;
-    S->generation++;
+    req.swap(j->ckpt);
+    return req;
+}
+
+// Writes one snapshot per request with `write(file, err)` and wakes the callers.
+template <class W>
+static void serve_requests(Server* S, std::vector<std::shared_ptr<Checkpoint>>& req, W write) {
+    if (req.empty()) return;
+    for (auto& c : req) {
+        std::string e;
+        c->rc = write(c->file, &e) ? 0 : 1;
+        c->error = e;
+    }
+    std::lock_guard<std::mutex> g(S->mu);
+    for (auto& c : req) c->done = true;
     S->cv_done.notify_all();
-    S->cv_work.notify_all();
+}
+
+static bool bodies_finite(const std::vector<struct reb_particle>& b) {
+    for (const auto& p : b)
+        if (!std::isfinite(p.x) || !std::isfinite(p.y) || !std::isfinite(p.z) || !std::isfinite(p.vx) ||
+            !std::isfinite(p.vy) || !std::isfinite(p.vz) || !std::isfinite(p.m))
+            return false;
+    return true;
+}
+
+// A job leaves the server: done, or failed with a message.  A job whose
+// bodies are no longer finite fails; the other jobs are not affected.
+// Checkpoint requests made while it ran are served from its final state.
+static void finish_job(Server* S, Job* j, bool ok, const std::string& msg) {
+    std::string why = msg;
+    if (ok && !bodies_finite(j->bodies)) {
+        ok = false;
+        why = "non-finite coordinates after " + std::to_string(j->n_steps) + " steps";
+    }
+    for (;;) {
+        std::vector<std::shared_ptr<Checkpoint>> req;
+        {
+            std::lock_guard<std::mutex> g(S->mu);
+            req.swap(j->ckpt);
+            if (req.empty()) {
+                j->t_done = since(S);
+                j->state = ok ? REB_AMD_JOB_DONE : REB_AMD_JOB_FAILED;
+                if (!ok) j->error = why;
+                S->used -= j->footprint;
+                if (ok)
+                    S->done++;
+                else
+                    S->failed++;
+                if (j->kind == JOB_NBODY) S->nbody_running--;
+                S->generation++;
+                S->cv_done.notify_all();
+                S->cv_work.notify_all();
+                return;
+            }
+        }
+        serve_requests(S, req, [&](const std::string& file, std::string* e) {
+            if (!ok) {
+                *e = "job failed: " + why;
+                return false;
+            }
+            return write_host_state(*j, true, file, e);
+        });
+    }
+}
+
+// Checkpoints of the systems in the pool, between ticks: the bodies and the
+// heliocentric state as the next tick would continue from them.
+static void pool_checkpoints(Server* S) {
+    Pool& P = S->pool;
+    for (Job* j : S->pool_active) {
+        auto req = take_requests(S, j);
+        if (req.empty()) continue;
+        if (j->fresh) {   // admitted, not yet stepped
+            serve_requests(S, req, [&](const std::string& f, std::string* e) { return write_host_state(*j, false, f, e); });
+            continue;
+        }
+        const size_t n = j->bodies.size();
+        std::vector<double> rows(17 * n);
+        unsigned int flag = 0;
+        bool got = true;
+        for (int f = 0; f < 10 && got; f++)
+            got = hipMemcpyAsync(rows.data() + f * n, P.body.as<double>() + f * P.cap + j->body0, sizeof(double) * n,
+                                 hipMemcpyDeviceToHost, P.stream) == hipSuccess;
+        for (int f = 0; f < 7 && got; f++)
+            got = hipMemcpyAsync(rows.data() + (10 + f) * n, P.ph.as<double>() + f * P.cap + j->body0,
+                                 sizeof(double) * n, hipMemcpyDeviceToHost, P.stream) == hipSuccess;
+        if (got)
+            got = hipMemcpyAsync(&flag, P.flags.as<unsigned int>() + j->slot, sizeof(flag), hipMemcpyDeviceToHost,
+                                 P.stream) == hipSuccess &&
+                  hipStreamSynchronize(P.stream) == hipSuccess;
+        std::vector<struct reb_particle> cur = j->bodies;
+        for (size_t b = 0; b < n; b++) {
+            double* f[10] = {&cur[b].x, &cur[b].y, &cur[b].z, &cur[b].vx, &cur[b].vy,
+                             &cur[b].vz, &cur[b].ax, &cur[b].ay, &cur[b].az, &cur[b].m};
+            for (int q = 0; q < 10; q++) *f[q] = rows[q * n + b];
+        }
+        const uint64_t steps = j->n_steps - j->remaining;
+        serve_requests(S, req, [&](const std::string& f, std::string* e) {
+            if (!got) {
+                *e = "device copy failed";
+                return false;
+            }
+            return write_snapshot(*j, cur.data(), rows.data() + 10 * n, (int)(flag & 1u), steps, f, e);
+        });
+    }
 }
 
 // One tick of the planetary pool: every active system advances by
@@ -424,7 +630,7 @@ static bool pool_tick(Server* S) {
         WHSys& w = P.hsys[j->slot];
         const uint64_t k = std::min<uint64_t>(j->remaining, (uint64_t)tick_max);
         w.steps = (int)k;
-        w.req = (k == j->remaining ? 1 : 0) | (j->fresh ? 2 : 0) | (j->safe_mode ? 4 : 0);
+        w.req = (k == j->remaining ? 1 : 0) | (j->fresh && j->pjh.empty() ? 2 : 0) | (j->safe_mode ? 4 : 0);
         tick = std::max<long>(tick, (long)k);
     }
     // tasks over the active systems, bucketed by planet count (largest first)
@@ -463,7 +669,7 @@ static bool pool_tick(Server* S) {
     if (!fin.empty()) {
         size_t nb = 0;
         for (Job* j : fin) nb += j->bodies.size();
-        const size_t out_bytes = sizeof(double) * 10 * nb;
+        const size_t out_bytes = sizeof(double) * 17 * nb;
         const size_t idx_bytes = sizeof(int) * nb;
         if (!P.h_stage.ensure(out_bytes + idx_bytes + sizeof(double) * P.nslots + 256) ||
             !P.stage.ensure(out_bytes + 256) || !P.idx.ensure(idx_bytes + 256))
@@ -474,7 +680,8 @@ static bool pool_tick(Server* S) {
             for (size_t b = 0; b < j->bodies.size(); b++) hi[k++] = (int)(j->body0 + b);
         POOL_HIP(S, hipMemcpyAsync(P.idx.p, hi, idx_bytes, hipMemcpyHostToDevice, P.stream));
         hipLaunchKernelGGL(k_pool_gather, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, P.stream,
-                           P.idx.as<int>(), (int)nb, P.body.as<double>(), P.cap, P.stage.as<double>());
+                           P.idx.as<int>(), (int)nb, P.body.as<double>(), P.ph.as<double>(), P.cap,
+                           P.stage.as<double>());
         if (whfast_energy_launch(W, P.energy.as<double>(), (int)P.nslots, P.stream))
             return pool_fail(S, "energy launch", hipGetLastError());
         double* he = reinterpret_cast<double*>(static_cast<char*>(P.h_stage.p) + out_bytes + idx_bytes + 64);
@@ -484,12 +691,17 @@ static bool pool_tick(Server* S) {
         const double* hd = static_cast<const double*>(P.h_stage.p);
         k = 0;
         for (Job* j : fin) {
-            for (auto& p : j->bodies) {
-                const double* f = hd + 10 * k++;
+            const size_t n = j->bodies.size();
+            j->pjh.assign(7 * n, 0.);
+            j->synchronized = 1;
+            for (size_t b = 0; b < n; b++) {
+                struct reb_particle& p = j->bodies[b];
+                const double* f = hd + 17 * k++;
                 p.x = f[0]; p.y = f[1]; p.z = f[2];
                 p.vx = f[3]; p.vy = f[4]; p.vz = f[5];
                 p.ax = f[6]; p.ay = f[7]; p.az = f[8];
                 p.m = f[9];
+                for (int q = 0; q < 7; q++) j->pjh[q * n + b] = f[10 + q];
             }
             j->e1 = he[j->slot];
             WHSys& w = P.hsys[j->slot];
@@ -526,19 +738,49 @@ static void run_nbody(Server* S, Job* j, hipStream_t stream) {
     for (const auto& p : j->bodies) reb_simulation_add(r, p);
     reb_amd_set_precision(r, j->precision);
     reb_amd_set_resident(r, 1);
+    r->t = j->t0;
+    r->steps_done = j->steps0;
     j->e0 = reb_simulation_energy(r);
-    reb_simulation_steps(r, (size_t)j->n_steps);
-    reb_amd_set_resident(r, 0);
-    j->e1 = reb_simulation_energy(r);
     std::string msg;
     bool ok = true;
-    char buf[REB_STRING_SIZE_MAX];
-    while (reb_pop_message(r->messages, buf)) {
-        if (buf[0] == 'e') {
-            ok = false;
-            if (msg.empty()) msg = buf + 1;
+    auto drain = [&]() {   // the first error the simulation reported fails the job
+        char buf[REB_STRING_SIZE_MAX];
+        while (reb_pop_message(r->messages, buf)) {
+            if (buf[0] == 'e' && ok) {
+                ok = false;
+                msg = buf + 1;
+            }
         }
+    };
+    // Stepped in segments of ~10 ms or more, between which checkpoint requests
+    // are served from the simulation itself (the steps are those of one call).
+    uint64_t left = j->n_steps, seg = 16;
+    while (left > 0 && ok) {
+        const uint64_t k = std::min(left, seg);
+        const auto t_seg = std::chrono::steady_clock::now();
+        reb_simulation_steps(r, (size_t)k);
+        left -= k;
+        drain();
+        const double took = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_seg).count();
+        if (took < 0.01) seg = std::min<uint64_t>(seg * 2, (uint64_t)1 << 20);
+        if (left == 0 || !ok) break;
+        auto req = take_requests(S, j);
+        serve_requests(S, req, [&](const std::string& f, std::string* e) {
+            reb_simulation_save_to_file(r, f.c_str());
+            char buf[REB_STRING_SIZE_MAX];
+            bool fine = true;
+            while (reb_pop_message(r->messages, buf)) {
+                if (buf[0] == 'e' && fine) {
+                    fine = false;
+                    *e = buf + 1;
+                }
+            }
+            return fine;
+        });
     }
+    reb_amd_set_resident(r, 0);
+    j->e1 = reb_simulation_energy(r);
+    drain();
     if (ok)
         for (size_t i = 0; i < j->bodies.size() && i < r->N; i++) j->bodies[i] = r->particles[i];
     reb_simulation_free(r);
@@ -701,6 +943,33 @@ static bool nb_collect(Server* S, long steps) {
     return true;
 }
 
+// Checkpoints of the batched N-body jobs between rounds.
+static void nb_checkpoints(Server* S) {
+    for (Job* j : S->nb_active) {
+        auto req = take_requests(S, j);
+        if (req.empty()) continue;
+        const size_t n = j->bodies.size();
+        std::vector<double> h(10 * n);
+        const bool got = hipMemcpyAsync(h.data(), j->soa.p, sizeof(double) * 10 * n, hipMemcpyDeviceToHost,
+                                        S->nb_stream) == hipSuccess &&
+                         hipStreamSynchronize(S->nb_stream) == hipSuccess;
+        std::vector<struct reb_particle> cur = j->bodies;
+        for (size_t i = 0; i < n; i++) {
+            double* f[10] = {&cur[i].x, &cur[i].y, &cur[i].z, &cur[i].vx, &cur[i].vy,
+                             &cur[i].vz, &cur[i].ax, &cur[i].ay, &cur[i].az, &cur[i].m};
+            for (int k = 0; k < 10; k++) *f[k] = h[k * n + i];
+        }
+        const uint64_t steps = j->n_steps - j->remaining;
+        serve_requests(S, req, [&](const std::string& f, std::string* e) {
+            if (!got) {
+                *e = "device copy failed";
+                return false;
+            }
+            return write_snapshot(*j, cur.data(), nullptr, 1, steps, f, e);
+        });
+    }
+}
+
 static void nbody_worker(Server* S, hipStream_t stream) {
     (void)hipSetDevice(S->device);
     for (;;) {
@@ -778,6 +1047,7 @@ static void scheduler(Server* S) {
             act.swap(S->pool_active);
             for (Job* j : act) finish_job(S, j, false, S->err);
         }
+        pool_checkpoints(S);
         if (nb_steps < 0 || (!S->nb_active.empty() && !nb_collect(S, nb_steps))) {
             std::vector<Job*> act;
             act.swap(S->nb_active);
@@ -790,6 +1060,7 @@ static void scheduler(Server* S) {
             }
             S->nb_dirty = true;
         }
+        nb_checkpoints(S);
     }
 }
 
@@ -951,6 +1222,131 @@ REB_API long reb_amd_server_submit_nbody(struct reb_amd_server* h, size_t n, con
     j->precision = precision;
     j->footprint = nbody_footprint(n, precision, gravity);
     return submit(&h->s, std::move(j));
+}
+
+// A simulation as a job that continues from its state, time and step count
+// for n_steps steps: WHFast in democratic heliocentric coordinates (1..65
+// bodies, no correctors, default kernel, no softening) or order-2 leapfrog
+// with BASIC or COMPENSATED direct gravity in the simulation's precision.  A
+// WHFast simulation that carries its heliocentric state (one read from a
+// checkpoint, or one that has stepped) continues from it as
+// reb_simulation_steps would.  The simulation is not modified.
+REB_API long reb_amd_server_submit_simulation(struct reb_amd_server* h, struct reb_simulation* r,
+                                              uint64_t n_steps) {
+    const char* fn = "reb_amd_server_submit_simulation";
+    auto fail = [&](const std::string& why) {
+        set_last_error(std::string(fn) + ": " + why);
+        return -1L;
+    };
+    if (!h || !r) return fail("server and simulation required");
+    if (!r->integrator.name) return fail("the simulation has no integrator");
+    if (r->N_var || r->additional_forces || dev_of(r)->dev_forces || r->N_ghost_x || r->N_ghost_y || r->N_ghost_z ||
+        r->collision != REB_COLLISION_NONE || r->boundary != REB_BOUNDARY_NONE ||
+        (r->N_active != SIZE_MAX && r->N_active != r->N))
+        return fail("variational particles, additional forces, ghost boxes, collisions, boundaries and test "
+                    "particles are not served");
+    if (!std::isfinite(r->dt) || !std::isfinite(r->G)) return fail("a finite dt and a finite G required");
+    if (!prepare_host(r)) return fail("could not read the simulation's state from the device");
+    auto j = std::make_unique<Job>();
+    j->bodies.assign(r->particles, r->particles + r->N);
+    j->dt = r->dt;
+    j->G = r->G;
+    j->softening = r->softening;
+    j->n_steps = n_steps;
+    j->t0 = r->t;
+    j->steps0 = r->steps_done;
+    j->resumed = true;
+    if (strcmp(r->integrator.name, "whfast") == 0) {
+        auto* s = (const struct reb_integrator_whfast_state*)r->integrator.state;
+        if (r->N < 1 || r->N > 65) return fail("WHFast jobs hold 1 to 65 bodies");
+        if (!s || s->coordinates != REB_INTEGRATOR_WHFAST_COORDINATES_DEMOCRATICHELIOCENTRIC || s->corrector ||
+            s->corrector2 || s->kernel || s->keep_unsynchronized)
+            return fail("WHFast jobs use democratic heliocentric coordinates without correctors");
+        if (r->gravity != REB_GRAVITY_BASIC || r->softening != 0.)
+            return fail("WHFast jobs use BASIC gravity without softening");
+        j->kind = JOB_WHFAST;
+        j->safe_mode = s->safe_mode ? 1 : 0;
+        // the heliocentric state the simulation would continue from
+        // (whfast.hip: a simulation read from a file steps from its p_jh)
+        if (s->p_jh && s->N_allocated == r->N && !r->did_modify_particles) {
+            const size_t n = r->N;
+            j->pjh.assign(7 * n, 0.);
+            for (size_t i = 0; i < n; i++) {
+                const struct reb_particle& q = s->p_jh[i];
+                const double f[7] = {q.x, q.y, q.z, q.vx, q.vy, q.vz, q.m};
+                for (int k = 0; k < 7; k++) j->pjh[k * n + i] = f[k];
+            }
+            j->synchronized = r->is_synchronized ? 1 : 0;
+        }
+        j->footprint = whfast_footprint(r->N);
+    } else if (strcmp(r->integrator.name, "leapfrog") == 0) {
+        auto* s = (const struct reb_integrator_leapfrog_state*)r->integrator.state;
+        if (!s || s->order != 2) return fail("leapfrog jobs are of order 2");
+        if (r->gravity != REB_GRAVITY_BASIC && r->gravity != REB_GRAVITY_COMPENSATED)
+            return fail("N-body jobs use BASIC or COMPENSATED direct gravity");
+        if (r->N < 1 || r->N > ((size_t)1 << 24)) return fail("N-body jobs hold 1 to 2^24 bodies");
+        j->kind = JOB_NBODY;
+        j->gravity = r->gravity;
+        j->precision = reb_amd_get_precision(r);
+        j->footprint = nbody_footprint(r->N, j->precision, j->gravity);
+    } else {
+        return fail("only WHFast and leapfrog simulations are served");
+    }
+    return submit(&h->s, std::move(j));
+}
+
+// Writes a job's current state to `filename` as one Simulationarchive
+// snapshot (appended when the file exists): a pending job as submitted, a
+// running job at the next point where its state is whole (between pool
+// ticks, batch rounds or step segments), a finished job as it ended.  Blocks
+// until the file is written.  Stepping a simulation read from the snapshot
+// for the job's remaining steps gives the job's result bit for bit, and so
+// does submitting it with reb_amd_server_submit_simulation.  0 on success.
+REB_API int reb_amd_server_checkpoint(struct reb_amd_server* h, long job, const char* filename) {
+    if (!h || !filename) {
+        set_last_error("reb_amd_server_checkpoint: server and file name required");
+        return 1;
+    }
+    Server* S = &h->s;
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto it = S->jobs.find(job);
+    if (it == S->jobs.end()) {
+        set_last_error("reb_amd_server_checkpoint: unknown job");
+        return 1;
+    }
+    Job* j = it->second.get();
+    std::string err;
+    if (j->state == REB_AMD_JOB_FAILED) {
+        set_last_error("reb_amd_server_checkpoint: job failed: " + j->error);
+        return 1;
+    }
+    if (j->state == REB_AMD_JOB_PENDING || j->state == REB_AMD_JOB_DONE) {
+        // the host copy is the state; a pending job is only read by admission
+        // and a finished one not at all, so the copy can be made unlocked
+        Job copy;
+        copy.kind = j->kind;
+        copy.bodies = j->bodies;
+        copy.dt = j->dt; copy.G = j->G; copy.softening = j->softening;
+        copy.safe_mode = j->safe_mode; copy.gravity = j->gravity; copy.precision = j->precision;
+        copy.n_steps = j->n_steps; copy.t0 = j->t0; copy.steps0 = j->steps0; copy.resumed = j->resumed;
+        copy.pjh = j->pjh;
+        copy.synchronized = j->synchronized;
+        const bool final = j->state == REB_AMD_JOB_DONE;
+        lk.unlock();
+        if (!write_host_state(copy, final, filename, &err)) {
+            set_last_error("reb_amd_server_checkpoint: " + err);
+            return 1;
+        }
+        return 0;
+    }
+    auto c = std::make_shared<Checkpoint>();
+    c->file = filename;
+    j->ckpt.push_back(c);
+    S->generation++;
+    S->cv_work.notify_all();
+    S->cv_done.wait(lk, [&] { return c->done; });
+    if (c->rc) set_last_error("reb_amd_server_checkpoint: " + c->error);
+    return c->rc;
 }
 
 // Device-memory footprints used for admission.

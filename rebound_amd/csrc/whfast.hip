@@ -98,43 +98,6 @@ __device__ __forceinline__ double quartic_next(double X, double beta, double r0,
     return (X * denom - 5. * f) / denom;
 }
 
-// The quartic iterates X_1, X_2, ... (X_k the update of X_{k-1}) are a
-// deterministic sequence, so the first k with X_k equal to an earlier iterate
-// is mu + lambda (tail length plus cycle length), the earlier one X_mu; every
-// later repeat spans a multiple of lambda.  Given X_63, the last iterate the
-// reference's loop produces, this returns mu + lambda when it is at most 63,
-// else 0, without a history: if the first repeat is at most 63, X_63 lies on
-// the cycle, so the cycle length is the first h in 64..125 with X_h == X_63
-// (lambda <= 62); mu is then the first i with X_i == X_{i+lambda}.  A NaN
-// iterate repeats nothing and every later iterate is NaN, so a NaN X_63, or a
-// NaN after it, means no repeat.
-__device__ __noinline__ int quartic_first_repeat(double X0, double X63, double beta, double r0, double eta0,
-                                                 double zeta0, double dt) {
-    if (X63 != X63) return 0;
-    double Gs[4];
-    double h = X63;
-    int lam = 0;
-    for (int k = 64; k < 126; k++) {
-        h = quartic_next(h, beta, r0, eta0, zeta0, dt, Gs);
-        if (h != h) return 0;
-        if (h == X63) {
-            lam = k - 63;
-            break;
-        }
-    }
-    if (lam == 0) return 0;
-    double tort = quartic_next(X0, beta, r0, eta0, zeta0, dt, Gs);
-    double hare = tort;
-    for (int k = 0; k < lam; k++) hare = quartic_next(hare, beta, r0, eta0, zeta0, dt, Gs);
-    int mu = 1;
-    while (!(tort == hare)) {
-        tort = quartic_next(tort, beta, r0, eta0, zeta0, dt, Gs);
-        hare = quartic_next(hare, beta, r0, eta0, zeta0, dt, Gs);
-        if (++mu + lam > 63) return 0;
-    }
-    return mu + lam;
-}
-
 // Universal-variable Kepler drift of one body (integrator_whfast.c:241-390),
 // in place.  Returns a bit mask: 1 = the step exceeded one orbital period
 // (the reference warns once), 4 = the quartic iteration ran, 8 = the
@@ -173,34 +136,30 @@ __device__ __forceinline__ int kepler_solver(double& px, double& py, double& pz,
     if (fastabs(X - oldX) > 0.01 * X_per_period) {
         // Quartic (Laguerre-Conway) iteration with cycle detection (:289-312):
         // converged at the first iterate equal to any earlier one.  The last
-        // four iterates are kept in registers (NaN: empty; it equals
-        // nothing), which finds that first repeat exactly when the cycle is
-        // at most four long (the iterates are a deterministic sequence, so
-        // every repeat spans a multiple of the cycle length); a longer cycle
-        // is located by quartic_first_repeat and replayed.  No history array.
-        // (X is X_63 when the loop ends without a repeat.)
+        // eight iterates are compared in registers (NaN: empty, equal to
+        // nothing); an iterate leaving that window is stored in `older`,
+        // which is read only when the loop runs past nine iterations.  Short
+        // loops (most of them) touch no memory.
         flags |= 4;
         X = beta * dt / mu;
-        const double Xq0 = X;
-        double w0 = __builtin_nan(""), w1 = w0, w2 = w0, w3 = w0;
+        double older[56];      // older[k] = X_k once it has left the window
+        double w0 = __builtin_nan(""), w1 = w0, w2 = w0, w3 = w0, w4 = w0, w5 = w0, w6 = w0, w7 = w0;
         for (int n_lag = 1; n_lag < 64; n_lag++) {
+            // X_{n_lag} from X_{n_lag-1}; w0..w7 hold X_{n_lag-1}..X_{n_lag-8}
             X = quartic_next(X, beta, r0, eta0, zeta0, dt, Gs);
-            if (X == w0 || X == w1 || X == w2 || X == w3) {
+            if (X == w0 || X == w1 || X == w2 || X == w3 || X == w4 || X == w5 || X == w6 || X == w7) {
                 converged = 1;
                 break;
             }
-            w3 = w2;
-            w2 = w1;
-            w1 = w0;
-            w0 = X;
-        }
-        if (!converged) {
-            const int n = quartic_first_repeat(Xq0, X, beta, r0, eta0, zeta0, dt);
-            if (n > 0) {
-                X = Xq0;
-                for (int k = 0; k < n; k++) X = quartic_next(X, beta, r0, eta0, zeta0, dt, Gs);
-                converged = 1;
+            for (int k = 1; k < n_lag - 8; k++) {
+                if (X == older[k]) {
+                    converged = 1;
+                    break;
+                }
             }
+            if (converged) break;
+            if (n_lag > 8) older[n_lag - 8] = w7;
+            w7 = w6; w6 = w5; w5 = w4; w4 = w3; w3 = w2; w2 = w1; w1 = w0; w0 = X;
         }
         const double e1 = eta0 * Gs[1] + zeta0 * Gs[2];
         ri = 1. / (r0 + e1);
@@ -384,7 +343,7 @@ __device__ __forceinline__ const CorrZ* corrector_chain(int order, int& n) {
 // lane of a group evaluates over the published bodies in the reference's order,
 // and the symplectic correctors run at the synchronisation points.
 template <int COORD>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_whfast(WHParams P) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_whfast(WHParams P) {
     constexpr bool JAC = COORD == WH_JACOBI;
     __shared__ Slot slot[64];
     __shared__ double aux[64][3];    // per-planet sums of the serial Jacobi gravity / jerk

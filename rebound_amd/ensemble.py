@@ -178,6 +178,13 @@ class Ensemble:
         _check(lib().reb_amd_ensemble_set_launch(self._h, int(grid), int(chunk)), "reb_amd_ensemble_set_launch",
                self._h)
 
+    def set_packing(self, tasks=0):
+        """About `tasks` wave tasks (0: densest); returns the task count."""
+        n = lib().reb_amd_ensemble_set_packing(self._h, int(tasks))
+        if n < 0:
+            raise RebError("reb_amd_ensemble_set_packing: " + lib().reb_amd_ensemble_error(self._h).decode())
+        return n
+
     @property
     def launch(self):
         g, c, t, m = ctypes.c_int(), ctypes.c_long(), ctypes.c_int(), ctypes.c_int()

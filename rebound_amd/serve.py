@@ -5,6 +5,8 @@ N-body jobs (leapfrog with direct gravity), admits them by device-memory
 footprint, batches the planetary systems continuously into one device pool
 and overlaps N-body jobs on their own streams.  Each job's result equals the
 same simulation integrated alone by reb_simulation_steps (simulation.c:504-513).
+A job's state can be written to a Simulationarchive file while it runs
+(checkpoint) and a simulation read from one submitted to continue it.
 """
 import ctypes
 
@@ -59,6 +61,16 @@ class Server:
         parts, n = _particles(state)
         return self._check_id(lib().reb_amd_server_submit_nbody(self._h, n, parts, G, softening, dt, int(steps),
                                                                 _abi.GRAVITY[gravity], PRECISION[precision]), n)
+
+    def submit_simulation(self, sim, steps=1):
+        """A Sim (WHFast democratic heliocentric, or order-2 leapfrog with
+        direct gravity) as a job continuing from its state for `steps` steps."""
+        return self._check_id(lib().reb_amd_server_submit_simulation(self._h, sim.ptr, int(steps)), sim.s.N)
+
+    def checkpoint(self, jid, filename):
+        """Appends the job's current state to an archive file; returns once written."""
+        if lib().reb_amd_server_checkpoint(self._h, jid, str(filename).encode()):
+            raise RebError("checkpoint of job %d: %s" % (jid, lib().reb_amd_last_error().decode()))
 
     def status(self, jid):
         return _abi.JOB_STATE[lib().reb_amd_server_status(self._h, jid)]

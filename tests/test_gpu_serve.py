@@ -130,3 +130,157 @@ def test_job_larger_than_budget_is_rejected(gpu):
     with pytest.raises(gpu.RebError, match="exceeds the memory budget"):
         srv.submit_nbody(_plummer(gpu, 4096, 6), steps=1)
     srv.close()
+
+
+def _wait_running(srv, jid, limit=30.0):
+    t0 = time.time()
+    while srv.status(jid) == "pending":
+        assert time.time() - t0 < limit
+        time.sleep(0.0005)
+
+
+def _t_after(t0, dt, k):
+    """The time reb_simulation_steps reaches after k steps: two half steps each."""
+    t = t0
+    for _ in range(k):
+        t += dt / 2.0
+        t += dt / 2.0
+    return t
+
+
+@pytest.mark.parametrize("safe", [0, 1])
+def test_whfast_checkpoint_resume(gpu, tmp_path, safe):
+    """A running planetary job written to an archive file mid-run: the file
+    read back and stepped for the remaining steps (alone, or submitted again)
+    gives the job's result bit for bit; a finished job's checkpoint continues
+    as the simulation stepped further would."""
+    (st, dt), = _systems(1)
+    total = 200000
+    path = str(tmp_path / "job.bin")
+    srv = gpu.Server(tick_steps=500)
+    jid = srv.submit_whfast(st, dt, safe_mode=safe, steps=total)
+    _wait_running(srv, jid)
+    srv.checkpoint(jid, path)
+    srv.checkpoint(jid, path)       # a second snapshot appended later in the run
+    assert srv.wait(jid, 120) == "done", srv.job_error(jid)
+    done_path = str(tmp_path / "done.bin")
+    srv.checkpoint(jid, done_path)
+    want, _, _ = srv.result(jid)
+    mid = 0
+    for snap in (0, 1):
+        sim = gpu.Sim.from_file(path, snap)
+        k = sim.s.steps_done
+        assert 0 <= k <= total
+        mid += 0 < k < total
+        assert sim.s.t == _t_after(0.0, dt, k) and sim.s.dt == dt
+        sim.steps(total - k)
+        sim.synchronize()
+        assert np.array_equal(sim.state()[:, :6], want[:, :6]), (snap, k)
+        # the same snapshot as a new job
+        again = gpu.Sim.from_file(path, snap)
+        j2 = srv.submit_simulation(again, steps=total - k)
+        assert srv.wait(j2, 120) == "done", srv.job_error(j2)
+        got, _, _ = srv.result(j2)
+        assert np.array_equal(got[:, :6], want[:, :6]), (snap, k)
+        again.free()
+        if snap == 1:
+            # the finished job's checkpoint continues like the stepped simulation
+            fin = gpu.Sim.from_file(done_path, 0)
+            assert fin.s.steps_done == total and np.array_equal(fin.state()[:, :6], want[:, :6])
+            fin.steps(300)
+            fin.synchronize()
+            sim.steps(300)
+            sim.synchronize()
+            assert np.array_equal(fin.state()[:, :6], sim.state()[:, :6])
+            fin.free()
+        sim.free()
+    assert mid >= 1, "no checkpoint landed inside the run"
+    srv.close()
+
+
+@pytest.mark.parametrize("precision,n,total", [("fp64", 512, 6000), ("split", 700, 6000), ("exact", 256, 3000)])
+def test_nbody_checkpoint_resume(gpu, tmp_path, precision, n, total):
+    """N-body jobs (batched, or alone for EXACT) checkpointed mid-run resume
+    to the same result, read back alone or submitted again."""
+    st = _plummer(gpu, n, 7)
+    path = str(tmp_path / "nb.bin")
+    srv = gpu.Server(max_nbody_jobs=4)
+    jid = srv.submit_nbody(st, G=1.0, softening=0.01, dt=1e-3, steps=total, precision=precision)
+    _wait_running(srv, jid)
+    srv.checkpoint(jid, path)
+    assert srv.wait(jid, 120) == "done", srv.job_error(jid)
+    want, _, _ = srv.result(jid)
+    sim = gpu.Sim.from_file(path, 0)
+    k = sim.s.steps_done
+    assert 0 <= k < total
+    assert sim.s.t == _t_after(0.0, 1e-3, k) and sim.s.softening == 0.01
+    sim.precision = precision           # an engine setting, not part of the file
+    j2 = srv.submit_simulation(sim, steps=total - k)
+    sim.steps(total - k)
+    assert np.array_equal(sim.state()[:, :6], want[:, :6]), k
+    assert srv.wait(j2, 120) == "done", srv.job_error(j2)
+    got, _, _ = srv.result(j2)
+    assert np.array_equal(got[:, :6], want[:, :6]), k
+    sim.free()
+    srv.close()
+
+
+def test_checkpoint_of_pending_and_unknown_jobs(gpu, tmp_path):
+    st = _plummer(gpu, 2000, 5)
+    one = gpu.lib().reb_amd_server_nbody_bytes(2000, gpu.PRECISION["fp64"])
+    srv = gpu.Server(memory_budget=int(1.5 * one), max_nbody_jobs=4)
+    a = srv.submit_nbody(st, softening=0.01, steps=400)
+    b = srv.submit_nbody(st, softening=0.01, steps=5)     # waits for a
+    path = str(tmp_path / "pending.bin")
+    if srv.status(b) == "pending":
+        srv.checkpoint(b, path)
+        sim = gpu.Sim.from_file(path, 0)
+        assert sim.s.steps_done == 0 and sim.s.t == 0.0 and np.array_equal(sim.state()[:, :6], st[:, :6])
+        sim.free()
+    with pytest.raises(gpu.RebError, match="unknown job"):
+        srv.checkpoint(12345, path)
+    for jid in (a, b):
+        assert srv.wait(jid, 120) == "done"
+        srv.result(jid)
+    srv.close()
+
+
+def test_failing_jobs_do_not_affect_others(gpu):
+    """Jobs whose state turns non-finite fail with a message; the planetary
+    and N-body jobs running beside them give their results alone, bit for
+    bit, and the server takes new work afterwards."""
+    systems = _systems(12)
+    want = [_alone_whfast(gpu, st, dt, 300) for st, dt in systems]
+    nb = [(_plummer(gpu, 300, 11), "fp64"), (_plummer(gpu, 256, 12), "exact")]
+    nb_want = [_alone_nbody(gpu, s, 40, p) for s, p in nb]
+    srv = gpu.Server(tick_steps=64, max_nbody_jobs=8)
+    bad = []
+    bst = systems[3][0].copy()
+    bst[2, 3] = np.nan                  # a planet's velocity
+    bad.append(srv.submit_whfast(bst, systems[3][1], steps=300))
+    ids = [srv.submit_whfast(st, dt, steps=300) for st, dt in systems]
+    nb_ids = [srv.submit_nbody(s, softening=0.01, dt=1e-3, steps=40, precision=p) for s, p in nb]
+    for s, p in nb:
+        b = s.copy()
+        b[5, 0] = np.nan
+        bad.append(srv.submit_nbody(b, softening=0.01, dt=1e-3, steps=40, precision=p))
+    for jid in bad:
+        assert srv.wait(jid, 120) == "failed"
+        assert "non-finite" in srv.job_error(jid)
+        with pytest.raises(gpu.RebError, match="job failed"):
+            srv.result(jid)
+    for k, jid in enumerate(ids):
+        assert srv.wait(jid, 120) == "done", srv.job_error(jid)
+        got, _, _ = srv.result(jid)
+        assert np.array_equal(got[:, :6], want[k][0][:, :6]), k
+    for k, jid in enumerate(nb_ids):
+        assert srv.wait(jid, 120) == "done", srv.job_error(jid)
+        got, _, _ = srv.result(jid)
+        assert np.array_equal(got[:, :6], nb_want[k][:, :6]), k
+    st, dt = systems[0]
+    again = srv.submit_whfast(st, dt, steps=300)
+    assert srv.wait(again, 60) == "done"
+    assert np.array_equal(srv.result(again)[0][:, :6], want[0][0][:, :6])
+    s = srv.stats()
+    assert s["failed"] == len(bad)
+    srv.close()
