@@ -511,12 +511,14 @@ def sheet_leg(amd, args):
     # the CPU baseline's window: the steps right after the warm-up step (the
     # densest collision steps), timed alone, then the rest of the run
     n0 = sim.s.collisions_log_n
+    window = max(1, min(args.sheet_cpu_steps, args.sheet_steps))
     t0 = time.perf_counter()
-    sim.steps(args.sheet_cpu_steps)
+    sim.steps(window)
     L.reb_amd_synchronize_device(sim.ptr)
     wall_window = time.perf_counter() - t0
     ncoll_window = sim.s.collisions_log_n - n0
-    sim.steps(args.sheet_steps - args.sheet_cpu_steps)
+    if args.sheet_steps > window:
+        sim.steps(args.sheet_steps - window)
     L.reb_amd_synchronize_device(sim.ptr)
     wall = time.perf_counter() - t0
     ncoll = sim.s.collisions_log_n - n0
@@ -527,9 +529,9 @@ def sheet_leg(amd, args):
            "value": args.sheet_steps / wall, "unit": "steps/s", "steps": args.sheet_steps, "seconds": wall,
            "ms_per_step": 1e3 * wall / args.sheet_steps, "collisions_per_step": ncoll / args.sheet_steps,
            "tree_interactions_last_step": int(inter),
-           "window": {"steps": args.sheet_cpu_steps, "steps_per_s": args.sheet_cpu_steps / wall_window,
-                      "collisions_per_step": ncoll_window / args.sheet_cpu_steps,
-                      "note": "steps 2..%d, the CPU baseline's window" % (1 + args.sheet_cpu_steps)},
+           "window": {"steps": window, "steps_per_s": window / wall_window,
+                      "collisions_per_step": ncoll_window / window,
+                      "note": "steps 2..%d, the CPU baseline's window" % (1 + window)},
            "config": {"N": args.sheet_n, "root_size": args.sheet_root, "n_root": args.sheet_nroot,
                       "ghost_boxes": 2, "opening_angle2": 0.25, "seed": args.sheet_seed}}
     log("sheet: %.2f steps/s (%.1f ms/step), %.0f collisions/step, %d tree interactions/step" % (
@@ -540,7 +542,7 @@ def sheet_leg(amd, args):
         if os.path.exists(exe):
             threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
             env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
-            cmd = [exe, "sheet", str(args.sheet_n), str(args.sheet_cpu_steps), repr(float(args.sheet_root)),
+            cmd = [exe, "sheet", str(args.sheet_n), str(window), repr(float(args.sheet_root)),
                    str(args.sheet_nroot), str(args.sheet_seed)]
             log("sheet cpu baseline:", " ".join(cmd), "threads", threads)
             out = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=900)

@@ -622,8 +622,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                 }
             }
         };
-        // One pair term of BASIC (gravity.c:215-232) or COMPENSATED (:308-410),
-        // added to (a, k) in the reference's form.
         // One pair term of BASIC (gravity.c:215-232) or COMPENSATED (:308-410)
         // in the reference's form, and its addition to (a, k).
         auto pair_term = [&](double dx, double dy, double dz, double mj, double& t_x, double& t_y, double& t_z) {
