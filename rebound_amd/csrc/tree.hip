@@ -352,14 +352,22 @@ struct __align__(64) GravRec {
     int next, pt;
     double pad;
 };
+// A cell's record for the collision walk.  Inner cell (pt < 0): its
+// geometric centre and width.  Leaf (pt >= 0): its body's x, y, z and radius
+// in cx, cy, cz, w and its velocity in v, copied when the tree is built (the
+// search runs on the state the tree was built from), so the walk reads one
+// record per visited cell and never the scattered body arrays.
 struct __align__(64) CollCell {
     double cx, cy, cz, w;
     int next, pt;
-    double pad[3];
+    double v[3];
 };
 
 __global__ void k_tree_pack(const double* __restrict__ nd, const int* __restrict__ ni, size_t cap, int total,
-                            GravRec* __restrict__ g, CollCell* __restrict__ cc) {
+                            GravRec* __restrict__ g, CollCell* __restrict__ cc, const double* __restrict__ x,
+                            const double* __restrict__ y, const double* __restrict__ z,
+                            const double* __restrict__ vx, const double* __restrict__ vy,
+                            const double* __restrict__ vz, const double* __restrict__ rad) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= total) return;
     GravRec a;
@@ -373,13 +381,24 @@ __global__ void k_tree_pack(const double* __restrict__ nd, const int* __restrict
     a.pad = 0.;
     g[c] = a;
     CollCell b;
-    b.cx = nd[c];
-    b.cy = nd[cap + c];
-    b.cz = nd[2 * cap + c];
-    b.w = a.w;
     b.next = a.next;
     b.pt = a.pt;
-    b.pad[0] = b.pad[1] = b.pad[2] = 0.;
+    if (a.pt >= 0) {
+        const int p = a.pt;
+        b.cx = x[p];
+        b.cy = y[p];
+        b.cz = z[p];
+        b.w = rad[p];
+        b.v[0] = vx[p];
+        b.v[1] = vy[p];
+        b.v[2] = vz[p];
+    } else {
+        b.cx = nd[c];
+        b.cy = nd[cap + c];
+        b.cz = nd[2 * cap + c];
+        b.w = a.w;
+        b.v[0] = b.v[1] = b.v[2] = 0.;
+    }
     cc[c] = b;
 }
 
@@ -539,7 +558,9 @@ bool tree_build(struct reb_simulation* r) {
         return false;
     }
     hipLaunchKernelGGL(k_tree_pack, dim3((total + 255) / 256), dim3(256), 0, d->stream, T->node_d.as<double>(),
-                       T->node_i.as<int>(), T->node_cap, total, T->grec.as<GravRec>(), T->crec.as<CollCell>());
+                       T->node_i.as<int>(), T->node_cap, total, T->grec.as<GravRec>(), T->crec.as<CollCell>(),
+                       d->field(FX), d->field(FY), d->field(FZ), d->field(FVX), d->field(FVY), d->field(FVZ),
+                       d->field(FR));
     return RB_HIP(r, hipGetLastError());
 }
 
@@ -810,21 +831,21 @@ __global__ __launch_bounds__(64) void k_tree_collide(const CollCell* __restrict_
             bool open = false;
             if (c >= skip) {
                 if (pt >= 0) {
-                    if (pt != me) {
-                        const double dx = gx - x[pt];
-                        const double dy = gy - y[pt];
-                        const double dz = gz - z[pt];
+                    if (pt != me) {   // the leaf's record holds its body (k_tree_pack)
+                        const double dx = gx - cur.cx;
+                        const double dy = gy - cur.cy;
+                        const double dz = gz - cur.cz;
                         if (LINE) {
-                            if (line_overlap(dx, dy, dz, gvx - vx[pt], gvy - vy[pt], gvz - vz[pt], dt_last,
-                                             p1_r + rad[pt]))
+                            if (line_overlap(dx, dy, dz, gvx - cur.v[0], gvy - cur.v[1], gvz - cur.v[2], dt_last,
+                                             p1_r + cur.w))
                                 emit(pt, g, ri);
                         } else {
                             const double r2 = dx * dx + dy * dy + dz * dz;
-                            const double rp = p1_r + rad[pt];
+                            const double rp = p1_r + cur.w;
                             if (!(r2 > rp * rp)) {
-                                const double dvx = gvx - vx[pt];
-                                const double dvy = gvy - vy[pt];
-                                const double dvz = gvz - vz[pt];
+                                const double dvx = gvx - cur.v[0];
+                                const double dvy = gvy - cur.v[1];
+                                const double dvz = gvz - cur.v[2];
                                 if (!(dvx * dx + dvy * dy + dvz * dz > 0)) emit(pt, g, ri);
                             }
                         }

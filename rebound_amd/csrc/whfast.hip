@@ -369,6 +369,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
         const bool live = me < N;
         const int NA = sys_ok ? S.Nact : 0;        // N_active of the whfast sums
         const int NAG = sys_ok ? S.Nact_grav : 0;  // N_active of the gravity loop
+        // Every system of a task has the same N, Nact, Nact_grav and
+        // testparticle_type (the callers build tasks from systems of one body
+        // count), so the loops over a system's bodies take the task's bounds,
+        // read from lane 0 (always a system of the task) into scalar
+        // registers.  Lanes without a system run those loops on their own
+        // slots and discard the results (they store nothing).
+        const int Nw = __builtin_amdgcn_readfirstlane(N);
+        const int NAw = __builtin_amdgcn_readfirstlane(NA);
+        const int NAGw = __builtin_amdgcn_readfirstlane(NAG);
+        const int TPw = __builtin_amdgcn_readfirstlane(sys_ok ? S.tp_type : 0);
         const size_t st = P.stride;
         const size_t b0 = (size_t)S.body0;
         // ---- load
@@ -426,7 +436,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
         auto interior_mass = [&]() {
             publish(Slot{0., 0., 0., 0., 0., 0., hm});
             double eta = sm;
-            GROUP_LOOP(k, 1, NA) {
+            GROUP_LOOP(k, 1, NAw) {
                 if (k <= me) eta += slot[gbase + k - 1].m;
             }
             eta_me = eta;
@@ -441,7 +451,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                 double eta = sm;
                 double s_x = eta * sx, s_y = eta * sy, s_z = eta * sz;
                 double s_vx = eta * svx, s_vy = eta * svy, s_vz = eta * svz;
-                GROUP_LOOP(k, 1, NA) {
+                GROUP_LOOP(k, 1, NAw) {
                     const Slot q = slot[gbase + k - 1];
                     const double ei = 1. / eta;
                     eta += q.m;
@@ -480,7 +490,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                     vx0 += svx * m; vy0 += svy * m; vz0 += svz * m;
                     m0 += m;
                 }
-                GROUP_LOOP(k, 1, NA) {
+                GROUP_LOOP(k, 1, NAw) {
                     const Slot q = slot[gbase + k - 1];
                     x0 += q.x; y0 += q.y; z0 += q.z;
                     vx0 += q.vx; vy0 += q.vy; vz0 += q.vz;
@@ -521,7 +531,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                         nvx = hvx + s_vx * ei; nvy = hvy + s_vy * ei; nvz = hvz + s_vz * ei;
                     }
                 }
-                for (int k = NA - 1; k > 0; k--) {
+                for (int k = NAw - 1; k > 0; k--) {
                     const Slot q = slot[gbase + k - 1];
                     const double ei = 1. / eta;
                     s_x = (s_x - q.m * q.x) * ei;
@@ -568,7 +578,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                 publish(t);
                 double x0 = 0., y0 = 0., z0 = 0.;
                 double vx0 = 0., vy0 = 0., vz0 = 0.;
-                GROUP_LOOP(k, 1, NA) {
+                GROUP_LOOP(k, 1, NAw) {
                     const Slot q = slot[gbase + k - 1];
                     x0 += q.x;
                     y0 += q.y;
@@ -608,7 +618,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
             if constexpr (!JAC) {
                 publish(Slot{hm * hvx, hm * hvy, hm * hvz, 0., 0., 0., 0.});
                 double qx = 0, qy = 0, qz = 0;
-                GROUP_LOOP(k, 1, NA) {
+                GROUP_LOOP(k, 1, NAw) {
                     const Slot q = slot[gbase + k - 1];
                     qx += q.x;
                     qy += q.y;
@@ -672,8 +682,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
             if constexpr (JAC) {
                 if (en) {
                     double a_x = 0., a_y = 0., a_z = 0., k_x = 0., k_y = 0., k_z = 0.;
-                    for (int j = 2; j < N; j++) {
-                        if (!(j < NAG || S.tp_type)) break;
+                    for (int j = 2; j < Nw; j++) {
+                        if (!(j < NAGw || TPw)) break;
                         const Slot q = slot[gbase + j - 1];
                         pair_add(q.vx, q.vy, q.vz, a_x, a_y, a_z, k_x, k_y, k_z);
                     }
@@ -689,9 +699,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                 pair_add(tx, ty, tz, ax, ay, az, kx, ky, kz);
             }
             for (int pass = 0; pass < 2; pass++) {
-                const int j0 = pass == 0 ? 1 : (NAG > 1 ? NAG : 1);
-                const int j1 = pass == 0 ? NAG : N;
-                if (pass == 1 && !(S.tp_type && me < NAG)) break;
+                const int j0 = pass == 0 ? 1 : (NAGw > 1 ? NAGw : 1);
+                const int j1 = pass == 0 ? NAGw : Nw;
+                if (pass == 1 && !(TPw && me < NAGw)) break;
                 for (int jb = j0; jb < j1; jb += 4) {
                     double tx[4], ty[4], tz[4];
 #pragma unroll
@@ -724,7 +734,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
             publish(Slot{ix, iy, iz, 0., 0., 0., im});
             if (gl == 0 && sys_ok) {
                 double Rx = 0., Ry = 0., Rz = 0., Mt = 0.;
-                for (int j = 0; j < N; j++) {
+                for (int j = 0; j < Nw; j++) {
                     double* aj = acc_of(j);
                     aj[0] = 0.; aj[1] = 0.; aj[2] = 0.;
                     const double xj = bx(j), yj = by(j), zj = bz(j), mj = bm(j);
@@ -779,7 +789,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                 auto bay = [&](int k) { return k == 0 ? say : slot[gbase + k - 1].vy; };
                 auto baz = [&](int k) { return k == 0 ? saz : slot[gbase + k - 1].vz; };
                 double Rx = 0., Ry = 0., Rz = 0., Mt = 0., Ax = 0., Ay = 0., Az = 0.;
-                for (int j = 0; j < N; j++) {
+                for (int j = 0; j < Nw; j++) {
                     double* kj = acc_of(j);
                     kj[0] = 0.; kj[1] = 0.; kj[2] = 0.;
                     const double xj = bx(j), yj = by(j), zj = bz(j), mj = bm(j);
@@ -871,7 +881,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                 double eta = sm;
                 double s_x = eta * sax, s_y = eta * say, s_z = eta * saz;
                 double jax = 0., jay = 0., jaz = 0.;
-                GROUP_LOOP(k, 1, NA) {
+                GROUP_LOOP(k, 1, NAw) {
                     const Slot q = slot[gbase + k - 1];
                     const double ei = 1. / eta;
                     eta += q.m;
