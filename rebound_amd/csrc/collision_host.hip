@@ -500,6 +500,11 @@ struct rbamd::ResolveScratch {
     std::vector<int> draws;
     std::vector<unsigned int> seeds;
     unsigned int draw_seed = 0;
+    // the shuffle made while the device gathers the bodies: perm_h holds it
+    // for perm_nc collisions, and r->rand_seed was seed_before
+    bool perm_ready = false;
+    int perm_nc = 0;
+    unsigned int seed_before = 0;
     std::vector<double> plog;
     std::vector<unsigned char> hit;
     void release() {
@@ -521,6 +526,8 @@ struct rbamd::ResolveScratch {
 };
 
 namespace rbamd {
+
+static void shuffle_perm(struct reb_simulation* r, ResolveScratch& W, int nc);
 
 void resolve_release(struct reb_simulation* r) {
     DeviceState* d = dev_of(r);
@@ -572,7 +579,7 @@ bool compact_resolve_mark(struct reb_simulation* r, const CollRecH* rec, size_t 
     W.ready = false;
     const int N = (int)r->N;
     size_t tmp = 0;
-    exclusive_sum(nullptr, tmp, W.flag.as<int>(), W.cidx.as<int>(), N + 1, d->stream);
+    (void)exclusive_sum(nullptr, tmp, W.flag.as<int>(), W.cidx.as<int>(), N + 1, d->stream);
     if (!W.flag.ensure(sizeof(int) * (N + 1)) || !W.cidx.ensure(sizeof(int) * (N + 1)) ||
         !W.prim_tmp.ensure(tmp + 256)) {
         reb_simulation_error(r, "Out of device memory for the collision resolve.");
@@ -582,7 +589,8 @@ bool compact_resolve_mark(struct reb_simulation* r, const CollRecH* rec, size_t 
     if (!RB_HIP(r, hipMemsetAsync(W.flag.p, 0, sizeof(int) * (N + 1), d->stream))) return false;
     hipLaunchKernelGGL(k_coll_mark, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, d->stream, rec, (int)nc,
                        W.flag.as<int>());
-    exclusive_sum(W.prim_tmp.p, tmp, W.flag.as<int>(), W.cidx.as<int>(), N + 1, d->stream);
+    if (!RB_HIP(r, exclusive_sum(W.prim_tmp.p, tmp, W.flag.as<int>(), W.cidx.as<int>(), N + 1, d->stream)))
+        return false;
     return RB_HIP(r, hipMemcpyAsync(W.n_h, W.cidx.as<int>() + N, sizeof(int), hipMemcpyDeviceToHost, d->stream));
 }
 
@@ -610,6 +618,12 @@ bool compact_resolve_gather(struct reb_simulation* r, const CollRecH* rec, size_
     W.drec = rec;
     W.n = n;
     W.ready = true;
+    // the resolve follows this search at once (its shuffle is the next use
+    // of the seed): shuffle now, while the device gathers the bodies
+    W.seed_before = r->rand_seed;
+    shuffle_perm(r, W, (int)nc);
+    W.perm_ready = true;
+    W.perm_nc = (int)nc;
     return true;
 }
 
@@ -657,7 +671,7 @@ static int device_levels(struct reb_simulation* r, ResolveScratch& W, int nc, in
     int* hist = start + kLevBins;
     int* offs = hist + (size_t)kLevBins * kLevBlocks;
     size_t t_scan = 0;
-    exclusive_sum(nullptr, t_scan, W.bcnt.as<int>(), W.boff.as<int>(), n + 1, d->stream);
+    (void)exclusive_sum(nullptr, t_scan, W.bcnt.as<int>(), W.boff.as<int>(), n + 1, d->stream);
     if (!W.lprim_tmp.ensure(t_scan + 256)) {
         reb_simulation_error(r, "Out of device memory for the collision resolve.");
         return -1;
@@ -669,7 +683,8 @@ static int device_levels(struct reb_simulation* r, ResolveScratch& W, int nc, in
         !RB_HIP(r, hipMemsetAsync(W.bcur.p, 0, sizeof(int) * n, d->stream)))
         return -1;
     hipLaunchKernelGGL(k_lev_body_count, gk, blk, 0, d->stream, cl, nc, W.bcnt.as<int>());
-    exclusive_sum(W.lprim_tmp.p, t_scan, W.bcnt.as<int>(), W.boff.as<int>(), n + 1, d->stream);
+    if (!RB_HIP(r, exclusive_sum(W.lprim_tmp.p, t_scan, W.bcnt.as<int>(), W.boff.as<int>(), n + 1, d->stream)))
+        return -1;
     hipLaunchKernelGGL(k_lev_body_fill, gk, blk, 0, d->stream, cl, nc, W.boff.as<int>(), W.bcur.as<int>(),
                        W.blist.as<int>());
     hipLaunchKernelGGL(k_lev_links, gm, blk, 0, d->stream, cl, nc, W.boff.as<int>(), W.blist.as<int>(),
@@ -742,23 +757,13 @@ static void host_levels(ResolveScratch& W, const HSPair* cl, int nc, int n, int*
     *nlev = top;
 }
 
-// The built-in hard-sphere resolver with a constant restitution, on the
-// compact records prepared by compact_resolve_gather: only the eight fields it
-// reads come over, only the velocities go back, and collisions of one level
-// run in parallel.
-static void resolve_hardsphere_compact(struct reb_simulation* r) {
-    DeviceState* d = dev_of(r);
-    ResolveScratch& W = *d->resolve;
-    W.ready = false;
-    const CollList& L = *d->coll;
-    const int nc = (int)L.n;
-    const int n = W.n;
-    // shuffle (collision.c:336-342) as a permutation of the detection order,
-    // the same swaps applied to indices instead of 72-byte records; drawn
-    // while the body records are on their way
+// Shuffle (collision.c:336-342) as a permutation of the detection order in
+// W.perm_h: the same swaps applied to indices instead of 72-byte records,
+// with the rand_r draws made ahead while the device searched when they start
+// from the current seed.  Advances r->rand_seed as the reference does.
+static void shuffle_perm(struct reb_simulation* r, ResolveScratch& W, int nc) {
     int* perm = W.perm_h;
     for (int k = 0; k < nc; k++) perm[k] = k;
-    // the draws made ahead, if they start from the current seed
     const int ahead = W.draw_seed == r->rand_seed ? std::min(nc, (int)W.draws.size()) : 0;
     for (int i = 0; i < ahead; i++) {
         const int nw = (int)((size_t)W.draws[i] % (size_t)nc);
@@ -770,6 +775,23 @@ static void resolve_hardsphere_compact(struct reb_simulation* r) {
         std::swap(perm[i], perm[nw]);
     }
     W.draws.clear();
+}
+
+// The built-in hard-sphere resolver with a constant restitution, on the
+// compact records prepared by compact_resolve_gather: only the eight fields it
+// reads come over, only the velocities go back, and collisions of one level
+// run in parallel.
+static void resolve_hardsphere_compact(struct reb_simulation* r) {
+    DeviceState* d = dev_of(r);
+    ResolveScratch& W = *d->resolve;
+    W.ready = false;
+    const CollList& L = *d->coll;
+    const int nc = (int)L.n;
+    const int n = W.n;
+    if (W.perm_ready && W.perm_nc != nc) r->rand_seed = W.seed_before;   // (not expected) shuffle again
+    if (!W.perm_ready || W.perm_nc != nc) shuffle_perm(r, W, nc);
+    W.perm_ready = false;
+    int* perm = W.perm_h;
     CollGB gb;
     gb.n = (int)L.gb.size();
     for (int k = 0; k < gb.n; k++) gb.g[k] = L.gb[k];
@@ -848,7 +870,13 @@ void collision_resolve_host(struct reb_simulation* r) {
         resolve_hardsphere_compact(r);   // shuffles, resolves and writes r->collisions
         return;
     }
-    if (d->resolve) d->resolve->ready = false;
+    if (d->resolve) {
+        d->resolve->ready = false;
+        if (d->resolve->perm_ready) {   // a shuffle made for the compact path: undo its draws
+            r->rand_seed = d->resolve->seed_before;
+            d->resolve->perm_ready = false;
+        }
+    }
     if (!materialize_collisions(r)) return;
     // shuffle (collision.c:336-342)
     for (size_t i = 0; i < r->N_collisions; i++) {

@@ -447,9 +447,9 @@ static bool tree_sort(struct reb_simulation* r, TreeBuffers* T, int n, bool two)
     int* Ib = T->idxb.as<int>();
     const int keybits = 3 * T->L1 + T->rbits;
     size_t tmp = 0;
-    rocprim::radix_sort_pairs(nullptr, tmp, K0, K0b, I, Ib, n, 0, keybits, d->stream);
+    (void)rocprim::radix_sort_pairs(nullptr, tmp, K0, K0b, I, Ib, n, 0, keybits, d->stream);
     size_t tmp2 = 0;
-    rocprim::radix_sort_pairs(nullptr, tmp2, K1, K1b, I, Ib, n, 0, 3 * T->L2, d->stream);
+    (void)rocprim::radix_sort_pairs(nullptr, tmp2, K1, K1b, I, Ib, n, 0, 3 * T->L2, d->stream);
     tmp = std::max(tmp, tmp2);
     if (!T->prim_tmp.ensure(tmp + 256)) return false;
     if (!two) {
@@ -505,9 +505,10 @@ bool tree_build(struct reb_simulation* r) {
         hipLaunchKernelGGL(k_tree_count, dim3((n + 255) / 256), dim3(256), 0, d->stream, T->lcp.as<int>(), n,
                            T->cnt.as<int>(), info);
         size_t tmp = 0;
-        exclusive_sum(nullptr, tmp, T->cnt.as<int>(), T->base.as<int>(), n, d->stream);
+        (void)exclusive_sum(nullptr, tmp, T->cnt.as<int>(), T->base.as<int>(), n, d->stream);
         if (!T->prim_tmp.ensure(tmp + 256)) return false;
-        exclusive_sum(T->prim_tmp.p, tmp, T->cnt.as<int>(), T->base.as<int>(), n, d->stream);
+        if (!RB_HIP(r, exclusive_sum(T->prim_tmp.p, tmp, T->cnt.as<int>(), T->base.as<int>(), n, d->stream)))
+            return false;
         if (!RB_HIP(r, hipMemcpyAsync(&last[0], T->base.as<int>() + (n - 1), 4, hipMemcpyDeviceToHost, d->stream)) ||
             !RB_HIP(r, hipMemcpyAsync(&last[1], T->cnt.as<int>() + (n - 1), 4, hipMemcpyDeviceToHost, d->stream)) ||
             !RB_HIP(r, hipMemcpyAsync(hinfo, info, 16, hipMemcpyDeviceToHost, d->stream)))
@@ -977,11 +978,13 @@ static bool ensure_coll(struct reb_simulation* r, TreeBuffers* T, size_t nslots,
 static long scan_counts(struct reb_simulation* r, TreeBuffers* T, int nslots) {
     DeviceState* d = dev_of(r);
     size_t tmp = 0;
-    exclusive_sum(nullptr, tmp, T->coll_cnt.as<int>(), T->coll_off.as<int>(), nslots + 1, d->stream);
+    (void)exclusive_sum(nullptr, tmp, T->coll_cnt.as<int>(), T->coll_off.as<int>(), nslots + 1, d->stream);
     if (!T->prim_tmp.ensure(tmp + 256)) return -1;
     // counts[nslots] = 0 so that offsets[nslots] is the total
     if (!RB_HIP(r, hipMemsetAsync(T->coll_cnt.as<int>() + nslots, 0, 4, d->stream))) return -1;
-    exclusive_sum(T->prim_tmp.p, tmp, T->coll_cnt.as<int>(), T->coll_off.as<int>(), nslots + 1, d->stream);
+    if (!RB_HIP(r, exclusive_sum(T->prim_tmp.p, tmp, T->coll_cnt.as<int>(), T->coll_off.as<int>(), nslots + 1,
+                                 d->stream)))
+        return -1;
     int total = 0;
     if (!RB_HIP(r, hipMemcpyAsync(&total, T->coll_off.as<int>() + nslots, 4, hipMemcpyDeviceToHost, d->stream)))
         return -1;
