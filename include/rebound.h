@@ -390,6 +390,7 @@ REB_API int reb_particle_cmp(struct reb_particle p1, struct reb_particle p2);
 /* particle.c:128-136 (-1 when p is not one of its simulation's particles),
  * :375-410; tools.c:454-461, :1524-1569 (-1 when the counts differ) */
 REB_API int reb_simulation_particle_index(struct reb_particle* p);
+REB_API int reb_simulation_particle_var_index(struct reb_particle* p);
 REB_API struct reb_particle reb_simulation_jacobi_com(struct reb_particle* p);
 REB_API void reb_particle_iadd(struct reb_particle* p1, struct reb_particle* p2);
 REB_API void reb_particle_isub(struct reb_particle* p1, struct reb_particle* p2);
@@ -447,6 +448,27 @@ REB_API double reb_random_rayleigh(struct reb_simulation* r, double sigma);
 /* ---------------------------------------------------------------- boundary
  * boundary.c:145-201 */
 REB_API struct reb_vec6d reb_boundary_get_ghostbox(struct reb_simulation* const r, int i, int j, int k);
+
+/* ---------------------------------------------------------------- variational equations, MEGNO
+ * rebound.h:702-734, tools.c:1313-1522.  First-order sets of all bodies are
+ * integrated by WHFast in Jacobi coordinates (default kernel, safe_mode for
+ * MEGNO), on the GPU, as the reference's integrator_whfast.c:392-490. */
+struct reb_variational_configuration {
+    struct reb_simulation* sim;
+    int order;              /* 1 or 2 */
+    int index;              /* first entry in particles_var */
+    int testparticle;       /* -1: every body varied */
+    int index_1st_order_a;  /* order 2 only */
+    int index_1st_order_b;
+    double lrescale;        /* sum of the logarithms of the rescalings */
+};
+REB_API int reb_simulation_add_variation_1st_order(struct reb_simulation* const r, int testparticle);
+REB_API int reb_simulation_add_variation_2nd_order(struct reb_simulation* const r, int testparticle, size_t index_1st_order_a, size_t index_1st_order_b);
+REB_API void reb_simulation_rescale_var(struct reb_simulation* const r);
+REB_API void reb_simulation_init_megno(struct reb_simulation* const r);
+REB_API void reb_simulation_init_megno_seed(struct reb_simulation* const r, unsigned int seed);
+REB_API double reb_simulation_megno(struct reb_simulation* r);
+REB_API double reb_simulation_lyapunov(struct reb_simulation* r);
 
 /* ---------------------------------------------------------------- WHFast helper
  * integrator_whfast.c:241-427.  Advances one body on a Kepler orbit (mu=G*M)

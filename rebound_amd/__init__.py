@@ -211,6 +211,43 @@ class Sim:
     def move_to_com(self):
         self.lib.reb_simulation_move_to_com(self.ptr)
 
+    # ------------------------------------------------------------ variational sets, MEGNO
+    def add_variation(self, order=1, testparticle=-1, first_a=0, first_b=0):
+        if order == 1:
+            i = self.lib.reb_simulation_add_variation_1st_order(self.ptr, testparticle)
+        else:
+            i = self.lib.reb_simulation_add_variation_2nd_order(self.ptr, testparticle, first_a, first_b)
+        self.check()
+        return i
+
+    def init_megno(self, seed=None):
+        if seed is None:
+            self.lib.reb_simulation_init_megno(self.ptr)
+        else:
+            self.lib.reb_simulation_init_megno_seed(self.ptr, seed)
+        self.check()
+
+    def megno(self):
+        return self.lib.reb_simulation_megno(self.ptr)
+
+    def lyapunov(self):
+        return self.lib.reb_simulation_lyapunov(self.ptr)
+
+    def var_particles(self):
+        """Structured numpy view of r->particles_var (valid until the array grows)."""
+        n = self.s.N_var
+        if n == 0:
+            return np.zeros(0, dtype=PARTICLE_DTYPE)
+        buf = (ctypes.c_char * (n * 112)).from_address(ctypes.addressof(self.s.particles_var.contents))
+        return np.frombuffer(buf, dtype=PARTICLE_DTYPE, count=n)
+
+    def var_configs(self):
+        n = self.s.N_var_config
+        if n == 0:
+            return []
+        arr = ctypes.cast(self.s.var_config, ctypes.POINTER(_abi.VarConfig))
+        return [arr[i] for i in range(n)]
+
 
 from .ensemble import Ensemble, planetary_systems  # noqa: E402
 from .serve import Server  # noqa: E402

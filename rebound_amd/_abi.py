@@ -107,6 +107,12 @@ Simulation._fields_ = [
     ("free_particle_ap", c_void_p), ("extras_cleanup", c_void_p), ("extras", c_void_p),
 ]
 
+class VarConfig(Structure):
+    """struct reb_variational_configuration (rebound.h:702-710)."""
+    _fields_ = [("sim", c_void_p), ("order", c_int), ("index", c_int), ("testparticle", c_int),
+                ("index_1st_order_a", c_int), ("index_1st_order_b", c_int), ("lrescale", c_double)]
+
+
 # enum values (rebound.h:209-233, 372-393; integrator_whfast.h:31-45)
 JOB_STATE = {-2: "unknown", -1: "failed", 0: "pending", 1: "running", 2: "done"}
 GRAVITY = {"none": 0, "basic": 1, "compensated": 2, "tree": 3, "jacobi": 5, "custom": 7}
@@ -148,6 +154,13 @@ def declare(lib):
         "reb_simulation_energy": (c_double, [S]),
         "reb_simulation_com": (Particle, [S]),
         "reb_simulation_move_to_com": (None, [S]),
+        "reb_simulation_add_variation_1st_order": (c_int, [S, c_int]),
+        "reb_simulation_add_variation_2nd_order": (c_int, [S, c_int, c_size_t, c_size_t]),
+        "reb_simulation_rescale_var": (None, [S]),
+        "reb_simulation_init_megno": (None, [S]),
+        "reb_simulation_init_megno_seed": (None, [S, c_uint]),
+        "reb_simulation_megno": (c_double, [S]),
+        "reb_simulation_lyapunov": (c_double, [S]),
         "reb_simulation_move_to_hel": (None, [S]),
         "reb_simulation_stop": (None, [S]),
         "reb_simulation_error": (None, [S, c_char_p]),

@@ -126,10 +126,12 @@ const Desc kSimFields[] = {
     SIMF(did_modify_particles, UINT),
     Desc{"particles", T::POINTER, offsetof(struct reb_simulation, particles), offsetof(struct reb_simulation, N),
          sizeof(struct reb_particle)},
-    // particles_var and var_config (variational particles) and display_settings
-    // are not part of this engine: written only when present, which they never are
+    // variational particles and their sets (binarydata.c:94-95); the sets
+    // are stored as their struct, as the reference does
     Desc{"particles_var", T::POINTER, offsetof(struct reb_simulation, particles_var),
          offsetof(struct reb_simulation, N_var), sizeof(struct reb_particle)},
+    Desc{"var_config", T::POINTER, offsetof(struct reb_simulation, var_config),
+         offsetof(struct reb_simulation, N_var_config), sizeof(struct reb_variational_configuration)},
     SIMF(simulationarchive_version, INT),
     SIMF(walltime, DOUBLE),
     SIMF(walltime_last_steps, DOUBLE),
@@ -534,6 +536,19 @@ void finish_input(struct reb_simulation* r, unsigned* warnings) {
         // a damaged file gave the count but not the bodies
         *warnings |= W_CORRUPT;
         r->N = 0;
+    }
+    if (r->N_var && !r->particles_var) {
+        *warnings |= W_CORRUPT;
+        r->N_var = 0;
+    }
+    if (r->N_var_config && !r->var_config) {
+        *warnings |= W_CORRUPT;
+        r->N_var_config = 0;
+    }
+    for (size_t l = 0; l < r->N_var_config; l++) r->var_config[l].sim = r;
+    for (size_t l = 0; l < r->N_var; l++) {
+        r->particles_var[l].ap = nullptr;
+        r->particles_var[l].sim = r;
     }
     r->N_allocated = r->N;
     for (size_t l = 0; l < r->N; l++) {

@@ -55,7 +55,8 @@ void update_acceleration_internal(struct reb_simulation* r) {
     mark_device_newer(r);
     if (!ok) return;
     if (r->N_var) {
-        reb_simulation_error(r, "Variational particles are not supported by this engine.");
+        // the WHFast plugin integrates variational sets inside its own kernel
+        reb_simulation_error(r, "Variational particles are supported with the WHFast integrator only on this engine.");
         return;
     }
     DeviceState* d = dev_of(r);
@@ -128,7 +129,13 @@ static void reb_simulation_step(struct reb_simulation* const r) {
         run_host_callback(r, r->post_timestep_modifications);
     }
     r->did_modify_particles = 0;
-    if (r->N_var) reb_simulation_error(r, "Variational particles are not supported by this engine.");
+    if (r->N_var) {
+        const char* n = r->integrator.name;
+        if (n && strcmp(n, "whfast") == 0)
+            reb_simulation_rescale_var(r);
+        else
+            reb_simulation_error(r, "Variational particles are supported with the WHFast integrator only on this engine.");
+    }
     if (r->boundary != REB_BOUNDARY_NONE) boundary_device(r);
     if (r->collision != REB_COLLISION_NONE) {
         if (collision_search_device(r)) collision_resolve_host(r);

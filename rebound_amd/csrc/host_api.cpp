@@ -235,6 +235,8 @@ REB_API void reb_simulation_free(struct reb_simulation* const r) {
     if (r->free_particle_ap)
         for (size_t i = 0; i < r->N; i++) r->free_particle_ap(&r->particles[i]);
     free(r->particles);
+    free(r->particles_var);
+    free(r->var_config);
     collisions_free(r);
     free(r->gravity_cs);
     free(r->simulationarchive_filename);
@@ -256,6 +258,19 @@ REB_API struct reb_simulation* reb_simulation_copy(struct reb_simulation* r) {
     c->particles = (struct reb_particle*)malloc(sizeof(struct reb_particle) * (r->N_allocated ? r->N_allocated : 1));
     if (r->N) memcpy(c->particles, r->particles, sizeof(struct reb_particle) * r->N);
     for (size_t i = 0; i < r->N; i++) c->particles[i].sim = c;
+    c->particles_var = nullptr;
+    c->var_config = nullptr;
+    if (r->N_var) {
+        c->particles_var = (struct reb_particle*)malloc(sizeof(struct reb_particle) * r->N_var);
+        memcpy(c->particles_var, r->particles_var, sizeof(struct reb_particle) * r->N_var);
+        for (size_t i = 0; i < r->N_var; i++) c->particles_var[i].sim = c;
+    }
+    if (r->N_var_config) {
+        c->var_config = (struct reb_variational_configuration*)malloc(sizeof(struct reb_variational_configuration) *
+                                                                       r->N_var_config);
+        memcpy(c->var_config, r->var_config, sizeof(struct reb_variational_configuration) * r->N_var_config);
+        for (size_t v = 0; v < r->N_var_config; v++) c->var_config[v].sim = c;
+    }
     c->collisions = nullptr;
     c->N_allocated_collisions = 0;
     c->N_collisions = 0;
@@ -585,12 +600,77 @@ REB_API struct reb_particle reb_simulation_com_range(struct reb_simulation* r, s
 
 REB_API struct reb_particle reb_simulation_com(struct reb_simulation* r) { return reb_simulation_com_range(r, 0, r->N); }
 
+// tools.c:201-375: the variational sets are shifted by the variation of the
+// centre of mass (second order first, then first order), then the bodies.
 REB_API void reb_simulation_move_to_com(struct reb_simulation* const r) {
-    if (r->N_var_config) {
-        reb_simulation_error(r, "Variational particles are not supported by this engine.");
-        return;
-    }
     struct reb_particle com = reb_simulation_com(r);
+    const size_t N = r->N;
+    struct reb_particle* const p = r->particles;
+    struct reb_particle* const pv = r->particles_var;
+    // component k of a particle: x y z vx vy vz
+    auto c = [](const struct reb_particle& q, int k) {
+        switch (k) {
+            case 0: return q.x;
+            case 1: return q.y;
+            case 2: return q.z;
+            case 3: return q.vx;
+            case 4: return q.vy;
+            default: return q.vz;
+        }
+    };
+    auto shift = [&](size_t index, const double* s) {
+        for (size_t i = 0; i < N; i++) {
+            pv[i + index].x -= s[0];
+            pv[i + index].y -= s[1];
+            pv[i + index].z -= s[2];
+            pv[i + index].vx -= s[3];
+            pv[i + index].vy -= s[4];
+            pv[i + index].vz -= s[5];
+        }
+    };
+    for (size_t v = 0; v < r->N_var_config; v++) {
+        const struct reb_variational_configuration& vc = r->var_config[v];
+        if (vc.testparticle >= 0 || vc.order != 2) continue;
+        const size_t index = (size_t)vc.index, a = (size_t)vc.index_1st_order_a, b = (size_t)vc.index_1st_order_b;
+        double dma = 0., dmb = 0., ddm = 0.;
+        for (size_t i = 0; i < N; i++) {
+            dma += pv[i + a].m;
+            dmb += pv[i + b].m;
+            ddm += pv[i + index].m;
+        }
+        double s[6] = {0., 0., 0., 0., 0., 0.};
+        for (size_t i = 0; i < N; i++)
+            for (int k = 0; k < 6; k++) {
+                s[k] += c(pv[i + index], k) / com.m * p[i].m;
+                s[k] += c(pv[i + a], k) / com.m * pv[i + b].m;
+                s[k] -= c(pv[i + a], k) * p[i].m / com.m / com.m * dmb;
+                s[k] += c(pv[i + b], k) / com.m * pv[i + a].m;
+                s[k] += c(p[i], k) / com.m * pv[i + index].m;
+                s[k] -= c(p[i], k) * pv[i + a].m / com.m / com.m * dmb;
+                s[k] -= c(pv[i + b], k) * p[i].m / com.m / com.m * dma;
+                s[k] -= c(p[i], k) * pv[i + b].m / com.m / com.m * dma;
+                s[k] += 2. * c(p[i], k) * p[i].m / com.m / com.m / com.m * dma * dmb;
+                s[k] -= c(p[i], k) * p[i].m / com.m / com.m * ddm;
+            }
+        shift(index, s);
+    }
+    for (size_t v = 0; v < r->N_var_config; v++) {
+        const struct reb_variational_configuration& vc = r->var_config[v];
+        if (vc.testparticle >= 0 || vc.order != 1) continue;
+        const size_t index = (size_t)vc.index;
+        // tools.c:319-321 sums particles[i+index].m (the bodies' array, at
+        // the set's offset); entries past the bodies count as zero here
+        double dm = 0.;
+        for (size_t i = 0; i < N; i++) dm += i + index < N ? p[i + index].m : 0.;
+        double s[6] = {0., 0., 0., 0., 0., 0.};
+        for (size_t i = 0; i < N; i++)
+            for (int k = 0; k < 6; k++) {
+                s[k] += p[i].m / com.m * c(pv[i + index], k);
+                s[k] += c(p[i], k) / com.m * pv[i + index].m;
+                s[k] -= c(p[i], k) / (com.m * com.m) * p[i].m * dm;
+            }
+        shift(index, s);
+    }
     for (size_t i = 0; i < r->N; i++) {
         r->particles[i].x -= com.x;
         r->particles[i].y -= com.y;
@@ -605,6 +685,140 @@ REB_API void reb_simulation_move_to_com(struct reb_simulation* const r) {
         if (ensure_device(r) && boundary_device(r)) ensure_host(r);
     }
 }
+
+// ---------------------------------------------------------------- variational sets
+// tools.c:1382-1430
+static int add_var_config(struct reb_simulation* r, int order, int testparticle, size_t a, size_t b) {
+    r->N_var_config++;
+    r->var_config = (struct reb_variational_configuration*)realloc(
+        r->var_config, sizeof(struct reb_variational_configuration) * r->N_var_config);
+    struct reb_variational_configuration& vc = r->var_config[r->N_var_config - 1];
+    memset(&vc, 0, sizeof(vc));
+    const size_t index = r->N_var;
+    vc.sim = r;
+    vc.order = order;
+    vc.index = (int)index;
+    vc.lrescale = 0;
+    vc.testparticle = testparticle;
+    vc.index_1st_order_a = (int)a;
+    vc.index_1st_order_b = (int)b;
+    const size_t n = testparticle >= 0 ? 1 : r->N;
+    r->particles_var = (struct reb_particle*)realloc(r->particles_var, sizeof(struct reb_particle) * (r->N_var + n));
+    for (size_t i = 0; i < n; i++) {
+        memset(&r->particles_var[r->N_var + i], 0, sizeof(struct reb_particle));
+        r->particles_var[r->N_var + i].sim = r;
+    }
+    r->N_var += n;
+    return (int)index;
+}
+
+REB_API int reb_simulation_add_variation_1st_order(struct reb_simulation* const r, int testparticle) {
+    return add_var_config(r, 1, testparticle, 0, 0);
+}
+
+REB_API int reb_simulation_add_variation_2nd_order(struct reb_simulation* const r, int testparticle,
+                                                   size_t index_1st_order_a, size_t index_1st_order_b) {
+    return add_var_config(r, 2, testparticle, index_1st_order_a, index_1st_order_b);
+}
+
+// tools.c:1313-1380, on the host copy of the variational particles (which
+// the WHFast plugin keeps current after every step).
+REB_API void reb_simulation_rescale_var(struct reb_simulation* const r) {
+    if (r->N_var_config == 0) return;
+    for (size_t v = 0; v < r->N_var_config; v++) {
+        struct reb_variational_configuration* vc = &r->var_config[v];
+        if (vc->lrescale < 0) continue;
+        const size_t N = vc->testparticle < 0 ? r->N : 1;
+        double scale = 0;
+        struct reb_particle* const q = r->particles_var + vc->index;
+        for (size_t i = 0; i < N; i++) {
+            const double f[6] = {fabs(q[i].x), fabs(q[i].y), fabs(q[i].z), fabs(q[i].vx), fabs(q[i].vy), fabs(q[i].vz)};
+            for (double a : f) scale = a > scale ? a : scale;
+        }
+        if (!(scale > 1e100)) continue;
+        if (vc->order == 1) {
+            for (size_t w = 0; w < r->N_var_config; w++) {
+                const struct reb_variational_configuration* wc = &r->var_config[w];
+                if (wc->order == 2 && (wc->index_1st_order_a == vc->index || wc->index_1st_order_b == vc->index) &&
+                    !(r->messages_var_rescale_warning & 4)) {
+                    r->messages_var_rescale_warning |= 4;
+                    reb_simulation_warning(r, "Rescaling a set of variational equations of order 1 which are being used by a set of variational equations of order 2. Order 2 equations will no longer be valid.");
+                }
+            }
+        } else {
+            if (!(r->messages_var_rescale_warning & 2)) {
+                r->messages_var_rescale_warning |= 2;
+                reb_simulation_warning(r, "Variational particles which are part of a second order variational equation have now large coordinates which might exceed range of floating point number range. REBOUND cannot rescale a second order variational equation as it is non-linear.");
+            }
+            return;
+        }
+        if (!r->is_synchronized) {
+            if (!(r->messages_var_rescale_warning & 1)) {
+                r->messages_var_rescale_warning |= 1;
+                reb_simulation_warning(r, "Variational particles have large coordinates which might exceed range of floating point numbers. Rescaling failed because integrator was not synchronized. Turn on safe_mode or manually synchronize. Then rescale.");
+            }
+            return;
+        }
+        vc->lrescale += log(scale);
+        for (size_t i = 0; i < N; i++) {
+            q[i].x /= scale;
+            q[i].y /= scale;
+            q[i].z /= scale;
+            q[i].vx /= scale;
+            q[i].vy /= scale;
+            q[i].vz /= scale;
+        }
+        r->did_modify_particles = 1;
+    }
+}
+
+// tools.c:1431-1470
+REB_API void reb_simulation_init_megno(struct reb_simulation* const r) {
+    r->megno_Ys = 0.;
+    r->megno_Yss = 0.;
+    r->megno_cov_Yt = 0.;
+    r->megno_var_t = 0.;
+    r->megno_n = 0;
+    r->megno_mean_Y = 0;
+    r->megno_initial_t = r->t;
+    r->megno_mean_t = 0;
+    reb_simulation_add_variation_1st_order(r, -1);
+    r->calculate_megno = 1;
+    struct reb_particle* const q = r->particles_var;
+    for (size_t i = 0; i < r->N; i++) {
+        q[i].m = 0.;
+        q[i].x = reb_random_normal(r, 1.);
+        q[i].y = reb_random_normal(r, 1.);
+        q[i].z = reb_random_normal(r, 1.);
+        q[i].vx = reb_random_normal(r, 1.);
+        q[i].vy = reb_random_normal(r, 1.);
+        q[i].vz = reb_random_normal(r, 1.);
+        const double deltad = 1. / sqrt(q[i].x * q[i].x + q[i].y * q[i].y + q[i].z * q[i].z + q[i].vx * q[i].vx +
+                                        q[i].vy * q[i].vy + q[i].vz * q[i].vz);
+        q[i].x *= deltad;
+        q[i].y *= deltad;
+        q[i].z *= deltad;
+        q[i].vx *= deltad;
+        q[i].vy *= deltad;
+        q[i].vz *= deltad;
+    }
+}
+
+REB_API void reb_simulation_init_megno_seed(struct reb_simulation* const r, unsigned int seed) {
+    r->rand_seed = seed;
+    reb_simulation_init_megno(r);
+}
+
+REB_API double reb_simulation_megno(struct reb_simulation* r) {
+    if (r->t == r->megno_initial_t) return 0.;
+    return r->megno_Yss / (r->t - r->megno_initial_t);
+}
+
+REB_API double reb_simulation_lyapunov(struct reb_simulation* r) {
+    if (r->megno_var_t == 0.0) return 0.;
+    return r->megno_cov_Yt / r->megno_var_t;
+}
+
 
 REB_API void reb_simulation_move_to_hel(struct reb_simulation* const r) {
     ensure_host(r);
@@ -675,6 +889,25 @@ REB_API struct reb_vec6d reb_boundary_get_ghostbox(struct reb_simulation* const 
 }
 
 }  // extern "C"
+
+namespace rbamd {
+// tools.c:1504-1522: the running MEGNO sums after a step (dY from the device)
+void megno_update(struct reb_simulation* r, double dY, double dt_done) {
+    r->megno_Ys += dY;
+    const double Y = r->megno_Ys / (r->t - r->megno_initial_t);
+    r->megno_Yss += Y * dt_done;
+    r->megno_n++;
+    const double _d_t = r->t - r->megno_initial_t - r->megno_mean_t;
+    r->megno_mean_t += _d_t / (double)r->megno_n;
+    const double _d_Y = reb_simulation_megno(r) - r->megno_mean_Y;
+    r->megno_mean_Y += _d_Y / (double)r->megno_n;
+    r->megno_cov_Yt += ((double)r->megno_n - 1.) / (double)r->megno_n * (r->t - r->megno_initial_t - r->megno_mean_t) *
+                       (reb_simulation_megno(r) - r->megno_mean_Y);
+    r->megno_var_t += ((double)r->megno_n - 1.) / (double)r->megno_n * (r->t - r->megno_initial_t - r->megno_mean_t) *
+                      (r->t - r->megno_initial_t - r->megno_mean_t);
+}
+}  // namespace rbamd
+
 
 namespace rbamd {
 std::vector<struct reb_vec6d> ghostboxes(struct reb_simulation* r, int nx, int ny, int nz) {

@@ -227,6 +227,13 @@ REB_API int reb_simulation_particle_index(struct reb_particle* p) {
     return i < p->sim->N ? (int)i : -1;
 }
 
+// particle.c:137-145: the same for a variational particle
+REB_API int reb_simulation_particle_var_index(struct reb_particle* p) {
+    if (!p || !p->sim || p < p->sim->particles_var) return -1;
+    const size_t i = (size_t)(p - p->sim->particles_var);
+    return i < p->sim->N_var ? (int)i : -1;
+}
+
 // tools.c:454-461: centre of mass of the particles before p
 REB_API struct reb_particle reb_simulation_jacobi_com(struct reb_particle* p) {
     const int i = reb_simulation_particle_index(p);

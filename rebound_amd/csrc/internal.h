@@ -211,6 +211,7 @@ bool check_device_flags(struct reb_simulation* r);           // error words set 
 void whfast_step_device(struct reb_simulation* r, struct reb_integrator_whfast_state* s);
 void whfast_synchronize_device(struct reb_simulation* r, struct reb_integrator_whfast_state* s);
 void whfast_release(struct reb_simulation* r);
+void megno_update(struct reb_simulation* r, double dY, double dt_done);   // host_api.cpp
 void tree_release(struct reb_simulation* r);
 void resolve_release(struct reb_simulation* r);
 // The built-in hard-sphere resolve with a constant restitution runs on compact

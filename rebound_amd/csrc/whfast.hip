@@ -27,8 +27,9 @@
 
 namespace rbamd {
 
-// 1/k! as the reference spells them (integrator_whfast.c:158); only 0..13 are used.
-__constant__ double kInvFact[14] = {1.,
+// 1/k! as the reference spells them (integrator_whfast.c:158); 0..13 are used
+// by the solver, 0..15 by the variational drift.
+__constant__ double kInvFact[16] = {1.,
                                     1.,
                                     1. / 2.,
                                     1. / 6.,
@@ -41,7 +42,9 @@ __constant__ double kInvFact[14] = {1.,
                                     1. / 3628800.,
                                     1. / 39916800.,
                                     1. / 479001600.,
-                                    1. / 6227020800.};
+                                    1. / 6227020800.,
+                                    1. / 87178291200.,
+                                    1. / 1307674368000.};
 
 __device__ __forceinline__ double fastabs(double x) { return (x > 0.) ? x : -x; }
 
@@ -86,6 +89,61 @@ __device__ __forceinline__ void stiefel_Gs3(double* Gs, double beta, double X) {
     Gs[3] *= X2 * X;
 }
 
+// c_0..c_5 (integrator_whfast.c:161-187), for the variational drift.
+__device__ __forceinline__ void stumpff_cs(double* cs, double z) {
+    if (isinf(z)) {
+        cs[0] = cs[1] = cs[2] = cs[3] = cs[4] = cs[5] = __builtin_nan("");
+        return;
+    }
+    unsigned int n = 0;
+    while (fastabs(z) > 0.1) {
+        z = z / 4.;
+        n++;
+    }
+    double c_odd = kInvFact[15];
+    double c_even = kInvFact[14];
+#pragma unroll
+    for (int np = 13; np >= 5; np -= 2) {
+        c_odd = kInvFact[np] - z * c_odd;
+        c_even = kInvFact[np - 1] - z * c_even;
+    }
+    cs[5] = c_odd;
+    cs[4] = c_even;
+    cs[3] = kInvFact[3] - z * cs[5];
+    cs[2] = kInvFact[2] - z * cs[4];
+    cs[1] = kInvFact[1] - z * cs[3];
+    for (; n > 0; n--) {
+        z = z * 4.;
+        cs[5] = (cs[5] + cs[4] + cs[3] * cs[2]) * 0.0625;
+        cs[4] = (1. + cs[1]) * cs[3] * 0.125;
+        cs[3] = 1. / 6. - z * cs[5];
+        cs[2] = 0.5 - z * cs[4];
+        cs[1] = 1. - z * cs[3];
+    }
+    cs[0] = kInvFact[0] - z * cs[2];
+}
+
+__device__ __forceinline__ void stiefel_Gs(double* Gs, double beta, double X) {
+    const double X2 = X * X;
+    stumpff_cs(Gs, beta * X2);
+    Gs[1] *= X;
+    Gs[2] *= X2;
+    double pw = X2 * X;
+    Gs[3] *= pw;
+    pw *= X;
+    Gs[4] *= pw;
+    pw *= X;
+    Gs[5] *= pw;
+}
+
+// What the variational drift of a body needs from its Kepler solve: the
+// state before the drift, the solver's scalars and the drift coefficients.
+struct KepOut {
+    double x0, y0, z0, vx0, vy0, vz0;
+    double r0, r0i, ri, beta, eta0, zeta0, X, mu;
+    double f, g, fd, gd;
+};
+
 // One quartic (Laguerre-Conway) update of X (integrator_whfast.c:292-300).
 // Gs receives the Stiefel functions of the X the update started from.
 __device__ __forceinline__ double quartic_next(double X, double beta, double r0, double eta0, double zeta0,
@@ -103,7 +161,7 @@ __device__ __forceinline__ double quartic_next(double X, double beta, double r0,
 // (the reference warns once), 4 = the quartic iteration ran, 8 = the
 // bisection fallback ran.
 __device__ __forceinline__ int kepler_solver(double& px, double& py, double& pz, double& pvx, double& pvy, double& pvz, double mu,
-                             double dt) {
+                             double dt, KepOut* out = nullptr) {
     const double x0 = px, y0 = py, z0 = pz, vx0 = pvx, vy0 = pvy, vz0 = pvz;
     int flags = 0;
     const double r0 = sqrt(x0 * x0 + y0 * y0 + z0 * z0);
@@ -230,7 +288,44 @@ __device__ __forceinline__ int kepler_solver(double& px, double& py, double& pz,
     pvx += fd * x0 + gd * vx0;
     pvy += fd * y0 + gd * vy0;
     pvz += fd * z0 + gd * vz0;
+    if (out) {
+        *out = KepOut{x0, y0, z0, vx0, vy0, vz0, r0, r0i, ri, beta, eta0, zeta0, X, mu, f, g, fd, gd};
+    }
     return flags;
+}
+
+// The drift of one body's variational particle d along its Kepler solve k
+// (integrator_whfast.c:392-426): the Stiefel functions at the final X, then
+// the derivatives of X, r and the drift coefficients.
+__device__ __forceinline__ void kepler_var(const KepOut& k, double& dx, double& dy, double& dz, double& dvx,
+                                           double& dvy, double& dvz) {
+    const double d_x = dx, d_y = dy, d_z = dz, d_vx = dvx, d_vy = dvy, d_vz = dvz;
+    double Gs[6];
+    stiefel_Gs(Gs, k.beta, k.X);
+    const double mu = k.mu, r0 = k.r0, r0i = k.r0i, ri = k.ri, X = k.X, beta = k.beta;
+    const double dr0 = (d_x * k.x0 + d_y * k.y0 + d_z * k.z0) * r0i;
+    const double dbeta = -2. * mu * dr0 * r0i * r0i - 2. * (d_vx * k.vx0 + d_vy * k.vy0 + d_vz * k.vz0);
+    const double deta0 = d_x * k.vx0 + d_y * k.vy0 + d_z * k.vz0 + k.x0 * d_vx + k.y0 * d_vy + k.z0 * d_vz;
+    const double dzeta0 = -beta * dr0 - r0 * dbeta;
+    const double G3beta = 0.5 * (3. * Gs[5] - X * Gs[4]);
+    const double G2beta = 0.5 * (2. * Gs[4] - X * Gs[3]);
+    const double G1beta = 0.5 * (Gs[3] - X * Gs[2]);
+    const double tbeta = k.eta0 * G2beta + k.zeta0 * G3beta;
+    const double dX = -1. * ri * (X * dr0 + Gs[2] * deta0 + Gs[3] * dzeta0 + tbeta * dbeta);
+    const double dG1 = Gs[0] * dX + G1beta * dbeta;
+    const double dG2 = Gs[1] * dX + G2beta * dbeta;
+    const double dG3 = Gs[2] * dX + G3beta * dbeta;
+    const double dr = dr0 + Gs[1] * deta0 + Gs[2] * dzeta0 + k.eta0 * dG1 + k.zeta0 * dG2;
+    const double df = mu * Gs[2] * dr0 * r0i * r0i - mu * dG2 * r0i;
+    const double dg = -mu * dG3;
+    const double dfd = -mu * dG1 * r0i * ri + mu * Gs[1] * (dr0 * r0i + dr * ri) * r0i * ri;
+    const double dgd = -mu * dG2 * ri + mu * Gs[2] * dr * ri * ri;
+    dx += k.f * d_x + k.g * d_vx + df * k.x0 + dg * k.vx0;
+    dy += k.f * d_y + k.g * d_vy + df * k.y0 + dg * k.vy0;
+    dz += k.f * d_z + k.g * d_vz + df * k.z0 + dg * k.vz0;
+    dvx += k.fd * d_x + k.gd * d_vx + dfd * k.x0 + dgd * k.vx0;
+    dvy += k.fd * d_y + k.gd * d_vy + dfd * k.y0 + dgd * k.vy0;
+    dvz += k.fd * d_z + k.gd * d_vz + dfd * k.z0 + dgd * k.vz0;
 }
 
 // ---------------------------------------------------------------- batch Kepler
@@ -342,9 +437,10 @@ __device__ __forceinline__ const CorrZ* corrector_chain(int order, int& n) {
 // the sequential scans of transformations.c:34-81,147-176,178-258, which every
 // lane of a group evaluates over the published bodies in the reference's order,
 // and the symplectic correctors run at the synchronisation points.
-template <int COORD>
+template <int COORD, bool VAR = false>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_whfast(WHParams P) {
     constexpr bool JAC = COORD == WH_JACOBI;
+    static_assert(!VAR || JAC, "variational equations run in Jacobi coordinates");
     __shared__ Slot slot[64];
     __shared__ double aux[64][3];    // per-planet sums of the serial Jacobi gravity / jerk
     __shared__ double aux0[64][3];   // the star's, at the group's first lane
@@ -443,6 +539,233 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
         };
         if (JAC) interior_mass();
 
+        // ------------------------------------------------ variational particles
+        // First-order sets (VAR: Jacobi, default kernel, one system).  Their
+        // state stays in device memory (P.pv inertial, P.jv Jacobi); every
+        // lane handles its planet's entries and lane 0 the star's and the
+        // barycentre's.  Each operation starts with a barrier, so that what
+        // another lane wrote before is visible.  The transforms take the real
+        // masses, as the reference passes `particles` as p_mass.
+        const size_t vs = P.vstride;
+        auto vget = [&](const double* a, int f, size_t i) { return a[(size_t)f * vs + i]; };
+        auto vput = [&](double* a, int f, size_t i, double x) { a[(size_t)f * vs + i] = x; };
+        // transformations.c:34-81 on particles_var (integrator_whfast.c:958-962)
+        auto var_from_inertial = [&](bool en) {
+            for (int v = 0; v < P.nvar; v++) {
+                const size_t b = (size_t)P.var_index[v];
+                __syncthreads();
+                double q[6] = {0., 0., 0., 0., 0., 0.}, d0[6] = {0., 0., 0., 0., 0., 0.}, dm = 0.;
+                if (live) {
+                    for (int f = 0; f < 6; f++) q[f] = vget(P.pv, f, b + me);
+                    dm = vget(P.pv, 9, b + me);
+                }
+                if (sys_ok)
+                    for (int f = 0; f < 6; f++) d0[f] = vget(P.pv, f, b);
+                publish(Slot{q[0], q[1], q[2], q[3], q[4], q[5], im});
+                double eta = sm;
+                double s_x = eta * d0[0], s_y = eta * d0[1], s_z = eta * d0[2];
+                double s_vx = eta * d0[3], s_vy = eta * d0[4], s_vz = eta * d0[5];
+                double j[6] = {0., 0., 0., 0., 0., 0.};
+                GROUP_LOOP(k, 1, NAw) {
+                    const Slot o = slot[gbase + k - 1];
+                    const double ei = 1. / eta;
+                    eta += o.m;
+                    const double pme = eta * ei;
+                    const double jx = o.x - s_x * ei, jy = o.y - s_y * ei, jz = o.z - s_z * ei;
+                    const double jvx = o.vx - s_vx * ei, jvy = o.vy - s_vy * ei, jvz = o.vz - s_vz * ei;
+                    if (k == me) {
+                        j[0] = jx; j[1] = jy; j[2] = jz;
+                        j[3] = jvx; j[4] = jvy; j[5] = jvz;
+                    }
+                    s_x = s_x * pme + o.m * jx;
+                    s_y = s_y * pme + o.m * jy;
+                    s_z = s_z * pme + o.m * jz;
+                    s_vx = s_vx * pme + o.m * jvx;
+                    s_vy = s_vy * pme + o.m * jvy;
+                    s_vz = s_vz * pme + o.m * jvz;
+                }
+                const double ei = 1. / eta;
+                if (me >= NA) {
+                    j[0] = q[0] - s_x * ei; j[1] = q[1] - s_y * ei; j[2] = q[2] - s_z * ei;
+                    j[3] = q[3] - s_vx * ei; j[4] = q[4] - s_vy * ei; j[5] = q[5] - s_vz * ei;
+                }
+                if (!en) continue;
+                if (live) {
+                    for (int f = 0; f < 6; f++) vput(P.jv, f, b + me, j[f]);
+                    vput(P.jv, 9, b + me, dm);
+                }
+                if (gl == 0 && sys_ok) {
+                    const double Mtotali = 1. / eta;
+                    vput(P.jv, 0, b, s_x * Mtotali); vput(P.jv, 1, b, s_y * Mtotali); vput(P.jv, 2, b, s_z * Mtotali);
+                    vput(P.jv, 3, b, s_vx * Mtotali); vput(P.jv, 4, b, s_vy * Mtotali); vput(P.jv, 5, b, s_vz * Mtotali);
+                    vput(P.jv, 9, b, eta);
+                }
+            }
+        };
+        // transformations.c:178-258 on particles_var (integrator_whfast.c:662-672, 1089-1092)
+        auto var_to_inertial = [&](bool en, bool with_vel) {
+            for (int v = 0; v < P.nvar; v++) {
+                const size_t b = (size_t)P.var_index[v];
+                __syncthreads();
+                double h[6] = {0., 0., 0., 0., 0., 0.}, c[6] = {0., 0., 0., 0., 0., 0.}, ceta = 0.;
+                if (live)
+                    for (int f = 0; f < 6; f++) h[f] = vget(P.jv, f, b + me);
+                if (sys_ok) {
+                    for (int f = 0; f < 6; f++) c[f] = vget(P.jv, f, b);
+                    ceta = vget(P.jv, 9, b);
+                }
+                publish(Slot{h[0], h[1], h[2], h[3], h[4], h[5], im});
+                double eta = ceta;
+                double s_x = c[0] * eta, s_y = c[1] * eta, s_z = c[2] * eta;
+                double s_vx = 0., s_vy = 0., s_vz = 0.;
+                if (with_vel) {
+                    s_vx = c[3] * eta; s_vy = c[4] * eta; s_vz = c[5] * eta;
+                }
+                double n[6] = {0., 0., 0., 0., 0., 0.};
+                if (me >= NA) {
+                    const double ei = 1. / eta;
+                    n[0] = h[0] + s_x * ei; n[1] = h[1] + s_y * ei; n[2] = h[2] + s_z * ei;
+                    if (with_vel) {
+                        n[3] = h[3] + s_vx * ei; n[4] = h[4] + s_vy * ei; n[5] = h[5] + s_vz * ei;
+                    }
+                }
+                for (int k = NAw - 1; k > 0; k--) {
+                    const Slot o = slot[gbase + k - 1];
+                    const double ei = 1. / eta;
+                    s_x = (s_x - o.m * o.x) * ei;
+                    s_y = (s_y - o.m * o.y) * ei;
+                    s_z = (s_z - o.m * o.z) * ei;
+                    if (with_vel) {
+                        s_vx = (s_vx - o.m * o.vx) * ei;
+                        s_vy = (s_vy - o.m * o.vy) * ei;
+                        s_vz = (s_vz - o.m * o.vz) * ei;
+                    }
+                    if (k == me) {
+                        n[0] = o.x + s_x; n[1] = o.y + s_y; n[2] = o.z + s_z;
+                        if (with_vel) {
+                            n[3] = o.vx + s_vx; n[4] = o.vy + s_vy; n[5] = o.vz + s_vz;
+                        }
+                    }
+                    eta -= o.m;
+                    s_x *= eta; s_y *= eta; s_z *= eta;
+                    if (with_vel) {
+                        s_vx *= eta; s_vy *= eta; s_vz *= eta;
+                    }
+                }
+                if (!en) continue;
+                const int nf = with_vel ? 6 : 3;
+                if (live)
+                    for (int f = 0; f < nf; f++) vput(P.pv, f, b + me, n[f]);
+                if (gl == 0 && sys_ok) {
+                    const double mi = 1. / eta;
+                    vput(P.pv, 0, b, s_x * mi); vput(P.pv, 1, b, s_y * mi); vput(P.pv, 2, b, s_z * mi);
+                    if (with_vel) {
+                        vput(P.pv, 3, b, s_vx * mi); vput(P.pv, 4, b, s_vy * mi); vput(P.pv, 5, b, s_vz * mi);
+                    }
+                }
+            }
+        };
+        // gravity.c:533-640, first order, all bodies varied: the pairs j < i
+        // from (starti, startj) as gravity_ignore_terms selects, i-major, each
+        // term added to i and subtracted from j.  Body k's sum therefore holds
+        // its terms as i (ascending j), then as j (ascending i), then those of
+        // test particles i >= N_active (type 1).  The star's is formed by lane 0.
+        auto var_gravity = [&](bool en, int starti, int startj) {
+            for (int v = 0; v < P.nvar; v++) {
+                const size_t b = (size_t)P.var_index[v];
+                __syncthreads();
+                double d[3] = {0., 0., 0.}, dm = 0., ds[3] = {0., 0., 0.}, dsm = 0.;
+                if (live) {
+                    d[0] = vget(P.pv, 0, b + me); d[1] = vget(P.pv, 1, b + me); d[2] = vget(P.pv, 2, b + me);
+                    dm = vget(P.pv, 9, b + me);
+                }
+                if (sys_ok) {
+                    ds[0] = vget(P.pv, 0, b); ds[1] = vget(P.pv, 1, b); ds[2] = vget(P.pv, 2, b);
+                    dsm = vget(P.pv, 9, b);
+                }
+                aux[lane][0] = dm;
+                publish(Slot{ix, iy, iz, d[0], d[1], d[2], im});
+                // body k: 0 the star, k >= 1 the slot of planet k
+                auto term = [&](int I, int J, bool to_i, double& tx, double& ty, double& tz) {
+                    const Slot oi = I == 0 ? Slot{sx, sy, sz, ds[0], ds[1], ds[2], sm} : slot[gbase + I - 1];
+                    const Slot oj = J == 0 ? Slot{sx, sy, sz, ds[0], ds[1], ds[2], sm} : slot[gbase + J - 1];
+                    const double dmi = I == 0 ? dsm : aux[gbase + I - 1][0];
+                    const double dmj = J == 0 ? dsm : aux[gbase + J - 1][0];
+                    const double dx = oi.x - oj.x;
+                    const double dy = oi.y - oj.y;
+                    const double dz = oi.z - oj.z;
+                    const double r2 = dx * dx + dy * dy + dz * dz;
+                    const double _r = sqrt(r2);
+                    const double r3inv = 1. / (r2 * _r);
+                    const double r5inv = 3. * r3inv / r2;
+                    const double ddx = oi.vx - oj.vx;
+                    const double ddy = oi.vy - oj.vy;
+                    const double ddz = oi.vz - oj.vz;
+                    const double dxdx = dx * dx * r5inv - r3inv;
+                    const double dydy = dy * dy * r5inv - r3inv;
+                    const double dzdz = dz * dz * r5inv - r3inv;
+                    const double dxdy = dx * dy * r5inv;
+                    const double dxdz = dx * dz * r5inv;
+                    const double dydz = dy * dz * r5inv;
+                    const double dax = ddx * dxdx + ddy * dxdy + ddz * dxdz;
+                    const double day = ddx * dxdy + ddy * dydy + ddz * dydz;
+                    const double daz = ddx * dxdz + ddy * dydz + ddz * dzdz;
+                    if (to_i) {
+                        const double Gmj = Gc * oj.m;
+                        const double dGmj = Gc * dmj;
+                        tx = Gmj * dax - dGmj * r3inv * dx;
+                        ty = Gmj * day - dGmj * r3inv * dy;
+                        tz = Gmj * daz - dGmj * r3inv * dz;
+                    } else {
+                        const double Gmi = Gc * oi.m;
+                        const double dGmi = Gc * dmi;
+                        tx = Gmi * dax - dGmi * r3inv * dx;
+                        ty = Gmi * day - dGmi * r3inv * dy;
+                        tz = Gmi * daz - dGmi * r3inv * dz;
+                    }
+                };
+                // sum of body k (k = 0: the star)
+                auto body_sum = [&](int k, double& a_x, double& a_y, double& a_z) {
+                    a_x = 0.; a_y = 0.; a_z = 0.;
+                    double tx, ty, tz;
+                    if (k < NAGw) {
+                        if (k >= starti)
+                            for (int j = startj; j < k; j++) {
+                                term(k, j, true, tx, ty, tz);
+                                a_x += tx; a_y += ty; a_z += tz;
+                            }
+                        if (k >= startj) {
+                            for (int i = (k + 1 > starti ? k + 1 : starti); i < NAGw; i++) {
+                                term(i, k, false, tx, ty, tz);
+                                a_x -= tx; a_y -= ty; a_z -= tz;
+                            }
+                            if (TPw)
+                                for (int i = NAGw; i < Nw; i++) {
+                                    term(i, k, false, tx, ty, tz);
+                                    a_x -= tx; a_y -= ty; a_z -= tz;
+                                }
+                        }
+                    } else {
+                        for (int j = startj; j < NAGw; j++) {
+                            term(k, j, true, tx, ty, tz);
+                            a_x += tx; a_y += ty; a_z += tz;
+                        }
+                    }
+                };
+                if (!en) continue;
+                if (live) {
+                    double a_x, a_y, a_z;
+                    body_sum(me, a_x, a_y, a_z);
+                    vput(P.pv, 6, b + me, a_x); vput(P.pv, 7, b + me, a_y); vput(P.pv, 8, b + me, a_z);
+                }
+                if (gl == 0 && sys_ok) {
+                    double a_x, a_y, a_z;
+                    body_sum(0, a_x, a_y, a_z);
+                    vput(P.pv, 6, b, a_x); vput(P.pv, 7, b, a_y); vput(P.pv, 8, b, a_z);
+                }
+            }
+        };
+
         // ------------------------------------------------ coordinate transforms
         // DH: transformations.c:388-424.  Jacobi: transformations.c:34-81.
         auto from_inertial = [&]() {
@@ -511,6 +834,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
         // skipped (with_vel == false) where nothing reads the inertial
         // velocities before they are recomputed.
         auto to_inertial = [&](bool en, bool with_vel) {
+            if constexpr (VAR) var_to_inertial(en, with_vel);
             if constexpr (JAC) {
                 Slot t{hx, hy, hz, 0., 0., 0., im};
                 if (with_vel) {
@@ -603,7 +927,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
         auto kepler = [&](double h, bool en) {
             if (!en || !live) return;
             const double mu = JAC ? eta_me * Gc : sm * Gc;
-            warn |= kepler_solver(hx, hy, hz, hvx, hvy, hvz, mu, h);
+            if constexpr (VAR) {
+                KepOut ko;
+                warn |= kepler_solver(hx, hy, hz, hvx, hvy, hvz, mu, h, &ko);
+                for (int v = 0; v < P.nvar; v++) {
+                    const size_t b = (size_t)P.var_index[v] + me;
+                    double d[6];
+                    for (int f = 0; f < 6; f++) d[f] = vget(P.jv, f, b);
+                    kepler_var(ko, d[0], d[1], d[2], d[3], d[4], d[5]);
+                    for (int f = 0; f < 6; f++) vput(P.jv, f, b, d[f]);
+                }
+            } else {
+                warn |= kepler_solver(hx, hy, hz, hvx, hvy, hvz, mu, h);
+            }
         };
         // integrator_whfast.c:638-651
         auto com = [&](double h, bool en) {
@@ -611,6 +947,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
             cx += h * cvx;
             cy += h * cvy;
             cz += h * cvz;
+            if constexpr (VAR) {
+                if (gl == 0)
+                    for (int v = 0; v < P.nvar; v++) {
+                        const size_t b = (size_t)P.var_index[v];
+                        vput(P.jv, 0, b, vget(P.jv, 0, b) + h * vget(P.jv, 3, b));
+                        vput(P.jv, 1, b, vget(P.jv, 1, b) + h * vget(P.jv, 4, b));
+                        vput(P.jv, 2, b, vget(P.jv, 2, b) + h * vget(P.jv, 5, b));
+                    }
+            }
         };
         // integrator_whfast.c:537-554, DH only (masses are the inertial ones,
         // equal to p_h[i].m here: the engine changes no masses)
@@ -867,6 +1212,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
         };
         // reb_simulation_update_acceleration for the kernel in use
         auto grav = [&](bool en) {
+            // gravity_ignore_terms BETWEEN_0_AND_1 (integrator_whfast.c:933)
+            if constexpr (VAR) var_gravity(en, 2, 0);
             if (JAC && P.jacgrav)
                 gravity_jacobi(en);
             else
@@ -875,7 +1222,73 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
         // DH: integrator_whfast.c:494-501.  Jacobi: :439-483 (accelerations to
         // Jacobi form, transformations.c:147-176, then Eq. 132 of the WHFast
         // paper with the Kepler terms of the interior bodies).
+        // integrator_whfast.c:441-447 and :467-490 for the variational sets:
+        // accelerations to Jacobi form, the linearised Kepler term of the
+        // interior mass, then the kick.  The real body's factors are those of
+        // the real kick below (its positions do not change in a kick).
+        auto var_interaction = [&](double h, bool en) {
+            for (int v = 0; v < P.nvar; v++) {
+                const size_t b = (size_t)P.var_index[v];
+                __syncthreads();
+                double a[3] = {0., 0., 0.}, sa[3] = {0., 0., 0.};
+                if (live) {
+                    a[0] = vget(P.pv, 6, b + me); a[1] = vget(P.pv, 7, b + me); a[2] = vget(P.pv, 8, b + me);
+                }
+                if (sys_ok) {
+                    sa[0] = vget(P.pv, 6, b); sa[1] = vget(P.pv, 7, b); sa[2] = vget(P.pv, 8, b);
+                }
+                publish(Slot{a[0], a[1], a[2], 0., 0., 0., im});
+                double eta = sm;
+                double s_x = eta * sa[0], s_y = eta * sa[1], s_z = eta * sa[2];
+                double jax = 0., jay = 0., jaz = 0.;
+                GROUP_LOOP(k, 1, NAw) {
+                    const Slot o = slot[gbase + k - 1];
+                    const double ei = 1. / eta;
+                    eta += o.m;
+                    const double pme = eta * ei;
+                    const double jx = o.x - s_x * ei, jy = o.y - s_y * ei, jz = o.z - s_z * ei;
+                    if (k == me) {
+                        jax = jx; jay = jy; jaz = jz;
+                    }
+                    s_x = s_x * pme + o.m * jx;
+                    s_y = s_y * pme + o.m * jy;
+                    s_z = s_z * pme + o.m * jz;
+                }
+                if (me >= NA) {
+                    const double ei = 1. / eta;
+                    jax = a[0] - s_x * ei; jay = a[1] - s_y * ei; jaz = a[2] - s_z * ei;
+                }
+                if (!en) continue;
+                if (gl == 0 && sys_ok) {
+                    const double Mtotali = 1. / eta;
+                    vput(P.jv, 6, b, s_x * Mtotali); vput(P.jv, 7, b, s_y * Mtotali); vput(P.jv, 8, b, s_z * Mtotali);
+                }
+                if (!live) continue;
+                const size_t bi = b + me;
+                vput(P.jv, 6, bi, jax); vput(P.jv, 7, bi, jay); vput(P.jv, 8, bi, jaz);
+                const double dx = vget(P.jv, 0, bi), dy = vget(P.jv, 1, bi), dz = vget(P.jv, 2, bi);
+                double dvx = vget(P.jv, 3, bi), dvy = vget(P.jv, 4, bi), dvz = vget(P.jv, 5, bi);
+                if (me > 1) {
+                    const double px = hx, py = hy, pz = hz;
+                    const double rj2i = 1. / (px * px + py * py + pz * pz);
+                    const double rji = sqrt(rj2i);
+                    const double rj3iM = rji * rj2i * Gc * eta_me;
+                    const double prefac1 = h * rj3iM;
+                    const double rj5M = rj3iM * rj2i;
+                    const double rdr = dx * px + dy * py + dz * pz;
+                    const double prefac2 = -h * 3. * rdr * rj5M;
+                    dvx += prefac1 * dx + prefac2 * px;
+                    dvy += prefac1 * dy + prefac2 * py;
+                    dvz += prefac1 * dz + prefac2 * pz;
+                }
+                dvx += h * jax;
+                dvy += h * jay;
+                dvz += h * jaz;
+                vput(P.jv, 3, bi, dvx); vput(P.jv, 4, bi, dvy); vput(P.jv, 5, bi, dvz);
+            }
+        };
         auto interaction = [&](double h, bool en) {
+            if constexpr (VAR) var_interaction(h, en);
             if constexpr (JAC) {
                 publish(Slot{ax, ay, az, 0., 0., 0., im});
                 double eta = sm;
@@ -987,6 +1400,36 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                 hx = hx0; hy = hy0; hz = hz0; hvx = hvx0; hvy = hvy0; hvz = hvz0; hm = hm0;
                 eta_me = eta0;
             }
+            if constexpr (VAR) var_from_inertial(en);
+        };
+        // integrator_whfast.c:1250-1258: the variational accelerations with
+        // every term (gravity_ignore_terms NONE), then tools.c:1483-1502 over
+        // particles_var[0 .. N-1] by lane 0; the host adds it to the MEGNO sums.
+        auto megno_step = [&](bool en) {
+            if constexpr (VAR) {
+                var_gravity(en, 1, 0);
+                __syncthreads();
+                if (en && gl == 0 && sys_ok) {
+                    double deltad = 0., delta2 = 0.;
+                    for (int i = 0; i < Nw; i++) {
+                        double q[9];
+                        for (int f = 0; f < 9; f++) q[f] = vget(P.pv, f, (size_t)i);
+                        deltad += q[3] * q[0];
+                        deltad += q[4] * q[1];
+                        deltad += q[5] * q[2];
+                        deltad += q[6] * q[3];
+                        deltad += q[7] * q[4];
+                        deltad += q[8] * q[5];
+                        delta2 += q[0] * q[0];
+                        delta2 += q[1] * q[1];
+                        delta2 += q[2] * q[2];
+                        delta2 += q[3] * q[3];
+                        delta2 += q[4] * q[4];
+                        delta2 += q[5] * q[5];
+                    }
+                    *P.megno_out = deltad / delta2;
+                }
+            }
         };
 
         // integrator_whfast.c:1113-1259
@@ -1052,6 +1495,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
             }
             if (act) sync = 0;
             if (__any(act && safe)) synchronize(act && safe);
+            if (VAR && P.megno) megno_step(act);
             if (act) modified = false;
         }
         synchronize(sys_ok && (P.sync_at_end || (S.req & 1)));
@@ -1116,7 +1560,9 @@ __global__ void k_reset_counter(unsigned int* c) { *c = 0u; }
 
 int whfast_launch(const WHParams& P, hipStream_t st) {
     hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, st, P.counter);
-    if (P.coords == WH_JACOBI)
+    if (P.coords == WH_JACOBI && P.nvar > 0)
+        hipLaunchKernelGGL((k_whfast<WH_JACOBI, true>), dim3(P.grid), dim3(64), 0, st, P);
+    else if (P.coords == WH_JACOBI)
         hipLaunchKernelGGL(k_whfast<WH_JACOBI>, dim3(P.grid), dim3(64), 0, st, P);
     else
         hipLaunchKernelGGL(k_whfast<WH_DH>, dim3(P.grid), dim3(64), 0, st, P);
@@ -1146,6 +1592,7 @@ int whfast_max_resident(int device) {
 struct WHFastDevice {
     DevBuf ph;       // 7 * cap doubles (p_h SoA, same stride as the particle block)
     DevBuf meta;     // WHSys, WHTask, order, flags, counter
+    DevBuf var;      // variational sets: pv and jv (10 * N_var doubles each), set offsets, MEGNO value
     size_t cap = 0;
 };
 
@@ -1154,6 +1601,7 @@ void whfast_release(struct reb_simulation* r) {
     if (d->whfast) {
         d->whfast->ph.release();
         d->whfast->meta.release();
+        d->whfast->var.release();
         delete d->whfast;
         d->whfast = nullptr;
     }
@@ -1162,8 +1610,49 @@ void whfast_release(struct reb_simulation* r) {
 // integrator_whfast.c:876-947, restricted to what this engine implements.
 static bool whfast_check(struct reb_simulation* r, struct reb_integrator_whfast_state* s) {
     if (r->N_var) {
-        reb_simulation_error(r, "Variational particles are not supported by this engine.");
-        return false;
+        // integrator_whfast.c:877-910
+        if (s->coordinates != REB_INTEGRATOR_WHFAST_COORDINATES_JACOBI) {
+            reb_simulation_error(r, "Variational particles are only compatible with Jacobi coordinates.");
+            return false;
+        }
+        if (s->kernel != REB_INTEGRATOR_WHFAST_KERNEL_DEFAULT) {
+            reb_simulation_error(r, "Variational particles are only compatible with the standard kernel.");
+            return false;
+        }
+        if (s->corrector2) {
+            reb_simulation_error(r, "Variational particles not compatible with 2nd corrector.");
+            return false;
+        }
+        if (s->safe_mode == 0 && r->calculate_megno) {
+            reb_simulation_error(r, "MEGNO is not compatible with WHFast's safe_mode=0.");
+            return false;
+        }
+        if (s->N_allocated_var != r->N_var) {
+            s->p_jh_var = (struct reb_particle*)realloc(s->p_jh_var, sizeof(struct reb_particle) * r->N_var);
+            for (size_t i = s->N_allocated_var; i < r->N_var; i++) memset(&s->p_jh_var[i], 0, sizeof(struct reb_particle));
+            s->N_allocated_var = r->N_var;
+        }
+        for (size_t v = 0; v < r->N_var_config; v++) {
+            const struct reb_variational_configuration& vc = r->var_config[v];
+            if (vc.order != 1) {
+                reb_simulation_error(r, "WHFast only supports first order variational equations.");
+                return false;
+            }
+            if (vc.testparticle >= 0) {
+                reb_simulation_error(r, "Test particle variations not supported with WHFast. Use IAS15.");
+                return false;
+            }
+            // the kernel reads entries index .. index + N - 1 of the set
+            if (vc.index < 0 || (size_t)vc.index + r->N > r->N_var) {
+                reb_simulation_error(r, "A set of variational particles does not match the number of bodies.");
+                return false;
+            }
+        }
+        // simulation.c:673-682: variational accelerations of BASIC gravity only
+        if (r->gravity != REB_GRAVITY_BASIC) {
+            reb_simulation_error(r, "Variational gravity calculation not implemented in selected gravity module. Please use REB_GRAVITY_BASIC.");
+            return false;
+        }
     }
     if (s->coordinates != REB_INTEGRATOR_WHFAST_COORDINATES_DEMOCRATICHELIOCENTRIC &&
         s->coordinates != REB_INTEGRATOR_WHFAST_COORDINATES_JACOBI) {
@@ -1358,11 +1847,73 @@ void whfast_refresh_host(struct reb_simulation* r) {
     whfast_mirror_pjh(r, s);
 }
 
+// The variational sets go to the device before each launch and come back
+// after it (r->particles_var and the integrator's p_jh_var stay the host's:
+// the sets are small and the plugin steps once per launch).  Layout: pv then
+// jv, 10 fields of N_var doubles each (x y z vx vy vz ax ay az m), then the
+// sets' offsets and the MEGNO value.
+static bool var_upload(struct reb_simulation* r, struct reb_integrator_whfast_state* s, WHParams& P) {
+    DeviceState* d = dev_of(r);
+    WHFastDevice* w = d->whfast;
+    const size_t nv = r->N_var;
+    const size_t nd = 20 * nv + 1 + (r->N_var_config + 1) / 2;
+    if (!w->var.ensure(sizeof(double) * nd)) {
+        reb_simulation_error(r, "Out of device memory for the variational particles.");
+        return false;
+    }
+    std::vector<double> h(nd, 0.);
+    auto pack = [&](const struct reb_particle* q, double* o) {
+        for (size_t i = 0; i < nv; i++) {
+            const double f[10] = {q[i].x, q[i].y, q[i].z, q[i].vx, q[i].vy, q[i].vz, q[i].ax, q[i].ay, q[i].az, q[i].m};
+            for (int k = 0; k < 10; k++) o[(size_t)k * nv + i] = f[k];
+        }
+    };
+    pack(r->particles_var, h.data());
+    pack(s->p_jh_var, h.data() + 10 * nv);
+    int* vi = reinterpret_cast<int*>(h.data() + 20 * nv + 1);
+    for (size_t v = 0; v < r->N_var_config; v++) vi[v] = r->var_config[v].index;
+    if (!RB_HIP(r, hipMemcpyAsync(w->var.p, h.data(), sizeof(double) * nd, hipMemcpyHostToDevice, d->stream)))
+        return false;
+    double* base = w->var.as<double>();
+    P.nvar = (int)r->N_var_config;
+    P.var_index = reinterpret_cast<const int*>(base + 20 * nv + 1);
+    P.pv = base;
+    P.jv = base + 10 * nv;
+    P.vstride = nv;
+    P.megno = r->calculate_megno ? 1 : 0;
+    P.megno_out = base + 20 * nv;
+    return stream_sync(r);
+}
+
+static bool var_download(struct reb_simulation* r, struct reb_integrator_whfast_state* s, double* megno_dd) {
+    DeviceState* d = dev_of(r);
+    const size_t nv = r->N_var;
+    std::vector<double> h(20 * nv + 1);
+    if (!RB_HIP(r, hipMemcpyAsync(h.data(), d->whfast->var.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost,
+                                  d->stream)) ||
+        !stream_sync(r))
+        return false;
+    auto unpack = [&](const double* o, struct reb_particle* q) {
+        for (size_t i = 0; i < nv; i++) {
+            q[i].x = o[0 * nv + i]; q[i].y = o[1 * nv + i]; q[i].z = o[2 * nv + i];
+            q[i].vx = o[3 * nv + i]; q[i].vy = o[4 * nv + i]; q[i].vz = o[5 * nv + i];
+            q[i].ax = o[6 * nv + i]; q[i].ay = o[7 * nv + i]; q[i].az = o[8 * nv + i];
+            q[i].m = o[9 * nv + i];
+        }
+    };
+    unpack(h.data(), r->particles_var);
+    unpack(h.data() + 10 * nv, s->p_jh_var);
+    *megno_dd = h[20 * nv];
+    return true;
+}
+
 // n steps in one launch (n == 1 for the per-step plugin call).
-bool whfast_run(struct reb_simulation* r, struct reb_integrator_whfast_state* s, long nsteps, int sync_at_end) {
+bool whfast_run(struct reb_simulation* r, struct reb_integrator_whfast_state* s, long nsteps, int sync_at_end,
+                double* megno_dd = nullptr) {
     if (!whfast_check(r, s)) return false;
     WHParams P;
     if (!whfast_prepare(r, P)) return false;
+    if (r->N_var && !var_upload(r, s, P)) return false;
     P.nsteps = nsteps;
     P.safe_mode = s->safe_mode ? 1 : 0;
     P.did_modify = r->did_modify_particles ? 1 : 0;
@@ -1373,14 +1924,26 @@ bool whfast_run(struct reb_simulation* r, struct reb_integrator_whfast_state* s,
     P.kernel = (int)s->kernel;
     P.jacgrav = r->gravity == REB_GRAVITY_JACOBI ? 1 : 0;
     if (whfast_launch(P, dev_of(r)->stream)) return RB_HIP(r, hipGetLastError());
-    return whfast_finish(r, s, P);
+    if (!whfast_finish(r, s, P)) return false;
+    double dd = 0.;
+    if (r->N_var && !var_download(r, s, &dd)) return false;
+    if (megno_dd) *megno_dd = dd;
+    return true;
 }
 
 void whfast_step_device(struct reb_simulation* r, struct reb_integrator_whfast_state* s) {
-    if (!whfast_run(r, s, 1, 0)) return;
+    double dd = 0.;
+    if (!whfast_run(r, s, 1, 0, &dd)) return;
     r->t += r->dt / 2.;
     r->t += r->dt / 2.;
     r->dt_last_done = r->dt;
+    if (r->calculate_megno && r->N_var) {
+        // integrator_whfast.c:1250-1258 (the device formed deltad/delta2);
+        // the reference leaves gravity_ignore_terms at INVOLVING_0 afterwards
+        r->gravity_ignore_terms = REB_GRAVITY_IGNORE_TERMS_INVOLVING_0;
+        const double dY = r->dt * 2. * (r->t - r->megno_initial_t) * dd;
+        megno_update(r, dY, r->dt);
+    }
 }
 
 void whfast_synchronize_device(struct reb_simulation* r, struct reb_integrator_whfast_state* s) {

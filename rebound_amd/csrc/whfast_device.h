@@ -57,6 +57,18 @@ struct WHParams {
     int corrector2 = 0;       // second corrector (Jacobi): 0 or 1
     int kernel = 0;           // 0 default, 1 modified kick, 2 composition, 3 lazy (Jacobi)
     int jacgrav = 0;          // accelerations by the Jacobi gravity (kernels 1 and 3)
+    // First-order variational equations (Jacobi, one system per launch).
+    // Variational particle k of the simulation is entry k of `pv` (inertial,
+    // fields x y z vx vy vz ax ay az m, `vstride` doubles apart) and of `jv`
+    // (its Jacobi copy, the same fields); set v covers entries
+    // var_index[v] .. var_index[v] + N - 1.
+    int nvar = 0;
+    const int* var_index = nullptr;
+    double* pv = nullptr;
+    double* jv = nullptr;
+    size_t vstride = 0;
+    int megno = 0;                // MEGNO: <deltad, delta>/|delta|^2 of the step into megno_out
+    double* megno_out = nullptr;
 };
 
 int whfast_launch(const WHParams& P, hipStream_t st);

@@ -18,6 +18,7 @@
  *   collision search + hardsphere        collision.c:49-405, 573-665 (DIRECT, TREE, LINE, LINETREE)
  *   SEI + shear boundary                 integrator_sei.c:86-174, boundary.c:35-201
  *   binary files / simulation archives   binarydata.c:42-932, simulationarchive.c:45-666
+ *   variational equations + MEGNO        integrator_whfast.c:392-490, tools.c:1313-1522
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -1248,10 +1249,148 @@ static void gen_shear_wrap(void){
     reb_simulation_free(r);
 }
 
+/* Variational equations and MEGNO with WHFast in Jacobi coordinates
+ * (integrator_whfast.c:392-490, 1250-1258; tools.c:201-375, 1313-1522;
+ * gravity.c:533-640).  Per case: the bodies as added (<tag>_init), the
+ * variational particles before the first step (<tag>_var0), and at each
+ * checkpoint the bodies, particles_var and p_jh_var (x y z vx vy vz ax ay az m)
+ * and a meta row: t, megno, lyapunov, the MEGNO sums (Ys Yss cov_Yt var_t
+ * mean_t mean_Y initial_t n), gravity_ignore_terms, is_synchronized,
+ * rand_seed, then the sets' lrescale.  A negative checkpoint synchronizes. */
+#define VCOL 10
+static double* var_rows(const struct reb_particle* q, size_t n){
+    double* s = malloc(sizeof(double)*VCOL*(n ? n : 1));
+    for (size_t i=0; i<n; i++){
+        double row[VCOL] = {q[i].x,q[i].y,q[i].z,q[i].vx,q[i].vy,q[i].vz,q[i].ax,q[i].ay,q[i].az,q[i].m};
+        memcpy(s+VCOL*i, row, sizeof(row));
+    }
+    return s;
+}
+static void dump_var_case(const char* tag, int k, struct reb_simulation* r){
+    char name[160];
+    snprintf(name, sizeof(name), "%s_k%d_state.npy", tag, k); dump_state(name, r);
+    double* v = var_rows(r->particles_var, r->N_var);
+    snprintf(name, sizeof(name), "%s_k%d_var.npy", tag, k); wd(name, v, r->N_var, VCOL);
+    free(v);
+    struct reb_integrator_whfast_state* wh = r->integrator.state;
+    v = var_rows(wh->p_jh_var, wh->N_allocated_var);
+    snprintf(name, sizeof(name), "%s_k%d_pjhvar.npy", tag, k); wd(name, v, wh->N_allocated_var, VCOL);
+    free(v);
+    double meta[32] = {r->t, reb_simulation_megno(r), reb_simulation_lyapunov(r), r->megno_Ys, r->megno_Yss,
+                       r->megno_cov_Yt, r->megno_var_t, r->megno_mean_t, r->megno_mean_Y, r->megno_initial_t,
+                       (double)r->megno_n, (double)r->gravity_ignore_terms, (double)r->is_synchronized,
+                       (double)r->rand_seed};
+    size_t nm = 14;
+    for (size_t c=0; c<r->N_var_config; c++) meta[nm++] = r->var_config[c].lrescale;
+    snprintf(name, sizeof(name), "%s_k%d_meta.npy", tag, k); wd(name, meta, nm, 0);
+}
+static void run_var_case(const char* tag, struct reb_simulation* r, const int64_t* ck, int nck){
+    char name[160];
+    double* v = var_rows(r->particles_var, r->N_var);
+    snprintf(name, sizeof(name), "%s_var0.npy", tag); wd(name, v, r->N_var, VCOL);
+    free(v);
+    snprintf(name, sizeof(name), "%s_ck.npy", tag); wi(name, ck, (size_t)nck, 0);
+    for (int k=0; k<nck; k++){
+        if (ck[k] < 0) reb_simulation_synchronize(r);
+        else reb_simulation_steps(r, (size_t)ck[k]);
+        dump_var_case(tag, k, r);
+    }
+}
+static struct reb_simulation* megno_bodies(const char* tag, int chaotic, int four){
+    struct reb_simulation* r = reb_simulation_create();
+    reb_simulation_add_fmt(r, "m", 1.);
+    if (chaotic){
+        reb_simulation_add_fmt(r, "m P", 1e-4, 1.);
+        reb_simulation_add_fmt(r, "m P", 1e-4, 1.17);
+    }else if (four){
+        reb_simulation_add_fmt(r, "m a e inc", 1e-3, 1.0, 0.05, 0.02);
+        reb_simulation_add_fmt(r, "m a e inc Omega", 3e-4, 1.6, 0.1, 0.05, 1.0);
+        reb_simulation_add_fmt(r, "m a e inc f", 5e-5, 2.5, 0.02, 0.1, 2.0);
+    }else{
+        reb_simulation_add_fmt(r, "m a e inc", 1e-3, 1.5, 0.1, 0.1);
+        reb_simulation_add_fmt(r, "m a e inc", 1e-3, 15., 0.1, 0.1);
+    }
+    char name[160];
+    snprintf(name, sizeof(name), "%s_init.npy", tag); dump_state(name, r);
+    return r;
+}
+static void gen_megno(void){
+    /* A: rebound/tests/test_megno.py:33-42 (regular), dt = 0.07 P_1 */
+    {
+        struct reb_simulation* r = megno_bodies("megno_reg", 0, 0);
+        reb_simulation_set_integrator(r, "whfast");
+        reb_simulation_init_megno_seed(r, 0);
+        r->dt = reb_orbit_from_particle(r->G, r->particles[1], r->particles[0]).P*0.07;
+        const int64_t ck[5] = {1, 9, 90, 900, 2000};
+        run_var_case("megno_reg", r, ck, 5);
+        reb_simulation_free(r);
+    }
+    /* B: test_megno.py:58-80 (chaotic), centre-of-mass frame after init_megno */
+    {
+        struct reb_simulation* r = megno_bodies("megno_chaos", 1, 0);
+        reb_simulation_set_integrator(r, "whfast");
+        reb_simulation_init_megno_seed(r, 0);
+        reb_simulation_move_to_com(r);
+        r->dt = 0.05;
+        const int64_t ck[5] = {1, 9, 90, 900, 1000};
+        run_var_case("megno_chaos", r, ck, 5);
+        reb_simulation_free(r);
+    }
+    /* C: A with the 11th-order symplectic corrector */
+    {
+        struct reb_simulation* r = megno_bodies("megno_corr", 0, 0);
+        struct reb_integrator_whfast_state* wh = reb_simulation_set_integrator(r, "whfast");
+        wh->corrector = 11;
+        reb_simulation_init_megno_seed(r, 7);
+        r->dt = reb_orbit_from_particle(r->G, r->particles[1], r->particles[0]).P*0.07;
+        const int64_t ck[4] = {1, 9, 90, 400};
+        run_var_case("megno_corr", r, ck, 4);
+        reb_simulation_free(r);
+    }
+    /* D: four bodies, two sets (one with a mass variation), safe_mode 0:
+     * unsynchronized checkpoints, then explicit synchronizations */
+    {
+        struct reb_simulation* r = megno_bodies("var_4b", 0, 1);
+        struct reb_integrator_whfast_state* wh = reb_simulation_set_integrator(r, "whfast");
+        wh->safe_mode = 0;
+        const int a = reb_simulation_add_variation_1st_order(r, -1);
+        const int b = reb_simulation_add_variation_1st_order(r, -1);
+        r->particles_var[a+1].x = 1.;
+        r->particles_var[a+2].vy = 0.5;
+        r->particles_var[b+2].vy = 1.;
+        r->particles_var[b+3].m = 1e-5;
+        r->particles_var[b+0].x = -0.25;
+        r->dt = 0.03;
+        const int64_t ck[6] = {1, 49, -1, 50, 100, -1};
+        run_var_case("var_4b", r, ck, 6);
+        reb_simulation_free(r);
+    }
+    /* E: rescaling (tools.c:1313-1380): a set started near 1e100 */
+    {
+        struct reb_simulation* r = megno_bodies("var_rescale", 0, 0);
+        reb_simulation_set_integrator(r, "whfast");
+        const int a = reb_simulation_add_variation_1st_order(r, -1);
+        r->rand_seed = 3;
+        for (size_t i=0; i<r->N; i++){
+            r->particles_var[a+i].x = 4e99*reb_random_normal(r, 1.);
+            r->particles_var[a+i].vz = 4e99*reb_random_normal(r, 1.);
+        }
+        r->dt = 0.2;
+        const int64_t ck[4] = {1, 1, 8, 90};
+        run_var_case("var_rescale", r, ck, 4);
+        reb_simulation_free(r);
+    }
+}
+
 int main(int argc, char** argv){
     if (argc < 3){ fprintf(stderr, "usage: gen_golden OUTDIR serial|omp\n"); return 1; }
     outdir = argv[1];
     rawdir = getenv("GEN_GOLDEN_RAWDIR");
+    if (argc > 3 && strcmp(argv[3], "megno") == 0){
+        /* only the variational and MEGNO fixtures */
+        if (strcmp(argv[2], "serial") == 0) gen_megno();
+        return 0;
+    }
     if (strcmp(argv[2], "omp") == 0){
         gen_plummer_omp();
         gen_ghostbox_basic("omp");
@@ -1287,5 +1426,6 @@ int main(int argc, char** argv){
     gen_integrate();
     gen_custom_integrator();
     gen_shear_wrap();
+    gen_megno();
     return 0;
 }

@@ -37,6 +37,8 @@ def hash_records(rawdir):
 
 
 def main():
+    # `python make_golden.py megno` writes only the variational/MEGNO fixtures
+    only = sys.argv[1:2]
     subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref", "-j8"])
     refdir = os.path.join(ROOT, "oracle", "_ref")
     for flavour in ("serial", "omp"):
@@ -55,8 +57,8 @@ def main():
             for f in os.listdir(rawdir):
                 os.remove(os.path.join(rawdir, f))
             env["GEN_GOLDEN_RAWDIR"] = rawdir
-        subprocess.check_call([exe, HERE, flavour], env=env)
-        if flavour == "serial":
+        subprocess.check_call([exe, HERE, flavour] + only, env=env)
+        if flavour == "serial" and not only:
             hash_records(rawdir)
     print("fixtures written to", HERE)
 

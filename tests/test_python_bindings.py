@@ -22,10 +22,6 @@ NOT_PROVIDED = {}
 for _n in ("reb_display_copy_data", "reb_display_init_data", "reb_display_prepare_data",
            "reb_simulation_output_screenshot", "reb_simulation_start_server", "reb_simulation_stop_server"):
     NOT_PROVIDED[_n] = "visualisation server and screenshots"
-for _n in ("reb_simulation_add_variation_1st_order", "reb_simulation_add_variation_2nd_order",
-           "reb_simulation_init_megno", "reb_simulation_init_megno_seed", "reb_simulation_megno",
-           "reb_simulation_lyapunov", "reb_simulation_particle_var_index"):
-    NOT_PROVIDED[_n] = "variational equations and MEGNO"
 for _n in ("reb_integrator_bs_update_particles", "reb_integrator_ias15_timescale", "reb_ode_create"):
     NOT_PROVIDED[_n] = "integrators other than leapfrog, SEI and WHFast, and user ODEs"
 for _n in ("reb_rotation_identity", "reb_rotation_init_angle_axis", "reb_rotation_init_from_to",
