@@ -202,6 +202,20 @@ REB_API int reb_amd_ensemble_set_whfast(struct reb_amd_ensemble* e, int coordina
  * kick, 2 composition, 3 lazy implementer's; 1 and 3 use the Jacobi gravity
  * (gravity.c:108-163).  Replaces setting the field on each reb_simulation. */
 REB_API int reb_amd_ensemble_set_whfast_kernel(struct reb_amd_ensemble* e, unsigned int kernel);
+/* Before the first steps (Jacobi, default kernel, safe_mode 1, BASIC): MEGNO
+ * for every system, with one first-order variational set per system whose
+ * initial particles var0 gives in body order (what reb_simulation_init_megno
+ * draws for that system); each system's time starts at 0.  Replaces
+ * reb_simulation_init_megno (tools.c:1431-1470) on each reb_simulation; the
+ * sets and sums evolve on the device as integrator_whfast.c:392-490,
+ * 1250-1258 and tools.c:1504-1522. */
+REB_API int reb_amd_ensemble_init_megno(struct reb_amd_ensemble* e, const struct reb_particle* var0);
+/* reb_simulation_megno and reb_simulation_lyapunov (tools.c:1471-1482) of
+ * every system, into megno[n_systems] and lyapunov[n_systems] (either may be
+ * NULL). */
+REB_API int reb_amd_ensemble_megno(struct reb_amd_ensemble* e, double* megno, double* lyapunov);
+/* The variational particles (inertial x y z vx vy vz ax ay az m), body order. */
+REB_API int reb_amd_ensemble_get_var_particles(struct reb_amd_ensemble* e, struct reb_particle* out);
 /* Queues n steps of every system, ending synchronized; does not wait. */
 REB_API int reb_amd_ensemble_steps(struct reb_amd_ensemble* e, uint64_t n_steps);
 REB_API int reb_amd_ensemble_wait(struct reb_amd_ensemble* e);

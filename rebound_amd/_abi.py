@@ -217,6 +217,9 @@ def declare(lib):
         "reb_amd_ensemble_n_systems": (c_size_t, [c_void_p]),
         "reb_amd_ensemble_n_bodies": (c_size_t, [c_void_p]),
         "reb_amd_ensemble_get_particles": (c_int, [c_void_p, POINTER(Particle)]),
+        "reb_amd_ensemble_init_megno": (c_int, [c_void_p, POINTER(Particle)]),
+        "reb_amd_ensemble_megno": (c_int, [c_void_p, POINTER(c_double), POINTER(c_double)]),
+        "reb_amd_ensemble_get_var_particles": (c_int, [c_void_p, POINTER(Particle)]),
         "reb_amd_ensemble_energy": (c_int, [c_void_p, POINTER(c_double)]),
         "reb_amd_ensemble_flags": (c_int, [c_void_p, POINTER(ctypes.c_uint)]),
         "reb_amd_ensemble_set_launch": (c_int, [c_void_p, c_int, ctypes.c_long]),

@@ -61,6 +61,11 @@ struct Ensemble {
     int max_resident = 0;
     GraphEntry graphs[4];
     unsigned long long use_clock = 0;
+    // MEGNO (one first-order set per system): pv, jv (10 * stride doubles
+    // each), the systems' MEGNO fields, the set offset {0}
+    bool megno = false;
+    DevBuf var;
+    size_t var_bytes = 0;
     std::string err;
 };
 
@@ -96,6 +101,16 @@ static WHParams ens_params(const Ensemble* e) {
     P.corrector2 = e->corrector2;
     P.kernel = e->kernel;
     P.jacgrav = e->kernel == 1 || e->kernel == 3;
+    if (e->megno) {
+        double* base = e->var.as<double>();
+        P.nvar = 1;
+        P.pv = base;
+        P.jv = base + 10 * e->stride;
+        P.vstride = e->stride;
+        P.megno = 2;
+        P.megno_sys = base + 20 * e->stride;
+        P.var_index = reinterpret_cast<const int*>(base + 20 * e->stride + (size_t)kMegnoFields * e->nsys);
+    }
     return P;
 }
 
@@ -339,7 +354,7 @@ REB_API void reb_amd_ensemble_free(struct reb_amd_ensemble* h) {
     (void)hipSetDevice(e->device);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     drop_graphs(e);
-    for (DevBuf* b : {&e->body, &e->ph, &e->sys, &e->tasks, &e->order, &e->flags, &e->counter, &e->energy})
+    for (DevBuf* b : {&e->body, &e->ph, &e->sys, &e->tasks, &e->order, &e->flags, &e->counter, &e->energy, &e->var})
         b->release();
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete h;
@@ -403,6 +418,92 @@ REB_API int reb_amd_ensemble_set_whfast_kernel(struct reb_amd_ensemble* h, unsig
     }
     drop_graphs(e);
     e->kernel = (int)kernel;
+    return 0;
+}
+
+// MEGNO for every system (tools.c:1431-1470 with the variational particles
+// given): var0 holds one variational particle per body, in body order (x y z
+// vx vy vz m are used), e.g. those reb_simulation_init_megno draws for the
+// system.  Jacobi coordinates, the default kernel, no second corrector, safe
+// mode and BASIC gravity, as integrator_whfast.c:877-910 requires; before
+// the first steps.  Each system's time starts at 0.
+REB_API int reb_amd_ensemble_init_megno(struct reb_amd_ensemble* h, const struct reb_particle* var0) {
+    Ensemble* e = &h->e;
+    const char* why = nullptr;
+    if (e->first_step_done) why = "the ensemble has already been stepped";
+    else if (e->coords != WH_JACOBI) why = "variational particles are only compatible with Jacobi coordinates";
+    else if (e->kernel != 0) why = "variational particles are only compatible with the standard kernel";
+    else if (e->corrector2) why = "variational particles not compatible with 2nd corrector";
+    else if (!e->safe_mode) why = "MEGNO is not compatible with WHFast's safe_mode=0";
+    else if (e->compensated) why = "variational gravity needs REB_GRAVITY_BASIC";
+    else if (!var0) why = "null argument";
+    if (why) {
+        set_last_error(std::string("reb_amd_ensemble_init_megno: ") + why);
+        return 1;
+    }
+    if (!ENS_HIP(e, hipSetDevice(e->device))) return 1;
+    const size_t nd = 20 * e->stride + (size_t)kMegnoFields * e->nsys + 1;
+    if (!e->var.ensure(sizeof(double) * nd)) {
+        set_last_error("reb_amd_ensemble_init_megno: out of device memory");
+        return 1;
+    }
+    std::vector<double> hb(nd, 0.);
+    for (size_t b = 0; b < e->nbody; b++) {
+        const struct reb_particle& q = var0[b];
+        const double f[6] = {q.x, q.y, q.z, q.vx, q.vy, q.vz};
+        for (int k = 0; k < 6; k++) hb[k * e->stride + b] = f[k];
+        hb[9 * e->stride + b] = q.m;
+    }
+    // the set offset 0 in the last double (its first four bytes)
+    if (!ENS_HIP(e, hipMemcpyAsync(e->var.p, hb.data(), sizeof(double) * nd, hipMemcpyHostToDevice, e->stream)) ||
+        !ENS_HIP(e, hipStreamSynchronize(e->stream)))
+        return 1;
+    e->var_bytes = sizeof(double) * nd;
+    e->megno = true;
+    drop_graphs(e);
+    return 0;
+}
+
+// Each system's MEGNO <Y> and Lyapunov number (tools.c:1471-1482); either
+// pointer may be null.
+REB_API int reb_amd_ensemble_megno(struct reb_amd_ensemble* h, double* megno, double* lyapunov) {
+    Ensemble* e = &h->e;
+    if (!e->megno) {
+        set_last_error("reb_amd_ensemble_megno: MEGNO is not enabled (reb_amd_ensemble_init_megno)");
+        return 1;
+    }
+    std::vector<double> M((size_t)kMegnoFields * e->nsys);
+    if (!ENS_HIP(e, hipSetDevice(e->device)) ||
+        !ENS_HIP(e, hipMemcpyAsync(M.data(), e->var.as<double>() + 20 * e->stride, sizeof(double) * M.size(),
+                                   hipMemcpyDeviceToHost, e->stream)) ||
+        !ENS_HIP(e, hipStreamSynchronize(e->stream)))
+        return 1;
+    for (size_t i = 0; i < e->nsys; i++) {
+        const double* m = &M[i * kMegnoFields];
+        if (megno) megno[i] = m[0] == m[1] ? 0. : m[3] / (m[0] - m[1]);
+        if (lyapunov) lyapunov[i] = m[5] == 0.0 ? 0. : m[4] / m[5];
+    }
+    return 0;
+}
+
+// The variational particles (inertial, x y z vx vy vz ax ay az m) in body order.
+REB_API int reb_amd_ensemble_get_var_particles(struct reb_amd_ensemble* h, struct reb_particle* out) {
+    Ensemble* e = &h->e;
+    if (!e->megno) {
+        set_last_error("reb_amd_ensemble_get_var_particles: MEGNO is not enabled");
+        return 1;
+    }
+    std::vector<double> hb(10 * e->stride);
+    if (!ENS_HIP(e, hipSetDevice(e->device)) ||
+        !ENS_HIP(e, hipMemcpyAsync(hb.data(), e->var.p, sizeof(double) * hb.size(), hipMemcpyDeviceToHost,
+                                   e->stream)) ||
+        !ENS_HIP(e, hipStreamSynchronize(e->stream)))
+        return 1;
+    for (size_t b = 0; b < e->nbody; b++) {
+        struct reb_particle& p = out[b];
+        double* f[10] = {&p.x, &p.y, &p.z, &p.vx, &p.vy, &p.vz, &p.ax, &p.ay, &p.az, &p.m};
+        for (int k = 0; k < 10; k++) *f[k] = hb[k * e->stride + b];
+    }
     return 0;
 }
 
@@ -503,19 +604,25 @@ REB_API int reb_amd_ensemble_tune(struct reb_amd_ensemble* h, long probe_steps, 
     Ensemble* e = &h->e;
     if (probe_steps < 4) probe_steps = 4;
     if (!ENS_HIP(e, hipSetDevice(e->device))) return 1;
-    DevBuf sb, sp, sf;
+    DevBuf sb, sp, sf, sv;
     const size_t bb = sizeof(double) * 10 * e->stride, pb = sizeof(double) * 7 * e->stride,
-                 fb = sizeof(unsigned int) * e->nsys;
-    if (!sb.ensure(bb) || !sp.ensure(pb) || !sf.ensure(fb)) {
+                 fb = sizeof(unsigned int) * e->nsys, vb = e->megno ? e->var_bytes : 0;
+    if (!sb.ensure(bb) || !sp.ensure(pb) || !sf.ensure(fb) || (vb && !sv.ensure(vb))) {
         set_last_error("reb_amd_ensemble_tune: out of device memory");
         return 1;
     }
-    auto copy_state = [&](void* db, void* dp, void* df, const void* xb, const void* xp, const void* xf) {
-        return ENS_HIP(e, hipMemcpyAsync(db, xb, bb, hipMemcpyDeviceToDevice, e->stream)) &&
-               ENS_HIP(e, hipMemcpyAsync(dp, xp, pb, hipMemcpyDeviceToDevice, e->stream)) &&
-               ENS_HIP(e, hipMemcpyAsync(df, xf, fb, hipMemcpyDeviceToDevice, e->stream));
+    // the bodies, heliocentric state and flags (and the MEGNO state) to or from the backup
+    auto copy_state = [&](bool save) {
+        void* bk[4] = {sb.p, sp.p, sf.p, sv.p};
+        void* live[4] = {e->body.p, e->ph.p, e->flags.p, e->var.p};
+        const size_t n[4] = {bb, pb, fb, vb};
+        for (int k = 0; k < 4; k++)
+            if (n[k] && !ENS_HIP(e, hipMemcpyAsync(save ? bk[k] : live[k], save ? live[k] : bk[k], n[k],
+                                                   hipMemcpyDeviceToDevice, e->stream)))
+                return false;
+        return true;
     };
-    bool ok = copy_state(sb.p, sp.p, sf.p, e->body.p, e->ph.p, e->flags.p);
+    bool ok = copy_state(true);
     const bool first_done = e->first_step_done;
     const uint64_t done = e->steps_done;
     const int grid0 = e->grid;
@@ -534,12 +641,12 @@ REB_API int reb_amd_ensemble_tune(struct reb_amd_ensemble* h, long probe_steps, 
     auto probe = [&](int g, long c, float* ms) {
         reb_amd_ensemble_set_launch(h, g, c);
         bool fine = ens_steps(e, (uint64_t)probe_steps);
-        fine = fine && copy_state(e->body.p, e->ph.p, e->flags.p, sb.p, sp.p, sf.p);
+        fine = fine && copy_state(false);
         e->first_step_done = first_done;
         fine = fine && ENS_HIP(e, hipEventRecord(a, e->stream)) && ens_steps(e, (uint64_t)probe_steps) &&
                ENS_HIP(e, hipEventRecord(b, e->stream)) && ENS_HIP(e, hipEventSynchronize(b)) &&
                ENS_HIP(e, hipEventElapsedTime(ms, a, b));
-        fine = fine && copy_state(e->body.p, e->ph.p, e->flags.p, sb.p, sp.p, sf.p);
+        fine = fine && copy_state(false);
         e->first_step_done = first_done;
         e->steps_done = done;
         return fine;

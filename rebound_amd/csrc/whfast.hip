@@ -552,7 +552,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
         // transformations.c:34-81 on particles_var (integrator_whfast.c:958-962)
         auto var_from_inertial = [&](bool en) {
             for (int v = 0; v < P.nvar; v++) {
-                const size_t b = (size_t)P.var_index[v];
+                const size_t b = b0 + (size_t)P.var_index[v];
                 __syncthreads();
                 double q[6] = {0., 0., 0., 0., 0., 0.}, d0[6] = {0., 0., 0., 0., 0., 0.}, dm = 0.;
                 if (live) {
@@ -605,7 +605,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
         // transformations.c:178-258 on particles_var (integrator_whfast.c:662-672, 1089-1092)
         auto var_to_inertial = [&](bool en, bool with_vel) {
             for (int v = 0; v < P.nvar; v++) {
-                const size_t b = (size_t)P.var_index[v];
+                const size_t b = b0 + (size_t)P.var_index[v];
                 __syncthreads();
                 double h[6] = {0., 0., 0., 0., 0., 0.}, c[6] = {0., 0., 0., 0., 0., 0.}, ceta = 0.;
                 if (live)
@@ -672,7 +672,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
         // test particles i >= N_active (type 1).  The star's is formed by lane 0.
         auto var_gravity = [&](bool en, int starti, int startj) {
             for (int v = 0; v < P.nvar; v++) {
-                const size_t b = (size_t)P.var_index[v];
+                const size_t b = b0 + (size_t)P.var_index[v];
                 __syncthreads();
                 double d[3] = {0., 0., 0.}, dm = 0., ds[3] = {0., 0., 0.}, dsm = 0.;
                 if (live) {
@@ -931,7 +931,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                 KepOut ko;
                 warn |= kepler_solver(hx, hy, hz, hvx, hvy, hvz, mu, h, &ko);
                 for (int v = 0; v < P.nvar; v++) {
-                    const size_t b = (size_t)P.var_index[v] + me;
+                    const size_t b = b0 + (size_t)P.var_index[v] + me;
                     double d[6];
                     for (int f = 0; f < 6; f++) d[f] = vget(P.jv, f, b);
                     kepler_var(ko, d[0], d[1], d[2], d[3], d[4], d[5]);
@@ -950,7 +950,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
             if constexpr (VAR) {
                 if (gl == 0)
                     for (int v = 0; v < P.nvar; v++) {
-                        const size_t b = (size_t)P.var_index[v];
+                        const size_t b = b0 + (size_t)P.var_index[v];
                         vput(P.jv, 0, b, vget(P.jv, 0, b) + h * vget(P.jv, 3, b));
                         vput(P.jv, 1, b, vget(P.jv, 1, b) + h * vget(P.jv, 4, b));
                         vput(P.jv, 2, b, vget(P.jv, 2, b) + h * vget(P.jv, 5, b));
@@ -1228,7 +1228,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
         // the real kick below (its positions do not change in a kick).
         auto var_interaction = [&](double h, bool en) {
             for (int v = 0; v < P.nvar; v++) {
-                const size_t b = (size_t)P.var_index[v];
+                const size_t b = b0 + (size_t)P.var_index[v];
                 __syncthreads();
                 double a[3] = {0., 0., 0.}, sa[3] = {0., 0., 0.};
                 if (live) {
@@ -1402,6 +1402,32 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
             }
             if constexpr (VAR) var_from_inertial(en);
         };
+        // tools.c:1313-1380 after a batched step (synchronized: safe mode):
+        // a set whose largest coordinate exceeds 1e100 is divided by it (the
+        // same IEEE divisions), which changes no MEGNO value; the logarithms
+        // of the scales are not kept.
+        auto var_rescale = [&](bool en) {
+            for (int v = 0; v < P.nvar; v++) {
+                const size_t b = b0 + (size_t)P.var_index[v];
+                __syncthreads();
+                if (gl == 0 && sys_ok) {
+                    double scale = 0.;
+                    for (int i = 0; i < Nw; i++)
+                        for (int f = 0; f < 6; f++) {
+                            const double a = fabs(vget(P.pv, f, b + (size_t)i));
+                            scale = a > scale ? a : scale;
+                        }
+                    aux0[gbase][0] = scale;
+                }
+                __syncthreads();
+                const double scale = sys_ok ? aux0[gbase][0] : 0.;
+                if (!en || !(scale > 1e100)) continue;
+                if (live)
+                    for (int f = 0; f < 6; f++) vput(P.pv, f, b + me, vget(P.pv, f, b + me) / scale);
+                if (gl == 0)
+                    for (int f = 0; f < 6; f++) vput(P.pv, f, b, vget(P.pv, f, b) / scale);
+            }
+        };
         // integrator_whfast.c:1250-1258: the variational accelerations with
         // every term (gravity_ignore_terms NONE), then tools.c:1483-1502 over
         // particles_var[0 .. N-1] by lane 0; the host adds it to the MEGNO sums.
@@ -1413,7 +1439,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                     double deltad = 0., delta2 = 0.;
                     for (int i = 0; i < Nw; i++) {
                         double q[9];
-                        for (int f = 0; f < 9; f++) q[f] = vget(P.pv, f, (size_t)i);
+                        for (int f = 0; f < 9; f++) q[f] = vget(P.pv, f, b0 + (size_t)i);
                         deltad += q[3] * q[0];
                         deltad += q[4] * q[1];
                         deltad += q[5] * q[2];
@@ -1427,7 +1453,32 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                         delta2 += q[4] * q[4];
                         delta2 += q[5] * q[5];
                     }
-                    *P.megno_out = deltad / delta2;
+                    const double dd = deltad / delta2;
+                    if (P.megno == 1) {
+                        *P.megno_out = dd;
+                    } else {
+                        // the system's time and MEGNO sums (tools.c:1504-1522),
+                        // t advanced as the reference's two half steps
+                        double* M = P.megno_sys + (size_t)sidx * kMegnoFields;
+                        double t = M[0];
+                        t += dt / 2.;
+                        t += dt / 2.;
+                        const double t0 = M[1];
+                        double Ys = M[2], Yss = M[3], cov = M[4], var = M[5], mean_t = M[6], mean_Y = M[7];
+                        const double n = M[8] + 1.;
+                        Ys += dt * 2. * (t - t0) * dd;
+                        const double Y = Ys / (t - t0);
+                        Yss += Y * dt;
+                        const double d_t = t - t0 - mean_t;
+                        mean_t += d_t / n;
+                        const double meg = t == t0 ? 0. : Yss / (t - t0);
+                        const double d_Y = meg - mean_Y;
+                        mean_Y += d_Y / n;
+                        cov += (n - 1.) / n * (t - t0 - mean_t) * (meg - mean_Y);
+                        var += (n - 1.) / n * (t - t0 - mean_t) * (t - t0 - mean_t);
+                        M[0] = t; M[2] = Ys; M[3] = Yss; M[4] = cov; M[5] = var; M[6] = mean_t; M[7] = mean_Y;
+                        M[8] = n;
+                    }
                 }
             }
         };
@@ -1496,6 +1547,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
             if (act) sync = 0;
             if (__any(act && safe)) synchronize(act && safe);
             if (VAR && P.megno) megno_step(act);
+            if (VAR && P.megno == 2) var_rescale(act);
             if (act) modified = false;
         }
         synchronize(sys_ok && (P.sync_at_end || (S.req & 1)));

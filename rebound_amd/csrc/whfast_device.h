@@ -57,19 +57,25 @@ struct WHParams {
     int corrector2 = 0;       // second corrector (Jacobi): 0 or 1
     int kernel = 0;           // 0 default, 1 modified kick, 2 composition, 3 lazy (Jacobi)
     int jacgrav = 0;          // accelerations by the Jacobi gravity (kernels 1 and 3)
-    // First-order variational equations (Jacobi, one system per launch).
-    // Variational particle k of the simulation is entry k of `pv` (inertial,
-    // fields x y z vx vy vz ax ay az m, `vstride` doubles apart) and of `jv`
-    // (its Jacobi copy, the same fields); set v covers entries
-    // var_index[v] .. var_index[v] + N - 1.
+    // First-order variational equations (Jacobi).  `pv` holds the inertial
+    // variational particles (fields x y z vx vy vz ax ay az m, `vstride`
+    // doubles apart) and `jv` their Jacobi copies (the same fields); set v of
+    // the system whose bodies start at body0 covers entries
+    // body0 + var_index[v] .. body0 + var_index[v] + N - 1.
     int nvar = 0;
     const int* var_index = nullptr;
     double* pv = nullptr;
     double* jv = nullptr;
     size_t vstride = 0;
-    int megno = 0;                // MEGNO: <deltad, delta>/|delta|^2 of the step into megno_out
+    // MEGNO: 1 = the step's <deltad, delta>/|delta|^2 into megno_out (one
+    // system, the host keeps the sums); 2 = each system's time and sums in
+    // megno_sys (kMegnoFields per system: t, initial t, Ys, Yss, cov_Yt,
+    // var_t, mean_t, mean_Y, n), updated on the device every step
+    int megno = 0;
     double* megno_out = nullptr;
+    double* megno_sys = nullptr;
 };
+constexpr int kMegnoFields = 9;
 
 int whfast_launch(const WHParams& P, hipStream_t st);
 int whfast_energy_launch(const WHParams& P, double* out, int nsys, hipStream_t st);

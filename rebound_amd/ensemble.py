@@ -158,6 +158,36 @@ class Ensemble:
                                     shape=(self._n * ctypes.sizeof(_abi.Particle) // 8,))
         return buf.reshape(self._n, -1)[:, :10].copy()
 
+    def init_megno(self, var0):
+        """MEGNO for every system from (bodies x >=7) initial variational
+        particles x y z vx vy vz ... m (columns as state(): m is column 9 when
+        given 10 columns)."""
+        var0 = np.asarray(var0, dtype=np.float64)
+        parts = (_abi.Particle * self._n)()
+        for i, row in enumerate(var0):
+            p = parts[i]
+            p.x, p.y, p.z, p.vx, p.vy, p.vz = (float(v) for v in row[:6])
+            p.m = float(row[9]) if var0.shape[1] >= 10 else 0.
+        _check(lib().reb_amd_ensemble_init_megno(self._h, parts), "reb_amd_ensemble_init_megno", self._h)
+
+    def megno(self):
+        """(megno, lyapunov) arrays, one value per system."""
+        m = np.zeros(self.n_systems)
+        ly = np.zeros(self.n_systems)
+        dp = ctypes.POINTER(ctypes.c_double)
+        _check(lib().reb_amd_ensemble_megno(self._h, m.ctypes.data_as(dp), ly.ctypes.data_as(dp)),
+               "reb_amd_ensemble_megno", self._h)
+        return m, ly
+
+    def var_state(self):
+        """(bodies x 10) variational particles x y z vx vy vz ax ay az m."""
+        parts = (_abi.Particle * self._n)()
+        _check(lib().reb_amd_ensemble_get_var_particles(self._h, parts), "reb_amd_ensemble_get_var_particles",
+               self._h)
+        buf = np.ctypeslib.as_array(ctypes.cast(parts, ctypes.POINTER(ctypes.c_double)),
+                                    shape=(self._n * ctypes.sizeof(_abi.Particle) // 8,))
+        return buf.reshape(self._n, -1)[:, :10].copy()
+
     def system(self, s, st=None):
         st = self.state() if st is None else st
         return st[self.first[s]:self.first[s] + self.n_bodies[s]]

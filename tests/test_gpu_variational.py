@@ -111,3 +111,86 @@ def test_variational_errors(gpu):
     with pytest.raises(gpu.RebError, match="safe_mode=0"):
         sim.steps(1)
     sim.free()
+
+
+def _bodies_at_start(gpu, tag):
+    """The bodies as the case's simulation starts stepping (the chaotic case
+    moved to the centre-of-mass frame after init_megno)."""
+    if tag != "megno_chaos":
+        return golden(tag + "_init.npy")
+    sim, _ = _setup(gpu, tag)
+    st = sim.state()
+    sim.free()
+    return st
+
+
+def _ens_case(gpu, tags, copies, corrector=0):
+    inits = [_bodies_at_start(gpu, t) for t in tags]
+    var0s = [golden(t + "_var0.npy") for t in tags]
+    dts = [float(golden(t + "_k0_meta.npy")[0]) for t in tags]
+    state = np.concatenate([inits[i % len(tags)] for i in range(copies * len(tags))])
+    var0 = np.concatenate([var0s[i % len(tags)] for i in range(copies * len(tags))])
+    nb = [inits[i % len(tags)].shape[0] for i in range(copies * len(tags))]
+    dt = [dts[i % len(tags)] for i in range(copies * len(tags))]
+    ens = gpu.Ensemble(nb, state, dt, safe_mode=1, coordinates="jacobi", corrector=corrector)
+    ens.init_megno(var0)
+    return ens
+
+
+def _ens_check(ens, tags, k_of):
+    st, vs = ens.state(), ens.var_state()
+    meg, ly = ens.megno()
+    for s in range(ens.n_systems):
+        tag = tags[s % len(tags)]
+        k = k_of[tag]
+        if k is None:
+            continue
+        lo, n = ens.first[s], ens.n_bodies[s]
+        want = golden("%s_k%d_state.npy" % (tag, k))[:, :10]
+        assert np.array_equal(st[lo:lo + n], want), (tag, k, s, "bodies")
+        assert np.array_equal(vs[lo:lo + n], golden("%s_k%d_var.npy" % (tag, k))), (tag, k, s, "var")
+        meta = golden("%s_k%d_meta.npy" % (tag, k))
+        assert meg[s] == meta[1] and ly[s] == meta[2], (tag, k, s, "megno")
+
+
+def test_ensemble_megno_bitwise(gpu):
+    """MEGNO for many systems in one launch (reb_amd_ensemble_init_megno):
+    every copy of the regular and the chaotic system of test_megno.py equals
+    the reference's single simulation at each checkpoint (bodies with
+    accelerations, variational particles, <Y> and the Lyapunov number), and so
+    do copies of the corrector case."""
+    tags = ["megno_reg", "megno_chaos"]
+    ens = _ens_case(gpu, tags, 40)
+    ens.tune(20)   # the probes must leave the MEGNO state as it was
+    for k, n in enumerate([1, 9, 90, 900]):
+        ens.steps(n)
+        _ens_check(ens, tags, {"megno_reg": k, "megno_chaos": k})
+    ens.steps(1000)
+    _ens_check(ens, tags, {"megno_reg": None, "megno_chaos": 4})
+    ens.steps(1000)
+    _ens_check(ens, tags, {"megno_reg": 4, "megno_chaos": None})
+    ens.free()
+    ens = _ens_case(gpu, ["megno_corr"], 5, corrector=11)
+    for k, n in enumerate([1, 9, 90, 400]):
+        ens.steps(n)
+        _ens_check(ens, ["megno_corr"], {"megno_corr": k})
+    ens.free()
+
+
+def test_ensemble_rescale_bitwise(gpu):
+    """A set that grows past 1e100 is rescaled on the device as
+    reb_simulation_rescale_var does after each step: bodies and variational
+    positions and velocities equal the reference's (its variational
+    accelerations are those of the MEGNO term here, which the fixture's
+    simulation does not compute)."""
+    ens = _ens_case(gpu, ["var_rescale"], 3)
+    for k, n in enumerate([1, 1, 8, 90]):
+        ens.steps(n)
+        st, vs = ens.state(), ens.var_state()
+        for s in range(ens.n_systems):
+            lo, nb = ens.first[s], ens.n_bodies[s]
+            assert np.array_equal(st[lo:lo + nb], golden("var_rescale_k%d_state.npy" % k)[:, :10]), (k, s)
+            want = golden("var_rescale_k%d_var.npy" % k)
+            assert np.array_equal(vs[lo:lo + nb, :6], want[:, :6]), (k, s)
+            assert np.array_equal(vs[lo:lo + nb, 9], want[:, 9]), (k, s)
+    ens.free()
