@@ -337,6 +337,77 @@ def ensemble_cpu(args, systems=None, steps=None):
             "max_rel_energy_error": r["max_rel_energy_error"], "mean_rel_energy_error": r["mean_rel_energy_error"]}
 
 
+def megno_leg(amd, args):
+    """MEGNO map: the ensemble's systems (seed `ens_seed`) in Jacobi
+    coordinates with safe_mode 1, each with reb_simulation_init_megno_seed(r, s)
+    drawn for system s, `megno_steps` steps in one request; the first
+    systems' <Y> compared with the reference's at the CPU sample's step count."""
+    nb, st, dt = amd.planetary_systems(args.megno_systems, args.ens_seed, 2, 8)
+    first = np.concatenate([[0], np.cumsum(nb)[:-1]])
+    var0 = np.zeros((int(nb.sum()), 10))
+    for s in range(len(nb)):
+        sim = amd.Sim()
+        sim.set_state(st[first[s]:first[s] + nb[s]])
+        sim.init_megno(seed=s)
+        v = sim.var_particles()
+        var0[first[s]:first[s] + nb[s]] = np.stack([v[f] for f in ("x", "y", "z", "vx", "vy", "vz", "ax", "ay",
+                                                                      "az", "m")], axis=1)
+        sim.free()
+
+    def make():
+        e = amd.Ensemble(nb, st, dt, safe_mode=1, coordinates="jacobi")
+        e.init_megno(var0)
+        return e
+
+    ens = make()
+    tune_ms, shape = ens.tune(probe_steps=args.ens_probe)
+    t0 = time.perf_counter()
+    ens.steps(args.megno_steps)
+    wall = time.perf_counter() - t0
+    meg, lya = ens.megno()
+    ens.free()
+    res = {"metric": "MEGNO map system*steps/s (WHFast Jacobi + variational set, %d systems, K in 2..8)" %
+                     len(nb),
+           "value": float(len(nb)) * args.megno_steps / wall, "unit": "system-steps/s", "seconds": wall,
+           "steps": args.megno_steps, "systems": int(len(nb)), "launch": shape,
+           "us_per_step": 1e6 * wall / args.megno_steps,
+           # systems whose orbits went non-finite have no MEGNO (as in the reference)
+           "megno": {"median": float(np.nanmedian(meg)), "p90": float(np.nanquantile(meg, 0.9)),
+                     "chaotic_fraction_Y_gt_4": float((meg > 4.).mean()),
+                     "non_finite_systems": int((~np.isfinite(meg)).sum())}}
+    log("megno: %.3e system-steps/s (%.2f us/step), <Y> median %.3f, %.1f%% with <Y> > 4, %d non-finite" % (
+        res["value"], res["us_per_step"], res["megno"]["median"], 100. * res["megno"]["chaotic_fraction_Y_gt_4"],
+        res["megno"]["non_finite_systems"]))
+    # the reference's serial build (its sums are the GPU's), OpenMP across systems
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_bench_serial")
+    if not args.no_cpu and os.path.exists(exe):
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+        cmd = [exe, "megno", str(len(nb)), str(args.megno_cpu_steps), str(args.ens_seed), "2", "8"]
+        log("megno cpu baseline:", " ".join(cmd), "threads", threads)
+        out = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=900)
+        if out.returncode == 0:
+            r = json.loads(out.stdout.strip().splitlines()[-1])
+            e2 = make()
+            e2.steps(args.megno_cpu_steps)
+            m2, _ = e2.megno()
+            e2.free()
+            same = ["%.17g" % a for a in m2[:len(r["megno_first"])]] == ["%.17g" % b for b in r["megno_first"]]
+            res["cpu_baseline"] = {
+                "value": r["system_steps_per_s"], "unit": "system-steps/s", "cores": r["threads"],
+                "kind": "reference",
+                "sample": "reference serial build, omp parallel for over the same %d systems with init_megno_seed, "
+                          "%d steps, %.1f s" % (
+                    len(nb), args.megno_cpu_steps, r["seconds"]),
+                "megno_first_systems_bitwise_equal": same}
+            res["speedup_vs_cpu"] = res["value"] / r["system_steps_per_s"]
+            log("megno cpu: %.3e system-steps/s on %d threads -> GPU %.0fx; first systems' <Y> bitwise equal: %s" % (
+                r["system_steps_per_s"], r["threads"], res["speedup_vs_cpu"], same))
+        else:
+            log(out.stderr)
+    return res
+
+
 def serving_jobs(args):
     """Poisson arrivals (seeded): 70% planetary systems (star + K planets, K
     uniform in {2..10}, generated as the ensemble's) for log-uniform 1e3..1e5
@@ -612,7 +683,10 @@ def main():
     ap.add_argument("--sheet-seed", type=int, default=99)
     ap.add_argument("--sheet-steps", type=int, default=500)
     ap.add_argument("--sheet-cpu-steps", type=int, default=3)
-    ap.add_argument("--only", choices=["direct", "ensemble", "serve", "sheet", "small"], default=None)
+    ap.add_argument("--megno-systems", type=int, default=16384)
+    ap.add_argument("--megno-steps", type=int, default=5000)
+    ap.add_argument("--megno-cpu-steps", type=int, default=200)
+    ap.add_argument("--only", choices=["direct", "ensemble", "serve", "sheet", "small", "megno"], default=None)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device (counter calibration)")
     args = ap.parse_args()
@@ -640,6 +714,9 @@ def main():
     if args.only == "small":
         print(json.dumps({"small": small_leg(amd, args)}))
         return
+    if args.only == "megno":
+        print(json.dumps({"megno": megno_leg(amd, args)}))
+        return
     head, _ = run_mode(amd, args, "fp64", "fp64", "basic", args.steps, args.warmup)
     modes = {"fp64": head}
     if not args.quick:
@@ -655,6 +732,7 @@ def main():
     serving = None if (args.quick or args.only == "direct") else serving_leg(amd, args)
     sheet = None if (args.quick or args.only == "direct") else sheet_leg(amd, args)
     small = None if (args.quick or args.only == "direct") else small_leg(amd, args)
+    megno = None if (args.quick or args.only == "direct") else megno_leg(amd, args)
 
     achieved = head["algorithmic_tflops"]
     tag = "direct_fp64_plummer65536"
@@ -693,6 +771,7 @@ def main():
         "serving": serving,
         "sheet": sheet,
         "small": small,
+        "megno": megno,
     }
     print(json.dumps(line))
 

@@ -7,6 +7,7 @@
  *       Plummer sphere (tools.c:463-502), leapfrog (integrator_leapfrog.c:97-209),
  *       direct gravity in the reference's OpenMP per-i form (gravity.c:215-232).
  *   ref_bench ensemble NSYS STEPS SEED SAFE_MODE KMIN KMAX
+ *   ref_bench megno NSYS STEPS SEED KMIN KMAX
  *       WHFast democratic-heliocentric planetary systems, one simulation per
  *       system, `omp parallel for` over systems (each integrated serially).
  *   ref_bench sheet N STEPS ROOT_SIZE N_ROOT SEED
@@ -95,6 +96,41 @@ static struct reb_simulation* planetary_system(struct reb_simulation* g, int kmi
     wh->safe_mode = safe_mode;
     r->dt = 2.*M_PI*sqrt(1./(1.+r->particles[1].m))/20.;
     return r;
+}
+
+/* MEGNO map: the ensemble's systems in Jacobi coordinates with safe_mode 1,
+ * reb_simulation_init_megno_seed(r, s) for system s (tools.c:1431-1470), each
+ * integrated for STEPS steps; prints the MEGNO of the first systems. */
+static int bench_megno(int argc, char** argv){
+    if (argc < 7){ fprintf(stderr, "megno NSYS STEPS SEED KMIN KMAX\n"); return 1; }
+    int nsys = atoi(argv[2]);
+    long steps = atol(argv[3]);
+    unsigned int seed = (unsigned int)strtoul(argv[4], 0, 10);
+    int kmin = atoi(argv[5]), kmax = atoi(argv[6]);
+    struct reb_simulation* g = reb_simulation_create();
+    g->rand_seed = seed;
+    struct reb_simulation** sims = malloc(sizeof(*sims)*nsys);
+    for (int s=0; s<nsys; s++){
+        sims[s] = planetary_system(g, kmin, kmax, 1);
+        struct reb_integrator_whfast_state* wh = sims[s]->integrator.state;
+        wh->coordinates = REB_INTEGRATOR_WHFAST_COORDINATES_JACOBI;
+        reb_simulation_init_megno_seed(sims[s], (unsigned int)s);
+    }
+    double t0 = now();
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int s=0; s<nsys; s++){
+        reb_simulation_steps(sims[s], (size_t)steps);
+    }
+    double t1 = now();
+    printf("{\"workload\": \"megno\", \"systems\": %d, \"steps\": %ld, \"seconds\": %.6f, "
+           "\"system_steps_per_s\": %.6e, \"threads\": %d, \"megno_first\": [",
+           nsys, steps, t1-t0, (double)nsys*(double)steps/(t1-t0), omp_get_max_threads());
+    for (int s=0; s<nsys && s<8; s++) printf("%s%.17g", s ? ", " : "", reb_simulation_megno(sims[s]));
+    printf("]}\n");
+    for (int s=0; s<nsys; s++) reb_simulation_free(sims[s]);
+    free(sims);
+    reb_simulation_free(g);
+    return 0;
 }
 
 static int bench_ensemble(int argc, char** argv){
@@ -255,6 +291,7 @@ int main(int argc, char** argv){
     if (argc < 2){ fprintf(stderr, "usage: ref_bench plummer|ensemble ...\n"); return 1; }
     if (strcmp(argv[1], "plummer") == 0) return bench_plummer(argc, argv);
     if (strcmp(argv[1], "ensemble") == 0) return bench_ensemble(argc, argv);
+    if (strcmp(argv[1], "megno") == 0) return bench_megno(argc, argv);
     if (strcmp(argv[1], "serve") == 0) return bench_serve(argc, argv);
     if (strcmp(argv[1], "sheet") == 0) return bench_sheet(argc, argv);
     fprintf(stderr, "unknown workload %s\n", argv[1]);
