@@ -437,8 +437,8 @@ __device__ __forceinline__ const CorrZ* corrector_chain(int order, int& n) {
 // the sequential scans of transformations.c:34-81,147-176,178-258, which every
 // lane of a group evaluates over the published bodies in the reference's order,
 // and the symplectic correctors run at the synchronisation points.
-template <int COORD, bool VAR = false>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_whfast(WHParams P) {
+template <int COORD, bool VAR>
+__device__ __forceinline__ void whfast_body(const WHParams& P) {
     constexpr bool JAC = COORD == WH_JACOBI;
     static_assert(!VAR || JAC, "variational equations run in Jacobi coordinates");
     __shared__ Slot slot[64];
@@ -1578,6 +1578,22 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
     }
 }
 
+template <int COORD>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_whfast(WHParams P) {
+    whfast_body<COORD, false>(P);
+}
+
+// The variational instance (Jacobi), two wavefronts per SIMD: its registers
+// are capped at 256 (spilling to scratch) so that every task of a 16384-system
+// map is resident at once (150 us per step against 205 at one wavefront).
+#ifndef RB_WHVAR_WAVES
+#define RB_WHVAR_WAVES 2
+#endif
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RB_WHVAR_WAVES, RB_WHVAR_WAVES))) void
+k_whfast_var(WHParams P) {
+    whfast_body<WH_JACOBI, true>(P);
+}
+
 // Energy of each system in the reference's order (tools.c:108-162):
 // kinetic over bodies, then potential over pairs (i<j), one thread per system.
 __global__ void k_system_energy(WHParams P, double* __restrict__ out, int nsys) {
@@ -1613,7 +1629,7 @@ __global__ void k_reset_counter(unsigned int* c) { *c = 0u; }
 int whfast_launch(const WHParams& P, hipStream_t st) {
     hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, st, P.counter);
     if (P.coords == WH_JACOBI && P.nvar > 0)
-        hipLaunchKernelGGL((k_whfast<WH_JACOBI, true>), dim3(P.grid), dim3(64), 0, st, P);
+        hipLaunchKernelGGL(k_whfast_var, dim3(P.grid), dim3(64), 0, st, P);
     else if (P.coords == WH_JACOBI)
         hipLaunchKernelGGL(k_whfast<WH_JACOBI>, dim3(P.grid), dim3(64), 0, st, P);
     else
