@@ -628,9 +628,12 @@ REB_API void reb_simulation_move_to_com(struct reb_simulation* const r) {
             pv[i + index].vz -= s[5];
         }
     };
+    // a set must lie inside particles_var (a file could say otherwise)
+    auto fits = [&](long long i) { return i >= 0 && (size_t)i + N <= r->N_var; };
     for (size_t v = 0; v < r->N_var_config; v++) {
         const struct reb_variational_configuration& vc = r->var_config[v];
         if (vc.testparticle >= 0 || vc.order != 2) continue;
+        if (!fits(vc.index) || !fits(vc.index_1st_order_a) || !fits(vc.index_1st_order_b)) continue;
         const size_t index = (size_t)vc.index, a = (size_t)vc.index_1st_order_a, b = (size_t)vc.index_1st_order_b;
         double dma = 0., dmb = 0., ddm = 0.;
         for (size_t i = 0; i < N; i++) {
@@ -656,7 +659,7 @@ REB_API void reb_simulation_move_to_com(struct reb_simulation* const r) {
     }
     for (size_t v = 0; v < r->N_var_config; v++) {
         const struct reb_variational_configuration& vc = r->var_config[v];
-        if (vc.testparticle >= 0 || vc.order != 1) continue;
+        if (vc.testparticle >= 0 || vc.order != 1 || !fits(vc.index)) continue;
         const size_t index = (size_t)vc.index;
         // tools.c:319-321 sums particles[i+index].m (the bodies' array, at
         // the set's offset); entries past the bodies count as zero here
@@ -729,6 +732,7 @@ REB_API void reb_simulation_rescale_var(struct reb_simulation* const r) {
         struct reb_variational_configuration* vc = &r->var_config[v];
         if (vc->lrescale < 0) continue;
         const size_t N = vc->testparticle < 0 ? r->N : 1;
+        if (vc->index < 0 || (size_t)vc->index + N > r->N_var) continue;   // not inside particles_var
         double scale = 0;
         struct reb_particle* const q = r->particles_var + vc->index;
         for (size_t i = 0; i < N; i++) {
